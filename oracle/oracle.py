@@ -1,0 +1,135 @@
+"""ctypes front end of the oracle (the CPU restatement of the reference path).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker or the timed CPU baseline.  The
+product (mdtk_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+POT_FBZL, POT_BRENNER, POT_REBO, POT_AIREBO, POT_ETORS = 1, 2, 4, 8, 16
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    if force or not os.path.exists(LIB):
+        subprocess.run(["make", "-B" if force else "-s", "-C", HERE], check=True,
+                       stdout=subprocess.DEVNULL)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_create.restype = ctypes.c_void_p
+        L.orc_create.argtypes = [ctypes.c_int, ip, dp, dp, dp, ctypes.POINTER(ctypes.c_ubyte),
+                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.orc_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_num_terms.argtypes = [ctypes.c_void_p]
+        L.orc_term_kind.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_energy_grad.argtypes = [ctypes.c_void_p, dp, dp]
+        L.orc_nl.argtypes = [ctypes.c_void_p, ctypes.c_int, ip, ip, ctypes.c_long]
+        L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, dp, dp, dp, dp]
+        L.orc_get_state.argtypes = [ctypes.c_void_p, dp, dp, dp, ip]
+        L.orc_last_error.restype = ctypes.c_char_p
+        L.orc_spline_eval.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.c_int, dp]
+        L.orc_set_alter_exact.argtypes = [ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def set_alter_exact(on: bool) -> None:
+    """Test switch: zero derivative for the b* pair's constant r_alter."""
+    lib().orc_set_alter_exact(int(bool(on)))
+
+
+def _p(a, t=ctypes.c_double):
+    return a.ctypes.data_as(ctypes.POINTER(t)) if a is not None else None
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class Oracle:
+    """One oracle simulation: a potential chain over one system (CGS arrays)."""
+
+    def __init__(self, elem, xyz_cm, vel_cm_s, pbc_cm, fixed=None, pots=POT_REBO,
+                 brenner_set=0, etors_compat=True, fbzl_r=(5.0e-8, 5.5e-8)):
+        L = lib()
+        self.n = int(len(elem))
+        self._keep = [np.ascontiguousarray(elem, np.int32), np.ascontiguousarray(xyz_cm, np.float64),
+                      None if vel_cm_s is None else np.ascontiguousarray(vel_cm_s, np.float64),
+                      np.ascontiguousarray(pbc_cm, np.float64),
+                      None if fixed is None else np.ascontiguousarray(fixed, np.uint8)]
+        e, x, v, p, f = self._keep
+        self.h = L.orc_create(self.n, _p(e, ctypes.c_int), _p(x), _p(v), _p(p), _p(f, ctypes.c_ubyte),
+                              int(pots), int(brenner_set), int(bool(etors_compat)), fbzl_r[0], fbzl_r[1])
+        if not self.h:
+            raise OracleError(L.orc_last_error().decode())
+        self.nterms = L.orc_num_terms(self.h)
+        self.kinds = [L.orc_term_kind(self.h, i) for i in range(self.nterms)]
+
+    @classmethod
+    def from_system(cls, sys, **kw):
+        e, x, v, p, f = sys.to_cgs()
+        return cls(e, x, v, p, f, **kw)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise OracleError(lib().orc_last_error().decode())
+
+    def energy_grad(self):
+        e = np.zeros(self.nterms)
+        g = np.zeros((self.n, 3))
+        self._chk(lib().orc_energy_grad(self.h, _p(e), _p(g)))
+        return e, g
+
+    def neighbour_lists(self, term: int):
+        counts = np.zeros(self.n, np.int32)
+        cap = self.n * 1024
+        idx = np.zeros(cap, np.int32)
+        rc = lib().orc_nl(self.h, term, _p(counts, ctypes.c_int), _p(idx, ctypes.c_int), cap)
+        if rc == -1:
+            raise OracleError("neighbour list capacity")
+        self._chk(rc)
+        off = np.concatenate([[0], np.cumsum(counts)])
+        return [idx[off[i]:off[i + 1]].copy() for i in range(self.n)]
+
+    def run(self, nsteps: int, fixed_dt: float = 0.0):
+        ep, ek, dt, nf = (np.zeros(nsteps) for _ in range(4))
+        self._chk(lib().orc_run(self.h, nsteps, fixed_dt, _p(ep), _p(ek), _p(dt), _p(nf)))
+        return dict(epot=ep, ekin=ek, dt=dt, netforce=nf)
+
+    def state(self):
+        x, v, a = (np.zeros((self.n, 3)) for _ in range(3))
+        c = np.zeros((self.n, 3), np.int32)
+        lib().orc_get_state(self.h, _p(x), _p(v), _p(a), _p(c, ctypes.c_int))
+        return x, v, a, c
+
+
+def spline_eval(table: int, x: float, y: float = 0.0, z: float = 0.0, which: int = 0) -> float:
+    out = np.zeros(1)
+    rc = lib().orc_spline_eval(table, x, y, z, which, _p(out))
+    if rc:
+        raise OracleError(lib().orc_last_error().decode())
+    return float(out[0])

@@ -1,0 +1,211 @@
+// C entry points of the oracle, for tests and the CPU-baseline leg of the
+// benchmark.  TEST INFRASTRUCTURE ONLY (see oracle_core.hpp).
+//
+// Units are the reference's CGS units (cm, cm/s, erg).  A non-periodic axis is
+// given as NO_PBC = 1e4 Angstrom = 1e-4 cm, as in mdtk/consts.hpp:96.
+#include <cstring>
+#include <string>
+
+#include "oracle_core.hpp"
+
+using namespace orc;
+
+namespace {
+thread_local std::string g_err;
+
+enum : int { POT_FBZL = 1, POT_BRENNER = 2, POT_REBO = 4, POT_AIREBO = 8, POT_ETORS = 16 };
+
+struct Handle {
+  SimLoop sl;
+  std::vector<int> kinds;  // kind of each chain entry, in chain order
+};
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+void orc_set_alter_exact(int on) { g_alter_exact = on != 0; }
+
+// Chain order follows applications/common.h:46-68: FBZL, (FLJ: inert for H/C/Ar),
+// REBO, AIREBO(LJ), ETors.  Brenner replaces REBO when selected.
+void* orc_create(int n, const int* elem, const double* xyz, const double* vel, const double* pbc,
+                 const unsigned char* fixed, int pot_mask, int brenner_set, int etors_compat,
+                 double fbzl_r0, double fbzl_r1) {
+  Handle* h = nullptr;
+  int rc = guarded([&] {
+    h = new Handle();
+    auto& at = h->sl.atoms;
+    at.resize(n);
+    for (int i = 0; i < n; i++) {
+      at[i].ID = elem[i];
+      at[i].coords = V3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+      if (vel) at[i].V = V3(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
+      at[i].PBC = V3(pbc[0], pbc[1], pbc[2]);
+      at[i].fixed = fixed ? (fixed[i] != 0) : false;
+      if (at[i].fixed) at[i].V = V3();
+      at[i].globalIndex = i;
+      at[i].setAttributesByElementID();
+    }
+    auto& pots = h->sl.fpot.pots;
+    if (pot_mask & POT_FBZL) {
+      pots.push_back(new FBZL(fbzl_r0, fbzl_r1));
+      h->kinds.push_back(POT_FBZL);
+    }
+    REBO* rebo = nullptr;
+    if (pot_mask & POT_BRENNER) {
+      pots.push_back(new Brenner(brenner_set));
+      h->kinds.push_back(POT_BRENNER);
+    }
+    if (pot_mask & (POT_REBO | POT_AIREBO)) {
+      rebo = new REBO();
+      pots.push_back(rebo);
+      h->kinds.push_back(POT_REBO);
+    }
+    if (pot_mask & POT_AIREBO) {
+      pots.push_back(new AIREBO_LJ(*rebo));
+      h->kinds.push_back(POT_AIREBO);
+    }
+    if (pot_mask & POT_ETORS) {
+      auto* t = new ETors();
+      t->compat = etors_compat != 0;
+      pots.push_back(t);
+      h->kinds.push_back(POT_ETORS);
+    }
+    if (pots.empty()) throw OracleError("no potentials selected");
+  });
+  if (rc) {
+    delete h;
+    return nullptr;
+  }
+  return h;
+}
+
+void orc_destroy(void* p) { delete static_cast<Handle*>(p); }
+
+int orc_num_terms(void* p) { return (int)static_cast<Handle*>(p)->kinds.size(); }
+int orc_term_kind(void* p, int i) { return static_cast<Handle*>(p)->kinds[i]; }
+
+// Energy (per chain entry, erg) and dE/dx (erg/cm) at the current positions,
+// exactly as one FProxy call made from SimLoop::energy().
+int orc_energy_grad(void* p, double* e_terms, double* grad) {
+  Handle* h = static_cast<Handle*>(p);
+  return guarded([&] {
+    auto& sl = h->sl;
+    if (!sl.prepared) sl.prepare();
+    sl.fpot.NL_UpdateIfNeeded(sl.atoms);
+    sl.fpot.compute(sl.atoms);
+    for (size_t i = 0; i < sl.fpot.lastE.size(); i++) e_terms[i] = sl.fpot.lastE[i];
+    if (grad)
+      for (size_t i = 0; i < sl.atoms.size(); i++) {
+        grad[3 * i] = sl.atoms[i].grad.x;
+        grad[3 * i + 1] = sl.atoms[i].grad.y;
+        grad[3 * i + 2] = sl.atoms[i].grad.z;
+      }
+  });
+}
+
+// Neighbour list of chain entry `term`: counts[n], then the concatenated lists
+// (ascending index per atom) into idx (capacity cap).  Returns -1 on overflow.
+int orc_nl(void* p, int term, int* counts, int* idx, long cap) {
+  Handle* h = static_cast<Handle*>(p);
+  auto& sl = h->sl;
+  int rc = guarded([&] {
+    if (!sl.prepared) sl.prepare();
+    sl.fpot.NL_UpdateIfNeeded(sl.atoms);
+  });
+  if (rc) return rc;
+  auto& nl = sl.fpot.pots[term]->nl.nl;
+  long pos = 0;
+  for (size_t i = 0; i < nl.size(); i++) {
+    counts[i] = (int)nl[i].size();
+    for (int j : nl[i]) {
+      if (pos >= cap) return -1;
+      idx[pos++] = j;
+    }
+  }
+  return 0;
+}
+
+// Run nsteps iterations of the Verlet loop.  fixed_dt > 0 fixes dt (seconds);
+// otherwise the reference's adaptive rule is used.  Per step: epot (erg, at
+// t+dt), ekin (erg, after the velocity update: synchronised with epot) and the
+// dt that was used.  Any output pointer may be null.
+int orc_run(void* p, int nsteps, double fixed_dt, double* epot, double* ekin, double* dts,
+            double* netforce) {
+  Handle* h = static_cast<Handle*>(p);
+  return guarded([&] {
+    auto& sl = h->sl;
+    sl.fixed_dt = fixed_dt > 0;
+    sl.fixed_dt_value = fixed_dt;
+    for (int s = 0; s < nsteps; s++) {
+      double dt_used = sl.fixed_dt ? fixed_dt : sl.dt;
+      sl.step();
+      if (epot) epot[s] = sl.Epot;
+      if (ekin) ekin[s] = sl.Ekin;
+      if (dts) dts[s] = dt_used;
+      if (netforce) netforce[s] = norm(sl.netForce);
+    }
+  });
+}
+
+int orc_get_state(void* p, double* x, double* v, double* a, int* pbc_count) {
+  Handle* h = static_cast<Handle*>(p);
+  auto& at = h->sl.atoms;
+  for (size_t i = 0; i < at.size(); i++) {
+    if (x) { x[3 * i] = at[i].coords.x; x[3 * i + 1] = at[i].coords.y; x[3 * i + 2] = at[i].coords.z; }
+    if (v) { v[3 * i] = at[i].V.x; v[3 * i + 1] = at[i].V.y; v[3 * i + 2] = at[i].V.z; }
+    if (a) { a[3 * i] = at[i].an.x; a[3 * i + 1] = at[i].an.y; a[3 * i + 2] = at[i].an.z; }
+    if (pbc_count)
+      for (int d = 0; d < 3; d++) pbc_count[3 * i + d] = at[i].pbc_count[d];
+  }
+  return 0;
+}
+
+// Evaluate one spline table of the oracle at (x,y,z): table ids
+// 0 G_C1, 1 G_C2, 2 G_H (x = cos), 3 P_CC, 4 P_CH (x=NH, y=NC),
+// 5 pi_rc_CC, 6 pi_rc_CH, 7 pi_rc_HH, 8 T_ij (x=Ni, y=Nj, z=Nconj),
+// 9/10 Brenner-I H_CC/H_CH, 11 Brenner-I F, 12-14 the same for Brenner-II.
+// which: 0 value, 1..3 partial derivative along x,y,z.
+int orc_spline_eval(int table, double x, double y, double z, int which, double* out) {
+  static REBO* rebo = nullptr;
+  static Brenner* b1 = nullptr;
+  static Brenner* b2 = nullptr;
+  return guarded([&] {
+    if (!rebo) rebo = new REBO();
+    if (!b1) b1 = new Brenner(0);
+    if (!b2) b2 = new Brenner(1);
+    double r = 0;
+    switch (table) {
+      case 0: r = which ? rebo->gC1.d(x) : rebo->gC1(x); break;
+      case 1: r = which ? rebo->gC2.d(x) : rebo->gC2(x); break;
+      case 2: r = which ? rebo->gH.d(x) : rebo->gH(x); break;
+      case 3: r = rebo->pCC.eval(x, y, which); break;
+      case 4: r = rebo->pCH.eval(x, y, which); break;
+      case 5: r = rebo->rcCC.eval(x, y, z, which); break;
+      case 6: r = rebo->rcCH.eval(x, y, z, which); break;
+      case 7: r = rebo->rcHH.eval(x, y, z, which); break;
+      case 8: r = rebo->tij.eval(x, y, z, which); break;
+      case 9: r = b1->hCC.eval(x, y, which); break;
+      case 10: r = b1->hCH.eval(x, y, which); break;
+      case 11: r = b1->fF.eval(x, y, z, which); break;
+      case 12: r = b2->hCC.eval(x, y, which); break;
+      case 13: r = b2->hCH.eval(x, y, which); break;
+      case 14: r = b2->fF.eval(x, y, z, which); break;
+      default: throw OracleError("unknown table");
+    }
+    *out = r;
+  });
+}
+}
