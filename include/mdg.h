@@ -1,0 +1,116 @@
+/* mdg.h - C ABI of the MI355X force-and-integration engine for MDTK-style
+ * hydrocarbon molecular dynamics (REBO/Brenner, AIREBO-LJ, torsion, ZBL and
+ * the velocity-Verlet step).
+ *
+ * All arguments are plain pointers and sizes; no C++ or torch types cross the
+ * boundary, no exceptions escape it.  Every call returns 0 on success and a
+ * nonzero status otherwise; mdg_last_error() describes the failure.
+ * Units are the reference's CGS units: positions in cm, velocities in cm/s,
+ * energies in erg, gradients in erg/cm (mdtk/consts.cxx:28-38).  An open
+ * (non-periodic) axis is given as NO_PBC = 1e-4 cm (mdtk/consts.hpp:96).
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef MDG_H
+#define MDG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mdg_ctx mdg_ctx;
+
+/* Potential chain members, in the order applications/common.h:46-68 adds
+ * them: FBZL (ZBL ion-target), Brenner (replaces REBO), REBO, AIREBO-LJ
+ * (requires REBO), ETors. */
+enum {
+  MDG_POT_FBZL = 1,
+  MDG_POT_BRENNER = 2,
+  MDG_POT_REBO = 4,
+  MDG_POT_AIREBO = 8,
+  MDG_POT_ETORS = 16
+};
+
+enum { MDG_PREC_FP64 = 0, MDG_PREC_MIXED = 1 };
+
+typedef struct mdg_config {
+  int pot_mask;          /* OR of MDG_POT_* */
+  int brenner_paramset;  /* 0: Brenner::POTENTIAL1, 1: POTENTIAL2 (Brenner.cxx:497-611) */
+  int etors_compat;      /* 1: reference torsion derivative (AIREBO_ETors.cxx:110-111), 0: exact adjoint */
+  int precision;         /* MDG_PREC_FP64 or MDG_PREC_MIXED */
+  double fbzl_r0;        /* FBZL switch start/end, cm (common.h:46: 5.0/5.5 Angstrom) */
+  double fbzl_r1;
+  int max_short_nb;      /* capacity of one atom's REBO-range list (0: default 64) */
+  int max_long_nb;       /* capacity of one atom's LJ-range list (0: default 512) */
+} mdg_config;
+
+/* Replaces constructing the potentials and FProxy::addPotential
+ * (applications/common.h:36-68, mdtk/potentials/FProxy.hpp:37-40). */
+int mdg_create(const mdg_config* cfg, mdg_ctx** out);
+
+/* Replaces FProxy::freePotentials / ~SimLoop (FProxy.hpp:49-53, SimLoop.cxx:136-139). */
+void mdg_destroy(mdg_ctx* ctx);
+
+/* Message for the last failing call on this context (or on mdg_create). */
+const char* mdg_last_error(const mdg_ctx* ctx);
+
+/* Uploads the system: replaces AtomsArray::prepareForSimulatation,
+ * checkMIC and FProxy::NL_init (AtomsContainer.cxx:106-123,
+ * SimLoop.cxx:236-250).  elem: mdtk ElementID (H=1, C=12, Ar=40, Xe=131);
+ * xyz/vel: n*3 interleaved; pbc[3]: box lengths, NO_PBC for an open axis;
+ * fixed: n flags (may be null).  Positions are wrapped into the box. */
+int mdg_set_system(mdg_ctx* ctx, int n, const int32_t* elem, const double* xyz, const double* vel,
+                   const double pbc[3], const uint8_t* fixed);
+
+/* Number of chain members and the MDG_POT_* kind of member i. */
+int mdg_num_terms(const mdg_ctx* ctx);
+int mdg_term_kind(const mdg_ctx* ctx, int i);
+
+/* One force call at the current positions: replaces FProxy::operator()
+ * (FProxy.cxx:45-61) with each FGeneral::operator() (FGeneral.hpp:94).
+ * e_terms[mdg_num_terms]: energy of each member (erg); grad[n*3]: dE/dx
+ * (erg/cm), i.e. Atom::grad after the call.  Rebuilds a neighbour list first
+ * when its skin test asks for it (NeighbourList.cxx:108-133). */
+int mdg_energy_grad(mdg_ctx* ctx, double* e_terms, double* grad);
+
+/* Neighbour list of chain member `term` as the reference would hold it
+ * (FGeneral::nl, NeighbourList.hpp:41-83): counts[n], then the lists
+ * concatenated (ascending index per atom) into idx[cap]. */
+int mdg_nl_export(mdg_ctx* ctx, int term, int32_t* counts, int32_t* idx, int64_t cap);
+
+/* nsteps iterations of SimLoop::executeMain's loop (SimLoop.cxx:252-445),
+ * NVE: eq.1 drift, skin test and rebuild, force call, eq.2-4 kick, PBC wrap,
+ * dt rule.  fixed_dt > 0 fixes dt (as a doBeforeIteration hook would);
+ * otherwise dt = 0.05 A / v_max clamped to [1e-20, 5e-16] s (:409-416).
+ * Per step outputs (each may be null): epot = E_pot(t+dt) (erg), ekin =
+ * E_kin after the velocity update (erg), dt_used (s), netforce = |sum F|. */
+int mdg_step_nve(mdg_ctx* ctx, int nsteps, double fixed_dt, double* epot, double* ekin, double* dt_used,
+                 double* netforce);
+
+/* Current state: replaces reading Atom::coords/V/an/PBC_count. */
+int mdg_get_state(mdg_ctx* ctx, double* xyz, double* vel, double* acc, int32_t* pbc_count);
+
+/* Runs the context's kernels on the given HIP stream (hipStream_t passed as
+ * void*; null restores the context's own stream).  No reference counterpart:
+ * lets a caller time or order the work on its own stream. */
+int mdg_set_stream(mdg_ctx* ctx, void* stream);
+
+/* Per-kernel timing with HIP events on the context's stream: on != 0 starts
+ * collecting; mdg_kernel_time returns the summed milliseconds and launch
+ * count of the named kernel since profiling was enabled.  No reference
+ * counterpart (instrumentation). */
+int mdg_profile(mdg_ctx* ctx, int on);
+int mdg_kernel_time(mdg_ctx* ctx, const char* kernel, double* total_ms, int64_t* launches);
+
+/* Interaction counts of the last force call (for roofline accounting):
+ * out[0] bonds, out[1] LJ pairs inside the cutoff, out[2] b* evaluations,
+ * out[3] torsion quads, out[4] short-list entries, out[5] long-list entries,
+ * out[6] neighbour-list rebuilds so far.  No reference counterpart. */
+int mdg_counters(mdg_ctx* ctx, int64_t out[8]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

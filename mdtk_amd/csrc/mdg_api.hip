@@ -1,0 +1,755 @@
+// Host side of the C ABI declared in include/mdg.h: one context owns one
+// system's device buffers, the parameter tables and a HIP stream; every call
+// is stream-ordered and synchronous unless it says otherwise.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mdg.h"
+#include "mdg_kernels.hpp"
+
+using namespace mdg;
+
+namespace {
+thread_local std::string g_create_err;
+
+struct KStat {
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+  double ms = 0;
+  long long launches = 0;
+};
+
+struct Buf {
+  void* p = nullptr;
+};
+}  // namespace
+
+struct mdg_ctx {
+  mdg_config cfg{};
+  std::string err;
+  std::vector<int> kinds;
+  int slot_fbzl = -1, slot_rebo = -1, slot_bren = -1, slot_lj = -1, slot_tors = -1;
+  bool have_short = false, have_long = false;
+  double rc_short = 0, rc_long = 0;
+
+  int n = 0;
+  Box box{};
+  Atoms A{};
+  Tables* dT = nullptr;
+  hipStream_t own = nullptr, stream = nullptr;
+  std::vector<void*> allocs;
+
+  CellGrid gs{}, gl{};
+  int ncs = 0, ncl = 0;
+  int *cell_of_s = nullptr, *cnt_s = nullptr, *start_s = nullptr, *atoms_s = nullptr;
+  int *cell_of_l = nullptr, *cnt_l = nullptr, *start_l = nullptr, *atoms_l = nullptr;
+  unsigned char *handled_s = nullptr, *handled_l = nullptr;
+  int* flags = nullptr;  // [0] short, [1] long rebuild requested
+  double* part = nullptr;
+  int2* bonds = nullptr;
+  int* nbonds = nullptr;
+  int bond_cap = 0;
+  int4* queue = nullptr;
+  int* qcount = nullptr;
+  int qcap = 0;
+  double* eacc = nullptr;
+  double* red = nullptr;
+  double* dtp = nullptr;
+  double* stepout = nullptr;
+  int step_cap = 0;
+  long long* counters = nullptr;  // [0] short rebuilds [1] long rebuilds [2] LJ pairs [3] quads
+  unsigned* derr = nullptr;
+  long iteration = 0;
+  double dt_host = 1e-20;  // SimLoop.cxx:50
+
+  bool profiling = false;
+  std::map<std::string, KStat> kst;
+  std::vector<hipEvent_t> event_pool;
+
+  ~mdg_ctx() {
+    if (stream) hipStreamSynchronize(stream);
+    for (auto& kv : kst)
+      for (auto& pr : kv.second.pending) {
+        hipEventDestroy(pr.first);
+        hipEventDestroy(pr.second);
+      }
+    for (void* p : allocs) hipFree(p);
+    if (dT) hipFree(dT);
+    if (own) hipStreamDestroy(own);
+  }
+};
+
+namespace {
+#define HIPCHK(ctx, call)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                      \
+      return 1;                                                                             \
+    }                                                                                       \
+  } while (0)
+
+template <class T>
+int dalloc(mdg_ctx* c, T*& p, size_t count) {
+  void* q = nullptr;
+  size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) {
+    c->err = "hipMalloc(" + std::to_string(bytes) + " bytes): " + hipGetErrorString(e);
+    return 1;
+  }
+  c->allocs.push_back(q);
+  p = static_cast<T*>(q);
+  return 0;
+}
+
+int type_of(int elem) {
+  switch (elem) {
+    case 12: return T_C;
+    case 1: return T_H;
+    case 40:
+    case 131: return T_ION;
+    default: return T_OTHER;
+  }
+}
+
+double mass_of(int elem) {  // mdtk/Atom.hpp:98-113
+  switch (elem) {
+    case 1: return 1.0 * kAmu;
+    case 12: return 12.0 * kAmu;
+    case 64: return 63.546 * kAmu;
+    case 108: return 107.868 * kAmu;
+    case 197: return 196.967 * kAmu;
+    case 40: return 39.948 * kAmu;
+    case 131: return 131.293 * kAmu;
+    default: return -1.0;
+  }
+}
+
+double charge_of(int elem) {
+  switch (elem) {
+    case 1: return 1.0 * kECharge;
+    case 12: return 6.0 * kECharge;
+    case 64: return 29.0 * kECharge;
+    case 108: return 47.0 * kECharge;
+    case 197: return 79.0 * kECharge;
+    case 40: return 18.0 * kECharge;
+    case 131: return 54.0 * kECharge;
+    default: return 0.0;
+  }
+}
+
+// Launch wrapper: records HIP events around the launch when profiling.
+template <class F>
+void launch(mdg_ctx* c, const char* name, F&& f) {
+  if (!c->profiling) {
+    f();
+    return;
+  }
+  hipEvent_t a, b;
+  hipEventCreate(&a);
+  hipEventCreate(&b);
+  hipEventRecord(a, c->stream);
+  f();
+  hipEventRecord(b, c->stream);
+  c->kst[name].pending.push_back({a, b});
+}
+
+void collect_profile(mdg_ctx* c) {
+  for (auto& kv : c->kst) {
+    for (auto& pr : kv.second.pending) {
+      hipEventSynchronize(pr.second);
+      float ms = 0;
+      hipEventElapsedTime(&ms, pr.first, pr.second);
+      kv.second.ms += ms;
+      kv.second.launches += 1;
+      hipEventDestroy(pr.first);
+      hipEventDestroy(pr.second);
+    }
+    kv.second.pending.clear();
+  }
+}
+
+CellGrid make_grid(const double L[3], const std::vector<double>& xyz, int n, double range) {
+  CellGrid g{};
+  for (int d = 0; d < 3; d++) {
+    if (L[d] != kNoPbc) {
+      g.periodic[d] = 1;
+      g.lo[d] = 0.0;
+      int m = (int)std::floor(L[d] / range);
+      if (m < 1) m = 1;
+      g.n[d] = m;
+      g.w[d] = L[d] / m;
+    } else {
+      g.periodic[d] = 0;
+      double lo = 1e300, hi = -1e300;
+      for (int i = 0; i < n; i++) {
+        lo = std::min(lo, xyz[3 * i + d]);
+        hi = std::max(hi, xyz[3 * i + d]);
+      }
+      double margin = 4.0 * range;
+      lo -= margin;
+      hi += margin;
+      double ext = hi - lo;
+      int m = (int)std::floor(ext / range);
+      if (m < 1) m = 1;
+      if (m > 512) m = 512;
+      g.n[d] = m;
+      g.lo[d] = lo;
+      g.w[d] = ext / m;
+    }
+  }
+  return g;
+}
+
+int check_err(mdg_ctx* c) {
+  unsigned h = 0;
+  HIPCHK(c, hipMemcpyAsync(&h, c->derr, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (h) {
+    std::string m = "device error:";
+    if (h & ERR_SHORT_NL_OVERFLOW) m += " short-list overflow (raise max_short_nb);";
+    if (h & ERR_LONG_NL_OVERFLOW) m += " long-list overflow (raise max_long_nb);";
+    if (h & ERR_BOND_OVERFLOW) m += " bond-list overflow;";
+    if (h & ERR_COS_RANGE) m += " cos(theta) outside [-1-1e-7, 1+1e-7] (FGeneral.hpp:147);";
+    if (h & ERR_NEG_COORD) m += " coordination below -1e-15 (REBO.cxx:434);";
+    if (h & ERR_QUEUE_OVERFLOW) m += " work-queue overflow;";
+    if (h & ERR_SPLINE_DOMAIN) m += " spline argument outside its domain;";
+    unsigned z = 0;
+    hipMemcpy(c->derr, &z, sizeof(unsigned), hipMemcpyHostToDevice);
+    c->err = m;
+    return 1;
+  }
+  return 0;
+}
+
+inline int blocks(long long work, int bs = 256, int cap = 1 << 20) {
+  long long b = (work + bs - 1) / bs;
+  if (b < 1) b = 1;
+  if (b > cap) b = cap;
+  return (int)b;
+}
+
+// Neighbour-list rebuild for one list, gated on the device flag.
+void nl_build(mdg_ctx* c, bool shortlist) {
+  const int* flag = c->flags + (shortlist ? 0 : 1);
+  CellGrid& g = shortlist ? c->gs : c->gl;
+  int nc = shortlist ? c->ncs : c->ncl;
+  int* cell_of = shortlist ? c->cell_of_s : c->cell_of_l;
+  int* cnt = shortlist ? c->cnt_s : c->cnt_l;
+  int* start = shortlist ? c->start_s : c->start_l;
+  int* cat = shortlist ? c->atoms_s : c->atoms_l;
+  double rc = shortlist ? c->rc_short : c->rc_long;
+  double rr = (1.0 + 0.44) * rc;  // NeighbourList.cxx:62
+  double range2 = rr * rr;
+  int n = c->n;
+  Atoms& A = c->A;
+  launch(c, "k_cell_clear", [&] { k_cell_clear<<<blocks(nc), 256, 0, c->stream>>>(flag, nc, cnt); });
+  launch(c, "k_cell_assign",
+         [&] { k_cell_assign<<<blocks(n), 256, 0, c->stream>>>(flag, n, A.x, A.y, A.z, g, cell_of, cnt); });
+  launch(c, "k_cell_scan", [&] { k_cell_scan<<<1, 1024, 0, c->stream>>>(flag, nc, cnt, start); });
+  launch(c, "k_cell_fill",
+         [&] { k_cell_fill<<<blocks(n), 256, 0, c->stream>>>(flag, n, cell_of, start, cnt, cat); });
+  if (shortlist) {
+    launch(c, "k_nl_build_short", [&] {
+      k_nl_build<64><<<blocks(n, 4), 256, 0, c->stream>>>(flag, n, A.x, A.y, A.z, c->handled_s, c->box, g, range2,
+                                                           cell_of, start, cat, A.nls_cnt, A.nls, A.disp_s,
+                                                           c->derr);
+    });
+  } else {
+    launch(c, "k_nl_build_long", [&] {
+      k_nl_build<512><<<blocks(n, 4), 256, 0, c->stream>>>(flag, n, A.x, A.y, A.z, c->handled_l, c->box, g,
+                                                            range2, cell_of, start, cat, A.nll_cnt, A.nll,
+                                                            A.disp_l, c->derr);
+    });
+  }
+}
+
+__global__ void k_clear_flags(int* flags) {
+  if (threadIdx.x < 2) flags[threadIdx.x] = 0;
+}
+
+__global__ void k_zero_counts(long long* counters) {
+  if (threadIdx.x == 2 || threadIdx.x == 3) counters[threadIdx.x] = 0;
+}
+
+// One force call (FProxy::operator(), FProxy.cxx:45-61): lists, coordination,
+// then every chain member adds into grad and its energy slot.
+void force_call(mdg_ctx* c) {
+  int n = c->n;
+  Atoms& A = c->A;
+  if (c->have_short) nl_build(c, true);
+  if (c->have_long) nl_build(c, false);
+  launch(c, "k_zero", [&] { k_zero<<<blocks(3LL * n, 256, 4096), 256, 0, c->stream>>>(A.grad, (size_t)3 * n); });
+  k_zero<<<1, 64, 0, c->stream>>>(c->eacc, 8);
+  k_zero_i<<<1, 64, 0, c->stream>>>(c->nbonds, 1);
+  k_zero_i<<<1, 64, 0, c->stream>>>(c->qcount, 1);
+  k_zero_counts<<<1, 64, 0, c->stream>>>(c->counters);
+  if (c->have_short) {
+    launch(c, "k_coord", [&] { k_coord<<<blocks(n), 256, 0, c->stream>>>(A, c->dT, c->box, c->derr); });
+    launch(c, "k_bond_enum", [&] {
+      k_bond_enum<<<blocks(n), 256, 0, c->stream>>>(A, c->bonds, c->nbonds, c->bond_cap, c->derr);
+    });
+  }
+  int bgrid = blocks(2LL * n, 256, 8192);
+  if (c->slot_rebo >= 0) {
+    launch(c, "k_rebo_bonds", [&] {
+      k_rebo_bonds<<<bgrid, 256, 0, c->stream>>>(A, c->dT, c->box, c->bonds, c->nbonds, A.grad,
+                                                 c->eacc + c->slot_rebo, c->derr);
+    });
+  }
+  if (c->slot_bren >= 0) {
+    launch(c, "k_brenner_bonds", [&] {
+      k_brenner_bonds<<<bgrid, 256, 0, c->stream>>>(A, c->dT, c->box, c->bonds, c->nbonds, A.grad,
+                                                    c->eacc + c->slot_bren, c->derr);
+    });
+  }
+  if (c->slot_lj >= 0) {
+    launch(c, "k_cij_build", [&] { k_cij_build<<<blocks(n), 256, 0, c->stream>>>(A, c->derr); });
+    launch(c, "k_lj_pairs", [&] {
+      k_lj_pairs<<<blocks(n), 256, 0, c->stream>>>(A, c->dT, c->box, A.grad, c->eacc + c->slot_lj, c->queue,
+                                                   c->qcount, c->qcap, c->counters + 2, c->derr);
+    });
+    launch(c, "k_bstar", [&] {
+      k_bstar<<<blocks(8LL * n, 256, 16384), 256, 0, c->stream>>>(A, c->dT, c->box, c->queue, c->qcount, c->qcap,
+                                                                  A.grad, c->eacc + c->slot_lj, c->derr);
+    });
+  }
+  if (c->slot_tors >= 0) {
+    launch(c, "k_etors", [&] {
+      k_etors<<<bgrid, 256, 0, c->stream>>>(A, c->dT, c->box, c->bonds, c->nbonds, c->cfg.etors_compat, A.grad,
+                                            c->eacc + c->slot_tors, c->counters + 3);
+    });
+  }
+  if (c->slot_fbzl >= 0) {
+    launch(c, "k_fbzl", [&] {
+      k_fbzl<<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->cfg.fbzl_r0, c->cfg.fbzl_r1, A.grad,
+                                               c->eacc + c->slot_fbzl);
+    });
+  }
+  k_clear_flags<<<1, 64, 0, c->stream>>>(c->flags);
+}
+}  // namespace
+
+extern "C" {
+
+const char* mdg_last_error(const mdg_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
+  *out = nullptr;
+  if (!cfg) {
+    g_create_err = "null config";
+    return 1;
+  }
+  int m = cfg->pot_mask;
+  if ((m & MDG_POT_AIREBO) && !(m & MDG_POT_REBO)) {
+    g_create_err = "AIREBO-LJ needs REBO in the chain (AIREBO_LJ.hpp:63)";
+    return 1;
+  }
+  if ((m & MDG_POT_BRENNER) && (m & (MDG_POT_REBO | MDG_POT_AIREBO))) {
+    g_create_err = "Brenner replaces REBO: select one of them";
+    return 1;
+  }
+  if (!(m & (MDG_POT_REBO | MDG_POT_BRENNER | MDG_POT_ETORS | MDG_POT_FBZL))) {
+    g_create_err = "no potentials selected";
+    return 1;
+  }
+  if (cfg->precision != MDG_PREC_FP64) {
+    g_create_err = "this library build computes in FP64 only (precision must be MDG_PREC_FP64)";
+    return 1;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    g_create_err = "no HIP device";
+    return 1;
+  }
+  mdg_ctx* c = new mdg_ctx();
+  c->cfg = *cfg;
+  if (c->cfg.max_short_nb <= 0) c->cfg.max_short_nb = 64;
+  if (c->cfg.max_long_nb <= 0) c->cfg.max_long_nb = 512;
+  if (c->cfg.max_short_nb != 64 || c->cfg.max_long_nb != 512) {
+    g_create_err = "list capacities are fixed at 64 (short) and 512 (long) in this build";
+    delete c;
+    return 1;
+  }
+  if (c->cfg.fbzl_r1 <= 0) {
+    c->cfg.fbzl_r0 = 5.0 * kAo;
+    c->cfg.fbzl_r1 = 5.5 * kAo;
+  }
+  // chain order of applications/common.h:46-68
+  if (m & MDG_POT_FBZL) { c->slot_fbzl = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_FBZL); }
+  if (m & MDG_POT_BRENNER) { c->slot_bren = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_BRENNER); }
+  if (m & MDG_POT_REBO) { c->slot_rebo = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_REBO); }
+  if (m & MDG_POT_AIREBO) { c->slot_lj = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_AIREBO); }
+  if (m & MDG_POT_ETORS) { c->slot_tors = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_ETORS); }
+  c->have_short = (m & (MDG_POT_REBO | MDG_POT_BRENNER | MDG_POT_ETORS)) != 0;
+  c->have_long = (m & (MDG_POT_AIREBO | MDG_POT_FBZL)) != 0;
+  c->rc_short = 2.0 * kAo;  // REBO/Brenner/ETors getRcutoff (REBO.hpp:130)
+  c->rc_long = 0;
+  if (m & MDG_POT_AIREBO) c->rc_long = 5.5 * kAo;  // AIREBO_LJ.hpp:81
+  if (m & MDG_POT_FBZL) c->rc_long = std::max(c->rc_long, c->cfg.fbzl_r1);
+  if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+    g_create_err = "hipStreamCreate failed";
+    delete c;
+    return 1;
+  }
+  c->stream = c->own;
+  Tables h;
+  build_tables(h, cfg->brenner_paramset);
+  if (hipMalloc(&c->dT, sizeof(Tables)) != hipSuccess ||
+      hipMemcpy(c->dT, &h, sizeof(Tables), hipMemcpyHostToDevice) != hipSuccess) {
+    g_create_err = "table upload failed";
+    delete c;
+    return 1;
+  }
+  if (dalloc(c, c->flags, 2) || dalloc(c, c->part, 4 * 256) || dalloc(c, c->eacc, 8) || dalloc(c, c->red, 8) ||
+      dalloc(c, c->dtp, 1) || dalloc(c, c->counters, 8) || dalloc(c, c->derr, 1) || dalloc(c, c->nbonds, 1) ||
+      dalloc(c, c->qcount, 1)) {
+    g_create_err = c->err;
+    delete c;
+    return 1;
+  }
+  hipMemset(c->counters, 0, 8 * sizeof(long long));
+  hipMemset(c->derr, 0, sizeof(unsigned));
+  hipMemset(c->red, 0, 8 * sizeof(double));
+  *out = c;
+  return 0;
+}
+
+void mdg_destroy(mdg_ctx* ctx) { delete ctx; }
+
+int mdg_num_terms(const mdg_ctx* ctx) { return (int)ctx->kinds.size(); }
+int mdg_term_kind(const mdg_ctx* ctx, int i) {
+  return (i >= 0 && i < (int)ctx->kinds.size()) ? ctx->kinds[i] : -1;
+}
+
+int mdg_set_stream(mdg_ctx* c, void* s) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->stream = s ? static_cast<hipStream_t>(s) : c->own;
+  return 0;
+}
+
+int mdg_set_system(mdg_ctx* c, int n, const int32_t* elem, const double* xyz, const double* vel,
+                   const double pbc[3], const uint8_t* fixed) {
+  if (n <= 0) {
+    c->err = "empty system (AtomsContainer.cxx:108)";
+    return 1;
+  }
+  if (c->n != 0) {
+    c->err = "mdg_set_system: a context holds one system; create a new context";
+    return 1;
+  }
+  std::vector<double> p(xyz, xyz + 3 * (size_t)n);
+  std::vector<int> pc(3 * (size_t)n, 0), ty(n), el(n);
+  std::vector<double> ms(n), zs(n);
+  std::vector<unsigned char> fx(n, 0), hs(n), hl(n);
+  for (int d = 0; d < 3; d++) c->box.L[d] = pbc[d];
+  for (int i = 0; i < n; i++) {
+    el[i] = elem[i];
+    ty[i] = type_of(elem[i]);
+    ms[i] = mass_of(elem[i]);
+    zs[i] = charge_of(elem[i]);
+    if (ms[i] <= 0) {
+      c->err = "unknown element id " + std::to_string(elem[i]);
+      return 1;
+    }
+    fx[i] = fixed ? (fixed[i] ? 1 : 0) : 0;
+    hs[i] = (ty[i] <= T_H) ? 1 : 0;
+    hl[i] = 1;
+    for (int d = 0; d < 3; d++) {  // Atom::applyPBC (Atom.cxx:93-125)
+      double L = pbc[d];
+      if (L == kNoPbc) continue;
+      double& q = p[3 * i + d];
+      int& k = pc[3 * i + d];
+      int guard = 0;
+      while (q < 0 && guard++ < 1000000) { q += L; --k; }
+      while (q >= L && guard++ < 1000000) { q -= L; ++k; }
+    }
+  }
+  double rcmax = std::max(c->rc_short, c->rc_long);  // checkMIC(2 Rc), SimLoop.cxx:239-245
+  for (int d = 0; d < 3; d++)
+    if (pbc[d] <= rcmax * 2.0) {
+      c->err = "Rcutoff is too large for given PBC ! (SimLoop.cxx:239-245)";
+      return 1;
+    }
+  c->n = n;
+  Atoms& A = c->A;
+  A.n = n;
+  A.cap_s = c->cfg.max_short_nb;
+  A.cap_l = c->cfg.max_long_nb;
+  A.maxb = 16;
+  A.cap_c = 96;
+  int r = 0;
+  r |= dalloc(c, A.x, n) | dalloc(c, A.y, n) | dalloc(c, A.z, n);
+  r |= dalloc(c, A.vx, n) | dalloc(c, A.vy, n) | dalloc(c, A.vz, n);
+  r |= dalloc(c, A.ax, n) | dalloc(c, A.ay, n) | dalloc(c, A.az, n);
+  r |= dalloc(c, A.grad, 3 * (size_t)n) | dalloc(c, A.mass, n) | dalloc(c, A.Z, n);
+  r |= dalloc(c, A.type, n) | dalloc(c, A.elem, n) | dalloc(c, A.fixed, n);
+  r |= dalloc(c, A.pcx, n) | dalloc(c, A.pcy, n) | dalloc(c, A.pcz, n);
+  r |= dalloc(c, A.disp_s, 3 * (size_t)n) | dalloc(c, A.disp_l, 3 * (size_t)n);
+  r |= dalloc(c, A.nls_cnt, n) | dalloc(c, A.nll_cnt, n);
+  if (c->have_short) r |= dalloc(c, A.nls, (size_t)n * A.cap_s);
+  if (c->have_long) r |= dalloc(c, A.nll, (size_t)n * A.cap_l);
+  r |= dalloc(c, A.bcnt, n) | dalloc(c, A.bj, (size_t)n * A.maxb);
+  r |= dalloc(c, A.br, (size_t)n * A.maxb) | dalloc(c, A.bf, (size_t)n * A.maxb) | dalloc(c, A.bdf, (size_t)n * A.maxb);
+  r |= dalloc(c, A.bx, (size_t)n * A.maxb) | dalloc(c, A.by, (size_t)n * A.maxb) | dalloc(c, A.bz, (size_t)n * A.maxb);
+  r |= dalloc(c, A.Nt, n) | dalloc(c, A.NH, n) | dalloc(c, A.NC, n);
+  if (c->slot_lj >= 0) {
+    r |= dalloc(c, A.c_cnt, n) | dalloc(c, A.c_tgt, (size_t)n * A.cap_c) | dalloc(c, A.c_k, (size_t)n * A.cap_c);
+    r |= dalloc(c, A.c_km, (size_t)n * A.cap_c) | dalloc(c, A.c_w, (size_t)n * A.cap_c);
+  }
+  c->bond_cap = n * A.maxb / 2 + 64;
+  A.bond_cap = c->bond_cap;
+  r |= dalloc(c, c->bonds, c->bond_cap);
+  c->qcap = 32 * n + 1024;
+  if (c->slot_lj >= 0) r |= dalloc(c, c->queue, c->qcap);
+  r |= dalloc(c, c->handled_s, n) | dalloc(c, c->handled_l, n);
+  if (r) return 1;
+  // cell grids
+  if (c->have_short) {
+    c->gs = make_grid(c->box.L, p, n, (1.0 + 0.44) * c->rc_short);
+    c->ncs = c->gs.n[0] * c->gs.n[1] * c->gs.n[2];
+    r |= dalloc(c, c->cell_of_s, n) | dalloc(c, c->cnt_s, c->ncs) | dalloc(c, c->start_s, c->ncs + 1) |
+         dalloc(c, c->atoms_s, n);
+  }
+  if (c->have_long) {
+    c->gl = make_grid(c->box.L, p, n, (1.0 + 0.44) * c->rc_long);
+    c->ncl = c->gl.n[0] * c->gl.n[1] * c->gl.n[2];
+    r |= dalloc(c, c->cell_of_l, n) | dalloc(c, c->cnt_l, c->ncl) | dalloc(c, c->start_l, c->ncl + 1) |
+         dalloc(c, c->atoms_l, n);
+  }
+  if (r) return 1;
+  std::vector<double> xs(n), ys(n), zs2(n), vxs(n, 0.0), vys(n, 0.0), vzs(n, 0.0);
+  std::vector<int> px(n), py(n), pz(n);
+  for (int i = 0; i < n; i++) {
+    xs[i] = p[3 * i];
+    ys[i] = p[3 * i + 1];
+    zs2[i] = p[3 * i + 2];
+    if (vel && !fx[i]) {
+      vxs[i] = vel[3 * i];
+      vys[i] = vel[3 * i + 1];
+      vzs[i] = vel[3 * i + 2];
+    }
+    px[i] = pc[3 * i];
+    py[i] = pc[3 * i + 1];
+    pz[i] = pc[3 * i + 2];
+  }
+  auto up = [&](void* d, const void* s, size_t b) { return hipMemcpy(d, s, b, hipMemcpyHostToDevice); };
+  HIPCHK(c, up(A.x, xs.data(), n * 8));
+  HIPCHK(c, up(A.y, ys.data(), n * 8));
+  HIPCHK(c, up(A.z, zs2.data(), n * 8));
+  HIPCHK(c, up(A.vx, vxs.data(), n * 8));
+  HIPCHK(c, up(A.vy, vys.data(), n * 8));
+  HIPCHK(c, up(A.vz, vzs.data(), n * 8));
+  HIPCHK(c, hipMemset(A.ax, 0, n * 8));
+  HIPCHK(c, hipMemset(A.ay, 0, n * 8));
+  HIPCHK(c, hipMemset(A.az, 0, n * 8));
+  HIPCHK(c, up(A.mass, ms.data(), n * 8));
+  HIPCHK(c, up(A.Z, zs.data(), n * 8));
+  HIPCHK(c, up(A.type, ty.data(), n * 4));
+  HIPCHK(c, up(A.elem, el.data(), n * 4));
+  HIPCHK(c, up(A.fixed, fx.data(), n));
+  HIPCHK(c, up(A.pcx, px.data(), n * 4));
+  HIPCHK(c, up(A.pcy, py.data(), n * 4));
+  HIPCHK(c, up(A.pcz, pz.data(), n * 4));
+  HIPCHK(c, hipMemset(A.disp_s, 0, 3 * (size_t)n * 8));
+  HIPCHK(c, hipMemset(A.disp_l, 0, 3 * (size_t)n * 8));
+  HIPCHK(c, hipMemset(A.nls_cnt, 0, n * 4));
+  HIPCHK(c, hipMemset(A.nll_cnt, 0, n * 4));
+  HIPCHK(c, hipMemset(A.bcnt, 0, n * 4));
+  HIPCHK(c, up(c->handled_s, hs.data(), n));
+  HIPCHK(c, up(c->handled_l, hl.data(), n));
+  int fl[2] = {1, 1};
+  HIPCHK(c, up(c->flags, fl, sizeof(fl)));
+  HIPCHK(c, up(c->dtp, &c->dt_host, sizeof(double)));
+  c->iteration = 0;
+  return 0;
+}
+
+int mdg_energy_grad(mdg_ctx* c, double* e_terms, double* grad) {
+  if (c->n == 0) {
+    c->err = "no system";
+    return 1;
+  }
+  force_call(c);
+  if (check_err(c)) return 1;
+  std::vector<double> e(8);
+  HIPCHK(c, hipMemcpyAsync(e.data(), c->eacc, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (grad)
+    HIPCHK(c, hipMemcpyAsync(grad, c->A.grad, 3 * (size_t)c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (e_terms)
+    for (size_t t = 0; t < c->kinds.size(); t++) e_terms[t] = e[t];
+  return 0;
+}
+
+int mdg_nl_export(mdg_ctx* c, int term, int32_t* counts, int32_t* idx, int64_t cap) {
+  if (term < 0 || term >= (int)c->kinds.size()) {
+    c->err = "bad term";
+    return 1;
+  }
+  int kind = c->kinds[term];
+  bool shortlist = (kind == MDG_POT_REBO || kind == MDG_POT_BRENNER || kind == MDG_POT_ETORS);
+  int n = c->n;
+  // make sure lists reflect the current positions' rebuild state
+  nl_build(c, shortlist);
+  k_zero_i<<<1, 64, 0, c->stream>>>(c->flags + (shortlist ? 0 : 1), 1);
+  if (check_err(c)) return 1;
+  int stride = shortlist ? c->A.cap_s : c->A.cap_l;
+  std::vector<int> cnt(n), lst((size_t)n * stride), ty(n), el(n);
+  HIPCHK(c, hipMemcpy(cnt.data(), shortlist ? c->A.nls_cnt : c->A.nll_cnt, n * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(lst.data(), shortlist ? c->A.nls : c->A.nll, (size_t)n * stride * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(ty.data(), c->A.type, n * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(el.data(), c->A.elem, n * 4, hipMemcpyDeviceToHost));
+  auto zbl_handled = [](int e) { return e == 1 || e == 12 || e == 64 || e == 197 || e == 108 || e == 40 || e == 131; };
+  int64_t pos = 0;
+  for (int i = 0; i < n; i++) {
+    int m = 0;
+    bool row_ok = true;
+    if (kind == MDG_POT_AIREBO) row_ok = ty[i] <= T_H;
+    if (kind == MDG_POT_FBZL) row_ok = zbl_handled(el[i]);
+    if (row_ok)
+      for (int q = 0; q < cnt[i]; q++) {
+        int j = lst[(size_t)i * stride + q];
+        if (kind == MDG_POT_AIREBO && ty[j] > T_H) continue;
+        if (kind == MDG_POT_FBZL && !zbl_handled(el[j])) continue;
+        if (pos >= cap) {
+          c->err = "nl_export capacity";
+          return 1;
+        }
+        idx[pos++] = j;
+        m++;
+      }
+    counts[i] = m;
+  }
+  return 0;
+}
+
+int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* ekin, double* dt_used,
+                 double* netforce) {
+  if (c->n == 0) {
+    c->err = "no system";
+    return 1;
+  }
+  if (nsteps <= 0) return 0;
+  Atoms& A = c->A;
+  int n = c->n;
+  if (c->step_cap < nsteps) {
+    if (dalloc(c, c->stepout, (size_t)nsteps * 8)) return 1;
+    c->step_cap = nsteps;
+  }
+  int fixed = fixed_dt > 0 ? 1 : 0;
+  if (fixed) HIPCHK(c, hipMemcpyAsync(c->dtp, &fixed_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int nterms = (int)c->kinds.size();
+  if (c->iteration == 0) {  // SimLoop.cxx:305-321
+    force_call(c);
+    launch(c, "k_init_acc", [&] { k_init_acc<<<blocks(n), 256, 0, c->stream>>>(A); });
+  }
+  double lim_s = 0.44 * c->rc_short, lim_l = 0.44 * c->rc_long;
+  if (!c->have_short) lim_s = 1e300;
+  if (!c->have_long) lim_l = 1e300;
+  for (int s = 0; s < nsteps; s++) {
+    launch(c, "k_drift", [&] { k_drift<<<blocks(n), 256, 0, c->stream>>>(A, c->dtp); });
+    launch(c, "k_disp_top2", [&] { k_disp_top2<<<256, 256, 0, c->stream>>>(n, A.disp_s, A.disp_l, c->part); });
+    launch(c, "k_disp_decide",
+           [&] { k_disp_decide<<<1, 64, 0, c->stream>>>(256, c->part, lim_s, lim_l, c->flags, c->counters); });
+    force_call(c);
+    launch(c, "k_kick", [&] { k_kick<<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->dtp, c->red); });
+    k_step_end<<<1, 64, 0, c->stream>>>(nterms, c->eacc, c->red, c->dtp, fixed, c->stepout, s);
+  }
+  c->iteration += nsteps;
+  if (check_err(c)) return 1;
+  std::vector<double> o((size_t)nsteps * 8);
+  HIPCHK(c, hipMemcpy(o.data(), c->stepout, o.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int s = 0; s < nsteps; s++) {
+    if (epot) epot[s] = o[8 * s + 0];
+    if (ekin) ekin[s] = o[8 * s + 1];
+    if (dt_used) dt_used[s] = o[8 * s + 2];
+    if (netforce) netforce[s] = o[8 * s + 3];
+  }
+  if (!fixed && nsteps > 0) HIPCHK(c, hipMemcpy(&c->dt_host, c->dtp, sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int mdg_get_state(mdg_ctx* c, double* xyz, double* vel, double* acc, int32_t* pbc_count) {
+  int n = c->n;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<double> a(n), b(n), d(n);
+  auto get3 = [&](double* out, double* X, double* Y, double* Z) -> int {
+    if (!out) return 0;
+    HIPCHK(c, hipMemcpy(a.data(), X, n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(b.data(), Y, n * 8, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(d.data(), Z, n * 8, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++) {
+      out[3 * i] = a[i];
+      out[3 * i + 1] = b[i];
+      out[3 * i + 2] = d[i];
+    }
+    return 0;
+  };
+  if (get3(xyz, c->A.x, c->A.y, c->A.z) || get3(vel, c->A.vx, c->A.vy, c->A.vz) || get3(acc, c->A.ax, c->A.ay, c->A.az))
+    return 1;
+  if (pbc_count) {
+    std::vector<int> q(n);
+    int* src[3] = {c->A.pcx, c->A.pcy, c->A.pcz};
+    for (int dd = 0; dd < 3; dd++) {
+      HIPCHK(c, hipMemcpy(q.data(), src[dd], n * 4, hipMemcpyDeviceToHost));
+      for (int i = 0; i < n; i++) pbc_count[3 * i + dd] = q[i];
+    }
+  }
+  return 0;
+}
+
+int mdg_profile(mdg_ctx* c, int on) {
+  if (on) {
+    collect_profile(c);
+    c->kst.clear();
+  }
+  c->profiling = on != 0;
+  return 0;
+}
+
+int mdg_kernel_time(mdg_ctx* c, const char* kernel, double* total_ms, int64_t* launches) {
+  collect_profile(c);
+  auto it = c->kst.find(kernel);
+  if (it == c->kst.end()) {
+    *total_ms = 0;
+    *launches = 0;
+    return 0;
+  }
+  *total_ms = it->second.ms;
+  *launches = it->second.launches;
+  return 0;
+}
+
+int mdg_counters(mdg_ctx* c, int64_t out[8]) {
+  long long h[8];
+  int nb = 0, nq = 0;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(h, c->counters, sizeof(h), hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(&nb, c->nbonds, 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(&nq, c->qcount, 4, hipMemcpyDeviceToHost));
+  int n = c->n;
+  std::vector<int> cs(n), cl(n);
+  HIPCHK(c, hipMemcpy(cs.data(), c->A.nls_cnt, n * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(cl.data(), c->A.nll_cnt, n * 4, hipMemcpyDeviceToHost));
+  long long ss = 0, sl = 0;
+  for (int i = 0; i < n; i++) {
+    ss += cs[i];
+    sl += cl[i];
+  }
+  out[0] = nb;
+  out[1] = h[2];
+  out[2] = nq;
+  out[3] = h[3];
+  out[4] = ss;
+  out[5] = sl;
+  out[6] = h[0] + h[1];
+  out[7] = 0;
+  return 0;
+}
+}

@@ -1,0 +1,159 @@
+"""ctypes binding of include/mdg.h and the Python face of the engine.
+
+``Engine`` plays the role of the reference's ``SimLoop`` with its ``FProxy``
+chain (mdtk/SimLoop.hpp, mdtk/potentials/FProxy.hpp): it owns one system on
+the GPU, returns per-potential energies and Atom::grad from one force call
+(FProxy::operator(), FProxy.cxx:45-61) and advances the velocity-Verlet loop
+(SimLoop::executeMain, SimLoop.cxx:252-445).  Units are the reference's CGS.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmdg.so")
+
+POT_FBZL, POT_BRENNER, POT_REBO, POT_AIREBO, POT_ETORS = 1, 2, 4, 8, 16
+PREC_FP64, PREC_MIXED = 0, 1
+_NAMES = {"FBZL": POT_FBZL, "Brenner": POT_BRENNER, "REBO": POT_REBO, "AIREBO": POT_AIREBO, "ETors": POT_ETORS}
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [("pot_mask", ctypes.c_int), ("brenner_paramset", ctypes.c_int), ("etors_compat", ctypes.c_int),
+                ("precision", ctypes.c_int), ("fbzl_r0", ctypes.c_double), ("fbzl_r1", ctypes.c_double),
+                ("max_short_nb", ctypes.c_int), ("max_long_nb", ctypes.c_int)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads libmdg.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise EngineError(f"{path} is missing: build it with `python -m mdtk_amd.build`")
+    L = ctypes.CDLL(path)
+    vp, dp, ip = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int32)
+    L.mdg_create.argtypes = [ctypes.POINTER(_Config), ctypes.POINTER(vp)]
+    L.mdg_destroy.argtypes = [vp]
+    L.mdg_last_error.argtypes = [vp]
+    L.mdg_last_error.restype = ctypes.c_char_p
+    L.mdg_set_system.argtypes = [vp, ctypes.c_int, ip, dp, dp, dp, ctypes.POINTER(ctypes.c_uint8)]
+    L.mdg_num_terms.argtypes = [vp]
+    L.mdg_term_kind.argtypes = [vp, ctypes.c_int]
+    L.mdg_energy_grad.argtypes = [vp, dp, dp]
+    L.mdg_nl_export.argtypes = [vp, ctypes.c_int, ip, ip, ctypes.c_int64]
+    L.mdg_step_nve.argtypes = [vp, ctypes.c_int, ctypes.c_double, dp, dp, dp, dp]
+    L.mdg_get_state.argtypes = [vp, dp, dp, dp, ip]
+    L.mdg_set_stream.argtypes = [vp, vp]
+    L.mdg_profile.argtypes = [vp, ctypes.c_int]
+    L.mdg_kernel_time.argtypes = [vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int64)]
+    L.mdg_counters.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    _lib = L
+    return L
+
+
+def _p(a, t=ctypes.c_double):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(t))
+
+
+class Engine:
+    """One GPU context: a potential chain over one system.
+
+    potentials: names from {"FBZL", "Brenner", "REBO", "AIREBO", "ETors"}; the
+    chain order is the reference's (applications/common.h:46-68).
+    """
+
+    def __init__(self, potentials=("REBO", "AIREBO", "ETors"), precision: str = "fp64",
+                 etors_compat: bool = True, brenner_paramset: int = 0, fbzl_r=(5.0e-8, 5.5e-8)):
+        L = load_library()
+        mask = 0
+        for p in potentials:
+            mask |= _NAMES[p]
+        cfg = _Config(mask, int(brenner_paramset), int(bool(etors_compat)),
+                      PREC_FP64 if precision == "fp64" else PREC_MIXED, float(fbzl_r[0]), float(fbzl_r[1]), 0, 0)
+        h = ctypes.c_void_p()
+        if L.mdg_create(ctypes.byref(cfg), ctypes.byref(h)) != 0:
+            raise EngineError(L.mdg_last_error(None).decode())
+        self._h = h
+        self.n = 0
+        self.nterms = L.mdg_num_terms(h)
+        self.kinds = [L.mdg_term_kind(h, i) for i in range(self.nterms)]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.mdg_destroy(h)
+            self._h = None
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise EngineError(_lib.mdg_last_error(self._h).decode())
+
+    def set_system(self, elem, xyz_cm, vel_cm_s, pbc_cm, fixed=None):
+        e = np.ascontiguousarray(elem, np.int32)
+        x = np.ascontiguousarray(xyz_cm, np.float64).reshape(-1)
+        v = None if vel_cm_s is None else np.ascontiguousarray(vel_cm_s, np.float64).reshape(-1)
+        p = np.ascontiguousarray(pbc_cm, np.float64)
+        f = None if fixed is None else np.ascontiguousarray(fixed, np.uint8)
+        self._chk(_lib.mdg_set_system(self._h, int(e.shape[0]), _p(e, ctypes.c_int32), _p(x), _p(v), _p(p),
+                                      _p(f, ctypes.c_uint8)))
+        self.n = int(e.shape[0])
+        return self
+
+    @classmethod
+    def from_system(cls, sys, **kw):
+        eng = cls(**kw)
+        return eng.set_system(*sys.to_cgs())
+
+    def energy_grad(self):
+        e = np.zeros(self.nterms)
+        g = np.zeros((self.n, 3))
+        self._chk(_lib.mdg_energy_grad(self._h, _p(e), _p(g)))
+        return e, g
+
+    def neighbour_lists(self, term: int):
+        counts = np.zeros(self.n, np.int32)
+        cap = self.n * 1024
+        idx = np.zeros(cap, np.int32)
+        self._chk(_lib.mdg_nl_export(self._h, int(term), _p(counts, ctypes.c_int32), _p(idx, ctypes.c_int32), cap))
+        off = np.concatenate([[0], np.cumsum(counts)])
+        return [idx[off[i]:off[i + 1]].copy() for i in range(self.n)]
+
+    def run(self, nsteps: int, fixed_dt: float = 0.0):
+        ep, ek, dt, nf = (np.zeros(nsteps) for _ in range(4))
+        self._chk(_lib.mdg_step_nve(self._h, int(nsteps), float(fixed_dt), _p(ep), _p(ek), _p(dt), _p(nf)))
+        return dict(epot=ep, ekin=ek, dt=dt, netforce=nf)
+
+    def state(self):
+        x, v, a = (np.zeros((self.n, 3)) for _ in range(3))
+        c = np.zeros((self.n, 3), np.int32)
+        self._chk(_lib.mdg_get_state(self._h, _p(x), _p(v), _p(a), _p(c, ctypes.c_int32)))
+        return x, v, a, c
+
+    def set_stream(self, stream_handle: int):
+        self._chk(_lib.mdg_set_stream(self._h, ctypes.c_void_p(stream_handle)))
+
+    def profile(self, on: bool):
+        self._chk(_lib.mdg_profile(self._h, int(bool(on))))
+
+    def kernel_time(self, name: str):
+        ms = ctypes.c_double()
+        cnt = ctypes.c_int64()
+        self._chk(_lib.mdg_kernel_time(self._h, name.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
+        return ms.value, cnt.value
+
+    def counters(self):
+        out = (ctypes.c_int64 * 8)()
+        self._chk(_lib.mdg_counters(self._h, out))
+        keys = ["bonds", "lj_pairs", "bstar", "quads", "short_entries", "long_entries", "rebuilds", "_"]
+        return dict(zip(keys, list(out)))

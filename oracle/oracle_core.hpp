@@ -388,7 +388,7 @@ void nl_update(AtomsArray& atoms, std::vector<Potential*>& pots);
 struct SimLoop {
   AtomsArray atoms;
   Proxy fpot;
-  double dt = 5e-16, simTime = 0;
+  double dt = 1e-20, simTime = 0;  // SimLoop.cxx:50
   bool fixed_dt = false;
   double fixed_dt_value = 0;
   long iteration = 0;
