@@ -64,6 +64,20 @@ const char* mdg_last_error(const mdg_ctx* ctx);
 int mdg_set_system(mdg_ctx* ctx, int n, const int32_t* elem, const double* xyz, const double* vel,
                    const double pbc[3], const uint8_t* fixed);
 
+/* New positions for the host-driven loop, where the caller's SimLoop moves
+ * the atoms (SimLoop.cxx:323-332): xyz[n*3] (cm, wrapped into the box as
+ * Atom::applyPBC leaves them), pbc_count[n*3] (may be null: unchanged) and
+ * dr[n*3], each atom's displacement since the previous call as
+ * FGeneral::incDisplacement receives it (FGeneral.hpp:87-91).  dr feeds the
+ * skin test of the next force call (NeighbourList.cxx:108-133); dr == null
+ * forces a rebuild of every neighbour list. */
+int mdg_set_positions(mdg_ctx* ctx, const double* xyz, const int32_t* pbc_count, const double* dr);
+
+/* Per-atom neighbour counts over all chain members' lists, as
+ * FProxy::hasNeighbors tests them (FProxy.hpp:99-104): counts[n] > 0 iff
+ * the atom has a neighbour in some list. */
+int mdg_neighbor_counts(mdg_ctx* ctx, int32_t* counts);
+
 /* Number of chain members and the MDG_POT_* kind of member i. */
 int mdg_num_terms(const mdg_ctx* ctx);
 int mdg_term_kind(const mdg_ctx* ctx, int i);

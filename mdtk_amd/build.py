@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmdg.so")
-SOURCES = ["mdg_api.hip", "mdg_kernels.hip", "mdg_brenner.hip", "mdg_tables.cpp"]
+SOURCES = ["mdg_api.hip", "mdg_kernels.hip", "mdg_force.hip", "mdg_brenner.hip", "mdg_tables.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -40,6 +41,8 @@ struct mdg_ctx {
   Box box{};
   Atoms A{};
   Tables* dT = nullptr;
+  TablesT<float>* dTf = nullptr;  // mixed precision: Angstrom/eV FP32 table
+  bool mixed = false;
   hipStream_t own = nullptr, stream = nullptr;
   std::vector<void*> allocs;
 
@@ -70,8 +73,16 @@ struct mdg_ctx {
   std::map<std::string, KStat> kst;
   std::vector<hipEvent_t> event_pool;
 
+  int* stepidx = nullptr;
+  bool use_graphs = true;
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // one step, adaptive / fixed dt
+  std::vector<hipGraph_t> graphs;
+
   ~mdg_ctx() {
     if (stream) hipStreamSynchronize(stream);
+    for (auto& e : graph_exec)
+      if (e) hipGraphExecDestroy(e);
+    for (auto& g : graphs) hipGraphDestroy(g);
     for (auto& kv : kst)
       for (auto& pr : kv.second.pending) {
         hipEventDestroy(pr.first);
@@ -79,6 +90,7 @@ struct mdg_ctx {
       }
     for (void* p : allocs) hipFree(p);
     if (dT) hipFree(dT);
+    if (dTf) hipFree(dTf);
     if (own) hipStreamDestroy(own);
   }
 };
@@ -279,6 +291,51 @@ __global__ void k_zero_counts(long long* counters) {
 
 // One force call (FProxy::operator(), FProxy.cxx:45-61): lists, coordination,
 // then every chain member adds into grad and its energy slot.
+// The bond-order chain members, in the geometry type R of the context.
+template <class R>
+void force_terms(mdg_ctx* c, const TablesT<R>* T) {
+  int n = c->n;
+  Atoms& A = c->A;
+  if (c->have_short) {
+    launch(c, "k_coord", [&] { k_coord<R><<<blocks(n), 256, 0, c->stream>>>(A, T, c->box, c->derr); });
+    launch(c, "k_bond_enum", [&] {
+      k_bond_enum<<<blocks(n), 256, 0, c->stream>>>(A, c->bonds, c->nbonds, c->bond_cap, c->derr);
+    });
+  }
+  int bgrid = blocks(2LL * n, 256, 8192);
+  if (c->slot_rebo >= 0) {
+    launch(c, "k_rebo_bonds", [&] {
+      k_rebo_bonds<R><<<bgrid, 256, 0, c->stream>>>(A, T, c->box, c->bonds, c->nbonds, A.grad,
+                                                    c->eacc + c->slot_rebo, c->derr);
+    });
+  }
+  if (c->slot_bren >= 0) {
+    launch(c, "k_brenner_bonds", [&] {
+      k_brenner_bonds<<<bgrid, 256, 0, c->stream>>>(A, c->dT, c->box, c->bonds, c->nbonds, A.grad,
+                                                    c->eacc + c->slot_bren, c->derr);
+    });
+  }
+  if (c->slot_lj >= 0) {
+    launch(c, "k_lj_wave", [&] {
+      k_lj_wave<R><<<blocks(n, 4, 65536), 256, 0, c->stream>>>(A, T, c->box, A.grad, c->eacc + c->slot_lj,
+                                                                c->queue, c->qcount, c->qcap, c->counters + 2,
+                                                                c->derr);
+    });
+    launch(c, "k_bstar", [&] {
+      k_bstar<R><<<blocks(8LL * n, 256, 16384), 256, 0, c->stream>>>(A, T, c->box, c->queue, c->qcount, c->qcap,
+                                                                     A.grad, c->eacc + c->slot_lj, c->derr);
+    });
+  }
+  if (c->slot_tors >= 0) {
+    launch(c, "k_etors", [&] {
+      k_etors<R><<<bgrid, 256, 0, c->stream>>>(A, T, c->box, c->bonds, c->nbonds, c->cfg.etors_compat, A.grad,
+                                               c->eacc + c->slot_tors, c->counters + 3);
+    });
+  }
+}
+
+// One force call (FProxy::operator(), FProxy.cxx:45-61): lists, coordination,
+// then every chain member adds into grad and its energy slot.
 void force_call(mdg_ctx* c) {
   int n = c->n;
   Atoms& A = c->A;
@@ -289,42 +346,8 @@ void force_call(mdg_ctx* c) {
   k_zero_i<<<1, 64, 0, c->stream>>>(c->nbonds, 1);
   k_zero_i<<<1, 64, 0, c->stream>>>(c->qcount, 1);
   k_zero_counts<<<1, 64, 0, c->stream>>>(c->counters);
-  if (c->have_short) {
-    launch(c, "k_coord", [&] { k_coord<<<blocks(n), 256, 0, c->stream>>>(A, c->dT, c->box, c->derr); });
-    launch(c, "k_bond_enum", [&] {
-      k_bond_enum<<<blocks(n), 256, 0, c->stream>>>(A, c->bonds, c->nbonds, c->bond_cap, c->derr);
-    });
-  }
-  int bgrid = blocks(2LL * n, 256, 8192);
-  if (c->slot_rebo >= 0) {
-    launch(c, "k_rebo_bonds", [&] {
-      k_rebo_bonds<<<bgrid, 256, 0, c->stream>>>(A, c->dT, c->box, c->bonds, c->nbonds, A.grad,
-                                                 c->eacc + c->slot_rebo, c->derr);
-    });
-  }
-  if (c->slot_bren >= 0) {
-    launch(c, "k_brenner_bonds", [&] {
-      k_brenner_bonds<<<bgrid, 256, 0, c->stream>>>(A, c->dT, c->box, c->bonds, c->nbonds, A.grad,
-                                                    c->eacc + c->slot_bren, c->derr);
-    });
-  }
-  if (c->slot_lj >= 0) {
-    launch(c, "k_cij_build", [&] { k_cij_build<<<blocks(n), 256, 0, c->stream>>>(A, c->derr); });
-    launch(c, "k_lj_pairs", [&] {
-      k_lj_pairs<<<blocks(n), 256, 0, c->stream>>>(A, c->dT, c->box, A.grad, c->eacc + c->slot_lj, c->queue,
-                                                   c->qcount, c->qcap, c->counters + 2, c->derr);
-    });
-    launch(c, "k_bstar", [&] {
-      k_bstar<<<blocks(8LL * n, 256, 16384), 256, 0, c->stream>>>(A, c->dT, c->box, c->queue, c->qcount, c->qcap,
-                                                                  A.grad, c->eacc + c->slot_lj, c->derr);
-    });
-  }
-  if (c->slot_tors >= 0) {
-    launch(c, "k_etors", [&] {
-      k_etors<<<bgrid, 256, 0, c->stream>>>(A, c->dT, c->box, c->bonds, c->nbonds, c->cfg.etors_compat, A.grad,
-                                            c->eacc + c->slot_tors, c->counters + 3);
-    });
-  }
+  if (c->mixed) force_terms<float>(c, c->dTf);
+  else force_terms<double>(c, c->dT);
   if (c->slot_fbzl >= 0) {
     launch(c, "k_fbzl", [&] {
       k_fbzl<<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->cfg.fbzl_r0, c->cfg.fbzl_r1, A.grad,
@@ -358,8 +381,12 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
     g_create_err = "no potentials selected";
     return 1;
   }
-  if (cfg->precision != MDG_PREC_FP64) {
-    g_create_err = "this library build computes in FP64 only (precision must be MDG_PREC_FP64)";
+  if (cfg->precision != MDG_PREC_FP64 && cfg->precision != MDG_PREC_MIXED) {
+    g_create_err = "precision must be MDG_PREC_FP64 or MDG_PREC_MIXED";
+    return 1;
+  }
+  if (cfg->precision == MDG_PREC_MIXED && (m & MDG_POT_BRENNER)) {
+    g_create_err = "the mixed-precision form covers REBO/AIREBO/ETors/FBZL; run Brenner in FP64";
     return 1;
   }
   int ndev = 0;
@@ -369,6 +396,7 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
   }
   mdg_ctx* c = new mdg_ctx();
   c->cfg = *cfg;
+  c->use_graphs = std::getenv("MDG_NO_GRAPH") == nullptr;
   if (c->cfg.max_short_nb <= 0) c->cfg.max_short_nb = 64;
   if (c->cfg.max_long_nb <= 0) c->cfg.max_long_nb = 512;
   if (c->cfg.max_short_nb != 64 || c->cfg.max_long_nb != 512) {
@@ -405,6 +433,17 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
     g_create_err = "table upload failed";
     delete c;
     return 1;
+  }
+  c->mixed = cfg->precision == MDG_PREC_MIXED;
+  if (c->mixed) {
+    TablesT<float> hf;
+    convert_tables(h, hf);
+    if (hipMalloc(&c->dTf, sizeof(hf)) != hipSuccess ||
+        hipMemcpy(c->dTf, &hf, sizeof(hf), hipMemcpyHostToDevice) != hipSuccess) {
+      g_create_err = "table upload failed";
+      delete c;
+      return 1;
+    }
   }
   if (dalloc(c, c->flags, 2) || dalloc(c, c->part, 4 * 256) || dalloc(c, c->eacc, 8) || dalloc(c, c->red, 8) ||
       dalloc(c, c->dtp, 1) || dalloc(c, c->counters, 8) || dalloc(c, c->derr, 1) || dalloc(c, c->nbonds, 1) ||
@@ -482,7 +521,6 @@ int mdg_set_system(mdg_ctx* c, int n, const int32_t* elem, const double* xyz, co
   A.cap_s = c->cfg.max_short_nb;
   A.cap_l = c->cfg.max_long_nb;
   A.maxb = 16;
-  A.cap_c = 96;
   int r = 0;
   r |= dalloc(c, A.x, n) | dalloc(c, A.y, n) | dalloc(c, A.z, n);
   r |= dalloc(c, A.vx, n) | dalloc(c, A.vy, n) | dalloc(c, A.vz, n);
@@ -498,9 +536,10 @@ int mdg_set_system(mdg_ctx* c, int n, const int32_t* elem, const double* xyz, co
   r |= dalloc(c, A.br, (size_t)n * A.maxb) | dalloc(c, A.bf, (size_t)n * A.maxb) | dalloc(c, A.bdf, (size_t)n * A.maxb);
   r |= dalloc(c, A.bx, (size_t)n * A.maxb) | dalloc(c, A.by, (size_t)n * A.maxb) | dalloc(c, A.bz, (size_t)n * A.maxb);
   r |= dalloc(c, A.Nt, n) | dalloc(c, A.NH, n) | dalloc(c, A.NC, n);
-  if (c->slot_lj >= 0) {
-    r |= dalloc(c, A.c_cnt, n) | dalloc(c, A.c_tgt, (size_t)n * A.cap_c) | dalloc(c, A.c_k, (size_t)n * A.cap_c);
-    r |= dalloc(c, A.c_km, (size_t)n * A.cap_c) | dalloc(c, A.c_w, (size_t)n * A.cap_c);
+  if (c->mixed) {
+    size_t nb = (size_t)n * A.maxb;
+    r |= dalloc(c, A.brf, nb) | dalloc(c, A.bff, nb) | dalloc(c, A.bdff, nb);
+    r |= dalloc(c, A.bxf, nb) | dalloc(c, A.byf, nb) | dalloc(c, A.bzf, nb);
   }
   c->bond_cap = n * A.maxb / 2 + 64;
   A.bond_cap = c->bond_cap;
@@ -638,9 +677,10 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
   if (nsteps <= 0) return 0;
   Atoms& A = c->A;
   int n = c->n;
-  if (c->step_cap < nsteps) {
-    if (dalloc(c, c->stepout, (size_t)nsteps * 8)) return 1;
-    c->step_cap = nsteps;
+  const int kChunk = 4096;  // per-step outputs are staged in chunks of this many steps
+  if (c->step_cap < kChunk) {
+    if (dalloc(c, c->stepout, (size_t)kChunk * 8) || dalloc(c, c->stepidx, 1)) return 1;
+    c->step_cap = kChunk;
   }
   int fixed = fixed_dt > 0 ? 1 : 0;
   if (fixed) HIPCHK(c, hipMemcpyAsync(c->dtp, &fixed_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -652,26 +692,45 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
   double lim_s = 0.44 * c->rc_short, lim_l = 0.44 * c->rc_long;
   if (!c->have_short) lim_s = 1e300;
   if (!c->have_long) lim_l = 1e300;
-  for (int s = 0; s < nsteps; s++) {
+  // One iteration of the loop body; every size and flag it needs lives in
+  // device memory, so the same launches can be captured once and replayed.
+  auto body = [&]() {
     launch(c, "k_drift", [&] { k_drift<<<blocks(n), 256, 0, c->stream>>>(A, c->dtp); });
     launch(c, "k_disp_top2", [&] { k_disp_top2<<<256, 256, 0, c->stream>>>(n, A.disp_s, A.disp_l, c->part); });
     launch(c, "k_disp_decide",
-           [&] { k_disp_decide<<<1, 64, 0, c->stream>>>(256, c->part, lim_s, lim_l, c->flags, c->counters); });
+           [&] { k_disp_decide<<<1, 256, 0, c->stream>>>(256, c->part, lim_s, lim_l, c->flags, c->counters); });
     force_call(c);
     launch(c, "k_kick", [&] { k_kick<<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->dtp, c->red); });
-    k_step_end<<<1, 64, 0, c->stream>>>(nterms, c->eacc, c->red, c->dtp, fixed, c->stepout, s);
+    k_step_end<<<1, 64, 0, c->stream>>>(nterms, c->eacc, c->red, c->dtp, fixed, c->stepout, c->stepidx);
+  };
+  bool use_graph = !c->profiling && c->use_graphs;
+  if (use_graph && !c->graph_exec[fixed]) {
+    hipGraph_t g = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    body();
+    HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+    HIPCHK(c, hipGraphInstantiate(&c->graph_exec[fixed], g, nullptr, nullptr, 0));
+    c->graphs.push_back(g);
+  }
+  std::vector<double> o((size_t)kChunk * 8);
+  for (int s0 = 0; s0 < nsteps; s0 += kChunk) {
+    int m = std::min(kChunk, nsteps - s0);
+    k_zero_i<<<1, 64, 0, c->stream>>>(c->stepidx, 1);
+    for (int s = 0; s < m; s++) {
+      if (use_graph) HIPCHK(c, hipGraphLaunch(c->graph_exec[fixed], c->stream));
+      else body();
+    }
+    if (check_err(c)) return 1;
+    HIPCHK(c, hipMemcpy(o.data(), c->stepout, (size_t)m * 8 * sizeof(double), hipMemcpyDeviceToHost));
+    for (int s = 0; s < m; s++) {
+      if (epot) epot[s0 + s] = o[8 * s + 0];
+      if (ekin) ekin[s0 + s] = o[8 * s + 1];
+      if (dt_used) dt_used[s0 + s] = o[8 * s + 2];
+      if (netforce) netforce[s0 + s] = o[8 * s + 3];
+    }
   }
   c->iteration += nsteps;
-  if (check_err(c)) return 1;
-  std::vector<double> o((size_t)nsteps * 8);
-  HIPCHK(c, hipMemcpy(o.data(), c->stepout, o.size() * sizeof(double), hipMemcpyDeviceToHost));
-  for (int s = 0; s < nsteps; s++) {
-    if (epot) epot[s] = o[8 * s + 0];
-    if (ekin) ekin[s] = o[8 * s + 1];
-    if (dt_used) dt_used[s] = o[8 * s + 2];
-    if (netforce) netforce[s] = o[8 * s + 3];
-  }
-  if (!fixed && nsteps > 0) HIPCHK(c, hipMemcpy(&c->dt_host, c->dtp, sizeof(double), hipMemcpyDeviceToHost));
+  if (!fixed) HIPCHK(c, hipMemcpy(&c->dt_host, c->dtp, sizeof(double), hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -700,6 +759,67 @@ int mdg_get_state(mdg_ctx* c, double* xyz, double* vel, double* acc, int32_t* pb
       HIPCHK(c, hipMemcpy(q.data(), src[dd], n * 4, hipMemcpyDeviceToHost));
       for (int i = 0; i < n; i++) pbc_count[3 * i + dd] = q[i];
     }
+  }
+  return 0;
+}
+
+int mdg_set_positions(mdg_ctx* c, const double* xyz, const int32_t* pbc_count, const double* dr) {
+  int n = c->n;
+  if (n == 0) {
+    c->err = "no system";
+    return 1;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<double> a(n), b(n), d(n);
+  for (int i = 0; i < n; i++) {
+    a[i] = xyz[3 * i];
+    b[i] = xyz[3 * i + 1];
+    d[i] = xyz[3 * i + 2];
+  }
+  HIPCHK(c, hipMemcpy(c->A.x, a.data(), n * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->A.y, b.data(), n * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->A.z, d.data(), n * 8, hipMemcpyHostToDevice));
+  if (pbc_count) {
+    std::vector<int> q(n);
+    int* dst[3] = {c->A.pcx, c->A.pcy, c->A.pcz};
+    for (int dd = 0; dd < 3; dd++) {
+      for (int i = 0; i < n; i++) q[i] = pbc_count[3 * i + dd];
+      HIPCHK(c, hipMemcpy(dst[dd], q.data(), n * 4, hipMemcpyHostToDevice));
+    }
+  }
+  if (!dr) {
+    int fl[2] = {1, 1};
+    HIPCHK(c, hipMemcpy(c->flags, fl, sizeof(fl), hipMemcpyHostToDevice));
+    return 0;
+  }
+  // accumulate dr into both lists' displacements, then the skin test
+  std::vector<double> s((size_t)3 * n), l((size_t)3 * n);
+  HIPCHK(c, hipMemcpy(s.data(), c->A.disp_s, 3 * (size_t)n * 8, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(l.data(), c->A.disp_l, 3 * (size_t)n * 8, hipMemcpyDeviceToHost));
+  for (size_t q = 0; q < (size_t)3 * n; q++) {
+    s[q] += dr[q];
+    l[q] += dr[q];
+  }
+  HIPCHK(c, hipMemcpy(c->A.disp_s, s.data(), 3 * (size_t)n * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->A.disp_l, l.data(), 3 * (size_t)n * 8, hipMemcpyHostToDevice));
+  double lim_s = c->have_short ? 0.44 * c->rc_short : 1e300;
+  double lim_l = c->have_long ? 0.44 * c->rc_long : 1e300;
+  k_disp_top2<<<256, 256, 0, c->stream>>>(n, c->A.disp_s, c->A.disp_l, c->part);
+  k_disp_decide<<<1, 256, 0, c->stream>>>(256, c->part, lim_s, lim_l, c->flags, c->counters);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mdg_neighbor_counts(mdg_ctx* c, int32_t* counts) {
+  int n = c->n;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<int> s(n, 0), l(n, 0), ty(n);
+  if (c->have_short) HIPCHK(c, hipMemcpy(s.data(), c->A.nls_cnt, n * 4, hipMemcpyDeviceToHost));
+  if (c->have_long) HIPCHK(c, hipMemcpy(l.data(), c->A.nll_cnt, n * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(ty.data(), c->A.type, n * 4, hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; i++) {
+    bool long_handled = (c->slot_fbzl >= 0) || (c->slot_lj >= 0 && ty[i] <= T_H);
+    counts[i] = s[i] + (long_handled ? l[i] : 0);
   }
   return 0;
 }

@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(256) k_brenner_bonds(Atoms A, const Tables* __
         }
         double fi = Ni, fj = Nj;
         double out[4];
-        tricubic<6>(T->BF, fi, fj, Nconj, 4, 5, out);
+        tricubic<double, 6>(T->BF, fi, fj, Nconj, 4, 5, out);
         for (int q = 0; q < 4; q++) FN[q] = out[q];
       }
       double bav = (Bv[0] + Bv[1]) / 2.0 + (FN[0] / 2.0);

@@ -1,0 +1,987 @@
+// Force kernels of the REBO / AIREBO-LJ / torsion chain, templated on the
+// geometry type R (double: the FP64 drop-in, reference operation order, CGS;
+// float: the mixed-precision form, Angstrom/eV geometry with FP64 sums).
+//
+// Force scheme: the reference evaluates every energy sub-term once for its
+// value and again for each derivative seed (REBO.cxx:200-814).  Here every
+// bond (and every b* pair of the LJ term) runs one forward pass that keeps the
+// scalar intermediates, then one adjoint pass seeded with dE/db that scatters
+// V*dterm/dx into the gradient with FP64 atomics.
+#include "mdg_kernels.hpp"
+
+namespace mdg {
+
+namespace {
+__device__ __forceinline__ void raise(unsigned* err, unsigned bit) { atomicOr(err, bit); }
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block sum of one double per thread, one atomic per block into *acc.
+__device__ void block_accumulate(double v, double* acc) {
+  __shared__ double red[16];
+  v = wave_sum(v);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    int nw = (blockDim.x + 63) >> 6;
+    for (int q = 0; q < nw; q++) s += red[q];
+    if (s != 0.0) unsafeAtomicAdd(acc, s);
+  }
+  __syncthreads();
+}
+
+// Energy of a kernel unit expressed in erg.
+template <class R>
+__device__ __forceinline__ double to_erg(double e, double eunit) {
+  if constexpr (std::is_same<R, double>::value) {
+    return e;
+  } else {
+    return e * eunit;
+  }
+}
+
+template <class R>
+struct BondRef {
+  int k;
+  R r, f, df;
+  V3<R> rv;  // x_a - x_k (minimum image)
+};
+
+template <class R>
+struct BondArr {
+  R *r, *f, *df, *x, *y, *z;
+};
+
+template <class R>
+__device__ __forceinline__ BondArr<R> barr(const Atoms& A) {
+  if constexpr (std::is_same<R, double>::value) {
+    return BondArr<R>{A.br, A.bf, A.bdf, A.bx, A.by, A.bz};
+  } else {
+    return BondArr<R>{A.brf, A.bff, A.bdff, A.bxf, A.byf, A.bzf};
+  }
+}
+
+template <class R>
+__device__ __forceinline__ BondRef<R> bond_at(const Atoms& A, int a, int s) {
+  BondArr<R> B = barr<R>(A);
+  int o = a * A.maxb + s;
+  BondRef<R> b;
+  b.k = A.bj[o];
+  b.r = B.r[o];
+  b.f = B.f[o];
+  b.df = B.df[o];
+  b.rv = v3<R>(B.x[o], B.y[o], B.z[o]);
+  return b;
+}
+
+// Coordination of a excluding b (REBO::Nt/NH/NC, REBO.cxx:423-493): the
+// cached FP64 sum minus f_ab when r_ab < R2(a,b), clamped at 0.
+template <class R>
+__device__ __forceinline__ double coord_excl(double Nsum, bool subtract, R r, R R2, R fab, unsigned* err) {
+  double N = Nsum;
+  if (subtract && r < R2) N -= double(fab);
+  if (!(N > -1e-15)) raise(err, ERR_NEG_COORD);
+  if (N < 0) N = 0;
+  return N;
+}
+}  // namespace
+
+// ----------------------------------------------------- coordination pass
+// REBO::countNeighbours (REBO.cxx:328-393) plus the per-atom bond list: all
+// short-list neighbours with r <= R2 in list order, with r, f, df and the
+// minimum-image vector.
+template <class R>
+__global__ void k_coord(Atoms A, const TablesT<R>* __restrict__ T, Box bx, unsigned* err) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n) return;
+  BondArr<R> B = barr<R>(A);
+  int ti = A.type[i];
+  double Nt = 0, NH = 0, NC = 0;
+  int nb = 0;
+  if (ti <= T_H) {
+    int cnt = A.nls_cnt[i];
+    const int* L = A.nls + (size_t)i * A.cap_s;
+    for (int q = 0; q < cnt; q++) {
+      int k = L[q];
+      int tk = A.type[k];
+      V3<R> rv = to_local<R>(depos(A.x, A.y, A.z, i, k, bx));
+      R r = sqrt(rv.x * rv.x + rv.y * rv.y + rv.z * rv.z);
+      R R2 = T->rebo.R[ti][tk][1];
+      if (r > R2) continue;
+      R R1 = T->rebo.R[ti][tk][0];
+      R f, df;
+      cutoff<R>(r, R1, R2, f, df);
+      Nt += f;
+      if (tk == T_H) NH += f;
+      if (tk == T_C) NC += f;
+      if (nb < A.maxb) {
+        int o = i * A.maxb + nb;
+        A.bj[o] = k;
+        B.r[o] = r;
+        B.f[o] = f;
+        B.df[o] = df;
+        B.x[o] = rv.x;
+        B.y[o] = rv.y;
+        B.z[o] = rv.z;
+      } else {
+        raise(err, ERR_BOND_OVERFLOW);
+      }
+      nb++;
+    }
+  }
+  A.Nt[i] = Nt;
+  A.NH[i] = NH;
+  A.NC[i] = NC;
+  A.bcnt[i] = nb < A.maxb ? nb : A.maxb;
+}
+
+// ------------------------------------------------------- bond order b_ij
+// The pair (i,j) whose bond order is needed: a REBO bond, or an LJ pair for
+// b* (then r_eff = R0 inside ExpTerm only, AIREBO_LJ.cxx:191-202).
+template <class R>
+struct BPair {
+  int i, j, ti, tj;
+  V3<R> rv;  // x_i - x_j
+  R r;
+  R f;       // REBO cutoff value of (i,j) (0 for b* pairs beyond R2)
+  R reff;
+};
+
+struct BOState {
+  double D[2];        // D_ij, D_ji
+  double N[2];        // Nt_ij, Nt_ji
+  double NHc[2], NCc[2];
+  double P[2][3];     // P and dP/dNH, dP/dNC per side
+  double sum[2];      // Nconj partial sums (i side, j side)
+  double Nconj;
+  double rc[4];       // pi_rc and d/dNi, d/dNj, d/dNconj in (i,j) orientation
+  double T[4];        // T_ij and derivatives
+  double dh;          // pi_dh angular sum
+  double b;
+};
+
+// Centre a, partner b of side s (0: centre i; 1: centre j).
+template <class R>
+struct Side {
+  int a, b, ta, tb;
+  V3<R> rv;  // x_a - x_b
+};
+template <class R>
+__device__ __forceinline__ Side<R> side_of(const BPair<R>& p, int s) {
+  Side<R> q;
+  if (s == 0) {
+    q.a = p.i; q.b = p.j; q.ta = p.ti; q.tb = p.tj; q.rv = p.rv;
+  } else {
+    q.a = p.j; q.b = p.i; q.ta = p.tj; q.tb = p.ti; q.rv = -p.rv;
+  }
+  return q;
+}
+
+// G(cos theta_bak) with its partial derivatives (REBO.cxx:277-326).
+template <class R>
+struct GVal {
+  R G, dGdc, dGdN;
+  bool has_dc, has_dN;
+};
+template <class R>
+__device__ __forceinline__ GVal<R> gfun(const ReboParamsT<R>& P, int ta, R c, R N) {
+  GVal<R> g;
+  if (ta == T_C) {
+    R Sp = sprime<R>(N, P.Ntot[0], P.Ntot[1]);
+    R g1, g1d, g2, g2d;
+    gspline<R>(P.gC1, c, g1, g1d);
+    gspline<R>(P.gC2, c, g2, g2d);
+    g.G = g2 + Sp * (g1 - g2);
+    g.has_dc = (g2d != R(0) || g1d != R(0));
+    g.dGdc = g2d + g1d * Sp + (-g2d * Sp);
+    R dSp = dsprime<R>(N, P.Ntot[0], P.Ntot[1]);
+    g.has_dN = ((g2 != R(0) || g1 != R(0)) && dSp != R(0));
+    g.dGdN = g1 * dSp + (-g2 * dSp);
+  } else {
+    R v, d;
+    gspline<R>(P.gH, c, v, d);
+    g.G = v;
+    g.has_dc = (d != R(0));
+    g.dGdc = d;
+    g.has_dN = false;
+    g.dGdN = R(0);
+  }
+  return g;
+}
+
+template <class R>
+__device__ __forceinline__ R exp_term(const ReboParamsT<R>& P, const Side<R>& q, int tk, R r_ak, R reff_ab) {
+  if (q.ta != T_H) return R(1);
+  return exp(P.expscale * ((P.rho[q.ta][tk] - r_ak) - (P.rho[q.ta][q.tb] - reff_ab)));
+}
+
+template <class R>
+__device__ __forceinline__ R cos_theta(V3<R> a, V3<R> b, R ra, R rb, unsigned* err, R& raw) {
+  R e1 = dot(a, b);
+  R e2 = ra * rb;
+  R c = e1 / e2;
+  raw = c;
+  if (!(c >= R(-1.0 - 1e-7) && c <= R(1.0 + 1e-7))) raise(err, ERR_COS_RANGE);
+  if (c < R(-1)) c = R(-1);
+  if (c > R(1)) c = R(1);
+  return c;
+}
+
+// Forward pass of b_ij = (p_ij + p_ji)/2 + pi_rc + pi_dh (REBO::Baver,
+// REBO.cxx:200-211, with D :235-262, p_sigma_pi :264-275, Nconj :540-624,
+// P :626-657, pi_rc :659-714, pi_dh :716-814).
+template <class R>
+__device__ void bo_forward(const Atoms& A, const TablesT<R>* __restrict__ T, const BPair<R>& p, BOState& st,
+                           unsigned* err) {
+  const ReboParamsT<R>& P = T->rebo;
+  R R2ij = P.R[p.ti][p.tj][1];
+  st.N[0] = coord_excl<R>(A.Nt[p.i], true, p.r, R2ij, p.f, err);
+  st.N[1] = coord_excl<R>(A.Nt[p.j], true, p.r, R2ij, p.f, err);
+  st.NHc[0] = coord_excl<R>(A.NH[p.i], p.tj == T_H, p.r, R2ij, p.f, err);
+  st.NCc[0] = coord_excl<R>(A.NC[p.i], p.tj == T_C, p.r, R2ij, p.f, err);
+  st.NHc[1] = coord_excl<R>(A.NH[p.j], p.ti == T_H, p.r, R2ij, p.f, err);
+  st.NCc[1] = coord_excl<R>(A.NC[p.j], p.ti == T_C, p.r, R2ij, p.f, err);
+  for (int s = 0; s < 2; s++) {
+    Side<R> q = side_of(p, s);
+    double D = 1.0;
+    int nb = A.bcnt[q.a];
+    for (int t = 0; t < nb; t++) {
+      BondRef<R> k = bond_at<R>(A, q.a, t);
+      if (k.k == q.b) continue;
+      int tk = A.type[k.k];
+      R E = exp_term<R>(P, q, tk, k.r, p.reff);
+      if (E != R(0)) {
+        R raw;
+        R c = cos_theta<R>(q.rv, k.rv, p.r, k.r, err, raw);
+        GVal<R> g = gfun<R>(P, q.ta, c, R(st.N[s]));
+        D += double(g.G * k.f * E);
+      }
+    }
+    st.P[s][0] = st.P[s][1] = st.P[s][2] = 0.0;
+    if (q.ta == T_C && q.tb <= T_H) {
+      R o[3];
+      bicubic<R>(T->P[q.tb == T_C ? 0 : 1], R(st.NHc[s]), R(st.NCc[s]), o);
+      st.P[s][0] = o[0];
+      st.P[s][1] = o[1];
+      st.P[s][2] = o[2];
+    }
+    D += st.P[s][0];
+    st.D[s] = D;
+  }
+  double pij = pow(st.D[0], -0.5);
+  double pji = pow(st.D[1], -0.5);
+  // Nconj partial sums: carbon neighbours k of the centre, f_ka * S'(N_ka)
+  for (int s = 0; s < 2; s++) {
+    Side<R> q = side_of(p, s);
+    double sum = 0.0;
+    int nb = A.bcnt[q.a];
+    for (int t = 0; t < nb; t++) {
+      BondRef<R> k = bond_at<R>(A, q.a, t);
+      if (k.k == q.b) continue;
+      int tk = A.type[k.k];
+      if (tk != T_C) continue;
+      double Nk = coord_excl<R>(A.Nt[k.k], true, k.r, P.R[tk][q.ta][1], k.f, err);
+      sum += double(k.f * sprime<R>(R(Nk), P.Ntot[0], P.Ntot[1]));
+    }
+    st.sum[s] = sum;
+  }
+  st.Nconj = 1.0 + st.sum[0] * st.sum[0] + st.sum[1] * st.sum[1];
+  // pi_rc by pair type; the H-C case evaluates F_CH(N_j, N_i, Nconj_ji)
+  for (int q = 0; q < 4; q++) st.rc[q] = 0.0;
+  R o[4];
+  if (p.ti == T_C && p.tj == T_C) {
+    tricubic<R, 16>(T->T3[0], R(st.N[0]), R(st.N[1]), R(st.Nconj), 15, 15, o);
+    for (int q = 0; q < 4; q++) st.rc[q] = o[q];
+  } else if (p.ti == T_C && p.tj == T_H) {
+    tricubic<R, 16>(T->T3[1], R(st.N[0]), R(st.N[1]), R(st.Nconj), 15, 15, o);
+    for (int q = 0; q < 4; q++) st.rc[q] = o[q];
+  } else if (p.ti == T_H && p.tj == T_C) {
+    double nconj_ji = 1.0 + st.sum[1] * st.sum[1] + st.sum[0] * st.sum[0];
+    tricubic<R, 16>(T->T3[1], R(st.N[1]), R(st.N[0]), R(nconj_ji), 15, 15, o);
+    st.rc[0] = o[0];
+    st.rc[1] = o[2];  // d/dN_i
+    st.rc[2] = o[1];  // d/dN_j
+    st.rc[3] = o[3];
+  } else if (p.ti == T_H && p.tj == T_H) {
+    tricubic<R, 16>(T->T3[2], R(st.N[0]), R(st.N[1]), R(st.Nconj), 15, 15, o);
+    for (int q = 0; q < 4; q++) st.rc[q] = o[q];
+  }
+  // pi_dh: C-C only, skipped when T_ij == 0 (REBO.cxx:802-814)
+  st.T[0] = st.T[1] = st.T[2] = st.T[3] = 0.0;
+  st.dh = 0.0;
+  double pidh = 0.0;
+  if (p.ti == T_C && p.tj == T_C) {
+    tricubic<R, 16>(T->T3[3], R(st.N[0]), R(st.N[1]), R(st.Nconj), 15, 15, o);
+    for (int q = 0; q < 4; q++) st.T[q] = o[q];
+    if (st.T[0] != 0.0) {
+      double s = 0.0;
+      int nbi = A.bcnt[p.i], nbj = A.bcnt[p.j];
+      for (int a = 0; a < nbi; a++) {
+        BondRef<R> k = bond_at<R>(A, p.i, a);
+        if (k.k == p.j) continue;
+        for (int b = 0; b < nbj; b++) {
+          BondRef<R> l = bond_at<R>(A, p.j, b);
+          if (l.k == p.i) continue;
+          if (k.k == l.k) continue;
+          R sin_ik = sqrt(norm2(cross(p.rv, k.rv))) / (p.r * k.r);
+          if (fabs(sin_ik) < R(0.1)) continue;
+          int tk = A.type[k.k], tl = A.type[l.k];
+          R fik = sprime<R>(k.r, P.Rp[tk][p.ti][0], P.Rp[tk][p.ti][1]);
+          R sin_jl = sqrt(norm2(cross(p.rv, l.rv))) / (p.r * l.r);
+          if (fabs(sin_jl) < R(0.1)) continue;
+          R fjl = sprime<R>(l.r, P.Rp[tl][p.tj][0], P.Rp[tl][p.tj][1]);
+          V3<R> Pp = cross(-p.rv, k.rv), Q = cross(p.rv, l.rv);
+          R e2 = sqrt(norm2(Pp)) * sqrt(norm2(Q));
+          R cd = (e2 == R(0)) ? R(0) : dot(Pp, Q) / e2;
+          if (cd < R(-1)) cd = R(-1);
+          if (cd > R(1)) cd = R(1);
+          R c = -cd;
+          s += double((R(1) - c * c) * fik * fjl);
+        }
+      }
+      st.dh = s;
+      pidh = st.T[0] * s;
+    }
+  }
+  st.b = (pij + pji) / 2.0 + st.rc[0] + pidh;
+}
+
+// Scatter c * d f_ak / dx for the pair (a,k) along u_ak = rv/r (a +=, k -=).
+template <class R>
+__device__ __forceinline__ void pair_push(double* g, int k, V3<R> rv, R r, R c, V3<R>& ga, double gs) {
+  V3<R> v = rv * (c / r);
+  ga = ga + v;
+  scatter3<R>(g, k, -v, gs);
+}
+
+// Adjoint pass: adds W * db_ij/dx to grad (W in the kernel's energy unit).
+template <class R>
+__device__ void bo_backward(const Atoms& A, const TablesT<R>* __restrict__ T, const BPair<R>& p,
+                            const BOState& st, double W, double* __restrict__ grad, double gs, unsigned* err) {
+  const ReboParamsT<R>& P = T->rebo;
+  V3<R> gi = v3<R>(0, 0, 0), gj = v3<R>(0, 0, 0);
+  double dN[2] = {0, 0}, dNH[2] = {0, 0}, dNC[2] = {0, 0}, dNconj = 0.0;
+  // sigma-pi terms
+  for (int s = 0; s < 2; s++) {
+    Side<R> q = side_of(p, s);
+    V3<R>& ga = (s == 0) ? gi : gj;
+    V3<R>& gb = (s == 0) ? gj : gi;
+    double seedD = (-0.5 * pow(st.D[s], -0.5 - 1.0)) * (W / 2.0);
+    R seedR = R(seedD);
+    int nb = A.bcnt[q.a];
+    for (int t = 0; t < nb; t++) {
+      BondRef<R> k = bond_at<R>(A, q.a, t);
+      if (k.k == q.b) continue;
+      int tk = A.type[k.k];
+      R E = exp_term<R>(P, q, tk, k.r, p.reff);
+      if (E == R(0)) continue;
+      R raw;
+      R c = cos_theta<R>(q.rv, k.rv, p.r, k.r, err, raw);
+      GVal<R> g = gfun<R>(P, q.ta, c, R(st.N[s]));
+      R V1 = k.f * E * seedR;
+      V3<R> gk = v3<R>(0, 0, 0);
+      if (g.has_dc) {
+        R cc = g.dGdc * V1;
+        R e2 = p.r * k.r;
+        V3<R> da = k.rv / e2 - q.rv * (raw / (p.r * p.r));
+        V3<R> db = q.rv / e2 - k.rv * (raw / (k.r * k.r));
+        ga = ga + (da + db) * cc;
+        gb = gb - da * cc;
+        gk = gk - db * cc;
+      }
+      if (g.has_dN) dN[s] += double(g.dGdN * V1);
+      if (k.df != R(0)) {
+        V3<R> v = k.rv * (k.df * (g.G * E * seedR) / k.r);
+        ga = ga + v;
+        gk = gk - v;
+      }
+      if (q.ta == T_H) {
+        R dt = E * P.expscale;
+        R V3s = g.G * k.f * seedR;
+        // r_ab (or the b* pair's r_alter, whose derivative is the unit vector)
+        V3<R> u = q.rv * (dt * V3s / p.r);
+        ga = ga + u;
+        gb = gb - u;
+        V3<R> v = k.rv * (-dt * V3s / k.r);
+        ga = ga + v;
+        gk = gk - v;
+      }
+      scatter3<R>(grad, k.k, gk, gs);
+    }
+    dNH[s] += st.P[s][1] * seedD;
+    dNC[s] += st.P[s][2] * seedD;
+  }
+  // pi_rc
+  dN[0] += st.rc[1] * W;
+  dN[1] += st.rc[2] * W;
+  dNconj += st.rc[3] * W;
+  // pi_dh
+  if (st.T[0] != 0.0) {
+    double Wt = st.dh * W;
+    if (st.dh != 0.0) {
+      dN[0] += st.T[1] * Wt;
+      dN[1] += st.T[2] * Wt;
+      dNconj += st.T[3] * Wt;
+    }
+    R Ws = R(st.T[0] * W);
+    int nbi = A.bcnt[p.i], nbj = A.bcnt[p.j];
+    for (int a = 0; a < nbi; a++) {
+      BondRef<R> k = bond_at<R>(A, p.i, a);
+      if (k.k == p.j) continue;
+      for (int b = 0; b < nbj; b++) {
+        BondRef<R> l = bond_at<R>(A, p.j, b);
+        if (l.k == p.i) continue;
+        if (k.k == l.k) continue;
+        R sin_ik = sqrt(norm2(cross(p.rv, k.rv))) / (p.r * k.r);
+        if (fabs(sin_ik) < R(0.1)) continue;
+        int tk = A.type[k.k], tl = A.type[l.k];
+        R R1k = P.Rp[tk][p.ti][0], R2k = P.Rp[tk][p.ti][1];
+        R fik = sprime<R>(k.r, R1k, R2k);
+        R sin_jl = sqrt(norm2(cross(p.rv, l.rv))) / (p.r * l.r);
+        if (fabs(sin_jl) < R(0.1)) continue;
+        R R1l = P.Rp[tl][p.tj][0], R2l = P.Rp[tl][p.tj][1];
+        R fjl = sprime<R>(l.r, R1l, R2l);
+        V3<R> a_ = p.rv, b_ = k.rv, c_ = l.rv;
+        V3<R> Pp = cross(-a_, b_), Q = cross(a_, c_);
+        R p2 = norm2(Pp), q2 = norm2(Q);
+        R e2 = sqrt(p2) * sqrt(q2);
+        R cs = (e2 == R(0)) ? R(0) : dot(Pp, Q) / e2;
+        R cd = cs;
+        if (cd < R(-1)) cd = R(-1);
+        if (cd > R(1)) cd = R(1);
+        R c = -cd;
+        V3<R> gk = v3<R>(0, 0, 0), gl = v3<R>(0, 0, 0);
+        R V = R(2) * c * fik * fjl * Ws;
+        if (V != R(0) && e2 != R(0)) {
+          V3<R> gP = Q / e2 - Pp * (cs / p2);
+          V3<R> gQ = Pp / e2 - Q * (cs / q2);
+          V3<R> da = cross(gP, b_) + cross(c_, gQ);
+          V3<R> db = cross(a_, gP);
+          V3<R> dc = cross(gQ, a_);
+          gi = gi + (da + db) * V;
+          gj = gj + (dc - da) * V;
+          gk = gk - db * V;
+          gl = gl - dc * V;
+        }
+        R Vk = (R(1) - c * c) * fjl * Ws;
+        if (Vk != R(0) && k.r >= R1k && k.r <= R2k) {
+          V3<R> v = k.rv * (dsprime<R>(k.r, R1k, R2k) * Vk / k.r);
+          gi = gi + v;
+          gk = gk - v;
+        }
+        R Vl = (R(1) - c * c) * fik * Ws;
+        if (Vl != R(0) && l.r >= R1l && l.r <= R2l) {
+          V3<R> v = l.rv * (dsprime<R>(l.r, R1l, R2l) * Vl / l.r);
+          gj = gj + v;
+          gl = gl - v;
+        }
+        scatter3<R>(grad, k.k, gk, gs);
+        scatter3<R>(grad, l.k, gl, gs);
+      }
+    }
+  }
+  // coordination derivatives (dfThem, REBO.cxx:408-421) and Nconj
+  for (int s = 0; s < 2; s++) {
+    Side<R> q = side_of(p, s);
+    V3<R>& ga = (s == 0) ? gi : gj;
+    double sconj = (st.sum[s] != 0.0) ? 2.0 * st.sum[s] * dNconj : 0.0;
+    int nb = A.bcnt[q.a];
+    for (int t = 0; t < nb; t++) {
+      BondRef<R> k = bond_at<R>(A, q.a, t);
+      if (k.k == q.b) continue;
+      int tk = A.type[k.k];
+      double c = dN[s];
+      if (tk == T_H) c += dNH[s];
+      if (tk == T_C) c += dNC[s];
+      double coefk = 0.0;
+      if (tk == T_C && sconj != 0.0) {
+        double Nk = coord_excl<R>(A.Nt[k.k], true, k.r, P.R[tk][q.ta][1], k.f, err);
+        double Sp = sprime<R>(R(Nk), P.Ntot[0], P.Ntot[1]);
+        c += Sp * sconj;
+        coefk = double(k.f) * double(dsprime<R>(R(Nk), P.Ntot[0], P.Ntot[1])) * sconj;
+      }
+      if (c != 0.0 && k.df != R(0)) pair_push<R>(grad, k.k, k.rv, k.r, R(double(k.df) * c), ga, gs);
+      if (coefk != 0.0) {
+        // d N_ka: neighbours m of k other than a
+        V3<R> gk = v3<R>(0, 0, 0);
+        int nbk = A.bcnt[k.k];
+        for (int u = 0; u < nbk; u++) {
+          BondRef<R> m = bond_at<R>(A, k.k, u);
+          if (m.k == q.a || m.df == R(0)) continue;
+          V3<R> v = m.rv * R(double(m.df) * coefk / double(m.r));
+          gk = gk + v;
+          scatter3<R>(grad, m.k, -v, gs);
+        }
+        scatter3<R>(grad, k.k, gk, gs);
+      }
+    }
+  }
+  scatter3<R>(grad, p.i, gi, gs);
+  scatter3<R>(grad, p.j, gj, gs);
+}
+
+// ------------------------------------------------------------ REBO bonds
+// REBO::operator() (REBO.cxx:32-65): E += VR + b_ij * VA for every bond.
+template <class R>
+__global__ void __launch_bounds__(256) k_rebo_bonds(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
+                                                   const int2* __restrict__ bonds, const int* __restrict__ nbonds,
+                                                   double* __restrict__ grad, double* __restrict__ eacc,
+                                                   unsigned* err) {
+  double e = 0.0;
+  const double gs = T->eunit / T->lunit;
+  int nbt = min(*nbonds, A.bond_cap);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nbt; t += gridDim.x * blockDim.x) {
+    const ReboParamsT<R>& P = T->rebo;
+    int2 bd = bonds[t];
+    BPair<R> p;
+    p.i = bd.x;
+    p.j = bd.y;
+    p.ti = A.type[p.i];
+    p.tj = A.type[p.j];
+    p.rv = to_local<R>(depos(A.x, A.y, A.z, p.i, p.j, bx));
+    p.r = sqrt(norm2(p.rv));
+    R f, df;
+    cutoff<R>(p.r, P.R[p.ti][p.tj][0], P.R[p.ti][p.tj][1], f, df);
+    p.f = f;
+    p.reff = p.r;
+    int a = p.ti, b = p.tj;
+    R r = p.r;
+    // VR (REBO.cxx:132-160) and VA (:162-198)
+    if (f != R(0)) {
+      R cr = R(0);
+      R t1 = P.A[a][b] * exp(-P.alpha[a][b] * r);
+      R val = (R(1) + P.Q[a][b] / r) * t1;
+      R der = -t1 * (P.Q[a][b] + P.alpha[a][b] * r * r + P.alpha[a][b] * r * P.Q[a][b]) / (r * r);
+      R VR = f * val;
+      cr += der * f + val * df;
+      R u1 = P.B[0][a][b] * exp(-P.beta[0][a][b] * r);
+      R u2 = P.B[1][a][b] * exp(-P.beta[1][a][b] * r);
+      R u3 = P.B[2][a][b] * exp(-P.beta[2][a][b] * r);
+      R vsum = u1 + u2 + u3;
+      R VA = -f * vsum;
+      double eb = double(VR);
+      if (VA != R(0)) {
+        BOState st;
+        bo_forward<R>(A, T, p, st, err);
+        double bb = st.b;
+        bo_backward<R>(A, T, p, st, double(VA), grad, gs, err);
+        R vder = (-P.beta[0][a][b]) * u1 + (-P.beta[1][a][b]) * u2 + (-P.beta[2][a][b]) * u3;
+        R bR = R(bb);
+        cr += (-vder * f * bR) + (-vsum * bR) * df;
+        eb += bb * double(VA);
+      }
+      e += to_erg<R>(eb, T->eunit);
+      V3<R> v = p.rv * (cr / r);
+      scatter3<R>(grad, p.i, v, gs);
+      scatter3<R>(grad, p.j, -v, gs);
+    }
+  }
+  block_accumulate(e, eacc);
+}
+
+// ------------------------------------------------------------ C_ij paths
+// C_ij = 1 - w_ij, w_ij the best product of REBO cutoff values over paths of
+// at most three bonds from i to j (AIREBO::fill_Cij / Cij, AIREBO_LJ.cxx:
+// 289-408).  The reference keeps, per target, the first path met in its loop
+// order (i->k, then k->m, then m->q) unless a later one has a larger weight,
+// or an equal weight and fewer bonds (:311,331,352).  That is: the winner is
+// the path of largest weight, then fewest bonds, then earliest in loop order.
+// Each path is described by (bonds, k, m) with the nodes i,[k],[m],j.
+struct CPath {
+  int hops, k, m;
+};
+
+// Gradient of C_ij = 1 - prod f over the path's bonds, seeded with V.
+template <class R>
+__device__ void cij_backward(const Atoms& A, const TablesT<R>* __restrict__ T, Box bx, int i, int j, CPath p,
+                             R V, double* __restrict__ grad, double gs) {
+  int nodes[4];
+  int nn = 0;
+  nodes[nn++] = i;
+  if (p.hops >= 2) nodes[nn++] = p.k;
+  if (p.hops >= 3) nodes[nn++] = p.m;
+  nodes[nn++] = j;
+  R f[3], df[3], r[3];
+  V3<R> rv[3];
+  for (int q = 0; q < p.hops; q++) {
+    int a = nodes[q], b = nodes[q + 1];
+    rv[q] = to_local<R>(depos(A.x, A.y, A.z, a, b, bx));
+    r[q] = sqrt(norm2(rv[q]));
+    int ta = A.type[a], tb = A.type[b];
+    cutoff<R>(r[q], T->rebo.R[ta][tb][0], T->rebo.R[ta][tb][1], f[q], df[q]);
+  }
+  for (int q = 0; q < p.hops; q++) {
+    if (df[q] == R(0)) continue;
+    R mm = R(1);
+    for (int u = 0; u < p.hops; u++)
+      if (u != q) mm *= f[u];
+    R c = df[q] * (-mm * V);
+    V3<R> v = rv[q] * (c / r[q]);
+    scatter3<R>(grad, nodes[q], v, gs);
+    scatter3<R>(grad, nodes[q + 1], -v, gs);
+  }
+}
+
+// ---------------------------------------------------------------- LJ
+template <class R>
+__device__ __forceinline__ R Sfun(R a, R mn, R mx) {
+  if (a < mn) return R(1);
+  if (a > mx) return R(0);
+  R t = (a - mn) / (mx - mn);
+  return R(1) - t * t * (R(3) - R(2) * t);
+}
+template <class R>
+__device__ __forceinline__ R dSfun(R a, R mn, R mx) {
+  if (a < mn || a > mx) return R(0);
+  R t = (a - mn) / (mx - mn);
+  return (R(-2) * t * (R(3) - R(2) * t) + R(2) * t * t) / (mx - mn);
+}
+
+constexpr int kPathCap = 320;  // path records per atom held in LDS
+
+// AIREBO::operator() (AIREBO_LJ.cxx:34-77), one wavefront per atom i.
+// Phase 1: the wave enumerates i's paths of <= 3 bonds into LDS, one lane per
+// (first, second) bond prefix, with key = bonds<<24 | s1<<16 | s2<<8 | s3
+// (the reference's loop order).  Phase 2: lanes sweep i's long list (j > i,
+// coalesced), look the target up among the path records, and either finish
+// the pair (S_r = 0) or queue it for the b* kernel.
+template <class R>
+__global__ void __launch_bounds__(256) k_lj_wave(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
+                                                double* __restrict__ grad, double* __restrict__ eacc,
+                                                int4* __restrict__ queue, int* __restrict__ qcount, int qcap,
+                                                long long* __restrict__ npairs, unsigned* err) {
+  __shared__ int s_t[4][kPathCap];
+  __shared__ int s_key[4][kPathCap];
+  __shared__ R s_w[4][kPathCap];
+  __shared__ int s_cnt[4];
+  __shared__ int s_off[4][17];
+  const LJParamsT<R>& L = T->lj;
+  BondArr<R> B = barr<R>(A);
+  const double gs = T->eunit / T->lunit;
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double e = 0.0;
+  long long np = 0;
+  for (int i = blockIdx.x * 4 + wv; i < A.n; i += gridDim.x * 4) {
+    int ti = A.type[i];
+    if (ti > T_H) continue;  // wave-uniform
+    if (lane == 0) s_cnt[wv] = 0;
+    int nb1 = A.bcnt[i];
+    int kk = -1, c2 = 0;
+    if (lane < nb1) {
+      kk = A.bj[i * A.maxb + lane];
+      c2 = A.bcnt[kk];
+    }
+    int incl = c2;
+    for (int o = 1; o < 16; o <<= 1) {
+      int v = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += v;
+    }
+    if (lane < 16) s_off[wv][lane + 1] = incl;
+    if (lane == 0) s_off[wv][0] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    int tot2 = s_off[wv][nb1];
+    auto push = [&](int target, R w, int key) {
+      if (target == i || !(w > R(0))) return;
+      int slot = atomicAdd(&s_cnt[wv], 1);
+      if (slot < kPathCap) {
+        s_t[wv][slot] = target;
+        s_w[wv][slot] = w;
+        s_key[wv][slot] = key;
+      } else {
+        raise(err, ERR_QUEUE_OVERFLOW);
+      }
+    };
+    if (lane < nb1) push(kk, B.f[i * A.maxb + lane], (1 << 24) | (lane << 16));
+    for (int c = lane; c < tot2; c += 64) {
+      int s1 = 0;
+      while (s1 + 1 < nb1 && s_off[wv][s1 + 1] <= c) s1++;
+      int s2 = c - s_off[wv][s1];
+      int k = A.bj[i * A.maxb + s1];
+      R f1 = B.f[i * A.maxb + s1];
+      int m = A.bj[k * A.maxb + s2];
+      R f2 = B.f[k * A.maxb + s2];
+      push(m, f1 * f2, (2 << 24) | (s1 << 16) | (s2 << 8));
+      int nb3 = A.bcnt[m];
+      for (int s3 = 0; s3 < nb3; s3++) {
+        int q = A.bj[m * A.maxb + s3];
+        R f3 = B.f[m * A.maxb + s3];
+        push(q, f1 * f2 * f3, (3 << 24) | (s1 << 16) | (s2 << 8) | s3);
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    int npath = min(s_cnt[wv], kPathCap);
+    // phase 2: pairs
+    int cnt = A.nll_cnt[i];
+    const int* list = A.nll + (size_t)i * A.cap_l;
+    double xi = A.x[i], yi = A.y[i], zi = A.z[i];
+    V3<R> gi = v3<R>(0, 0, 0);
+    for (int q = lane; q < cnt; q += 64) {
+      int j = list[q];
+      if (j <= i) continue;
+      int tj = A.type[j];
+      if (tj > T_H) continue;
+      D3 rd = d3(mic(xi - A.x[j], bx.L[0]), mic(yi - A.y[j], bx.L[1]), mic(zi - A.z[j], bx.L[2]));
+      double d2 = rd.x * rd.x + rd.y * rd.y + rd.z * rd.z;
+      if (d2 > double(L.R2sq[ti][tj]) * (T->lunit * T->lunit)) continue;
+      np++;
+      R w = R(0);
+      int bkey = 0x7fffffff;
+      for (int r = 0; r < npath; r++) {
+        if (s_t[wv][r] != j) continue;
+        R rw = s_w[wv][r];
+        int rk = s_key[wv][r];
+        if (rw > w || (rw == w && rk < bkey)) {
+          w = rw;
+          bkey = rk;
+        }
+      }
+      R C = R(1) - w;
+      if (!(C > R(0))) continue;
+      CPath path{0, -1, -1};
+      if (w > R(0)) {
+        path.hops = bkey >> 24;
+        int s1 = (bkey >> 16) & 0xff, s2 = (bkey >> 8) & 0xff;
+        path.k = A.bj[i * A.maxb + s1];
+        if (path.hops == 3) path.m = A.bj[path.k * A.maxb + s2];
+      }
+      V3<R> rv = to_local<R>(rd);
+      R r = sqrt(norm2(rv));
+      R Str = Sfun<R>(r, L.RLJ[ti][tj][0], L.RLJ[ti][tj][1]);
+      if (Str != R(0)) {
+        int slot = atomicAdd(qcount, 1);
+        if (slot < qcap) queue[slot] = make_int4(i, j, path.k, (path.hops << 28) | (path.m & 0x0fffffff));
+        else raise(err, ERR_QUEUE_OVERFLOW);
+        continue;
+      }
+      R f, df;
+      cutoff<R>(r, L.R[ti][tj][0], L.R[ti][tj][1], f, df);
+      if (f == R(0)) continue;
+      R s = L.sigma[ti][tj] / r;
+      R s6 = s;
+      for (int u = 2; u <= 6; u++) s6 *= s;
+      R s12 = s6 * s6;
+      R z = L.zeta[ti][tj];
+      R val = R(4) * z * (s12 - s6);
+      R der = R(4) * z * (R(-12) * s12 / r + R(6) * s6 / r);
+      R LJ = f * val;
+      R V = R(1) * C;  // VLJ seeded with SrSb*C, SrSb = 1
+      R cr = der * f * V + val * V * df;
+      V3<R> v = rv * (cr / r);
+      gi = gi + v;
+      scatter3<R>(grad, j, -v, gs);
+      if (path.hops > 0) cij_backward<R>(A, T, bx, i, j, path, R(1) * LJ, grad, gs);
+      e += to_erg<R>(double(R(1) * C * LJ), T->eunit);
+    }
+    double gx = wave_sum(double(gi.x)), gy = wave_sum(double(gi.y)), gz = wave_sum(double(gi.z));
+    if (lane == 0 && (gx != 0.0 || gy != 0.0 || gz != 0.0)) {
+      if constexpr (std::is_same<R, double>::value) atomic_add3(grad, i, d3(gx, gy, gz));
+      else atomic_add3(grad, i, d3(gx * gs, gy * gs, gz * gs));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+  }
+  block_accumulate(e, eacc);
+  if (np) atomicAdd((unsigned long long*)npairs, (unsigned long long)np);
+}
+
+// LJ pairs that need b*_ij (S_r != 0): the whole StrStb / VLJ / Cij chain.
+template <class R>
+__global__ void __launch_bounds__(256) k_bstar(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
+                                              const int4* __restrict__ queue, const int* __restrict__ qcount,
+                                              int qcap, double* __restrict__ grad, double* __restrict__ eacc,
+                                              unsigned* err) {
+  double e = 0.0;
+  const double gs = T->eunit / T->lunit;
+  int nq = min(*qcount, qcap);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nq; t += gridDim.x * blockDim.x) {
+    const LJParamsT<R>& L = T->lj;
+    const ReboParamsT<R>& P = T->rebo;
+    int4 qd = queue[t];
+    int i = qd.x, j = qd.y;
+    CPath path{qd.w >> 28, qd.z, qd.w & 0x0fffffff};
+    int ti = A.type[i], tj = A.type[j];
+    V3<R> rv = to_local<R>(depos(A.x, A.y, A.z, i, j, bx));
+    R r = sqrt(norm2(rv));
+    // path weight, product in the reference's order ((f1*f2)*f3)
+    R w = R(0);
+    if (path.hops > 0) {
+      int nodes[4] = {i, path.k, path.m, j};
+      nodes[path.hops] = j;
+      for (int q = 0; q < path.hops; q++) {
+        int a = nodes[q], b = nodes[q + 1];
+        R rr = sqrt(norm2(to_local<R>(depos(A.x, A.y, A.z, a, b, bx))));
+        R f, df;
+        cutoff<R>(rr, P.R[A.type[a]][A.type[b]][0], P.R[A.type[a]][A.type[b]][1], f, df);
+        w = (q == 0) ? f : w * f;
+      }
+    }
+    R C = R(1) - w;
+    R rmin = L.RLJ[ti][tj][0], rmax = L.RLJ[ti][tj][1];
+    R Str = Sfun<R>(r, rmin, rmax);
+    // b* pair: REBO cutoffs at the actual geometry, r_alter = R0 in ExpTerm
+    BPair<R> p;
+    p.i = i;
+    p.j = j;
+    p.ti = ti;
+    p.tj = tj;
+    p.rv = rv;
+    p.r = r;
+    R fb, dfb;
+    cutoff<R>(r, P.R[ti][tj][0], P.R[ti][tj][1], fb, dfb);
+    p.f = fb;
+    p.reff = P.R[ti][tj][0];
+    BOState st;
+    bo_forward<R>(A, T, p, st, err);
+    R bs = R(st.b);
+    R bmin = L.b[ti][tj][0], bmax = L.b[ti][tj][1];
+    R Stb = Sfun<R>(bs, bmin, bmax);
+    R SrSb = R(1) + (Str * Stb - Str);
+    R f, df;
+    cutoff<R>(r, L.R[ti][tj][0], L.R[ti][tj][1], f, df);
+    R s = L.sigma[ti][tj] / r;
+    R s6 = s;
+    for (int u = 2; u <= 6; u++) s6 *= s;
+    R s12 = s6 * s6;
+    R z = L.zeta[ti][tj];
+    R val = R(4) * z * (s12 - s6);
+    R der = R(4) * z * (R(-12) * s12 / r + R(6) * s6 / r);
+    R LJ = (f == R(0)) ? R(0) : f * val;
+    R cr = R(0);
+    if (f != R(0)) {
+      R V = SrSb * C;
+      cr += der * f * V + val * V * df;
+    }
+    // StrStb seeded with C*LJ
+    R V2 = C * LJ;
+    if (V2 != R(0)) {
+      R dStr = dSfun<R>(r, rmin, rmax);
+      if (dStr != R(0)) cr += -dStr * V2;
+      R dStb = dSfun<R>(bs, bmin, bmax);
+      if (dStr != R(0) && Stb != R(0)) cr += dStr * Stb * V2;
+      if (dStb != R(0)) bo_backward<R>(A, T, p, st, double(Str * dStb * V2), grad, gs, err);
+    }
+    V3<R> v = rv * (cr / r);
+    scatter3<R>(grad, i, v, gs);
+    scatter3<R>(grad, j, -v, gs);
+    if (path.hops > 0 && SrSb * LJ != R(0)) cij_backward<R>(A, T, bx, i, j, path, SrSb * LJ, grad, gs);
+    e += to_erg<R>(double(SrSb * C * LJ), T->eunit);
+  }
+  block_accumulate(e, eacc);
+}
+
+// ------------------------------------------------------------ torsion
+// ETors::operator() (AIREBO_ETors.cxx:32-115) over C-C bonds.
+template <class R>
+__global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
+                                              const int2* __restrict__ bonds, const int* __restrict__ nbonds,
+                                              int compat, double* __restrict__ grad, double* __restrict__ eacc,
+                                              long long* __restrict__ nquads) {
+  double e = 0.0;
+  long long nq = 0;
+  const double gs = T->eunit / T->lunit;
+  int nbt = min(*nbonds, A.bond_cap);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nbt; t += gridDim.x * blockDim.x) {
+    const ReboParamsT<R>& P = T->rebo;
+    int2 bd = bonds[t];
+    int i = bd.x, j = bd.y;
+    if (A.type[i] == T_C && A.type[j] == T_C) {
+      V3<R> a_ = to_local<R>(depos(A.x, A.y, A.z, i, j, bx));
+      R rij = sqrt(norm2(a_));
+      R wij, dwij;
+      cutoff<R>(rij, P.R[0][0][0], P.R[0][0][1], wij, dwij);
+      V3<R> gi = v3<R>(0, 0, 0), gj = v3<R>(0, 0, 0);
+      int nbi = A.bcnt[i], nbj = A.bcnt[j];
+      double eb = 0.0;
+      for (int s = 0; s < nbi; s++) {
+        BondRef<R> k = bond_at<R>(A, i, s);
+        if (k.k == j) continue;
+        R sin_ik = sqrt(norm2(cross(a_, k.rv))) / (rij * k.r);
+        for (int u = 0; u < nbj; u++) {
+          BondRef<R> l = bond_at<R>(A, j, u);
+          if (l.k == i || l.k == k.k) continue;
+          if (fabs(sin_ik) < R(0.1)) continue;
+          R sin_jl = sqrt(norm2(cross(a_, l.rv))) / (rij * l.r);
+          if (fabs(sin_jl) < R(0.1)) continue;
+          nq++;
+          int tk = A.type[k.k], tl = A.type[l.k];
+          R wki = k.f, wjl = l.f;
+          R zt = T->zetaT[tk][tl];
+          V3<R> b_ = k.rv, c_ = l.rv;
+          V3<R> Pp = cross(-a_, b_), Q = cross(a_, c_);
+          R p2 = norm2(Pp), q2 = norm2(Q);
+          R e2 = sqrt(p2) * sqrt(q2);
+          R cs = (e2 == R(0)) ? R(0) : dot(Pp, Q) / e2;
+          R cd = cs;
+          if (cd < R(-1)) cd = R(-1);
+          if (cd > R(1)) cd = R(1);
+          R c = -cd;
+          R x = R(0.5) * (R(1) + c);
+          R vt = R(256.0 / 405.0) * zt * pow(x, R(5)) - R(0.1) * zt;
+          V3<R> gk = v3<R>(0, 0, 0), gl = v3<R>(0, 0, 0);
+          if (vt != R(0)) {
+            R der = R(256.0 / 405.0) * zt * pow(x, R(5) - R(1)) * R(0.5);
+            if (!compat) der *= R(5);
+            R V = R(-1) * der * (wki * wij * wjl);
+            if (V != R(0) && e2 != R(0)) {
+              V3<R> gP = Q / e2 - Pp * (cs / p2);
+              V3<R> gQ = Pp / e2 - Q * (cs / q2);
+              V3<R> da = cross(gP, b_) + cross(c_, gQ);
+              V3<R> db = cross(a_, gP);
+              V3<R> dc = cross(gQ, a_);
+              gi = gi + (da + db) * V;
+              gj = gj + (dc - da) * V;
+              gk = gk - db * V;
+              gl = gl - dc * V;
+            }
+          }
+          // cutoff-weight derivatives
+          if (k.df != R(0)) {
+            V3<R> v = k.rv * (k.df * (wij * wjl * vt) / k.r);
+            gi = gi + v;
+            gk = gk - v;
+          }
+          if (dwij != R(0)) {
+            V3<R> v = a_ * (dwij * (wki * wjl * vt) / rij);
+            gi = gi + v;
+            gj = gj - v;
+          }
+          if (l.df != R(0)) {
+            V3<R> v = l.rv * (l.df * (wki * wij * vt) / l.r);
+            gj = gj + v;
+            gl = gl - v;
+          }
+          scatter3<R>(grad, k.k, gk, gs);
+          scatter3<R>(grad, l.k, gl, gs);
+          eb += double(wki * wij * wjl * vt);
+        }
+      }
+      e += to_erg<R>(eb, T->eunit);
+      scatter3<R>(grad, i, gi, gs);
+      scatter3<R>(grad, j, gj, gs);
+    }
+  }
+  block_accumulate(e, eacc);
+  if (nq) atomicAdd((unsigned long long*)nquads, (unsigned long long)nq);
+}
+
+#define MDG_INSTANTIATE(R)                                                                                     \
+  template __global__ void k_coord<R>(Atoms, const TablesT<R>*, Box, unsigned*);                              \
+  template __global__ void k_rebo_bonds<R>(Atoms, const TablesT<R>*, Box, const int2*, const int*, double*,  \
+                                           double*, unsigned*);                                                \
+  template __global__ void k_lj_wave<R>(Atoms, const TablesT<R>*, Box, double*, double*, int4*, int*, int,    \
+                                        long long*, unsigned*);                                                \
+  template __global__ void k_bstar<R>(Atoms, const TablesT<R>*, Box, const int4*, const int*, int, double*,   \
+                                      double*, unsigned*);                                                     \
+  template __global__ void k_etors<R>(Atoms, const TablesT<R>*, Box, const int2*, const int*, int, double*,   \
+                                      double*, long long*);
+MDG_INSTANTIATE(double)
+MDG_INSTANTIATE(float)
+
+}  // namespace mdg

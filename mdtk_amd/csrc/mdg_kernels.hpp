@@ -20,11 +20,9 @@ struct Atoms {
   int bond_cap;               // capacity of the (i<j) bond array
   int* bcnt;
   int* bj;
-  double *br, *bf, *bdf, *bx, *by, *bz;
+  double *br, *bf, *bdf, *bx, *by, *bz;       // FP64 geometry (cm)
+  float *brf, *bff, *bdff, *bxf, *byf, *bzf;  // FP32 geometry (Angstrom), mixed precision
   double *Nt, *NH, *NC;       // REBO coordination sums (REBO.hpp:250-258)
-  int cap_c;                  // C_ij table capacity per atom
-  int *c_cnt, *c_tgt, *c_k, *c_km;
-  double* c_w;
 };
 
 struct CellGrid {
@@ -47,16 +45,20 @@ __global__ void k_nl_build(const int* flag, int n, const double* x, const double
 __global__ void k_disp_top2(int n, const double* ds, const double* dl, double* part);
 __global__ void k_disp_decide(int nblk, const double* part, double lim_s, double lim_l, int* flags,
                               long long* rebuilds);
-__global__ void k_coord(Atoms A, const Tables* T, Box bx, unsigned* err);
 __global__ void k_bond_enum(Atoms A, int2* out, int* count, int cap, unsigned* err);
-__global__ void k_rebo_bonds(Atoms A, const Tables* T, Box bx, const int2* bonds, const int* nbonds,
+template <class R>
+__global__ void k_coord(Atoms A, const TablesT<R>* T, Box bx, unsigned* err);
+template <class R>
+__global__ void k_rebo_bonds(Atoms A, const TablesT<R>* T, Box bx, const int2* bonds, const int* nbonds,
                              double* grad, double* eacc, unsigned* err);
-__global__ void k_cij_build(Atoms A, unsigned* err);
-__global__ void k_lj_pairs(Atoms A, const Tables* T, Box bx, double* grad, double* eacc, int4* queue,
-                           int* qcount, int qcap, long long* npairs, unsigned* err);
-__global__ void k_bstar(Atoms A, const Tables* T, Box bx, const int4* queue, const int* qcount, int qcap,
+template <class R>
+__global__ void k_lj_wave(Atoms A, const TablesT<R>* T, Box bx, double* grad, double* eacc, int4* queue,
+                          int* qcount, int qcap, long long* npairs, unsigned* err);
+template <class R>
+__global__ void k_bstar(Atoms A, const TablesT<R>* T, Box bx, const int4* queue, const int* qcount, int qcap,
                         double* grad, double* eacc, unsigned* err);
-__global__ void k_etors(Atoms A, const Tables* T, Box bx, const int2* bonds, const int* nbonds, int compat,
+template <class R>
+__global__ void k_etors(Atoms A, const TablesT<R>* T, Box bx, const int2* bonds, const int* nbonds, int compat,
                         double* grad, double* eacc, long long* nquads);
 __global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* grad, double* eacc);
 __global__ void k_brenner_bonds(Atoms A, const Tables* T, Box bx, const int2* bonds, const int* nbonds,
@@ -64,10 +66,11 @@ __global__ void k_brenner_bonds(Atoms A, const Tables* T, Box bx, const int2* bo
 __global__ void k_drift(Atoms A, const double* dtp);
 __global__ void k_kick(Atoms A, Box bx, const double* dtp, double* red);
 __global__ void k_init_acc(Atoms A);
-__global__ void k_step_end(int nterms, double* eacc, double* red, double* dtp, int fixed_dt, double* out, int step);
+__global__ void k_step_end(int nterms, double* eacc, double* red, double* dtp, int fixed_dt, double* out, int* stepidx);
 __global__ void k_zero(double* p, size_t n);
 __global__ void k_zero_i(int* p, int n);
 
 void build_tables(Tables& T, int brenner_set);
+void convert_tables(const Tables& d, TablesT<float>& f);
 
 }  // namespace mdg

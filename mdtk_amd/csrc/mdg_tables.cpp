@@ -67,6 +67,8 @@ struct Grid {
 
 void build_tables(Tables& T, int brenner_set) {
   std::memset(&T, 0, sizeof(T));
+  T.lunit = 1.0;  // the FP64 table works in the reference's cm and erg
+  T.eunit = 1.0;
   const int C = 0, H = 1;
   const double Ao = kAo, eV = kEV;
   ReboParams& p = T.rebo;
@@ -89,6 +91,7 @@ void build_tables(Tables& T, int brenner_set) {
     for (int b = 0; b < 2; b++) p.R2sq[a][b] = p.R[a][b][1] * p.R[a][b][1];
   p.Ntot[0] = 2;
   p.Ntot[1] = 3;
+  p.expscale = 4.0 / Ao;  // REBO.cxx:221
   {
     const double x1[5] = {-1.0, -2.0 / 3.0, -1.0 / 2.0, -1.0 / 3.0, 1.0};
     const double y1[5] = {-0.010000, 0.028207, 0.052804, 0.097321, 1.000000};
@@ -274,6 +277,50 @@ void build_tables(Tables& T, int brenner_set) {
       for (int j = 0; j < 5; j++)
         for (int k = 3; k < 6; k++) F[i][j][k][0] = F[i][j][2][0];
   }
+}
+
+// The FP32 table of the mixed-precision form: lengths in Angstrom, energies in
+// eV, every entry rounded once from the FP64 CGS table.
+void convert_tables(const Tables& d, TablesT<float>& f) {
+  std::memset(&f, 0, sizeof(f));
+  const double L = 1.0 / kAo;  // cm -> Angstrom
+  const double E = 1.0 / kEV;  // erg -> eV
+  f.lunit = kAo;
+  f.eunit = kEV;
+  auto cv = [](float* out, const double* in, int n, double s) {
+    for (int q = 0; q < n; q++) out[q] = float(in[q] * s);
+  };
+  const ReboParams& a = d.rebo;
+  ReboParamsT<float>& b = f.rebo;
+  cv(&b.Q[0][0], &a.Q[0][0], 4, L);
+  cv(&b.alpha[0][0], &a.alpha[0][0], 4, 1.0 / L);
+  cv(&b.A[0][0], &a.A[0][0], 4, E);
+  cv(&b.B[0][0][0], &a.B[0][0][0], 12, E);
+  cv(&b.beta[0][0][0], &a.beta[0][0][0], 12, 1.0 / L);
+  cv(&b.rho[0][0], &a.rho[0][0], 4, L);
+  cv(&b.R[0][0][0], &a.R[0][0][0], 8, L);
+  cv(&b.Rp[0][0][0], &a.Rp[0][0][0], 8, L);
+  cv(&b.R2sq[0][0], &a.R2sq[0][0], 4, L * L);
+  cv(b.Ntot, a.Ntot, 2, 1.0);
+  b.expscale = float(a.expscale / L);
+  const GSpline* gs[3] = {&a.gC1, &a.gC2, &a.gH};
+  GSplineT<float>* gd[3] = {&b.gC1, &b.gC2, &b.gH};
+  for (int g = 0; g < 3; g++) {
+    gd[g]->count = gs[g]->count;
+    cv(gd[g]->xa, gs[g]->xa, 4, 1.0);
+    cv(gd[g]->xb, gs[g]->xb, 4, 1.0);
+    cv(gd[g]->h, gs[g]->h, 4, 1.0);
+    cv(&gd[g]->c[0][0], &gs[g]->c[0][0], 24, 1.0);
+  }
+  cv(&f.lj.sigma[0][0], &d.lj.sigma[0][0], 4, L);
+  cv(&f.lj.zeta[0][0], &d.lj.zeta[0][0], 4, E);
+  cv(&f.lj.RLJ[0][0][0], &d.lj.RLJ[0][0][0], 8, L);
+  cv(&f.lj.R[0][0][0], &d.lj.R[0][0][0], 8, L);
+  cv(&f.lj.R2sq[0][0], &d.lj.R2sq[0][0], 4, L * L);
+  cv(&f.lj.b[0][0][0], &d.lj.b[0][0][0], 8, 1.0);
+  cv(&f.zetaT[0][0], &d.zetaT[0][0], 4, E);
+  cv(&f.P[0][0][0][0], &d.P[0][0][0][0], 2 * 5 * 5 * 3, 1.0);
+  cv(&f.T3[0][0][0][0][0], &d.T3[0][0][0][0][0], 4 * 5 * 5 * 16 * 4, 1.0);
 }
 
 }  // namespace mdg
