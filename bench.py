@@ -1,19 +1,26 @@
-"""Benchmark: AIREBO NVE steps of a polyethylene crystal on one MI355X.
+"""Benchmark: batched AIREBO bombardment trajectories on one MI355X.
 
-Workload (one "step" = one velocity-Verlet iteration: drift, skin test and
-neighbour-list rebuild when due, REBO + AIREBO-LJ + torsion force call, kick,
-PBC wrap): orthorhombic polyethylene, physical two-chain Bunn cell
-(a=7.42, b=4.95, c=2.55 A, 12 atoms), 24x24x16 cells = 110,592 atoms,
-3-D periodic, 300 K Maxwell-Boltzmann velocities (Philox stream 0, seed
-697860), dt = 0.2 fs, reference parameter set (LJ switched off 5.0-5.5 A).
-Synthetic input; positions and velocities are resident in HBM before the
-timed region.
+Headline workload (one "step" = one velocity-Verlet iteration of every
+trajectory: drift, per-trajectory skin test and neighbour-list rebuild when
+due, FBZL + REBO + AIREBO-LJ + torsion force call, kick, PBC wrap, per-
+trajectory energies and adaptive-dt bookkeeping):
+  256 independent polyethylene slabs, each ~U[2000, 12000] atoms (seeded,
+  whole layers of 288 atoms; chains in the surface plane, x/y periodic, z
+  open, bottom chain row fixed), 300 K Maxwell-Boltzmann velocities, one Ar
+  ion 8 A above the surface at 100 eV, normal incidence, seeded impact point;
+  dt = 0.1 fs fixed.  All trajectories are advanced by one hipGraph launch
+  per step, in the mixed FP32/FP64 form.
+The plain FP64 drop-in runs the same trajectories the way the reference
+does, one trajectory per context and one after another; it is timed on a
+sample of the trajectories.  Synthetic input; positions and velocities are
+resident in HBM before every timed region.
 
-Prints ONE JSON line (rank 0).  `value` = atoms x steps / wall seconds of the
-timed region (hipEvents on the engine's stream, bracketed by device
-synchronisation).  `roofline` prices the dominant kernel against MI355X's FP64
-peak; `cpu_baseline` times the oracle (the CPU restatement of the reference's
-algorithm, single thread) on a bounded sample of the same workload.
+Prints ONE JSON line.  `value` = sum over trajectories of atoms x steps /
+wall seconds of the timed region (hipEvents on the engine's stream, bracketed
+by device synchronisation).  `roofline` prices the dominant kernel against
+MI355X's vector peak of its compute type; `cpu_baseline` times the oracle (the
+CPU restatement of the reference's algorithm, single thread) on a bounded
+sample of the same workload.
 """
 from __future__ import annotations
 
@@ -29,14 +36,24 @@ sys.path.insert(0, ROOT)
 METRIC = "atom-steps/s on 1 GPU, AIREBO: mixed+batched vs plain FP64 at force err ≤1e-5"
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
-HBM_PEAK_GBS = 8000.0
+POTS = ("FBZL", "REBO", "AIREBO", "ETors")
 
-# Algorithmic FP64 work per evaluation (forward + adjoint), tallied from the
-# kernel source with FMA=2, add/mul=1, div/sqrt=8, exp/sin/cos=20, pow=40
-# flops; per-neighbour terms use the measured neighbour counts of the workload.
+# Algorithmic work per evaluation (forward + adjoint), tallied from the kernel
+# source with FMA=2, add/mul=1, div/sqrt=8, exp/sin/cos=20, pow=40 flops;
+# per-neighbour terms use the measured neighbour counts of the workload.
 FLOP_BOND_ORDER_BASE = 900.0       # pair set-up, coordination, pi_rc tricubic, p_ij pow, Nconj
 FLOP_PER_SIGMA_NEIGHBOUR = 260.0   # cos theta, G quintic, cutoff/ExpTerm, adjoint scatter (per k, both passes)
 FLOP_LJ_PAIR = 120.0               # cutoff, sigma/r powers, C_ij merge, gradient
+FLOP_TORSION_QUAD = 420.0          # two cross products, dihedral cos, pow, adjoint of four atoms
+SIGMA_NEIGHBOURS = 2.0 * 3.3       # sigma-pi neighbours visited per bond order (both sides)
+
+KERNELS = ("k_bstar", "k_rebo_bonds", "k_lj_wave", "k_etors", "k_coord", "k_bond_enum", "k_nl_build_long",
+           "k_nl_build_short", "k_kick", "k_drift", "k_sys_disp", "k_sys_step_end", "k_zero", "k_fbzl",
+           "k_cell_assign", "k_cell_scan", "k_cell_fill", "k_cell_clear")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def parse():
@@ -44,44 +61,17 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cells", type=str, default="24,24,16")
+    ap.add_argument("--ntraj", type=int, default=256)
+    ap.add_argument("--dropin-sample", type=int, default=8, help="trajectories timed in the FP64 drop-in form")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-extra", action="store_true", help="skip the single-system PE and drift figures")
     return ap.parse_args()
 
 
-def cpu_baseline(budget_s: float):
-    """Oracle (single-threaded CPU restatement) on a 4x6x10-cell PE sample."""
-    import numpy as np
-    from mdtk_amd import systems as S
-    from oracle.oracle import Oracle, POT_REBO, POT_AIREBO, POT_ETORS, build
-    build()
-    s = S.polyethylene(4, 6, 10)
-    s.vel = S.maxwell_velocities(s, 300.0, stream=0)
-    o = Oracle.from_system(s, pots=POT_REBO | POT_AIREBO | POT_ETORS)
-    o.run(1, fixed_dt=0.2e-15)  # prepare + first step (NL build, initial force) untimed
-    steps = 0
-    t0 = time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
-        o.run(1, fixed_dt=0.2e-15)
-        steps += 1
-    dt = time.perf_counter() - t0
-    return {"value": s.n * steps / dt, "unit": "atom-steps/s", "cores": 1, "kind": "port",
-            "sample": f"polyethylene 4x6x10 Bunn cells ({s.n} atoms), REBO+AIREBO-LJ+ETors FP64, "
-                      f"dt=0.2 fs, {steps} steps in {dt:.1f} s, single thread"}
-
-
-KERNELS = ("k_bstar", "k_rebo_bonds", "k_lj_wave", "k_etors", "k_coord", "k_bond_enum", "k_nl_build_long",
-           "k_nl_build_short", "k_kick", "k_drift", "k_disp_top2", "k_disp_decide", "k_zero")
-
-
-def measure(sysm, precision, steps, warmup, dt):
-    """Times `steps` Verlet iterations (one hipGraph launch each) between two
-    HIP events on the engine's stream, then repeats them with per-kernel event
-    timing for the roofline."""
+def timed(eng, steps, warmup, dt):
+    """Milliseconds for `steps` Verlet iterations on the engine's stream."""
     import torch
-    from mdtk_amd import Engine
-    eng = Engine.from_system(sysm, potentials=("REBO", "AIREBO", "ETors"), precision=precision)
     stream = torch.cuda.Stream()
     eng.set_stream(stream.cuda_stream)
     eng.run(max(warmup, 1), fixed_dt=dt)
@@ -92,60 +82,133 @@ def measure(sysm, precision, steps, warmup, dt):
     res = eng.run(steps, fixed_dt=dt)
     t1.record(stream)
     torch.cuda.synchronize()
-    ms = t0.elapsed_time(t1)
+    return t0.elapsed_time(t1), res, stream
+
+
+def kernel_times(eng, steps, dt):
+    """Per-kernel HIP-event times over `steps` eager iterations (same stream)."""
+    import torch
     eng.profile(True)
     eng.run(steps, fixed_dt=dt)
+    torch.cuda.synchronize()
     kt = {k: eng.kernel_time(k) for k in KERNELS}
     eng.profile(False)
-    cnt = eng.counters()
-    return eng, ms, kt, cnt, res
+    return kt
 
 
-def roofline(kt, cnt):
+def roofline(kt, cnt, peak):
     dom = max(kt, key=lambda k: kt[k][0])
     dom_ms, dom_n = kt[dom]
     avg_s = dom_ms / max(dom_n, 1) / 1e3
-    n_sigma = 2.0 * 3.3  # sigma-pi neighbours visited per bond order (both sides), PE average
-    per_bo = FLOP_BOND_ORDER_BASE + FLOP_PER_SIGMA_NEIGHBOUR * n_sigma
-    if dom == "k_bstar":
-        flops = cnt["bstar"] * (per_bo + FLOP_LJ_PAIR)
-    elif dom == "k_rebo_bonds":
-        flops = cnt["bonds"] * (per_bo + 200.0)
-    else:
-        flops = cnt["lj_pairs"] * FLOP_LJ_PAIR
-    return dom, dom_n, avg_s, flops / avg_s / 1e12
+    per_bo = FLOP_BOND_ORDER_BASE + FLOP_PER_SIGMA_NEIGHBOUR * SIGMA_NEIGHBOURS
+    flops = {"k_bstar": cnt["bstar"] * (per_bo + FLOP_LJ_PAIR),
+             "k_rebo_bonds": cnt["bonds"] * (per_bo + 200.0),
+             "k_lj_wave": cnt["lj_pairs"] * FLOP_LJ_PAIR,
+             "k_etors": cnt["quads"] * FLOP_TORSION_QUAD}.get(dom)
+    if flops is None:
+        return {"bound": "hbm", "kernel": dom, "achieved": None, "peak": 8000.0, "unit": "GB/s", "frac": None,
+                "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": dom_n}
+    ach = flops / avg_s / 1e12
+    return {"bound": "mfma", "kernel": dom, "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+            "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": dom_n, "flop_per_launch": flops,
+            "note": "vector-ALU peak of the compute type (no dense contraction on this path, no MFMA)"}
+
+
+def force_rms(systems):
+    """RMS over atoms of |F_mixed - F_fp64| in eV/A at the initial configuration."""
+    import numpy as np
+    from mdtk_amd import Engine
+    from mdtk_amd import systems as S
+    e64 = Engine.from_systems(systems, potentials=POTS, etors_compat=False)
+    _, g64 = e64.energy_grad()
+    del e64
+    emx = Engine.from_systems(systems, potentials=POTS, etors_compat=False, precision="mixed")
+    _, gmx = emx.energy_grad()
+    del emx
+    return float(np.sqrt(((g64 - gmx) ** 2).sum(1).mean()) / (S.EV / S.ANG))
+
+
+def nve_drift(sysm, precision, steps, dt):
+    """Slope of E_tot/N (least squares) in eV/atom/ps, exact torsion adjoint."""
+    import numpy as np
+    from mdtk_amd import Engine
+    from mdtk_amd import systems as S
+    eng = Engine.from_system(sysm, potentials=("REBO", "AIREBO", "ETors"), etors_compat=False, precision=precision)
+    r = eng.run(steps, fixed_dt=dt)
+    e = (r["epot"] + r["ekin"]) / S.EV / sysm.n
+    t = np.arange(steps) * dt * 1e12
+    slope = float(np.polyfit(t, e, 1)[0])
+    return slope, float(np.abs(e - e[0]).max())
+
+
+def cpu_baseline(traj, budget_s: float):
+    """Oracle (single-threaded CPU restatement) on one trajectory of the batch."""
+    from oracle.oracle import Oracle, POT_FBZL, POT_REBO, POT_AIREBO, POT_ETORS, build
+    build()
+    o = Oracle.from_system(traj, pots=POT_FBZL | POT_REBO | POT_AIREBO | POT_ETORS, etors_compat=False)
+    o.run(1, fixed_dt=0.1e-15)  # set-up and first step (list build, initial force) untimed
+    steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        o.run(1, fixed_dt=0.1e-15)
+        steps += 1
+    dt = time.perf_counter() - t0
+    return {"value": traj.n * steps / dt, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+            "sample": f"one bombardment trajectory of the batch ({traj.n} atoms: PE slab + Ar at 100 eV), "
+                      f"FBZL+REBO+AIREBO-LJ+ETors FP64, dt=0.1 fs, {steps} steps in {dt:.1f} s, single thread"}
 
 
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py: this engine runs on one GPU per process; --gpus {a.gpus} is not supported "
-              "(independent trajectories shard with no communication; see DESIGN.md).", file=sys.stderr)
+              "(independent trajectories shard across GPUs with no communication).", file=sys.stderr)
         sys.exit(2)
     import numpy as np
+    from mdtk_amd import Engine
     from mdtk_amd import systems as S
 
-    nx, ny, nz = (int(v) for v in a.cells.split(","))
-    sysm = S.polyethylene(nx, ny, nz)
-    sysm.vel = S.maxwell_velocities(sysm, 300.0, stream=0)
-    dt = 0.2e-15
-    # force error of the mixed form against FP64 at the initial configuration
-    from mdtk_amd import Engine
-    e64 = Engine.from_system(sysm, potentials=("REBO", "AIREBO", "ETors"))
-    emx = Engine.from_system(sysm, potentials=("REBO", "AIREBO", "ETors"), precision="mixed")
-    E64, g64 = e64.energy_grad()
-    Emx, gmx = emx.energy_grad()
-    ev_per_a = S.EV / S.ANG
-    f_rms = float(np.sqrt(((g64 - gmx) ** 2).sum(1).mean()) / ev_per_a)
-    del e64, emx
-    _, ms64, kt64, cnt64, _ = measure(sysm, "fp64", a.steps, a.warmup, dt)
-    _, msmx, ktmx, cntmx, res = measure(sysm, "mixed", a.steps, a.warmup, dt)
-    v64 = sysm.n * a.steps / (ms64 / 1e3)
-    vmx = sysm.n * a.steps / (msmx / 1e3)
+    dt = 0.1e-15
+    batch = S.bombard_batch(a.ntraj)
+    natoms = sum(s.n for s in batch)
+    log(f"batch: {len(batch)} trajectories, {natoms} atoms")
+
+    f_rms = force_rms(batch)
+    log(f"mixed force RMS error vs FP64: {f_rms:.3e} eV/A")
+
+    # headline: mixed + batched, one graph launch per step for all trajectories
+    emx = Engine.from_systems(batch, potentials=POTS, etors_compat=False, precision="mixed")
+    ms_mx, res_mx, _ = timed(emx, a.steps, a.warmup, dt)
+    v_mx = natoms * a.steps / (ms_mx / 1e3)
+    log(f"mixed batched: {ms_mx / a.steps:.3f} ms/step, {v_mx:.4e} atom-steps/s")
+    kt_mx = kernel_times(emx, a.steps, dt)
+    cnt_mx = emx.counters()
+    del emx
+
+    # FP64 batched (same kernels in FP64)
+    e64 = Engine.from_systems(batch, potentials=POTS, etors_compat=False)
+    ms_64b, _, _ = timed(e64, a.steps, a.warmup, dt)
+    v_64b = natoms * a.steps / (ms_64b / 1e3)
+    log(f"fp64 batched: {ms_64b / a.steps:.3f} ms/step, {v_64b:.4e} atom-steps/s")
+    del e64
+
+    # plain FP64 drop-in: one trajectory per context, one after another
+    sample = batch[:a.dropin_sample]
+    tot_ms, tot_as = 0.0, 0
+    for s in sample:
+        e = Engine.from_system(s, potentials=POTS, etors_compat=False)
+        ms, _, _ = timed(e, a.steps, a.warmup, dt)
+        tot_ms += ms
+        tot_as += s.n * a.steps
+        del e
+    v_drop = tot_as / (tot_ms / 1e3)
+    log(f"fp64 drop-in (sequential single trajectories): {v_drop:.4e} atom-steps/s")
+
     use_mixed = f_rms <= 1e-5
-    value, ms, kt, cnt = (vmx, msmx, ktmx, cntmx) if use_mixed else (v64, ms64, kt64, cnt64)
-    dom, dom_n, avg_s, achieved = roofline(kt, cnt)
-    peak = FP32_PEAK_TFLOPS if use_mixed else FP64_PEAK_TFLOPS
+    value = v_mx if use_mixed else v_64b
+    ms = ms_mx if use_mixed else ms_64b
+    rl = roofline(kt_mx, cnt_mx, FP32_PEAK_TFLOPS)
+    et = (res_mx["epot"] + res_mx["ekin"]) / S.EV
     out = {
         "metric": METRIC,
         "value": value,
@@ -159,28 +222,42 @@ def main():
         "vs_baseline": None,
         "dtype": "f32/f64 mixed" if use_mixed else "f64",
         "data": "synthetic",
-        "config": {"workload": f"AIREBO NVE polyethylene {nx}x{ny}x{nz} Bunn cells (2 chains/cell)",
-                   "atoms": sysm.n, "dt_fs": 0.2, "potentials": "REBO+AIREBO-LJ+ETors (reference cutoffs)",
-                   "precision": "mixed" if use_mixed else "fp64", "batched": False,
-                   "parallelism": "single GPU"},
-        "fp64_value": v64,
-        "fp64_ms_per_step": ms64 / a.steps,
-        "mixed_value": vmx,
-        "mixed_over_fp64": vmx / v64,
+        "config": {"workload": (f"batched bombardment: {len(batch)} PE slabs ~U[2000,12000] atoms + Ar 100 eV, "
+                                "FBZL+REBO+AIREBO-LJ+ETors, dt 0.1 fs, one hipGraph launch per step"),
+                   "trajectories": len(batch), "atoms": natoms,
+                   "atoms_min": min(s.n for s in batch), "atoms_max": max(s.n for s in batch),
+                   "precision": "mixed" if use_mixed else "fp64", "batched": True,
+                   "torsion": "exact adjoint", "parallelism": "single GPU"},
+        "mixed_batched_value": v_mx,
+        "fp64_batched_value": v_64b,
+        "fp64_dropin_value": v_drop,
+        "fp64_dropin_sample": f"{len(sample)} trajectories, {sum(s.n for s in sample)} atoms, sequential contexts",
+        "mixed_batched_over_fp64_dropin": v_mx / v_drop,
         "mixed_force_rms_err_eV_per_A": f_rms,
-        "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                     "avg_launch_ms": avg_s * 1e3, "launches": dom_n,
-                     "note": ("vector-ALU peak of the compute type (no dense contraction on this path: "
-                              "no MFMA)")},
-        "kernels_ms_per_step": {k: v[0] / max(a.steps, 1) for k, v in kt.items()},
-        "kernels_ms_per_step_fp64": {k: v[0] / max(a.steps, 1) for k, v in kt64.items()},
-        "counters": cnt,
-        "energy_check": {"etot_first_eV": float((res["epot"][0] + res["ekin"][0]) / S.EV),
-                         "etot_last_eV": float((res["epot"][-1] + res["ekin"][-1]) / S.EV)},
+        "roofline": rl,
+        "kernels_ms_per_step": {k: v[0] / max(a.steps, 1) for k, v in kt_mx.items()},
+        "counters": cnt_mx,
+        "energy_check_eV": {"max_abs_dE_tot_over_steps": float(np.abs(et - et[0]).max())},
     }
+    if not a.no_extra:
+        pe = S.polyethylene(24, 24, 16)
+        pe.vel = S.maxwell_velocities(pe, 300.0, stream=0)
+        for prec in ("fp64", "mixed"):
+            e = Engine.from_system(pe, potentials=("REBO", "AIREBO", "ETors"), precision=prec)
+            ms_pe, _, _ = timed(e, a.steps, a.warmup, 0.2e-15)
+            out[f"pe110k_{prec}_value"] = pe.n * a.steps / (ms_pe / 1e3)
+            del e
+        log(f"pe110k fp64 {out['pe110k_fp64_value']:.4e}, mixed {out['pe110k_mixed_value']:.4e} atom-steps/s")
+        small = S.polyethylene(8, 12, 8)
+        small.vel = S.maxwell_velocities(small, 300.0, stream=1)
+        drift = {}
+        for prec in ("fp64", "mixed"):
+            slope, dmax = nve_drift(small, prec, 2000, 0.2e-15)
+            drift[prec] = {"slope_eV_per_atom_per_ps": slope, "max_abs_dE_eV_per_atom": dmax}
+        out["nve_drift"] = {"system": f"PE 8x12x8 ({small.n} atoms), 300 K, 2000 steps of 0.2 fs", **drift}
+        log(f"nve drift {drift}")
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(min(batch, key=lambda s: s.n), a.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

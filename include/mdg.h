@@ -64,6 +64,23 @@ const char* mdg_last_error(const mdg_ctx* ctx);
 int mdg_set_system(mdg_ctx* ctx, int n, const int32_t* elem, const double* xyz, const double* vel,
                    const double pbc[3], const uint8_t* fixed);
 
+/* A batch of nsys independent systems in one context, advanced together by
+ * every call (one launch per kernel for the whole batch): the reference runs
+ * one SimLoop per trajectory (e.g. one per bombardment in
+ * applications/mdtk-bombard, SimLoop.cxx:252-445); this replaces nsys of
+ * them.  counts[nsys] atoms per system; elem/xyz/vel/fixed hold the systems'
+ * atoms one after another (sum(counts) entries, as mdg_set_system); pbc[3*nsys]
+ * each system's box.  Systems never interact; each keeps its own neighbour
+ * lists, skin test and adaptive dt.  Calls that return per-system values
+ * (mdg_energy_grad's e_terms, mdg_step_nve's outputs) then return nsys of
+ * them, system-minor.  No single reference interface: the batch is the
+ * caller's loop over trajectories. */
+int mdg_set_systems(mdg_ctx* ctx, int nsys, const int32_t* counts, const int32_t* elem, const double* xyz,
+                    const double* vel, const double* pbc, const uint8_t* fixed);
+
+/* Number of systems in the context (1 after mdg_set_system). */
+int mdg_num_systems(const mdg_ctx* ctx);
+
 /* New positions for the host-driven loop, where the caller's SimLoop moves
  * the atoms (SimLoop.cxx:323-332): xyz[n*3] (cm, wrapped into the box as
  * Atom::applyPBC leaves them), pbc_count[n*3] (may be null: unchanged) and
@@ -84,7 +101,8 @@ int mdg_term_kind(const mdg_ctx* ctx, int i);
 
 /* One force call at the current positions: replaces FProxy::operator()
  * (FProxy.cxx:45-61) with each FGeneral::operator() (FGeneral.hpp:94).
- * e_terms[mdg_num_terms]: energy of each member (erg); grad[n*3]: dE/dx
+ * e_terms[nsys*mdg_num_terms]: energy of each member (erg), system s at
+ * e_terms[s*mdg_num_terms + t]; grad[n*3]: dE/dx
  * (erg/cm), i.e. Atom::grad after the call.  Rebuilds a neighbour list first
  * when its skin test asks for it (NeighbourList.cxx:108-133). */
 int mdg_energy_grad(mdg_ctx* ctx, double* e_terms, double* grad);
@@ -99,7 +117,8 @@ int mdg_nl_export(mdg_ctx* ctx, int term, int32_t* counts, int32_t* idx, int64_t
  * dt rule.  fixed_dt > 0 fixes dt (as a doBeforeIteration hook would);
  * otherwise dt = 0.05 A / v_max clamped to [1e-20, 5e-16] s (:409-416).
  * Per step outputs (each may be null): epot = E_pot(t+dt) (erg), ekin =
- * E_kin after the velocity update (erg), dt_used (s), netforce = |sum F|. */
+ * E_kin after the velocity update (erg), dt_used (s), netforce = |sum F|;
+ * each holds nsteps*nsys values, step k of system s at [k*nsys + s]. */
 int mdg_step_nve(mdg_ctx* ctx, int nsteps, double fixed_dt, double* epot, double* ekin, double* dt_used,
                  double* netforce);
 

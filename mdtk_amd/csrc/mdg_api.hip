@@ -1,6 +1,7 @@
-// Host side of the C ABI declared in include/mdg.h: one context owns one
-// system's device buffers, the parameter tables and a HIP stream; every call
-// is stream-ordered and synchronous unless it says otherwise.
+// Host side of the C ABI declared in include/mdg.h: one context owns the
+// device buffers of one system or of a batch of independent systems, the
+// parameter tables and a HIP stream; every call is stream-ordered and
+// synchronous unless it says otherwise.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -38,7 +39,11 @@ struct mdg_ctx {
   double rc_short = 0, rc_long = 0;
 
   int n = 0;
-  Box box{};
+  int nsys = 0;
+  std::vector<double> Lh;      // 3*nsys box lengths (host copy)
+  std::vector<int> off_h;      // nsys+1 atom offsets (host copy)
+  std::vector<double> dt_host;  // per-system dt carried between calls
+  Box box{};                   // device: L[3*nsys], sys[n]
   Atoms A{};
   Tables* dT = nullptr;
   TablesT<float>* dTf = nullptr;  // mixed precision: Angstrom/eV FP32 table
@@ -46,28 +51,29 @@ struct mdg_ctx {
   hipStream_t own = nullptr, stream = nullptr;
   std::vector<void*> allocs;
 
-  CellGrid gs{}, gl{};
+  // one cell grid per system and list; cell ids offset per system
+  CellGrid *gs_d = nullptr, *gl_d = nullptr;
+  int *coff_s = nullptr, *coff_l = nullptr;
   int ncs = 0, ncl = 0;
   int *cell_of_s = nullptr, *cnt_s = nullptr, *start_s = nullptr, *atoms_s = nullptr;
   int *cell_of_l = nullptr, *cnt_l = nullptr, *start_l = nullptr, *atoms_l = nullptr;
   unsigned char *handled_s = nullptr, *handled_l = nullptr;
-  int* flags = nullptr;  // [0] short, [1] long rebuild requested
-  double* part = nullptr;
+  int* flags = nullptr;   // [0] short, [1] long rebuild requested by some system
+  int* sflags = nullptr;  // [2*s + list] rebuild requested by system s
   int2* bonds = nullptr;
   int* nbonds = nullptr;
   int bond_cap = 0;
   int4* queue = nullptr;
   int* qcount = nullptr;
   int qcap = 0;
-  double* eacc = nullptr;
-  double* red = nullptr;
-  double* dtp = nullptr;
+  double* eatom = nullptr;  // nterms*n per-atom energies of the last force call
+  double* eacc = nullptr;   // 8*nsys per-system energies of each chain member
+  double* dtp = nullptr;    // nsys
   double* stepout = nullptr;
-  int step_cap = 0;
+  int step_cap = 0;         // steps per output chunk
   long long* counters = nullptr;  // [0] short rebuilds [1] long rebuilds [2] LJ pairs [3] quads
   unsigned* derr = nullptr;
   long iteration = 0;
-  double dt_host = 1e-20;  // SimLoop.cxx:50
 
   bool profiling = false;
   std::map<std::string, KStat> kst;
@@ -249,7 +255,8 @@ inline int blocks(long long work, int bs = 256, int cap = 1 << 20) {
 // Neighbour-list rebuild for one list, gated on the device flag.
 void nl_build(mdg_ctx* c, bool shortlist) {
   const int* flag = c->flags + (shortlist ? 0 : 1);
-  CellGrid& g = shortlist ? c->gs : c->gl;
+  const CellGrid* g = shortlist ? c->gs_d : c->gl_d;
+  const int* coff = shortlist ? c->coff_s : c->coff_l;
   int nc = shortlist ? c->ncs : c->ncl;
   int* cell_of = shortlist ? c->cell_of_s : c->cell_of_l;
   int* cnt = shortlist ? c->cnt_s : c->cnt_l;
@@ -261,28 +268,37 @@ void nl_build(mdg_ctx* c, bool shortlist) {
   int n = c->n;
   Atoms& A = c->A;
   launch(c, "k_cell_clear", [&] { k_cell_clear<<<blocks(nc), 256, 0, c->stream>>>(flag, nc, cnt); });
-  launch(c, "k_cell_assign",
-         [&] { k_cell_assign<<<blocks(n), 256, 0, c->stream>>>(flag, n, A.x, A.y, A.z, g, cell_of, cnt); });
+  launch(c, "k_cell_assign", [&] {
+    k_cell_assign<<<blocks(n), 256, 0, c->stream>>>(flag, n, A.x, A.y, A.z, g, coff, A.sys, cell_of, cnt);
+  });
   launch(c, "k_cell_scan", [&] { k_cell_scan<<<1, 1024, 0, c->stream>>>(flag, nc, cnt, start); });
   launch(c, "k_cell_fill",
          [&] { k_cell_fill<<<blocks(n), 256, 0, c->stream>>>(flag, n, cell_of, start, cnt, cat); });
   if (shortlist) {
     launch(c, "k_nl_build_short", [&] {
-      k_nl_build<64><<<blocks(n, 4), 256, 0, c->stream>>>(flag, n, A.x, A.y, A.z, c->handled_s, c->box, g, range2,
-                                                           cell_of, start, cat, A.nls_cnt, A.nls, A.disp_s,
-                                                           c->derr);
+      k_nl_build<64><<<blocks(n, 4), 256, 0, c->stream>>>(flag, c->sflags, 0, n, A.x, A.y, A.z, c->handled_s,
+                                                           c->box, g, coff, range2, cell_of, start, cat,
+                                                           A.nls_cnt, A.nls, A.disp_s, c->derr);
     });
   } else {
     launch(c, "k_nl_build_long", [&] {
-      k_nl_build<512><<<blocks(n, 4), 256, 0, c->stream>>>(flag, n, A.x, A.y, A.z, c->handled_l, c->box, g,
-                                                            range2, cell_of, start, cat, A.nll_cnt, A.nll,
-                                                            A.disp_l, c->derr);
+      k_nl_build<512><<<blocks(n, 4), 256, 0, c->stream>>>(flag, c->sflags, 1, n, A.x, A.y, A.z, c->handled_l,
+                                                            c->box, g, coff, range2, cell_of, start, cat,
+                                                            A.nll_cnt, A.nll, A.disp_l, c->derr);
     });
   }
 }
 
-__global__ void k_clear_flags(int* flags) {
-  if (threadIdx.x < 2) flags[threadIdx.x] = 0;
+// Clears one list's rebuild flags (all systems).
+__global__ void k_clear_list(int nsys, int list, int* flags, int* sflags) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) flags[list] = 0;
+  if (s < nsys) sflags[2 * s + list] = 0;
+}
+
+__global__ void k_fill(double* p, int n, double v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 __global__ void k_zero_counts(long long* counters) {
@@ -306,30 +322,30 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   if (c->slot_rebo >= 0) {
     launch(c, "k_rebo_bonds", [&] {
       k_rebo_bonds<R><<<bgrid, 256, 0, c->stream>>>(A, T, c->box, c->bonds, c->nbonds, A.grad,
-                                                    c->eacc + c->slot_rebo, c->derr);
+                                                    c->eatom + (size_t)c->slot_rebo * n, c->derr);
     });
   }
   if (c->slot_bren >= 0) {
     launch(c, "k_brenner_bonds", [&] {
       k_brenner_bonds<<<bgrid, 256, 0, c->stream>>>(A, c->dT, c->box, c->bonds, c->nbonds, A.grad,
-                                                    c->eacc + c->slot_bren, c->derr);
+                                                    c->eatom + (size_t)c->slot_bren * n, c->derr);
     });
   }
   if (c->slot_lj >= 0) {
     launch(c, "k_lj_wave", [&] {
-      k_lj_wave<R><<<blocks(n, 4, 65536), 256, 0, c->stream>>>(A, T, c->box, A.grad, c->eacc + c->slot_lj,
+      k_lj_wave<R><<<blocks(n, 4, 65536), 256, 0, c->stream>>>(A, T, c->box, A.grad, c->eatom + (size_t)c->slot_lj * n,
                                                                 c->queue, c->qcount, c->qcap, c->counters + 2,
                                                                 c->derr);
     });
     launch(c, "k_bstar", [&] {
       k_bstar<R><<<blocks(8LL * n, 256, 16384), 256, 0, c->stream>>>(A, T, c->box, c->queue, c->qcount, c->qcap,
-                                                                     A.grad, c->eacc + c->slot_lj, c->derr);
+                                                                     A.grad, c->eatom + (size_t)c->slot_lj * n, c->derr);
     });
   }
   if (c->slot_tors >= 0) {
     launch(c, "k_etors", [&] {
       k_etors<R><<<bgrid, 256, 0, c->stream>>>(A, T, c->box, c->bonds, c->nbonds, c->cfg.etors_compat, A.grad,
-                                               c->eacc + c->slot_tors, c->counters + 3);
+                                               c->eatom + (size_t)c->slot_tors * n, c->counters + 3);
     });
   }
 }
@@ -341,8 +357,11 @@ void force_call(mdg_ctx* c) {
   Atoms& A = c->A;
   if (c->have_short) nl_build(c, true);
   if (c->have_long) nl_build(c, false);
-  launch(c, "k_zero", [&] { k_zero<<<blocks(3LL * n, 256, 4096), 256, 0, c->stream>>>(A.grad, (size_t)3 * n); });
-  k_zero<<<1, 64, 0, c->stream>>>(c->eacc, 8);
+  size_t nt = c->kinds.size();
+  launch(c, "k_zero", [&] {
+    k_zero<<<blocks(3LL * n, 256, 4096), 256, 0, c->stream>>>(A.grad, (size_t)3 * n);
+    k_zero<<<blocks((long long)nt * n, 256, 4096), 256, 0, c->stream>>>(c->eatom, nt * n);
+  });
   k_zero_i<<<1, 64, 0, c->stream>>>(c->nbonds, 1);
   k_zero_i<<<1, 64, 0, c->stream>>>(c->qcount, 1);
   k_zero_counts<<<1, 64, 0, c->stream>>>(c->counters);
@@ -351,10 +370,10 @@ void force_call(mdg_ctx* c) {
   if (c->slot_fbzl >= 0) {
     launch(c, "k_fbzl", [&] {
       k_fbzl<<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->cfg.fbzl_r0, c->cfg.fbzl_r1, A.grad,
-                                               c->eacc + c->slot_fbzl);
+                                               c->eatom + (size_t)c->slot_fbzl * n);
     });
   }
-  k_clear_flags<<<1, 64, 0, c->stream>>>(c->flags);
+  k_clear_flags<<<blocks(2LL * c->nsys), 256, 0, c->stream>>>(c->nsys, c->flags, c->sflags);
 }
 }  // namespace
 
@@ -445,8 +464,7 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
       return 1;
     }
   }
-  if (dalloc(c, c->flags, 2) || dalloc(c, c->part, 4 * 256) || dalloc(c, c->eacc, 8) || dalloc(c, c->red, 8) ||
-      dalloc(c, c->dtp, 1) || dalloc(c, c->counters, 8) || dalloc(c, c->derr, 1) || dalloc(c, c->nbonds, 1) ||
+  if (dalloc(c, c->flags, 2) || dalloc(c, c->counters, 8) || dalloc(c, c->derr, 1) || dalloc(c, c->nbonds, 1) ||
       dalloc(c, c->qcount, 1)) {
     g_create_err = c->err;
     delete c;
@@ -454,7 +472,6 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
   }
   hipMemset(c->counters, 0, 8 * sizeof(long long));
   hipMemset(c->derr, 0, sizeof(unsigned));
-  hipMemset(c->red, 0, 8 * sizeof(double));
   *out = c;
   return 0;
 }
@@ -472,56 +489,85 @@ int mdg_set_stream(mdg_ctx* c, void* s) {
   return 0;
 }
 
-int mdg_set_system(mdg_ctx* c, int n, const int32_t* elem, const double* xyz, const double* vel,
-                   const double pbc[3], const uint8_t* fixed) {
-  if (n <= 0) {
-    c->err = "empty system (AtomsContainer.cxx:108)";
+int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* elem, const double* xyz,
+                    const double* vel, const double* pbc, const uint8_t* fixed) {
+  if (nsys <= 0) {
+    c->err = "no systems";
     return 1;
   }
   if (c->n != 0) {
-    c->err = "mdg_set_system: a context holds one system; create a new context";
+    c->err = "mdg_set_system(s): a context holds one system or one batch; create a new context";
     return 1;
   }
+  std::vector<int> off(nsys + 1, 0);
+  for (int s = 0; s < nsys; s++) {
+    if (counts[s] <= 0) {
+      c->err = "empty system (AtomsContainer.cxx:108)";
+      return 1;
+    }
+    if ((long long)off[s] + counts[s] > (1LL << 30)) {
+      c->err = "too many atoms in one context";
+      return 1;
+    }
+    off[s + 1] = off[s] + counts[s];
+  }
+  int n = off[nsys];
+  double rcmax = std::max(c->rc_short, c->rc_long);  // checkMIC(2 Rc), SimLoop.cxx:239-245
+  for (int s = 0; s < nsys; s++)
+    for (int d = 0; d < 3; d++)
+      if (pbc[3 * s + d] <= rcmax * 2.0) {
+        c->err = "Rcutoff is too large for given PBC ! (SimLoop.cxx:239-245)";
+        if (nsys > 1) c->err += " [system " + std::to_string(s) + "]";
+        return 1;
+      }
   std::vector<double> p(xyz, xyz + 3 * (size_t)n);
-  std::vector<int> pc(3 * (size_t)n, 0), ty(n), el(n);
+  std::vector<int> pc(3 * (size_t)n, 0), ty(n), el(n), sy(n);
   std::vector<double> ms(n), zs(n);
   std::vector<unsigned char> fx(n, 0), hs(n), hl(n);
-  for (int d = 0; d < 3; d++) c->box.L[d] = pbc[d];
-  for (int i = 0; i < n; i++) {
-    el[i] = elem[i];
-    ty[i] = type_of(elem[i]);
-    ms[i] = mass_of(elem[i]);
-    zs[i] = charge_of(elem[i]);
-    if (ms[i] <= 0) {
-      c->err = "unknown element id " + std::to_string(elem[i]);
-      return 1;
-    }
-    fx[i] = fixed ? (fixed[i] ? 1 : 0) : 0;
-    hs[i] = (ty[i] <= T_H) ? 1 : 0;
-    hl[i] = 1;
-    for (int d = 0; d < 3; d++) {  // Atom::applyPBC (Atom.cxx:93-125)
-      double L = pbc[d];
-      if (L == kNoPbc) continue;
-      double& q = p[3 * i + d];
-      int& k = pc[3 * i + d];
-      int guard = 0;
-      while (q < 0 && guard++ < 1000000) { q += L; --k; }
-      while (q >= L && guard++ < 1000000) { q -= L; ++k; }
-    }
-  }
-  double rcmax = std::max(c->rc_short, c->rc_long);  // checkMIC(2 Rc), SimLoop.cxx:239-245
-  for (int d = 0; d < 3; d++)
-    if (pbc[d] <= rcmax * 2.0) {
-      c->err = "Rcutoff is too large for given PBC ! (SimLoop.cxx:239-245)";
-      return 1;
+  for (int s = 0; s < nsys; s++)
+    for (int i = off[s]; i < off[s + 1]; i++) {
+      sy[i] = s;
+      el[i] = elem[i];
+      ty[i] = type_of(elem[i]);
+      ms[i] = mass_of(elem[i]);
+      zs[i] = charge_of(elem[i]);
+      if (ms[i] <= 0) {
+        c->err = "unknown element id " + std::to_string(elem[i]);
+        return 1;
+      }
+      fx[i] = fixed ? (fixed[i] ? 1 : 0) : 0;
+      hs[i] = (ty[i] <= T_H) ? 1 : 0;
+      hl[i] = 1;
+      for (int d = 0; d < 3; d++) {  // Atom::applyPBC (Atom.cxx:93-125)
+        double L = pbc[3 * s + d];
+        if (L == kNoPbc) continue;
+        double& q = p[3 * i + d];
+        int& k = pc[3 * i + d];
+        int guard = 0;
+        while (q < 0 && guard++ < 1000000) { q += L; --k; }
+        while (q >= L && guard++ < 1000000) { q -= L; ++k; }
+      }
     }
   c->n = n;
+  c->nsys = nsys;
+  c->off_h = off;
+  c->Lh.assign(pbc, pbc + 3 * (size_t)nsys);
+  c->dt_host.assign(nsys, 1e-20);  // SimLoop.cxx:50
   Atoms& A = c->A;
   A.n = n;
+  A.nsys = nsys;
   A.cap_s = c->cfg.max_short_nb;
   A.cap_l = c->cfg.max_long_nb;
   A.maxb = 16;
   int r = 0;
+  double* Ld = nullptr;
+  int* sysd = nullptr;
+  r |= dalloc(c, Ld, 3 * (size_t)nsys) | dalloc(c, sysd, n) | dalloc(c, A.sys_off, nsys + 1);
+  A.sys = sysd;
+  c->box.L = Ld;
+  c->box.sys = sysd;
+  r |= dalloc(c, c->sflags, 2 * (size_t)nsys) | dalloc(c, c->dtp, nsys) | dalloc(c, c->eacc, 8 * (size_t)nsys);
+  r |= dalloc(c, c->eatom, c->kinds.size() * (size_t)n);
   r |= dalloc(c, A.x, n) | dalloc(c, A.y, n) | dalloc(c, A.z, n);
   r |= dalloc(c, A.vx, n) | dalloc(c, A.vy, n) | dalloc(c, A.vz, n);
   r |= dalloc(c, A.ax, n) | dalloc(c, A.ay, n) | dalloc(c, A.az, n);
@@ -548,16 +594,33 @@ int mdg_set_system(mdg_ctx* c, int n, const int32_t* elem, const double* xyz, co
   if (c->slot_lj >= 0) r |= dalloc(c, c->queue, c->qcap);
   r |= dalloc(c, c->handled_s, n) | dalloc(c, c->handled_l, n);
   if (r) return 1;
-  // cell grids
+  // cell grids, one per system and list
+  auto grids = [&](double rc, CellGrid*& gd, int*& coff, int& ncell) -> int {
+    std::vector<CellGrid> g(nsys);
+    std::vector<int> co(nsys + 1, 0);
+    for (int s = 0; s < nsys; s++) {
+      std::vector<double> ps(p.begin() + 3 * (size_t)off[s], p.begin() + 3 * (size_t)off[s + 1]);
+      g[s] = make_grid(&c->Lh[3 * s], ps, off[s + 1] - off[s], (1.0 + 0.44) * rc);
+      long long cells = (long long)g[s].n[0] * g[s].n[1] * g[s].n[2];
+      if (co[s] + cells > (1LL << 28)) {
+        c->err = "cell grid too large";
+        return 1;
+      }
+      co[s + 1] = co[s] + (int)cells;
+    }
+    ncell = co[nsys];
+    if (dalloc(c, gd, nsys) || dalloc(c, coff, nsys + 1)) return 1;
+    HIPCHK(c, hipMemcpy(gd, g.data(), nsys * sizeof(CellGrid), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(coff, co.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice));
+    return 0;
+  };
   if (c->have_short) {
-    c->gs = make_grid(c->box.L, p, n, (1.0 + 0.44) * c->rc_short);
-    c->ncs = c->gs.n[0] * c->gs.n[1] * c->gs.n[2];
+    if (grids(c->rc_short, c->gs_d, c->coff_s, c->ncs)) return 1;
     r |= dalloc(c, c->cell_of_s, n) | dalloc(c, c->cnt_s, c->ncs) | dalloc(c, c->start_s, c->ncs + 1) |
          dalloc(c, c->atoms_s, n);
   }
   if (c->have_long) {
-    c->gl = make_grid(c->box.L, p, n, (1.0 + 0.44) * c->rc_long);
-    c->ncl = c->gl.n[0] * c->gl.n[1] * c->gl.n[2];
+    if (grids(c->rc_long, c->gl_d, c->coff_l, c->ncl)) return 1;
     r |= dalloc(c, c->cell_of_l, n) | dalloc(c, c->cnt_l, c->ncl) | dalloc(c, c->start_l, c->ncl + 1) |
          dalloc(c, c->atoms_l, n);
   }
@@ -578,36 +641,53 @@ int mdg_set_system(mdg_ctx* c, int n, const int32_t* elem, const double* xyz, co
     pz[i] = pc[3 * i + 2];
   }
   auto up = [&](void* d, const void* s, size_t b) { return hipMemcpy(d, s, b, hipMemcpyHostToDevice); };
-  HIPCHK(c, up(A.x, xs.data(), n * 8));
-  HIPCHK(c, up(A.y, ys.data(), n * 8));
-  HIPCHK(c, up(A.z, zs2.data(), n * 8));
-  HIPCHK(c, up(A.vx, vxs.data(), n * 8));
-  HIPCHK(c, up(A.vy, vys.data(), n * 8));
-  HIPCHK(c, up(A.vz, vzs.data(), n * 8));
-  HIPCHK(c, hipMemset(A.ax, 0, n * 8));
-  HIPCHK(c, hipMemset(A.ay, 0, n * 8));
-  HIPCHK(c, hipMemset(A.az, 0, n * 8));
-  HIPCHK(c, up(A.mass, ms.data(), n * 8));
-  HIPCHK(c, up(A.Z, zs.data(), n * 8));
-  HIPCHK(c, up(A.type, ty.data(), n * 4));
-  HIPCHK(c, up(A.elem, el.data(), n * 4));
+  HIPCHK(c, up(Ld, c->Lh.data(), 3 * (size_t)nsys * 8));
+  HIPCHK(c, up(sysd, sy.data(), (size_t)n * 4));
+  HIPCHK(c, up(A.sys_off, off.data(), (nsys + 1) * 4));
+  HIPCHK(c, up(A.x, xs.data(), (size_t)n * 8));
+  HIPCHK(c, up(A.y, ys.data(), (size_t)n * 8));
+  HIPCHK(c, up(A.z, zs2.data(), (size_t)n * 8));
+  HIPCHK(c, up(A.vx, vxs.data(), (size_t)n * 8));
+  HIPCHK(c, up(A.vy, vys.data(), (size_t)n * 8));
+  HIPCHK(c, up(A.vz, vzs.data(), (size_t)n * 8));
+  HIPCHK(c, hipMemset(A.ax, 0, (size_t)n * 8));
+  HIPCHK(c, hipMemset(A.ay, 0, (size_t)n * 8));
+  HIPCHK(c, hipMemset(A.az, 0, (size_t)n * 8));
+  HIPCHK(c, up(A.mass, ms.data(), (size_t)n * 8));
+  HIPCHK(c, up(A.Z, zs.data(), (size_t)n * 8));
+  HIPCHK(c, up(A.type, ty.data(), (size_t)n * 4));
+  HIPCHK(c, up(A.elem, el.data(), (size_t)n * 4));
   HIPCHK(c, up(A.fixed, fx.data(), n));
-  HIPCHK(c, up(A.pcx, px.data(), n * 4));
-  HIPCHK(c, up(A.pcy, py.data(), n * 4));
-  HIPCHK(c, up(A.pcz, pz.data(), n * 4));
+  HIPCHK(c, up(A.pcx, px.data(), (size_t)n * 4));
+  HIPCHK(c, up(A.pcy, py.data(), (size_t)n * 4));
+  HIPCHK(c, up(A.pcz, pz.data(), (size_t)n * 4));
   HIPCHK(c, hipMemset(A.disp_s, 0, 3 * (size_t)n * 8));
   HIPCHK(c, hipMemset(A.disp_l, 0, 3 * (size_t)n * 8));
-  HIPCHK(c, hipMemset(A.nls_cnt, 0, n * 4));
-  HIPCHK(c, hipMemset(A.nll_cnt, 0, n * 4));
-  HIPCHK(c, hipMemset(A.bcnt, 0, n * 4));
+  HIPCHK(c, hipMemset(A.nls_cnt, 0, (size_t)n * 4));
+  HIPCHK(c, hipMemset(A.nll_cnt, 0, (size_t)n * 4));
+  HIPCHK(c, hipMemset(A.bcnt, 0, (size_t)n * 4));
   HIPCHK(c, up(c->handled_s, hs.data(), n));
   HIPCHK(c, up(c->handled_l, hl.data(), n));
   int fl[2] = {1, 1};
+  std::vector<int> sfl(2 * (size_t)nsys, 1);
   HIPCHK(c, up(c->flags, fl, sizeof(fl)));
-  HIPCHK(c, up(c->dtp, &c->dt_host, sizeof(double)));
+  HIPCHK(c, up(c->sflags, sfl.data(), sfl.size() * 4));
+  HIPCHK(c, up(c->dtp, c->dt_host.data(), (size_t)nsys * 8));
   c->iteration = 0;
   return 0;
 }
+
+int mdg_set_system(mdg_ctx* c, int n, const int32_t* elem, const double* xyz, const double* vel,
+                   const double pbc[3], const uint8_t* fixed) {
+  if (n <= 0) {
+    c->err = "empty system (AtomsContainer.cxx:108)";
+    return 1;
+  }
+  int32_t cnt = n;
+  return mdg_set_systems(c, 1, &cnt, elem, xyz, vel, pbc, fixed);
+}
+
+int mdg_num_systems(const mdg_ctx* c) { return c->nsys; }
 
 int mdg_energy_grad(mdg_ctx* c, double* e_terms, double* grad) {
   if (c->n == 0) {
@@ -615,14 +695,17 @@ int mdg_energy_grad(mdg_ctx* c, double* e_terms, double* grad) {
     return 1;
   }
   force_call(c);
+  int nt = (int)c->kinds.size();
+  k_sys_energy<<<c->nsys, 256, 0, c->stream>>>(nt, c->A.sys_off, c->n, c->eatom, c->eacc);
   if (check_err(c)) return 1;
-  std::vector<double> e(8);
-  HIPCHK(c, hipMemcpyAsync(e.data(), c->eacc, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> e(8 * (size_t)c->nsys);
+  HIPCHK(c, hipMemcpyAsync(e.data(), c->eacc, e.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   if (grad)
     HIPCHK(c, hipMemcpyAsync(grad, c->A.grad, 3 * (size_t)c->n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (e_terms)
-    for (size_t t = 0; t < c->kinds.size(); t++) e_terms[t] = e[t];
+    for (int s = 0; s < c->nsys; s++)
+      for (int t = 0; t < nt; t++) e_terms[(size_t)s * nt + t] = e[8 * (size_t)s + t];
   return 0;
 }
 
@@ -636,7 +719,7 @@ int mdg_nl_export(mdg_ctx* c, int term, int32_t* counts, int32_t* idx, int64_t c
   int n = c->n;
   // make sure lists reflect the current positions' rebuild state
   nl_build(c, shortlist);
-  k_zero_i<<<1, 64, 0, c->stream>>>(c->flags + (shortlist ? 0 : 1), 1);
+  k_clear_list<<<blocks(c->nsys), 256, 0, c->stream>>>(c->nsys, shortlist ? 0 : 1, c->flags, c->sflags);
   if (check_err(c)) return 1;
   int stride = shortlist ? c->A.cap_s : c->A.cap_l;
   std::vector<int> cnt(n), lst((size_t)n * stride), ty(n), el(n);
@@ -676,14 +759,16 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
   }
   if (nsteps <= 0) return 0;
   Atoms& A = c->A;
-  int n = c->n;
-  const int kChunk = 4096;  // per-step outputs are staged in chunks of this many steps
-  if (c->step_cap < kChunk) {
-    if (dalloc(c, c->stepout, (size_t)kChunk * 8) || dalloc(c, c->stepidx, 1)) return 1;
-    c->step_cap = kChunk;
+  int n = c->n, nsys = c->nsys;
+  // per-step, per-system outputs are staged in chunks of at most 2^20 entries
+  const int chunk = std::max(1, std::min(4096, (1 << 20) / nsys));
+  if (c->step_cap < chunk) {
+    if (dalloc(c, c->stepout, (size_t)chunk * nsys * 4)) return 1;
+    if (!c->stepidx && dalloc(c, c->stepidx, 1)) return 1;
+    c->step_cap = chunk;
   }
   int fixed = fixed_dt > 0 ? 1 : 0;
-  if (fixed) HIPCHK(c, hipMemcpyAsync(c->dtp, &fixed_dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (fixed) k_fill<<<blocks(nsys), 256, 0, c->stream>>>(c->dtp, nsys, fixed_dt);
   int nterms = (int)c->kinds.size();
   if (c->iteration == 0) {  // SimLoop.cxx:305-321
     force_call(c);
@@ -692,16 +777,22 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
   double lim_s = 0.44 * c->rc_short, lim_l = 0.44 * c->rc_long;
   if (!c->have_short) lim_s = 1e300;
   if (!c->have_long) lim_l = 1e300;
-  // One iteration of the loop body; every size and flag it needs lives in
-  // device memory, so the same launches can be captured once and replayed.
+  // One iteration of the loop body for every system of the batch; every size
+  // and flag it needs lives in device memory, so the same launches can be
+  // captured once and replayed.
   auto body = [&]() {
     launch(c, "k_drift", [&] { k_drift<<<blocks(n), 256, 0, c->stream>>>(A, c->dtp); });
-    launch(c, "k_disp_top2", [&] { k_disp_top2<<<256, 256, 0, c->stream>>>(n, A.disp_s, A.disp_l, c->part); });
-    launch(c, "k_disp_decide",
-           [&] { k_disp_decide<<<1, 256, 0, c->stream>>>(256, c->part, lim_s, lim_l, c->flags, c->counters); });
+    launch(c, "k_sys_disp", [&] {
+      k_sys_disp<<<nsys, 256, 0, c->stream>>>(A.sys_off, A.disp_s, A.disp_l, lim_s, lim_l, c->sflags, c->flags,
+                                              c->counters);
+    });
     force_call(c);
-    launch(c, "k_kick", [&] { k_kick<<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->dtp, c->red); });
-    k_step_end<<<1, 64, 0, c->stream>>>(nterms, c->eacc, c->red, c->dtp, fixed, c->stepout, c->stepidx);
+    launch(c, "k_kick", [&] { k_kick<<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->dtp); });
+    launch(c, "k_sys_step_end", [&] {
+      k_sys_step_end<<<nsys, 256, 0, c->stream>>>(A, nterms, c->eatom, c->dtp, fixed, c->stepout, c->stepidx,
+                                                  nsys);
+    });
+    k_step_advance<<<1, 64, 0, c->stream>>>(c->stepidx);
   };
   bool use_graph = !c->profiling && c->use_graphs;
   if (use_graph && !c->graph_exec[fixed]) {
@@ -712,25 +803,27 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
     HIPCHK(c, hipGraphInstantiate(&c->graph_exec[fixed], g, nullptr, nullptr, 0));
     c->graphs.push_back(g);
   }
-  std::vector<double> o((size_t)kChunk * 8);
-  for (int s0 = 0; s0 < nsteps; s0 += kChunk) {
-    int m = std::min(kChunk, nsteps - s0);
+  std::vector<double> o((size_t)chunk * nsys * 4);
+  for (int s0 = 0; s0 < nsteps; s0 += chunk) {
+    int m = std::min(chunk, nsteps - s0);
     k_zero_i<<<1, 64, 0, c->stream>>>(c->stepidx, 1);
     for (int s = 0; s < m; s++) {
       if (use_graph) HIPCHK(c, hipGraphLaunch(c->graph_exec[fixed], c->stream));
       else body();
     }
     if (check_err(c)) return 1;
-    HIPCHK(c, hipMemcpy(o.data(), c->stepout, (size_t)m * 8 * sizeof(double), hipMemcpyDeviceToHost));
-    for (int s = 0; s < m; s++) {
-      if (epot) epot[s0 + s] = o[8 * s + 0];
-      if (ekin) ekin[s0 + s] = o[8 * s + 1];
-      if (dt_used) dt_used[s0 + s] = o[8 * s + 2];
-      if (netforce) netforce[s0 + s] = o[8 * s + 3];
-    }
+    HIPCHK(c, hipMemcpy(o.data(), c->stepout, (size_t)m * nsys * 4 * sizeof(double), hipMemcpyDeviceToHost));
+    for (int s = 0; s < m; s++)
+      for (int q = 0; q < nsys; q++) {
+        const double* v = &o[((size_t)s * nsys + q) * 4];
+        size_t at = (size_t)(s0 + s) * nsys + q;
+        if (epot) epot[at] = v[0];
+        if (ekin) ekin[at] = v[1];
+        if (dt_used) dt_used[at] = v[2];
+        if (netforce) netforce[at] = v[3];
+      }
   }
   c->iteration += nsteps;
-  if (!fixed) HIPCHK(c, hipMemcpy(&c->dt_host, c->dtp, sizeof(double), hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -789,7 +882,9 @@ int mdg_set_positions(mdg_ctx* c, const double* xyz, const int32_t* pbc_count, c
   }
   if (!dr) {
     int fl[2] = {1, 1};
+    std::vector<int> sfl(2 * (size_t)c->nsys, 1);
     HIPCHK(c, hipMemcpy(c->flags, fl, sizeof(fl), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->sflags, sfl.data(), sfl.size() * 4, hipMemcpyHostToDevice));
     return 0;
   }
   // accumulate dr into both lists' displacements, then the skin test
@@ -804,8 +899,8 @@ int mdg_set_positions(mdg_ctx* c, const double* xyz, const int32_t* pbc_count, c
   HIPCHK(c, hipMemcpy(c->A.disp_l, l.data(), 3 * (size_t)n * 8, hipMemcpyHostToDevice));
   double lim_s = c->have_short ? 0.44 * c->rc_short : 1e300;
   double lim_l = c->have_long ? 0.44 * c->rc_long : 1e300;
-  k_disp_top2<<<256, 256, 0, c->stream>>>(n, c->A.disp_s, c->A.disp_l, c->part);
-  k_disp_decide<<<1, 256, 0, c->stream>>>(256, c->part, lim_s, lim_l, c->flags, c->counters);
+  k_sys_disp<<<c->nsys, 256, 0, c->stream>>>(c->A.sys_off, c->A.disp_s, c->A.disp_l, lim_s, lim_l, c->sflags,
+                                             c->flags, c->counters);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }

@@ -67,9 +67,8 @@ __device__ __forceinline__ double dFsw(double x) {
 __global__ void __launch_bounds__(256) k_brenner_bonds(Atoms A, const Tables* __restrict__ T, Box bx,
                                                       const int2* __restrict__ bonds,
                                                       const int* __restrict__ nbonds, double* __restrict__ grad,
-                                                      double* __restrict__ eacc, unsigned* err) {
+                                                      double* __restrict__ eatom, unsigned* err) {
   const BrennerParams& P = T->bren;
-  double e = 0.0;
   int nbt = min(*nbonds, A.bond_cap);
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nbt; t += gridDim.x * blockDim.x) {
     int2 bd = bonds[t];
@@ -88,7 +87,7 @@ __global__ void __launch_bounds__(256) k_brenner_bonds(Atoms A, const Tables* __
     double va = aA * exp(-bA * (r - Re));
     double VR = f * vr, VA = f * va;
     double cr = vr * (-bR) * f + vr * df;  // VR seeded with 1
-    e += VR;
+    double e = VR;
     D3 gi = d3(0, 0, 0), gj = d3(0, 0, 0);
     if (VA != 0.0) {
       double W = -VA;  // Baver seed
@@ -245,15 +244,7 @@ __global__ void __launch_bounds__(256) k_brenner_bonds(Atoms A, const Tables* __
     gj = gj - v;
     atomic_add3(grad, i, gi);
     atomic_add3(grad, j, gj);
-  }
-  __shared__ double red[16];
-  for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0;
-    for (int q = 0; q < (int)(blockDim.x >> 6); q++) s += red[q];
-    if (s != 0.0) unsafeAtomicAdd(eacc, s);
+    if (e != 0.0) unsafeAtomicAdd(eatom + i, e);
   }
 }
 

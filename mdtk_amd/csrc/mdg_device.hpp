@@ -142,13 +142,18 @@ __device__ __forceinline__ double mic(double r, double L) {
   return r;
 }
 
+// Boxes of a batch of independent systems: L[3*s..3*s+2] is system s's box,
+// sys[a] the system of atom a (all zero for a single system).
 struct Box {
-  double L[3];
+  const double* L;
+  const int* sys;
 };
+__device__ __forceinline__ const double* box_of(const Box& bx, int a) { return bx.L + 3 * bx.sys[a]; }
 
 __device__ __forceinline__ D3 depos(const double* __restrict__ x, const double* __restrict__ y,
                                     const double* __restrict__ z, int a, int b, const Box& bx) {
-  return d3(mic(x[a] - x[b], bx.L[0]), mic(y[a] - y[b], bx.L[1]), mic(z[a] - z[b], bx.L[2]));
+  const double* L = box_of(bx, a);
+  return d3(mic(x[a] - x[b], L[0]), mic(y[a] - y[b], L[1]), mic(z[a] - z[b], L[2]));
 }
 
 // A displacement taken in FP64 cm, expressed in the kernel's unit and type:

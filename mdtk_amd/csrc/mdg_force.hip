@@ -19,20 +19,10 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Block sum of one double per thread, one atomic per block into *acc.
-__device__ void block_accumulate(double v, double* acc) {
-  __shared__ double red[16];
-  v = wave_sum(v);
-  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) red[w] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0;
-    int nw = (blockDim.x + 63) >> 6;
-    for (int q = 0; q < nw; q++) s += red[q];
-    if (s != 0.0) unsafeAtomicAdd(acc, s);
-  }
-  __syncthreads();
+// Energy of one bond, pair or quadruplet, credited to its owner atom (the
+// per-system totals are summed over atoms afterwards).
+__device__ __forceinline__ void add_energy(double* eatom, int owner, double e) {
+  if (e != 0.0) unsafeAtomicAdd(eatom + owner, e);
 }
 
 // Energy of a kernel unit expressed in erg.
@@ -530,9 +520,8 @@ __device__ void bo_backward(const Atoms& A, const TablesT<R>* __restrict__ T, co
 template <class R>
 __global__ void __launch_bounds__(256) k_rebo_bonds(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
                                                    const int2* __restrict__ bonds, const int* __restrict__ nbonds,
-                                                   double* __restrict__ grad, double* __restrict__ eacc,
+                                                   double* __restrict__ grad, double* __restrict__ eatom,
                                                    unsigned* err) {
-  double e = 0.0;
   const double gs = T->eunit / T->lunit;
   int nbt = min(*nbonds, A.bond_cap);
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nbt; t += gridDim.x * blockDim.x) {
@@ -575,13 +564,12 @@ __global__ void __launch_bounds__(256) k_rebo_bonds(Atoms A, const TablesT<R>* _
         cr += (-vder * f * bR) + (-vsum * bR) * df;
         eb += bb * double(VA);
       }
-      e += to_erg<R>(eb, T->eunit);
+      add_energy(eatom, p.i, to_erg<R>(eb, T->eunit));
       V3<R> v = p.rv * (cr / r);
       scatter3<R>(grad, p.i, v, gs);
       scatter3<R>(grad, p.j, -v, gs);
     }
   }
-  block_accumulate(e, eacc);
 }
 
 // ------------------------------------------------------------ C_ij paths
@@ -652,7 +640,7 @@ constexpr int kPathCap = 320;  // path records per atom held in LDS
 // the pair (S_r = 0) or queue it for the b* kernel.
 template <class R>
 __global__ void __launch_bounds__(256) k_lj_wave(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
-                                                double* __restrict__ grad, double* __restrict__ eacc,
+                                                double* __restrict__ grad, double* __restrict__ eatom,
                                                 int4* __restrict__ queue, int* __restrict__ qcount, int qcap,
                                                 long long* __restrict__ npairs, unsigned* err) {
   __shared__ int s_t[4][kPathCap];
@@ -664,11 +652,12 @@ __global__ void __launch_bounds__(256) k_lj_wave(Atoms A, const TablesT<R>* __re
   BondArr<R> B = barr<R>(A);
   const double gs = T->eunit / T->lunit;
   int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  double e = 0.0;
   long long np = 0;
   for (int i = blockIdx.x * 4 + wv; i < A.n; i += gridDim.x * 4) {
     int ti = A.type[i];
     if (ti > T_H) continue;  // wave-uniform
+    const double* Lb = box_of(bx, i);
+    double e = 0.0;
     if (lane == 0) s_cnt[wv] = 0;
     int nb1 = A.bcnt[i];
     int kk = -1, c2 = 0;
@@ -727,7 +716,7 @@ __global__ void __launch_bounds__(256) k_lj_wave(Atoms A, const TablesT<R>* __re
       if (j <= i) continue;
       int tj = A.type[j];
       if (tj > T_H) continue;
-      D3 rd = d3(mic(xi - A.x[j], bx.L[0]), mic(yi - A.y[j], bx.L[1]), mic(zi - A.z[j], bx.L[2]));
+      D3 rd = d3(mic(xi - A.x[j], Lb[0]), mic(yi - A.y[j], Lb[1]), mic(zi - A.z[j], Lb[2]));
       double d2 = rd.x * rd.x + rd.y * rd.y + rd.z * rd.z;
       if (d2 > double(L.R2sq[ti][tj]) * (T->lunit * T->lunit)) continue;
       np++;
@@ -780,14 +769,17 @@ __global__ void __launch_bounds__(256) k_lj_wave(Atoms A, const TablesT<R>* __re
       e += to_erg<R>(double(R(1) * C * LJ), T->eunit);
     }
     double gx = wave_sum(double(gi.x)), gy = wave_sum(double(gi.y)), gz = wave_sum(double(gi.z));
-    if (lane == 0 && (gx != 0.0 || gy != 0.0 || gz != 0.0)) {
-      if constexpr (std::is_same<R, double>::value) atomic_add3(grad, i, d3(gx, gy, gz));
-      else atomic_add3(grad, i, d3(gx * gs, gy * gs, gz * gs));
+    e = wave_sum(e);
+    if (lane == 0) {
+      if (gx != 0.0 || gy != 0.0 || gz != 0.0) {
+        if constexpr (std::is_same<R, double>::value) atomic_add3(grad, i, d3(gx, gy, gz));
+        else atomic_add3(grad, i, d3(gx * gs, gy * gs, gz * gs));
+      }
+      add_energy(eatom, i, e);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
   }
-  block_accumulate(e, eacc);
   if (np) atomicAdd((unsigned long long*)npairs, (unsigned long long)np);
 }
 
@@ -795,9 +787,8 @@ __global__ void __launch_bounds__(256) k_lj_wave(Atoms A, const TablesT<R>* __re
 template <class R>
 __global__ void __launch_bounds__(256) k_bstar(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
                                               const int4* __restrict__ queue, const int* __restrict__ qcount,
-                                              int qcap, double* __restrict__ grad, double* __restrict__ eacc,
+                                              int qcap, double* __restrict__ grad, double* __restrict__ eatom,
                                               unsigned* err) {
-  double e = 0.0;
   const double gs = T->eunit / T->lunit;
   int nq = min(*qcount, qcap);
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nq; t += gridDim.x * blockDim.x) {
@@ -871,9 +862,8 @@ __global__ void __launch_bounds__(256) k_bstar(Atoms A, const TablesT<R>* __rest
     scatter3<R>(grad, i, v, gs);
     scatter3<R>(grad, j, -v, gs);
     if (path.hops > 0 && SrSb * LJ != R(0)) cij_backward<R>(A, T, bx, i, j, path, SrSb * LJ, grad, gs);
-    e += to_erg<R>(double(SrSb * C * LJ), T->eunit);
+    add_energy(eatom, i, to_erg<R>(double(SrSb * C * LJ), T->eunit));
   }
-  block_accumulate(e, eacc);
 }
 
 // ------------------------------------------------------------ torsion
@@ -881,9 +871,8 @@ __global__ void __launch_bounds__(256) k_bstar(Atoms A, const TablesT<R>* __rest
 template <class R>
 __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
                                               const int2* __restrict__ bonds, const int* __restrict__ nbonds,
-                                              int compat, double* __restrict__ grad, double* __restrict__ eacc,
+                                              int compat, double* __restrict__ grad, double* __restrict__ eatom,
                                               long long* __restrict__ nquads) {
-  double e = 0.0;
   long long nq = 0;
   const double gs = T->eunit / T->lunit;
   int nbt = min(*nbonds, A.bond_cap);
@@ -962,12 +951,11 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
           eb += double(wki * wij * wjl * vt);
         }
       }
-      e += to_erg<R>(eb, T->eunit);
+      add_energy(eatom, i, to_erg<R>(eb, T->eunit));
       scatter3<R>(grad, i, gi, gs);
       scatter3<R>(grad, j, gj, gs);
     }
   }
-  block_accumulate(e, eacc);
   if (nq) atomicAdd((unsigned long long*)nquads, (unsigned long long)nq);
 }
 

@@ -1,4 +1,6 @@
-// Neighbour lists, skin test, ZBL and the velocity-Verlet kernels (FP64).
+// Neighbour lists, skin test, ZBL, the velocity-Verlet kernels and the
+// per-system reductions (FP64).  A context holds one system or a batch of
+// independent systems; A.sys[i] is atom i's system, A.sys_off the atom ranges.
 // Kernels are compiled without FMA contraction so that a*b+c rounds twice, as
 // in the reference's x86-64 build.
 #include "mdg_kernels.hpp"
@@ -14,29 +16,38 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
-// Block sum of one double per thread, one atomic per block into *acc.
-__device__ void block_accumulate(double v, double* acc) {
+// Block (256 threads) sum; the result is valid in thread 0.
+__device__ double block_sum(double v) {
   __shared__ double red[16];
   v = wave_sum(v);
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();
   if (lane == 0) red[w] = v;
   __syncthreads();
+  double s = 0;
   if (threadIdx.x == 0) {
-    double s = 0;
     int nw = (blockDim.x + 63) >> 6;
     for (int q = 0; q < nw; q++) s += red[q];
-    if (s != 0.0) unsafeAtomicAdd(acc, s);
   }
-  __syncthreads();
+  return s;
 }
 
 // ------------------------------------------------------------ cell lists
+// Every system has its own grid; cell ids are offset by cell_off[system].
+__global__ void k_cell_clear(const int* __restrict__ flag, int ncell, int* __restrict__ cnt) {
+  if (!*flag) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ncell; i += gridDim.x * blockDim.x) cnt[i] = 0;
+}
+
 __global__ void k_cell_assign(const int* __restrict__ flag, int n, const double* __restrict__ x,
-                              const double* __restrict__ y, const double* __restrict__ z, CellGrid g,
-                              int* __restrict__ cell_of, int* __restrict__ cell_cnt) {
+                              const double* __restrict__ y, const double* __restrict__ z,
+                              const CellGrid* __restrict__ grids, const int* __restrict__ cell_off,
+                              const int* __restrict__ sys, int* __restrict__ cell_of, int* __restrict__ cell_cnt) {
   if (!*flag) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  int s = sys[i];
+  const CellGrid& g = grids[s];
   int c[3];
   double p[3] = {x[i], y[i], z[i]};
   for (int d = 0; d < 3; d++) {
@@ -45,7 +56,7 @@ __global__ void k_cell_assign(const int* __restrict__ flag, int n, const double*
     if (q >= g.n[d]) q = g.n[d] - 1;
     c[d] = q;
   }
-  int id = (c[0] * g.n[1] + c[1]) * g.n[2] + c[2];
+  int id = cell_off[s] + (c[0] * g.n[1] + c[1]) * g.n[2] + c[2];
   cell_of[i] = id;
   atomicAdd(&cell_cnt[id], 1);
 }
@@ -91,19 +102,27 @@ __global__ void k_cell_fill(const int* __restrict__ flag, int n, const int* __re
 // with d^2 = dx*dx+dy*dy+dz*dz of the minimum-image displacement.  Candidates
 // from the stencil cells are compacted with a ballot/popcount prefix into
 // LDS, then sorted ascending (the reference's list order) with a bitonic sort.
+// Only atoms whose system asked for a rebuild (sflags[2*s+list]) are redone.
 template <int CAP>
-__global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, int n, const double* __restrict__ x,
+__global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, const int* __restrict__ sflags,
+                                                  int list_id, int n, const double* __restrict__ x,
                                                   const double* __restrict__ y, const double* __restrict__ z,
-                                                  const unsigned char* __restrict__ handled, Box bx, CellGrid g,
-                                                  double range2, const int* __restrict__ cell_of,
-                                                  const int* __restrict__ start, const int* __restrict__ cell_atoms,
-                                                  int* __restrict__ cnt, int* __restrict__ list,
-                                                  double* __restrict__ disp, unsigned* err) {
+                                                  const unsigned char* __restrict__ handled, Box bx,
+                                                  const CellGrid* __restrict__ grids,
+                                                  const int* __restrict__ cell_off, double range2,
+                                                  const int* __restrict__ cell_of, const int* __restrict__ start,
+                                                  const int* __restrict__ cell_atoms, int* __restrict__ cnt,
+                                                  int* __restrict__ list, double* __restrict__ disp, unsigned* err) {
   if (!*flag) return;
   __shared__ int buf[4][CAP];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int i = blockIdx.x * 4 + w;
   if (i >= n) return;
+  int sy = bx.sys[i];
+  if (!sflags[2 * sy + list_id]) return;
+  const CellGrid& g = grids[sy];
+  const double* L = box_of(bx, i);
+  int base_cell = cell_off[sy];
   int* my = buf[w];
   if (lane == 0) {
     disp[3 * i] = 0.0;
@@ -113,7 +132,7 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
   int m = 0;
   if (handled[i]) {
     double xi = x[i], yi = y[i], zi = z[i];
-    int ci = cell_of[i];
+    int ci = cell_of[i] - base_cell;
     int c2 = ci % g.n[2], c1 = (ci / g.n[2]) % g.n[1], c0 = ci / (g.n[1] * g.n[2]);
     int lo[3], hi[3], cc[3] = {c0, c1, c2};
     for (int d = 0; d < 3; d++) {
@@ -135,7 +154,7 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
           else if (q1 < 0 || q1 >= g.n[1]) continue;
           if (g.periodic[2]) q2 = (q2 + g.n[2]) % g.n[2];
           else if (q2 < 0 || q2 >= g.n[2]) continue;
-          int cid = (q0 * g.n[1] + q1) * g.n[2] + q2;
+          int cid = base_cell + (q0 * g.n[1] + q1) * g.n[2] + q2;
           int s = start[cid], e = start[cid + 1];
           for (int base = s; base < e; base += 64) {
             int p = base + lane;
@@ -144,9 +163,9 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
             if (p < e) {
               j = cell_atoms[p];
               if (j != i && handled[j]) {
-                double dx = mic(xi - x[j], bx.L[0]);
-                double dy = mic(yi - y[j], bx.L[1]);
-                double dz = mic(zi - z[j], bx.L[2]);
+                double dx = mic(xi - x[j], L[0]);
+                double dy = mic(yi - y[j], L[1]);
+                double dz = mic(zi - z[j], L[2]);
                 double d2 = dx * dx + dy * dy + dz * dz;
                 ok = d2 < range2;
               }
@@ -187,20 +206,22 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
     }
-  int stride = CAP;
-  for (int q = lane; q < m; q += 64) list[(size_t)i * stride + q] = my[q];
+  for (int q = lane; q < m; q += 64) list[(size_t)i * CAP + q] = my[q];
   if (lane == 0) cnt[i] = m;
 }
 
-template __global__ void k_nl_build<64>(const int*, int, const double*, const double*, const double*,
-                                        const unsigned char*, Box, CellGrid, double, const int*, const int*,
-                                        const int*, int*, int*, double*, unsigned*);
-template __global__ void k_nl_build<512>(const int*, int, const double*, const double*, const double*,
-                                         const unsigned char*, Box, CellGrid, double, const int*, const int*,
-                                         const int*, int*, int*, double*, unsigned*);
+template __global__ void k_nl_build<64>(const int*, const int*, int, int, const double*, const double*,
+                                        const double*, const unsigned char*, Box, const CellGrid*, const int*,
+                                        double, const int*, const int*, const int*, int*, int*, double*, unsigned*);
+template __global__ void k_nl_build<512>(const int*, const int*, int, int, const double*, const double*,
+                                         const double*, const unsigned char*, Box, const CellGrid*, const int*,
+                                         double, const int*, const int*, const int*, int*, int*, double*,
+                                         unsigned*);
 
-// Top-2 accumulated displacement per neighbour list, NeighbourList.cxx:108-133.
-// Stage 1: per-block top-2; stage 2: one block merges and sets the flags.
+// ------------------------------------------------------------ skin test
+// Top-2 accumulated displacement per system and list (NeighbourList.cxx:
+// 108-133): rebuild iff d1 + d2 > 0.44 Rc.  The two largest values of a set
+// are exact whatever the merge order.
 __device__ __forceinline__ void top2_insert(double v, double& d1, double& d2) {
   if (v >= d1) {
     d2 = d1;
@@ -210,8 +231,6 @@ __device__ __forceinline__ void top2_insert(double v, double& d1, double& d2) {
   }
 }
 
-// Merge of two top-2 sets (the two largest values of the union are exact
-// whatever the merge order).
 __device__ __forceinline__ void top2_merge(double& d1, double& d2, double e1, double e2) {
   top2_insert(e1, d1, d2);
   top2_insert(e2, d1, d2);
@@ -224,11 +243,15 @@ __device__ __forceinline__ void top2_wave(double& d1, double& d2) {
   }
 }
 
-__global__ void k_disp_top2(int n, const double* __restrict__ ds, const double* __restrict__ dl,
-                            double* __restrict__ part) {
+// One block per system; sflags[2*s+{0,1}] and the any-system flags[0..1].
+__global__ void k_sys_disp(const int* __restrict__ sys_off, const double* __restrict__ ds,
+                           const double* __restrict__ dl, double lim_s, double lim_l, int* __restrict__ sflags,
+                           int* __restrict__ flags, long long* __restrict__ rebuilds) {
   __shared__ double sh[4][4];
+  int s = blockIdx.x;
+  int b = sys_off[s], e = sys_off[s + 1];
   double a1 = 0, a2 = 0, b1 = 0, b2 = 0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  for (int i = b + threadIdx.x; i < e; i += blockDim.x) {
     double m = sqrt(ds[3 * i] * ds[3 * i] + ds[3 * i + 1] * ds[3 * i + 1] + ds[3 * i + 2] * ds[3 * i + 2]);
     top2_insert(m, a1, a2);
     double q = sqrt(dl[3 * i] * dl[3 * i] + dl[3 * i + 1] * dl[3 * i + 1] + dl[3 * i + 2] * dl[3 * i + 2]);
@@ -247,46 +270,25 @@ __global__ void k_disp_top2(int n, const double* __restrict__ ds, const double* 
       top2_merge(x1, x2, sh[q][0], sh[q][1]);
       top2_merge(y1, y2, sh[q][2], sh[q][3]);
     }
-    part[4 * blockIdx.x + 0] = x1;
-    part[4 * blockIdx.x + 1] = x2;
-    part[4 * blockIdx.x + 2] = y1;
-    part[4 * blockIdx.x + 3] = y2;
-  }
-}
-
-// One 256-thread block merges the per-block top-2 values and sets the flags
-// (rebuild iff d1 + d2 > 0.44 Rc).
-__global__ void k_disp_decide(int nblk, const double* __restrict__ part, double lim_s, double lim_l,
-                              int* __restrict__ flags, long long* __restrict__ rebuilds) {
-  __shared__ double sh[4][4];
-  double x1 = 0, x2 = 0, y1 = 0, y2 = 0;
-  for (int q = threadIdx.x; q < nblk; q += blockDim.x) {
-    top2_merge(x1, x2, part[4 * q], part[4 * q + 1]);
-    top2_merge(y1, y2, part[4 * q + 2], part[4 * q + 3]);
-  }
-  top2_wave(x1, x2);
-  top2_wave(y1, y2);
-  int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    sh[w][0] = x1; sh[w][1] = x2; sh[w][2] = y1; sh[w][3] = y2;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double a1 = 0, a2 = 0, b1 = 0, b2 = 0;
-    for (int q = 0; q < (int)(blockDim.x >> 6); q++) {
-      top2_merge(a1, a2, sh[q][0], sh[q][1]);
-      top2_merge(b1, b2, sh[q][2], sh[q][3]);
+    int fs = (x1 + x2 > lim_s) ? 1 : 0;
+    int fl = (y1 + y2 > lim_l) ? 1 : 0;
+    sflags[2 * s] |= fs;
+    sflags[2 * s + 1] |= fl;
+    if (fs) {
+      atomicOr(&flags[0], 1);
+      atomicAdd((unsigned long long*)&rebuilds[0], 1ull);
     }
-    flags[0] |= (a1 + a2 > lim_s) ? 1 : 0;
-    flags[1] |= (b1 + b2 > lim_l) ? 1 : 0;
-    if (flags[0]) rebuilds[0] += 1;
-    if (flags[1]) rebuilds[1] += 1;
+    if (fl) {
+      atomicOr(&flags[1], 1);
+      atomicAdd((unsigned long long*)&rebuilds[1], 1ull);
+    }
   }
 }
 
-__global__ void k_cell_clear(const int* __restrict__ flag, int ncell, int* __restrict__ cnt) {
-  if (!*flag) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ncell; i += gridDim.x * blockDim.x) cnt[i] = 0;
+__global__ void k_clear_flags(int nsys, int* __restrict__ flags, int* __restrict__ sflags) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 2) flags[i] = 0;
+  if (i < 2 * nsys) sflags[i] = 0;
 }
 
 // Bonds (i<j) in the order of i's list, appended to one array.
@@ -315,42 +317,41 @@ __device__ __forceinline__ bool zbl_target(int elem) {
 }
 __device__ __forceinline__ bool zbl_ion(int elem) { return elem == 40 || elem == 131; }
 
-__global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* __restrict__ grad, double* __restrict__ eacc) {
+__global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* __restrict__ grad, double* __restrict__ eatom) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.n || !zbl_ion(A.elem[i])) return;
   double e = 0.0;
-  if (i < A.n && zbl_ion(A.elem[i])) {
-    int cnt = A.nll_cnt[i];
-    const int* list = A.nll + (size_t)i * A.cap_l;
-    for (int q = 0; q < cnt; q++) {
-      int j = list[q];
-      bool jion = zbl_ion(A.elem[j]);
-      if (!(jion || zbl_target(A.elem[j]))) continue;
-      if (jion && j < i) continue;  // ion-ion pairs once
-      int a = i < j ? i : j, b = i < j ? j : i;
-      D3 rv = depos(A.x, A.y, A.z, a, b, bx);
-      double d2 = norm2(rv);
-      if (d2 > R1 * R1) continue;
-      double r = sqrt(d2);
-      double f, df;
-      cutoff(r, R0, R1, f, df);
-      if (r > R1) continue;
-      double ZA = A.Z[a], ZB = A.Z[b];
-      double AS = 8.8534e-1 * 0.53e-8 / (pow(ZA / kECharge, 0.23) + pow(ZB / kECharge, 0.23));
-      double Y = r / AS;
-      double e1 = exp(-3.1998 * Y), e2 = exp(-0.94229 * Y), e3 = exp(-0.4029 * Y), e4 = exp(-0.20162 * Y);
-      double sum = 0.18175 * e1 + 0.50986 * e2 + 0.28022 * e3 + 0.02817 * e4;
-      double val = ZA * ZB / r * sum;
-      double der = -ZA * ZB / (r * r) * sum -
-                   ZA * ZB / (r * AS) *
-                       (0.18175 * 3.1998 * e1 + 0.50986 * 0.94229 * e2 + 0.28022 * 0.4029 * e3 + 0.02817 * 0.20162 * e4);
-      double cr = der * f + df * val;
-      D3 v = rv * (cr / r);
-      atomic_add3(grad, a, v);
-      atomic_add3(grad, b, -v);
-      e += val * f;
-    }
+  int cnt = A.nll_cnt[i];
+  const int* list = A.nll + (size_t)i * A.cap_l;
+  for (int q = 0; q < cnt; q++) {
+    int j = list[q];
+    bool jion = zbl_ion(A.elem[j]);
+    if (!(jion || zbl_target(A.elem[j]))) continue;
+    if (jion && j < i) continue;  // ion-ion pairs once
+    int a = i < j ? i : j, b = i < j ? j : i;
+    D3 rv = depos(A.x, A.y, A.z, a, b, bx);
+    double d2 = norm2(rv);
+    if (d2 > R1 * R1) continue;
+    double r = sqrt(d2);
+    double f, df;
+    cutoff(r, R0, R1, f, df);
+    if (r > R1) continue;
+    double ZA = A.Z[a], ZB = A.Z[b];
+    double AS = 8.8534e-1 * 0.53e-8 / (pow(ZA / kECharge, 0.23) + pow(ZB / kECharge, 0.23));
+    double Y = r / AS;
+    double e1 = exp(-3.1998 * Y), e2 = exp(-0.94229 * Y), e3 = exp(-0.4029 * Y), e4 = exp(-0.20162 * Y);
+    double sum = 0.18175 * e1 + 0.50986 * e2 + 0.28022 * e3 + 0.02817 * e4;
+    double val = ZA * ZB / r * sum;
+    double der = -ZA * ZB / (r * r) * sum -
+                 ZA * ZB / (r * AS) *
+                     (0.18175 * 3.1998 * e1 + 0.50986 * 0.94229 * e2 + 0.28022 * 0.4029 * e3 + 0.02817 * 0.20162 * e4);
+    double cr = der * f + df * val;
+    D3 v = rv * (cr / r);
+    atomic_add3(grad, a, v);
+    atomic_add3(grad, b, -v);
+    e += val * f;
   }
-  block_accumulate(e, eacc);
+  if (e != 0.0) unsafeAtomicAdd(eatom + i, e);
 }
 
 // ------------------------------------------------------------ Verlet
@@ -359,7 +360,7 @@ __global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* __restrict
 __global__ void k_drift(Atoms A, const double* __restrict__ dtp) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n || A.fixed[i]) return;
-  double dt = *dtp;
+  double dt = dtp[A.sys[i]];
   double dx = A.vx[i] * dt + A.ax[i] * dt * dt / 2.0;
   double dy = A.vy[i] * dt + A.ay[i] * dt * dt / 2.0;
   double dz = A.vz[i] * dt + A.az[i] * dt * dt / 2.0;
@@ -386,51 +387,29 @@ __device__ __forceinline__ void wrap(double& c, double L, int& cnt) {
   }
 }
 
-// eq.2-4 (SimLoop.cxx:339-399), v_max over atoms with neighbours, the PBC
-// wrap (:401, Atom.cxx:93-125), kinetic energy and net force partials.
-__global__ void k_kick(Atoms A, Box bx, const double* __restrict__ dtp, double* __restrict__ red) {
+// eq.2-4 (SimLoop.cxx:339-399) and the PBC wrap (:401, Atom.cxx:93-125).
+__global__ void k_kick(Atoms A, Box bx, const double* __restrict__ dtp) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double ek = 0.0, fx = 0.0, fy = 0.0, fz = 0.0, vm = 0.0;
-  if (i < A.n) {
-    double gx = A.grad[3 * i], gy = A.grad[3 * i + 1], gz = A.grad[3 * i + 2];
-    fx = -gx;
-    fy = -gy;
-    fz = -gz;
-    if (!A.fixed[i]) {
-      double dt = *dtp;
-      double M = A.mass[i];
-      double hx = A.vx[i] + A.ax[i] * dt / 2.0;
-      double hy = A.vy[i] + A.ay[i] * dt / 2.0;
-      double hz = A.vz[i] + A.az[i] * dt / 2.0;
-      double ax = fx / M, ay = fy / M, az = fz / M;
-      A.ax[i] = ax;
-      A.ay[i] = ay;
-      A.az[i] = az;
-      double vx = hx + ax * dt / 2.0, vy = hy + ay * dt / 2.0, vz = hz + az * dt / 2.0;
-      A.vx[i] = vx;
-      A.vy[i] = vy;
-      A.vz[i] = vz;
-      double v = sqrt(vx * vx + vy * vy + vz * vz);
-      if (A.nls_cnt[i] > 0 || A.nll_cnt[i] > 0) vm = v;
-    }
-    double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i];
-    double vv = sqrt(vx * vx + vy * vy + vz * vz);
-    ek = A.mass[i] * (vv * vv) / 2.0;
-    wrap(A.x[i], bx.L[0], A.pcx[i]);
-    wrap(A.y[i], bx.L[1], A.pcy[i]);
-    wrap(A.z[i], bx.L[2], A.pcz[i]);
+  if (i >= A.n) return;
+  if (!A.fixed[i]) {
+    double dt = dtp[A.sys[i]];
+    double M = A.mass[i];
+    double fx = -A.grad[3 * i], fy = -A.grad[3 * i + 1], fz = -A.grad[3 * i + 2];
+    double hx = A.vx[i] + A.ax[i] * dt / 2.0;
+    double hy = A.vy[i] + A.ay[i] * dt / 2.0;
+    double hz = A.vz[i] + A.az[i] * dt / 2.0;
+    double ax = fx / M, ay = fy / M, az = fz / M;
+    A.ax[i] = ax;
+    A.ay[i] = ay;
+    A.az[i] = az;
+    A.vx[i] = hx + ax * dt / 2.0;
+    A.vy[i] = hy + ay * dt / 2.0;
+    A.vz[i] = hz + az * dt / 2.0;
   }
-  block_accumulate(ek, red + 0);
-  block_accumulate(fx, red + 1);
-  block_accumulate(fy, red + 2);
-  block_accumulate(fz, red + 3);
-  // v_max: max of non-negative doubles via integer max on the bit pattern
-  __shared__ unsigned long long vmx;
-  if (threadIdx.x == 0) vmx = 0ull;
-  __syncthreads();
-  atomicMax(&vmx, (unsigned long long)__double_as_longlong(vm));
-  __syncthreads();
-  if (threadIdx.x == 0) atomicMax((unsigned long long*)(red + 4), vmx);
+  const double* L = box_of(bx, i);
+  wrap(A.x[i], L[0], A.pcx[i]);
+  wrap(A.y[i], L[1], A.pcy[i]);
+  wrap(A.z[i], L[2], A.pcz[i]);
 }
 
 // First force evaluation (iteration 0, SimLoop.cxx:305-321): a = -grad/M.
@@ -443,30 +422,72 @@ __global__ void k_init_acc(Atoms A) {
   A.az[i] = -A.grad[3 * i + 2] / M;
 }
 
-// Per-step bookkeeping: energies out, dt rule (SimLoop.cxx:409-416), reset.
-__global__ void k_step_end(int nterms, double* __restrict__ eacc, double* __restrict__ red, double* __restrict__ dtp,
-                           int fixed_dt, double* __restrict__ out, int* __restrict__ stepidx) {
-  if (threadIdx.x != 0) return;
-  int step = *stepidx;
-  *stepidx = step + 1;
-  double* o = out + (size_t)step * 8;
-  double ep = 0;
-  for (int t = 0; t < nterms; t++) ep += eacc[t];
-  o[0] = ep;
-  o[1] = red[0];
-  o[2] = *dtp;
-  o[3] = sqrt(red[1] * red[1] + red[2] * red[2] + red[3] * red[3]);
-  if (!fixed_dt) {
-    double vmax = red[4];
-    const double dt_max = 5e-16, dt_min = 1e-20;
-    double dt;
-    if (vmax != 0.0) dt = 0.05 * (1.0 * kAo) / vmax;
-    else dt = dt_max;
-    if (dt > dt_max) dt = dt_max;
-    if (dt < dt_min) dt = dt_min;
-    *dtp = dt;
+// Per-system energies of each chain member: eacc[8*s+t] = sum of the
+// per-atom energies eatom[t*n+i] over the system's atoms.
+__global__ void k_sys_energy(int nterms, const int* __restrict__ sys_off, int n, const double* __restrict__ eatom,
+                             double* __restrict__ eacc) {
+  int s = blockIdx.x;
+  int b = sys_off[s], e = sys_off[s + 1];
+  for (int t = 0; t < nterms; t++) {
+    double v = 0;
+    for (int i = b + threadIdx.x; i < e; i += blockDim.x) v += eatom[(size_t)t * n + i];
+    v = block_sum(v);
+    if (threadIdx.x == 0) eacc[8 * s + t] = v;
   }
-  for (int q = 0; q < 8; q++) red[q] = 0.0;
+}
+
+// Per-system end of an iteration: E_pot (t+dt), E_kin after the kick
+// (SimLoop.cxx:500-514), |sum F| (:418-430), the dt rule over atoms with
+// neighbours (:394-398, :409-416), one block per system.
+// out[(step*nsys + s)*4 + {0,1,2,3}] = epot, ekin, dt used, |sum F|.
+__global__ void k_sys_step_end(Atoms A, int nterms, const double* __restrict__ eatom, double* __restrict__ dtp,
+                               int fixed_dt, double* __restrict__ out, const int* __restrict__ stepidx, int nsys) {
+  int s = blockIdx.x;
+  int b = A.sys_off[s], e = A.sys_off[s + 1];
+  int n = A.n;
+  double ep = 0, ek = 0, fx = 0, fy = 0, fz = 0, vm = 0;
+  for (int i = b + threadIdx.x; i < e; i += blockDim.x) {
+    for (int t = 0; t < nterms; t++) ep += eatom[(size_t)t * n + i];
+    double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i];
+    double vv = sqrt(vx * vx + vy * vy + vz * vz);
+    ek += A.mass[i] * (vv * vv) / 2.0;
+    fx -= A.grad[3 * i];
+    fy -= A.grad[3 * i + 1];
+    fz -= A.grad[3 * i + 2];
+    if (!A.fixed[i] && (A.nls_cnt[i] > 0 || A.nll_cnt[i] > 0) && vv > vm) vm = vv;
+  }
+  ep = block_sum(ep);
+  ek = block_sum(ek);
+  fx = block_sum(fx);
+  fy = block_sum(fy);
+  fz = block_sum(fz);
+  for (int o = 32; o > 0; o >>= 1) vm = fmax(vm, __shfl_xor(vm, o, 64));
+  __shared__ double vms[16];
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) vms[threadIdx.x >> 6] = vm;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) vm = fmax(vm, vms[q]);
+    int step = *stepidx;
+    double* o = out + ((size_t)step * nsys + s) * 4;
+    o[0] = ep;
+    o[1] = ek;
+    o[2] = dtp[s];
+    o[3] = sqrt(fx * fx + fy * fy + fz * fz);
+    if (!fixed_dt) {
+      const double dt_max = 5e-16, dt_min = 1e-20;
+      double dt;
+      if (vm != 0.0) dt = 0.05 * (1.0 * kAo) / vm;
+      else dt = dt_max;
+      if (dt > dt_max) dt = dt_max;
+      if (dt < dt_min) dt = dt_min;
+      dtp[s] = dt;
+    }
+  }
+}
+
+__global__ void k_step_advance(int* __restrict__ stepidx) {
+  if (threadIdx.x == 0) *stepidx += 1;
 }
 
 __global__ void k_zero(double* __restrict__ p, size_t n) {

@@ -5,8 +5,13 @@
 namespace mdg {
 
 // Device view of the atoms and of the per-step derived data (SoA in HBM).
+// A batch of independent systems is one array of atoms: system s owns atoms
+// [sys_off[s], sys_off[s+1]) and sys[i] names atom i's system.
 struct Atoms {
   int n;
+  int nsys;
+  int* sys;
+  int* sys_off;
   double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az;
   double* grad;   // n*3 interleaved, dE/dx
   double *mass, *Z;
@@ -32,41 +37,47 @@ struct CellGrid {
 };
 
 __global__ void k_cell_clear(const int* flag, int ncell, int* cnt);
-__global__ void k_cell_assign(const int* flag, int n, const double* x, const double* y, const double* z, CellGrid g,
-                              int* cell_of, int* cell_cnt);
+__global__ void k_cell_assign(const int* flag, int n, const double* x, const double* y, const double* z,
+                              const CellGrid* grids, const int* cell_off, const int* sys, int* cell_of,
+                              int* cell_cnt);
 __global__ void k_cell_scan(const int* flag, int ncell, int* cnt, int* start);
 __global__ void k_cell_fill(const int* flag, int n, const int* cell_of, const int* start, int* cursor,
                             int* cell_atoms);
 template <int CAP>
-__global__ void k_nl_build(const int* flag, int n, const double* x, const double* y, const double* z,
-                           const unsigned char* handled, Box bx, CellGrid g, double range2, const int* cell_of,
-                           const int* start, const int* cell_atoms, int* cnt, int* list, double* disp,
-                           unsigned* err);
-__global__ void k_disp_top2(int n, const double* ds, const double* dl, double* part);
-__global__ void k_disp_decide(int nblk, const double* part, double lim_s, double lim_l, int* flags,
-                              long long* rebuilds);
+__global__ void k_nl_build(const int* flag, const int* sflags, int list_id, int n, const double* x, const double* y,
+                           const double* z, const unsigned char* handled, Box bx, const CellGrid* grids,
+                           const int* cell_off, double range2, const int* cell_of, const int* start,
+                           const int* cell_atoms, int* cnt, int* list, double* disp, unsigned* err);
+__global__ void k_sys_disp(const int* sys_off, const double* ds, const double* dl, double lim_s, double lim_l,
+                           int* sflags, int* flags, long long* rebuilds);
+__global__ void k_clear_flags(int nsys, int* flags, int* sflags);
 __global__ void k_bond_enum(Atoms A, int2* out, int* count, int cap, unsigned* err);
+// Energies go to eatom (per-atom, per chain member), owned by the first atom
+// of the bond, pair or quadruplet.
 template <class R>
 __global__ void k_coord(Atoms A, const TablesT<R>* T, Box bx, unsigned* err);
 template <class R>
 __global__ void k_rebo_bonds(Atoms A, const TablesT<R>* T, Box bx, const int2* bonds, const int* nbonds,
-                             double* grad, double* eacc, unsigned* err);
+                             double* grad, double* eatom, unsigned* err);
 template <class R>
-__global__ void k_lj_wave(Atoms A, const TablesT<R>* T, Box bx, double* grad, double* eacc, int4* queue,
+__global__ void k_lj_wave(Atoms A, const TablesT<R>* T, Box bx, double* grad, double* eatom, int4* queue,
                           int* qcount, int qcap, long long* npairs, unsigned* err);
 template <class R>
 __global__ void k_bstar(Atoms A, const TablesT<R>* T, Box bx, const int4* queue, const int* qcount, int qcap,
-                        double* grad, double* eacc, unsigned* err);
+                        double* grad, double* eatom, unsigned* err);
 template <class R>
 __global__ void k_etors(Atoms A, const TablesT<R>* T, Box bx, const int2* bonds, const int* nbonds, int compat,
-                        double* grad, double* eacc, long long* nquads);
-__global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* grad, double* eacc);
+                        double* grad, double* eatom, long long* nquads);
+__global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* grad, double* eatom);
 __global__ void k_brenner_bonds(Atoms A, const Tables* T, Box bx, const int2* bonds, const int* nbonds,
-                                double* grad, double* eacc, unsigned* err);
+                                double* grad, double* eatom, unsigned* err);
 __global__ void k_drift(Atoms A, const double* dtp);
-__global__ void k_kick(Atoms A, Box bx, const double* dtp, double* red);
+__global__ void k_kick(Atoms A, Box bx, const double* dtp);
 __global__ void k_init_acc(Atoms A);
-__global__ void k_step_end(int nterms, double* eacc, double* red, double* dtp, int fixed_dt, double* out, int* stepidx);
+__global__ void k_sys_energy(int nterms, const int* sys_off, int n, const double* eatom, double* eacc);
+__global__ void k_sys_step_end(Atoms A, int nterms, const double* eatom, double* dtp, int fixed_dt, double* out,
+                               const int* stepidx, int nsys);
+__global__ void k_step_advance(int* stepidx);
 __global__ void k_zero(double* p, size_t n);
 __global__ void k_zero_i(int* p, int n);
 

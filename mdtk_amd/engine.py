@@ -1,10 +1,11 @@
 """ctypes binding of include/mdg.h and the Python face of the engine.
 
 ``Engine`` plays the role of the reference's ``SimLoop`` with its ``FProxy``
-chain (mdtk/SimLoop.hpp, mdtk/potentials/FProxy.hpp): it owns one system on
-the GPU, returns per-potential energies and Atom::grad from one force call
-(FProxy::operator(), FProxy.cxx:45-61) and advances the velocity-Verlet loop
-(SimLoop::executeMain, SimLoop.cxx:252-445).  Units are the reference's CGS.
+chain (mdtk/SimLoop.hpp, mdtk/potentials/FProxy.hpp): it owns one system, or
+a batch of independent systems, on the GPU, returns per-potential energies and
+Atom::grad from one force call (FProxy::operator(), FProxy.cxx:45-61) and
+advances the velocity-Verlet loop (SimLoop::executeMain, SimLoop.cxx:252-445).
+Units are the reference's CGS.
 """
 from __future__ import annotations
 
@@ -48,6 +49,9 @@ def load_library(path: str = LIB_PATH):
     L.mdg_last_error.argtypes = [vp]
     L.mdg_last_error.restype = ctypes.c_char_p
     L.mdg_set_system.argtypes = [vp, ctypes.c_int, ip, dp, dp, dp, ctypes.POINTER(ctypes.c_uint8)]
+    L.mdg_set_systems.argtypes = [vp, ctypes.c_int, ip, ip, dp, dp, dp, ctypes.POINTER(ctypes.c_uint8)]
+    L.mdg_num_systems.argtypes = [vp]
+    L.mdg_set_positions.argtypes = [vp, dp, ip, dp]
     L.mdg_num_terms.argtypes = [vp]
     L.mdg_term_kind.argtypes = [vp, ctypes.c_int]
     L.mdg_energy_grad.argtypes = [vp, dp, dp]
@@ -86,6 +90,8 @@ class Engine:
             raise EngineError(L.mdg_last_error(None).decode())
         self._h = h
         self.n = 0
+        self.nsys = 0
+        self.offsets = np.zeros(1, np.int64)
         self.nterms = L.mdg_num_terms(h)
         self.kinds = [L.mdg_term_kind(h, i) for i in range(self.nterms)]
 
@@ -108,6 +114,34 @@ class Engine:
         self._chk(_lib.mdg_set_system(self._h, int(e.shape[0]), _p(e, ctypes.c_int32), _p(x), _p(v), _p(p),
                                       _p(f, ctypes.c_uint8)))
         self.n = int(e.shape[0])
+        self.nsys = 1
+        self.offsets = np.array([0, self.n], np.int64)
+        return self
+
+    def set_systems(self, parts):
+        """A batch: parts is a sequence of (elem, xyz_cm, vel_cm_s, pbc_cm,
+        fixed) tuples, one per independent system."""
+        counts = np.array([len(q[0]) for q in parts], np.int32)
+        e = np.ascontiguousarray(np.concatenate([np.asarray(q[0], np.int32) for q in parts]))
+        x = np.ascontiguousarray(np.concatenate([np.asarray(q[1], np.float64).reshape(-1) for q in parts]))
+        vs = [q[2] for q in parts]
+        v = None
+        if any(a is not None for a in vs):
+            v = np.ascontiguousarray(np.concatenate(
+                [np.zeros(3 * len(q[0])) if q[2] is None else np.asarray(q[2], np.float64).reshape(-1)
+                 for q in parts]))
+        p = np.ascontiguousarray(np.concatenate([np.asarray(q[3], np.float64).reshape(3) for q in parts]))
+        fs = [q[4] if len(q) > 4 else None for q in parts]
+        f = None
+        if any(a is not None for a in fs):
+            f = np.ascontiguousarray(np.concatenate(
+                [np.zeros(len(q[0]), np.uint8) if fq is None else np.asarray(fq, np.uint8)
+                 for q, fq in zip(parts, fs)]))
+        self._chk(_lib.mdg_set_systems(self._h, len(parts), _p(counts, ctypes.c_int32), _p(e, ctypes.c_int32),
+                                       _p(x), _p(v), _p(p), _p(f, ctypes.c_uint8)))
+        self.n = int(e.shape[0])
+        self.nsys = len(parts)
+        self.offsets = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
         return self
 
     @classmethod
@@ -115,11 +149,24 @@ class Engine:
         eng = cls(**kw)
         return eng.set_system(*sys.to_cgs())
 
+    @classmethod
+    def from_systems(cls, systems, **kw):
+        eng = cls(**kw)
+        return eng.set_systems([s.to_cgs() for s in systems])
+
     def energy_grad(self):
-        e = np.zeros(self.nterms)
+        """Energies per chain member (shape (nterms,), or (nsys, nterms) for
+        a batch) in erg and dE/dx (n, 3) in erg/cm."""
+        e = np.zeros(max(self.nsys, 1) * self.nterms)
         g = np.zeros((self.n, 3))
         self._chk(_lib.mdg_energy_grad(self._h, _p(e), _p(g)))
-        return e, g
+        return (e if self.nsys <= 1 else e.reshape(self.nsys, self.nterms)), g
+
+    def set_positions(self, xyz_cm, pbc_count=None, dr_cm=None):
+        x = np.ascontiguousarray(xyz_cm, np.float64).reshape(-1)
+        c = None if pbc_count is None else np.ascontiguousarray(pbc_count, np.int32).reshape(-1)
+        d = None if dr_cm is None else np.ascontiguousarray(dr_cm, np.float64).reshape(-1)
+        self._chk(_lib.mdg_set_positions(self._h, _p(x), _p(c, ctypes.c_int32), _p(d)))
 
     def neighbour_lists(self, term: int):
         counts = np.zeros(self.n, np.int32)
@@ -130,9 +177,15 @@ class Engine:
         return [idx[off[i]:off[i + 1]].copy() for i in range(self.n)]
 
     def run(self, nsteps: int, fixed_dt: float = 0.0):
-        ep, ek, dt, nf = (np.zeros(nsteps) for _ in range(4))
+        """nsteps Verlet iterations; per-step arrays of shape (nsteps,), or
+        (nsteps, nsys) for a batch."""
+        m = max(self.nsys, 1)
+        ep, ek, dt, nf = (np.zeros(nsteps * m) for _ in range(4))
         self._chk(_lib.mdg_step_nve(self._h, int(nsteps), float(fixed_dt), _p(ep), _p(ek), _p(dt), _p(nf)))
-        return dict(epot=ep, ekin=ek, dt=dt, netforce=nf)
+        out = dict(epot=ep, ekin=ek, dt=dt, netforce=nf)
+        if m > 1:
+            out = {k: v.reshape(nsteps, m) for k, v in out.items()}
+        return out
 
     def state(self):
         x, v, a = (np.zeros((self.n, 3)) for _ in range(3))

@@ -131,7 +131,7 @@ def diamond(nx: int, ny: int, nz: int, a: float = 3.567) -> System:
 
 
 def polyethylene(nx: int, ny: int, nz: int, a: float = 7.42, b: float = 4.95, c: float = 2.55,
-                 chains_per_cell: int = 2) -> System:
+                 chains_per_cell: int = 2, wrap: bool = True) -> System:
     """Orthorhombic polyethylene (Bunn cell), CH2 zig-zag chains along z.
 
     chains_per_cell=2 is the physical two-chain cell (12 atoms); 1 keeps only the
@@ -170,7 +170,7 @@ def polyethylene(nx: int, ny: int, nz: int, a: float = 7.42, b: float = 4.95, c:
     pos = (shift[:, None, :] + base_p[None]).reshape(-1, 3)
     elem = np.tile(base_e, len(g))
     s = System(elem, pos, np.array([nx * a, ny * b, nz * c]), name=f"pe{nx}x{ny}x{nz}x{chains_per_cell}")
-    return s.wrapped()
+    return s.wrapped() if wrap else s
 
 
 def graphite(nx: int, ny: int, nz: int, a: float = 2.46, d: float = 3.35) -> System:
@@ -269,6 +269,58 @@ def _place(a, b, c, bond, theta, dih):
     m = np.cross(n, bc)
     d2 = np.array([-bond * math.cos(theta), bond * math.sin(theta) * math.cos(dih), bond * math.sin(theta) * math.sin(dih)])
     return c + d2[0] * bc + d2[1] * m + d2[2] * n
+
+
+def pe_slab(layers: int, lateral: tuple[int, int] = (8, 3)) -> System:
+    """Polyethylene slab with the chains in the surface plane: chains run along
+    x (periodic, lateral[0] chain periods c), the a axis of the Bunn cell lies
+    along y (periodic, lateral[1] cells), and `layers` cells of b stack along
+    z, which is open (NO_PBC) with a free top and bottom surface.
+    12*lateral[0]*lateral[1] atoms per layer."""
+    nc, na = lateral
+    pe = polyethylene(na, layers, nc, wrap=False)   # chains along the pe's z
+    a, b, c = pe.pbc
+    raw = pe.pos
+    pos = np.stack([raw[:, 2], raw[:, 0], raw[:, 1]], 1)  # x <- chain axis, y <- a, z <- b
+    pos[:, 2] -= pos[:, 2].min()
+    s = System(pe.elem.copy(), pos, np.array([c, a, NO_PBC]), name=f"peslab{nc}x{na}x{layers}")
+    return s.wrapped()
+
+
+def bombard_target(n_target: int, stream: int, energy_ev: float = 100.0, T: float = 300.0,
+                   lateral: tuple[int, int] = (8, 3), height: float = 8.0, seed: int = BASE_SEED) -> System:
+    """One bombardment trajectory: a pe_slab with about n_target atoms (whole
+    layers), the bottom chain row fixed (Atom::fix, mdtk/Atom.hpp:58), 300 K
+    Maxwell-Boltzmann velocities, and one Ar ion `height` Angstrom above the
+    top surface at a seeded random lateral point, moving along -z with
+    `energy_ev` (normal incidence)."""
+    per_layer = 12 * lateral[0] * lateral[1]
+    layers = max(1, int(round(n_target / per_layer)))
+    s = pe_slab(layers, lateral)
+    z = s.pos[:, 2]
+    # bottom chain row: the corner chain (first 6 atoms of a cell) of layer 0
+    idx = np.arange(s.n)
+    cell = idx // 12
+    layer = (cell // lateral[0]) % layers
+    fixed = ((layer == 0) & (idx % 12 < 6)).astype(np.uint8)
+    s.fixed = fixed
+    s.vel = maxwell_velocities(s, T, stream=stream, seed=seed)
+    u = uniforms(2, stream + 2000003, seed)
+    ion = np.array([[u[0] * s.pbc[0], u[1] * s.pbc[1], z.max() + height]])
+    v_ion = math.sqrt(2.0 * energy_ev * EV / (MASS_AMU[AR] * AMU)) / (ANG / FS)
+    elem = np.concatenate([s.elem, [AR]]).astype(np.int32)
+    pos = np.concatenate([s.pos, ion])
+    vel = np.concatenate([s.vel, [[0.0, 0.0, -v_ion]]])
+    fx = np.concatenate([fixed, [0]]).astype(np.uint8)
+    return System(elem, pos, s.pbc.copy(), vel, fx, name=f"bombard{stream}",
+                  meta=dict(layers=layers, ion_energy_ev=energy_ev))
+
+
+def bombard_batch(ntraj: int = 256, nmin: int = 2000, nmax: int = 12000, **kw) -> list[System]:
+    """ntraj bombardment trajectories with slab sizes ~U[nmin, nmax] (seeded,
+    rounded to whole layers); trajectory t uses Philox sub-stream t."""
+    u = uniforms(ntraj, 4000037)
+    return [bombard_target(int(nmin + u[t] * (nmax - nmin)), t, **kw) for t in range(ntraj)]
 
 
 def jitter(sys: System, sigma: float, stream: int) -> System:
