@@ -107,6 +107,22 @@ int mdg_term_kind(const mdg_ctx* ctx, int i);
  * when its skin test asks for it (NeighbourList.cxx:108-133). */
 int mdg_energy_grad(mdg_ctx* ctx, double* e_terms, double* grad);
 
+/* Differentiable parameters theta[MDG_NUM_PARAMS] (CGS): for the type pairs
+ * p = 0 (C-C), 1 (C-H), 2 (H-H), REBO's pair parameters {Q, A, alpha, B1, B2,
+ * B3, beta1, beta2, beta3} at theta[9p..9p+8] (REBO::REBO, REBO.cxx:856-941;
+ * protected members REBO.hpp:155-175) and the LJ {epsilon, sigma} at
+ * theta[27+2p], theta[28+2p] (AIREBO::setupPotential, AIREBO_LJ.cxx:229-247;
+ * sigma also sets the S_r range [sigma, 2^(1/6) sigma], :239-247).  The
+ * reference sets them only in its constructors; mdg_set_params replaces a
+ * subclass that overwrites them.  mdg_energy_grad_params is mdg_energy_grad
+ * plus dE_s/dtheta of every system s (dtheta[nsys*MDG_NUM_PARAMS], system s at
+ * dtheta[s*MDG_NUM_PARAMS]), from adjoint kernels of the FP64 form; no
+ * reference counterpart (the reference has no parameter derivatives). */
+enum { MDG_NUM_PARAMS = 33 };
+int mdg_get_params(const mdg_ctx* ctx, double* theta);
+int mdg_set_params(mdg_ctx* ctx, const double* theta);
+int mdg_energy_grad_params(mdg_ctx* ctx, double* e_terms, double* grad, double* dtheta);
+
 /* Neighbour list of chain member `term` as the reference would hold it
  * (FGeneral::nl, NeighbourList.hpp:41-83): counts[n], then the lists
  * concatenated (ascending index per atom) into idx[cap]. */

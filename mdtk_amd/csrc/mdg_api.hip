@@ -67,6 +67,9 @@ struct mdg_ctx {
   int* qcount = nullptr;
   int qcap = 0;
   double* eatom = nullptr;  // nterms*n per-atom energies of the last force call
+  double* dtheta = nullptr;  // kNumTheta*nsys parameter gradients (allocated on first use)
+  bool theta_mode = false;
+  Tables host_T;             // parameter tables as uploaded (for parameter overrides)
   double* eacc = nullptr;   // 8*nsys per-system energies of each chain member
   double* dtp = nullptr;    // nsys
   double* stepout = nullptr;
@@ -289,6 +292,38 @@ void nl_build(mdg_ctx* c, bool shortlist) {
   }
 }
 
+// The differentiable parameters theta (kNumTheta values, CGS) inside the FP64
+// tables: for p = C-C, C-H, H-H, REBO's {Q, A, alpha, B1-3, beta1-3} at 9p
+// (REBO.cxx:856-941) and LJ {epsilon, sigma} at 27 + 2p (AIREBO_LJ.cxx:229-247).
+// write = false reads them out.  sigma also sets the S_r range [sigma,
+// 2^(1/6) sigma] (AIREBO_LJ.cxx:239-247).
+void theta_io(Tables& T, double* th, bool write) {
+  const int pa[3] = {T_C, T_C, T_H}, pb[3] = {T_C, T_H, T_H};
+  for (int p = 0; p < 3; p++) {
+    int a = pa[p], b = pb[p];
+    double* v[9] = {&T.rebo.Q[a][b], &T.rebo.A[a][b], &T.rebo.alpha[a][b], &T.rebo.B[0][a][b], &T.rebo.B[1][a][b],
+                    &T.rebo.B[2][a][b], &T.rebo.beta[0][a][b], &T.rebo.beta[1][a][b], &T.rebo.beta[2][a][b]};
+    double* w[9] = {&T.rebo.Q[b][a], &T.rebo.A[b][a], &T.rebo.alpha[b][a], &T.rebo.B[0][b][a], &T.rebo.B[1][b][a],
+                    &T.rebo.B[2][b][a], &T.rebo.beta[0][b][a], &T.rebo.beta[1][b][a], &T.rebo.beta[2][b][a]};
+    for (int k = 0; k < 9; k++) {
+      if (write) *v[k] = *w[k] = th[9 * p + k];
+      else th[9 * p + k] = *v[k];
+    }
+    if (write) {
+      T.lj.zeta[a][b] = T.lj.zeta[b][a] = th[27 + 2 * p];
+      T.lj.sigma[a][b] = T.lj.sigma[b][a] = th[27 + 2 * p + 1];
+      for (int q = 0; q < 2; q++) {
+        int x = q ? b : a, y = q ? a : b;
+        T.lj.RLJ[x][y][0] = T.lj.sigma[x][y];
+        T.lj.RLJ[x][y][1] = T.lj.sigma[x][y] * std::pow(2.0, 1.0 / 6.0);
+      }
+    } else {
+      th[27 + 2 * p] = T.lj.zeta[a][b];
+      th[27 + 2 * p + 1] = T.lj.sigma[a][b];
+    }
+  }
+}
+
 // Clears one list's rebuild flags (all systems).
 __global__ void k_clear_list(int nsys, int list, int* flags, int* sflags) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -308,10 +343,11 @@ __global__ void k_zero_counts(long long* counters) {
 // One force call (FProxy::operator(), FProxy.cxx:45-61): lists, coordination,
 // then every chain member adds into grad and its energy slot.
 // The bond-order chain members, in the geometry type R of the context.
-template <class R>
+template <class R, bool TH>
 void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   int n = c->n;
   Atoms& A = c->A;
+  double* dth = TH ? c->dtheta : nullptr;
   if (c->have_short) {
     launch(c, "k_coord", [&] { k_coord<R><<<blocks(n), 256, 0, c->stream>>>(A, T, c->box, c->derr); });
     launch(c, "k_bond_enum", [&] {
@@ -321,8 +357,8 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   int bgrid = blocks(2LL * n, 256, 8192);
   if (c->slot_rebo >= 0) {
     launch(c, "k_rebo_bonds", [&] {
-      k_rebo_bonds<R><<<bgrid, 256, 0, c->stream>>>(A, T, c->box, c->bonds, c->nbonds, A.grad,
-                                                    c->eatom + (size_t)c->slot_rebo * n, c->derr);
+      k_rebo_bonds<R, TH><<<bgrid, 256, 0, c->stream>>>(A, T, c->box, c->bonds, c->nbonds, A.grad,
+                                                        c->eatom + (size_t)c->slot_rebo * n, c->derr, dth);
     });
   }
   if (c->slot_bren >= 0) {
@@ -333,13 +369,13 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   }
   if (c->slot_lj >= 0) {
     launch(c, "k_lj_wave", [&] {
-      k_lj_wave<R><<<blocks(n, 4, 65536), 256, 0, c->stream>>>(A, T, c->box, A.grad, c->eatom + (size_t)c->slot_lj * n,
-                                                                c->queue, c->qcount, c->qcap, c->counters + 2,
-                                                                c->derr);
+      k_lj_wave<R, TH><<<blocks(n, 4, 65536), 256, 0, c->stream>>>(
+          A, T, c->box, A.grad, c->eatom + (size_t)c->slot_lj * n, c->queue, c->qcount, c->qcap, c->counters + 2,
+          c->derr, dth);
     });
     launch(c, "k_bstar", [&] {
-      k_bstar<R><<<blocks(8LL * n, 256, 16384), 256, 0, c->stream>>>(A, T, c->box, c->queue, c->qcount, c->qcap,
-                                                                     A.grad, c->eatom + (size_t)c->slot_lj * n, c->derr);
+      k_bstar<R, TH><<<blocks(8LL * n, 256, 16384), 256, 0, c->stream>>>(
+          A, T, c->box, c->queue, c->qcount, c->qcap, A.grad, c->eatom + (size_t)c->slot_lj * n, c->derr, dth);
     });
   }
   if (c->slot_tors >= 0) {
@@ -365,8 +401,9 @@ void force_call(mdg_ctx* c) {
   k_zero_i<<<1, 64, 0, c->stream>>>(c->nbonds, 1);
   k_zero_i<<<1, 64, 0, c->stream>>>(c->qcount, 1);
   k_zero_counts<<<1, 64, 0, c->stream>>>(c->counters);
-  if (c->mixed) force_terms<float>(c, c->dTf);
-  else force_terms<double>(c, c->dT);
+  if (c->mixed) force_terms<float, false>(c, c->dTf);
+  else if (c->theta_mode) force_terms<double, true>(c, c->dT);
+  else force_terms<double, false>(c, c->dT);
   if (c->slot_fbzl >= 0) {
     launch(c, "k_fbzl", [&] {
       k_fbzl<<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->cfg.fbzl_r0, c->cfg.fbzl_r1, A.grad,
@@ -447,6 +484,7 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
   c->stream = c->own;
   Tables h;
   build_tables(h, cfg->brenner_paramset);
+  c->host_T = h;
   if (hipMalloc(&c->dT, sizeof(Tables)) != hipSuccess ||
       hipMemcpy(c->dT, &h, sizeof(Tables), hipMemcpyHostToDevice) != hipSuccess) {
     g_create_err = "table upload failed";
@@ -706,6 +744,58 @@ int mdg_energy_grad(mdg_ctx* c, double* e_terms, double* grad) {
   if (e_terms)
     for (int s = 0; s < c->nsys; s++)
       for (int t = 0; t < nt; t++) e_terms[(size_t)s * nt + t] = e[8 * (size_t)s + t];
+  return 0;
+}
+
+int mdg_get_params(const mdg_ctx* c, double* theta) {
+  Tables t = c->host_T;
+  theta_io(t, theta, false);
+  return 0;
+}
+
+int mdg_set_params(mdg_ctx* c, const double* theta) {
+  std::vector<double> th(theta, theta + kNumTheta);
+  for (double v : th)
+    if (!std::isfinite(v)) {
+      c->err = "mdg_set_params: non-finite parameter";
+      return 1;
+    }
+  Tables t = c->host_T;
+  theta_io(t, th.data(), true);
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      if (!(t.lj.R[a][b][0] > t.lj.RLJ[a][b][1]) || !(t.lj.sigma[a][b] > 0)) {
+        c->err = "mdg_set_params: 2^(1/6) sigma must stay below the LJ switch start (AIREBO_LJ.cxx:254-257)";
+        return 1;
+      }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(c->dT, &t, sizeof(Tables), hipMemcpyHostToDevice));
+  if (c->mixed) {
+    TablesT<float> hf;
+    convert_tables(t, hf);
+    HIPCHK(c, hipMemcpy(c->dTf, &hf, sizeof(hf), hipMemcpyHostToDevice));
+  }
+  c->host_T = t;
+  return 0;
+}
+
+int mdg_energy_grad_params(mdg_ctx* c, double* e_terms, double* grad, double* dtheta) {
+  if (c->n == 0) {
+    c->err = "no system";
+    return 1;
+  }
+  if (c->mixed) {
+    c->err = "parameter gradients are computed by the FP64 form (create the context with MDG_PREC_FP64)";
+    return 1;
+  }
+  if (!c->dtheta && dalloc(c, c->dtheta, (size_t)kNumTheta * c->nsys)) return 1;
+  k_zero<<<blocks((long long)kNumTheta * c->nsys), 256, 0, c->stream>>>(c->dtheta, (size_t)kNumTheta * c->nsys);
+  c->theta_mode = true;
+  int rc = mdg_energy_grad(c, e_terms, grad);
+  c->theta_mode = false;
+  if (rc) return rc;
+  if (dtheta)
+    HIPCHK(c, hipMemcpy(dtheta, c->dtheta, (size_t)kNumTheta * c->nsys * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
 }
 

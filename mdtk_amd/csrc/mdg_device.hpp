@@ -23,6 +23,7 @@ constexpr double kAmu = 1.6605655e-24;
 constexpr double kECharge = 4.8e-10;
 constexpr double kNoPbc = 10000.0 * kAo;
 constexpr double kPi = 3.14159265358979323846;
+constexpr int kNumTheta = 33;  // REBO pair (3 x 9) and LJ (3 x 2) parameters per system
 
 enum : int { T_C = 0, T_H = 1, T_ION = 2, T_OTHER = 3 };
 

@@ -19,6 +19,14 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Parameter-gradient layout (per system, kNumTheta doubles): for the type
+// pairs p = C-C, C-H, H-H (p = ti + tj), REBO's pair parameters at 9p + {Q, A,
+// alpha, B1, B2, B3, beta1, beta2, beta3} (REBO.cxx:856-941) and the LJ
+// parameters at 27 + 2p + {epsilon, sigma} (AIREBO_LJ.cxx:229-247).
+__device__ __forceinline__ double* theta_of(double* dth, const Atoms& A, int owner) {
+  return dth + (size_t)kNumTheta * A.sys[owner];
+}
+
 // Energy of one bond, pair or quadruplet, credited to its owner atom (the
 // per-system totals are summed over atoms afterwards).
 __device__ __forceinline__ void add_energy(double* eatom, int owner, double e) {
@@ -517,11 +525,12 @@ __device__ void bo_backward(const Atoms& A, const TablesT<R>* __restrict__ T, co
 
 // ------------------------------------------------------------ REBO bonds
 // REBO::operator() (REBO.cxx:32-65): E += VR + b_ij * VA for every bond.
-template <class R>
+// TH: also accumulate dE/dtheta of the pair parameters into dth (FP64 only).
+template <class R, bool TH>
 __global__ void __launch_bounds__(256) k_rebo_bonds(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
                                                    const int2* __restrict__ bonds, const int* __restrict__ nbonds,
                                                    double* __restrict__ grad, double* __restrict__ eatom,
-                                                   unsigned* err) {
+                                                   unsigned* err, double* __restrict__ dth) {
   const double gs = T->eunit / T->lunit;
   int nbt = min(*nbonds, A.bond_cap);
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nbt; t += gridDim.x * blockDim.x) {
@@ -554,15 +563,31 @@ __global__ void __launch_bounds__(256) k_rebo_bonds(Atoms A, const TablesT<R>* _
       R vsum = u1 + u2 + u3;
       R VA = -f * vsum;
       double eb = double(VR);
-      if (VA != R(0)) {
+      double bb = 0.0;
+      if (VA != R(0) || TH) {
         BOState st;
         bo_forward<R>(A, T, p, st, err);
-        double bb = st.b;
-        bo_backward<R>(A, T, p, st, double(VA), grad, gs, err);
-        R vder = (-P.beta[0][a][b]) * u1 + (-P.beta[1][a][b]) * u2 + (-P.beta[2][a][b]) * u3;
-        R bR = R(bb);
-        cr += (-vder * f * bR) + (-vsum * bR) * df;
-        eb += bb * double(VA);
+        bb = st.b;
+        if (VA != R(0)) {
+          bo_backward<R>(A, T, p, st, double(VA), grad, gs, err);
+          R vder = (-P.beta[0][a][b]) * u1 + (-P.beta[1][a][b]) * u2 + (-P.beta[2][a][b]) * u3;
+          R bR = R(bb);
+          cr += (-vder * f * bR) + (-vsum * bR) * df;
+          eb += bb * double(VA);
+        }
+      }
+      if constexpr (TH) {
+        // VR = f (1 + Q/r) A e^{-alpha r}, b VA = -b f sum_n B_n e^{-beta_n r}
+        double* th = theta_of(dth, A, p.i) + 9 * (a + b);
+        double ea = exp(-double(P.alpha[a][b]) * double(r));
+        unsafeAtomicAdd(th + 0, double(f) * double(P.A[a][b]) * ea / double(r));
+        unsafeAtomicAdd(th + 1, double(f) * (1.0 + double(P.Q[a][b]) / double(r)) * ea);
+        unsafeAtomicAdd(th + 2, -double(r) * double(VR));
+        for (int n = 0; n < 3; n++) {
+          double en = exp(-double(P.beta[n][a][b]) * double(r));
+          unsafeAtomicAdd(th + 3 + n, -bb * double(f) * en);
+          unsafeAtomicAdd(th + 6 + n, bb * double(f) * double(P.B[n][a][b]) * double(r) * en);
+        }
       }
       add_energy(eatom, p.i, to_erg<R>(eb, T->eunit));
       V3<R> v = p.rv * (cr / r);
@@ -638,11 +663,12 @@ constexpr int kPathCap = 320;  // path records per atom held in LDS
 // (the reference's loop order).  Phase 2: lanes sweep i's long list (j > i,
 // coalesced), look the target up among the path records, and either finish
 // the pair (S_r = 0) or queue it for the b* kernel.
-template <class R>
+template <class R, bool TH>
 __global__ void __launch_bounds__(256) k_lj_wave(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
                                                 double* __restrict__ grad, double* __restrict__ eatom,
                                                 int4* __restrict__ queue, int* __restrict__ qcount, int qcap,
-                                                long long* __restrict__ npairs, unsigned* err) {
+                                                long long* __restrict__ npairs, unsigned* err,
+                                                double* __restrict__ dth) {
   __shared__ int s_t[4][kPathCap];
   __shared__ int s_key[4][kPathCap];
   __shared__ R s_w[4][kPathCap];
@@ -658,6 +684,7 @@ __global__ void __launch_bounds__(256) k_lj_wave(Atoms A, const TablesT<R>* __re
     if (ti > T_H) continue;  // wave-uniform
     const double* Lb = box_of(bx, i);
     double e = 0.0;
+    double th_e[2] = {0.0, 0.0}, th_s[2] = {0.0, 0.0};  // dE/d(epsilon, sigma) by tj
     if (lane == 0) s_cnt[wv] = 0;
     int nb1 = A.bcnt[i];
     int kk = -1, c2 = 0;
@@ -767,6 +794,18 @@ __global__ void __launch_bounds__(256) k_lj_wave(Atoms A, const TablesT<R>* __re
       scatter3<R>(grad, j, -v, gs);
       if (path.hops > 0) cij_backward<R>(A, T, bx, i, j, path, R(1) * LJ, grad, gs);
       e += to_erg<R>(double(R(1) * C * LJ), T->eunit);
+      if constexpr (TH) {  // S_r = 0: E = C f 4 eps ((sigma/r)^12 - (sigma/r)^6)
+        double de = double(C) * double(f) * 4.0 * double(s12 - s6);
+        double ds = double(C) * double(f) * 4.0 * double(z) * double(R(12) * s12 - R(6) * s6) /
+                    double(L.sigma[ti][tj]);
+        if (tj == T_C) {
+          th_e[0] += de;
+          th_s[0] += ds;
+        } else {
+          th_e[1] += de;
+          th_s[1] += ds;
+        }
+      }
     }
     double gx = wave_sum(double(gi.x)), gy = wave_sum(double(gi.y)), gz = wave_sum(double(gi.z));
     e = wave_sum(e);
@@ -777,6 +816,16 @@ __global__ void __launch_bounds__(256) k_lj_wave(Atoms A, const TablesT<R>* __re
       }
       add_energy(eatom, i, e);
     }
+    if constexpr (TH) {
+      for (int q = 0; q < 2; q++) {
+        double de = wave_sum(th_e[q]), ds = wave_sum(th_s[q]);
+        if (lane == 0 && (de != 0.0 || ds != 0.0)) {
+          double* th = theta_of(dth, A, i) + 27 + 2 * (ti + q);
+          unsafeAtomicAdd(th, de);
+          unsafeAtomicAdd(th + 1, ds);
+        }
+      }
+    }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
   }
@@ -784,11 +833,11 @@ __global__ void __launch_bounds__(256) k_lj_wave(Atoms A, const TablesT<R>* __re
 }
 
 // LJ pairs that need b*_ij (S_r != 0): the whole StrStb / VLJ / Cij chain.
-template <class R>
+template <class R, bool TH>
 __global__ void __launch_bounds__(256) k_bstar(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
                                               const int4* __restrict__ queue, const int* __restrict__ qcount,
                                               int qcap, double* __restrict__ grad, double* __restrict__ eatom,
-                                              unsigned* err) {
+                                              unsigned* err, double* __restrict__ dth) {
   const double gs = T->eunit / T->lunit;
   int nq = min(*qcount, qcap);
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nq; t += gridDim.x * blockDim.x) {
@@ -863,6 +912,18 @@ __global__ void __launch_bounds__(256) k_bstar(Atoms A, const TablesT<R>* __rest
     scatter3<R>(grad, j, -v, gs);
     if (path.hops > 0 && SrSb * LJ != R(0)) cij_backward<R>(A, T, bx, i, j, path, SrSb * LJ, grad, gs);
     add_energy(eatom, i, to_erg<R>(double(SrSb * C * LJ), T->eunit));
+    if constexpr (TH) {
+      // S_r depends on r/sigma only: dS_r/dsigma = -(r/sigma) dS_r/dr
+      double* th = theta_of(dth, A, i) + 27 + 2 * (ti + tj);
+      double sg = double(L.sigma[ti][tj]);
+      double fv = (f == R(0)) ? 0.0 : double(f);
+      double de = double(SrSb) * double(C) * fv * 4.0 * double(s12 - s6);
+      double ds = double(SrSb) * double(C) * fv * 4.0 * double(z) * double(R(12) * s12 - R(6) * s6) / sg +
+                  (-(double(r) / sg) * double(dSfun<R>(r, rmin, rmax))) * double(Stb - R(1)) * double(C) *
+                      double(LJ);
+      unsafeAtomicAdd(th, de);
+      unsafeAtomicAdd(th + 1, ds);
+    }
   }
 }
 
@@ -959,17 +1020,21 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
   if (nq) atomicAdd((unsigned long long*)nquads, (unsigned long long)nq);
 }
 
+#define MDG_INSTANTIATE_TH(R, TH)                                                                              \
+  template __global__ void k_rebo_bonds<R, TH>(Atoms, const TablesT<R>*, Box, const int2*, const int*,       \
+                                               double*, double*, unsigned*, double*);                          \
+  template __global__ void k_lj_wave<R, TH>(Atoms, const TablesT<R>*, Box, double*, double*, int4*, int*,    \
+                                            int, long long*, unsigned*, double*);                              \
+  template __global__ void k_bstar<R, TH>(Atoms, const TablesT<R>*, Box, const int4*, const int*, int,       \
+                                          double*, double*, unsigned*, double*);
 #define MDG_INSTANTIATE(R)                                                                                     \
   template __global__ void k_coord<R>(Atoms, const TablesT<R>*, Box, unsigned*);                              \
-  template __global__ void k_rebo_bonds<R>(Atoms, const TablesT<R>*, Box, const int2*, const int*, double*,  \
-                                           double*, unsigned*);                                                \
-  template __global__ void k_lj_wave<R>(Atoms, const TablesT<R>*, Box, double*, double*, int4*, int*, int,    \
-                                        long long*, unsigned*);                                                \
-  template __global__ void k_bstar<R>(Atoms, const TablesT<R>*, Box, const int4*, const int*, int, double*,   \
-                                      double*, unsigned*);                                                     \
   template __global__ void k_etors<R>(Atoms, const TablesT<R>*, Box, const int2*, const int*, int, double*,   \
                                       double*, long long*);
 MDG_INSTANTIATE(double)
 MDG_INSTANTIATE(float)
+MDG_INSTANTIATE_TH(double, false)
+MDG_INSTANTIATE_TH(double, true)
+MDG_INSTANTIATE_TH(float, false)
 
 }  // namespace mdg

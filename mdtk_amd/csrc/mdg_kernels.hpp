@@ -56,15 +56,17 @@ __global__ void k_bond_enum(Atoms A, int2* out, int* count, int cap, unsigned* e
 // of the bond, pair or quadruplet.
 template <class R>
 __global__ void k_coord(Atoms A, const TablesT<R>* T, Box bx, unsigned* err);
-template <class R>
+// TH = true (FP64 only): also accumulate dE/dtheta per system into dth
+// (kNumTheta doubles per system).
+template <class R, bool TH>
 __global__ void k_rebo_bonds(Atoms A, const TablesT<R>* T, Box bx, const int2* bonds, const int* nbonds,
-                             double* grad, double* eatom, unsigned* err);
-template <class R>
+                             double* grad, double* eatom, unsigned* err, double* dth);
+template <class R, bool TH>
 __global__ void k_lj_wave(Atoms A, const TablesT<R>* T, Box bx, double* grad, double* eatom, int4* queue,
-                          int* qcount, int qcap, long long* npairs, unsigned* err);
-template <class R>
+                          int* qcount, int qcap, long long* npairs, unsigned* err, double* dth);
+template <class R, bool TH>
 __global__ void k_bstar(Atoms A, const TablesT<R>* T, Box bx, const int4* queue, const int* qcount, int qcap,
-                        double* grad, double* eatom, unsigned* err);
+                        double* grad, double* eatom, unsigned* err, double* dth);
 template <class R>
 __global__ void k_etors(Atoms A, const TablesT<R>* T, Box bx, const int2* bonds, const int* nbonds, int compat,
                         double* grad, double* eatom, long long* nquads);

@@ -18,6 +18,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmdg.so")
 
 POT_FBZL, POT_BRENNER, POT_REBO, POT_AIREBO, POT_ETORS = 1, 2, 4, 8, 16
+NUM_PARAMS = 33  # REBO pair {Q, A, alpha, B1-3, beta1-3} x (CC, CH, HH), then LJ {eps, sigma} x (CC, CH, HH)
+PARAM_NAMES = [f"{n}_{p}" for p in ("CC", "CH", "HH")
+               for n in ("Q", "A", "alpha", "B1", "B2", "B3", "beta1", "beta2", "beta3")] + \
+              [f"{n}_{p}" for p in ("CC", "CH", "HH") for n in ("eps", "sigma")]
 PREC_FP64, PREC_MIXED = 0, 1
 _NAMES = {"FBZL": POT_FBZL, "Brenner": POT_BRENNER, "REBO": POT_REBO, "AIREBO": POT_AIREBO, "ETors": POT_ETORS}
 
@@ -52,6 +56,9 @@ def load_library(path: str = LIB_PATH):
     L.mdg_set_systems.argtypes = [vp, ctypes.c_int, ip, ip, dp, dp, dp, ctypes.POINTER(ctypes.c_uint8)]
     L.mdg_num_systems.argtypes = [vp]
     L.mdg_set_positions.argtypes = [vp, dp, ip, dp]
+    L.mdg_get_params.argtypes = [vp, dp]
+    L.mdg_set_params.argtypes = [vp, dp]
+    L.mdg_energy_grad_params.argtypes = [vp, dp, dp, dp]
     L.mdg_num_terms.argtypes = [vp]
     L.mdg_term_kind.argtypes = [vp, ctypes.c_int]
     L.mdg_energy_grad.argtypes = [vp, dp, dp]
@@ -161,6 +168,28 @@ class Engine:
         g = np.zeros((self.n, 3))
         self._chk(_lib.mdg_energy_grad(self._h, _p(e), _p(g)))
         return (e if self.nsys <= 1 else e.reshape(self.nsys, self.nterms)), g
+
+    def params(self):
+        """The NUM_PARAMS differentiable parameters (CGS), see PARAM_NAMES."""
+        th = np.zeros(NUM_PARAMS)
+        self._chk(_lib.mdg_get_params(self._h, _p(th)))
+        return th
+
+    def set_params(self, theta):
+        th = np.ascontiguousarray(theta, np.float64).reshape(NUM_PARAMS)
+        self._chk(_lib.mdg_set_params(self._h, _p(th)))
+
+    def energy_grad_params(self):
+        """energy_grad plus dE_s/dtheta per system: (nsys, NUM_PARAMS), or
+        (NUM_PARAMS,) for one system.  FP64 contexts only."""
+        m = max(self.nsys, 1)
+        e = np.zeros(m * self.nterms)
+        g = np.zeros((self.n, 3))
+        d = np.zeros(m * NUM_PARAMS)
+        self._chk(_lib.mdg_energy_grad_params(self._h, _p(e), _p(g), _p(d)))
+        if m == 1:
+            return e, g, d
+        return e.reshape(m, self.nterms), g, d.reshape(m, NUM_PARAMS)
 
     def set_positions(self, xyz_cm, pbc_count=None, dr_cm=None):
         x = np.ascontiguousarray(xyz_cm, np.float64).reshape(-1)

@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
 POT_FBZL, POT_BRENNER, POT_REBO, POT_AIREBO, POT_ETORS = 1, 2, 4, 8, 16
+NUM_PARAMS = 33  # REBO pair {Q, A, alpha, B1-3, beta1-3} x (CC, CH, HH), then LJ {eps, sigma} x 3
 _lib = None
 
 
@@ -48,6 +49,8 @@ def lib():
         L.orc_spline_eval.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_int, dp]
         L.orc_set_alter_exact.argtypes = [ctypes.c_int]
+        L.orc_set_params.argtypes = [ctypes.c_void_p, dp]
+        L.orc_get_params.argtypes = [ctypes.c_void_p, dp]
         _lib = L
     return _lib
 
@@ -103,6 +106,17 @@ class Oracle:
         g = np.zeros((self.n, 3))
         self._chk(lib().orc_energy_grad(self.h, _p(e), _p(g)))
         return e, g
+
+    def params(self):
+        """The 33 differentiable parameters (REBO pair and LJ, CGS)."""
+        th = np.zeros(NUM_PARAMS)
+        self._chk(lib().orc_get_params(self.h, _p(th)))
+        return th
+
+    def set_params(self, theta):
+        th = np.ascontiguousarray(theta, np.float64)
+        assert th.shape == (NUM_PARAMS,)
+        self._chk(lib().orc_set_params(self.h, _p(th)))
 
     def neighbour_lists(self, term: int):
         counts = np.zeros(self.n, np.int32)

@@ -116,6 +116,56 @@ int orc_energy_grad(void* p, double* e_terms, double* grad) {
   });
 }
 
+// Parameter override (the harness-subclass route: REBO's pair parameters are
+// protected members, REBO.hpp:155-175; the LJ ones private, AIREBO_LJ.hpp).
+// theta layout: for p = C-C, C-H, H-H, {Q, A, alpha, B1-3, beta1-3} at 9p and
+// {epsilon, sigma} at 27 + 2p; sigma also sets RLJ = [sigma, 2^(1/6) sigma] as
+// AIREBO::setupPotential does (AIREBO_LJ.cxx:239-247).
+static void theta_rw(Handle* h, double* th, bool write) {
+  const int pa[3] = {REBO::C, REBO::C, REBO::H}, pb[3] = {REBO::C, REBO::H, REBO::H};
+  for (Potential* pot : h->sl.fpot.pots) {
+    if (auto* r = dynamic_cast<REBO*>(pot)) {
+      for (int p = 0; p < 3; p++) {
+        int a = pa[p], b = pb[p];
+        double* v[9] = {&r->Q_[a][b], &r->A_[a][b], &r->alpha_[a][b], &r->B_[0][a][b], &r->B_[1][a][b],
+                        &r->B_[2][a][b], &r->beta_[0][a][b], &r->beta_[1][a][b], &r->beta_[2][a][b]};
+        double* w[9] = {&r->Q_[b][a], &r->A_[b][a], &r->alpha_[b][a], &r->B_[0][b][a], &r->B_[1][b][a],
+                        &r->B_[2][b][a], &r->beta_[0][b][a], &r->beta_[1][b][a], &r->beta_[2][b][a]};
+        for (int k = 0; k < 9; k++) {
+          if (write) *v[k] = *w[k] = th[9 * p + k];
+          else if (!dynamic_cast<ETors*>(pot)) th[9 * p + k] = *v[k];
+        }
+      }
+    }
+    if (auto* l = dynamic_cast<AIREBO_LJ*>(pot)) {
+      for (int p = 0; p < 3; p++) {
+        int a = pa[p], b = pb[p];
+        if (write) {
+          l->zeta_[a][b] = l->zeta_[b][a] = th[27 + 2 * p];
+          l->sigma_[a][b] = l->sigma_[b][a] = th[28 + 2 * p];
+          for (int q = 0; q < 2; q++) {
+            int x = q ? b : a, y = q ? a : b;
+            l->RLJ_[x][y][0] = l->sigma_[x][y];
+            l->RLJ_[x][y][1] = l->sigma_[x][y] * std::pow(2.0, 1.0 / 6.0);
+          }
+        } else {
+          th[27 + 2 * p] = l->zeta_[a][b];
+          th[28 + 2 * p] = l->sigma_[a][b];
+        }
+      }
+    }
+  }
+}
+
+int orc_set_params(void* p, const double* theta) {
+  std::vector<double> th(theta, theta + 33);
+  return guarded([&] { theta_rw(static_cast<Handle*>(p), th.data(), true); });
+}
+
+int orc_get_params(void* p, double* theta) {
+  return guarded([&] { theta_rw(static_cast<Handle*>(p), theta, false); });
+}
+
 // Neighbour list of chain entry `term`: counts[n], then the concatenated lists
 // (ascending index per atom) into idx (capacity cap).  Returns -1 on overflow.
 int orc_nl(void* p, int term, int* counts, int* idx, long cap) {
