@@ -70,27 +70,22 @@ def parse():
 
 
 def timed(eng, steps, warmup, dt):
-    """Milliseconds for `steps` Verlet iterations on the engine's stream."""
-    import torch
-    stream = torch.cuda.Stream()
-    eng.set_stream(stream.cuda_stream)
+    """Milliseconds for `steps` Verlet iterations between two HIP events on the
+    engine's stream, with the device synchronised before and after."""
     eng.run(max(warmup, 1), fixed_dt=dt)
-    torch.cuda.synchronize()
-    t0 = torch.cuda.Event(enable_timing=True)
-    t1 = torch.cuda.Event(enable_timing=True)
-    t0.record(stream)
+    eng.synchronize()
+    eng.record(0)
     res = eng.run(steps, fixed_dt=dt)
-    t1.record(stream)
-    torch.cuda.synchronize()
-    return t0.elapsed_time(t1), res, stream
+    eng.record(1)
+    eng.synchronize()
+    return eng.elapsed_ms(0, 1), res
 
 
 def kernel_times(eng, steps, dt):
     """Per-kernel HIP-event times over `steps` eager iterations (same stream)."""
-    import torch
     eng.profile(True)
     eng.run(steps, fixed_dt=dt)
-    torch.cuda.synchronize()
+    eng.synchronize()
     kt = {k: eng.kernel_time(k) for k in KERNELS}
     eng.profile(False)
     return kt
@@ -178,7 +173,7 @@ def main():
 
     # headline: mixed + batched, one graph launch per step for all trajectories
     emx = Engine.from_systems(batch, potentials=POTS, etors_compat=False, precision="mixed")
-    ms_mx, res_mx, _ = timed(emx, a.steps, a.warmup, dt)
+    ms_mx, res_mx = timed(emx, a.steps, a.warmup, dt)
     v_mx = natoms * a.steps / (ms_mx / 1e3)
     log(f"mixed batched: {ms_mx / a.steps:.3f} ms/step, {v_mx:.4e} atom-steps/s")
     kt_mx = kernel_times(emx, a.steps, dt)
@@ -187,7 +182,7 @@ def main():
 
     # FP64 batched (same kernels in FP64)
     e64 = Engine.from_systems(batch, potentials=POTS, etors_compat=False)
-    ms_64b, _, _ = timed(e64, a.steps, a.warmup, dt)
+    ms_64b, _ = timed(e64, a.steps, a.warmup, dt)
     v_64b = natoms * a.steps / (ms_64b / 1e3)
     log(f"fp64 batched: {ms_64b / a.steps:.3f} ms/step, {v_64b:.4e} atom-steps/s")
     del e64
@@ -197,7 +192,7 @@ def main():
     tot_ms, tot_as = 0.0, 0
     for s in sample:
         e = Engine.from_system(s, potentials=POTS, etors_compat=False)
-        ms, _, _ = timed(e, a.steps, a.warmup, dt)
+        ms, _ = timed(e, a.steps, a.warmup, dt)
         tot_ms += ms
         tot_as += s.n * a.steps
         del e
@@ -244,7 +239,7 @@ def main():
         pe.vel = S.maxwell_velocities(pe, 300.0, stream=0)
         for prec in ("fp64", "mixed"):
             e = Engine.from_system(pe, potentials=("REBO", "AIREBO", "ETors"), precision=prec)
-            ms_pe, _, _ = timed(e, a.steps, a.warmup, 0.2e-15)
+            ms_pe, _ = timed(e, a.steps, a.warmup, 0.2e-15)
             out[f"pe110k_{prec}_value"] = pe.n * a.steps / (ms_pe / 1e3)
             del e
         log(f"pe110k fp64 {out['pe110k_fp64_value']:.4e}, mixed {out['pe110k_mixed_value']:.4e} atom-steps/s")

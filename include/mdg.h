@@ -142,8 +142,10 @@ int mdg_step_nve(mdg_ctx* ctx, int nsteps, double fixed_dt, double* epot, double
 int mdg_get_state(mdg_ctx* ctx, double* xyz, double* vel, double* acc, int32_t* pbc_count);
 
 /* Runs the context's kernels on the given HIP stream (hipStream_t passed as
- * void*; null restores the context's own stream).  No reference counterpart:
- * lets a caller time or order the work on its own stream. */
+ * void*; null restores the context's own stream).  The stream must come from
+ * the HIP runtime this library is linked against (a framework that bundles
+ * its own HIP runtime has streams this library cannot use).  No reference
+ * counterpart: lets a caller order the work on its own stream. */
 int mdg_set_stream(mdg_ctx* ctx, void* stream);
 
 /* Per-kernel timing with HIP events on the context's stream: on != 0 starts
@@ -152,6 +154,14 @@ int mdg_set_stream(mdg_ctx* ctx, void* stream);
  * counterpart (instrumentation). */
 int mdg_profile(mdg_ctx* ctx, int on);
 int mdg_kernel_time(mdg_ctx* ctx, const char* kernel, double* total_ms, int64_t* launches);
+
+/* Timing with HIP events on the context's stream: mdg_event_record records
+ * event `slot` (0-7); mdg_event_elapsed waits for event b and returns the
+ * milliseconds from a to b; mdg_synchronize waits for the stream.  No
+ * reference counterpart (instrumentation). */
+int mdg_event_record(mdg_ctx* ctx, int slot);
+int mdg_event_elapsed(mdg_ctx* ctx, int a, int b, double* ms);
+int mdg_synchronize(mdg_ctx* ctx);
 
 /* Interaction counts of the last force call (for roofline accounting):
  * out[0] bonds, out[1] LJ pairs inside the cutoff, out[2] b* evaluations,

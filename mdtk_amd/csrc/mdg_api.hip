@@ -80,6 +80,7 @@ struct mdg_ctx {
 
   bool profiling = false;
   std::map<std::string, KStat> kst;
+  hipEvent_t timer_ev[8] = {};
   std::vector<hipEvent_t> event_pool;
 
   int* stepidx = nullptr;
@@ -97,6 +98,8 @@ struct mdg_ctx {
         hipEventDestroy(pr.first);
         hipEventDestroy(pr.second);
       }
+    for (auto& e : timer_ev)
+      if (e) hipEventDestroy(e);
     for (void* p : allocs) hipFree(p);
     if (dT) hipFree(dT);
     if (dTf) hipFree(dTf);
@@ -357,7 +360,7 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   int bgrid = blocks(2LL * n, 256, 8192);
   if (c->slot_rebo >= 0) {
     launch(c, "k_rebo_bonds", [&] {
-      k_rebo_bonds<R, TH><<<bgrid, 256, 0, c->stream>>>(A, T, c->box, c->bonds, c->nbonds, A.grad,
+      k_rebo_bonds<R, TH><<<bgrid, 256, 0, c->stream>>>(A, T, c->dT, c->box, c->bonds, c->nbonds, A.grad,
                                                         c->eatom + (size_t)c->slot_rebo * n, c->derr, dth);
     });
   }
@@ -1028,6 +1031,33 @@ int mdg_kernel_time(mdg_ctx* c, const char* kernel, double* total_ms, int64_t* l
   }
   *total_ms = it->second.ms;
   *launches = it->second.launches;
+  return 0;
+}
+
+int mdg_event_record(mdg_ctx* c, int slot) {
+  if (slot < 0 || slot >= 8) {
+    c->err = "event slot out of range (0-7)";
+    return 1;
+  }
+  if (!c->timer_ev[slot]) HIPCHK(c, hipEventCreate(&c->timer_ev[slot]));
+  HIPCHK(c, hipEventRecord(c->timer_ev[slot], c->stream));
+  return 0;
+}
+
+int mdg_event_elapsed(mdg_ctx* c, int a, int b, double* ms) {
+  if (a < 0 || a >= 8 || b < 0 || b >= 8 || !c->timer_ev[a] || !c->timer_ev[b]) {
+    c->err = "event not recorded";
+    return 1;
+  }
+  HIPCHK(c, hipEventSynchronize(c->timer_ev[b]));
+  float v = 0;
+  HIPCHK(c, hipEventElapsedTime(&v, c->timer_ev[a], c->timer_ev[b]));
+  *ms = v;
+  return 0;
+}
+
+int mdg_synchronize(mdg_ctx* c) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
 

@@ -525,74 +525,80 @@ __device__ void bo_backward(const Atoms& A, const TablesT<R>* __restrict__ T, co
 
 // ------------------------------------------------------------ REBO bonds
 // REBO::operator() (REBO.cxx:32-65): E += VR + b_ij * VA for every bond.
+// The pair terms VR, VA and their radial derivatives are evaluated in FP64
+// from the FP64 table T64 in both forms: near equilibrium they are two
+// ~100 eV/A forces that cancel, so FP32's 1e-7 relative rounding would leave
+// ~1e-5 eV/A per bond.  The bond order b_ij (and its adjoint) runs in R.
 // TH: also accumulate dE/dtheta of the pair parameters into dth (FP64 only).
 template <class R, bool TH>
-__global__ void __launch_bounds__(256) k_rebo_bonds(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
+__global__ void __launch_bounds__(256) k_rebo_bonds(Atoms A, const TablesT<R>* __restrict__ T,
+                                                   const Tables* __restrict__ T64, Box bx,
                                                    const int2* __restrict__ bonds, const int* __restrict__ nbonds,
                                                    double* __restrict__ grad, double* __restrict__ eatom,
                                                    unsigned* err, double* __restrict__ dth) {
   const double gs = T->eunit / T->lunit;
+  const double su = 1.0 / T->eunit;  // erg -> the table's energy unit (adjoint seed)
   int nbt = min(*nbonds, A.bond_cap);
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nbt; t += gridDim.x * blockDim.x) {
-    const ReboParamsT<R>& P = T->rebo;
+    const ReboParamsT<double>& P = T64->rebo;
     int2 bd = bonds[t];
     BPair<R> p;
     p.i = bd.x;
     p.j = bd.y;
     p.ti = A.type[p.i];
     p.tj = A.type[p.j];
-    p.rv = to_local<R>(depos(A.x, A.y, A.z, p.i, p.j, bx));
+    D3 rd = depos(A.x, A.y, A.z, p.i, p.j, bx);
+    p.rv = to_local<R>(rd);
     p.r = sqrt(norm2(p.rv));
-    R f, df;
-    cutoff<R>(p.r, P.R[p.ti][p.tj][0], P.R[p.ti][p.tj][1], f, df);
-    p.f = f;
-    p.reff = p.r;
     int a = p.ti, b = p.tj;
-    R r = p.r;
+    double r = sqrt(norm2(rd));
+    double f, df;
+    cutoff<double>(r, P.R[a][b][0], P.R[a][b][1], f, df);
+    p.f = R(f);
+    p.reff = p.r;
     // VR (REBO.cxx:132-160) and VA (:162-198)
-    if (f != R(0)) {
-      R cr = R(0);
-      R t1 = P.A[a][b] * exp(-P.alpha[a][b] * r);
-      R val = (R(1) + P.Q[a][b] / r) * t1;
-      R der = -t1 * (P.Q[a][b] + P.alpha[a][b] * r * r + P.alpha[a][b] * r * P.Q[a][b]) / (r * r);
-      R VR = f * val;
+    if (f != 0.0) {
+      double cr = 0.0;
+      double t1 = P.A[a][b] * exp(-P.alpha[a][b] * r);
+      double val = (1.0 + P.Q[a][b] / r) * t1;
+      double der = -t1 * (P.Q[a][b] + P.alpha[a][b] * r * r + P.alpha[a][b] * r * P.Q[a][b]) / (r * r);
+      double VR = f * val;
       cr += der * f + val * df;
-      R u1 = P.B[0][a][b] * exp(-P.beta[0][a][b] * r);
-      R u2 = P.B[1][a][b] * exp(-P.beta[1][a][b] * r);
-      R u3 = P.B[2][a][b] * exp(-P.beta[2][a][b] * r);
-      R vsum = u1 + u2 + u3;
-      R VA = -f * vsum;
-      double eb = double(VR);
+      double u1 = P.B[0][a][b] * exp(-P.beta[0][a][b] * r);
+      double u2 = P.B[1][a][b] * exp(-P.beta[1][a][b] * r);
+      double u3 = P.B[2][a][b] * exp(-P.beta[2][a][b] * r);
+      double vsum = u1 + u2 + u3;
+      double VA = -f * vsum;
+      double eb = VR;
       double bb = 0.0;
-      if (VA != R(0) || TH) {
+      if (VA != 0.0 || TH) {
         BOState st;
         bo_forward<R>(A, T, p, st, err);
         bb = st.b;
-        if (VA != R(0)) {
-          bo_backward<R>(A, T, p, st, double(VA), grad, gs, err);
-          R vder = (-P.beta[0][a][b]) * u1 + (-P.beta[1][a][b]) * u2 + (-P.beta[2][a][b]) * u3;
-          R bR = R(bb);
-          cr += (-vder * f * bR) + (-vsum * bR) * df;
-          eb += bb * double(VA);
+        if (VA != 0.0) {
+          bo_backward<R>(A, T, p, st, VA * su, grad, gs, err);
+          double vder = (-P.beta[0][a][b]) * u1 + (-P.beta[1][a][b]) * u2 + (-P.beta[2][a][b]) * u3;
+          cr += (-vder * f * bb) + (-vsum * bb) * df;
+          eb += bb * VA;
         }
       }
       if constexpr (TH) {
         // VR = f (1 + Q/r) A e^{-alpha r}, b VA = -b f sum_n B_n e^{-beta_n r}
         double* th = theta_of(dth, A, p.i) + 9 * (a + b);
-        double ea = exp(-double(P.alpha[a][b]) * double(r));
-        unsafeAtomicAdd(th + 0, double(f) * double(P.A[a][b]) * ea / double(r));
-        unsafeAtomicAdd(th + 1, double(f) * (1.0 + double(P.Q[a][b]) / double(r)) * ea);
-        unsafeAtomicAdd(th + 2, -double(r) * double(VR));
+        double ea = exp(-P.alpha[a][b] * r);
+        unsafeAtomicAdd(th + 0, f * P.A[a][b] * ea / r);
+        unsafeAtomicAdd(th + 1, f * (1.0 + P.Q[a][b] / r) * ea);
+        unsafeAtomicAdd(th + 2, -r * VR);
         for (int n = 0; n < 3; n++) {
-          double en = exp(-double(P.beta[n][a][b]) * double(r));
-          unsafeAtomicAdd(th + 3 + n, -bb * double(f) * en);
-          unsafeAtomicAdd(th + 6 + n, bb * double(f) * double(P.B[n][a][b]) * double(r) * en);
+          double en = exp(-P.beta[n][a][b] * r);
+          unsafeAtomicAdd(th + 3 + n, -bb * f * en);
+          unsafeAtomicAdd(th + 6 + n, bb * f * P.B[n][a][b] * r * en);
         }
       }
-      add_energy(eatom, p.i, to_erg<R>(eb, T->eunit));
-      V3<R> v = p.rv * (cr / r);
-      scatter3<R>(grad, p.i, v, gs);
-      scatter3<R>(grad, p.j, -v, gs);
+      add_energy(eatom, p.i, eb);
+      D3 v = rd * (cr / r);
+      atomic_add3(grad, p.i, v);
+      atomic_add3(grad, p.j, -v);
     }
   }
 }
@@ -1021,8 +1027,8 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
 }
 
 #define MDG_INSTANTIATE_TH(R, TH)                                                                              \
-  template __global__ void k_rebo_bonds<R, TH>(Atoms, const TablesT<R>*, Box, const int2*, const int*,       \
-                                               double*, double*, unsigned*, double*);                          \
+  template __global__ void k_rebo_bonds<R, TH>(Atoms, const TablesT<R>*, const Tables*, Box, const int2*,     \
+                                               const int*, double*, double*, unsigned*, double*);              \
   template __global__ void k_lj_wave<R, TH>(Atoms, const TablesT<R>*, Box, double*, double*, int4*, int*,    \
                                             int, long long*, unsigned*, double*);                              \
   template __global__ void k_bstar<R, TH>(Atoms, const TablesT<R>*, Box, const int4*, const int*, int,       \

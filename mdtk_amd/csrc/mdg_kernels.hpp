@@ -59,8 +59,8 @@ __global__ void k_coord(Atoms A, const TablesT<R>* T, Box bx, unsigned* err);
 // TH = true (FP64 only): also accumulate dE/dtheta per system into dth
 // (kNumTheta doubles per system).
 template <class R, bool TH>
-__global__ void k_rebo_bonds(Atoms A, const TablesT<R>* T, Box bx, const int2* bonds, const int* nbonds,
-                             double* grad, double* eatom, unsigned* err, double* dth);
+__global__ void k_rebo_bonds(Atoms A, const TablesT<R>* T, const Tables* T64, Box bx, const int2* bonds,
+                             const int* nbonds, double* grad, double* eatom, unsigned* err, double* dth);
 template <class R, bool TH>
 __global__ void k_lj_wave(Atoms A, const TablesT<R>* T, Box bx, double* grad, double* eatom, int4* queue,
                           int* qcount, int qcap, long long* npairs, unsigned* err, double* dth);

@@ -69,6 +69,9 @@ def load_library(path: str = LIB_PATH):
     L.mdg_profile.argtypes = [vp, ctypes.c_int]
     L.mdg_kernel_time.argtypes = [vp, ctypes.c_char_p, dp, ctypes.POINTER(ctypes.c_int64)]
     L.mdg_counters.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    L.mdg_event_record.argtypes = [vp, ctypes.c_int]
+    L.mdg_event_elapsed.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
+    L.mdg_synchronize.argtypes = [vp]
     _lib = L
     return L
 
@@ -233,6 +236,18 @@ class Engine:
         cnt = ctypes.c_int64()
         self._chk(_lib.mdg_kernel_time(self._h, name.encode(), ctypes.byref(ms), ctypes.byref(cnt)))
         return ms.value, cnt.value
+
+    def record(self, slot: int):
+        """Records HIP event `slot` (0-7) on the engine's stream."""
+        self._chk(_lib.mdg_event_record(self._h, int(slot)))
+
+    def elapsed_ms(self, a: int, b: int) -> float:
+        ms = ctypes.c_double()
+        self._chk(_lib.mdg_event_elapsed(self._h, int(a), int(b), ctypes.byref(ms)))
+        return ms.value
+
+    def synchronize(self):
+        self._chk(_lib.mdg_synchronize(self._h))
 
     def counters(self):
         out = (ctypes.c_int64 * 8)()
