@@ -155,6 +155,16 @@ int mdg_set_stream(mdg_ctx* ctx, void* stream);
 int mdg_profile(mdg_ctx* ctx, int on);
 int mdg_kernel_time(mdg_ctx* ctx, const char* kernel, double* total_ms, int64_t* launches);
 
+/* Autotuner: times candidate launch shapes of the force kernels on the
+ * context's own systems (reps force calls per candidate, which leave the
+ * trajectory unchanged) and keeps the fastest for every later call: the
+ * waves-per-SIMD variant (1 or 2) of the bond-order kernels and the grid sizes
+ * of the bond, LJ, b* and torsion kernels.  report (may be null) receives 8
+ * values: occ_bonds, grid_bonds, occ_bstar, grid_bstar, grid_lj, grid_tors
+ * (0 = built-in heuristic), and the tuned kernels' summed ms per call before
+ * and after.  No reference counterpart. */
+int mdg_autotune(mdg_ctx* ctx, int reps, double* report);
+
 /* Timing with HIP events on the context's stream: mdg_event_record records
  * event `slot` (0-7); mdg_event_elapsed waits for event b and returns the
  * milliseconds from a to b; mdg_synchronize waits for the stream.  No

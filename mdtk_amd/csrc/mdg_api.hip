@@ -25,8 +25,11 @@ struct KStat {
   long long launches = 0;
 };
 
-struct Buf {
-  void* p = nullptr;
+// Launch shapes of the force kernels (0 = the built-in heuristic); set by
+// mdg_autotune from timings on the context's own system.
+struct Tune {
+  int occ_bonds = 1, occ_bstar = 1;  // waves/SIMD variant of the bond-order kernels
+  int grid_bonds = 0, grid_bstar = 0, grid_lj = 0, grid_tors = 0;
 };
 }  // namespace
 
@@ -78,6 +81,7 @@ struct mdg_ctx {
   unsigned* derr = nullptr;
   long iteration = 0;
 
+  Tune tune;
   bool profiling = false;
   std::map<std::string, KStat> kst;
   hipEvent_t timer_ev[8] = {};
@@ -357,11 +361,21 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
       k_bond_enum<<<blocks(n), 256, 0, c->stream>>>(A, c->bonds, c->nbonds, c->bond_cap, c->derr);
     });
   }
-  int bgrid = blocks(2LL * n, 256, 8192);
+  const Tune& tu = c->tune;
+  int bgrid = tu.grid_bonds ? tu.grid_bonds : blocks(2LL * n, 256, 8192);
+  int tgrid = tu.grid_tors ? tu.grid_tors : blocks(2LL * n, 256, 8192);
+  int lgrid = tu.grid_lj ? tu.grid_lj : blocks(n, 4, 65536);
+  int sgrid = tu.grid_bstar ? tu.grid_bstar : blocks(8LL * n, 256, 16384);
   if (c->slot_rebo >= 0) {
     launch(c, "k_rebo_bonds", [&] {
-      k_rebo_bonds<R, TH><<<bgrid, 256, 0, c->stream>>>(A, T, c->dT, c->box, c->bonds, c->nbonds, A.grad,
-                                                        c->eatom + (size_t)c->slot_rebo * n, c->derr, dth);
+      auto go = [&](auto occ) {
+        k_rebo_bonds<R, TH, decltype(occ)::value><<<bgrid, 256, 0, c->stream>>>(
+            A, T, c->dT, c->box, c->bonds, c->nbonds, A.grad, c->eatom + (size_t)c->slot_rebo * n, c->derr, dth);
+      };
+      if constexpr (!TH) {
+        if (tu.occ_bonds == 2) return go(std::integral_constant<int, 2>());
+      }
+      go(std::integral_constant<int, 1>());
     });
   }
   if (c->slot_bren >= 0) {
@@ -372,18 +386,24 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   }
   if (c->slot_lj >= 0) {
     launch(c, "k_lj_wave", [&] {
-      k_lj_wave<R, TH><<<blocks(n, 4, 65536), 256, 0, c->stream>>>(
+      k_lj_wave<R, TH><<<lgrid, 256, 0, c->stream>>>(
           A, T, c->box, A.grad, c->eatom + (size_t)c->slot_lj * n, c->queue, c->qcount, c->qcap, c->counters + 2,
           c->derr, dth);
     });
     launch(c, "k_bstar", [&] {
-      k_bstar<R, TH><<<blocks(8LL * n, 256, 16384), 256, 0, c->stream>>>(
-          A, T, c->box, c->queue, c->qcount, c->qcap, A.grad, c->eatom + (size_t)c->slot_lj * n, c->derr, dth);
+      auto go = [&](auto occ) {
+        k_bstar<R, TH, decltype(occ)::value><<<sgrid, 256, 0, c->stream>>>(
+            A, T, c->box, c->queue, c->qcount, c->qcap, A.grad, c->eatom + (size_t)c->slot_lj * n, c->derr, dth);
+      };
+      if constexpr (!TH) {
+        if (tu.occ_bstar == 2) return go(std::integral_constant<int, 2>());
+      }
+      go(std::integral_constant<int, 1>());
     });
   }
   if (c->slot_tors >= 0) {
     launch(c, "k_etors", [&] {
-      k_etors<R><<<bgrid, 256, 0, c->stream>>>(A, T, c->box, c->bonds, c->nbonds, c->cfg.etors_compat, A.grad,
+      k_etors<R><<<tgrid, 256, 0, c->stream>>>(A, T, c->box, c->bonds, c->nbonds, c->cfg.etors_compat, A.grad,
                                                c->eatom + (size_t)c->slot_tors * n, c->counters + 3);
     });
   }
@@ -1032,6 +1052,85 @@ int mdg_kernel_time(mdg_ctx* c, const char* kernel, double* total_ms, int64_t* l
   *total_ms = it->second.ms;
   *launches = it->second.launches;
   return 0;
+}
+
+int mdg_autotune(mdg_ctx* c, int reps, double* report) {
+  if (c->n == 0) {
+    c->err = "no system";
+    return 1;
+  }
+  if (reps < 1) reps = 1;
+  bool was = c->profiling;
+  c->profiling = true;
+  collect_profile(c);
+  c->kst.clear();
+  // Force calls leave the trajectory's state untouched (positions, velocities,
+  // lists between rebuilds), so candidates are timed on repeated calls.
+  auto measure = [&](const char* kernel) -> double {
+    c->kst.clear();
+    for (int r = 0; r < reps; r++) force_call(c);
+    collect_profile(c);
+    auto it = c->kst.find(kernel);
+    return it == c->kst.end() ? 0.0 : it->second.ms / reps;
+  };
+  force_call(c);  // lists, bond array and queue sizes for the candidates
+  if (check_err(c)) {
+    c->profiling = was;
+    return 1;
+  }
+  int nb = 0, nq = 0;
+  HIPCHK(c, hipMemcpy(&nb, c->nbonds, 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(&nq, c->qcount, 4, hipMemcpyDeviceToHost));
+  auto grids = [](long long units, int per_block) {
+    std::vector<int> g;
+    for (int v : {1024, 2048, 4096, 8192, 16384, 32768}) g.push_back(v);
+    long long one = (units + per_block - 1) / per_block;
+    if (one >= 1 && one < (1 << 20)) g.push_back((int)one);
+    return g;
+  };
+  double before = 0, after = 0;
+  auto tune_knob = [&](const char* kernel, int& knob, const std::vector<int>& cand) {
+    double t0 = measure(kernel);
+    before += t0;
+    int best = knob;
+    double bt = t0;
+    for (int v : cand) {
+      int keep = knob;
+      knob = v;
+      double t = measure(kernel);
+      if (t < bt) {
+        bt = t;
+        best = v;
+      }
+      knob = keep;
+    }
+    knob = best;
+    after += bt;
+  };
+  Tune& tu = c->tune;
+  if (c->slot_rebo >= 0) {
+    tune_knob("k_rebo_bonds", tu.occ_bonds, {1, 2});
+    tune_knob("k_rebo_bonds", tu.grid_bonds, grids(nb, 256));
+  }
+  if (c->slot_lj >= 0) {
+    tune_knob("k_lj_wave", tu.grid_lj, grids(c->n, 4));
+    tune_knob("k_bstar", tu.occ_bstar, {1, 2});
+    tune_knob("k_bstar", tu.grid_bstar, grids(nq, 256));
+  }
+  if (c->slot_tors >= 0) tune_knob("k_etors", tu.grid_tors, grids(nb, 256));
+  c->kst.clear();
+  c->profiling = was;
+  for (auto& e : c->graph_exec)  // captured launch shapes are stale now
+    if (e) {
+      hipGraphExecDestroy(e);
+      e = nullptr;
+    }
+  if (report) {
+    double r[8] = {(double)tu.occ_bonds, (double)tu.grid_bonds, (double)tu.occ_bstar, (double)tu.grid_bstar,
+                   (double)tu.grid_lj,   (double)tu.grid_tors,  before,                after};
+    for (int q = 0; q < 8; q++) report[q] = r[q];
+  }
+  return check_err(c);
 }
 
 int mdg_event_record(mdg_ctx* c, int slot) {

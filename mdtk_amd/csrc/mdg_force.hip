@@ -530,8 +530,9 @@ __device__ void bo_backward(const Atoms& A, const TablesT<R>* __restrict__ T, co
 // ~100 eV/A forces that cancel, so FP32's 1e-7 relative rounding would leave
 // ~1e-5 eV/A per bond.  The bond order b_ij (and its adjoint) runs in R.
 // TH: also accumulate dE/dtheta of the pair parameters into dth (FP64 only).
-template <class R, bool TH>
-__global__ void __launch_bounds__(256) k_rebo_bonds(Atoms A, const TablesT<R>* __restrict__ T,
+// OCC: waves per SIMD the register allocation is held to (a tuning choice).
+template <class R, bool TH, int OCC>
+__global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<R>* __restrict__ T,
                                                    const Tables* __restrict__ T64, Box bx,
                                                    const int2* __restrict__ bonds, const int* __restrict__ nbonds,
                                                    double* __restrict__ grad, double* __restrict__ eatom,
@@ -839,8 +840,8 @@ __global__ void __launch_bounds__(256) k_lj_wave(Atoms A, const TablesT<R>* __re
 }
 
 // LJ pairs that need b*_ij (S_r != 0): the whole StrStb / VLJ / Cij chain.
-template <class R, bool TH>
-__global__ void __launch_bounds__(256) k_bstar(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
+template <class R, bool TH, int OCC>
+__global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
                                               const int4* __restrict__ queue, const int* __restrict__ qcount,
                                               int qcap, double* __restrict__ grad, double* __restrict__ eatom,
                                               unsigned* err, double* __restrict__ dth) {
@@ -1026,13 +1027,14 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
   if (nq) atomicAdd((unsigned long long*)nquads, (unsigned long long)nq);
 }
 
+#define MDG_INSTANTIATE_OCC(R, TH, OCC)                                                                        \
+  template __global__ void k_rebo_bonds<R, TH, OCC>(Atoms, const TablesT<R>*, const Tables*, Box, const int2*, \
+                                                    const int*, double*, double*, unsigned*, double*);         \
+  template __global__ void k_bstar<R, TH, OCC>(Atoms, const TablesT<R>*, Box, const int4*, const int*, int,  \
+                                               double*, double*, unsigned*, double*);
 #define MDG_INSTANTIATE_TH(R, TH)                                                                              \
-  template __global__ void k_rebo_bonds<R, TH>(Atoms, const TablesT<R>*, const Tables*, Box, const int2*,     \
-                                               const int*, double*, double*, unsigned*, double*);              \
   template __global__ void k_lj_wave<R, TH>(Atoms, const TablesT<R>*, Box, double*, double*, int4*, int*,    \
-                                            int, long long*, unsigned*, double*);                              \
-  template __global__ void k_bstar<R, TH>(Atoms, const TablesT<R>*, Box, const int4*, const int*, int,       \
-                                          double*, double*, unsigned*, double*);
+                                            int, long long*, unsigned*, double*);
 #define MDG_INSTANTIATE(R)                                                                                     \
   template __global__ void k_coord<R>(Atoms, const TablesT<R>*, Box, unsigned*);                              \
   template __global__ void k_etors<R>(Atoms, const TablesT<R>*, Box, const int2*, const int*, int, double*,   \
@@ -1042,5 +1044,10 @@ MDG_INSTANTIATE(float)
 MDG_INSTANTIATE_TH(double, false)
 MDG_INSTANTIATE_TH(double, true)
 MDG_INSTANTIATE_TH(float, false)
+MDG_INSTANTIATE_OCC(double, false, 1)
+MDG_INSTANTIATE_OCC(double, false, 2)
+MDG_INSTANTIATE_OCC(double, true, 1)
+MDG_INSTANTIATE_OCC(float, false, 1)
+MDG_INSTANTIATE_OCC(float, false, 2)
 
 }  // namespace mdg

@@ -57,14 +57,15 @@ __global__ void k_bond_enum(Atoms A, int2* out, int* count, int cap, unsigned* e
 template <class R>
 __global__ void k_coord(Atoms A, const TablesT<R>* T, Box bx, unsigned* err);
 // TH = true (FP64 only): also accumulate dE/dtheta per system into dth
-// (kNumTheta doubles per system).
-template <class R, bool TH>
+// (kNumTheta doubles per system).  OCC: waves per SIMD the bond-order
+// kernels' register allocation is held to (1 or 2; chosen by the autotuner).
+template <class R, bool TH, int OCC>
 __global__ void k_rebo_bonds(Atoms A, const TablesT<R>* T, const Tables* T64, Box bx, const int2* bonds,
                              const int* nbonds, double* grad, double* eatom, unsigned* err, double* dth);
 template <class R, bool TH>
 __global__ void k_lj_wave(Atoms A, const TablesT<R>* T, Box bx, double* grad, double* eatom, int4* queue,
                           int* qcount, int qcap, long long* npairs, unsigned* err, double* dth);
-template <class R, bool TH>
+template <class R, bool TH, int OCC>
 __global__ void k_bstar(Atoms A, const TablesT<R>* T, Box bx, const int4* queue, const int* qcount, int qcap,
                         double* grad, double* eatom, unsigned* err, double* dth);
 template <class R>

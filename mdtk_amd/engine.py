@@ -72,6 +72,7 @@ def load_library(path: str = LIB_PATH):
     L.mdg_event_record.argtypes = [vp, ctypes.c_int]
     L.mdg_event_elapsed.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
     L.mdg_synchronize.argtypes = [vp]
+    L.mdg_autotune.argtypes = [vp, ctypes.c_int, dp]
     _lib = L
     return L
 
@@ -245,6 +246,16 @@ class Engine:
         ms = ctypes.c_double()
         self._chk(_lib.mdg_event_elapsed(self._h, int(a), int(b), ctypes.byref(ms)))
         return ms.value
+
+    def autotune(self, reps: int = 3) -> dict:
+        """Picks the force kernels' launch shapes by timing candidates on this
+        engine's systems; returns the choices and the tuned kernels' ms per
+        force call before and after."""
+        r = np.zeros(8)
+        self._chk(_lib.mdg_autotune(self._h, int(reps), _p(r)))
+        keys = ["occ_bonds", "grid_bonds", "occ_bstar", "grid_bstar", "grid_lj", "grid_tors", "ms_before",
+                "ms_after"]
+        return {k: (float(v) if k.startswith("ms") else int(v)) for k, v in zip(keys, r)}
 
     def synchronize(self):
         self._chk(_lib.mdg_synchronize(self._h))

@@ -117,3 +117,24 @@ def test_batch_uneven_bombardment_runs():
         drift = np.abs(et[:, k] - et[0, k]).max() / s.n
         print(f"member {k} n={s.n}: max |dE|/atom {drift:.3e} eV", flush=True)
         assert drift < 1e-4
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_autotune_keeps_results(precision):
+    """The autotuner changes launch shapes only: forces and a trajectory after
+    tuning equal an untuned engine's (same kernels, other grid sizes and
+    register budgets; only the order of FP64 atomic sums may differ)."""
+    from mdtk_amd import Engine
+    pots = ("FBZL",) + POTS
+    mem = _members()
+    a = Engine.from_systems(mem, potentials=pots, precision=precision)
+    b = Engine.from_systems(mem, potentials=pots, precision=precision)
+    rep = b.autotune(reps=2)
+    print(f"{precision} autotune: {rep}", flush=True)
+    ea, ga = a.energy_grad()
+    eb, gb = b.energy_grad()
+    assert np.abs(ea - eb).max() <= 1e-12 * np.abs(ea).max()
+    assert force_rel_err(gb, ga) <= 1e-12
+    ra = a.run(20, fixed_dt=0.2e-15)
+    rb = b.run(20, fixed_dt=0.2e-15)
+    assert np.abs(ra["epot"] - rb["epot"]).max() <= 1e-10 * np.abs(ra["epot"]).max()
