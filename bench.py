@@ -173,6 +173,8 @@ def main():
 
     # headline: mixed + batched, one graph launch per step for all trajectories
     emx = Engine.from_systems(batch, potentials=POTS, etors_compat=False, precision="mixed")
+    tune_mx = emx.autotune(reps=2)
+    log(f"mixed batched autotune: {tune_mx}")
     ms_mx, res_mx = timed(emx, a.steps, a.warmup, dt)
     v_mx = natoms * a.steps / (ms_mx / 1e3)
     log(f"mixed batched: {ms_mx / a.steps:.3f} ms/step, {v_mx:.4e} atom-steps/s")
@@ -182,6 +184,8 @@ def main():
 
     # FP64 batched (same kernels in FP64)
     e64 = Engine.from_systems(batch, potentials=POTS, etors_compat=False)
+    tune_64 = e64.autotune(reps=2)
+    log(f"fp64 batched autotune: {tune_64}")
     ms_64b, _ = timed(e64, a.steps, a.warmup, dt)
     v_64b = natoms * a.steps / (ms_64b / 1e3)
     log(f"fp64 batched: {ms_64b / a.steps:.3f} ms/step, {v_64b:.4e} atom-steps/s")
@@ -229,6 +233,7 @@ def main():
         "fp64_dropin_sample": f"{len(sample)} trajectories, {sum(s.n for s in sample)} atoms, sequential contexts",
         "mixed_batched_over_fp64_dropin": v_mx / v_drop,
         "mixed_force_rms_err_eV_per_A": f_rms,
+        "autotune": {"mixed_batched": tune_mx, "fp64_batched": tune_64},
         "roofline": rl,
         "kernels_ms_per_step": {k: v[0] / max(a.steps, 1) for k, v in kt_mx.items()},
         "counters": cnt_mx,

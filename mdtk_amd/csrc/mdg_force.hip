@@ -555,7 +555,11 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
     double r = sqrt(norm2(rd));
     double f, df;
     cutoff<double>(r, P.R[a][b][0], P.R[a][b][1], f, df);
-    p.f = R(f);
+    {  // f_ij inside the bond order in R, as the coordination pass summed it
+      R fr, dfr;
+      cutoff<R>(p.r, T->rebo.R[a][b][0], T->rebo.R[a][b][1], fr, dfr);
+      p.f = fr;
+    }
     p.reff = p.r;
     // VR (REBO.cxx:132-160) and VA (:162-198)
     if (f != 0.0) {

@@ -97,8 +97,10 @@ def test_energy_function_backward():
     _, g, d = eng.energy_grad_params()
     off = eng.offsets
     want_x = np.concatenate([g[off[s]:off[s + 1]] * w[s].item() for s in range(2)])
-    assert np.allclose(x.grad.numpy(), want_x, rtol=1e-14, atol=0)
-    assert np.allclose(th.grad.numpy(), (d * w.numpy()[:, None]).sum(0), rtol=1e-14, atol=0)
+    # a second force call: equal up to the order of the FP64 atomic sums
+    assert np.abs(x.grad.numpy() - want_x).max() <= 1e-12 * np.abs(want_x).max()
+    want_t = (d * w.numpy()[:, None]).sum(0)
+    assert np.all(np.abs(th.grad.numpy() - want_t) <= 1e-12 * np.abs(d).max(0) + 1e-300)
 
 
 def test_force_matching_loss_gradient():
