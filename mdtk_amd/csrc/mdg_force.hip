@@ -224,7 +224,11 @@ __device__ __forceinline__ R cos_theta(V3<R> a, V3<R> b, R ra, R rb, unsigned* e
   R e2 = ra * rb;
   R c = e1 / e2;
   raw = c;
-  if (!(c >= R(-1.0 - 1e-7) && c <= R(1.0 + 1e-7))) raise(err, ERR_COS_RANGE);
+  // FGeneral.hpp:147 allows 1e-7 beyond +-1 in FP64; FP32 rounding of a
+  // near-collinear triplet alone reaches a few ulps (1.2e-7 each), so the
+  // FP32 form allows 32 ulps.
+  constexpr double tol = std::is_same<R, double>::value ? 1e-7 : 32.0 * 1.1920929e-7;
+  if (!(c >= R(-1.0 - tol) && c <= R(1.0 + tol))) raise(err, ERR_COS_RANGE);
   if (c < R(-1)) c = R(-1);
   if (c > R(1)) c = R(1);
   return c;
