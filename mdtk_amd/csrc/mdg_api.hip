@@ -286,13 +286,13 @@ void nl_build(mdg_ctx* c, bool shortlist) {
          [&] { k_cell_fill<<<blocks(n), 256, 0, c->stream>>>(flag, n, cell_of, start, cnt, cat); });
   if (shortlist) {
     launch(c, "k_nl_build_short", [&] {
-      k_nl_build<64><<<blocks(n, 4), 256, 0, c->stream>>>(flag, c->sflags, 0, n, A.x, A.y, A.z, c->handled_s,
+      k_nl_build<64><<<blocks(n, 4, 8192), 256, 0, c->stream>>>(flag, c->sflags, 0, n, A.x, A.y, A.z, c->handled_s,
                                                            c->box, g, coff, range2, cell_of, start, cat,
                                                            A.nls_cnt, A.nls, A.disp_s, c->derr);
     });
   } else {
     launch(c, "k_nl_build_long", [&] {
-      k_nl_build<512><<<blocks(n, 4), 256, 0, c->stream>>>(flag, c->sflags, 1, n, A.x, A.y, A.z, c->handled_l,
+      k_nl_build<512><<<blocks(n, 4, 8192), 256, 0, c->stream>>>(flag, c->sflags, 1, n, A.x, A.y, A.z, c->handled_l,
                                                             c->box, g, coff, range2, cell_of, start, cat,
                                                             A.nll_cnt, A.nll, A.disp_l, c->derr);
     });

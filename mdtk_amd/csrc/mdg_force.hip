@@ -671,35 +671,96 @@ __device__ __forceinline__ R dSfun(R a, R mn, R mx) {
 }
 
 constexpr int kPathCap = 320;  // path records per atom held in LDS
+constexpr int kHash = 128;     // C_ij hash slots per atom (power of two)
+constexpr int kPairCap = 256;  // LJ pairs (j > i inside the LJ cutoff) per atom
+
+// Path weights w > 0 compare like their bit patterns.
+template <class R>
+__device__ __forceinline__ unsigned long long wbits(R w) {
+  if constexpr (std::is_same<R, double>::value) return (unsigned long long)__double_as_longlong(w);
+  else return (unsigned long long)__float_as_uint(w);
+}
+template <class R>
+__device__ __forceinline__ R wfrom(unsigned long long b) {
+  if constexpr (std::is_same<R, double>::value) return __longlong_as_double((long long)b);
+  else return __uint_as_float((unsigned)b);
+}
+__device__ __forceinline__ int hslot(int t) { return (int)(((unsigned)t * 2654435761u) >> 25); }
 
 // AIREBO::operator() (AIREBO_LJ.cxx:34-77), one wavefront per atom i.
-// Phase 1: the wave enumerates i's paths of <= 3 bonds into LDS, one lane per
-// (first, second) bond prefix, with key = bonds<<24 | s1<<16 | s2<<8 | s3
-// (the reference's loop order).  Phase 2: lanes sweep i's long list (j > i,
-// coalesced), look the target up among the path records, and either finish
-// the pair (S_r = 0) or queue it for the b* kernel.
+// Phase 0: lanes sweep i's long list and ballot-compact the LJ pairs (j > i,
+// inside the LJ cutoff) into LDS.  Phase 1: one lane per (first, second) bond
+// prefix enumerates i's paths of <= 3 bonds to targets j > i, with key =
+// bonds<<24 | s1<<16 | s2<<8 | s3 (the reference's loop order), and merges
+// them into an LDS hash keyed by target: the largest weight by an atomic max
+// on its bit pattern, then, among the paths of that weight, the smallest key
+// by an atomic min -- the reference's choice whatever order the lanes run in.
+// Phase 2: lanes take the compacted pairs, look C_ij up in the hash and either
+// finish the pair (S_r = 0) or queue it for the b* kernel.
 template <class R, bool TH>
 __global__ void __launch_bounds__(256) k_lj_wave(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
                                                 double* __restrict__ grad, double* __restrict__ eatom,
                                                 int4* __restrict__ queue, int* __restrict__ qcount, int qcap,
                                                 long long* __restrict__ npairs, unsigned* err,
                                                 double* __restrict__ dth) {
-  __shared__ int s_t[4][kPathCap];
-  __shared__ int s_key[4][kPathCap];
-  __shared__ R s_w[4][kPathCap];
+  __shared__ int s_pair[4][kPairCap];
+  __shared__ int s_ht[4][kHash];                 // target of a slot, -1 when empty
+  __shared__ unsigned long long s_hw[4][kHash];  // largest path weight (bit pattern)
+  __shared__ int s_hk[4][kHash];                 // smallest key among those paths
+  __shared__ short s_rs[4][kPathCap];            // path record: hash slot, key, weight
+  __shared__ int s_rk[4][kPathCap];
+  __shared__ R s_rw[4][kPathCap];
   __shared__ int s_cnt[4];
   __shared__ int s_off[4][17];
   const LJParamsT<R>& L = T->lj;
   BondArr<R> B = barr<R>(A);
   const double gs = T->eunit / T->lunit;
   int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
   long long np = 0;
   for (int i = blockIdx.x * 4 + wv; i < A.n; i += gridDim.x * 4) {
     int ti = A.type[i];
     if (ti > T_H) continue;  // wave-uniform
     const double* Lb = box_of(bx, i);
-    double e = 0.0;
-    double th_e[2] = {0.0, 0.0}, th_s[2] = {0.0, 0.0};  // dE/d(epsilon, sigma) by tj
+    double xi = A.x[i], yi = A.y[i], zi = A.z[i];
+    // phase 0: compact the LJ pairs
+    int cnt = A.nll_cnt[i];
+    const int* list = A.nll + (size_t)i * A.cap_l;
+    int npair = 0;
+    for (int base = 0; base < cnt; base += 64) {
+      int q = base + lane;
+      bool ok = false;
+      int j = -1;
+      if (q < cnt) {
+        j = list[q];
+        if (j > i) {
+          int tj = A.type[j];
+          if (tj <= T_H) {
+            double dx = mic(xi - A.x[j], Lb[0]), dy = mic(yi - A.y[j], Lb[1]), dz = mic(zi - A.z[j], Lb[2]);
+            double d2 = dx * dx + dy * dy + dz * dz;
+            ok = !(d2 > double(L.R2sq[ti][tj]) * (T->lunit * T->lunit));
+          }
+        }
+      }
+      unsigned long long m = __ballot(ok);
+      if (ok) {
+        int slot = npair + __popcll(m & below);
+        if (slot < kPairCap) s_pair[wv][slot] = j;
+      }
+      npair += __popcll(m);
+    }
+    if (npair > kPairCap) {
+      if (lane == 0) raise(err, ERR_QUEUE_OVERFLOW);
+      npair = kPairCap;
+    }
+    if (npair == 0) continue;  // wave-uniform
+    if (lane == 0) np += npair;
+    // phase 1: paths into the hash
+    for (int q = lane; q < kHash; q += 64) {
+      s_ht[wv][q] = -1;
+      s_hw[wv][q] = 0ull;
+      s_hk[wv][q] = 0x7fffffff;
+    }
     if (lane == 0) s_cnt[wv] = 0;
     int nb1 = A.bcnt[i];
     int kk = -1, c2 = 0;
@@ -718,12 +779,23 @@ __global__ void __launch_bounds__(256) k_lj_wave(Atoms A, const TablesT<R>* __re
     __builtin_amdgcn_wave_barrier();
     int tot2 = s_off[wv][nb1];
     auto push = [&](int target, R w, int key) {
-      if (target == i || !(w > R(0))) return;
+      if (target <= i || !(w > R(0))) return;  // only targets j > i are LJ pairs of this wave
+      int h = hslot(target);
+      for (int probes = 0;; probes++) {
+        if (probes == kHash) {
+          raise(err, ERR_QUEUE_OVERFLOW);
+          return;
+        }
+        int t = atomicCAS(&s_ht[wv][h], -1, target);
+        if (t == -1 || t == target) break;
+        h = (h + 1) & (kHash - 1);
+      }
+      atomicMax(&s_hw[wv][h], wbits(w));
       int slot = atomicAdd(&s_cnt[wv], 1);
       if (slot < kPathCap) {
-        s_t[wv][slot] = target;
-        s_w[wv][slot] = w;
-        s_key[wv][slot] = key;
+        s_rs[wv][slot] = (short)h;
+        s_rk[wv][slot] = key;
+        s_rw[wv][slot] = w;
       } else {
         raise(err, ERR_QUEUE_OVERFLOW);
       }
@@ -748,30 +820,32 @@ __global__ void __launch_bounds__(256) k_lj_wave(Atoms A, const TablesT<R>* __re
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
     int npath = min(s_cnt[wv], kPathCap);
+    for (int r = lane; r < npath; r += 64) {
+      int h = s_rs[wv][r];
+      if (wbits(s_rw[wv][r]) == s_hw[wv][h]) atomicMin(&s_hk[wv][h], s_rk[wv][r]);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
     // phase 2: pairs
-    int cnt = A.nll_cnt[i];
-    const int* list = A.nll + (size_t)i * A.cap_l;
-    double xi = A.x[i], yi = A.y[i], zi = A.z[i];
+    double e = 0.0;
+    double th_e[2] = {0.0, 0.0}, th_s[2] = {0.0, 0.0};  // dE/d(epsilon, sigma) by tj
     V3<R> gi = v3<R>(0, 0, 0);
-    for (int q = lane; q < cnt; q += 64) {
-      int j = list[q];
-      if (j <= i) continue;
+    for (int q = lane; q < npair; q += 64) {
+      int j = s_pair[wv][q];
       int tj = A.type[j];
-      if (tj > T_H) continue;
       D3 rd = d3(mic(xi - A.x[j], Lb[0]), mic(yi - A.y[j], Lb[1]), mic(zi - A.z[j], Lb[2]));
-      double d2 = rd.x * rd.x + rd.y * rd.y + rd.z * rd.z;
-      if (d2 > double(L.R2sq[ti][tj]) * (T->lunit * T->lunit)) continue;
-      np++;
       R w = R(0);
       int bkey = 0x7fffffff;
-      for (int r = 0; r < npath; r++) {
-        if (s_t[wv][r] != j) continue;
-        R rw = s_w[wv][r];
-        int rk = s_key[wv][r];
-        if (rw > w || (rw == w && rk < bkey)) {
-          w = rw;
-          bkey = rk;
+      int h = hslot(j);
+      for (int probes = 0; probes < kHash; probes++) {
+        int t = s_ht[wv][h];
+        if (t == j) {
+          w = wfrom<R>(s_hw[wv][h]);
+          bkey = s_hk[wv][h];
+          break;
         }
+        if (t == -1) break;
+        h = (h + 1) & (kHash - 1);
       }
       R C = R(1) - w;
       if (!(C > R(0))) continue;
@@ -785,8 +859,13 @@ __global__ void __launch_bounds__(256) k_lj_wave(Atoms A, const TablesT<R>* __re
       V3<R> rv = to_local<R>(rd);
       R r = sqrt(norm2(rv));
       R Str = Sfun<R>(r, L.RLJ[ti][tj][0], L.RLJ[ti][tj][1]);
-      if (Str != R(0)) {
-        int slot = atomicAdd(qcount, 1);
+      if (Str != R(0)) {  // b* needed: one queue atomic per wave
+        unsigned long long m = __ballot(1);
+        int leader = __ffsll((long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(qcount, __popcll(m));
+        base = __shfl(base, leader, 64);
+        int slot = base + __popcll(m & below);
         if (slot < qcap) queue[slot] = make_int4(i, j, path.k, (path.hops << 28) | (path.m & 0x0fffffff));
         else raise(err, ERR_QUEUE_OVERFLOW);
         continue;

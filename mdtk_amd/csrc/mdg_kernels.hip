@@ -102,7 +102,9 @@ __global__ void k_cell_fill(const int* __restrict__ flag, int n, const int* __re
 // with d^2 = dx*dx+dy*dy+dz*dz of the minimum-image displacement.  Candidates
 // from the stencil cells are compacted with a ballot/popcount prefix into
 // LDS, then sorted ascending (the reference's list order) with a bitonic sort.
-// Only atoms whose system asked for a rebuild (sflags[2*s+list]) are redone.
+// Only atoms whose system asked for a rebuild (sflags[2*s+list]) are redone;
+// the grid strides over the atoms, so a launch with nothing to rebuild costs
+// a few thousand empty waves, not one per atom.
 template <int CAP>
 __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, const int* __restrict__ sflags,
                                                   int list_id, int n, const double* __restrict__ x,
@@ -116,98 +118,100 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
   if (!*flag) return;
   __shared__ int buf[4][CAP];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  int i = blockIdx.x * 4 + w;
-  if (i >= n) return;
-  int sy = bx.sys[i];
-  if (!sflags[2 * sy + list_id]) return;
-  const CellGrid& g = grids[sy];
-  const double* L = box_of(bx, i);
-  int base_cell = cell_off[sy];
-  int* my = buf[w];
-  if (lane == 0) {
-    disp[3 * i] = 0.0;
-    disp[3 * i + 1] = 0.0;
-    disp[3 * i + 2] = 0.0;
-  }
-  int m = 0;
-  if (handled[i]) {
-    double xi = x[i], yi = y[i], zi = z[i];
-    int ci = cell_of[i] - base_cell;
-    int c2 = ci % g.n[2], c1 = (ci / g.n[2]) % g.n[1], c0 = ci / (g.n[1] * g.n[2]);
-    int lo[3], hi[3], cc[3] = {c0, c1, c2};
-    for (int d = 0; d < 3; d++) {
-      if (g.n[d] >= 3) {
-        lo[d] = -1;
-        hi[d] = 1;
-      } else {
-        lo[d] = -cc[d];
-        hi[d] = g.n[d] - 1 - cc[d];
-      }
+  for (int i = blockIdx.x * 4 + w; i < n; i += gridDim.x * 4) {
+    int sy = bx.sys[i];
+    if (!sflags[2 * sy + list_id]) continue;  // wave-uniform
+    const CellGrid& g = grids[sy];
+    const double* L = box_of(bx, i);
+    int base_cell = cell_off[sy];
+    int* my = buf[w];
+    if (lane == 0) {
+      disp[3 * i] = 0.0;
+      disp[3 * i + 1] = 0.0;
+      disp[3 * i + 2] = 0.0;
     }
-    for (int a = lo[0]; a <= hi[0]; a++)
-      for (int b = lo[1]; b <= hi[1]; b++)
-        for (int c = lo[2]; c <= hi[2]; c++) {
-          int q0 = cc[0] + a, q1 = cc[1] + b, q2 = cc[2] + c;
-          if (g.periodic[0]) q0 = (q0 + g.n[0]) % g.n[0];
-          else if (q0 < 0 || q0 >= g.n[0]) continue;
-          if (g.periodic[1]) q1 = (q1 + g.n[1]) % g.n[1];
-          else if (q1 < 0 || q1 >= g.n[1]) continue;
-          if (g.periodic[2]) q2 = (q2 + g.n[2]) % g.n[2];
-          else if (q2 < 0 || q2 >= g.n[2]) continue;
-          int cid = base_cell + (q0 * g.n[1] + q1) * g.n[2] + q2;
-          int s = start[cid], e = start[cid + 1];
-          for (int base = s; base < e; base += 64) {
-            int p = base + lane;
-            bool ok = false;
-            int j = -1;
-            if (p < e) {
-              j = cell_atoms[p];
-              if (j != i && handled[j]) {
-                double dx = mic(xi - x[j], L[0]);
-                double dy = mic(yi - y[j], L[1]);
-                double dz = mic(zi - z[j], L[2]);
-                double d2 = dx * dx + dy * dy + dz * dz;
-                ok = d2 < range2;
+    int m = 0;
+    if (handled[i]) {
+      double xi = x[i], yi = y[i], zi = z[i];
+      int ci = cell_of[i] - base_cell;
+      int c2 = ci % g.n[2], c1 = (ci / g.n[2]) % g.n[1], c0 = ci / (g.n[1] * g.n[2]);
+      int lo[3], hi[3], cc[3] = {c0, c1, c2};
+      for (int d = 0; d < 3; d++) {
+        if (g.n[d] >= 3) {
+          lo[d] = -1;
+          hi[d] = 1;
+        } else {
+          lo[d] = -cc[d];
+          hi[d] = g.n[d] - 1 - cc[d];
+        }
+      }
+      for (int a = lo[0]; a <= hi[0]; a++)
+        for (int b = lo[1]; b <= hi[1]; b++)
+          for (int c = lo[2]; c <= hi[2]; c++) {
+            int q0 = cc[0] + a, q1 = cc[1] + b, q2 = cc[2] + c;
+            if (g.periodic[0]) q0 = (q0 + g.n[0]) % g.n[0];
+            else if (q0 < 0 || q0 >= g.n[0]) continue;
+            if (g.periodic[1]) q1 = (q1 + g.n[1]) % g.n[1];
+            else if (q1 < 0 || q1 >= g.n[1]) continue;
+            if (g.periodic[2]) q2 = (q2 + g.n[2]) % g.n[2];
+            else if (q2 < 0 || q2 >= g.n[2]) continue;
+            int cid = base_cell + (q0 * g.n[1] + q1) * g.n[2] + q2;
+            int s = start[cid], e = start[cid + 1];
+            for (int base = s; base < e; base += 64) {
+              int p = base + lane;
+              bool ok = false;
+              int j = -1;
+              if (p < e) {
+                j = cell_atoms[p];
+                if (j != i && handled[j]) {
+                  double dx = mic(xi - x[j], L[0]);
+                  double dy = mic(yi - y[j], L[1]);
+                  double dz = mic(zi - z[j], L[2]);
+                  double d2 = dx * dx + dy * dy + dz * dz;
+                  ok = d2 < range2;
+                }
               }
+              unsigned long long mask = __ballot(ok);
+              int before = __popcll(mask & ((1ull << lane) - 1ull));
+              if (ok) {
+                int slot = m + before;
+                if (slot < CAP) my[slot] = j;
+              }
+              m += __popcll(mask);
             }
-            unsigned long long mask = __ballot(ok);
-            int before = __popcll(mask & ((1ull << lane) - 1ull));
-            if (ok) {
-              int slot = m + before;
-              if (slot < CAP) my[slot] = j;
-            }
-            m += __popcll(mask);
           }
-        }
-  }
-  if (m > CAP) {
-    if (lane == 0) raise(err, CAP >= 256 ? ERR_LONG_NL_OVERFLOW : ERR_SHORT_NL_OVERFLOW);
-    m = CAP;
-  }
-  // pad to a power of two and sort ascending
-  int p2 = 1;
-  while (p2 < m) p2 <<= 1;
-  for (int q = m + lane; q < p2; q += 64) my[q] = 0x7fffffff;
-  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  for (int k = 2; k <= p2; k <<= 1)
-    for (int jj = k >> 1; jj > 0; jj >>= 1) {
-      for (int q = lane; q < p2; q += 64) {
-        int partner = q ^ jj;
-        if (partner > q) {
-          int a = my[q], b = my[partner];
-          bool up = ((q & k) == 0);
-          if ((a > b) == up) {
-            my[q] = b;
-            my[partner] = a;
-          }
-        }
-      }
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
     }
-  for (int q = lane; q < m; q += 64) list[(size_t)i * CAP + q] = my[q];
-  if (lane == 0) cnt[i] = m;
+    if (m > CAP) {
+      if (lane == 0) raise(err, CAP >= 256 ? ERR_LONG_NL_OVERFLOW : ERR_SHORT_NL_OVERFLOW);
+      m = CAP;
+    }
+    // pad to a power of two and sort ascending
+    int p2 = 1;
+    while (p2 < m) p2 <<= 1;
+    for (int q = m + lane; q < p2; q += 64) my[q] = 0x7fffffff;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int k = 2; k <= p2; k <<= 1)
+      for (int jj = k >> 1; jj > 0; jj >>= 1) {
+        for (int q = lane; q < p2; q += 64) {
+          int partner = q ^ jj;
+          if (partner > q) {
+            int a = my[q], b = my[partner];
+            bool up = ((q & k) == 0);
+            if ((a > b) == up) {
+              my[q] = b;
+              my[partner] = a;
+            }
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+    for (int q = lane; q < m; q += 64) list[(size_t)i * CAP + q] = my[q];
+    if (lane == 0) cnt[i] = m;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
 }
 
 template __global__ void k_nl_build<64>(const int*, const int*, int, int, const double*, const double*,
@@ -291,15 +295,25 @@ __global__ void k_clear_flags(int nsys, int* __restrict__ flags, int* __restrict
   if (i < 2 * nsys) sflags[i] = 0;
 }
 
-// Bonds (i<j) in the order of i's list, appended to one array.
+// Bonds (i<j) in the order of i's list, appended to one array; one atomic
+// per wave reserves the wave's bonds (an inclusive scan of the counts).
 __global__ void k_bond_enum(Atoms A, int2* __restrict__ out, int* __restrict__ count, int cap, unsigned* err) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.n) return;
-  int nb = A.bcnt[i];
+  int lane = threadIdx.x & 63;
+  int nb = (i < A.n) ? A.bcnt[i] : 0;
   int mine = 0;
   for (int s = 0; s < nb; s++) mine += (A.bj[i * A.maxb + s] > i);
+  int incl = mine;
+  for (int o = 1; o < 64; o <<= 1) {
+    int v = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += v;
+  }
+  int total = __shfl(incl, 63, 64);
+  int wbase = 0;
+  if (lane == 63 && total > 0) wbase = atomicAdd(count, total);
+  wbase = __shfl(wbase, 63, 64);
   if (!mine) return;
-  int base = atomicAdd(count, mine);
+  int base = wbase + incl - mine;
   for (int s = 0; s < nb; s++) {
     int j = A.bj[i * A.maxb + s];
     if (j > i) {
