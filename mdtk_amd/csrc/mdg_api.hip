@@ -29,6 +29,7 @@ struct KStat {
 // mdg_autotune from timings on the context's own system.
 struct Tune {
   int occ_bonds = 1, occ_bstar = 1;  // waves/SIMD variant of the bond-order kernels
+  int occ_lj = 1;                    // 1 or 3: waves/SIMD variant of the LJ kernel
   int grid_bonds = 0, grid_bstar = 0, grid_lj = 0, grid_tors = 0;
 };
 }  // namespace
@@ -267,7 +268,6 @@ void nl_build(mdg_ctx* c, bool shortlist) {
   const int* flag = c->flags + (shortlist ? 0 : 1);
   const CellGrid* g = shortlist ? c->gs_d : c->gl_d;
   const int* coff = shortlist ? c->coff_s : c->coff_l;
-  int nc = shortlist ? c->ncs : c->ncl;
   int* cell_of = shortlist ? c->cell_of_s : c->cell_of_l;
   int* cnt = shortlist ? c->cnt_s : c->cnt_l;
   int* start = shortlist ? c->start_s : c->start_l;
@@ -277,24 +277,31 @@ void nl_build(mdg_ctx* c, bool shortlist) {
   double range2 = rr * rr;
   int n = c->n;
   Atoms& A = c->A;
-  launch(c, "k_cell_clear", [&] { k_cell_clear<<<blocks(nc), 256, 0, c->stream>>>(flag, nc, cnt); });
+  int L = shortlist ? 0 : 1;
+  int nsys = c->nsys;
+  launch(c, "k_cell_clear",
+         [&] { k_cell_clear<<<nsys, 256, 0, c->stream>>>(flag, c->sflags, L, coff, cnt); });
   launch(c, "k_cell_assign", [&] {
-    k_cell_assign<<<blocks(n), 256, 0, c->stream>>>(flag, n, A.x, A.y, A.z, g, coff, A.sys, cell_of, cnt);
+    k_cell_assign<<<blocks(n), 256, 0, c->stream>>>(flag, c->sflags, L, n, A.x, A.y, A.z, g, coff, A.sys, cell_of,
+                                                    cnt);
   });
-  launch(c, "k_cell_scan", [&] { k_cell_scan<<<1, 1024, 0, c->stream>>>(flag, nc, cnt, start); });
-  launch(c, "k_cell_fill",
-         [&] { k_cell_fill<<<blocks(n), 256, 0, c->stream>>>(flag, n, cell_of, start, cnt, cat); });
+  launch(c, "k_cell_scan", [&] {
+    k_cell_scan<<<nsys, 1024, 0, c->stream>>>(flag, c->sflags, L, coff, A.sys_off, cnt, start);
+  });
+  launch(c, "k_cell_fill", [&] {
+    k_cell_fill<<<blocks(n), 256, 0, c->stream>>>(flag, c->sflags, L, n, A.sys, cell_of, start, cnt, cat);
+  });
   if (shortlist) {
     launch(c, "k_nl_build_short", [&] {
       k_nl_build<64><<<blocks(n, 4, 8192), 256, 0, c->stream>>>(flag, c->sflags, 0, n, A.x, A.y, A.z, c->handled_s,
                                                            c->box, g, coff, range2, cell_of, start, cat,
-                                                           A.nls_cnt, A.nls, A.disp_s, c->derr);
+                                                           A.nls_cnt, A.nls, A.disp_s, c->derr, nullptr);
     });
   } else {
     launch(c, "k_nl_build_long", [&] {
       k_nl_build<512><<<blocks(n, 4, 8192), 256, 0, c->stream>>>(flag, c->sflags, 1, n, A.x, A.y, A.z, c->handled_l,
                                                             c->box, g, coff, range2, cell_of, start, cat,
-                                                            A.nll_cnt, A.nll, A.disp_l, c->derr);
+                                                            A.nll_cnt, A.nll, A.disp_l, c->derr, A.nll_split);
     });
   }
 }
@@ -386,9 +393,15 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   }
   if (c->slot_lj >= 0) {
     launch(c, "k_lj_wave", [&] {
-      k_lj_wave<R, TH><<<lgrid, 256, 0, c->stream>>>(
-          A, T, c->box, A.grad, c->eatom + (size_t)c->slot_lj * n, c->queue, c->qcount, c->qcap, c->counters + 2,
-          c->derr, dth);
+      auto go = [&](auto occ) {
+        k_lj_wave<R, TH, decltype(occ)::value><<<lgrid, 256, 0, c->stream>>>(
+            A, T, c->box, A.grad, c->eatom + (size_t)c->slot_lj * n, c->queue, c->qcount, c->qcap,
+            c->counters + 2, c->derr, dth);
+      };
+      if constexpr (!TH) {
+        if (tu.occ_lj == 3) return go(std::integral_constant<int, 3>());
+      }
+      go(std::integral_constant<int, 1>());
     });
     launch(c, "k_bstar", [&] {
       auto go = [&](auto occ) {
@@ -638,7 +651,7 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   r |= dalloc(c, A.disp_s, 3 * (size_t)n) | dalloc(c, A.disp_l, 3 * (size_t)n);
   r |= dalloc(c, A.nls_cnt, n) | dalloc(c, A.nll_cnt, n);
   if (c->have_short) r |= dalloc(c, A.nls, (size_t)n * A.cap_s);
-  if (c->have_long) r |= dalloc(c, A.nll, (size_t)n * A.cap_l);
+  if (c->have_long) r |= dalloc(c, A.nll, (size_t)n * A.cap_l) | dalloc(c, A.nll_split, n);
   r |= dalloc(c, A.bcnt, n) | dalloc(c, A.bj, (size_t)n * A.maxb);
   r |= dalloc(c, A.br, (size_t)n * A.maxb) | dalloc(c, A.bf, (size_t)n * A.maxb) | dalloc(c, A.bdf, (size_t)n * A.maxb);
   r |= dalloc(c, A.bx, (size_t)n * A.maxb) | dalloc(c, A.by, (size_t)n * A.maxb) | dalloc(c, A.bz, (size_t)n * A.maxb);
@@ -726,6 +739,7 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   HIPCHK(c, hipMemset(A.disp_l, 0, 3 * (size_t)n * 8));
   HIPCHK(c, hipMemset(A.nls_cnt, 0, (size_t)n * 4));
   HIPCHK(c, hipMemset(A.nll_cnt, 0, (size_t)n * 4));
+  if (A.nll_split) HIPCHK(c, hipMemset(A.nll_split, 0, (size_t)n * 4));
   HIPCHK(c, hipMemset(A.bcnt, 0, (size_t)n * 4));
   HIPCHK(c, up(c->handled_s, hs.data(), n));
   HIPCHK(c, up(c->handled_l, hl.data(), n));
@@ -1113,6 +1127,7 @@ int mdg_autotune(mdg_ctx* c, int reps, double* report) {
     tune_knob("k_rebo_bonds", tu.grid_bonds, grids(nb, 256));
   }
   if (c->slot_lj >= 0) {
+    tune_knob("k_lj_wave", tu.occ_lj, {1, 3});
     tune_knob("k_lj_wave", tu.grid_lj, grids(c->n, 4));
     tune_knob("k_bstar", tu.occ_bstar, {1, 2});
     tune_knob("k_bstar", tu.grid_bstar, grids(nq, 256));
@@ -1126,9 +1141,10 @@ int mdg_autotune(mdg_ctx* c, int reps, double* report) {
       e = nullptr;
     }
   if (report) {
-    double r[8] = {(double)tu.occ_bonds, (double)tu.grid_bonds, (double)tu.occ_bstar, (double)tu.grid_bstar,
-                   (double)tu.grid_lj,   (double)tu.grid_tors,  before,                after};
-    for (int q = 0; q < 8; q++) report[q] = r[q];
+    double r[9] = {(double)tu.occ_bonds, (double)tu.grid_bonds, (double)tu.occ_bstar, (double)tu.grid_bstar,
+                   (double)tu.grid_lj,   (double)tu.grid_tors,  before,                after,
+                   (double)tu.occ_lj};
+    for (int q = 0; q < 9; q++) report[q] = r[q];
   }
   return check_err(c);
 }

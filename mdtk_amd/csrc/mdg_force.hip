@@ -688,8 +688,10 @@ __device__ __forceinline__ R wfrom(unsigned long long b) {
 __device__ __forceinline__ int hslot(int t) { return (int)(((unsigned)t * 2654435761u) >> 25); }
 
 // AIREBO::operator() (AIREBO_LJ.cxx:34-77), one wavefront per atom i.
-// Phase 0: lanes sweep i's long list and ballot-compact the LJ pairs (j > i,
-// inside the LJ cutoff) into LDS.  Phase 1: one lane per (first, second) bond
+// Phase 0: lanes sweep the j > i part of i's long list (the row is sorted;
+// its split point is stored at list build), two 64-entry chunks per round so
+// their gathers overlap, and ballot-compact the LJ pairs (inside the LJ
+// cutoff) into LDS.  Phase 1: one lane per (first, second) bond
 // prefix enumerates i's paths of <= 3 bonds to targets j > i, with key =
 // bonds<<24 | s1<<16 | s2<<8 | s3 (the reference's loop order), and merges
 // them into an LDS hash keyed by target: the largest weight by an atomic max
@@ -697,8 +699,8 @@ __device__ __forceinline__ int hslot(int t) { return (int)(((unsigned)t * 265443
 // by an atomic min -- the reference's choice whatever order the lanes run in.
 // Phase 2: lanes take the compacted pairs, look C_ij up in the hash and either
 // finish the pair (S_r = 0) or queue it for the b* kernel.
-template <class R, bool TH>
-__global__ void __launch_bounds__(256) k_lj_wave(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
+template <class R, bool TH, int OCC>
+__global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
                                                 double* __restrict__ grad, double* __restrict__ eatom,
                                                 int4* __restrict__ queue, int* __restrict__ qcount, int qcap,
                                                 long long* __restrict__ npairs, unsigned* err,
@@ -723,31 +725,43 @@ __global__ void __launch_bounds__(256) k_lj_wave(Atoms A, const TablesT<R>* __re
     if (ti > T_H) continue;  // wave-uniform
     const double* Lb = box_of(bx, i);
     double xi = A.x[i], yi = A.y[i], zi = A.z[i];
+    // bond-graph loads of phase 1, issued early
+    int nb1 = A.bcnt[i];
+    int kk = -1, c2 = 0;
+    if (lane < nb1) {
+      kk = A.bj[i * A.maxb + lane];
+      c2 = A.bcnt[kk];
+    }
     // phase 0: compact the LJ pairs
     int cnt = A.nll_cnt[i];
     const int* list = A.nll + (size_t)i * A.cap_l;
     int npair = 0;
-    for (int base = 0; base < cnt; base += 64) {
-      int q = base + lane;
-      bool ok = false;
-      int j = -1;
-      if (q < cnt) {
-        j = list[q];
-        if (j > i) {
-          int tj = A.type[j];
-          if (tj <= T_H) {
-            double dx = mic(xi - A.x[j], Lb[0]), dy = mic(yi - A.y[j], Lb[1]), dz = mic(zi - A.z[j], Lb[2]);
-            double d2 = dx * dx + dy * dy + dz * dz;
-            ok = !(d2 > double(L.R2sq[ti][tj]) * (T->lunit * T->lunit));
-          }
-        }
+    const double cut2 = T->lunit * T->lunit;
+    auto keep = [&](int j) -> bool {
+      if (j <= i) return false;
+      int tj = A.type[j];
+      if (tj > T_H) return false;
+      double dx = mic(xi - A.x[j], Lb[0]), dy = mic(yi - A.y[j], Lb[1]), dz = mic(zi - A.z[j], Lb[2]);
+      double d2 = dx * dx + dy * dy + dz * dz;
+      return !(d2 > double(L.R2sq[ti][tj]) * cut2);
+    };
+    for (int base = A.nll_split[i]; base < cnt; base += 128) {
+      int q0 = base + lane, q1 = base + 64 + lane;
+      int j0 = (q0 < cnt) ? list[q0] : -1;
+      int j1 = (q1 < cnt) ? list[q1] : -1;
+      bool ok0 = j0 >= 0 && keep(j0);
+      bool ok1 = j1 >= 0 && keep(j1);
+      unsigned long long m0 = __ballot(ok0), m1 = __ballot(ok1);
+      if (ok0) {
+        int slot = npair + __popcll(m0 & below);
+        if (slot < kPairCap) s_pair[wv][slot] = j0;
       }
-      unsigned long long m = __ballot(ok);
-      if (ok) {
-        int slot = npair + __popcll(m & below);
-        if (slot < kPairCap) s_pair[wv][slot] = j;
+      npair += __popcll(m0);
+      if (ok1) {
+        int slot = npair + __popcll(m1 & below);
+        if (slot < kPairCap) s_pair[wv][slot] = j1;
       }
-      npair += __popcll(m);
+      npair += __popcll(m1);
     }
     if (npair > kPairCap) {
       if (lane == 0) raise(err, ERR_QUEUE_OVERFLOW);
@@ -762,12 +776,6 @@ __global__ void __launch_bounds__(256) k_lj_wave(Atoms A, const TablesT<R>* __re
       s_hk[wv][q] = 0x7fffffff;
     }
     if (lane == 0) s_cnt[wv] = 0;
-    int nb1 = A.bcnt[i];
-    int kk = -1, c2 = 0;
-    if (lane < nb1) {
-      kk = A.bj[i * A.maxb + lane];
-      c2 = A.bcnt[kk];
-    }
     int incl = c2;
     for (int o = 1; o < 16; o <<= 1) {
       int v = __shfl_up(incl, o, 64);
@@ -1119,18 +1127,20 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
                                                     const int*, double*, double*, unsigned*, double*);         \
   template __global__ void k_bstar<R, TH, OCC>(Atoms, const TablesT<R>*, Box, const int4*, const int*, int,  \
                                                double*, double*, unsigned*, double*);
-#define MDG_INSTANTIATE_TH(R, TH)                                                                              \
-  template __global__ void k_lj_wave<R, TH>(Atoms, const TablesT<R>*, Box, double*, double*, int4*, int*,    \
-                                            int, long long*, unsigned*, double*);
+#define MDG_INSTANTIATE_TH(R, TH, OCC)                                                                         \
+  template __global__ void k_lj_wave<R, TH, OCC>(Atoms, const TablesT<R>*, Box, double*, double*, int4*, int*, \
+                                                 int, long long*, unsigned*, double*);
 #define MDG_INSTANTIATE(R)                                                                                     \
   template __global__ void k_coord<R>(Atoms, const TablesT<R>*, Box, unsigned*);                              \
   template __global__ void k_etors<R>(Atoms, const TablesT<R>*, Box, const int2*, const int*, int, double*,   \
                                       double*, long long*);
 MDG_INSTANTIATE(double)
 MDG_INSTANTIATE(float)
-MDG_INSTANTIATE_TH(double, false)
-MDG_INSTANTIATE_TH(double, true)
-MDG_INSTANTIATE_TH(float, false)
+MDG_INSTANTIATE_TH(double, false, 1)
+MDG_INSTANTIATE_TH(double, false, 3)
+MDG_INSTANTIATE_TH(double, true, 1)
+MDG_INSTANTIATE_TH(float, false, 1)
+MDG_INSTANTIATE_TH(float, false, 3)
 MDG_INSTANTIATE_OCC(double, false, 1)
 MDG_INSTANTIATE_OCC(double, false, 2)
 MDG_INSTANTIATE_OCC(double, true, 1)

@@ -34,19 +34,27 @@ __device__ double block_sum(double v) {
 
 // ------------------------------------------------------------ cell lists
 // Every system has its own grid; cell ids are offset by cell_off[system].
-__global__ void k_cell_clear(const int* __restrict__ flag, int ncell, int* __restrict__ cnt) {
+// Per-system cell passes: system s owns the cells [cell_off[s], cell_off[s+1])
+// and the slots [sys_off[s], sys_off[s+1]) of cell_atoms, so only the systems
+// whose rebuild flag is set (sflags[2*s + list]) are redone.
+__global__ void k_cell_clear(const int* __restrict__ flag, const int* __restrict__ sflags, int list_id,
+                             const int* __restrict__ cell_off, int* __restrict__ cnt) {
   if (!*flag) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ncell; i += gridDim.x * blockDim.x) cnt[i] = 0;
+  int s = blockIdx.x;
+  if (!sflags[2 * s + list_id]) return;
+  for (int c = cell_off[s] + threadIdx.x; c < cell_off[s + 1]; c += blockDim.x) cnt[c] = 0;
 }
 
-__global__ void k_cell_assign(const int* __restrict__ flag, int n, const double* __restrict__ x,
-                              const double* __restrict__ y, const double* __restrict__ z,
-                              const CellGrid* __restrict__ grids, const int* __restrict__ cell_off,
-                              const int* __restrict__ sys, int* __restrict__ cell_of, int* __restrict__ cell_cnt) {
+__global__ void k_cell_assign(const int* __restrict__ flag, const int* __restrict__ sflags, int list_id, int n,
+                              const double* __restrict__ x, const double* __restrict__ y,
+                              const double* __restrict__ z, const CellGrid* __restrict__ grids,
+                              const int* __restrict__ cell_off, const int* __restrict__ sys,
+                              int* __restrict__ cell_of, int* __restrict__ cell_cnt) {
   if (!*flag) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int s = sys[i];
+  if (!sflags[2 * s + list_id]) return;
   const CellGrid& g = grids[s];
   int c[3];
   double p[3] = {x[i], y[i], z[i]};
@@ -61,13 +69,19 @@ __global__ void k_cell_assign(const int* __restrict__ flag, int n, const double*
   atomicAdd(&cell_cnt[id], 1);
 }
 
-// Exclusive scan of the cell counts by one workgroup (1024 threads).
-__global__ void k_cell_scan(const int* __restrict__ flag, int ncell, int* __restrict__ cnt, int* __restrict__ start) {
+// Exclusive scan of one system's cell counts (one 1024-thread workgroup per
+// system), offset by the system's first atom slot.
+__global__ void k_cell_scan(const int* __restrict__ flag, const int* __restrict__ sflags, int list_id,
+                            const int* __restrict__ cell_off, const int* __restrict__ sys_off, int* __restrict__ cnt,
+                            int* __restrict__ start) {
   if (!*flag) return;
+  int sy = blockIdx.x;
+  if (!sflags[2 * sy + list_id]) return;
   __shared__ int part[1024];
+  int c0 = cell_off[sy], ncell = cell_off[sy + 1] - c0;
   int t = threadIdx.x;
   int chunk = (ncell + 1023) / 1024;
-  int b = t * chunk, e = min(b + chunk, ncell);
+  int b = c0 + min(t * chunk, ncell), e = c0 + min((t + 1) * chunk, ncell);
   int s = 0;
   for (int q = b; q < e; q++) s += cnt[q];
   part[t] = s;
@@ -78,20 +92,22 @@ __global__ void k_cell_scan(const int* __restrict__ flag, int ncell, int* __rest
     part[t] += v;
     __syncthreads();
   }
-  int run = part[t] - s;
+  int run = sys_off[sy] + part[t] - s;
   for (int q = b; q < e; q++) {
     start[q] = run;
     run += cnt[q];
     cnt[q] = 0;  // reused as fill cursor
   }
-  if (t == 1023) start[ncell] = part[1023];
+  if (t == 1023) start[c0 + ncell] = sys_off[sy] + part[1023];
 }
 
-__global__ void k_cell_fill(const int* __restrict__ flag, int n, const int* __restrict__ cell_of,
+__global__ void k_cell_fill(const int* __restrict__ flag, const int* __restrict__ sflags, int list_id, int n,
+                            const int* __restrict__ sys, const int* __restrict__ cell_of,
                             const int* __restrict__ start, int* __restrict__ cursor, int* __restrict__ cell_atoms) {
   if (!*flag) return;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if (!sflags[2 * sys[i] + list_id]) return;
   int c = cell_of[i];
   int p = atomicAdd(&cursor[c], 1);
   cell_atoms[start[c] + p] = i;
@@ -114,7 +130,8 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
                                                   const int* __restrict__ cell_off, double range2,
                                                   const int* __restrict__ cell_of, const int* __restrict__ start,
                                                   const int* __restrict__ cell_atoms, int* __restrict__ cnt,
-                                                  int* __restrict__ list, double* __restrict__ disp, unsigned* err) {
+                                                  int* __restrict__ list, double* __restrict__ disp, unsigned* err,
+                                                  int* __restrict__ split) {
   if (!*flag) return;
   __shared__ int buf[4][CAP];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -209,6 +226,12 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
       }
     for (int q = lane; q < m; q += 64) list[(size_t)i * CAP + q] = my[q];
     if (lane == 0) cnt[i] = m;
+    if (split) {  // entries below i: the row's j > i part starts there
+      int lt = 0;
+      for (int q = lane; q < m; q += 64) lt += (my[q] < i);
+      lt = wave_sum(lt);
+      if (lane == 0) split[i] = lt;
+    }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
   }
@@ -216,11 +239,12 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
 
 template __global__ void k_nl_build<64>(const int*, const int*, int, int, const double*, const double*,
                                         const double*, const unsigned char*, Box, const CellGrid*, const int*,
-                                        double, const int*, const int*, const int*, int*, int*, double*, unsigned*);
+                                        double, const int*, const int*, const int*, int*, int*, double*, unsigned*,
+                                        int*);
 template __global__ void k_nl_build<512>(const int*, const int*, int, int, const double*, const double*,
                                          const double*, const unsigned char*, Box, const CellGrid*, const int*,
                                          double, const int*, const int*, const int*, int*, int*, double*,
-                                         unsigned*);
+                                         unsigned*, int*);
 
 // ------------------------------------------------------------ skin test
 // Top-2 accumulated displacement per system and list (NeighbourList.cxx:

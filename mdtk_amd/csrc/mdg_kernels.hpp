@@ -21,6 +21,7 @@ struct Atoms {
   double *disp_s, *disp_l;    // n*3 displacement since the last rebuild
   int cap_s, cap_l;           // row strides of the short / long lists
   int *nls_cnt, *nls, *nll_cnt, *nll;
+  int* nll_split;             // per long-list row: index of the first neighbour j > i
   int maxb;                   // bond-list capacity per atom
   int bond_cap;               // capacity of the (i<j) bond array
   int* bcnt;
@@ -36,18 +37,19 @@ struct CellGrid {
   int periodic[3];
 };
 
-__global__ void k_cell_clear(const int* flag, int ncell, int* cnt);
-__global__ void k_cell_assign(const int* flag, int n, const double* x, const double* y, const double* z,
-                              const CellGrid* grids, const int* cell_off, const int* sys, int* cell_of,
-                              int* cell_cnt);
-__global__ void k_cell_scan(const int* flag, int ncell, int* cnt, int* start);
-__global__ void k_cell_fill(const int* flag, int n, const int* cell_of, const int* start, int* cursor,
-                            int* cell_atoms);
+__global__ void k_cell_clear(const int* flag, const int* sflags, int list_id, const int* cell_off, int* cnt);
+__global__ void k_cell_assign(const int* flag, const int* sflags, int list_id, int n, const double* x,
+                              const double* y, const double* z, const CellGrid* grids, const int* cell_off,
+                              const int* sys, int* cell_of, int* cell_cnt);
+__global__ void k_cell_scan(const int* flag, const int* sflags, int list_id, const int* cell_off, const int* sys_off,
+                            int* cnt, int* start);
+__global__ void k_cell_fill(const int* flag, const int* sflags, int list_id, int n, const int* sys,
+                            const int* cell_of, const int* start, int* cursor, int* cell_atoms);
 template <int CAP>
 __global__ void k_nl_build(const int* flag, const int* sflags, int list_id, int n, const double* x, const double* y,
                            const double* z, const unsigned char* handled, Box bx, const CellGrid* grids,
                            const int* cell_off, double range2, const int* cell_of, const int* start,
-                           const int* cell_atoms, int* cnt, int* list, double* disp, unsigned* err);
+                           const int* cell_atoms, int* cnt, int* list, double* disp, unsigned* err, int* split);
 __global__ void k_sys_disp(const int* sys_off, const double* ds, const double* dl, double lim_s, double lim_l,
                            int* sflags, int* flags, long long* rebuilds);
 __global__ void k_clear_flags(int nsys, int* flags, int* sflags);
@@ -62,7 +64,7 @@ __global__ void k_coord(Atoms A, const TablesT<R>* T, Box bx, unsigned* err);
 template <class R, bool TH, int OCC>
 __global__ void k_rebo_bonds(Atoms A, const TablesT<R>* T, const Tables* T64, Box bx, const int2* bonds,
                              const int* nbonds, double* grad, double* eatom, unsigned* err, double* dth);
-template <class R, bool TH>
+template <class R, bool TH, int OCC>
 __global__ void k_lj_wave(Atoms A, const TablesT<R>* T, Box bx, double* grad, double* eatom, int4* queue,
                           int* qcount, int qcap, long long* npairs, unsigned* err, double* dth);
 template <class R, bool TH, int OCC>
