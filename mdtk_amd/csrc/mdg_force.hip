@@ -619,34 +619,48 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
 // order (i->k, then k->m, then m->q) unless a later one has a larger weight,
 // or an equal weight and fewer bonds (:311,331,352).  That is: the winner is
 // the path of largest weight, then fewest bonds, then earliest in loop order.
-// Each path is described by (bonds, k, m) with the nodes i,[k],[m],j.
-struct CPath {
-  int hops, k, m;
-};
+// A path is named by key = bonds<<24 | s1<<16 | s2<<8 | s3 (its bond slots
+// in the per-step bond lists, in the reference's loop order), so ties on
+// weight go to the smaller key.
+
+// Path weight prod f over the path's bonds, read from the per-step bond
+// lists (r, f, df and the minimum-image vector as the coordination pass
+// computed them); o receives the bonds' list offsets.
+template <class R>
+__device__ R path_weight(const Atoms& A, int i, int key, int* o) {
+  BondArr<R> B = barr<R>(A);
+  int hops = key >> 24;
+  int a = i;
+  R w = R(0);
+  for (int q = 0; q < hops; q++) {
+    o[q] = a * A.maxb + ((key >> (16 - 8 * q)) & 0xff);
+    w = (q == 0) ? B.f[o[q]] : w * B.f[o[q]];
+    a = A.bj[o[q]];
+  }
+  return w;
+}
 
 // Gradient of C_ij = 1 - prod f over the path's bonds, seeded with V.
 template <class R>
-__device__ void cij_backward(const Atoms& A, const TablesT<R>* __restrict__ T, Box bx, int i, int j, CPath p,
-                             R V, double* __restrict__ grad, double gs) {
+__device__ void cij_backward_bonds(const Atoms& A, int i, int key, const int* o, R V, double* __restrict__ grad,
+                                   double gs) {
+  BondArr<R> B = barr<R>(A);
+  int hops = key >> 24;
   int nodes[4];
-  int nn = 0;
-  nodes[nn++] = i;
-  if (p.hops >= 2) nodes[nn++] = p.k;
-  if (p.hops >= 3) nodes[nn++] = p.m;
-  nodes[nn++] = j;
+  nodes[0] = i;
   R f[3], df[3], r[3];
   V3<R> rv[3];
-  for (int q = 0; q < p.hops; q++) {
-    int a = nodes[q], b = nodes[q + 1];
-    rv[q] = to_local<R>(depos(A.x, A.y, A.z, a, b, bx));
-    r[q] = sqrt(norm2(rv[q]));
-    int ta = A.type[a], tb = A.type[b];
-    cutoff<R>(r[q], T->rebo.R[ta][tb][0], T->rebo.R[ta][tb][1], f[q], df[q]);
+  for (int q = 0; q < hops; q++) {
+    nodes[q + 1] = A.bj[o[q]];
+    f[q] = B.f[o[q]];
+    df[q] = B.df[o[q]];
+    r[q] = B.r[o[q]];
+    rv[q] = v3<R>(B.x[o[q]], B.y[o[q]], B.z[o[q]]);
   }
-  for (int q = 0; q < p.hops; q++) {
+  for (int q = 0; q < hops; q++) {
     if (df[q] == R(0)) continue;
     R mm = R(1);
-    for (int u = 0; u < p.hops; u++)
+    for (int u = 0; u < hops; u++)
       if (u != q) mm *= f[u];
     R c = df[q] * (-mm * V);
     V3<R> v = rv[q] * (c / r[q]);
@@ -673,6 +687,7 @@ __device__ __forceinline__ R dSfun(R a, R mn, R mx) {
 constexpr int kPathCap = 320;  // path records per atom held in LDS
 constexpr int kHash = 128;     // C_ij hash slots per atom (power of two)
 constexpr int kPairCap = 256;  // LJ pairs (j > i inside the LJ cutoff) per atom
+constexpr int kL2 = 128;       // 2-bond path prefixes per atom
 
 // Path weights w > 0 compare like their bit patterns.
 template <class R>
@@ -712,8 +727,11 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
   __shared__ short s_rs[4][kPathCap];            // path record: hash slot, key, weight
   __shared__ int s_rk[4][kPathCap];
   __shared__ R s_rw[4][kPathCap];
-  __shared__ int s_cnt[4];
-  __shared__ int s_off[4][17];
+  __shared__ int s_off[4][17];   // 2-bond prefixes of each first bond
+  __shared__ int s_m[4][kL2];     // 2-bond prefix: end atom, weight, key bits
+  __shared__ R s_w2[4][kL2];
+  __shared__ int s_k2[4][kL2];
+  __shared__ int s_o3[4][kL2 + 1];  // first 3-bond record of each prefix
   const LJParamsT<R>& L = T->lj;
   BondArr<R> B = barr<R>(A);
   const double gs = T->eunit / T->lunit;
@@ -769,68 +787,112 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     }
     if (npair == 0) continue;  // wave-uniform
     if (lane == 0) np += npair;
-    // phase 1: paths into the hash
+    // phase 1: paths into the hash.  Every path has a fixed record index --
+    // 1-bond paths [0, nb1), 2-bond paths [nb1, nb1 + tot2), 3-bond paths
+    // after them -- from prefix sums over the bond graph, so the lanes insert
+    // all records in one or two rounds.
     for (int q = lane; q < kHash; q += 64) {
       s_ht[wv][q] = -1;
       s_hw[wv][q] = 0ull;
       s_hk[wv][q] = 0x7fffffff;
     }
-    if (lane == 0) s_cnt[wv] = 0;
     int incl = c2;
     for (int o = 1; o < 16; o <<= 1) {
       int v = __shfl_up(incl, o, 64);
       if (lane >= o) incl += v;
     }
     if (lane < 16) s_off[wv][lane + 1] = incl;
-    if (lane == 0) s_off[wv][0] = 0;
+    if (lane == 0) {
+      s_off[wv][0] = 0;
+      s_o3[wv][0] = 0;
+    }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
     int tot2 = s_off[wv][nb1];
-    auto push = [&](int target, R w, int key) {
-      if (target <= i || !(w > R(0))) return;  // only targets j > i are LJ pairs of this wave
-      int h = hslot(target);
-      for (int probes = 0;; probes++) {
-        if (probes == kHash) {
-          raise(err, ERR_QUEUE_OVERFLOW);
-          return;
-        }
-        int t = atomicCAS(&s_ht[wv][h], -1, target);
-        if (t == -1 || t == target) break;
-        h = (h + 1) & (kHash - 1);
-      }
-      atomicMax(&s_hw[wv][h], wbits(w));
-      int slot = atomicAdd(&s_cnt[wv], 1);
-      if (slot < kPathCap) {
-        s_rs[wv][slot] = (short)h;
-        s_rk[wv][slot] = key;
-        s_rw[wv][slot] = w;
-      } else {
-        raise(err, ERR_QUEUE_OVERFLOW);
-      }
-    };
-    if (lane < nb1) push(kk, B.f[i * A.maxb + lane], (1 << 24) | (lane << 16));
-    for (int c = lane; c < tot2; c += 64) {
+    if (tot2 > kL2) {
+      if (lane == 0) raise(err, ERR_QUEUE_OVERFLOW);
+      tot2 = kL2;
+    }
+    for (int c = lane; c < tot2; c += 64) {  // 2-bond prefixes (s1, s2)
       int s1 = 0;
       while (s1 + 1 < nb1 && s_off[wv][s1 + 1] <= c) s1++;
       int s2 = c - s_off[wv][s1];
       int k = A.bj[i * A.maxb + s1];
-      R f1 = B.f[i * A.maxb + s1];
       int m = A.bj[k * A.maxb + s2];
-      R f2 = B.f[k * A.maxb + s2];
-      push(m, f1 * f2, (2 << 24) | (s1 << 16) | (s2 << 8));
-      int nb3 = A.bcnt[m];
-      for (int s3 = 0; s3 < nb3; s3++) {
-        int q = A.bj[m * A.maxb + s3];
-        R f3 = B.f[m * A.maxb + s3];
-        push(q, f1 * f2 * f3, (3 << 24) | (s1 << 16) | (s2 << 8) | s3);
-      }
+      s_m[wv][c] = m;
+      s_w2[wv][c] = B.f[i * A.maxb + s1] * B.f[k * A.maxb + s2];
+      s_k2[wv][c] = (s1 << 16) | (s2 << 8);
+      s_o3[wv][c + 1] = A.bcnt[m];
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
-    int npath = min(s_cnt[wv], kPathCap);
-    for (int r = lane; r < npath; r += 64) {
+    for (int base = 0, run = 0; base < tot2; base += 64) {  // s_o3[c]: first 3-bond record of prefix c
+      int c = base + lane;
+      int x = (c < tot2) ? s_o3[wv][c + 1] : 0;
+      for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+      }
+      if (c < tot2) s_o3[wv][c + 1] = run + x;
+      run += __shfl(x, 63, 64);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    int nrec = nb1 + tot2 + s_o3[wv][tot2];
+    if (nrec > kPathCap) {
+      if (lane == 0) raise(err, ERR_QUEUE_OVERFLOW);
+      nrec = kPathCap;
+    }
+    for (int r = lane; r < nrec; r += 64) {
+      int target, key;
+      R w;
+      if (r < nb1) {
+        target = A.bj[i * A.maxb + r];
+        w = B.f[i * A.maxb + r];
+        key = (1 << 24) | (r << 16);
+      } else if (r < nb1 + tot2) {
+        int c = r - nb1;
+        target = s_m[wv][c];
+        w = s_w2[wv][c];
+        key = (2 << 24) | s_k2[wv][c];
+      } else {
+        int d = r - nb1 - tot2;
+        int lo = 0, hi = tot2 - 1;  // last prefix whose first record is <= d
+        while (lo < hi) {
+          int mid = (lo + hi + 1) >> 1;
+          if (s_o3[wv][mid] <= d) lo = mid;
+          else hi = mid - 1;
+        }
+        int s3 = d - s_o3[wv][lo];
+        int m = s_m[wv][lo];
+        target = A.bj[m * A.maxb + s3];
+        w = s_w2[wv][lo] * B.f[m * A.maxb + s3];
+        key = (3 << 24) | s_k2[wv][lo] | s3;
+      }
+      int h = -1;
+      if (target > i && w > R(0)) {  // only targets j > i are LJ pairs of this wave
+        h = hslot(target);
+        for (int probes = 0;; probes++) {
+          if (probes == kHash) {
+            raise(err, ERR_QUEUE_OVERFLOW);
+            h = -1;
+            break;
+          }
+          int t = atomicCAS(&s_ht[wv][h], -1, target);
+          if (t == -1 || t == target) break;
+          h = (h + 1) & (kHash - 1);
+        }
+        if (h >= 0) atomicMax(&s_hw[wv][h], wbits(w));
+      }
+      s_rs[wv][r] = (short)h;
+      s_rk[wv][r] = key;
+      s_rw[wv][r] = w;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    for (int r = lane; r < nrec; r += 64) {
       int h = s_rs[wv][r];
-      if (wbits(s_rw[wv][r]) == s_hw[wv][h]) atomicMin(&s_hk[wv][h], s_rk[wv][r]);
+      if (h >= 0 && wbits(s_rw[wv][r]) == s_hw[wv][h]) atomicMin(&s_hk[wv][h], s_rk[wv][r]);
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -857,13 +919,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       }
       R C = R(1) - w;
       if (!(C > R(0))) continue;
-      CPath path{0, -1, -1};
-      if (w > R(0)) {
-        path.hops = bkey >> 24;
-        int s1 = (bkey >> 16) & 0xff, s2 = (bkey >> 8) & 0xff;
-        path.k = A.bj[i * A.maxb + s1];
-        if (path.hops == 3) path.m = A.bj[path.k * A.maxb + s2];
-      }
+      int pkey = (w > R(0)) ? bkey : 0;  // C_ij path (0: none)
       V3<R> rv = to_local<R>(rd);
       R r = sqrt(norm2(rv));
       R Str = Sfun<R>(r, L.RLJ[ti][tj][0], L.RLJ[ti][tj][1]);
@@ -874,7 +930,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
         if (lane == leader) base = atomicAdd(qcount, __popcll(m));
         base = __shfl(base, leader, 64);
         int slot = base + __popcll(m & below);
-        if (slot < qcap) queue[slot] = make_int4(i, j, path.k, (path.hops << 28) | (path.m & 0x0fffffff));
+        if (slot < qcap) queue[slot] = make_int4(i, j, pkey, 0);
         else raise(err, ERR_QUEUE_OVERFLOW);
         continue;
       }
@@ -894,7 +950,11 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       V3<R> v = rv * (cr / r);
       gi = gi + v;
       scatter3<R>(grad, j, -v, gs);
-      if (path.hops > 0) cij_backward<R>(A, T, bx, i, j, path, R(1) * LJ, grad, gs);
+      if (pkey) {
+        int o[3];
+        path_weight<R>(A, i, pkey, o);
+        cij_backward_bonds<R>(A, i, pkey, o, R(1) * LJ, grad, gs);
+      }
       e += to_erg<R>(double(R(1) * C * LJ), T->eunit);
       if constexpr (TH) {  // S_r = 0: E = C f 4 eps ((sigma/r)^12 - (sigma/r)^6)
         double de = double(C) * double(f) * 4.0 * double(s12 - s6);
@@ -947,23 +1007,13 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
     const ReboParamsT<R>& P = T->rebo;
     int4 qd = queue[t];
     int i = qd.x, j = qd.y;
-    CPath path{qd.w >> 28, qd.z, qd.w & 0x0fffffff};
+    int pkey = qd.z;  // C_ij path (0: none), bond slots as in k_lj_wave
     int ti = A.type[i], tj = A.type[j];
     V3<R> rv = to_local<R>(depos(A.x, A.y, A.z, i, j, bx));
     R r = sqrt(norm2(rv));
-    // path weight, product in the reference's order ((f1*f2)*f3)
-    R w = R(0);
-    if (path.hops > 0) {
-      int nodes[4] = {i, path.k, path.m, j};
-      nodes[path.hops] = j;
-      for (int q = 0; q < path.hops; q++) {
-        int a = nodes[q], b = nodes[q + 1];
-        R rr = sqrt(norm2(to_local<R>(depos(A.x, A.y, A.z, a, b, bx))));
-        R f, df;
-        cutoff<R>(rr, P.R[A.type[a]][A.type[b]][0], P.R[A.type[a]][A.type[b]][1], f, df);
-        w = (q == 0) ? f : w * f;
-      }
-    }
+    // path weight from the bond lists, product in the reference's order ((f1*f2)*f3)
+    int po[3];
+    R w = pkey ? path_weight<R>(A, i, pkey, po) : R(0);
     R C = R(1) - w;
     R rmin = L.RLJ[ti][tj][0], rmax = L.RLJ[ti][tj][1];
     R Str = Sfun<R>(r, rmin, rmax);
@@ -1012,7 +1062,7 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
     V3<R> v = rv * (cr / r);
     scatter3<R>(grad, i, v, gs);
     scatter3<R>(grad, j, -v, gs);
-    if (path.hops > 0 && SrSb * LJ != R(0)) cij_backward<R>(A, T, bx, i, j, path, SrSb * LJ, grad, gs);
+    if (pkey && SrSb * LJ != R(0)) cij_backward_bonds<R>(A, i, pkey, po, SrSb * LJ, grad, gs);
     add_energy(eatom, i, to_erg<R>(double(SrSb * C * LJ), T->eunit));
     if constexpr (TH) {
       // S_r depends on r/sigma only: dS_r/dsigma = -(r/sigma) dS_r/dr
