@@ -158,12 +158,13 @@ int mdg_kernel_time(mdg_ctx* ctx, const char* kernel, double* total_ms, int64_t*
 /* Autotuner: times candidate launch shapes of the force kernels on the
  * context's own systems (reps force calls per candidate, which leave the
  * trajectory unchanged) and keeps the fastest for every later call: the
- * waves-per-SIMD variant of the bond-order (1 or 2) and LJ (1 or 3) kernels
- * and the grid sizes of the bond, LJ, b* and torsion kernels.  report (may be
- * null) receives 9
- * values: occ_bonds, grid_bonds, occ_bstar, grid_bstar, grid_lj, grid_tors
- * (0 = built-in heuristic), the tuned kernels' summed ms per call before and
- * after, and occ_lj.  No reference counterpart. */
+ * waves-per-SIMD variant of the bond-order kernels (1 or 2), the shape of
+ * the LJ kernel (lanes per atom 64, 64 held to 3 waves/SIMD, 32 or 16) and
+ * the grid sizes of the bond, LJ, b* and torsion kernels.  report (may be
+ * null) receives 9 values: occ_bonds, grid_bonds, occ_bstar, grid_bstar,
+ * grid_lj, grid_tors (0 = built-in heuristic), the tuned kernels' summed ms
+ * per call before and after, and the LJ shape (0-3).  No reference
+ * counterpart. */
 int mdg_autotune(mdg_ctx* ctx, int reps, double* report);
 
 /* Timing with HIP events on the context's stream: mdg_event_record records

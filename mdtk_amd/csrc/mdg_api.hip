@@ -29,7 +29,9 @@ struct KStat {
 // mdg_autotune from timings on the context's own system.
 struct Tune {
   int occ_bonds = 1, occ_bstar = 1;  // waves/SIMD variant of the bond-order kernels
-  int occ_lj = 1;                    // 1 or 3: waves/SIMD variant of the LJ kernel
+  // LJ kernel shape: 0 = 64 lanes per atom, 1 = the same held to 3 waves/SIMD,
+  // 2 = 32 lanes per atom, 3 = 16 lanes per atom (more atoms in flight)
+  int lj_shape = 3;
   int grid_bonds = 0, grid_bstar = 0, grid_lj = 0, grid_tors = 0;
 };
 }  // namespace
@@ -371,7 +373,6 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   const Tune& tu = c->tune;
   int bgrid = tu.grid_bonds ? tu.grid_bonds : blocks(2LL * n, 256, 8192);
   int tgrid = tu.grid_tors ? tu.grid_tors : blocks(2LL * n, 256, 8192);
-  int lgrid = tu.grid_lj ? tu.grid_lj : blocks(n, 4, 65536);
   int sgrid = tu.grid_bstar ? tu.grid_bstar : blocks(8LL * n, 256, 16384);
   if (c->slot_rebo >= 0) {
     launch(c, "k_rebo_bonds", [&] {
@@ -393,15 +394,24 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   }
   if (c->slot_lj >= 0) {
     launch(c, "k_lj_wave", [&] {
-      auto go = [&](auto occ) {
-        k_lj_wave<R, TH, decltype(occ)::value><<<lgrid, 256, 0, c->stream>>>(
+      auto go = [&](auto occ, auto lw) {
+        constexpr int LW = decltype(lw)::value;
+        int grid = tu.grid_lj ? tu.grid_lj : blocks(n, 4 * (64 / LW), 65536);
+        k_lj_wave<R, TH, decltype(occ)::value, LW><<<grid, 256, 0, c->stream>>>(
             A, T, c->box, A.grad, c->eatom + (size_t)c->slot_lj * n, c->queue, c->qcount, c->qcap,
             c->counters + 2, c->derr, dth);
       };
-      if constexpr (!TH) {
-        if (tu.occ_lj == 3) return go(std::integral_constant<int, 3>());
+      using I1 = std::integral_constant<int, 1>;
+      if constexpr (TH) {
+        go(I1(), std::integral_constant<int, 16>());
+      } else {
+        switch (tu.lj_shape) {
+          case 0: go(I1(), std::integral_constant<int, 64>()); break;
+          case 1: go(std::integral_constant<int, 3>(), std::integral_constant<int, 64>()); break;
+          case 2: go(I1(), std::integral_constant<int, 32>()); break;
+          default: go(I1(), std::integral_constant<int, 16>()); break;
+        }
       }
-      go(std::integral_constant<int, 1>());
     });
     launch(c, "k_bstar", [&] {
       auto go = [&](auto occ) {
@@ -1102,10 +1112,10 @@ int mdg_autotune(mdg_ctx* c, int reps, double* report) {
     if (one >= 1 && one < (1 << 20)) g.push_back((int)one);
     return g;
   };
-  double before = 0, after = 0;
+  std::map<std::string, double> base_ms, best_ms;  // per kernel: untuned, tuned
   auto tune_knob = [&](const char* kernel, int& knob, const std::vector<int>& cand) {
     double t0 = measure(kernel);
-    before += t0;
+    if (!base_ms.count(kernel)) base_ms[kernel] = t0;
     int best = knob;
     double bt = t0;
     for (int v : cand) {
@@ -1119,7 +1129,7 @@ int mdg_autotune(mdg_ctx* c, int reps, double* report) {
       knob = keep;
     }
     knob = best;
-    after += bt;
+    best_ms[kernel] = bt;
   };
   Tune& tu = c->tune;
   if (c->slot_rebo >= 0) {
@@ -1127,8 +1137,8 @@ int mdg_autotune(mdg_ctx* c, int reps, double* report) {
     tune_knob("k_rebo_bonds", tu.grid_bonds, grids(nb, 256));
   }
   if (c->slot_lj >= 0) {
-    tune_knob("k_lj_wave", tu.occ_lj, {1, 3});
-    tune_knob("k_lj_wave", tu.grid_lj, grids(c->n, 4));
+    tune_knob("k_lj_wave", tu.lj_shape, {0, 1, 2, 3});
+    tune_knob("k_lj_wave", tu.grid_lj, grids(c->n, 4 * (64 / (tu.lj_shape == 3 ? 16 : tu.lj_shape == 2 ? 32 : 64))));
     tune_knob("k_bstar", tu.occ_bstar, {1, 2});
     tune_knob("k_bstar", tu.grid_bstar, grids(nq, 256));
   }
@@ -1140,10 +1150,13 @@ int mdg_autotune(mdg_ctx* c, int reps, double* report) {
       hipGraphExecDestroy(e);
       e = nullptr;
     }
+  double before = 0, after = 0;
+  for (auto& kv : base_ms) before += kv.second;
+  for (auto& kv : best_ms) after += kv.second;
   if (report) {
     double r[9] = {(double)tu.occ_bonds, (double)tu.grid_bonds, (double)tu.occ_bstar, (double)tu.grid_bstar,
                    (double)tu.grid_lj,   (double)tu.grid_tors,  before,                after,
-                   (double)tu.occ_lj};
+                   (double)tu.lj_shape};
     for (int q = 0; q < 9; q++) report[q] = r[q];
   }
   return check_err(c);

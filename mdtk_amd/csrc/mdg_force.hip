@@ -623,49 +623,54 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
 // in the per-step bond lists, in the reference's loop order), so ties on
 // weight go to the smaller key.
 
-// Path weight prod f over the path's bonds, read from the per-step bond
-// lists (r, f, df and the minimum-image vector as the coordination pass
-// computed them); o receives the bonds' list offsets.
+// A C_ij path's bonds read from the per-step bond lists (r, f, df and the
+// minimum-image vector as the coordination pass computed them), loaded
+// before any of the caller's scatters so that no gather waits behind them.
 template <class R>
-__device__ R path_weight(const Atoms& A, int i, int key, int* o) {
+struct PathBonds {
+  int hops;
+  int nodes[4];
+  R f[3], df[3], r[3];
+  V3<R> rv[3];
+};
+
+template <class R>
+__device__ __forceinline__ PathBonds<R> load_path(const Atoms& A, int i, int key) {
   BondArr<R> B = barr<R>(A);
-  int hops = key >> 24;
-  int a = i;
-  R w = R(0);
-  for (int q = 0; q < hops; q++) {
-    o[q] = a * A.maxb + ((key >> (16 - 8 * q)) & 0xff);
-    w = (q == 0) ? B.f[o[q]] : w * B.f[o[q]];
-    a = A.bj[o[q]];
+  PathBonds<R> pb;
+  pb.hops = key >> 24;
+  pb.nodes[0] = i;
+  for (int q = 0; q < pb.hops; q++) {
+    int o = pb.nodes[q] * A.maxb + ((key >> (16 - 8 * q)) & 0xff);
+    pb.nodes[q + 1] = A.bj[o];
+    pb.f[q] = B.f[o];
+    pb.df[q] = B.df[o];
+    pb.r[q] = B.r[o];
+    pb.rv[q] = v3<R>(B.x[o], B.y[o], B.z[o]);
   }
+  return pb;
+}
+
+// Path weight, product in the reference's order ((f1*f2)*f3).
+template <class R>
+__device__ __forceinline__ R path_weight(const PathBonds<R>& pb) {
+  R w = pb.f[0];
+  for (int q = 1; q < pb.hops; q++) w = w * pb.f[q];
   return w;
 }
 
 // Gradient of C_ij = 1 - prod f over the path's bonds, seeded with V.
 template <class R>
-__device__ void cij_backward_bonds(const Atoms& A, int i, int key, const int* o, R V, double* __restrict__ grad,
-                                   double gs) {
-  BondArr<R> B = barr<R>(A);
-  int hops = key >> 24;
-  int nodes[4];
-  nodes[0] = i;
-  R f[3], df[3], r[3];
-  V3<R> rv[3];
-  for (int q = 0; q < hops; q++) {
-    nodes[q + 1] = A.bj[o[q]];
-    f[q] = B.f[o[q]];
-    df[q] = B.df[o[q]];
-    r[q] = B.r[o[q]];
-    rv[q] = v3<R>(B.x[o[q]], B.y[o[q]], B.z[o[q]]);
-  }
-  for (int q = 0; q < hops; q++) {
-    if (df[q] == R(0)) continue;
+__device__ __forceinline__ void cij_backward(const PathBonds<R>& pb, R V, double* __restrict__ grad, double gs) {
+  for (int q = 0; q < pb.hops; q++) {
+    if (pb.df[q] == R(0)) continue;
     R mm = R(1);
-    for (int u = 0; u < hops; u++)
-      if (u != q) mm *= f[u];
-    R c = df[q] * (-mm * V);
-    V3<R> v = rv[q] * (c / r[q]);
-    scatter3<R>(grad, nodes[q], v, gs);
-    scatter3<R>(grad, nodes[q + 1], -v, gs);
+    for (int u = 0; u < pb.hops; u++)
+      if (u != q) mm *= pb.f[u];
+    R c = pb.df[q] * (-mm * V);
+    V3<R> v = pb.rv[q] * (c / pb.r[q]);
+    scatter3<R>(grad, pb.nodes[q], v, gs);
+    scatter3<R>(grad, pb.nodes[q + 1], -v, gs);
   }
 }
 
@@ -684,10 +689,10 @@ __device__ __forceinline__ R dSfun(R a, R mn, R mx) {
   return (R(-2) * t * (R(3) - R(2) * t) + R(2) * t * t) / (mx - mn);
 }
 
-constexpr int kPathCap = 320;  // path records per atom held in LDS
-constexpr int kHash = 128;     // C_ij hash slots per atom (power of two)
-constexpr int kPairCap = 256;  // LJ pairs (j > i inside the LJ cutoff) per atom
-constexpr int kL2 = 128;       // 2-bond path prefixes per atom
+constexpr int kPathCap = 128;  // C_ij path records per atom (FP64 form)
+constexpr int kHash = 64;      // C_ij hash slots per atom (power of two)
+constexpr int kPairCap = 128;  // LJ pairs (j > i inside the LJ cutoff) per atom
+constexpr int kL2 = 32;        // 2-bond path prefixes per atom
 
 // Path weights w > 0 compare like their bit patterns.
 template <class R>
@@ -700,56 +705,78 @@ __device__ __forceinline__ R wfrom(unsigned long long b) {
   if constexpr (std::is_same<R, double>::value) return __longlong_as_double((long long)b);
   else return __uint_as_float((unsigned)b);
 }
-__device__ __forceinline__ int hslot(int t) { return (int)(((unsigned)t * 2654435761u) >> 25); }
+__device__ __forceinline__ int hslot(int t) { return (int)(((unsigned)t * 2654435761u) >> 26); }
 
-// AIREBO::operator() (AIREBO_LJ.cxx:34-77), one wavefront per atom i.
-// Phase 0: lanes sweep the j > i part of i's long list (the row is sorted;
-// its split point is stored at list build), two 64-entry chunks per round so
-// their gathers overlap, and ballot-compact the LJ pairs (inside the LJ
-// cutoff) into LDS.  Phase 1: one lane per (first, second) bond
-// prefix enumerates i's paths of <= 3 bonds to targets j > i, with key =
-// bonds<<24 | s1<<16 | s2<<8 | s3 (the reference's loop order), and merges
-// them into an LDS hash keyed by target: the largest weight by an atomic max
-// on its bit pattern, then, among the paths of that weight, the smallest key
-// by an atomic min -- the reference's choice whatever order the lanes run in.
-// Phase 2: lanes take the compacted pairs, look C_ij up in the hash and either
-// finish the pair (S_r = 0) or queue it for the b* kernel.
-template <class R, bool TH, int OCC>
+// Sums and inclusive scans within a group of LW consecutive lanes.
+template <int LW, class T>
+__device__ __forceinline__ T group_sum(T v) {
+  for (int o = LW / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, LW);
+  return v;
+}
+template <int LW>
+__device__ __forceinline__ int group_scan(int x, int sl) {
+  for (int o = 1; o < LW; o <<= 1) {
+    int y = __shfl_up(x, o, LW);
+    if (sl >= o) x += y;
+  }
+  return x;
+}
+
+// AIREBO::operator() (AIREBO_LJ.cxx:34-77).  A group of LW lanes handles
+// one atom i, so a wavefront keeps 64/LW atoms (and their chains of
+// dependent gathers) in flight; LW is a tuning choice.
+// Phase 0: the group sweeps the j > i part of i's long list (the row is
+// sorted; its split point is stored at list build), two LW-entry chunks per
+// round, and ballot-compacts the LJ pairs (inside the LJ cutoff) into LDS.
+// Phase 1: every path of <= 3 bonds from i to a target j > i gets a fixed
+// record index (1-bond paths, then 2-bond, then 3-bond, from prefix sums over
+// the bond graph) with key = bonds<<24 | s1<<16 | s2<<8 | s3 (the reference's
+// loop order), and is merged into an LDS hash keyed by target: the largest
+// weight, then the smallest key -- the reference's choice whatever order the
+// lanes run in.  FP32 weights and the key share one 64-bit atomic max;
+// FP64 weights take an atomic max, then an atomic min on the keys of the
+// paths of that weight.
+// Phase 2: the group takes the compacted pairs, looks C_ij up in the hash and
+// either finishes the pair (S_r = 0) or queues it for the b* kernel; all of
+// a pair's gathers come before its scatters (a load issued after the
+// fire-and-forget scatter atomics waits for them).
+// All LDS arrays are per group, so the phases are ordered by wavefront-scope
+// fences (a wave's LDS operations complete in order).
+template <class R, bool TH, int OCC, int LW>
 __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
-                                                double* __restrict__ grad, double* __restrict__ eatom,
-                                                int4* __restrict__ queue, int* __restrict__ qcount, int qcap,
-                                                long long* __restrict__ npairs, unsigned* err,
-                                                double* __restrict__ dth) {
-  __shared__ int s_pair[4][kPairCap];
-  __shared__ int s_ht[4][kHash];                 // target of a slot, -1 when empty
-  __shared__ unsigned long long s_hw[4][kHash];  // largest path weight (bit pattern)
-  __shared__ int s_hk[4][kHash];                 // smallest key among those paths
-  __shared__ short s_rs[4][kPathCap];            // path record: hash slot, key, weight
-  __shared__ int s_rk[4][kPathCap];
-  __shared__ R s_rw[4][kPathCap];
-  __shared__ int s_off[4][17];   // 2-bond prefixes of each first bond
-  __shared__ int s_m[4][kL2];     // 2-bond prefix: end atom, weight, key bits
-  __shared__ R s_w2[4][kL2];
-  __shared__ int s_k2[4][kL2];
-  __shared__ int s_o3[4][kL2 + 1];  // first 3-bond record of each prefix
+                                                     double* __restrict__ grad, double* __restrict__ eatom,
+                                                     int4* __restrict__ queue, int* __restrict__ qcount, int qcap,
+                                                     long long* __restrict__ npairs, unsigned* err,
+                                                     double* __restrict__ dth) {
+  constexpr int G = 4 * (64 / LW);  // atoms in flight per workgroup
+  constexpr bool PACK = !std::is_same<R, double>::value;
+  constexpr int RC = PACK ? 1 : kPathCap;
+  __shared__ int s_pair[G][kPairCap];
+  __shared__ int s_ht[G][kHash];                 // target of a slot, -1 when empty
+  __shared__ unsigned long long s_hw[G][kHash];  // best weight (bits) [| ~key in the FP32 form]
+  __shared__ int s_hk[G][PACK ? 1 : kHash];      // smallest key among the best-weight paths (FP64)
+  __shared__ int s_off[G][17];                   // 2-bond prefixes of each first bond
+  __shared__ int s_m[G][kL2];                    // 2-bond prefix: end atom, weight, key bits
+  __shared__ R s_w2[G][kL2];
+  __shared__ int s_k2[G][kL2];
+  __shared__ int s_o3[G][kL2 + 1];               // first 3-bond record of each prefix
+  __shared__ short s_rs[G][RC];                  // FP64 records: hash slot, key, weight
+  __shared__ int s_rk[G][RC];
+  __shared__ R s_rw[G][RC];
   const LJParamsT<R>& L = T->lj;
   BondArr<R> B = barr<R>(A);
   const double gs = T->eunit / T->lunit;
-  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int sl = lane % LW;
+  const int gid = (threadIdx.x >> 6) * (64 / LW) + lane / LW;
   const unsigned long long below = (1ull << lane) - 1ull;
+  const unsigned long long mine = (LW == 64) ? ~0ull : (((1ull << LW) - 1ull) << (lane / LW * LW));
   long long np = 0;
-  for (int i = blockIdx.x * 4 + wv; i < A.n; i += gridDim.x * 4) {
+  for (int i = blockIdx.x * G + gid; i < A.n; i += gridDim.x * G) {
     int ti = A.type[i];
-    if (ti > T_H) continue;  // wave-uniform
+    if (ti > T_H) continue;  // group-uniform
     const double* Lb = box_of(bx, i);
     double xi = A.x[i], yi = A.y[i], zi = A.z[i];
-    // bond-graph loads of phase 1, issued early
-    int nb1 = A.bcnt[i];
-    int kk = -1, c2 = 0;
-    if (lane < nb1) {
-      kk = A.bj[i * A.maxb + lane];
-      c2 = A.bcnt[kk];
-    }
     // phase 0: compact the LJ pairs
     int cnt = A.nll_cnt[i];
     const int* list = A.nll + (size_t)i * A.cap_l;
@@ -763,87 +790,80 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       double d2 = dx * dx + dy * dy + dz * dz;
       return !(d2 > double(L.R2sq[ti][tj]) * cut2);
     };
-    for (int base = A.nll_split[i]; base < cnt; base += 128) {
-      int q0 = base + lane, q1 = base + 64 + lane;
+    for (int base = A.nll_split[i]; base < cnt; base += 2 * LW) {
+      int q0 = base + sl, q1 = base + LW + sl;
       int j0 = (q0 < cnt) ? list[q0] : -1;
       int j1 = (q1 < cnt) ? list[q1] : -1;
       bool ok0 = j0 >= 0 && keep(j0);
       bool ok1 = j1 >= 0 && keep(j1);
-      unsigned long long m0 = __ballot(ok0), m1 = __ballot(ok1);
+      unsigned long long m0 = __ballot(ok0) & mine, m1 = __ballot(ok1) & mine;
       if (ok0) {
         int slot = npair + __popcll(m0 & below);
-        if (slot < kPairCap) s_pair[wv][slot] = j0;
+        if (slot < kPairCap) s_pair[gid][slot] = j0;
       }
       npair += __popcll(m0);
       if (ok1) {
         int slot = npair + __popcll(m1 & below);
-        if (slot < kPairCap) s_pair[wv][slot] = j1;
+        if (slot < kPairCap) s_pair[gid][slot] = j1;
       }
       npair += __popcll(m1);
     }
     if (npair > kPairCap) {
-      if (lane == 0) raise(err, ERR_QUEUE_OVERFLOW);
+      if (sl == 0) raise(err, ERR_QUEUE_OVERFLOW);
       npair = kPairCap;
     }
-    if (npair == 0) continue;  // wave-uniform
-    if (lane == 0) np += npair;
-    // phase 1: paths into the hash.  Every path has a fixed record index --
-    // 1-bond paths [0, nb1), 2-bond paths [nb1, nb1 + tot2), 3-bond paths
-    // after them -- from prefix sums over the bond graph, so the lanes insert
-    // all records in one or two rounds.
-    for (int q = lane; q < kHash; q += 64) {
-      s_ht[wv][q] = -1;
-      s_hw[wv][q] = 0ull;
-      s_hk[wv][q] = 0x7fffffff;
+    if (npair == 0) continue;  // group-uniform
+    if (sl == 0) np += npair;
+    // phase 1: paths into the hash
+    for (int q = sl; q < kHash; q += LW) {
+      s_ht[gid][q] = -1;
+      s_hw[gid][q] = 0ull;
+      if constexpr (!PACK) s_hk[gid][q] = 0x7fffffff;
     }
-    int incl = c2;
-    for (int o = 1; o < 16; o <<= 1) {
-      int v = __shfl_up(incl, o, 64);
-      if (lane >= o) incl += v;
+    int nb1 = A.bcnt[i];
+    int c2 = 0;
+    if (sl < nb1) c2 = A.bcnt[A.bj[i * A.maxb + sl]];
+    int incl = group_scan<LW>(c2, sl);
+    if (sl < 16) s_off[gid][sl + 1] = incl;
+    if (sl == 0) {
+      s_off[gid][0] = 0;
+      s_o3[gid][0] = 0;
     }
-    if (lane < 16) s_off[wv][lane + 1] = incl;
-    if (lane == 0) {
-      s_off[wv][0] = 0;
-      s_o3[wv][0] = 0;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    int tot2 = s_off[wv][nb1];
+    int tot2 = s_off[gid][nb1];
     if (tot2 > kL2) {
-      if (lane == 0) raise(err, ERR_QUEUE_OVERFLOW);
+      if (sl == 0) raise(err, ERR_QUEUE_OVERFLOW);
       tot2 = kL2;
     }
-    for (int c = lane; c < tot2; c += 64) {  // 2-bond prefixes (s1, s2)
+    for (int c = sl; c < tot2; c += LW) {  // 2-bond prefixes (s1, s2)
       int s1 = 0;
-      while (s1 + 1 < nb1 && s_off[wv][s1 + 1] <= c) s1++;
-      int s2 = c - s_off[wv][s1];
+      while (s1 + 1 < nb1 && s_off[gid][s1 + 1] <= c) s1++;
+      int s2 = c - s_off[gid][s1];
       int k = A.bj[i * A.maxb + s1];
       int m = A.bj[k * A.maxb + s2];
-      s_m[wv][c] = m;
-      s_w2[wv][c] = B.f[i * A.maxb + s1] * B.f[k * A.maxb + s2];
-      s_k2[wv][c] = (s1 << 16) | (s2 << 8);
-      s_o3[wv][c + 1] = A.bcnt[m];
+      s_m[gid][c] = m;
+      s_w2[gid][c] = B.f[i * A.maxb + s1] * B.f[k * A.maxb + s2];
+      s_k2[gid][c] = (s1 << 16) | (s2 << 8);
+      s_o3[gid][c + 1] = A.bcnt[m];
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    for (int base = 0, run = 0; base < tot2; base += 64) {  // s_o3[c]: first 3-bond record of prefix c
-      int c = base + lane;
-      int x = (c < tot2) ? s_o3[wv][c + 1] : 0;
-      for (int o = 1; o < 64; o <<= 1) {
-        int y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-      }
-      if (c < tot2) s_o3[wv][c + 1] = run + x;
-      run += __shfl(x, 63, 64);
+    for (int base = 0, run = 0; base < tot2; base += LW) {  // s_o3[c]: first 3-bond record of prefix c
+      int c = base + sl;
+      int x = (c < tot2) ? s_o3[gid][c + 1] : 0;
+      x = group_scan<LW>(x, sl);
+      if (c < tot2) s_o3[gid][c + 1] = run + x;
+      run += __shfl(x, LW - 1, LW);
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    int nrec = nb1 + tot2 + s_o3[wv][tot2];
-    if (nrec > kPathCap) {
-      if (lane == 0) raise(err, ERR_QUEUE_OVERFLOW);
+    int nrec = nb1 + tot2 + s_o3[gid][tot2];
+    if (!PACK && nrec > kPathCap) {
+      if (sl == 0) raise(err, ERR_QUEUE_OVERFLOW);
       nrec = kPathCap;
     }
-    for (int r = lane; r < nrec; r += 64) {
+    for (int r = sl; r < nrec; r += LW) {
       int target, key;
       R w;
       if (r < nb1) {
@@ -852,25 +872,25 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
         key = (1 << 24) | (r << 16);
       } else if (r < nb1 + tot2) {
         int c = r - nb1;
-        target = s_m[wv][c];
-        w = s_w2[wv][c];
-        key = (2 << 24) | s_k2[wv][c];
+        target = s_m[gid][c];
+        w = s_w2[gid][c];
+        key = (2 << 24) | s_k2[gid][c];
       } else {
         int d = r - nb1 - tot2;
         int lo = 0, hi = tot2 - 1;  // last prefix whose first record is <= d
         while (lo < hi) {
           int mid = (lo + hi + 1) >> 1;
-          if (s_o3[wv][mid] <= d) lo = mid;
+          if (s_o3[gid][mid] <= d) lo = mid;
           else hi = mid - 1;
         }
-        int s3 = d - s_o3[wv][lo];
-        int m = s_m[wv][lo];
+        int s3 = d - s_o3[gid][lo];
+        int m = s_m[gid][lo];
         target = A.bj[m * A.maxb + s3];
-        w = s_w2[wv][lo] * B.f[m * A.maxb + s3];
-        key = (3 << 24) | s_k2[wv][lo] | s3;
+        w = s_w2[gid][lo] * B.f[m * A.maxb + s3];
+        key = (3 << 24) | s_k2[gid][lo] | s3;
       }
       int h = -1;
-      if (target > i && w > R(0)) {  // only targets j > i are LJ pairs of this wave
+      if (target > i && w > R(0)) {  // only targets j > i are LJ pairs of this atom
         h = hslot(target);
         for (int probes = 0;; probes++) {
           if (probes == kHash) {
@@ -878,40 +898,53 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
             h = -1;
             break;
           }
-          int t = atomicCAS(&s_ht[wv][h], -1, target);
+          int t = atomicCAS(&s_ht[gid][h], -1, target);
           if (t == -1 || t == target) break;
           h = (h + 1) & (kHash - 1);
         }
-        if (h >= 0) atomicMax(&s_hw[wv][h], wbits(w));
+        if (h >= 0) {
+          if constexpr (PACK) atomicMax(&s_hw[gid][h], (wbits(w) << 32) | (unsigned long long)(~(unsigned)key));
+          else atomicMax(&s_hw[gid][h], wbits(w));
+        }
       }
-      s_rs[wv][r] = (short)h;
-      s_rk[wv][r] = key;
-      s_rw[wv][r] = w;
+      if constexpr (!PACK) {
+        s_rs[gid][r] = (short)h;
+        s_rk[gid][r] = key;
+        s_rw[gid][r] = w;
+      }
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    for (int r = lane; r < nrec; r += 64) {
-      int h = s_rs[wv][r];
-      if (h >= 0 && wbits(s_rw[wv][r]) == s_hw[wv][h]) atomicMin(&s_hk[wv][h], s_rk[wv][r]);
+    if constexpr (!PACK) {
+      for (int r = sl; r < nrec; r += LW) {
+        int h = s_rs[gid][r];
+        if (h >= 0 && wbits(s_rw[gid][r]) == s_hw[gid][h]) atomicMin(&s_hk[gid][h], s_rk[gid][r]);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    __builtin_amdgcn_wave_barrier();
     // phase 2: pairs
     double e = 0.0;
     double th_e[2] = {0.0, 0.0}, th_s[2] = {0.0, 0.0};  // dE/d(epsilon, sigma) by tj
     V3<R> gi = v3<R>(0, 0, 0);
-    for (int q = lane; q < npair; q += 64) {
-      int j = s_pair[wv][q];
+    for (int q = sl; q < npair; q += LW) {
+      int j = s_pair[gid][q];
       int tj = A.type[j];
       D3 rd = d3(mic(xi - A.x[j], Lb[0]), mic(yi - A.y[j], Lb[1]), mic(zi - A.z[j], Lb[2]));
       R w = R(0);
       int bkey = 0x7fffffff;
       int h = hslot(j);
       for (int probes = 0; probes < kHash; probes++) {
-        int t = s_ht[wv][h];
+        int t = s_ht[gid][h];
         if (t == j) {
-          w = wfrom<R>(s_hw[wv][h]);
-          bkey = s_hk[wv][h];
+          unsigned long long v = s_hw[gid][h];
+          if constexpr (PACK) {
+            w = wfrom<R>(v >> 32);
+            bkey = (int)(~(unsigned)(v & 0xffffffffull));
+          } else {
+            w = wfrom<R>(v);
+            bkey = s_hk[gid][h];
+          }
           break;
         }
         if (t == -1) break;
@@ -934,6 +967,9 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
         else raise(err, ERR_QUEUE_OVERFLOW);
         continue;
       }
+      PathBonds<R> pb;
+      pb.hops = 0;
+      if (pkey) pb = load_path<R>(A, i, pkey);
       R f, df;
       cutoff<R>(r, L.R[ti][tj][0], L.R[ti][tj][1], f, df);
       if (f == R(0)) continue;
@@ -950,11 +986,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       V3<R> v = rv * (cr / r);
       gi = gi + v;
       scatter3<R>(grad, j, -v, gs);
-      if (pkey) {
-        int o[3];
-        path_weight<R>(A, i, pkey, o);
-        cij_backward_bonds<R>(A, i, pkey, o, R(1) * LJ, grad, gs);
-      }
+      if (pkey) cij_backward<R>(pb, R(1) * LJ, grad, gs);
       e += to_erg<R>(double(R(1) * C * LJ), T->eunit);
       if constexpr (TH) {  // S_r = 0: E = C f 4 eps ((sigma/r)^12 - (sigma/r)^6)
         double de = double(C) * double(f) * 4.0 * double(s12 - s6);
@@ -969,9 +1001,9 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
         }
       }
     }
-    double gx = wave_sum(double(gi.x)), gy = wave_sum(double(gi.y)), gz = wave_sum(double(gi.z));
-    e = wave_sum(e);
-    if (lane == 0) {
+    double gx = group_sum<LW>(double(gi.x)), gy = group_sum<LW>(double(gi.y)), gz = group_sum<LW>(double(gi.z));
+    e = group_sum<LW>(e);
+    if (sl == 0) {
       if (gx != 0.0 || gy != 0.0 || gz != 0.0) {
         if constexpr (std::is_same<R, double>::value) atomic_add3(grad, i, d3(gx, gy, gz));
         else atomic_add3(grad, i, d3(gx * gs, gy * gs, gz * gs));
@@ -980,15 +1012,15 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     }
     if constexpr (TH) {
       for (int q = 0; q < 2; q++) {
-        double de = wave_sum(th_e[q]), ds = wave_sum(th_s[q]);
-        if (lane == 0 && (de != 0.0 || ds != 0.0)) {
+        double de = group_sum<LW>(th_e[q]), ds = group_sum<LW>(th_s[q]);
+        if (sl == 0 && (de != 0.0 || ds != 0.0)) {
           double* th = theta_of(dth, A, i) + 27 + 2 * (ti + q);
           unsafeAtomicAdd(th, de);
           unsafeAtomicAdd(th + 1, ds);
         }
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
   }
   if (np) atomicAdd((unsigned long long*)npairs, (unsigned long long)np);
@@ -1011,9 +1043,10 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
     int ti = A.type[i], tj = A.type[j];
     V3<R> rv = to_local<R>(depos(A.x, A.y, A.z, i, j, bx));
     R r = sqrt(norm2(rv));
-    // path weight from the bond lists, product in the reference's order ((f1*f2)*f3)
-    int po[3];
-    R w = pkey ? path_weight<R>(A, i, pkey, po) : R(0);
+    PathBonds<R> pb;
+    pb.hops = 0;
+    if (pkey) pb = load_path<R>(A, i, pkey);
+    R w = pkey ? path_weight<R>(pb) : R(0);
     R C = R(1) - w;
     R rmin = L.RLJ[ti][tj][0], rmax = L.RLJ[ti][tj][1];
     R Str = Sfun<R>(r, rmin, rmax);
@@ -1062,7 +1095,7 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
     V3<R> v = rv * (cr / r);
     scatter3<R>(grad, i, v, gs);
     scatter3<R>(grad, j, -v, gs);
-    if (pkey && SrSb * LJ != R(0)) cij_backward_bonds<R>(A, i, pkey, po, SrSb * LJ, grad, gs);
+    if (pkey && SrSb * LJ != R(0)) cij_backward<R>(pb, SrSb * LJ, grad, gs);
     add_energy(eatom, i, to_erg<R>(double(SrSb * C * LJ), T->eunit));
     if constexpr (TH) {
       // S_r depends on r/sigma only: dS_r/dsigma = -(r/sigma) dS_r/dr
@@ -1177,20 +1210,24 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
                                                     const int*, double*, double*, unsigned*, double*);         \
   template __global__ void k_bstar<R, TH, OCC>(Atoms, const TablesT<R>*, Box, const int4*, const int*, int,  \
                                                double*, double*, unsigned*, double*);
-#define MDG_INSTANTIATE_TH(R, TH, OCC)                                                                         \
-  template __global__ void k_lj_wave<R, TH, OCC>(Atoms, const TablesT<R>*, Box, double*, double*, int4*, int*, \
-                                                 int, long long*, unsigned*, double*);
+#define MDG_INSTANTIATE_TH(R, TH, OCC, LW)                                                                     \
+  template __global__ void k_lj_wave<R, TH, OCC, LW>(Atoms, const TablesT<R>*, Box, double*, double*, int4*,   \
+                                                     int*, int, long long*, unsigned*, double*);
 #define MDG_INSTANTIATE(R)                                                                                     \
   template __global__ void k_coord<R>(Atoms, const TablesT<R>*, Box, unsigned*);                              \
   template __global__ void k_etors<R>(Atoms, const TablesT<R>*, Box, const int2*, const int*, int, double*,   \
                                       double*, long long*);
 MDG_INSTANTIATE(double)
 MDG_INSTANTIATE(float)
-MDG_INSTANTIATE_TH(double, false, 1)
-MDG_INSTANTIATE_TH(double, false, 3)
-MDG_INSTANTIATE_TH(double, true, 1)
-MDG_INSTANTIATE_TH(float, false, 1)
-MDG_INSTANTIATE_TH(float, false, 3)
+MDG_INSTANTIATE_TH(double, false, 1, 64)
+MDG_INSTANTIATE_TH(double, false, 3, 64)
+MDG_INSTANTIATE_TH(double, false, 1, 32)
+MDG_INSTANTIATE_TH(double, false, 1, 16)
+MDG_INSTANTIATE_TH(double, true, 1, 16)
+MDG_INSTANTIATE_TH(float, false, 1, 64)
+MDG_INSTANTIATE_TH(float, false, 3, 64)
+MDG_INSTANTIATE_TH(float, false, 1, 32)
+MDG_INSTANTIATE_TH(float, false, 1, 16)
 MDG_INSTANTIATE_OCC(double, false, 1)
 MDG_INSTANTIATE_OCC(double, false, 2)
 MDG_INSTANTIATE_OCC(double, true, 1)

@@ -64,7 +64,7 @@ __global__ void k_coord(Atoms A, const TablesT<R>* T, Box bx, unsigned* err);
 template <class R, bool TH, int OCC>
 __global__ void k_rebo_bonds(Atoms A, const TablesT<R>* T, const Tables* T64, Box bx, const int2* bonds,
                              const int* nbonds, double* grad, double* eatom, unsigned* err, double* dth);
-template <class R, bool TH, int OCC>
+template <class R, bool TH, int OCC, int LW>
 __global__ void k_lj_wave(Atoms A, const TablesT<R>* T, Box bx, double* grad, double* eatom, int4* queue,
                           int* qcount, int qcap, long long* npairs, unsigned* err, double* dth);
 template <class R, bool TH, int OCC>

@@ -254,7 +254,7 @@ class Engine:
         r = np.zeros(9)
         self._chk(_lib.mdg_autotune(self._h, int(reps), _p(r)))
         keys = ["occ_bonds", "grid_bonds", "occ_bstar", "grid_bstar", "grid_lj", "grid_tors", "ms_before",
-                "ms_after", "occ_lj"]
+                "ms_after", "lj_shape"]
         return {k: (float(v) if k.startswith("ms") else int(v)) for k, v in zip(keys, r)}
 
     def synchronize(self):
