@@ -926,7 +926,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     // phase 2: pairs
     double e = 0.0;
     double th_e[2] = {0.0, 0.0}, th_s[2] = {0.0, 0.0};  // dE/d(epsilon, sigma) by tj
-    V3<R> gi = v3<R>(0, 0, 0);
+    D3 gi = d3(0, 0, 0);  // FP64 in both forms: the lane width only reorders FP64 sums
     for (int q = sl; q < npair; q += LW) {
       int j = s_pair[gid][q];
       int tj = A.type[j];
@@ -984,7 +984,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       R V = R(1) * C;  // VLJ seeded with SrSb*C, SrSb = 1
       R cr = der * f * V + val * V * df;
       V3<R> v = rv * (cr / r);
-      gi = gi + v;
+      gi = gi + d3(double(v.x), double(v.y), double(v.z));
       scatter3<R>(grad, j, -v, gs);
       if (pkey) cij_backward<R>(pb, R(1) * LJ, grad, gs);
       e += to_erg<R>(double(R(1) * C * LJ), T->eunit);
@@ -1001,7 +1001,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
         }
       }
     }
-    double gx = group_sum<LW>(double(gi.x)), gy = group_sum<LW>(double(gi.y)), gz = group_sum<LW>(double(gi.z));
+    double gx = group_sum<LW>(gi.x), gy = group_sum<LW>(gi.y), gz = group_sum<LW>(gi.z);
     e = group_sum<LW>(e);
     if (sl == 0) {
       if (gx != 0.0 || gy != 0.0 || gz != 0.0) {
