@@ -284,8 +284,7 @@ void nl_build(mdg_ctx* c, bool shortlist) {
   launch(c, "k_cell_clear",
          [&] { k_cell_clear<<<nsys, 256, 0, c->stream>>>(flag, c->sflags, L, coff, cnt); });
   launch(c, "k_cell_assign", [&] {
-    k_cell_assign<<<blocks(n), 256, 0, c->stream>>>(flag, c->sflags, L, n, A.x, A.y, A.z, g, coff, A.sys, cell_of,
-                                                    cnt);
+    k_cell_assign<<<blocks(n), 256, 0, c->stream>>>(flag, c->sflags, L, n, A.p, g, coff, A.sys, cell_of, cnt);
   });
   launch(c, "k_cell_scan", [&] {
     k_cell_scan<<<nsys, 1024, 0, c->stream>>>(flag, c->sflags, L, coff, A.sys_off, cnt, start);
@@ -295,13 +294,13 @@ void nl_build(mdg_ctx* c, bool shortlist) {
   });
   if (shortlist) {
     launch(c, "k_nl_build_short", [&] {
-      k_nl_build<64><<<blocks(n, 4, 8192), 256, 0, c->stream>>>(flag, c->sflags, 0, n, A.x, A.y, A.z, c->handled_s,
+      k_nl_build<64><<<blocks(n, 4, 8192), 256, 0, c->stream>>>(flag, c->sflags, 0, n, A.p, c->handled_s,
                                                            c->box, g, coff, range2, cell_of, start, cat,
                                                            A.nls_cnt, A.nls, A.disp_s, c->derr, nullptr);
     });
   } else {
     launch(c, "k_nl_build_long", [&] {
-      k_nl_build<512><<<blocks(n, 4, 8192), 256, 0, c->stream>>>(flag, c->sflags, 1, n, A.x, A.y, A.z, c->handled_l,
+      k_nl_build<512><<<blocks(n, 4, 8192), 256, 0, c->stream>>>(flag, c->sflags, 1, n, A.p, c->handled_l,
                                                             c->box, g, coff, range2, cell_of, start, cat,
                                                             A.nll_cnt, A.nll, A.disp_l, c->derr, A.nll_split);
     });
@@ -663,13 +662,11 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   if (c->have_short) r |= dalloc(c, A.nls, (size_t)n * A.cap_s);
   if (c->have_long) r |= dalloc(c, A.nll, (size_t)n * A.cap_l) | dalloc(c, A.nll_split, n);
   r |= dalloc(c, A.bcnt, n) | dalloc(c, A.bj, (size_t)n * A.maxb);
-  r |= dalloc(c, A.br, (size_t)n * A.maxb) | dalloc(c, A.bf, (size_t)n * A.maxb) | dalloc(c, A.bdf, (size_t)n * A.maxb);
-  r |= dalloc(c, A.bx, (size_t)n * A.maxb) | dalloc(c, A.by, (size_t)n * A.maxb) | dalloc(c, A.bz, (size_t)n * A.maxb);
+  r |= dalloc(c, A.p, n);
+  if (!c->mixed) r |= dalloc(c, A.b64, (size_t)n * A.maxb);
   r |= dalloc(c, A.Nt, n) | dalloc(c, A.NH, n) | dalloc(c, A.NC, n);
   if (c->mixed) {
-    size_t nb = (size_t)n * A.maxb;
-    r |= dalloc(c, A.brf, nb) | dalloc(c, A.bff, nb) | dalloc(c, A.bdff, nb);
-    r |= dalloc(c, A.bxf, nb) | dalloc(c, A.byf, nb) | dalloc(c, A.bzf, nb);
+    r |= dalloc(c, A.b32, (size_t)n * A.maxb);
   }
   c->bond_cap = n * A.maxb / 2 + 64;
   A.bond_cap = c->bond_cap;
@@ -731,6 +728,7 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   HIPCHK(c, up(A.x, xs.data(), (size_t)n * 8));
   HIPCHK(c, up(A.y, ys.data(), (size_t)n * 8));
   HIPCHK(c, up(A.z, zs2.data(), (size_t)n * 8));
+  k_pack_pos<<<blocks(n), 256, 0, c->stream>>>(A);
   HIPCHK(c, up(A.vx, vxs.data(), (size_t)n * 8));
   HIPCHK(c, up(A.vy, vys.data(), (size_t)n * 8));
   HIPCHK(c, up(A.vz, vzs.data(), (size_t)n * 8));
@@ -1009,6 +1007,8 @@ int mdg_set_positions(mdg_ctx* c, const double* xyz, const int32_t* pbc_count, c
   HIPCHK(c, hipMemcpy(c->A.x, a.data(), n * 8, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->A.y, b.data(), n * 8, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->A.z, d.data(), n * 8, hipMemcpyHostToDevice));
+  k_pack_pos<<<blocks(n), 256, 0, c->stream>>>(c->A);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   if (pbc_count) {
     std::vector<int> q(n);
     int* dst[3] = {c->A.pcx, c->A.pcy, c->A.pcz};

@@ -15,12 +15,13 @@ struct BRef {
 };
 __device__ __forceinline__ BRef bref(const Atoms& A, int a, int s) {
   int o = a * A.maxb + s;
+  BondRec<double> q = A.b64[o];
   BRef b;
   b.k = A.bj[o];
-  b.r = A.br[o];
-  b.f = A.bf[o];
-  b.df = A.bdf[o];
-  b.rv = d3(A.bx[o], A.by[o], A.bz[o]);
+  b.r = q.r;
+  b.f = q.f;
+  b.df = q.df;
+  b.rv = d3(q.x, q.y, q.z);
   return b;
 }
 
@@ -74,7 +75,7 @@ __global__ void __launch_bounds__(256) k_brenner_bonds(Atoms A, const Tables* __
     int2 bd = bonds[t];
     int i = bd.x, j = bd.y;
     int ti = A.type[i], tj = A.type[j];
-    D3 rv = depos(A.x, A.y, A.z, i, j, bx);
+    D3 rv = depos(A.p, i, j, bx);
     double r = sqrt(norm2(rv));
     double f, df;
     cutoff(r, P.R[ti][tj][0], P.R[ti][tj][1], f, df);

@@ -151,10 +151,12 @@ struct Box {
 };
 __device__ __forceinline__ const double* box_of(const Box& bx, int a) { return bx.L + 3 * bx.sys[a]; }
 
-__device__ __forceinline__ D3 depos(const double* __restrict__ x, const double* __restrict__ y,
-                                    const double* __restrict__ z, int a, int b, const Box& bx) {
+// Positions are gathered from a packed copy p[a] = (x, y, z, 0): one 32-byte
+// load per atom instead of three scattered ones.
+__device__ __forceinline__ D3 depos(const double4* __restrict__ p, int a, int b, const Box& bx) {
   const double* L = box_of(bx, a);
-  return d3(mic(x[a] - x[b], L[0]), mic(y[a] - y[b], L[1]), mic(z[a] - z[b], L[2]));
+  double4 pa = p[a], pb = p[b];
+  return d3(mic(pa.x - pb.x, L[0]), mic(pa.y - pb.y, L[1]), mic(pa.z - pb.z, L[2]));
 }
 
 // A displacement taken in FP64 cm, expressed in the kernel's unit and type:

@@ -50,30 +50,26 @@ struct BondRef {
   V3<R> rv;  // x_a - x_k (minimum image)
 };
 
+// The bond records of the kernel's type.
 template <class R>
-struct BondArr {
-  R *r, *f, *df, *x, *y, *z;
-};
-
-template <class R>
-__device__ __forceinline__ BondArr<R> barr(const Atoms& A) {
+__device__ __forceinline__ BondRec<R>* brec(const Atoms& A) {
   if constexpr (std::is_same<R, double>::value) {
-    return BondArr<R>{A.br, A.bf, A.bdf, A.bx, A.by, A.bz};
+    return A.b64;
   } else {
-    return BondArr<R>{A.brf, A.bff, A.bdff, A.bxf, A.byf, A.bzf};
+    return A.b32;
   }
 }
 
 template <class R>
 __device__ __forceinline__ BondRef<R> bond_at(const Atoms& A, int a, int s) {
-  BondArr<R> B = barr<R>(A);
   int o = a * A.maxb + s;
+  BondRec<R> q = brec<R>(A)[o];
   BondRef<R> b;
   b.k = A.bj[o];
-  b.r = B.r[o];
-  b.f = B.f[o];
-  b.df = B.df[o];
-  b.rv = v3<R>(B.x[o], B.y[o], B.z[o]);
+  b.r = q.r;
+  b.f = q.f;
+  b.df = q.df;
+  b.rv = v3<R>(q.x, q.y, q.z);
   return b;
 }
 
@@ -97,7 +93,7 @@ template <class R>
 __global__ void k_coord(Atoms A, const TablesT<R>* __restrict__ T, Box bx, unsigned* err) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
-  BondArr<R> B = barr<R>(A);
+  BondRec<R>* B = brec<R>(A);
   int ti = A.type[i];
   double Nt = 0, NH = 0, NC = 0;
   int nb = 0;
@@ -107,7 +103,7 @@ __global__ void k_coord(Atoms A, const TablesT<R>* __restrict__ T, Box bx, unsig
     for (int q = 0; q < cnt; q++) {
       int k = L[q];
       int tk = A.type[k];
-      V3<R> rv = to_local<R>(depos(A.x, A.y, A.z, i, k, bx));
+      V3<R> rv = to_local<R>(depos(A.p, i, k, bx));
       R r = sqrt(rv.x * rv.x + rv.y * rv.y + rv.z * rv.z);
       R R2 = T->rebo.R[ti][tk][1];
       if (r > R2) continue;
@@ -120,12 +116,7 @@ __global__ void k_coord(Atoms A, const TablesT<R>* __restrict__ T, Box bx, unsig
       if (nb < A.maxb) {
         int o = i * A.maxb + nb;
         A.bj[o] = k;
-        B.r[o] = r;
-        B.f[o] = f;
-        B.df[o] = df;
-        B.x[o] = rv.x;
-        B.y[o] = rv.y;
-        B.z[o] = rv.z;
+        B[o] = BondRec<R>{rv.x, rv.y, rv.z, r, f, df, R(0), R(0)};
       } else {
         raise(err, ERR_BOND_OVERFLOW);
       }
@@ -552,7 +543,7 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
     p.j = bd.y;
     p.ti = A.type[p.i];
     p.tj = A.type[p.j];
-    D3 rd = depos(A.x, A.y, A.z, p.i, p.j, bx);
+    D3 rd = depos(A.p, p.i, p.j, bx);
     p.rv = to_local<R>(rd);
     p.r = sqrt(norm2(p.rv));
     int a = p.ti, b = p.tj;
@@ -636,17 +627,18 @@ struct PathBonds {
 
 template <class R>
 __device__ __forceinline__ PathBonds<R> load_path(const Atoms& A, int i, int key) {
-  BondArr<R> B = barr<R>(A);
+  const BondRec<R>* B = brec<R>(A);
   PathBonds<R> pb;
   pb.hops = key >> 24;
   pb.nodes[0] = i;
   for (int q = 0; q < pb.hops; q++) {
     int o = pb.nodes[q] * A.maxb + ((key >> (16 - 8 * q)) & 0xff);
+    BondRec<R> b = B[o];
     pb.nodes[q + 1] = A.bj[o];
-    pb.f[q] = B.f[o];
-    pb.df[q] = B.df[o];
-    pb.r[q] = B.r[o];
-    pb.rv[q] = v3<R>(B.x[o], B.y[o], B.z[o]);
+    pb.f[q] = b.f;
+    pb.df[q] = b.df;
+    pb.r[q] = b.r;
+    pb.rv[q] = v3<R>(b.x, b.y, b.z);
   }
   return pb;
 }
@@ -764,7 +756,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
   __shared__ int s_rk[G][RC];
   __shared__ R s_rw[G][RC];
   const LJParamsT<R>& L = T->lj;
-  BondArr<R> B = barr<R>(A);
+  const BondRec<R>* B = brec<R>(A);
   const double gs = T->eunit / T->lunit;
   const int lane = threadIdx.x & 63;
   const int sl = lane % LW;
@@ -776,7 +768,8 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     int ti = A.type[i];
     if (ti > T_H) continue;  // group-uniform
     const double* Lb = box_of(bx, i);
-    double xi = A.x[i], yi = A.y[i], zi = A.z[i];
+    double4 pi4 = A.p[i];
+    double xi = pi4.x, yi = pi4.y, zi = pi4.z;
     // phase 0: compact the LJ pairs
     int cnt = A.nll_cnt[i];
     const int* list = A.nll + (size_t)i * A.cap_l;
@@ -786,7 +779,8 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       if (j <= i) return false;
       int tj = A.type[j];
       if (tj > T_H) return false;
-      double dx = mic(xi - A.x[j], Lb[0]), dy = mic(yi - A.y[j], Lb[1]), dz = mic(zi - A.z[j], Lb[2]);
+      double4 pj = A.p[j];
+      double dx = mic(xi - pj.x, Lb[0]), dy = mic(yi - pj.y, Lb[1]), dz = mic(zi - pj.z, Lb[2]);
       double d2 = dx * dx + dy * dy + dz * dz;
       return !(d2 > double(L.R2sq[ti][tj]) * cut2);
     };
@@ -843,7 +837,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       int k = A.bj[i * A.maxb + s1];
       int m = A.bj[k * A.maxb + s2];
       s_m[gid][c] = m;
-      s_w2[gid][c] = B.f[i * A.maxb + s1] * B.f[k * A.maxb + s2];
+      s_w2[gid][c] = B[i * A.maxb + s1].f * B[k * A.maxb + s2].f;
       s_k2[gid][c] = (s1 << 16) | (s2 << 8);
       s_o3[gid][c + 1] = A.bcnt[m];
     }
@@ -868,7 +862,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       R w;
       if (r < nb1) {
         target = A.bj[i * A.maxb + r];
-        w = B.f[i * A.maxb + r];
+        w = B[i * A.maxb + r].f;
         key = (1 << 24) | (r << 16);
       } else if (r < nb1 + tot2) {
         int c = r - nb1;
@@ -886,7 +880,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
         int s3 = d - s_o3[gid][lo];
         int m = s_m[gid][lo];
         target = A.bj[m * A.maxb + s3];
-        w = s_w2[gid][lo] * B.f[m * A.maxb + s3];
+        w = s_w2[gid][lo] * B[m * A.maxb + s3].f;
         key = (3 << 24) | s_k2[gid][lo] | s3;
       }
       int h = -1;
@@ -930,7 +924,8 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     for (int q = sl; q < npair; q += LW) {
       int j = s_pair[gid][q];
       int tj = A.type[j];
-      D3 rd = d3(mic(xi - A.x[j], Lb[0]), mic(yi - A.y[j], Lb[1]), mic(zi - A.z[j], Lb[2]));
+      double4 pj = A.p[j];
+      D3 rd = d3(mic(xi - pj.x, Lb[0]), mic(yi - pj.y, Lb[1]), mic(zi - pj.z, Lb[2]));
       R w = R(0);
       int bkey = 0x7fffffff;
       int h = hslot(j);
@@ -1041,7 +1036,7 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
     int i = qd.x, j = qd.y;
     int pkey = qd.z;  // C_ij path (0: none), bond slots as in k_lj_wave
     int ti = A.type[i], tj = A.type[j];
-    V3<R> rv = to_local<R>(depos(A.x, A.y, A.z, i, j, bx));
+    V3<R> rv = to_local<R>(depos(A.p, i, j, bx));
     R r = sqrt(norm2(rv));
     PathBonds<R> pb;
     pb.hops = 0;
@@ -1127,7 +1122,7 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
     int2 bd = bonds[t];
     int i = bd.x, j = bd.y;
     if (A.type[i] == T_C && A.type[j] == T_C) {
-      V3<R> a_ = to_local<R>(depos(A.x, A.y, A.z, i, j, bx));
+      V3<R> a_ = to_local<R>(depos(A.p, i, j, bx));
       R rij = sqrt(norm2(a_));
       R wij, dwij;
       cutoff<R>(rij, P.R[0][0][0], P.R[0][0][1], wij, dwij);

@@ -46,8 +46,7 @@ __global__ void k_cell_clear(const int* __restrict__ flag, const int* __restrict
 }
 
 __global__ void k_cell_assign(const int* __restrict__ flag, const int* __restrict__ sflags, int list_id, int n,
-                              const double* __restrict__ x, const double* __restrict__ y,
-                              const double* __restrict__ z, const CellGrid* __restrict__ grids,
+                              const double4* __restrict__ pos, const CellGrid* __restrict__ grids,
                               const int* __restrict__ cell_off, const int* __restrict__ sys,
                               int* __restrict__ cell_of, int* __restrict__ cell_cnt) {
   if (!*flag) return;
@@ -57,7 +56,8 @@ __global__ void k_cell_assign(const int* __restrict__ flag, const int* __restric
   if (!sflags[2 * s + list_id]) return;
   const CellGrid& g = grids[s];
   int c[3];
-  double p[3] = {x[i], y[i], z[i]};
+  double4 pi = pos[i];
+  double p[3] = {pi.x, pi.y, pi.z};
   for (int d = 0; d < 3; d++) {
     int q = (int)floor((p[d] - g.lo[d]) / g.w[d]);
     if (q < 0) q = 0;
@@ -123,8 +123,7 @@ __global__ void k_cell_fill(const int* __restrict__ flag, const int* __restrict_
 // a few thousand empty waves, not one per atom.
 template <int CAP>
 __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, const int* __restrict__ sflags,
-                                                  int list_id, int n, const double* __restrict__ x,
-                                                  const double* __restrict__ y, const double* __restrict__ z,
+                                                  int list_id, int n, const double4* __restrict__ pos,
                                                   const unsigned char* __restrict__ handled, Box bx,
                                                   const CellGrid* __restrict__ grids,
                                                   const int* __restrict__ cell_off, double range2,
@@ -149,7 +148,8 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
     }
     int m = 0;
     if (handled[i]) {
-      double xi = x[i], yi = y[i], zi = z[i];
+      double4 pi = pos[i];
+      double xi = pi.x, yi = pi.y, zi = pi.z;
       int ci = cell_of[i] - base_cell;
       int c2 = ci % g.n[2], c1 = (ci / g.n[2]) % g.n[1], c0 = ci / (g.n[1] * g.n[2]);
       int lo[3], hi[3], cc[3] = {c0, c1, c2};
@@ -181,9 +181,10 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
               if (p < e) {
                 j = cell_atoms[p];
                 if (j != i && handled[j]) {
-                  double dx = mic(xi - x[j], L[0]);
-                  double dy = mic(yi - y[j], L[1]);
-                  double dz = mic(zi - z[j], L[2]);
+                  double4 pj = pos[j];
+                  double dx = mic(xi - pj.x, L[0]);
+                  double dy = mic(yi - pj.y, L[1]);
+                  double dz = mic(zi - pj.z, L[2]);
                   double d2 = dx * dx + dy * dy + dz * dz;
                   ok = d2 < range2;
                 }
@@ -237,14 +238,12 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
   }
 }
 
-template __global__ void k_nl_build<64>(const int*, const int*, int, int, const double*, const double*,
-                                        const double*, const unsigned char*, Box, const CellGrid*, const int*,
-                                        double, const int*, const int*, const int*, int*, int*, double*, unsigned*,
-                                        int*);
-template __global__ void k_nl_build<512>(const int*, const int*, int, int, const double*, const double*,
-                                         const double*, const unsigned char*, Box, const CellGrid*, const int*,
-                                         double, const int*, const int*, const int*, int*, int*, double*,
-                                         unsigned*, int*);
+template __global__ void k_nl_build<64>(const int*, const int*, int, int, const double4*, const unsigned char*,
+                                        Box, const CellGrid*, const int*, double, const int*, const int*,
+                                        const int*, int*, int*, double*, unsigned*, int*);
+template __global__ void k_nl_build<512>(const int*, const int*, int, int, const double4*, const unsigned char*,
+                                         Box, const CellGrid*, const int*, double, const int*, const int*,
+                                         const int*, int*, int*, double*, unsigned*, int*);
 
 // ------------------------------------------------------------ skin test
 // Top-2 accumulated displacement per system and list (NeighbourList.cxx:
@@ -367,7 +366,7 @@ __global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* __restrict
     if (!(jion || zbl_target(A.elem[j]))) continue;
     if (jion && j < i) continue;  // ion-ion pairs once
     int a = i < j ? i : j, b = i < j ? j : i;
-    D3 rv = depos(A.x, A.y, A.z, a, b, bx);
+    D3 rv = depos(A.p, a, b, bx);
     double d2 = norm2(rv);
     if (d2 > R1 * R1) continue;
     double r = sqrt(d2);
@@ -402,9 +401,11 @@ __global__ void k_drift(Atoms A, const double* __restrict__ dtp) {
   double dx = A.vx[i] * dt + A.ax[i] * dt * dt / 2.0;
   double dy = A.vy[i] * dt + A.ay[i] * dt * dt / 2.0;
   double dz = A.vz[i] * dt + A.az[i] * dt * dt / 2.0;
-  A.x[i] += dx;
-  A.y[i] += dy;
-  A.z[i] += dz;
+  double x = A.x[i] + dx, y = A.y[i] + dy, z = A.z[i] + dz;
+  A.x[i] = x;
+  A.y[i] = y;
+  A.z[i] = z;
+  A.p[i] = make_double4(x, y, z, 0.0);
   A.disp_s[3 * i] += dx;
   A.disp_s[3 * i + 1] += dy;
   A.disp_s[3 * i + 2] += dz;
@@ -445,9 +446,24 @@ __global__ void k_kick(Atoms A, Box bx, const double* __restrict__ dtp) {
     A.vz[i] = hz + az * dt / 2.0;
   }
   const double* L = box_of(bx, i);
-  wrap(A.x[i], L[0], A.pcx[i]);
-  wrap(A.y[i], L[1], A.pcy[i]);
-  wrap(A.z[i], L[2], A.pcz[i]);
+  double x = A.x[i], y = A.y[i], z = A.z[i];
+  int cx = A.pcx[i], cy = A.pcy[i], cz = A.pcz[i];
+  wrap(x, L[0], cx);
+  wrap(y, L[1], cy);
+  wrap(z, L[2], cz);
+  A.x[i] = x;
+  A.y[i] = y;
+  A.z[i] = z;
+  A.pcx[i] = cx;
+  A.pcy[i] = cy;
+  A.pcz[i] = cz;
+  A.p[i] = make_double4(x, y, z, 0.0);
+}
+
+// Refreshes the packed positions after the host wrote x, y, z.
+__global__ void k_pack_pos(Atoms A) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < A.n) A.p[i] = make_double4(A.x[i], A.y[i], A.z[i], 0.0);
 }
 
 // First force evaluation (iteration 0, SimLoop.cxx:305-321): a = -grad/M.

@@ -4,6 +4,14 @@
 
 namespace mdg {
 
+// One bond-list slot of atom a: the minimum-image vector x_a - x_k, r, the
+// REBO cutoff f and df/dr, in one 32-byte (FP32) or 64-byte (FP64) record so
+// that a bond is one contiguous load.
+template <class F>
+struct alignas(8 * sizeof(F)) BondRec {
+  F x, y, z, r, f, df, pad0, pad1;
+};
+
 // Device view of the atoms and of the per-step derived data (SoA in HBM).
 // A batch of independent systems is one array of atoms: system s owns atoms
 // [sys_off[s], sys_off[s+1]) and sys[i] names atom i's system.
@@ -13,6 +21,7 @@ struct Atoms {
   int* sys;
   int* sys_off;
   double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az;
+  double4* p;                 // packed copy of x, y, z for gathers (kept in step by every writer)
   double* grad;   // n*3 interleaved, dE/dx
   double *mass, *Z;
   int *type, *elem;
@@ -26,8 +35,8 @@ struct Atoms {
   int bond_cap;               // capacity of the (i<j) bond array
   int* bcnt;
   int* bj;
-  double *br, *bf, *bdf, *bx, *by, *bz;       // FP64 geometry (cm)
-  float *brf, *bff, *bdff, *bxf, *byf, *bzf;  // FP32 geometry (Angstrom), mixed precision
+  BondRec<double>* b64;       // FP64 bond geometry (cm)
+  BondRec<float>* b32;        // FP32 bond geometry (Angstrom), mixed precision
   double *Nt, *NH, *NC;       // REBO coordination sums (REBO.hpp:250-258)
 };
 
@@ -38,16 +47,16 @@ struct CellGrid {
 };
 
 __global__ void k_cell_clear(const int* flag, const int* sflags, int list_id, const int* cell_off, int* cnt);
-__global__ void k_cell_assign(const int* flag, const int* sflags, int list_id, int n, const double* x,
-                              const double* y, const double* z, const CellGrid* grids, const int* cell_off,
-                              const int* sys, int* cell_of, int* cell_cnt);
+__global__ void k_cell_assign(const int* flag, const int* sflags, int list_id, int n, const double4* p,
+                              const CellGrid* grids, const int* cell_off, const int* sys, int* cell_of,
+                              int* cell_cnt);
 __global__ void k_cell_scan(const int* flag, const int* sflags, int list_id, const int* cell_off, const int* sys_off,
                             int* cnt, int* start);
 __global__ void k_cell_fill(const int* flag, const int* sflags, int list_id, int n, const int* sys,
                             const int* cell_of, const int* start, int* cursor, int* cell_atoms);
 template <int CAP>
-__global__ void k_nl_build(const int* flag, const int* sflags, int list_id, int n, const double* x, const double* y,
-                           const double* z, const unsigned char* handled, Box bx, const CellGrid* grids,
+__global__ void k_nl_build(const int* flag, const int* sflags, int list_id, int n, const double4* p,
+                           const unsigned char* handled, Box bx, const CellGrid* grids,
                            const int* cell_off, double range2, const int* cell_of, const int* start,
                            const int* cell_atoms, int* cnt, int* list, double* disp, unsigned* err, int* split);
 __global__ void k_sys_disp(const int* sys_off, const double* ds, const double* dl, double lim_s, double lim_l,
@@ -79,6 +88,7 @@ __global__ void k_brenner_bonds(Atoms A, const Tables* T, Box bx, const int2* bo
 __global__ void k_drift(Atoms A, const double* dtp);
 __global__ void k_kick(Atoms A, Box bx, const double* dtp);
 __global__ void k_init_acc(Atoms A);
+__global__ void k_pack_pos(Atoms A);
 __global__ void k_sys_energy(int nterms, const int* sys_off, int n, const double* eatom, double* eacc);
 __global__ void k_sys_step_end(Atoms A, int nterms, const double* eatom, double* dtp, int fixed_dt, double* out,
                                const int* stepidx, int nsys);
