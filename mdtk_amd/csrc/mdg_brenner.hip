@@ -71,8 +71,7 @@ __global__ void __launch_bounds__(256) k_brenner_bonds(Atoms A, const Tables* __
                                                       double* __restrict__ eatom, unsigned* err) {
   const BrennerParams& P = T->bren;
   int nbt = min(*nbonds, A.bond_cap);
-  Range rg = block_range(nbt);
-  for (int t = rg.lo + threadIdx.x; t < rg.hi; t += blockDim.x) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nbt; t += gridDim.x * blockDim.x) {
     int2 bd = bonds[t];
     int i = bd.x, j = bd.y;
     int ti = A.type[i], tj = A.type[j];

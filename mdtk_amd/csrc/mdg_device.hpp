@@ -151,31 +151,6 @@ struct Box {
 };
 __device__ __forceinline__ const double* box_of(const Box& bx, int a) { return bx.L + 3 * bx.sys[a]; }
 
-// XCD-aware work assignment.  Workgroups are dispatched round-robin over the
-// 8 XCDs, each with its own L2: block b runs on XCD b % 8.  xcd_block()
-// renumbers the blocks so that XCD x gets the x-th contiguous run of numbers,
-// and block_range() gives each block one contiguous chunk of the work units,
-// so one XCD's L2 serves one contiguous part of the data (for a batch, whole
-// systems) instead of slices of every part.
-__device__ __forceinline__ int xcd_block() {
-  const int nx = 8;
-  int nb = gridDim.x, b = blockIdx.x;
-  int q = nb / nx, r = nb % nx, x = b % nx, k = b / nx;
-  return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
-}
-struct Range {
-  int lo, hi;
-};
-__device__ __forceinline__ Range block_range(int units) {
-  long long nb = gridDim.x;
-  long long per = (units + nb - 1) / nb;
-  long long lo = per * xcd_block();
-  long long hi = lo + per;
-  if (lo > units) lo = units;
-  if (hi > units) hi = units;
-  return Range{(int)lo, (int)hi};
-}
-
 // Positions are gathered from a packed copy p[a] = (x, y, z, 0): one 32-byte
 // load per atom instead of three scattered ones.
 __device__ __forceinline__ D3 depos(const double4* __restrict__ p, int a, int b, const Box& bx) {

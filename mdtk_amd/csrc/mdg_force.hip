@@ -91,7 +91,7 @@ __device__ __forceinline__ double coord_excl(double Nsum, bool subtract, R r, R 
 // minimum-image vector.
 template <class R>
 __global__ void k_coord(Atoms A, const TablesT<R>* __restrict__ T, Box bx, unsigned* err) {
-  int i = xcd_block() * blockDim.x + threadIdx.x;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
   BondRec<R>* B = brec<R>(A);
   int ti = A.type[i];
@@ -535,8 +535,7 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
   const double gs = T->eunit / T->lunit;
   const double su = 1.0 / T->eunit;  // erg -> the table's energy unit (adjoint seed)
   int nbt = min(*nbonds, A.bond_cap);
-  Range rg = block_range(nbt);
-  for (int t = rg.lo + threadIdx.x; t < rg.hi; t += blockDim.x) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nbt; t += gridDim.x * blockDim.x) {
     const ReboParamsT<double>& P = T64->rebo;
     int2 bd = bonds[t];
     BPair<R> p;
@@ -765,8 +764,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
   const unsigned long long below = (1ull << lane) - 1ull;
   const unsigned long long mine = (LW == 64) ? ~0ull : (((1ull << LW) - 1ull) << (lane / LW * LW));
   long long np = 0;
-  Range rg = block_range(A.n);
-  for (int i = rg.lo + gid; i < rg.hi; i += G) {
+  for (int i = blockIdx.x * G + gid; i < A.n; i += gridDim.x * G) {
     int ti = A.type[i];
     if (ti > T_H) continue;  // group-uniform
     const double* Lb = box_of(bx, i);
@@ -1031,8 +1029,7 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
                                               unsigned* err, double* __restrict__ dth) {
   const double gs = T->eunit / T->lunit;
   int nq = min(*qcount, qcap);
-  Range rg = block_range(nq);
-  for (int t = rg.lo + threadIdx.x; t < rg.hi; t += blockDim.x) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nq; t += gridDim.x * blockDim.x) {
     const LJParamsT<R>& L = T->lj;
     const ReboParamsT<R>& P = T->rebo;
     int4 qd = queue[t];
@@ -1120,8 +1117,7 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
   long long nq = 0;
   const double gs = T->eunit / T->lunit;
   int nbt = min(*nbonds, A.bond_cap);
-  Range rg = block_range(nbt);
-  for (int t = rg.lo + threadIdx.x; t < rg.hi; t += blockDim.x) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nbt; t += gridDim.x * blockDim.x) {
     const ReboParamsT<R>& P = T->rebo;
     int2 bd = bonds[t];
     int i = bd.x, j = bd.y;

@@ -134,8 +134,7 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
   if (!*flag) return;
   __shared__ int buf[4][CAP];
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  Range rg = block_range(n);
-  for (int i = rg.lo + w; i < rg.hi; i += 4) {
+  for (int i = blockIdx.x * 4 + w; i < n; i += gridDim.x * 4) {
     int sy = bx.sys[i];
     if (!sflags[2 * sy + list_id]) continue;  // wave-uniform
     const CellGrid& g = grids[sy];
