@@ -239,6 +239,7 @@ __device__ void bo_forward(const Atoms& A, const TablesT<R>* __restrict__ T, con
   st.NCc[0] = coord_excl<R>(A.NC[p.i], p.tj == T_C, p.r, R2ij, p.f, err);
   st.NHc[1] = coord_excl<R>(A.NH[p.j], p.ti == T_H, p.r, R2ij, p.f, err);
   st.NCc[1] = coord_excl<R>(A.NC[p.j], p.ti == T_C, p.r, R2ij, p.f, err);
+#pragma unroll
   for (int s = 0; s < 2; s++) {
     Side<R> q = side_of(p, s);
     double D = 1.0;
@@ -269,6 +270,7 @@ __device__ void bo_forward(const Atoms& A, const TablesT<R>* __restrict__ T, con
   double pij = pow(st.D[0], -0.5);
   double pji = pow(st.D[1], -0.5);
   // Nconj partial sums: carbon neighbours k of the centre, f_ka * S'(N_ka)
+#pragma unroll
   for (int s = 0; s < 2; s++) {
     Side<R> q = side_of(p, s);
     double sum = 0.0;
@@ -360,6 +362,7 @@ __device__ void bo_backward(const Atoms& A, const TablesT<R>* __restrict__ T, co
   V3<R> gi = v3<R>(0, 0, 0), gj = v3<R>(0, 0, 0);
   double dN[2] = {0, 0}, dNH[2] = {0, 0}, dNC[2] = {0, 0}, dNconj = 0.0;
   // sigma-pi terms
+#pragma unroll
   for (int s = 0; s < 2; s++) {
     Side<R> q = side_of(p, s);
     V3<R>& ga = (s == 0) ? gi : gj;
@@ -479,6 +482,7 @@ __device__ void bo_backward(const Atoms& A, const TablesT<R>* __restrict__ T, co
     }
   }
   // coordination derivatives (dfThem, REBO.cxx:408-421) and Nconj
+#pragma unroll
   for (int s = 0; s < 2; s++) {
     Side<R> q = side_of(p, s);
     V3<R>& ga = (s == 0) ? gi : gj;
@@ -589,6 +593,7 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
         unsafeAtomicAdd(th + 0, f * P.A[a][b] * ea / r);
         unsafeAtomicAdd(th + 1, f * (1.0 + P.Q[a][b] / r) * ea);
         unsafeAtomicAdd(th + 2, -r * VR);
+#pragma unroll
         for (int n = 0; n < 3; n++) {
           double en = exp(-P.beta[n][a][b] * r);
           unsafeAtomicAdd(th + 3 + n, -bb * f * en);
@@ -631,14 +636,23 @@ __device__ __forceinline__ PathBonds<R> load_path(const Atoms& A, int i, int key
   PathBonds<R> pb;
   pb.hops = key >> 24;
   pb.nodes[0] = i;
-  for (int q = 0; q < pb.hops; q++) {
-    int o = pb.nodes[q] * A.maxb + ((key >> (16 - 8 * q)) & 0xff);
-    BondRec<R> b = B[o];
-    pb.nodes[q + 1] = A.bj[o];
-    pb.f[q] = b.f;
-    pb.df[q] = b.df;
-    pb.r[q] = b.r;
-    pb.rv[q] = v3<R>(b.x, b.y, b.z);
+#pragma unroll
+  for (int q = 0; q < 3; q++) {  // fixed trip count: the arrays stay in registers
+    if (q < pb.hops) {
+      int o = pb.nodes[q] * A.maxb + ((key >> (16 - 8 * q)) & 0xff);
+      BondRec<R> b = B[o];
+      pb.nodes[q + 1] = A.bj[o];
+      pb.f[q] = b.f;
+      pb.df[q] = b.df;
+      pb.r[q] = b.r;
+      pb.rv[q] = v3<R>(b.x, b.y, b.z);
+    } else {
+      pb.nodes[q + 1] = -1;
+      pb.f[q] = R(1);
+      pb.df[q] = R(0);
+      pb.r[q] = R(1);
+      pb.rv[q] = v3<R>(0, 0, 0);
+    }
   }
   return pb;
 }
@@ -647,18 +661,22 @@ __device__ __forceinline__ PathBonds<R> load_path(const Atoms& A, int i, int key
 template <class R>
 __device__ __forceinline__ R path_weight(const PathBonds<R>& pb) {
   R w = pb.f[0];
-  for (int q = 1; q < pb.hops; q++) w = w * pb.f[q];
+#pragma unroll
+  for (int q = 1; q < 3; q++)
+    if (q < pb.hops) w = w * pb.f[q];
   return w;
 }
 
 // Gradient of C_ij = 1 - prod f over the path's bonds, seeded with V.
 template <class R>
 __device__ __forceinline__ void cij_backward(const PathBonds<R>& pb, R V, double* __restrict__ grad, double gs) {
-  for (int q = 0; q < pb.hops; q++) {
-    if (pb.df[q] == R(0)) continue;
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    if (q >= pb.hops || pb.df[q] == R(0)) continue;
     R mm = R(1);
-    for (int u = 0; u < pb.hops; u++)
-      if (u != q) mm *= pb.f[u];
+#pragma unroll
+    for (int u = 0; u < 3; u++)
+      if (u != q && u < pb.hops) mm *= pb.f[u];
     R c = pb.df[q] * (-mm * V);
     V3<R> v = pb.rv[q] * (c / pb.r[q]);
     scatter3<R>(grad, pb.nodes[q], v, gs);
@@ -1006,6 +1024,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       add_energy(eatom, i, e);
     }
     if constexpr (TH) {
+#pragma unroll
       for (int q = 0; q < 2; q++) {
         double de = group_sum<LW>(th_e[q]), ds = group_sum<LW>(th_s[q]);
         if (sl == 0 && (de != 0.0 || ds != 0.0)) {
