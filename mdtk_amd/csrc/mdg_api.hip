@@ -71,7 +71,7 @@ struct mdg_ctx {
   int bond_cap = 0;
   int4* queue = nullptr;
   int* qcount = nullptr;
-  int qcap = 0;
+  int qsub = 0;  // entries per b* sub-queue (kQueues of them)
   double* eatom = nullptr;  // nterms*n per-atom energies of the last force call
   double* dtheta = nullptr;  // kNumTheta*nsys parameter gradients (allocated on first use)
   bool theta_mode = false;
@@ -265,6 +265,15 @@ inline int blocks(long long work, int bs = 256, int cap = 1 << 20) {
   return (int)b;
 }
 
+// Entries in the b* sub-queues after the last force call.
+hipError_t queued(mdg_ctx* c, int& nq) {
+  std::vector<int> h((size_t)kQueues * kQStride);
+  hipError_t e = hipMemcpy(h.data(), c->qcount, h.size() * sizeof(int), hipMemcpyDeviceToHost);
+  nq = 0;
+  for (int q = 0; q < kQueues; q++) nq += std::min(h[(size_t)q * kQStride], c->qsub);
+  return e;
+}
+
 // Neighbour-list rebuild for one list, gated on the device flag.
 void nl_build(mdg_ctx* c, bool shortlist) {
   const int* flag = c->flags + (shortlist ? 0 : 1);
@@ -397,7 +406,7 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
         constexpr int LW = decltype(lw)::value;
         int grid = tu.grid_lj ? tu.grid_lj : blocks(n, 4 * (64 / LW), 65536);
         k_lj_wave<R, TH, decltype(occ)::value, LW><<<grid, 256, 0, c->stream>>>(
-            A, T, c->box, A.grad, c->eatom + (size_t)c->slot_lj * n, c->queue, c->qcount, c->qcap,
+            A, T, c->box, A.grad, c->eatom + (size_t)c->slot_lj * n, c->queue, c->qcount, c->qsub,
             c->counters + 2, c->derr, dth);
       };
       using I1 = std::integral_constant<int, 1>;
@@ -415,7 +424,7 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
     launch(c, "k_bstar", [&] {
       auto go = [&](auto occ) {
         k_bstar<R, TH, decltype(occ)::value><<<sgrid, 256, 0, c->stream>>>(
-            A, T, c->box, c->queue, c->qcount, c->qcap, A.grad, c->eatom + (size_t)c->slot_lj * n, c->derr, dth);
+            A, T, c->box, c->queue, c->qcount, c->qsub, A.grad, c->eatom + (size_t)c->slot_lj * n, c->derr, dth);
       };
       if constexpr (!TH) {
         if (tu.occ_bstar == 2) return go(std::integral_constant<int, 2>());
@@ -444,7 +453,7 @@ void force_call(mdg_ctx* c) {
     k_zero<<<blocks((long long)nt * n, 256, 4096), 256, 0, c->stream>>>(c->eatom, nt * n);
   });
   k_zero_i<<<1, 64, 0, c->stream>>>(c->nbonds, 1);
-  k_zero_i<<<1, 64, 0, c->stream>>>(c->qcount, 1);
+  k_zero_i<<<kQueues * kQStride / 256, 256, 0, c->stream>>>(c->qcount, kQueues * kQStride);
   k_zero_counts<<<1, 64, 0, c->stream>>>(c->counters);
   if (c->mixed) force_terms<float, false>(c, c->dTf);
   else if (c->theta_mode) force_terms<double, true>(c, c->dT);
@@ -548,7 +557,7 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
     }
   }
   if (dalloc(c, c->flags, 2) || dalloc(c, c->counters, 8) || dalloc(c, c->derr, 1) || dalloc(c, c->nbonds, 1) ||
-      dalloc(c, c->qcount, 1)) {
+      dalloc(c, c->qcount, kQueues * kQStride)) {
     g_create_err = c->err;
     delete c;
     return 1;
@@ -671,8 +680,8 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   c->bond_cap = n * A.maxb / 2 + 64;
   A.bond_cap = c->bond_cap;
   r |= dalloc(c, c->bonds, c->bond_cap);
-  c->qcap = 32 * n + 1024;
-  if (c->slot_lj >= 0) r |= dalloc(c, c->queue, c->qcap);
+  c->qsub = (32 * n + kQueues - 1) / kQueues + 4096;  // >= 16 atoms x 128 pairs for any one workgroup pass
+  if (c->slot_lj >= 0) r |= dalloc(c, c->queue, (size_t)c->qsub * kQueues);
   r |= dalloc(c, c->handled_s, n) | dalloc(c, c->handled_l, n);
   if (r) return 1;
   // cell grids, one per system and list
@@ -1104,7 +1113,7 @@ int mdg_autotune(mdg_ctx* c, int reps, double* report) {
   }
   int nb = 0, nq = 0;
   HIPCHK(c, hipMemcpy(&nb, c->nbonds, 4, hipMemcpyDeviceToHost));
-  HIPCHK(c, hipMemcpy(&nq, c->qcount, 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, queued(c, nq));
   auto grids = [](long long units, int per_block) {
     std::vector<int> g;
     for (int v : {1024, 2048, 4096, 8192, 16384, 32768}) g.push_back(v);
@@ -1195,7 +1204,7 @@ int mdg_counters(mdg_ctx* c, int64_t out[8]) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(h, c->counters, sizeof(h), hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(&nb, c->nbonds, 4, hipMemcpyDeviceToHost));
-  HIPCHK(c, hipMemcpy(&nq, c->qcount, 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, queued(c, nq));
   int n = c->n;
   std::vector<int> cs(n), cl(n);
   HIPCHK(c, hipMemcpy(cs.data(), c->A.nls_cnt, n * 4, hipMemcpyDeviceToHost));

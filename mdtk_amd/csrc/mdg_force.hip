@@ -755,10 +755,13 @@ __device__ __forceinline__ int group_scan(int x, int sl) {
 template <class R, bool TH, int OCC, int LW>
 __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
                                                      double* __restrict__ grad, double* __restrict__ eatom,
-                                                     int4* __restrict__ queue, int* __restrict__ qcount, int qcap,
+                                                     int4* __restrict__ queue, int* __restrict__ qcount, int qsub,
                                                      long long* __restrict__ npairs, unsigned* err,
                                                      double* __restrict__ dth) {
   constexpr int G = 4 * (64 / LW);  // atoms in flight per workgroup
+  // this workgroup's b* sub-queue
+  int* const qc = qcount + (blockIdx.x & (kQueues - 1)) * kQStride;
+  int4* const qb = queue + (size_t)(blockIdx.x & (kQueues - 1)) * qsub;
   constexpr bool PACK = !std::is_same<R, double>::value;
   constexpr int RC = PACK ? 1 : kPathCap;
   __shared__ int s_pair[G][kPairCap];
@@ -973,10 +976,10 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
         unsigned long long m = __ballot(1);
         int leader = __ffsll((long long)m) - 1;
         int base = 0;
-        if (lane == leader) base = atomicAdd(qcount, __popcll(m));
+        if (lane == leader) base = atomicAdd(qc, __popcll(m));
         base = __shfl(base, leader, 64);
         int slot = base + __popcll(m & below);
-        if (slot < qcap) queue[slot] = make_int4(i, j, pkey, 0);
+        if (slot < qsub) qb[slot] = make_int4(i, j, pkey, 0);
         else raise(err, ERR_QUEUE_OVERFLOW);
         continue;
       }
@@ -1044,14 +1047,39 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
 template <class R, bool TH, int OCC>
 __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
                                               const int4* __restrict__ queue, const int* __restrict__ qcount,
-                                              int qcap, double* __restrict__ grad, double* __restrict__ eatom,
+                                              int qsub, double* __restrict__ grad, double* __restrict__ eatom,
                                               unsigned* err, double* __restrict__ dth) {
+  static_assert(kQueues == 256, "one thread per sub-queue in the prefix below (blockDim 256)");
   const double gs = T->eunit / T->lunit;
-  int nq = min(*qcount, qcap);
+  // s_pre[q]: first flat index of sub-queue q (exclusive prefix of the counts)
+  __shared__ int s_pre[kQueues + 1];
+  __shared__ int s_wsum[4];
+  {
+    int q = threadIdx.x;
+    int c = min(qcount[q * kQStride], qsub);
+    int x = c;  // inclusive scan within the wave, then across the four waves
+    for (int o = 1; o < 64; o <<= 1) {
+      int y = __shfl_up(x, o, 64);
+      if ((q & 63) >= o) x += y;
+    }
+    if ((q & 63) == 63) s_wsum[q >> 6] = x;
+    __syncthreads();
+    for (int w = 0; w < (q >> 6); w++) x += s_wsum[w];
+    s_pre[q + 1] = x;
+    if (q == 0) s_pre[0] = 0;
+    __syncthreads();
+  }
+  const int nq = s_pre[kQueues];
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nq; t += gridDim.x * blockDim.x) {
     const LJParamsT<R>& L = T->lj;
     const ReboParamsT<R>& P = T->rebo;
-    int4 qd = queue[t];
+    int lo = 0, hi = kQueues - 1;  // last sub-queue whose first index is <= t
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (s_pre[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    int4 qd = queue[(size_t)lo * qsub + (t - s_pre[lo])];
     int i = qd.x, j = qd.y;
     int pkey = qd.z;  // C_ij path (0: none), bond slots as in k_lj_wave
     int ti = A.type[i], tj = A.type[j];

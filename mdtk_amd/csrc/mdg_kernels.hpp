@@ -73,11 +73,17 @@ __global__ void k_coord(Atoms A, const TablesT<R>* T, Box bx, unsigned* err);
 template <class R, bool TH, int OCC>
 __global__ void k_rebo_bonds(Atoms A, const TablesT<R>* T, const Tables* T64, Box bx, const int2* bonds,
                              const int* nbonds, double* grad, double* eatom, unsigned* err, double* dth);
+// The b* work queue is kQueues sub-queues of qsub entries each; sub-queue q
+// holds entries [q * qsub, (q + 1) * qsub) and its count sits at
+// qcount[q * kQStride], one count per 128-byte line, so the LJ kernel's
+// appends spread over many addresses instead of serialising on one.
+constexpr int kQueues = 256;
+constexpr int kQStride = 32;
 template <class R, bool TH, int OCC, int LW>
 __global__ void k_lj_wave(Atoms A, const TablesT<R>* T, Box bx, double* grad, double* eatom, int4* queue,
-                          int* qcount, int qcap, long long* npairs, unsigned* err, double* dth);
+                          int* qcount, int qsub, long long* npairs, unsigned* err, double* dth);
 template <class R, bool TH, int OCC>
-__global__ void k_bstar(Atoms A, const TablesT<R>* T, Box bx, const int4* queue, const int* qcount, int qcap,
+__global__ void k_bstar(Atoms A, const TablesT<R>* T, Box bx, const int4* queue, const int* qcount, int qsub,
                         double* grad, double* eatom, unsigned* err, double* dth);
 template <class R>
 __global__ void k_etors(Atoms A, const TablesT<R>* T, Box bx, const int2* bonds, const int* nbonds, int compat,
