@@ -18,7 +18,7 @@ resident in HBM before every timed region.
 Prints ONE JSON line.  `value` = sum over trajectories of atoms x steps /
 wall seconds of the timed region (hipEvents on the engine's stream, bracketed
 by device synchronisation).  `roofline` prices the dominant kernel against
-MI355X's vector peak of its compute type; `cpu_baseline` times the oracle (the
+MI355X's HBM peak by its algorithmic bytes; `cpu_baseline` times the oracle (the
 CPU restatement of the reference's algorithm, single thread) on a bounded
 sample of the same workload.
 """
@@ -34,18 +34,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "atom-steps/s on 1 GPU, AIREBO: mixed+batched vs plain FP64 at force err ≤1e-5"
-FP64_PEAK_TFLOPS = 78.6
-FP32_PEAK_TFLOPS = 157.3
 POTS = ("FBZL", "REBO", "AIREBO", "ETors")
-
-# Algorithmic work per evaluation (forward + adjoint), tallied from the kernel
-# source with FMA=2, add/mul=1, div/sqrt=8, exp/sin/cos=20, pow=40 flops;
-# per-neighbour terms use the measured neighbour counts of the workload.
-FLOP_BOND_ORDER_BASE = 900.0       # pair set-up, coordination, pi_rc tricubic, p_ij pow, Nconj
-FLOP_PER_SIGMA_NEIGHBOUR = 260.0   # cos theta, G quintic, cutoff/ExpTerm, adjoint scatter (per k, both passes)
-FLOP_LJ_PAIR = 120.0               # cutoff, sigma/r powers, C_ij merge, gradient
-FLOP_TORSION_QUAD = 420.0          # two cross products, dihedral cos, pow, adjoint of four atoms
-SIGMA_NEIGHBOURS = 2.0 * 3.3       # sigma-pi neighbours visited per bond order (both sides)
 
 KERNELS = ("k_bstar", "k_rebo_bonds", "k_lj_wave", "k_etors", "k_coord", "k_bond_enum", "k_nl_build_long",
            "k_nl_build_short", "k_kick", "k_drift", "k_sys_disp", "k_sys_step_end", "k_zero", "k_fbzl",
@@ -91,22 +80,50 @@ def kernel_times(eng, steps, dt):
     return kt
 
 
-def roofline(kt, cnt, peak):
+LJ_SHAPES = {0: (1, 64), 1: (3, 64), 2: (1, 32), 3: (1, 16)}  # autotuner's lj_shape -> (OCC, LW)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_bombard256.json")
+
+
+def lj_algorithmic_bytes(cnt, natoms):
+    """Compulsory memory traffic of one k_lj_wave launch (every byte read or
+    written once): the j > i half of each long-list row (4 B per entry); per
+    atom its list count and split (8 B), packed position (32 B), type (4 B),
+    bond count (4 B), gradient read + write (48 B) and energy (8 B); per bond,
+    both directions' records (32 B) and partner index (4 B); per queued b*
+    pair one 16-byte entry."""
+    return (4.0 * cnt["long_entries"] / 2.0 + natoms * (8 + 32 + 4 + 4 + 48 + 8)
+            + 2.0 * cnt["bonds"] * 36 + 16.0 * cnt["bstar"])
+
+
+def measured_traffic(kernel):
+    """Bytes beyond L2 per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
+    from the committed rocprofv3 counter summary of this workload."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)["bytes_per_launch"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if t is None else t["read_bytes"] + t["write_bytes"]
+
+
+def roofline(kt, cnt, natoms, tune):
+    """The dominant kernel against MI355X's HBM peak (8 TB/s): algorithmic
+    bytes per launch over the launch's average HIP-event duration."""
     dom = max(kt, key=lambda k: kt[k][0])
     dom_ms, dom_n = kt[dom]
     avg_s = dom_ms / max(dom_n, 1) / 1e3
-    per_bo = FLOP_BOND_ORDER_BASE + FLOP_PER_SIGMA_NEIGHBOUR * SIGMA_NEIGHBOURS
-    flops = {"k_bstar": cnt["bstar"] * (per_bo + FLOP_LJ_PAIR),
-             "k_rebo_bonds": cnt["bonds"] * (per_bo + 200.0),
-             "k_lj_wave": cnt["lj_pairs"] * FLOP_LJ_PAIR,
-             "k_etors": cnt["quads"] * FLOP_TORSION_QUAD}.get(dom)
-    if flops is None:
-        return {"bound": "hbm", "kernel": dom, "achieved": None, "peak": 8000.0, "unit": "GB/s", "frac": None,
-                "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": dom_n}
-    ach = flops / avg_s / 1e12
-    return {"bound": "mfma", "kernel": dom, "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
-            "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": dom_n, "flop_per_launch": flops,
-            "note": "vector-ALU peak of the compute type (no dense contraction on this path, no MFMA)"}
+    out = {"bound": "hbm", "kernel": dom, "achieved": None, "peak": 8000.0, "unit": "GB/s", "frac": None,
+           "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": dom_n}
+    if dom == "k_lj_wave":
+        occ, lw = LJ_SHAPES[int(tune.get("lj_shape", 3))]
+        name = f"k_lj_wave<float, false, {occ}, {lw}>"
+        alg = lj_algorithmic_bytes(cnt, natoms)
+        ach = alg / avg_s / 1e9
+        out.update(kernel=name, achieved=ach, frac=ach / 8000.0, traffic=measured_traffic(name),
+                   algorithmic_bytes=alg,
+                   note="gather-latency bound (dependent list -> position -> bond-record loads); "
+                        "traffic = bytes beyond L2 from rocprofv3 FETCH_SIZE/WRITE_SIZE")
+    return out
 
 
 def force_rms(systems):
@@ -206,7 +223,7 @@ def main():
     use_mixed = f_rms <= 1e-5
     value = v_mx if use_mixed else v_64b
     ms = ms_mx if use_mixed else ms_64b
-    rl = roofline(kt_mx, cnt_mx, FP32_PEAK_TFLOPS)
+    rl = roofline(kt_mx, cnt_mx, natoms, tune_mx)
     et = (res_mx["epot"] + res_mx["ekin"]) / S.EV
     out = {
         "metric": METRIC,

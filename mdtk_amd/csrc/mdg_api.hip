@@ -737,7 +737,6 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   HIPCHK(c, up(A.x, xs.data(), (size_t)n * 8));
   HIPCHK(c, up(A.y, ys.data(), (size_t)n * 8));
   HIPCHK(c, up(A.z, zs2.data(), (size_t)n * 8));
-  k_pack_pos<<<blocks(n), 256, 0, c->stream>>>(A);
   HIPCHK(c, up(A.vx, vxs.data(), (size_t)n * 8));
   HIPCHK(c, up(A.vy, vys.data(), (size_t)n * 8));
   HIPCHK(c, up(A.vz, vzs.data(), (size_t)n * 8));
@@ -752,6 +751,7 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   HIPCHK(c, up(A.pcx, px.data(), (size_t)n * 4));
   HIPCHK(c, up(A.pcy, py.data(), (size_t)n * 4));
   HIPCHK(c, up(A.pcz, pz.data(), (size_t)n * 4));
+  k_pack_pos<<<blocks(n), 256, 0, c->stream>>>(A);  // after the positions and types are uploaded
   HIPCHK(c, hipMemset(A.disp_s, 0, 3 * (size_t)n * 8));
   HIPCHK(c, hipMemset(A.disp_l, 0, 3 * (size_t)n * 8));
   HIPCHK(c, hipMemset(A.nls_cnt, 0, (size_t)n * 4));

@@ -19,6 +19,14 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Adds a per-lane tally into a global counter with one atomic per wave (all
+// 64 lanes must be active): a counter that every lane bumped itself would
+// serialise the whole grid on one address.
+__device__ __forceinline__ void count_add(long long* counter, long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd((unsigned long long*)counter, (unsigned long long)v);
+}
+
 // Parameter-gradient layout (per system, kNumTheta doubles): for the type
 // pairs p = C-C, C-H, H-H (p = ti + tj), REBO's pair parameters at 9p + {Q, A,
 // alpha, B1, B2, B3, beta1, beta2, beta3} (REBO.cxx:856-941) and the LJ
@@ -796,32 +804,35 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     const int* list = A.nll + (size_t)i * A.cap_l;
     int npair = 0;
     const double cut2 = T->lunit * T->lunit;
-    auto keep = [&](int j) -> bool {
-      if (j <= i) return false;
-      int tj = A.type[j];
-      if (tj > T_H) return false;
-      double4 pj = A.p[j];
-      double dx = mic(xi - pj.x, Lb[0]), dy = mic(yi - pj.y, Lb[1]), dz = mic(zi - pj.z, Lb[2]);
-      double d2 = dx * dx + dy * dy + dz * dz;
-      return !(d2 > double(L.R2sq[ti][tj]) * cut2);
-    };
-    for (int base = A.nll_split[i]; base < cnt; base += 2 * LW) {
-      int q0 = base + sl, q1 = base + LW + sl;
-      int j0 = (q0 < cnt) ? list[q0] : -1;
-      int j1 = (q1 < cnt) ? list[q1] : -1;
-      bool ok0 = j0 >= 0 && keep(j0);
-      bool ok1 = j1 >= 0 && keep(j1);
-      unsigned long long m0 = __ballot(ok0) & mine, m1 = __ballot(ok1) & mine;
-      if (ok0) {
-        int slot = npair + __popcll(m0 & below);
-        if (slot < kPairCap) s_pair[gid][slot] = j0;
+    // U chunks of LW entries per round: every list load of the round is in
+    // flight before the position gathers, and those before the compaction
+    constexpr int U = 4;
+    for (int base = A.nll_split[i]; base < cnt; base += U * LW) {
+      int jj[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        int q = base + u * LW + sl;
+        jj[u] = (q < cnt) ? list[q] : -1;
       }
-      npair += __popcll(m0);
-      if (ok1) {
-        int slot = npair + __popcll(m1 & below);
-        if (slot < kPairCap) s_pair[gid][slot] = j1;
+      double4 pj[U];
+#pragma unroll
+      for (int u = 0; u < U; u++) pj[u] = (jj[u] > i) ? A.p[jj[u]] : make_double4(0.0, 0.0, 0.0, double(T_OTHER));
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        int tj = (int)pj[u].w;  // the packed position carries the type
+        bool ok = false;
+        if (jj[u] > i && (unsigned)tj <= (unsigned)T_H) {
+          double dx = mic(xi - pj[u].x, Lb[0]), dy = mic(yi - pj[u].y, Lb[1]), dz = mic(zi - pj[u].z, Lb[2]);
+          double d2 = dx * dx + dy * dy + dz * dz;
+          ok = !(d2 > double(L.R2sq[ti][tj]) * cut2);
+        }
+        unsigned long long m = __ballot(ok) & mine;
+        if (ok) {
+          int slot = npair + __popcll(m & below);
+          if (slot < kPairCap) s_pair[gid][slot] = jj[u];
+        }
+        npair += __popcll(m);
       }
-      npair += __popcll(m1);
     }
     if (npair > kPairCap) {
       if (sl == 0) raise(err, ERR_QUEUE_OVERFLOW);
@@ -1040,7 +1051,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
   }
-  if (np) atomicAdd((unsigned long long*)npairs, (unsigned long long)np);
+  count_add(npairs, np);
 }
 
 // LJ pairs that need b*_ij (S_r != 0): the whole StrStb / VLJ / Cij chain.
@@ -1244,7 +1255,7 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
       scatter3<R>(grad, j, gj, gs);
     }
   }
-  if (nq) atomicAdd((unsigned long long*)nquads, (unsigned long long)nq);
+  count_add(nquads, nq);
 }
 
 #define MDG_INSTANTIATE_OCC(R, TH, OCC)                                                                        \

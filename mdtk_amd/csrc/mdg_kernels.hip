@@ -319,10 +319,14 @@ __global__ void k_clear_flags(int nsys, int* __restrict__ flags, int* __restrict
 }
 
 // Bonds (i<j) in the order of i's list, appended to one array; one atomic
-// per wave reserves the wave's bonds (an inclusive scan of the counts).
-__global__ void k_bond_enum(Atoms A, int2* __restrict__ out, int* __restrict__ count, int cap, unsigned* err) {
+// per workgroup (blockDim 256) reserves the group's bonds, placed by an
+// inclusive scan of the per-atom counts.
+__global__ void __launch_bounds__(256) k_bond_enum(Atoms A, int2* __restrict__ out, int* __restrict__ count, int cap,
+                                                   unsigned* err) {
+  __shared__ int s_w[4];
+  __shared__ int s_base;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int lane = threadIdx.x & 63;
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int nb = (i < A.n) ? A.bcnt[i] : 0;
   int mine = 0;
   for (int s = 0; s < nb; s++) mine += (A.bj[i * A.maxb + s] > i);
@@ -331,12 +335,16 @@ __global__ void k_bond_enum(Atoms A, int2* __restrict__ out, int* __restrict__ c
     int v = __shfl_up(incl, o, 64);
     if (lane >= o) incl += v;
   }
-  int total = __shfl(incl, 63, 64);
-  int wbase = 0;
-  if (lane == 63 && total > 0) wbase = atomicAdd(count, total);
-  wbase = __shfl(wbase, 63, 64);
+  if (lane == 63) s_w[w] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int total = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    s_base = total > 0 ? atomicAdd(count, total) : 0;
+  }
+  __syncthreads();
+  int base = s_base + incl - mine;
+  for (int u = 0; u < w; u++) base += s_w[u];
   if (!mine) return;
-  int base = wbase + incl - mine;
   for (int s = 0; s < nb; s++) {
     int j = A.bj[i * A.maxb + s];
     if (j > i) {
@@ -405,7 +413,7 @@ __global__ void k_drift(Atoms A, const double* __restrict__ dtp) {
   A.x[i] = x;
   A.y[i] = y;
   A.z[i] = z;
-  A.p[i] = make_double4(x, y, z, 0.0);
+  A.p[i] = make_double4(x, y, z, double(A.type[i]));
   A.disp_s[3 * i] += dx;
   A.disp_s[3 * i + 1] += dy;
   A.disp_s[3 * i + 2] += dz;
@@ -457,13 +465,13 @@ __global__ void k_kick(Atoms A, Box bx, const double* __restrict__ dtp) {
   A.pcx[i] = cx;
   A.pcy[i] = cy;
   A.pcz[i] = cz;
-  A.p[i] = make_double4(x, y, z, 0.0);
+  A.p[i] = make_double4(x, y, z, double(A.type[i]));
 }
 
 // Refreshes the packed positions after the host wrote x, y, z.
 __global__ void k_pack_pos(Atoms A) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < A.n) A.p[i] = make_double4(A.x[i], A.y[i], A.z[i], 0.0);
+  if (i < A.n) A.p[i] = make_double4(A.x[i], A.y[i], A.z[i], double(A.type[i]));
 }
 
 // First force evaluation (iteration 0, SimLoop.cxx:305-321): a = -grad/M.

@@ -21,7 +21,7 @@ struct Atoms {
   int* sys;
   int* sys_off;
   double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az;
-  double4* p;                 // packed copy of x, y, z for gathers (kept in step by every writer)
+  double4* p;                 // packed x, y, z and the type (as a double) for gathers; every position writer refreshes it
   double* grad;   // n*3 interleaved, dE/dx
   double *mass, *Z;
   int *type, *elem;
