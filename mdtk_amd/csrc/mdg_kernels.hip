@@ -115,9 +115,11 @@ __global__ void k_cell_fill(const int* __restrict__ flag, const int* __restrict_
 
 // Neighbour list build, one wavefront per atom (NeighbourList.cxx:31-106):
 // pair (i,j) is listed iff both atoms are handled and d^2 < ((1+0.44) Rc)^2
-// with d^2 = dx*dx+dy*dy+dz*dz of the minimum-image displacement.  Candidates
-// from the stencil cells are compacted with a ballot/popcount prefix into
-// LDS, then sorted ascending (the reference's list order) with a bitonic sort.
+// with d^2 = dx*dx+dy*dy+dz*dz of the minimum-image displacement.  The
+// candidates of all stencil cells are swept 64 per round (a prefix over the
+// cells' counts maps a lane to its cell), compacted with a ballot/popcount
+// prefix into LDS, then sorted ascending (the reference's list order) with a
+// bitonic sort.
 // Only atoms whose system asked for a rebuild (sflags[2*s+list]) are redone;
 // the grid strides over the atoms, so a launch with nothing to rebuild costs
 // a few thousand empty waves, not one per atom.
@@ -133,6 +135,7 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
                                                   int* __restrict__ split) {
   if (!*flag) return;
   __shared__ int buf[4][CAP];
+  __shared__ int s_pre[4][32], s_st[4][32];  // stencil cells: first candidate, first atom
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int i = blockIdx.x * 4 + w; i < n; i += gridDim.x * 4) {
     int sy = bx.sys[i];
@@ -162,42 +165,69 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
           hi[d] = g.n[d] - 1 - cc[d];
         }
       }
-      for (int a = lo[0]; a <= hi[0]; a++)
-        for (int b = lo[1]; b <= hi[1]; b++)
-          for (int c = lo[2]; c <= hi[2]; c++) {
-            int q0 = cc[0] + a, q1 = cc[1] + b, q2 = cc[2] + c;
-            if (g.periodic[0]) q0 = (q0 + g.n[0]) % g.n[0];
-            else if (q0 < 0 || q0 >= g.n[0]) continue;
-            if (g.periodic[1]) q1 = (q1 + g.n[1]) % g.n[1];
-            else if (q1 < 0 || q1 >= g.n[1]) continue;
-            if (g.periodic[2]) q2 = (q2 + g.n[2]) % g.n[2];
-            else if (q2 < 0 || q2 >= g.n[2]) continue;
-            int cid = base_cell + (q0 * g.n[1] + q1) * g.n[2] + q2;
-            int s = start[cid], e = start[cid + 1];
-            for (int base = s; base < e; base += 64) {
-              int p = base + lane;
-              bool ok = false;
-              int j = -1;
-              if (p < e) {
-                j = cell_atoms[p];
-                if (j != i && handled[j]) {
-                  double4 pj = pos[j];
-                  double dx = mic(xi - pj.x, L[0]);
-                  double dy = mic(yi - pj.y, L[1]);
-                  double dz = mic(zi - pj.z, L[2]);
-                  double d2 = dx * dx + dy * dy + dz * dz;
-                  ok = d2 < range2;
-                }
-              }
-              unsigned long long mask = __ballot(ok);
-              int before = __popcll(mask & ((1ull << lane) - 1ull));
-              if (ok) {
-                int slot = m + before;
-                if (slot < CAP) my[slot] = j;
-              }
-              m += __popcll(mask);
-            }
+      // the stencil's cells (at most 27), one per lane: start and atom count,
+      // then an exclusive prefix over the counts, so that the candidates of
+      // all cells are swept 64 at a time (the list is sorted afterwards, so
+      // the sweep order does not matter)
+      int e1 = hi[1] - lo[1] + 1, e2 = hi[2] - lo[2] + 1;
+      int ncs = (hi[0] - lo[0] + 1) * e1 * e2;
+      int cst = 0, ccn = 0;
+      if (lane < ncs) {
+        int q0 = cc[0] + lo[0] + lane / (e1 * e2), q1 = cc[1] + lo[1] + (lane / e2) % e1, q2 = cc[2] + lo[2] + lane % e2;
+        bool in = true;
+        if (g.periodic[0]) q0 = (q0 + g.n[0]) % g.n[0];
+        else in = in && q0 >= 0 && q0 < g.n[0];
+        if (g.periodic[1]) q1 = (q1 + g.n[1]) % g.n[1];
+        else in = in && q1 >= 0 && q1 < g.n[1];
+        if (g.periodic[2]) q2 = (q2 + g.n[2]) % g.n[2];
+        else in = in && q2 >= 0 && q2 < g.n[2];
+        if (in) {
+          int cid = base_cell + (q0 * g.n[1] + q1) * g.n[2] + q2;
+          cst = start[cid];
+          ccn = start[cid + 1] - cst;
+        }
+      }
+      int incl = ccn;
+      for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+      }
+      int total = __shfl(incl, 63, 64);
+      if (lane < 32) {
+        s_pre[w][lane] = incl - ccn;
+        s_st[w][lane] = cst;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      for (int base = 0; base < total; base += 64) {
+        int t = base + lane;
+        bool ok = false;
+        int j = -1;
+        if (t < total) {
+          int lo_c = 0, hi_c = ncs - 1;  // last stencil cell whose first candidate is <= t
+          while (lo_c < hi_c) {
+            int mid = (lo_c + hi_c + 1) >> 1;
+            if (s_pre[w][mid] <= t) lo_c = mid;
+            else hi_c = mid - 1;
           }
+          j = cell_atoms[s_st[w][lo_c] + (t - s_pre[w][lo_c])];
+          if (j != i && handled[j]) {
+            double4 pj = pos[j];
+            double dx = mic(xi - pj.x, L[0]);
+            double dy = mic(yi - pj.y, L[1]);
+            double dz = mic(zi - pj.z, L[2]);
+            double d2 = dx * dx + dy * dy + dz * dz;
+            ok = d2 < range2;
+          }
+        }
+        unsigned long long mask = __ballot(ok);
+        int before = __popcll(mask & ((1ull << lane) - 1ull));
+        if (ok) {
+          int slot = m + before;
+          if (slot < CAP) my[slot] = j;
+        }
+        m += __popcll(mask);
+      }
     }
     if (m > CAP) {
       if (lane == 0) raise(err, CAP >= 256 ? ERR_LONG_NL_OVERFLOW : ERR_SHORT_NL_OVERFLOW);
