@@ -36,7 +36,7 @@ sys.path.insert(0, ROOT)
 METRIC = "atom-steps/s on 1 GPU, AIREBO: mixed+batched vs plain FP64 at force err ≤1e-5"
 POTS = ("FBZL", "REBO", "AIREBO", "ETors")
 
-KERNELS = ("k_bstar", "k_rebo_bonds", "k_lj_wave", "k_etors", "k_coord", "k_bond_enum", "k_nl_build_long",
+KERNELS = ("k_bstar", "k_rebo_bonds", "k_lj_wave", "k_pair_rows", "k_etors", "k_coord", "k_bond_enum", "k_nl_build_long",
            "k_nl_build_short", "k_kick", "k_drift", "k_sys_disp", "k_sys_step_end", "k_zero", "k_fbzl",
            "k_cell_assign", "k_cell_scan", "k_cell_fill", "k_cell_clear")
 

@@ -178,7 +178,9 @@ int mdg_synchronize(mdg_ctx* ctx);
 /* Interaction counts of the last force call (for roofline accounting):
  * out[0] bonds, out[1] LJ pairs inside the cutoff, out[2] b* evaluations,
  * out[3] torsion quads, out[4] short-list entries, out[5] long-list entries,
- * out[6] neighbour-list rebuilds so far.  No reference counterpart. */
+ * out[6] neighbour-list rebuilds so far, out[7] LJ pair rows built so far
+ * (one per atom and build; the rows are the long-list entries within the LJ
+ * cutoff plus a 1 A margin).  No reference counterpart. */
 int mdg_counters(mdg_ctx* ctx, int64_t out[8]);
 
 #ifdef __cplusplus

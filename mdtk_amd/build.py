@@ -18,7 +18,8 @@ SOURCES = ["mdg_api.hip", "mdg_kernels.hip", "mdg_force.hip", "mdg_brenner.hip",
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-         "-Wno-unused-result", "-Wno-unused-value", "-I" + os.path.join(HERE, "..", "include")]
+         "-Wno-unused-result", "-Wno-unused-value", "-Wl,-z,defs",  # an unresolved symbol fails the link
+         "-I" + os.path.join(HERE, "..", "include")]
 
 
 def build(verbose: bool = False) -> str:

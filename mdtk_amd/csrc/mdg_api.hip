@@ -64,8 +64,10 @@ struct mdg_ctx {
   int *cell_of_s = nullptr, *cnt_s = nullptr, *start_s = nullptr, *atoms_s = nullptr;
   int *cell_of_l = nullptr, *cnt_l = nullptr, *start_l = nullptr, *atoms_l = nullptr;
   unsigned char *handled_s = nullptr, *handled_l = nullptr;
-  int* flags = nullptr;   // [0] short, [1] long rebuild requested by some system
+  int* flags = nullptr;   // [0] short, [1] long list, [2] LJ pair rows: rebuild requested by some system
   int* sflags = nullptr;  // [2*s + list] rebuild requested by system s
+  int* pflags = nullptr;  // [s] LJ pair rows of system s to be rebuilt
+  double pair_margin = 1.0e-8;  // LJ pair-row margin (cm)
   int2* bonds = nullptr;
   int* nbonds = nullptr;
   int bond_cap = 0;
@@ -80,7 +82,8 @@ struct mdg_ctx {
   double* dtp = nullptr;    // nsys
   double* stepout = nullptr;
   int step_cap = 0;         // steps per output chunk
-  long long* counters = nullptr;  // [0] short rebuilds [1] long rebuilds [2] LJ pairs [3] quads
+  long long* counters = nullptr;  // [0] short rebuilds [1] long rebuilds (per system and rebuild)
+  long long* tally = nullptr;     // kTallySlots spread slots: [0] LJ pairs [1] quads [2] pair rows built
   unsigned* derr = nullptr;
   long iteration = 0;
 
@@ -274,6 +277,16 @@ hipError_t queued(mdg_ctx* c, int& nq) {
   return e;
 }
 
+// Requests a rebuild of every system's LJ pair rows at the next force call.
+int mark_pair_rows(mdg_ctx* c) {
+  if (!c->A.pl || c->nsys == 0) return 0;
+  std::vector<int> one((size_t)c->nsys, 1);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(c->pflags, one.data(), one.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->flags + 2, one.data(), 4, hipMemcpyHostToDevice));
+  return 0;
+}
+
 // Neighbour-list rebuild for one list, gated on the device flag.
 void nl_build(mdg_ctx* c, bool shortlist) {
   const int* flag = c->flags + (shortlist ? 0 : 1);
@@ -360,8 +373,10 @@ __global__ void k_fill(double* p, int n, double v) {
   if (i < n) p[i] = v;
 }
 
-__global__ void k_zero_counts(long long* counters) {
-  if (threadIdx.x == 2 || threadIdx.x == 3) counters[threadIdx.x] = 0;
+// Clears the per-call tallies (LJ pairs, quads) of every slot; blockDim kTallySlots.
+__global__ void k_zero_counts(long long* tally) {
+  tally[threadIdx.x * kTallyStride] = 0;
+  tally[threadIdx.x * kTallyStride + 1] = 0;
 }
 
 // One force call (FProxy::operator(), FProxy.cxx:45-61): lists, coordination,
@@ -401,13 +416,17 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
     });
   }
   if (c->slot_lj >= 0) {
+    launch(c, "k_pair_rows", [&] {
+      k_pair_rows<R><<<blocks(n, 16, 16384), 256, 0, c->stream>>>(A, T, c->box, c->flags + 2, c->pflags,
+                                                                 c->pair_margin, c->tally + 2);
+    });
     launch(c, "k_lj_wave", [&] {
       auto go = [&](auto occ, auto lw) {
         constexpr int LW = decltype(lw)::value;
         int grid = tu.grid_lj ? tu.grid_lj : blocks(n, 4 * (64 / LW), 65536);
         k_lj_wave<R, TH, decltype(occ)::value, LW><<<grid, 256, 0, c->stream>>>(
             A, T, c->box, A.grad, c->eatom + (size_t)c->slot_lj * n, c->queue, c->qcount, c->qsub,
-            c->counters + 2, c->derr, dth);
+            c->tally + 0, c->derr, dth);
       };
       using I1 = std::integral_constant<int, 1>;
       if constexpr (TH) {
@@ -435,7 +454,7 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   if (c->slot_tors >= 0) {
     launch(c, "k_etors", [&] {
       k_etors<R><<<tgrid, 256, 0, c->stream>>>(A, T, c->box, c->bonds, c->nbonds, c->cfg.etors_compat, A.grad,
-                                               c->eatom + (size_t)c->slot_tors * n, c->counters + 3);
+                                               c->eatom + (size_t)c->slot_tors * n, c->tally + 1);
     });
   }
 }
@@ -454,7 +473,7 @@ void force_call(mdg_ctx* c) {
   });
   k_zero_i<<<1, 64, 0, c->stream>>>(c->nbonds, 1);
   k_zero_i<<<kQueues * kQStride / 256, 256, 0, c->stream>>>(c->qcount, kQueues * kQStride);
-  k_zero_counts<<<1, 64, 0, c->stream>>>(c->counters);
+  k_zero_counts<<<1, kTallySlots, 0, c->stream>>>(c->tally);
   if (c->mixed) force_terms<float, false>(c, c->dTf);
   else if (c->theta_mode) force_terms<double, true>(c, c->dT);
   else force_terms<double, false>(c, c->dT);
@@ -464,7 +483,7 @@ void force_call(mdg_ctx* c) {
                                                c->eatom + (size_t)c->slot_fbzl * n);
     });
   }
-  k_clear_flags<<<blocks(2LL * c->nsys), 256, 0, c->stream>>>(c->nsys, c->flags, c->sflags);
+  k_clear_flags<<<blocks(2LL * c->nsys), 256, 0, c->stream>>>(c->nsys, c->flags, c->sflags, c->pflags);
 }
 }  // namespace
 
@@ -556,13 +575,14 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
       return 1;
     }
   }
-  if (dalloc(c, c->flags, 2) || dalloc(c, c->counters, 8) || dalloc(c, c->derr, 1) || dalloc(c, c->nbonds, 1) ||
+  if (dalloc(c, c->flags, 3) || dalloc(c, c->counters, 8) || dalloc(c, c->tally, kTallySlots * kTallyStride) || dalloc(c, c->derr, 1) || dalloc(c, c->nbonds, 1) ||
       dalloc(c, c->qcount, kQueues * kQStride)) {
     g_create_err = c->err;
     delete c;
     return 1;
   }
   hipMemset(c->counters, 0, 8 * sizeof(long long));
+  hipMemset(c->tally, 0, kTallySlots * kTallyStride * sizeof(long long));
   hipMemset(c->derr, 0, sizeof(unsigned));
   *out = c;
   return 0;
@@ -658,7 +678,7 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   A.sys = sysd;
   c->box.L = Ld;
   c->box.sys = sysd;
-  r |= dalloc(c, c->sflags, 2 * (size_t)nsys) | dalloc(c, c->dtp, nsys) | dalloc(c, c->eacc, 8 * (size_t)nsys);
+  r |= dalloc(c, c->sflags, 2 * (size_t)nsys) | dalloc(c, c->pflags, nsys) | dalloc(c, c->dtp, nsys) | dalloc(c, c->eacc, 8 * (size_t)nsys);
   r |= dalloc(c, c->eatom, c->kinds.size() * (size_t)n);
   r |= dalloc(c, A.x, n) | dalloc(c, A.y, n) | dalloc(c, A.z, n);
   r |= dalloc(c, A.vx, n) | dalloc(c, A.vy, n) | dalloc(c, A.vz, n);
@@ -682,6 +702,12 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   r |= dalloc(c, c->bonds, c->bond_cap);
   c->qsub = (32 * n + kQueues - 1) / kQueues + 4096;  // >= 16 atoms x 128 pairs for any one workgroup pass
   if (c->slot_lj >= 0) r |= dalloc(c, c->queue, (size_t)c->qsub * kQueues);
+  A.cap_p = 128;
+  A.pl = nullptr;
+  A.pl_cnt = nullptr;
+  A.disp_p = nullptr;
+  if (c->slot_lj >= 0)
+    r |= dalloc(c, A.pl, (size_t)n * A.cap_p) | dalloc(c, A.pl_cnt, n) | dalloc(c, A.disp_p, 3 * (size_t)n);
   r |= dalloc(c, c->handled_s, n) | dalloc(c, c->handled_l, n);
   if (r) return 1;
   // cell grids, one per system and list
@@ -754,16 +780,19 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   k_pack_pos<<<blocks(n), 256, 0, c->stream>>>(A);  // after the positions and types are uploaded
   HIPCHK(c, hipMemset(A.disp_s, 0, 3 * (size_t)n * 8));
   HIPCHK(c, hipMemset(A.disp_l, 0, 3 * (size_t)n * 8));
+  if (A.disp_p) HIPCHK(c, hipMemset(A.disp_p, 0, 3 * (size_t)n * 8));
+  if (A.pl_cnt) HIPCHK(c, hipMemset(A.pl_cnt, 0xff, (size_t)n * 4));  // -1: no pair row yet
   HIPCHK(c, hipMemset(A.nls_cnt, 0, (size_t)n * 4));
   HIPCHK(c, hipMemset(A.nll_cnt, 0, (size_t)n * 4));
   if (A.nll_split) HIPCHK(c, hipMemset(A.nll_split, 0, (size_t)n * 4));
   HIPCHK(c, hipMemset(A.bcnt, 0, (size_t)n * 4));
   HIPCHK(c, up(c->handled_s, hs.data(), n));
   HIPCHK(c, up(c->handled_l, hl.data(), n));
-  int fl[2] = {1, 1};
+  int fl[3] = {1, 1, 1};
   std::vector<int> sfl(2 * (size_t)nsys, 1);
   HIPCHK(c, up(c->flags, fl, sizeof(fl)));
   HIPCHK(c, up(c->sflags, sfl.data(), sfl.size() * 4));
+  HIPCHK(c, up(c->pflags, sfl.data(), (size_t)nsys * 4));
   HIPCHK(c, up(c->dtp, c->dt_host.data(), (size_t)nsys * 8));
   c->iteration = 0;
   return 0;
@@ -830,7 +859,7 @@ int mdg_set_params(mdg_ctx* c, const double* theta) {
     HIPCHK(c, hipMemcpy(c->dTf, &hf, sizeof(hf), hipMemcpyHostToDevice));
   }
   c->host_T = t;
-  return 0;
+  return mark_pair_rows(c);  // the LJ cutoffs bound the pair rows
 }
 
 int mdg_energy_grad_params(mdg_ctx* c, double* e_terms, double* grad, double* dtheta) {
@@ -865,6 +894,7 @@ int mdg_nl_export(mdg_ctx* c, int term, int32_t* counts, int32_t* idx, int64_t c
   nl_build(c, shortlist);
   k_clear_list<<<blocks(c->nsys), 256, 0, c->stream>>>(c->nsys, shortlist ? 0 : 1, c->flags, c->sflags);
   if (check_err(c)) return 1;
+  if (!shortlist && mark_pair_rows(c)) return 1;  // rows are filtered from the long list just rebuilt
   int stride = shortlist ? c->A.cap_s : c->A.cap_l;
   std::vector<int> cnt(n), lst((size_t)n * stride), ty(n), el(n);
   HIPCHK(c, hipMemcpy(cnt.data(), shortlist ? c->A.nls_cnt : c->A.nll_cnt, n * 4, hipMemcpyDeviceToHost));
@@ -927,8 +957,8 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
   auto body = [&]() {
     launch(c, "k_drift", [&] { k_drift<<<blocks(n), 256, 0, c->stream>>>(A, c->dtp); });
     launch(c, "k_sys_disp", [&] {
-      k_sys_disp<<<nsys, 256, 0, c->stream>>>(A.sys_off, A.disp_s, A.disp_l, lim_s, lim_l, c->sflags, c->flags,
-                                              c->counters);
+      k_sys_disp<<<nsys, 256, 0, c->stream>>>(A.sys_off, A.disp_s, A.disp_l, A.disp_p, lim_s, lim_l,
+                                              c->pair_margin, c->sflags, c->pflags, c->flags, c->counters);
     });
     force_call(c);
     launch(c, "k_kick", [&] { k_kick<<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->dtp); });
@@ -1027,10 +1057,11 @@ int mdg_set_positions(mdg_ctx* c, const double* xyz, const int32_t* pbc_count, c
     }
   }
   if (!dr) {
-    int fl[2] = {1, 1};
+    int fl[3] = {1, 1, 1};
     std::vector<int> sfl(2 * (size_t)c->nsys, 1);
     HIPCHK(c, hipMemcpy(c->flags, fl, sizeof(fl), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->sflags, sfl.data(), sfl.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->pflags, sfl.data(), (size_t)c->nsys * 4, hipMemcpyHostToDevice));
     return 0;
   }
   // accumulate dr into both lists' displacements, then the skin test
@@ -1043,10 +1074,15 @@ int mdg_set_positions(mdg_ctx* c, const double* xyz, const int32_t* pbc_count, c
   }
   HIPCHK(c, hipMemcpy(c->A.disp_s, s.data(), 3 * (size_t)n * 8, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->A.disp_l, l.data(), 3 * (size_t)n * 8, hipMemcpyHostToDevice));
+  if (c->A.disp_p) {
+    HIPCHK(c, hipMemcpy(s.data(), c->A.disp_p, 3 * (size_t)n * 8, hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < (size_t)3 * n; q++) s[q] += dr[q];
+    HIPCHK(c, hipMemcpy(c->A.disp_p, s.data(), 3 * (size_t)n * 8, hipMemcpyHostToDevice));
+  }
   double lim_s = c->have_short ? 0.44 * c->rc_short : 1e300;
   double lim_l = c->have_long ? 0.44 * c->rc_long : 1e300;
-  k_sys_disp<<<c->nsys, 256, 0, c->stream>>>(c->A.sys_off, c->A.disp_s, c->A.disp_l, lim_s, lim_l, c->sflags,
-                                             c->flags, c->counters);
+  k_sys_disp<<<c->nsys, 256, 0, c->stream>>>(c->A.sys_off, c->A.disp_s, c->A.disp_l, c->A.disp_p, lim_s, lim_l,
+                                             c->pair_margin, c->sflags, c->pflags, c->flags, c->counters);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1203,6 +1239,11 @@ int mdg_counters(mdg_ctx* c, int64_t out[8]) {
   int nb = 0, nq = 0;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(h, c->counters, sizeof(h), hipMemcpyDeviceToHost));
+  std::vector<long long> t((size_t)kTallySlots * kTallyStride);
+  HIPCHK(c, hipMemcpy(t.data(), c->tally, t.size() * sizeof(long long), hipMemcpyDeviceToHost));
+  long long tk[3] = {0, 0, 0};
+  for (int q = 0; q < kTallySlots; q++)
+    for (int k = 0; k < 3; k++) tk[k] += t[(size_t)q * kTallyStride + k];
   HIPCHK(c, hipMemcpy(&nb, c->nbonds, 4, hipMemcpyDeviceToHost));
   HIPCHK(c, queued(c, nq));
   int n = c->n;
@@ -1215,13 +1256,13 @@ int mdg_counters(mdg_ctx* c, int64_t out[8]) {
     sl += cl[i];
   }
   out[0] = nb;
-  out[1] = h[2];
+  out[1] = tk[0];
   out[2] = nq;
-  out[3] = h[3];
+  out[3] = tk[1];
   out[4] = ss;
   out[5] = sl;
   out[6] = h[0] + h[1];
-  out[7] = 0;
+  out[7] = tk[2];
   return 0;
 }
 }

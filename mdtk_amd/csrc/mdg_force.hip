@@ -19,12 +19,13 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Adds a per-lane tally into a global counter with one atomic per wave (all
-// 64 lanes must be active): a counter that every lane bumped itself would
-// serialise the whole grid on one address.
-__device__ __forceinline__ void count_add(long long* counter, long long v) {
+// Adds a per-lane tally into this workgroup's slot of a spread tally (kind
+// pointer = tally + k) with one atomic per wave (all 64 lanes must be
+// active): one address bumped by every wave of a grid would serialise it.
+__device__ __forceinline__ void count_add(long long* kind, long long v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd((unsigned long long*)counter, (unsigned long long)v);
+  if ((threadIdx.x & 63) == 0 && v)
+    atomicAdd((unsigned long long*)(kind + (blockIdx.x & (kTallySlots - 1)) * kTallyStride), (unsigned long long)v);
 }
 
 // Parameter-gradient layout (per system, kNumTheta doubles): for the type
@@ -740,6 +741,82 @@ __device__ __forceinline__ int group_scan(int x, int sl) {
   return x;
 }
 
+// LJ pair rows: for each atom i (C or H) of a system whose pflags is set,
+// the j > i entries of i's long-list row (C or H) with minimum-image distance
+// <= sqrt(R2sq) + margin, kept in row order.  Between two builds no atom of
+// the system moved more than the margin in sum with any other, so every pair
+// inside the LJ cutoff now was inside cutoff + margin then: the LJ pass finds
+// exactly the pairs the full row would give it, in the same order.  One group
+// of 16 lanes per atom, four chunks of loads in flight per round.
+template <class R>
+__global__ void __launch_bounds__(256) k_pair_rows(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
+                                                   const int* __restrict__ pflag, const int* __restrict__ pflags,
+                                                   double margin, long long* __restrict__ rebuilt) {
+  if (!*pflag) return;
+  constexpr int LW = 16, U = 4, G = 256 / LW;
+  const LJParamsT<R>& L = T->lj;
+  const int lane = threadIdx.x & 63;
+  const int sl = lane % LW;
+  const int gid = threadIdx.x / LW;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const unsigned long long mine = ((1ull << LW) - 1ull) << (lane / LW * LW);
+  long long rows = 0;
+  for (int i = blockIdx.x * G + gid; i < A.n; i += gridDim.x * G) {
+    if (!pflags[A.sys[i]]) continue;  // group-uniform
+    if (sl == 0) {
+      A.disp_p[3 * i] = 0.0;
+      A.disp_p[3 * i + 1] = 0.0;
+      A.disp_p[3 * i + 2] = 0.0;
+      rows++;
+    }
+    int ti = A.type[i];
+    int m = 0;
+    if (ti <= T_H) {
+      double lim2[2];
+#pragma unroll
+      for (int t = 0; t < 2; t++) {
+        double rc = sqrt(double(L.R2sq[ti][t])) * T->lunit + margin;
+        lim2[t] = rc * rc;
+      }
+      double4 pi4 = A.p[i];
+      const double* Lb = box_of(bx, i);
+      int cnt = A.nll_cnt[i];
+      const int* list = A.nll + (size_t)i * A.cap_l;
+      int* out = A.pl + (size_t)i * A.cap_p;
+      for (int base = A.nll_split[i]; base < cnt; base += U * LW) {
+        int jj[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          int q = base + u * LW + sl;
+          jj[u] = (q < cnt) ? list[q] : -1;
+        }
+        double4 pj[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          pj[u] = (jj[u] > i) ? A.p[jj[u]] : make_double4(0.0, 0.0, 0.0, double(T_OTHER));
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          int tj = (int)pj[u].w;
+          bool ok = false;
+          if (jj[u] > i && (unsigned)tj <= (unsigned)T_H) {
+            double dx = mic(pi4.x - pj[u].x, Lb[0]), dy = mic(pi4.y - pj[u].y, Lb[1]),
+                   dz = mic(pi4.z - pj[u].z, Lb[2]);
+            ok = dx * dx + dy * dy + dz * dz <= lim2[tj];
+          }
+          unsigned long long b = __ballot(ok) & mine;
+          if (ok) {
+            int slot = m + __popcll(b & below);
+            if (slot < A.cap_p) out[slot] = jj[u];
+          }
+          m += __popcll(b);
+        }
+      }
+    }
+    if (sl == 0) A.pl_cnt[i] = (m > A.cap_p) ? -1 : m;  // -1: the LJ pass sweeps the long row
+  }
+  count_add(rebuilt, rows);
+}
+
 // AIREBO::operator() (AIREBO_LJ.cxx:34-77).  A group of LW lanes handles
 // one atom i, so a wavefront keeps 64/LW atoms (and their chains of
 // dependent gathers) in flight; LW is a tuning choice.
@@ -800,14 +877,21 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     double4 pi4 = A.p[i];
     double xi = pi4.x, yi = pi4.y, zi = pi4.z;
     // phase 0: compact the LJ pairs
-    int cnt = A.nll_cnt[i];
-    const int* list = A.nll + (size_t)i * A.cap_l;
+    // the pair row, or the j > i part of the long row when the pair row overflowed
+    int cnt = A.pl_cnt[i];
+    const int* list = A.pl + (size_t)i * A.cap_p;
+    int first = 0;
+    if (cnt < 0) {
+      cnt = A.nll_cnt[i];
+      list = A.nll + (size_t)i * A.cap_l;
+      first = A.nll_split[i];
+    }
     int npair = 0;
     const double cut2 = T->lunit * T->lunit;
     // U chunks of LW entries per round: every list load of the round is in
     // flight before the position gathers, and those before the compaction
     constexpr int U = 4;
-    for (int base = A.nll_split[i]; base < cnt; base += U * LW) {
+    for (int base = first; base < cnt; base += U * LW) {
       int jj[U];
 #pragma unroll
       for (int u = 0; u < U; u++) {
@@ -1269,7 +1353,9 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
 #define MDG_INSTANTIATE(R)                                                                                     \
   template __global__ void k_coord<R>(Atoms, const TablesT<R>*, Box, unsigned*);                              \
   template __global__ void k_etors<R>(Atoms, const TablesT<R>*, Box, const int2*, const int*, int, double*,   \
-                                      double*, long long*);
+                                      double*, long long*);                                                  \
+  template __global__ void k_pair_rows<R>(Atoms, const TablesT<R>*, Box, const int*, const int*, double,      \
+                                          long long*);
 MDG_INSTANTIATE(double)
 MDG_INSTANTIATE(float)
 MDG_INSTANTIATE_TH(double, false, 1, 64)

@@ -300,35 +300,43 @@ __device__ __forceinline__ void top2_wave(double& d1, double& d2) {
   }
 }
 
-// One block per system; sflags[2*s+{0,1}] and the any-system flags[0..1].
+// One block per system; sflags[2*s+{0,1}], pflags[s] and the any-system
+// flags[0..2].  The LJ pair rows are redone with the long list or when their
+// own displacements pass the margin.
 __global__ void k_sys_disp(const int* __restrict__ sys_off, const double* __restrict__ ds,
-                           const double* __restrict__ dl, double lim_s, double lim_l, int* __restrict__ sflags,
-                           int* __restrict__ flags, long long* __restrict__ rebuilds) {
-  __shared__ double sh[4][4];
+                           const double* __restrict__ dl, const double* __restrict__ dp, double lim_s, double lim_l,
+                           double lim_p, int* __restrict__ sflags, int* __restrict__ pflags, int* __restrict__ flags,
+                           long long* __restrict__ rebuilds) {
+  __shared__ double sh[4][6];
   int s = blockIdx.x;
   int b = sys_off[s], e = sys_off[s + 1];
-  double a1 = 0, a2 = 0, b1 = 0, b2 = 0;
+  double a1 = 0, a2 = 0, b1 = 0, b2 = 0, c1 = 0, c2 = 0;
+  auto len = [](const double* d, int i) {
+    return sqrt(d[3 * i] * d[3 * i] + d[3 * i + 1] * d[3 * i + 1] + d[3 * i + 2] * d[3 * i + 2]);
+  };
   for (int i = b + threadIdx.x; i < e; i += blockDim.x) {
-    double m = sqrt(ds[3 * i] * ds[3 * i] + ds[3 * i + 1] * ds[3 * i + 1] + ds[3 * i + 2] * ds[3 * i + 2]);
-    top2_insert(m, a1, a2);
-    double q = sqrt(dl[3 * i] * dl[3 * i] + dl[3 * i + 1] * dl[3 * i + 1] + dl[3 * i + 2] * dl[3 * i + 2]);
-    top2_insert(q, b1, b2);
+    top2_insert(len(ds, i), a1, a2);
+    top2_insert(len(dl, i), b1, b2);
+    if (dp) top2_insert(len(dp, i), c1, c2);
   }
   top2_wave(a1, a2);
   top2_wave(b1, b2);
+  top2_wave(c1, c2);
   int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
-    sh[w][0] = a1; sh[w][1] = a2; sh[w][2] = b1; sh[w][3] = b2;
+    sh[w][0] = a1; sh[w][1] = a2; sh[w][2] = b1; sh[w][3] = b2; sh[w][4] = c1; sh[w][5] = c2;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double x1 = 0, x2 = 0, y1 = 0, y2 = 0;
+    double x1 = 0, x2 = 0, y1 = 0, y2 = 0, z1 = 0, z2 = 0;
     for (int q = 0; q < (int)(blockDim.x >> 6); q++) {
       top2_merge(x1, x2, sh[q][0], sh[q][1]);
       top2_merge(y1, y2, sh[q][2], sh[q][3]);
+      top2_merge(z1, z2, sh[q][4], sh[q][5]);
     }
     int fs = (x1 + x2 > lim_s) ? 1 : 0;
     int fl = (y1 + y2 > lim_l) ? 1 : 0;
+    int fp = (dp && (fl || z1 + z2 > lim_p)) ? 1 : 0;
     sflags[2 * s] |= fs;
     sflags[2 * s + 1] |= fl;
     if (fs) {
@@ -339,13 +347,18 @@ __global__ void k_sys_disp(const int* __restrict__ sys_off, const double* __rest
       atomicOr(&flags[1], 1);
       atomicAdd((unsigned long long*)&rebuilds[1], 1ull);
     }
+    if (fp) {
+      pflags[s] = 1;
+      atomicOr(&flags[2], 1);
+    }
   }
 }
 
-__global__ void k_clear_flags(int nsys, int* __restrict__ flags, int* __restrict__ sflags) {
+__global__ void k_clear_flags(int nsys, int* __restrict__ flags, int* __restrict__ sflags, int* __restrict__ pflags) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 2) flags[i] = 0;
+  if (i < 3) flags[i] = 0;
   if (i < 2 * nsys) sflags[i] = 0;
+  if (i < nsys) pflags[i] = 0;
 }
 
 // Bonds (i<j) in the order of i's list, appended to one array; one atomic
@@ -431,7 +444,7 @@ __global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* __restrict
 
 // ------------------------------------------------------------ Verlet
 // eq.1 of SimLoop::executeMain (SimLoop.cxx:323-332): x += v dt + a dt^2/2,
-// accumulated into both neighbour lists' displacements.
+// accumulated into both neighbour lists' (and the LJ pair rows') displacements.
 __global__ void k_drift(Atoms A, const double* __restrict__ dtp) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n || A.fixed[i]) return;
@@ -450,6 +463,11 @@ __global__ void k_drift(Atoms A, const double* __restrict__ dtp) {
   A.disp_l[3 * i] += dx;
   A.disp_l[3 * i + 1] += dy;
   A.disp_l[3 * i + 2] += dz;
+  if (A.disp_p) {
+    A.disp_p[3 * i] += dx;
+    A.disp_p[3 * i + 1] += dy;
+    A.disp_p[3 * i + 2] += dz;
+  }
 }
 
 __device__ __forceinline__ void wrap(double& c, double L, int& cnt) {

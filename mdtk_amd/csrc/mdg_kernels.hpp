@@ -31,6 +31,14 @@ struct Atoms {
   int cap_s, cap_l;           // row strides of the short / long lists
   int *nls_cnt, *nls, *nll_cnt, *nll;
   int* nll_split;             // per long-list row: index of the first neighbour j > i
+  // LJ pair rows: the j > i entries of each long-list row that lay within
+  // the pair's LJ cutoff plus a margin when the row was built, in row order;
+  // pl_cnt[i] = -1 when they did not fit cap_p (the LJ pass then sweeps the
+  // long row).  Rebuilt with the long list and whenever the two largest
+  // displacements since the last build (disp_p) add up to more than the margin.
+  int cap_p;
+  int *pl_cnt, *pl;
+  double* disp_p;
   int maxb;                   // bond-list capacity per atom
   int bond_cap;               // capacity of the (i<j) bond array
   int* bcnt;
@@ -59,9 +67,11 @@ __global__ void k_nl_build(const int* flag, const int* sflags, int list_id, int 
                            const unsigned char* handled, Box bx, const CellGrid* grids,
                            const int* cell_off, double range2, const int* cell_of, const int* start,
                            const int* cell_atoms, int* cnt, int* list, double* disp, unsigned* err, int* split);
-__global__ void k_sys_disp(const int* sys_off, const double* ds, const double* dl, double lim_s, double lim_l,
-                           int* sflags, int* flags, long long* rebuilds);
-__global__ void k_clear_flags(int nsys, int* flags, int* sflags);
+// flags[0..2]: some system rebuilds its short list / long list / LJ pair
+// rows; sflags[2*s+{0,1}] and pflags[s] per system.
+__global__ void k_sys_disp(const int* sys_off, const double* ds, const double* dl, const double* dp, double lim_s,
+                           double lim_l, double lim_p, int* sflags, int* pflags, int* flags, long long* rebuilds);
+__global__ void k_clear_flags(int nsys, int* flags, int* sflags, int* pflags);
 __global__ void k_bond_enum(Atoms A, int2* out, int* count, int cap, unsigned* err);
 // Energies go to eatom (per-atom, per chain member), owned by the first atom
 // of the bond, pair or quadruplet.
@@ -79,6 +89,14 @@ __global__ void k_rebo_bonds(Atoms A, const TablesT<R>* T, const Tables* T64, Bo
 // appends spread over many addresses instead of serialising on one.
 constexpr int kQueues = 256;
 constexpr int kQStride = 32;
+// Event tallies (LJ pairs, torsion quads, pair rows built) are spread over
+// kTallySlots slots, one per 128-byte line, picked by workgroup: tally kind k
+// of slot q is tally[q * kTallyStride + k]; the host adds the slots up.
+constexpr int kTallySlots = 256;
+constexpr int kTallyStride = 16;
+template <class R>
+__global__ void k_pair_rows(Atoms A, const TablesT<R>* T, Box bx, const int* pflag, const int* pflags,
+                            double margin, long long* rebuilt);
 template <class R, bool TH, int OCC, int LW>
 __global__ void k_lj_wave(Atoms A, const TablesT<R>* T, Box bx, double* grad, double* eatom, int4* queue,
                           int* qcount, int qsub, long long* npairs, unsigned* err, double* dth);

@@ -263,5 +263,6 @@ class Engine:
     def counters(self):
         out = (ctypes.c_int64 * 8)()
         self._chk(_lib.mdg_counters(self._h, out))
-        keys = ["bonds", "lj_pairs", "bstar", "quads", "short_entries", "long_entries", "rebuilds", "_"]
+        keys = ["bonds", "lj_pairs", "bstar", "quads", "short_entries", "long_entries", "rebuilds",
+                "pair_rows_built"]
         return dict(zip(keys, list(out)))
