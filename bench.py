@@ -48,8 +48,8 @@ def log(msg):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--ntraj", type=int, default=256)
     ap.add_argument("--dropin-sample", type=int, default=8, help="trajectories timed in the FP64 drop-in form")
     ap.add_argument("--no-cpu", action="store_true")
@@ -60,13 +60,18 @@ def parse():
 
 def timed(eng, steps, warmup, dt):
     """Milliseconds for `steps` Verlet iterations between two HIP events on the
-    engine's stream, with the device synchronised before and after."""
+    engine's stream, with the device synchronised before and after; the
+    per-step outputs and the list rebuilds that fell inside the window."""
     eng.run(max(warmup, 1), fixed_dt=dt)
     eng.synchronize()
+    c0 = eng.counters()
     eng.record(0)
     res = eng.run(steps, fixed_dt=dt)
     eng.record(1)
     eng.synchronize()
+    c1 = eng.counters()
+    res["rebuilds"] = {"neighbour_lists (per system)": c1["rebuilds"] - c0["rebuilds"],
+                       "lj_pair_rows (per atom)": c1["pair_rows_built"] - c0["pair_rows_built"]}
     return eng.elapsed_ms(0, 1), res
 
 
@@ -255,6 +260,7 @@ def main():
         "kernels_ms_per_step": {k: v[0] / max(a.steps, 1) for k, v in kt_mx.items()},
         "counters": cnt_mx,
         "energy_check_eV": {"max_abs_dE_tot_over_steps": float(np.abs(et - et[0]).max())},
+        "rebuilds_in_timed_window": res_mx["rebuilds"],
     }
     if not a.no_extra:
         pe = S.polyethylene(24, 24, 16)
