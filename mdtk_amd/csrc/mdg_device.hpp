@@ -151,8 +151,8 @@ struct Box {
 };
 __device__ __forceinline__ const double* box_of(const Box& bx, int a) { return bx.L + 3 * bx.sys[a]; }
 
-// Positions are gathered from a packed copy p[a] = (x, y, z, 0): one 32-byte
-// load per atom instead of three scattered ones.
+// Positions are gathered from a packed copy p[a] = (x, y, z, type): one
+// 32-byte load per atom instead of three scattered ones.
 __device__ __forceinline__ D3 depos(const double4* __restrict__ p, int a, int b, const Box& bx) {
   const double* L = box_of(bx, a);
   double4 pa = p[a], pb = p[b];
@@ -283,10 +283,91 @@ __device__ __forceinline__ void atomic_add3(double* g, int a, D3 v) {
   unsafeAtomicAdd(g + 3 * a + 2, v.z);
 }
 
+// Workgroup gradient cache: S slots of LDS keyed by atom index (linear
+// probing from a multiplicative hash).  A scatter finds or claims its atom's
+// slot and adds there with FP64 LDS atomics; after 8 probes it adds into the
+// global gradient instead.  flush() moves every claimed slot into the global
+// gradient with one atomic per component, so the atoms that a workgroup's
+// bonds or pairs share cost one global update per flush instead of one per
+// contribution.  clear() and flush() are called by the whole workgroup with a
+// barrier between them and any add().
+#define MDG_LDS __attribute__((address_space(3)))
+template <int S>
+struct GradCache {
+  static_assert((S & (S - 1)) == 0 && S >= 64, "S: power of two");
+  MDG_LDS int* key;     // [S], -1 when free
+  MDG_LDS double* val;  // [3 S]
+  MDG_LDS int* fill;    // slots claimed since the last clear
+  double* grad;         // global gradient, n*3
+
+  __device__ __forceinline__ GradCache(int* k, double* v, int* f, double* g)
+      : key((MDG_LDS int*)k), val((MDG_LDS double*)v), fill((MDG_LDS int*)f), grad(g) {}
+
+  __device__ __forceinline__ void clear() const {
+    for (int q = threadIdx.x; q < S; q += blockDim.x) {
+      key[q] = -1;
+      val[3 * q] = 0.0;
+      val[3 * q + 1] = 0.0;
+      val[3 * q + 2] = 0.0;
+    }
+    if (threadIdx.x == 0) *fill = 0;
+  }
+  __device__ __forceinline__ void add(int a, double x, double y, double z) const {
+    constexpr int kBits = __builtin_ctz(S);
+    int h = (int)(((unsigned)a * 2654435761u) >> (32 - kBits));
+#pragma unroll 1
+    for (int p = 0; p < 8; p++) {
+      int k = __hip_atomic_load(&key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (k == -1) {
+        int expect = -1;
+        if (__hip_atomic_compare_exchange_strong(&key[h], &expect, a, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP)) {
+          k = a;
+          __hip_atomic_fetch_add(fill, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+          k = expect;
+        }
+      }
+      if (k == a) {
+        __hip_atomic_fetch_add(&val[3 * h], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&val[3 * h + 1], y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&val[3 * h + 2], z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        return;
+      }
+      h = (h + 1) & (S - 1);
+    }
+    atomic_add3(grad, a, d3(x, y, z));
+  }
+  __device__ __forceinline__ void flush() const {
+    for (int q = threadIdx.x; q < S; q += blockDim.x) {
+      int a = key[q];
+      if (a >= 0) atomic_add3(grad, a, d3(val[3 * q], val[3 * q + 1], val[3 * q + 2]));
+    }
+  }
+  // Called by the whole workgroup after a barrier: empties the cache when it
+  // is more than half full (the decision reads one LDS value, the same for
+  // every thread).
+  __device__ __forceinline__ void drain_if_crowded() const {
+    if (*fill > S / 2) {
+      __syncthreads();
+      flush();
+      __syncthreads();
+      clear();
+    }
+    __syncthreads();
+  }
+};
+
+template <int S>
+__device__ __forceinline__ void atomic_add3(const GradCache<S>& g, int a, D3 v) {
+  g.add(a, v.x, v.y, v.z);
+}
+
 // Gradient scatter from a kernel working in unit/type R into the FP64 CGS
-// gradient: gs = (erg per energy unit) / (cm per length unit).
-template <class R>
-__device__ __forceinline__ void scatter3(double* g, int a, V3<R> v, double gs) {
+// gradient (global, or through a workgroup cache): gs = (erg per energy
+// unit) / (cm per length unit).
+template <class R, class G>
+__device__ __forceinline__ void scatter3(const G& g, int a, V3<R> v, double gs) {
   if constexpr (std::is_same<R, double>::value) {
     atomic_add3(g, a, v);
   } else {

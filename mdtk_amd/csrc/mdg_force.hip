@@ -19,6 +19,21 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// LDS gradient-cache slots per workgroup of each scattering kernel.
+constexpr int kTorsCache = 1024;
+constexpr int kBondCache = 1024;
+constexpr int kPairCache = 512;
+
+// The contiguous share [b0, b1) of nwork items of workgroup blockIdx.x, a
+// multiple of the `unit` items it takes per pass (so that its gradient cache
+// sees neighbouring work).
+__device__ __forceinline__ void block_share(int nwork, int& b0, int& b1, int unit = 256) {
+  long long per = ((long long)(nwork + gridDim.x - 1) / gridDim.x + unit - 1) / unit * unit;
+  long long lo = per * blockIdx.x;
+  b0 = (int)min(lo, (long long)nwork);
+  b1 = (int)min(lo + per, (long long)nwork);
+}
+
 // Adds a per-lane tally into this workgroup's slot of a spread tally (kind
 // pointer = tally + k) with one atomic per wave (all 64 lanes must be
 // active): one address bumped by every wave of a grid would serialise it.
@@ -238,8 +253,8 @@ __device__ __forceinline__ R cos_theta(V3<R> a, V3<R> b, R ra, R rb, unsigned* e
 // REBO.cxx:200-211, with D :235-262, p_sigma_pi :264-275, Nconj :540-624,
 // P :626-657, pi_rc :659-714, pi_dh :716-814).
 template <class R>
-__device__ void bo_forward(const Atoms& A, const TablesT<R>* __restrict__ T, const BPair<R>& p, BOState& st,
-                           unsigned* err) {
+__device__ __forceinline__ void bo_forward(const Atoms& A, const TablesT<R>* __restrict__ T, const BPair<R>& p,
+                                           BOState& st, unsigned* err) {
   const ReboParamsT<R>& P = T->rebo;
   R R2ij = P.R[p.ti][p.tj][1];
   st.N[0] = coord_excl<R>(A.Nt[p.i], true, p.r, R2ij, p.f, err);
@@ -356,17 +371,17 @@ __device__ void bo_forward(const Atoms& A, const TablesT<R>* __restrict__ T, con
 }
 
 // Scatter c * d f_ak / dx for the pair (a,k) along u_ak = rv/r (a +=, k -=).
-template <class R>
-__device__ __forceinline__ void pair_push(double* g, int k, V3<R> rv, R r, R c, V3<R>& ga, double gs) {
+template <class R, class G>
+__device__ __forceinline__ void pair_push(const G& g, int k, V3<R> rv, R r, R c, V3<R>& ga, double gs) {
   V3<R> v = rv * (c / r);
   ga = ga + v;
   scatter3<R>(g, k, -v, gs);
 }
 
 // Adjoint pass: adds W * db_ij/dx to grad (W in the kernel's energy unit).
-template <class R>
-__device__ void bo_backward(const Atoms& A, const TablesT<R>* __restrict__ T, const BPair<R>& p,
-                            const BOState& st, double W, double* __restrict__ grad, double gs, unsigned* err) {
+template <class R, class G>
+__device__ __forceinline__ void bo_backward(const Atoms& A, const TablesT<R>* __restrict__ T, const BPair<R>& p,
+                                            const BOState& st, double W, G grad, double gs, unsigned* err) {
   const ReboParamsT<R>& P = T->rebo;
   V3<R> gi = v3<R>(0, 0, 0), gj = v3<R>(0, 0, 0);
   double dN[2] = {0, 0}, dNH[2] = {0, 0}, dNC[2] = {0, 0}, dNconj = 0.0;
@@ -545,12 +560,21 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
                                                    const int2* __restrict__ bonds, const int* __restrict__ nbonds,
                                                    double* __restrict__ grad, double* __restrict__ eatom,
                                                    unsigned* err, double* __restrict__ dth) {
+  __shared__ int s_key[kBondCache];
+  __shared__ double s_val[3 * kBondCache];
+  __shared__ int s_fill;
+  GradCache<kBondCache> gc{s_key, s_val, &s_fill, grad};
+  gc.clear();
+  __syncthreads();
   const double gs = T->eunit / T->lunit;
   const double su = 1.0 / T->eunit;  // erg -> the table's energy unit (adjoint seed)
   int nbt = min(*nbonds, A.bond_cap);
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nbt; t += gridDim.x * blockDim.x) {
+  int b0, b1;
+  block_share(nbt, b0, b1);
+  for (int base = b0; base < b1; base += 256) {
+    const int t = base + threadIdx.x;
     const ReboParamsT<double>& P = T64->rebo;
-    int2 bd = bonds[t];
+    int2 bd = (t < b1) ? bonds[t] : make_int2(0, 0);
     BPair<R> p;
     p.i = bd.x;
     p.j = bd.y;
@@ -563,6 +587,7 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
     double r = sqrt(norm2(rd));
     double f, df;
     cutoff<double>(r, P.R[a][b][0], P.R[a][b][1], f, df);
+    if (t >= b1) f = 0.0;  // past the share: no work this pass
     {  // f_ij inside the bond order in R, as the coordination pass summed it
       R fr, dfr;
       cutoff<R>(p.r, T->rebo.R[a][b][0], T->rebo.R[a][b][1], fr, dfr);
@@ -589,7 +614,7 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
         bo_forward<R>(A, T, p, st, err);
         bb = st.b;
         if (VA != 0.0) {
-          bo_backward<R>(A, T, p, st, VA * su, grad, gs, err);
+          bo_backward<R>(A, T, p, st, VA * su, gc, gs, err);
           double vder = (-P.beta[0][a][b]) * u1 + (-P.beta[1][a][b]) * u2 + (-P.beta[2][a][b]) * u3;
           cr += (-vder * f * bb) + (-vsum * bb) * df;
           eb += bb * VA;
@@ -611,10 +636,13 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
       }
       add_energy(eatom, p.i, eb);
       D3 v = rd * (cr / r);
-      atomic_add3(grad, p.i, v);
-      atomic_add3(grad, p.j, -v);
+      atomic_add3(gc, p.i, v);
+      atomic_add3(gc, p.j, -v);
     }
+    __syncthreads();
+    gc.drain_if_crowded();
   }
+  gc.flush();
 }
 
 // ------------------------------------------------------------ C_ij paths
@@ -677,8 +705,8 @@ __device__ __forceinline__ R path_weight(const PathBonds<R>& pb) {
 }
 
 // Gradient of C_ij = 1 - prod f over the path's bonds, seeded with V.
-template <class R>
-__device__ __forceinline__ void cij_backward(const PathBonds<R>& pb, R V, double* __restrict__ grad, double gs) {
+template <class R, class G>
+__device__ __forceinline__ void cij_backward(const PathBonds<R>& pb, R V, const G& grad, double gs) {
 #pragma unroll
   for (int q = 0; q < 3; q++) {
     if (q >= pb.hops || pb.df[q] == R(0)) continue;
@@ -869,10 +897,17 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
   const int gid = (threadIdx.x >> 6) * (64 / LW) + lane / LW;
   const unsigned long long below = (1ull << lane) - 1ull;
   const unsigned long long mine = (LW == 64) ? ~0ull : (((1ull << LW) - 1ull) << (lane / LW * LW));
+  __shared__ int s_key[kPairCache];
+  __shared__ double s_val[3 * kPairCache];
+  __shared__ int s_fill;
+  GradCache<kPairCache> gc{s_key, s_val, &s_fill, grad};
+  gc.clear();
+  __syncthreads();
   long long np = 0;
-  for (int i = blockIdx.x * G + gid; i < A.n; i += gridDim.x * G) {
+  // one atom per group (returns are group-uniform)
+  auto atom = [&](const int i) {
     int ti = A.type[i];
-    if (ti > T_H) continue;  // group-uniform
+    if (ti > T_H) return;
     const double* Lb = box_of(bx, i);
     double4 pi4 = A.p[i];
     double xi = pi4.x, yi = pi4.y, zi = pi4.z;
@@ -922,7 +957,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       if (sl == 0) raise(err, ERR_QUEUE_OVERFLOW);
       npair = kPairCap;
     }
-    if (npair == 0) continue;  // group-uniform
+    if (npair == 0) return;
     if (sl == 0) np += npair;
     // phase 1: paths into the hash
     for (int q = sl; q < kHash; q += LW) {
@@ -1096,8 +1131,8 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       R cr = der * f * V + val * V * df;
       V3<R> v = rv * (cr / r);
       gi = gi + d3(double(v.x), double(v.y), double(v.z));
-      scatter3<R>(grad, j, -v, gs);
-      if (pkey) cij_backward<R>(pb, R(1) * LJ, grad, gs);
+      scatter3<R>(gc, j, -v, gs);
+      if (pkey) cij_backward<R>(pb, R(1) * LJ, gc, gs);
       e += to_erg<R>(double(R(1) * C * LJ), T->eunit);
       if constexpr (TH) {  // S_r = 0: E = C f 4 eps ((sigma/r)^12 - (sigma/r)^6)
         double de = double(C) * double(f) * 4.0 * double(s12 - s6);
@@ -1116,8 +1151,8 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     e = group_sum<LW>(e);
     if (sl == 0) {
       if (gx != 0.0 || gy != 0.0 || gz != 0.0) {
-        if constexpr (std::is_same<R, double>::value) atomic_add3(grad, i, d3(gx, gy, gz));
-        else atomic_add3(grad, i, d3(gx * gs, gy * gs, gz * gs));
+        if constexpr (std::is_same<R, double>::value) atomic_add3(gc, i, d3(gx, gy, gz));
+        else atomic_add3(gc, i, d3(gx * gs, gy * gs, gz * gs));
       }
       add_energy(eatom, i, e);
     }
@@ -1134,7 +1169,16 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
+  };
+  // a contiguous share of the atoms per workgroup, G atoms per pass
+  int a0, a1;
+  block_share(A.n, a0, a1, G);
+  for (int base = a0; base < a1; base += G) {
+    if (base + gid < a1) atom(base + gid);
+    __syncthreads();
+    gc.drain_if_crowded();
   }
+  gc.flush();
   count_add(npairs, np);
 }
 
@@ -1164,8 +1208,18 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
     if (q == 0) s_pre[0] = 0;
     __syncthreads();
   }
+  __shared__ int s_key[kBondCache];
+  __shared__ double s_val[3 * kBondCache];
+  __shared__ int s_fill;
+  GradCache<kBondCache> gc{s_key, s_val, &s_fill, grad};
+  gc.clear();
+  __syncthreads();
   const int nq = s_pre[kQueues];
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nq; t += gridDim.x * blockDim.x) {
+  int b0, b1;
+  block_share(nq, b0, b1);
+  for (int base = b0; base < b1; base += 256) {
+   const int t = base + threadIdx.x;
+   if (t < b1) {
     const LJParamsT<R>& L = T->lj;
     const ReboParamsT<R>& P = T->rebo;
     int lo = 0, hi = kQueues - 1;  // last sub-queue whose first index is <= t
@@ -1227,12 +1281,12 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
       if (dStr != R(0)) cr += -dStr * V2;
       R dStb = dSfun<R>(bs, bmin, bmax);
       if (dStr != R(0) && Stb != R(0)) cr += dStr * Stb * V2;
-      if (dStb != R(0)) bo_backward<R>(A, T, p, st, double(Str * dStb * V2), grad, gs, err);
+      if (dStb != R(0)) bo_backward<R>(A, T, p, st, double(Str * dStb * V2), gc, gs, err);
     }
     V3<R> v = rv * (cr / r);
-    scatter3<R>(grad, i, v, gs);
-    scatter3<R>(grad, j, -v, gs);
-    if (pkey && SrSb * LJ != R(0)) cij_backward<R>(pb, SrSb * LJ, grad, gs);
+    scatter3<R>(gc, i, v, gs);
+    scatter3<R>(gc, j, -v, gs);
+    if (pkey && SrSb * LJ != R(0)) cij_backward<R>(pb, SrSb * LJ, gc, gs);
     add_energy(eatom, i, to_erg<R>(double(SrSb * C * LJ), T->eunit));
     if constexpr (TH) {
       // S_r depends on r/sigma only: dS_r/dsigma = -(r/sigma) dS_r/dr
@@ -1246,7 +1300,11 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
       unsafeAtomicAdd(th, de);
       unsafeAtomicAdd(th + 1, ds);
     }
+   }
+    __syncthreads();
+    gc.drain_if_crowded();
   }
+  gc.flush();
 }
 
 // ------------------------------------------------------------ torsion
@@ -1256,14 +1314,23 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
                                               const int2* __restrict__ bonds, const int* __restrict__ nbonds,
                                               int compat, double* __restrict__ grad, double* __restrict__ eatom,
                                               long long* __restrict__ nquads) {
+  __shared__ int s_key[kTorsCache];
+  __shared__ double s_val[3 * kTorsCache];
+  __shared__ int s_fill;
+  GradCache<kTorsCache> gc{s_key, s_val, &s_fill, grad};
+  gc.clear();
+  __syncthreads();
   long long nq = 0;
   const double gs = T->eunit / T->lunit;
   int nbt = min(*nbonds, A.bond_cap);
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nbt; t += gridDim.x * blockDim.x) {
+  int b0, b1;
+  block_share(nbt, b0, b1);
+  for (int base = b0; base < b1; base += 256) {
+    const int t = base + threadIdx.x;
     const ReboParamsT<R>& P = T->rebo;
-    int2 bd = bonds[t];
+    int2 bd = (t < b1) ? bonds[t] : make_int2(0, 0);
     int i = bd.x, j = bd.y;
-    if (A.type[i] == T_C && A.type[j] == T_C) {
+    if (t < b1 && A.type[i] == T_C && A.type[j] == T_C) {
       V3<R> a_ = to_local<R>(depos(A.p, i, j, bx));
       R rij = sqrt(norm2(a_));
       R wij, dwij;
@@ -1329,16 +1396,19 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
             gj = gj + v;
             gl = gl - v;
           }
-          scatter3<R>(grad, k.k, gk, gs);
-          scatter3<R>(grad, l.k, gl, gs);
+          scatter3<R>(gc, k.k, gk, gs);
+          scatter3<R>(gc, l.k, gl, gs);
           eb += double(wki * wij * wjl * vt);
         }
       }
       add_energy(eatom, i, to_erg<R>(eb, T->eunit));
-      scatter3<R>(grad, i, gi, gs);
-      scatter3<R>(grad, j, gj, gs);
+      scatter3<R>(gc, i, gi, gs);
+      scatter3<R>(gc, j, gj, gs);
     }
+    __syncthreads();
+    gc.drain_if_crowded();
   }
+  gc.flush();
   count_add(nquads, nq);
 }
 
