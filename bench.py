@@ -91,12 +91,12 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_bombard256.json")
 
 def lj_algorithmic_bytes(cnt, natoms):
     """Compulsory memory traffic of one k_lj_wave launch (every byte read or
-    written once): the j > i half of each long-list row (4 B per entry); per
-    atom its list count and split (8 B), packed position (32 B), type (4 B),
-    bond count (4 B), gradient read + write (48 B) and energy (8 B); per bond,
-    both directions' records (32 B) and partner index (4 B); per queued b*
-    pair one 16-byte entry."""
-    return (4.0 * cnt["long_entries"] / 2.0 + natoms * (8 + 32 + 4 + 4 + 48 + 8)
+    written once): one 4-byte index per LJ pair inside the cutoff (the half
+    list, i < j); per atom its row count (4 B), packed position and type
+    (32 B), bond count (4 B), gradient read + write (48 B) and energy (8 B);
+    per bond, both directions' records (32 B) and partner index (4 B); per
+    queued b* pair one 16-byte entry."""
+    return (4.0 * cnt["lj_pairs"] + natoms * (4 + 32 + 4 + 48 + 8)
             + 2.0 * cnt["bonds"] * 36 + 16.0 * cnt["bstar"])
 
 
@@ -156,6 +156,24 @@ def nve_drift(sysm, precision, steps, dt):
     t = np.arange(steps) * dt * 1e12
     slope = float(np.polyfit(t, e, 1)[0])
     return slope, float(np.abs(e - e[0]).max())
+
+
+def host_driven(traj, calls):
+    """Force calls through host buffers, as the FGeneral adapter makes them
+    (positions with a zero displacement in, energies and gradient out, PCIe
+    both ways): atom-force-evaluations/s of one trajectory in FP64."""
+    import numpy as np
+    from mdtk_amd import Engine
+    e = Engine.from_system(traj, potentials=POTS, etors_compat=False)
+    x = e.state()[0]
+    dr = np.zeros_like(x)
+    e.set_positions(x, dr_cm=dr)
+    e.energy_grad()
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        e.set_positions(x, dr_cm=dr)
+        e.energy_grad()
+    return traj.n * calls / (time.perf_counter() - t0)
 
 
 def cpu_baseline(traj, budget_s: float):
@@ -224,6 +242,8 @@ def main():
         del e
     v_drop = tot_as / (tot_ms / 1e3)
     log(f"fp64 drop-in (sequential single trajectories): {v_drop:.4e} atom-steps/s")
+    v_host = host_driven(sample[0], 50)
+    log(f"fp64 force calls through host buffers: {v_host:.4e} atom-evaluations/s")
 
     use_mixed = f_rms <= 1e-5
     value = v_mx if use_mixed else v_64b
@@ -254,6 +274,9 @@ def main():
         "fp64_dropin_value": v_drop,
         "fp64_dropin_sample": f"{len(sample)} trajectories, {sum(s.n for s in sample)} atoms, sequential contexts",
         "mixed_batched_over_fp64_dropin": v_mx / v_drop,
+        "host_driven_fp64_value": v_host,
+        "host_driven_sample": f"{sample[0].n}-atom trajectory, 50 force calls through host buffers "
+                              "(positions in, energies and gradient out), FP64",
         "mixed_force_rms_err_eV_per_A": f_rms,
         "autotune": {"mixed_batched": tune_mx, "fp64_batched": tune_64},
         "roofline": rl,
