@@ -388,7 +388,7 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   Atoms& A = c->A;
   double* dth = TH ? c->dtheta : nullptr;
   if (c->have_short) {
-    launch(c, "k_coord", [&] { k_coord<R><<<blocks(n), 256, 0, c->stream>>>(A, T, c->box, c->derr); });
+    launch(c, "k_coord", [&] { k_coord<R><<<blocks(n, 16), 256, 0, c->stream>>>(A, T, c->box, c->derr); });
     launch(c, "k_bond_enum", [&] {
       k_bond_enum<<<blocks(n), 256, 0, c->stream>>>(A, c->bonds, c->nbonds, c->bond_cap, c->derr);
     });
@@ -957,13 +957,13 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
   auto body = [&]() {
     launch(c, "k_drift", [&] { k_drift<<<blocks(n), 256, 0, c->stream>>>(A, c->dtp); });
     launch(c, "k_sys_disp", [&] {
-      k_sys_disp<<<nsys, 256, 0, c->stream>>>(A.sys_off, A.disp_s, A.disp_l, A.disp_p, lim_s, lim_l,
+      k_sys_disp<<<nsys, 1024, 0, c->stream>>>(A.sys_off, A.disp_s, A.disp_l, A.disp_p, lim_s, lim_l,
                                               c->pair_margin, c->sflags, c->pflags, c->flags, c->counters);
     });
     force_call(c);
     launch(c, "k_kick", [&] { k_kick<<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->dtp); });
     launch(c, "k_sys_step_end", [&] {
-      k_sys_step_end<<<nsys, 256, 0, c->stream>>>(A, nterms, c->eatom, c->dtp, fixed, c->stepout, c->stepidx,
+      k_sys_step_end<<<nsys, 1024, 0, c->stream>>>(A, nterms, c->eatom, c->dtp, fixed, c->stepout, c->stepidx,
                                                   nsys);
     });
     k_step_advance<<<1, 64, 0, c->stream>>>(c->stepidx);
@@ -1081,7 +1081,7 @@ int mdg_set_positions(mdg_ctx* c, const double* xyz, const int32_t* pbc_count, c
   }
   double lim_s = c->have_short ? 0.44 * c->rc_short : 1e300;
   double lim_l = c->have_long ? 0.44 * c->rc_long : 1e300;
-  k_sys_disp<<<c->nsys, 256, 0, c->stream>>>(c->A.sys_off, c->A.disp_s, c->A.disp_l, c->A.disp_p, lim_s, lim_l,
+  k_sys_disp<<<c->nsys, 1024, 0, c->stream>>>(c->A.sys_off, c->A.disp_s, c->A.disp_l, c->A.disp_p, lim_s, lim_l,
                                              c->pair_margin, c->sflags, c->pflags, c->flags, c->counters);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;

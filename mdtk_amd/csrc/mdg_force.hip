@@ -112,45 +112,72 @@ __device__ __forceinline__ double coord_excl(double Nsum, bool subtract, R r, R 
 // ----------------------------------------------------- coordination pass
 // REBO::countNeighbours (REBO.cxx:328-393) plus the per-atom bond list: all
 // short-list neighbours with r <= R2 in list order, with r, f, df and the
-// minimum-image vector.
+// minimum-image vector.  A group of 16 lanes per atom takes 16 row entries
+// per round (coalesced row reads and record writes); bonds keep their row
+// order by ballot compaction, and Nt/NH/NC are summed in that order (the
+// reference's), one shuffle per bond.
 template <class R>
-__global__ void k_coord(Atoms A, const TablesT<R>* __restrict__ T, Box bx, unsigned* err) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.n) return;
+__global__ void __launch_bounds__(256) k_coord(Atoms A, const TablesT<R>* __restrict__ T, Box bx, unsigned* err) {
+  constexpr int LW = 16;
+  const int lane = threadIdx.x & 63;
+  const int sl = lane % LW;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const unsigned long long mine = ((1ull << LW) - 1ull) << (lane / LW * LW);
+  const int i = blockIdx.x * (blockDim.x / LW) + threadIdx.x / LW;
+  if (i >= A.n) return;  // group-uniform
   BondRec<R>* B = brec<R>(A);
-  int ti = A.type[i];
+  const int ti = A.type[i];
   double Nt = 0, NH = 0, NC = 0;
   int nb = 0;
   if (ti <= T_H) {
-    int cnt = A.nls_cnt[i];
+    const int cnt = A.nls_cnt[i];
     const int* L = A.nls + (size_t)i * A.cap_s;
-    for (int q = 0; q < cnt; q++) {
-      int k = L[q];
-      int tk = A.type[k];
-      V3<R> rv = to_local<R>(depos(A.p, i, k, bx));
-      R r = sqrt(rv.x * rv.x + rv.y * rv.y + rv.z * rv.z);
-      R R2 = T->rebo.R[ti][tk][1];
-      if (r > R2) continue;
-      R R1 = T->rebo.R[ti][tk][0];
-      R f, df;
-      cutoff<R>(r, R1, R2, f, df);
-      Nt += f;
-      if (tk == T_H) NH += f;
-      if (tk == T_C) NC += f;
-      if (nb < A.maxb) {
-        int o = i * A.maxb + nb;
-        A.bj[o] = k;
-        B[o] = BondRec<R>{rv.x, rv.y, rv.z, r, f, df, R(0), R(0)};
-      } else {
-        raise(err, ERR_BOND_OVERFLOW);
+    const double4 pi = A.p[i];
+    const double* Lb = box_of(bx, i);
+    for (int b0 = 0; b0 < cnt; b0 += LW) {
+      const int q = b0 + sl;
+      bool ok = false;
+      int k = 0, tk = 0;
+      V3<R> rv = v3<R>(0, 0, 0);
+      R r = R(0), f = R(0), df = R(0);
+      if (q < cnt) {
+        k = L[q];
+        const double4 pk = A.p[k];
+        tk = (int)pk.w;  // the packed position carries the type
+        rv = to_local<R>(d3(mic(pi.x - pk.x, Lb[0]), mic(pi.y - pk.y, Lb[1]), mic(pi.z - pk.z, Lb[2])));
+        r = sqrt(rv.x * rv.x + rv.y * rv.y + rv.z * rv.z);
+        if ((unsigned)tk <= (unsigned)T_H && !(r > T->rebo.R[ti][tk][1])) {
+          ok = true;
+          cutoff<R>(r, T->rebo.R[ti][tk][0], T->rebo.R[ti][tk][1], f, df);
+        }
       }
-      nb++;
+      const unsigned long long m = __ballot(ok) & mine;
+      if (ok) {
+        const int slot = nb + __popcll(m & below);
+        if (slot < A.maxb) {
+          const int o = i * A.maxb + slot;
+          A.bj[o] = k;
+          B[o] = BondRec<R>{rv.x, rv.y, rv.z, r, f, df, R(0), R(0)};
+        }
+      }
+      for (unsigned long long mm = m; mm; mm &= mm - 1) {  // in row order
+        const int src = __ffsll((long long)mm) - 1;
+        const double fs = __shfl(double(f), src, 64);
+        const int ts = __shfl(tk, src, 64);
+        Nt += fs;
+        if (ts == T_H) NH += fs;
+        if (ts == T_C) NC += fs;
+      }
+      nb += __popcll(m);
     }
   }
-  A.Nt[i] = Nt;
-  A.NH[i] = NH;
-  A.NC[i] = NC;
-  A.bcnt[i] = nb < A.maxb ? nb : A.maxb;
+  if (sl == 0) {
+    if (nb > A.maxb) raise(err, ERR_BOND_OVERFLOW);
+    A.Nt[i] = Nt;
+    A.NH[i] = NH;
+    A.NC[i] = NC;
+    A.bcnt[i] = nb < A.maxb ? nb : A.maxb;
+  }
 }
 
 // ------------------------------------------------------- bond order b_ij

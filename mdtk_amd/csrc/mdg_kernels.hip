@@ -307,7 +307,7 @@ __global__ void k_sys_disp(const int* __restrict__ sys_off, const double* __rest
                            const double* __restrict__ dl, const double* __restrict__ dp, double lim_s, double lim_l,
                            double lim_p, int* __restrict__ sflags, int* __restrict__ pflags, int* __restrict__ flags,
                            long long* __restrict__ rebuilds) {
-  __shared__ double sh[4][6];
+  __shared__ double sh[16][6];  // up to 1024 threads
   int s = blockIdx.x;
   int b = sys_off[s], e = sys_off[s + 1];
   double a1 = 0, a2 = 0, b1 = 0, b2 = 0, c1 = 0, c2 = 0;
