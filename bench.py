@@ -85,7 +85,7 @@ def kernel_times(eng, steps, dt):
     return kt
 
 
-LJ_SHAPES = {0: (1, 64), 1: (3, 64), 2: (1, 32), 3: (1, 16)}  # autotuner's lj_shape -> (OCC, LW)
+LJ_SHAPES = {0: (1, 64), 1: (3, 64), 2: (1, 32), 3: (1, 16), 4: (4, 16)}  # autotuner's lj_shape -> (OCC, LW)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_bombard256.json")
 
 

@@ -436,6 +436,12 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
           case 0: go(I1(), std::integral_constant<int, 64>()); break;
           case 1: go(std::integral_constant<int, 3>(), std::integral_constant<int, 64>()); break;
           case 2: go(I1(), std::integral_constant<int, 32>()); break;
+          case 4:  // 4 waves/SIMD (FP32 form only: the FP64 form's LDS allows 2)
+            if constexpr (std::is_same<R, float>::value) {
+              go(std::integral_constant<int, 4>(), std::integral_constant<int, 16>());
+              break;
+            }
+            [[fallthrough]];
           default: go(I1(), std::integral_constant<int, 16>()); break;
         }
       }
@@ -1182,8 +1188,10 @@ int mdg_autotune(mdg_ctx* c, int reps, double* report) {
     tune_knob("k_rebo_bonds", tu.grid_bonds, grids(nb, 256));
   }
   if (c->slot_lj >= 0) {
-    tune_knob("k_lj_wave", tu.lj_shape, {0, 1, 2, 3});
-    tune_knob("k_lj_wave", tu.grid_lj, grids(c->n, 4 * (64 / (tu.lj_shape == 3 ? 16 : tu.lj_shape == 2 ? 32 : 64))));
+    if (c->mixed) tune_knob("k_lj_wave", tu.lj_shape, {0, 1, 2, 3, 4});
+    else tune_knob("k_lj_wave", tu.lj_shape, {0, 1, 2, 3});
+    int lw = (tu.lj_shape >= 3) ? 16 : (tu.lj_shape == 2) ? 32 : 64;
+    tune_knob("k_lj_wave", tu.grid_lj, grids(c->n, 4 * (64 / lw)));
     tune_knob("k_bstar", tu.occ_bstar, {1, 2});
     tune_knob("k_bstar", tu.grid_bstar, grids(nq, 256));
   }

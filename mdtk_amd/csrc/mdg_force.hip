@@ -924,10 +924,11 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
   const int gid = (threadIdx.x >> 6) * (64 / LW) + lane / LW;
   const unsigned long long below = (1ull << lane) - 1ull;
   const unsigned long long mine = (LW == 64) ? ~0ull : (((1ull << LW) - 1ull) << (lane / LW * LW));
-  __shared__ int s_key[kPairCache];
-  __shared__ double s_val[3 * kPairCache];
+  constexpr int CS = (OCC >= 4) ? kPairCache / 2 : kPairCache;  // 4 workgroups per CU must fit the LDS
+  __shared__ int s_key[CS];
+  __shared__ double s_val[3 * CS];
   __shared__ int s_fill;
-  GradCache<kPairCache> gc{s_key, s_val, &s_fill, grad};
+  GradCache<CS> gc{s_key, s_val, &s_fill, grad};
   gc.clear();
   __syncthreads();
   long long np = 0;
@@ -1099,10 +1100,10 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     double e = 0.0;
     double th_e[2] = {0.0, 0.0}, th_s[2] = {0.0, 0.0};  // dE/d(epsilon, sigma) by tj
     D3 gi = d3(0, 0, 0);  // FP64 in both forms: the lane width only reorders FP64 sums
-    for (int q = sl; q < npair; q += LW) {
-      int j = s_pair[gid][q];
-      int tj = A.type[j];
-      double4 pj = A.p[j];
+    // one pair (j, its packed position); the loads of a lane's next two
+    // pairs are issued together below
+    auto pair = [&](const int j, const double4 pj) {
+      const int tj = (int)pj.w;  // phase 0 kept C/H partners only
       D3 rd = d3(mic(xi - pj.x, Lb[0]), mic(yi - pj.y, Lb[1]), mic(zi - pj.z, Lb[2]));
       R w = R(0);
       int bkey = 0x7fffffff;
@@ -1124,7 +1125,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
         h = (h + 1) & (kHash - 1);
       }
       R C = R(1) - w;
-      if (!(C > R(0))) continue;
+      if (!(C > R(0))) return;
       int pkey = (w > R(0)) ? bkey : 0;  // C_ij path (0: none)
       V3<R> rv = to_local<R>(rd);
       R r = sqrt(norm2(rv));
@@ -1138,14 +1139,14 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
         int slot = base + __popcll(m & below);
         if (slot < qsub) qb[slot] = make_int4(i, j, pkey, 0);
         else raise(err, ERR_QUEUE_OVERFLOW);
-        continue;
+        return;
       }
       PathBonds<R> pb;
       pb.hops = 0;
       if (pkey) pb = load_path<R>(A, i, pkey);
       R f, df;
       cutoff<R>(r, L.R[ti][tj][0], L.R[ti][tj][1], f, df);
-      if (f == R(0)) continue;
+      if (f == R(0)) return;
       R s = L.sigma[ti][tj] / r;
       R s6 = s;
       for (int u = 2; u <= 6; u++) s6 *= s;
@@ -1173,6 +1174,14 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
           th_s[1] += ds;
         }
       }
+    };
+    for (int q = sl; q < npair; q += 2 * LW) {
+      const int q2 = q + LW;
+      const int ja = s_pair[gid][q];
+      const int jb = (q2 < npair) ? s_pair[gid][q2] : ja;
+      const double4 pa = A.p[ja], pb4 = A.p[jb];
+      pair(ja, pa);
+      if (q2 < npair) pair(jb, pb4);
     }
     double gx = group_sum<LW>(gi.x), gy = group_sum<LW>(gi.y), gz = group_sum<LW>(gi.z);
     e = group_sum<LW>(e);
@@ -1464,6 +1473,7 @@ MDG_INSTANTIATE_TH(float, false, 1, 64)
 MDG_INSTANTIATE_TH(float, false, 3, 64)
 MDG_INSTANTIATE_TH(float, false, 1, 32)
 MDG_INSTANTIATE_TH(float, false, 1, 16)
+MDG_INSTANTIATE_TH(float, false, 4, 16)
 MDG_INSTANTIATE_OCC(double, false, 1)
 MDG_INSTANTIATE_OCC(double, false, 2)
 MDG_INSTANTIATE_OCC(double, true, 1)
