@@ -50,6 +50,9 @@ def lib():
                                       ctypes.c_int, dp]
         L.orc_set_alter_exact.argtypes = [ctypes.c_int]
         L.orc_set_params.argtypes = [ctypes.c_void_p, dp]
+        L.orc_set_bath.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_double] * 5 + [dp, ctypes.c_double]
+        L.orc_set_bath_atoms.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_ubyte)]
+        L.orc_bath_energy.argtypes = [ctypes.c_void_p, dp]
         L.orc_get_params.argtypes = [ctypes.c_void_p, dp]
         _lib = L
     return _lib
@@ -117,6 +120,25 @@ class Oracle:
         th = np.ascontiguousarray(theta, np.float64)
         assert th.shape == (NUM_PARAMS,)
         self._chk(lib().orc_set_params(self.h, _p(th)))
+
+    def set_bath(self, geometry: int, To: float, gamma: float = 1.0e13, z_min: float = 1e6 * 1e-8,
+                 d_boundary: float = 0.0, z_min_free: float = -5e-8, center=(0.0, 0.0, 0.0),
+                 radius: float = 1e6 * 1e-8):
+        """Thermal bath (SimLoop.hpp:118-189): geometry 0 none, 1 everywhere,
+        2 box, 3 sphere; To in K, gamma in 1/s, lengths in cm."""
+        c = np.ascontiguousarray(center, np.float64)
+        self._chk(lib().orc_set_bath(self.h, int(geometry), float(To), float(gamma), float(z_min),
+                                     float(d_boundary), float(z_min_free), _p(c), float(radius)))
+
+    def set_bath_atoms(self, apply):
+        a = np.ascontiguousarray(apply, np.uint8)
+        assert a.shape == (self.n,)
+        self._chk(lib().orc_set_bath_atoms(self.h, _p(a, ctypes.c_ubyte)))
+
+    def bath_energy(self) -> float:
+        e = np.zeros(1)
+        self._chk(lib().orc_bath_energy(self.h, _p(e)))
+        return float(e[0])
 
     def neighbour_lists(self, term: int):
         counts = np.zeros(self.n, np.int32)

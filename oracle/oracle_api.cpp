@@ -166,6 +166,41 @@ int orc_get_params(void* p, double* theta) {
   return guarded([&] { theta_rw(static_cast<Handle*>(p), theta, false); });
 }
 
+// Thermal bath (SimLoop.hpp:118-189): geometry TB_NONE/UNIVERSE/BOX/SPHERE,
+// To (K), gamma (1/s); lengths in cm.  zMinOfFreeZone is the selected
+// geometry's (box or sphere) own field.
+int orc_set_bath(void* p, int geom, double To, double gamma, double zMin, double dBoundary, double zMinOfFreeZone,
+                 const double* center, double radius) {
+  Handle* h = static_cast<Handle*>(p);
+  return guarded([&] {
+    if (geom < TB_NONE || geom > TB_SPHERE) throw OracleError("bad thermal bath geometry");
+    auto& b = h->sl.bath;
+    b.geom = geom;
+    b.To = To;
+    b.gamma = gamma;
+    b.zMin = zMin;
+    b.dBoundary = dBoundary;
+    b.zMinOfFreeZone = zMinOfFreeZone;
+    b.sphereZMinOfFreeZone = zMinOfFreeZone;
+    if (center) b.center = V3(center[0], center[1], center[2]);
+    b.radius = radius;
+  });
+}
+
+// Atom::apply_ThermalBath per atom (null: all true).
+int orc_set_bath_atoms(void* p, const unsigned char* apply) {
+  Handle* h = static_cast<Handle*>(p);
+  auto& at = h->sl.atoms;
+  for (size_t i = 0; i < at.size(); i++) at[i].apply_tb = apply ? apply[i] != 0 : true;
+  return 0;
+}
+
+// Energy transferred from the bath so far (erg, SimLoop::check).
+int orc_bath_energy(void* p, double* e) {
+  *e = static_cast<Handle*>(p)->sl.energyTransferredFromBath;
+  return 0;
+}
+
 // Neighbour list of chain entry `term`: counts[n], then the concatenated lists
 // (ascending index per atom) into idx (capacity cap).  Returns -1 on overflow.
 int orc_nl(void* p, int term, int* counts, int* idx, long cap) {

@@ -79,6 +79,8 @@ struct Atom {
   V3 coords, V, an, grad;
   int pbc_count[3] = {0, 0, 0};
   bool fixed = false;
+  bool apply_tb = true;  // Atom::apply_ThermalBath (Atom.hpp:52)
+  V3 an_no_tb;           // acceleration without the bath's force
   V3 PBC{NO_PBC_L, NO_PBC_L, NO_PBC_L};
   size_t globalIndex = 0;
   void setAttributesByElementID();  // mdtk/Atom.hpp:98-113
@@ -384,10 +386,26 @@ struct Proxy {
 
 void nl_update(AtomsArray& atoms, std::vector<Potential*>& pots);
 
-// mdtk/SimLoop.cxx:225-474 (NVE part; no thermal bath, no file output)
+// The thermal bath of SimLoop (SimLoop.hpp:118-189): friction
+// -V M gamma (1 - sqrt(To/T)) on the atoms inside its geometry, T the kinetic
+// temperature of those atoms at the start of the step.
+enum { TB_NONE = 0, TB_UNIVERSE = 1, TB_BOX = 2, TB_SPHERE = 3 };
+struct ThermalBath {
+  int geom = TB_NONE;
+  double To = 0.0, gamma = 1.0e13;                                          // SimLoop.hpp:118-127
+  double zMin = 1000000.0 * Ao, dBoundary = 0.0, zMinOfFreeZone = -5.0 * Ao;  // :128-152
+  V3 center;                                                                // :153-171
+  double radius = 1000000.0 * Ao;
+  double sphereZMinOfFreeZone = -5.0 * Ao;
+  bool inside(const Atom& a) const;  // SimLoop.hpp:139-151,164-170,174-185
+};
+
+// mdtk/SimLoop.cxx:225-474 (no file output)
 struct SimLoop {
   AtomsArray atoms;
   Proxy fpot;
+  ThermalBath bath;
+  double energyTransferredFromBath = 0.0;  // SimLoop.hpp:65
   double dt = 1e-20, simTime = 0;  // SimLoop.cxx:50
   bool fixed_dt = false;
   double fixed_dt_value = 0;
@@ -398,6 +416,8 @@ struct SimLoop {
   void prepare();
   void step();  // one iteration of the while loop
   double energyKin() const;
+  bool bathApplies(const Atom& a) const { return a.apply_tb && bath.inside(a); }  // SimLoop.hpp:186-190
+  double bathTemperature() const;  // actualTemperatureOfThermalBath, SimLoop.cxx:516-535
 };
 
 }  // namespace orc

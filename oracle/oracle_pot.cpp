@@ -1174,18 +1174,54 @@ void SimLoop::prepare() {  // AtomsContainer.cxx:106-123, SimLoop.cxx:236-250
   prepared = true;
 }
 
+bool ThermalBath::inside(const Atom& a) const {
+  switch (geom) {
+    case TB_UNIVERSE:
+      return true;
+    case TB_BOX: {
+      bool lateral = a.PBC.x != NO_PBC_L && a.PBC.y != NO_PBC_L;  // Atom::lateralPBCEnabled
+      ORC_REQUIRE(lateral);
+      return (a.coords.z > zMin) ||
+             (lateral &&
+              ((a.coords.x < 0.0 + dBoundary) || (a.coords.x > a.PBC.x - dBoundary) ||
+               (a.coords.y < 0.0 + dBoundary) || (a.coords.y > a.PBC.y - dBoundary)) &&
+              a.coords.z > zMinOfFreeZone);
+    }
+    case TB_SPHERE:
+      ORC_REQUIRE(a.PBC.x == NO_PBC_L && a.PBC.y == NO_PBC_L && a.PBC.z == NO_PBC_L);
+      return a.coords.z > sphereZMinOfFreeZone && norm(center - a.coords) > radius;
+    default:
+      return false;
+  }
+}
+
+double SimLoop::bathTemperature() const {
+  double e = 0.0;
+  int n = 0;
+  for (const auto& a : atoms) {
+    if (a.fixed) continue;
+    if (bathApplies(a)) {
+      e += a.M * SQR(norm(a.V)) / 2.0;
+      n++;
+    }
+  }
+  if (n == 0) return 0.0;
+  return e / (3.0 / 2.0 * kb * n);
+}
+
 double SimLoop::energyKin() const {  // SimLoop.cxx:500-514
   double e = 0.0;
   for (const auto& a : atoms) e += a.M * SQR(norm(a.V)) / 2.0;
   return e;
 }
 
-// One iteration of SimLoop::executeMain's loop (SimLoop.cxx:252-445) for the
-// NVE case: no thermal bath, no file output.  A fixed dt is applied the way a
+// One iteration of SimLoop::executeMain's loop (SimLoop.cxx:252-445), with
+// the thermal bath and without file output.  A fixed dt is applied the way a
 // doBeforeIteration() hook would set it.
 void SimLoop::step() {
   if (!prepared) prepare();
   if (fixed_dt) dt = fixed_dt_value;
+  const double Tb = bathTemperature();  // SimLoop.cxx:280, before the drift
   netForce = V3();
   double v_max = 0.0;
   for (auto& a : atoms) a.grad = V3();
@@ -1195,6 +1231,7 @@ void SimLoop::step() {
       if (a.fixed) continue;
       V3 force = -a.grad;
       a.an = force / a.M;
+      a.an_no_tb = force / a.M;
       a.grad = V3();
     }
   }
@@ -1213,6 +1250,22 @@ void SimLoop::step() {
     netForce += force;
     if (a.fixed) continue;
     V3 vdt2 = a.V + a.an * dt / 2.0;  // eq 2
+    if (bathApplies(a)) {             // SimLoop.cxx:352-386
+      double To_by_T = (std::fabs(Tb) < 1e-5) ? 0 : (bath.To / Tb);
+      const double max_To_by_T = 5.0;
+      if (To_by_T < -max_To_by_T) To_by_T = -max_To_by_T;
+      if (To_by_T > +max_To_by_T) To_by_T = +max_To_by_T;
+      V3 dforce = -a.V * a.M * bath.gamma * (1.0 - std::sqrt(To_by_T));
+      V3 an_no_tb_new = force / a.M;
+      V3 dv_no_tb = a.an_no_tb * dt / 2.0 + an_no_tb_new * dt / 2.0;
+      a.an_no_tb = an_no_tb_new;
+      V3 an_new = (force + dforce) / a.M;
+      V3 dv = a.an * dt / 2.0 + an_new * dt / 2.0;
+      a.an = an_new;
+      double dEkin = a.M * SQR(norm(a.V + dv)) / 2.0 - a.M * SQR(norm(a.V + dv_no_tb)) / 2.0;
+      energyTransferredFromBath += dEkin;
+      force += dforce;
+    }
     a.an = force / a.M;               // eq 3
     a.V = vdt2 + a.an * dt / 2.0;     // eq 4
     double v = norm(a.V);
