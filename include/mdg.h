@@ -128,15 +128,45 @@ int mdg_energy_grad_params(mdg_ctx* ctx, double* e_terms, double* grad, double* 
  * concatenated (ascending index per atom) into idx[cap]. */
 int mdg_nl_export(mdg_ctx* ctx, int term, int32_t* counts, int32_t* idx, int64_t cap);
 
-/* nsteps iterations of SimLoop::executeMain's loop (SimLoop.cxx:252-445),
- * NVE: eq.1 drift, skin test and rebuild, force call, eq.2-4 kick, PBC wrap,
- * dt rule.  fixed_dt > 0 fixes dt (as a doBeforeIteration hook would);
+/* nsteps iterations of SimLoop::executeMain's loop (SimLoop.cxx:252-445):
+ * eq.1 drift, skin test and rebuild, force call, eq.2-4 kick (with the
+ * thermal bath of mdg_set_thermal_bath, NVE without one), PBC wrap, dt
+ * rule.  fixed_dt > 0 fixes dt (as a doBeforeIteration hook would);
  * otherwise dt = 0.05 A / v_max clamped to [1e-20, 5e-16] s (:409-416).
  * Per step outputs (each may be null): epot = E_pot(t+dt) (erg), ekin =
  * E_kin after the velocity update (erg), dt_used (s), netforce = |sum F|;
  * each holds nsteps*nsys values, step k of system s at [k*nsys + s]. */
 int mdg_step_nve(mdg_ctx* ctx, int nsteps, double fixed_dt, double* epot, double* ekin, double* dt_used,
                  double* netforce);
+
+/* Thermal bath of one system (SimLoop::thermalBathCommon / thermalBathGeom*,
+ * SimLoop.hpp:118-189), applied inside mdg_step_nve's kick as the reference
+ * applies it (SimLoop.cxx:280, 352-386): atoms (non-fixed, opted in, inside
+ * the geometry at x(t+dt)) get the force -V M gamma (1 - sqrt(To/T)), T the
+ * kinetic temperature of the atoms inside at the start of the step (To/T
+ * clamped to [-5, 5], 0 when |T| < 1e-5).  Geometries: universe (all atoms),
+ * box (z > z_min, or within d_boundary of a lateral face and z > z_min_free;
+ * needs x/y PBC), sphere (outside radius around center and z > z_min_free;
+ * needs open boundaries).  Lengths in cm, To in K, gamma in 1/s; defaults of
+ * the reference: gamma 1e13, z_min 1e6 A, d_boundary 0, z_min_free -5 A,
+ * radius 1e6 A.  system -1 sets every system.  The energy each system took
+ * from its bath since mdg_set_systems (SimLoop::check.energyTransferredFromBath,
+ * :384) comes from mdg_bath_energy (out[nsys], erg); mdg_set_bath_atoms sets
+ * Atom::apply_ThermalBath per atom (Atom.hpp:52; null = all). */
+enum { MDG_TB_NONE = 0, MDG_TB_UNIVERSE = 1, MDG_TB_BOX = 2, MDG_TB_SPHERE = 3 };
+typedef struct mdg_thermal_bath {
+  int geometry;      /* MDG_TB_* */
+  double To;         /* K */
+  double gamma;      /* 1/s */
+  double z_min;      /* box */
+  double d_boundary; /* box */
+  double z_min_free; /* box and sphere: zMinOfFreeZone */
+  double center[3];  /* sphere */
+  double radius;     /* sphere */
+} mdg_thermal_bath;
+int mdg_set_thermal_bath(mdg_ctx* ctx, int system, const mdg_thermal_bath* tb);
+int mdg_set_bath_atoms(mdg_ctx* ctx, const uint8_t* apply);
+int mdg_bath_energy(mdg_ctx* ctx, double* out);
 
 /* Current state: replaces reading Atom::coords/V/an/PBC_count. */
 int mdg_get_state(mdg_ctx* ctx, double* xyz, double* vel, double* acc, int32_t* pbc_count);

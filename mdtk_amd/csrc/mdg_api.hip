@@ -94,6 +94,12 @@ struct mdg_ctx {
   std::vector<hipEvent_t> event_pool;
 
   int* stepidx = nullptr;
+  // thermal bath (allocated on first use)
+  std::vector<Bath> bath_h;
+  Bath* bath_d = nullptr;
+  unsigned char* tb_apply_d = nullptr;
+  double* ebath = nullptr;  // per system: energy taken from the bath so far (erg)
+  bool bath_on = false;     // some system has a bath geometry
   bool use_graphs = true;
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // one step, adaptive / fixed dt
   std::vector<hipGraph_t> graphs;
@@ -285,6 +291,38 @@ int mark_pair_rows(mdg_ctx* c) {
   HIPCHK(c, hipMemcpy(c->pflags, one.data(), one.size() * 4, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->flags + 2, one.data(), 4, hipMemcpyHostToDevice));
   return 0;
+}
+
+// Allocates the bath's device state of the current systems (no geometry,
+// every atom opted in, nothing transferred yet).
+int bath_alloc(mdg_ctx* c) {
+  if (c->bath_d) return 0;
+  Atoms& A = c->A;
+  int n = c->n, nsys = c->nsys;
+  double* tbT = nullptr;
+  double* tbdE = nullptr;
+  if (dalloc(c, c->bath_d, nsys) || dalloc(c, c->tb_apply_d, n) || dalloc(c, tbT, nsys) || dalloc(c, tbdE, n) ||
+      dalloc(c, c->ebath, nsys))
+    return 1;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemset(c->tb_apply_d, 1, n));
+  HIPCHK(c, hipMemset(tbT, 0, nsys * sizeof(double)));
+  HIPCHK(c, hipMemset(tbdE, 0, n * sizeof(double)));
+  HIPCHK(c, hipMemset(c->ebath, 0, nsys * sizeof(double)));
+  HIPCHK(c, hipMemcpy(c->bath_d, c->bath_h.data(), nsys * sizeof(Bath), hipMemcpyHostToDevice));
+  A.bath = c->bath_d;
+  A.tb_apply = c->tb_apply_d;
+  A.tb_T = tbT;
+  A.tb_dE = tbdE;
+  return 0;
+}
+
+void drop_graphs(mdg_ctx* c) {  // the step's launches changed
+  for (auto& e : c->graph_exec)
+    if (e) {
+      hipGraphExecDestroy(e);
+      e = nullptr;
+    }
 }
 
 // Neighbour-list rebuild for one list, gated on the device flag.
@@ -690,6 +728,17 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   r |= dalloc(c, A.vx, n) | dalloc(c, A.vy, n) | dalloc(c, A.vz, n);
   r |= dalloc(c, A.ax, n) | dalloc(c, A.ay, n) | dalloc(c, A.az, n);
   r |= dalloc(c, A.grad, 3 * (size_t)n) | dalloc(c, A.mass, n) | dalloc(c, A.Z, n);
+  r |= dalloc(c, A.an_nt, 3 * (size_t)n);
+  A.bath = nullptr;  // a new set of systems starts without a bath
+  A.tb_apply = nullptr;
+  A.tb_T = nullptr;
+  A.tb_dE = nullptr;
+  c->bath_d = nullptr;
+  c->tb_apply_d = nullptr;
+  c->ebath = nullptr;
+  c->bath_on = false;
+  c->bath_h.assign(nsys, Bath{TB_NONE, 0.0, 1.0e13, 1000000.0 * kAo, 0.0, -5.0 * kAo, 0.0, 0.0, 0.0,
+                              1000000.0 * kAo});
   r |= dalloc(c, A.type, n) | dalloc(c, A.elem, n) | dalloc(c, A.fixed, n);
   r |= dalloc(c, A.pcx, n) | dalloc(c, A.pcy, n) | dalloc(c, A.pcz, n);
   r |= dalloc(c, A.disp_s, 3 * (size_t)n) | dalloc(c, A.disp_l, 3 * (size_t)n);
@@ -815,6 +864,69 @@ int mdg_set_system(mdg_ctx* c, int n, const int32_t* elem, const double* xyz, co
 }
 
 int mdg_num_systems(const mdg_ctx* c) { return c->nsys; }
+
+
+int mdg_set_thermal_bath(mdg_ctx* c, int system, const mdg_thermal_bath* tb) {
+  if (c->n == 0) {
+    c->err = "no system";
+    return 1;
+  }
+  if (!tb || system < -1 || system >= c->nsys) {
+    c->err = "mdg_set_thermal_bath: bad system index or null parameters";
+    return 1;
+  }
+  if (tb->geometry < MDG_TB_NONE || tb->geometry > MDG_TB_SPHERE || !std::isfinite(tb->To) ||
+      !std::isfinite(tb->gamma)) {
+    c->err = "mdg_set_thermal_bath: bad geometry or non-finite To/gamma";
+    return 1;
+  }
+  int s0 = system < 0 ? 0 : system, s1 = system < 0 ? c->nsys : system + 1;
+  for (int s = s0; s < s1; s++) {
+    const double* L = &c->Lh[3 * s];
+    if (tb->geometry == MDG_TB_BOX && (L[0] == kNoPbc || L[1] == kNoPbc)) {  // SimLoop.hpp:141
+      c->err = "thermal bath box needs lateral PBC (SimLoop.hpp:141) [system " + std::to_string(s) + "]";
+      return 1;
+    }
+    if (tb->geometry == MDG_TB_SPHERE && (L[0] != kNoPbc || L[1] != kNoPbc || L[2] != kNoPbc)) {  // :166
+      c->err = "thermal bath sphere needs open boundaries (SimLoop.hpp:166) [system " + std::to_string(s) + "]";
+      return 1;
+    }
+  }
+  if (bath_alloc(c)) return 1;
+  for (int s = s0; s < s1; s++)
+    c->bath_h[s] = Bath{tb->geometry, tb->To,        tb->gamma,      tb->z_min,  tb->d_boundary, tb->z_min_free,
+                        tb->center[0], tb->center[1], tb->center[2], tb->radius};
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(c->bath_d, c->bath_h.data(), c->nsys * sizeof(Bath), hipMemcpyHostToDevice));
+  c->bath_on = false;
+  for (const Bath& b : c->bath_h) c->bath_on = c->bath_on || b.geom != TB_NONE;
+  drop_graphs(c);
+  return 0;
+}
+
+int mdg_set_bath_atoms(mdg_ctx* c, const uint8_t* apply) {
+  if (c->n == 0) {
+    c->err = "no system";
+    return 1;
+  }
+  if (bath_alloc(c)) return 1;
+  std::vector<unsigned char> a(c->n, 1);
+  if (apply)
+    for (int i = 0; i < c->n; i++) a[i] = apply[i] ? 1 : 0;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(c->tb_apply_d, a.data(), c->n, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int mdg_bath_energy(mdg_ctx* c, double* out) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (!c->ebath) {
+    for (int s = 0; s < c->nsys; s++) out[s] = 0.0;
+    return 0;
+  }
+  HIPCHK(c, hipMemcpy(out, c->ebath, c->nsys * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
 
 int mdg_energy_grad(mdg_ctx* c, double* e_terms, double* grad) {
   if (c->n == 0) {
@@ -961,6 +1073,8 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
   // and flag it needs lives in device memory, so the same launches can be
   // captured once and replayed.
   auto body = [&]() {
+    if (c->bath_on)
+      launch(c, "k_bath_temp", [&] { k_bath_temp<<<nsys, 1024, 0, c->stream>>>(A, c->box); });
     launch(c, "k_drift", [&] { k_drift<<<blocks(n), 256, 0, c->stream>>>(A, c->dtp); });
     launch(c, "k_sys_disp", [&] {
       k_sys_disp<<<nsys, 1024, 0, c->stream>>>(A.sys_off, A.disp_s, A.disp_l, A.disp_p, lim_s, lim_l,
@@ -970,7 +1084,7 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
     launch(c, "k_kick", [&] { k_kick<<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->dtp); });
     launch(c, "k_sys_step_end", [&] {
       k_sys_step_end<<<nsys, 1024, 0, c->stream>>>(A, nterms, c->eatom, c->dtp, fixed, c->stepout, c->stepidx,
-                                                  nsys);
+                                                  nsys, c->ebath);
     });
     k_step_advance<<<1, 64, 0, c->stream>>>(c->stepidx);
   };

@@ -482,17 +482,97 @@ __device__ __forceinline__ void wrap(double& c, double L, int& cnt) {
   }
 }
 
-// eq.2-4 (SimLoop.cxx:339-399) and the PBC wrap (:401, Atom.cxx:93-125).
+// Membership of the bath geometry (SimLoop.hpp:139-151,164-170,174-185) at
+// the given (unwrapped) position; L is the system's box.
+__device__ __forceinline__ bool bath_inside(const Bath& b, double x, double y, double z, const double* L) {
+  switch (b.geom) {
+    case TB_UNIVERSE:
+      return true;
+    case TB_BOX:
+      return (z > b.zMin) ||
+             (((x < 0.0 + b.dBoundary) || (x > L[0] - b.dBoundary) || (y < 0.0 + b.dBoundary) ||
+               (y > L[1] - b.dBoundary)) &&
+              z > b.zFree);
+    case TB_SPHERE: {
+      double dx = b.cx - x, dy = b.cy - y, dz = b.cz - z;
+      return z > b.zFree && sqrt(dx * dx + dy * dy + dz * dz) > b.radius;
+    }
+    default:
+      return false;
+  }
+}
+
+// actualTemperatureOfThermalBath (SimLoop.cxx:516-535), one block per system,
+// at the start of the step (positions and velocities at t).
+__global__ void k_bath_temp(Atoms A, Box bx) {
+  int s = blockIdx.x;
+  const Bath b = A.bath[s];
+  if (b.geom == TB_NONE) {
+    if (threadIdx.x == 0) A.tb_T[s] = 0.0;
+    return;
+  }
+  const double* L = bx.L + 3 * s;
+  double e = 0.0, cnt = 0.0;
+  for (int i = A.sys_off[s] + threadIdx.x; i < A.sys_off[s + 1]; i += blockDim.x) {
+    if (A.fixed[i] || !A.tb_apply[i] || !bath_inside(b, A.x[i], A.y[i], A.z[i], L)) continue;
+    double vv = sqrt(A.vx[i] * A.vx[i] + A.vy[i] * A.vy[i] + A.vz[i] * A.vz[i]);
+    e += A.mass[i] * (vv * vv) / 2.0;
+    cnt += 1.0;
+  }
+  e = block_sum(e);
+  cnt = block_sum(cnt);
+  if (threadIdx.x == 0) A.tb_T[s] = (cnt == 0.0) ? 0.0 : e / (3.0 / 2.0 * (8.617343E-5 * kEV) * cnt);
+}
+
+// eq.2-4 of SimLoop::executeMain (SimLoop.cxx:338-399) with the thermal
+// bath's friction (:352-386) on atoms inside its geometry at x(t+dt), then
+// the PBC wrap (:401, Atom.cxx:93-125).
 __global__ void k_kick(Atoms A, Box bx, const double* __restrict__ dtp) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
+  const double* L = box_of(bx, i);
+  double x = A.x[i], y = A.y[i], z = A.z[i];
   if (!A.fixed[i]) {
-    double dt = dtp[A.sys[i]];
+    const int s = A.sys[i];
+    double dt = dtp[s];
     double M = A.mass[i];
     double fx = -A.grad[3 * i], fy = -A.grad[3 * i + 1], fz = -A.grad[3 * i + 2];
     double hx = A.vx[i] + A.ax[i] * dt / 2.0;
     double hy = A.vy[i] + A.ay[i] * dt / 2.0;
     double hz = A.vz[i] + A.az[i] * dt / 2.0;
+    if (A.bath) {
+      double dE = 0.0;
+      const Bath& b = A.bath[s];
+      if (b.geom != TB_NONE && A.tb_apply[i] && bath_inside(b, x, y, z, L)) {
+        const double T = A.tb_T[s];
+        double To_by_T = (fabs(T) < 1e-5) ? 0 : (b.To / T);
+        if (To_by_T < -5.0) To_by_T = -5.0;
+        if (To_by_T > 5.0) To_by_T = 5.0;
+        const double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i];
+        const double k = 1.0 - sqrt(To_by_T);
+        const double dfx = -vx * M * b.gamma * k, dfy = -vy * M * b.gamma * k, dfz = -vz * M * b.gamma * k;
+        // energy taken from the bath: kinetic energy after the step with and
+        // without its force (eq.2 and eq.4 with both accelerations)
+        const double nx = fx / M, ny = fy / M, nz = fz / M;
+        const double ux = A.an_nt[3 * i] * dt / 2.0 + nx * dt / 2.0;
+        const double uy = A.an_nt[3 * i + 1] * dt / 2.0 + ny * dt / 2.0;
+        const double uz = A.an_nt[3 * i + 2] * dt / 2.0 + nz * dt / 2.0;
+        A.an_nt[3 * i] = nx;
+        A.an_nt[3 * i + 1] = ny;
+        A.an_nt[3 * i + 2] = nz;
+        const double bx_ = (fx + dfx) / M, by_ = (fy + dfy) / M, bz_ = (fz + dfz) / M;
+        const double wx = A.ax[i] * dt / 2.0 + bx_ * dt / 2.0;
+        const double wy = A.ay[i] * dt / 2.0 + by_ * dt / 2.0;
+        const double wz = A.az[i] * dt / 2.0 + bz_ * dt / 2.0;
+        const double r1 = sqrt((vx + wx) * (vx + wx) + (vy + wy) * (vy + wy) + (vz + wz) * (vz + wz));
+        const double r0 = sqrt((vx + ux) * (vx + ux) + (vy + uy) * (vy + uy) + (vz + uz) * (vz + uz));
+        dE = M * (r1 * r1) / 2.0 - M * (r0 * r0) / 2.0;
+        fx += dfx;
+        fy += dfy;
+        fz += dfz;
+      }
+      A.tb_dE[i] = dE;
+    }
     double ax = fx / M, ay = fy / M, az = fz / M;
     A.ax[i] = ax;
     A.ay[i] = ay;
@@ -501,8 +581,6 @@ __global__ void k_kick(Atoms A, Box bx, const double* __restrict__ dtp) {
     A.vy[i] = hy + ay * dt / 2.0;
     A.vz[i] = hz + az * dt / 2.0;
   }
-  const double* L = box_of(bx, i);
-  double x = A.x[i], y = A.y[i], z = A.z[i];
   int cx = A.pcx[i], cy = A.pcy[i], cz = A.pcz[i];
   wrap(x, L[0], cx);
   wrap(y, L[1], cy);
@@ -530,6 +608,9 @@ __global__ void k_init_acc(Atoms A) {
   A.ax[i] = -A.grad[3 * i] / M;
   A.ay[i] = -A.grad[3 * i + 1] / M;
   A.az[i] = -A.grad[3 * i + 2] / M;
+  A.an_nt[3 * i] = A.ax[i];
+  A.an_nt[3 * i + 1] = A.ay[i];
+  A.an_nt[3 * i + 2] = A.az[i];
 }
 
 // Per-system energies of each chain member: eacc[8*s+t] = sum of the
@@ -551,7 +632,8 @@ __global__ void k_sys_energy(int nterms, const int* __restrict__ sys_off, int n,
 // neighbours (:394-398, :409-416), one block per system.
 // out[(step*nsys + s)*4 + {0,1,2,3}] = epot, ekin, dt used, |sum F|.
 __global__ void k_sys_step_end(Atoms A, int nterms, const double* __restrict__ eatom, double* __restrict__ dtp,
-                               int fixed_dt, double* __restrict__ out, const int* __restrict__ stepidx, int nsys) {
+                               int fixed_dt, double* __restrict__ out, const int* __restrict__ stepidx, int nsys,
+                               double* __restrict__ ebath) {
   int s = blockIdx.x;
   int b = A.sys_off[s], e = A.sys_off[s + 1];
   int n = A.n;
@@ -571,6 +653,12 @@ __global__ void k_sys_step_end(Atoms A, int nterms, const double* __restrict__ e
   fx = block_sum(fx);
   fy = block_sum(fy);
   fz = block_sum(fz);
+  if (A.tb_dE) {  // energy taken from the bath this step (SimLoop.cxx:384)
+    double eb = 0.0;
+    for (int i = b + threadIdx.x; i < e; i += blockDim.x) eb += A.tb_dE[i];
+    eb = block_sum(eb);
+    if (threadIdx.x == 0) ebath[s] += eb;
+  }
   for (int o = 32; o > 0; o >>= 1) vm = fmax(vm, __shfl_xor(vm, o, 64));
   __shared__ double vms[16];
   __syncthreads();

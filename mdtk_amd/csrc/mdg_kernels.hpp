@@ -12,6 +12,14 @@ struct alignas(8 * sizeof(F)) BondRec {
   F x, y, z, r, f, df, pad0, pad1;
 };
 
+// Thermal bath of one system (SimLoop.hpp:118-189): friction
+// -V M gamma (1 - sqrt(To/T)) on the atoms inside the geometry; lengths in cm.
+enum : int { TB_NONE = 0, TB_UNIVERSE = 1, TB_BOX = 2, TB_SPHERE = 3 };
+struct Bath {
+  int geom;
+  double To, gamma, zMin, dBoundary, zFree, cx, cy, cz, radius;
+};
+
 // Device view of the atoms and of the per-step derived data (SoA in HBM).
 // A batch of independent systems is one array of atoms: system s owns atoms
 // [sys_off[s], sys_off[s+1]) and sys[i] names atom i's system.
@@ -46,6 +54,12 @@ struct Atoms {
   BondRec<double>* b64;       // FP64 bond geometry (cm)
   BondRec<float>* b32;        // FP32 bond geometry (Angstrom), mixed precision
   double *Nt, *NH, *NC;       // REBO coordination sums (REBO.hpp:250-258)
+  // thermal bath (null when no system of the context has one)
+  const Bath* bath;           // per system
+  const unsigned char* tb_apply;  // per atom: Atom::apply_ThermalBath
+  double* tb_T;               // per system: bath temperature at the start of the step
+  double* tb_dE;              // per atom: energy taken from the bath this step
+  double* an_nt;              // n*3 acceleration without the bath (kept with or without a bath)
 };
 
 struct CellGrid {
@@ -115,7 +129,8 @@ __global__ void k_init_acc(Atoms A);
 __global__ void k_pack_pos(Atoms A);
 __global__ void k_sys_energy(int nterms, const int* sys_off, int n, const double* eatom, double* eacc);
 __global__ void k_sys_step_end(Atoms A, int nterms, const double* eatom, double* dtp, int fixed_dt, double* out,
-                               const int* stepidx, int nsys);
+                               const int* stepidx, int nsys, double* ebath);
+__global__ void k_bath_temp(Atoms A, Box bx);
 __global__ void k_step_advance(int* stepidx);
 __global__ void k_zero(double* p, size_t n);
 __global__ void k_zero_i(int* p, int n);

@@ -73,8 +73,20 @@ def load_library(path: str = LIB_PATH):
     L.mdg_event_elapsed.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
     L.mdg_synchronize.argtypes = [vp]
     L.mdg_autotune.argtypes = [vp, ctypes.c_int, dp]
+    L.mdg_set_thermal_bath.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Bath)]
+    L.mdg_set_bath_atoms.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8)]
+    L.mdg_bath_energy.argtypes = [vp, dp]
     _lib = L
     return L
+
+
+class _Bath(ctypes.Structure):
+    _fields_ = [("geometry", ctypes.c_int), ("To", ctypes.c_double), ("gamma", ctypes.c_double),
+                ("z_min", ctypes.c_double), ("d_boundary", ctypes.c_double), ("z_min_free", ctypes.c_double),
+                ("center", ctypes.c_double * 3), ("radius", ctypes.c_double)]
+
+
+BATH_GEOMETRIES = {"none": 0, "universe": 1, "box": 2, "sphere": 3}
 
 
 def _p(a, t=ctypes.c_double):
@@ -172,6 +184,27 @@ class Engine:
         g = np.zeros((self.n, 3))
         self._chk(_lib.mdg_energy_grad(self._h, _p(e), _p(g)))
         return (e if self.nsys <= 1 else e.reshape(self.nsys, self.nterms)), g
+
+    def set_thermal_bath(self, geometry="box", To=300.0, gamma=1.0e13, z_min=1.0e6 * 1e-8, d_boundary=0.0,
+                         z_min_free=-5.0e-8, center=(0.0, 0.0, 0.0), radius=1.0e6 * 1e-8, system=-1):
+        """Thermal bath of SimLoop (SimLoop.hpp:118-189) for one system or all
+        (system=-1): geometry "none", "universe", "box" or "sphere"; To in K,
+        gamma in 1/s, lengths in cm (defaults are the reference's)."""
+        tb = _Bath(BATH_GEOMETRIES[geometry] if isinstance(geometry, str) else int(geometry), float(To), float(gamma),
+                   float(z_min), float(d_boundary), float(z_min_free), (ctypes.c_double * 3)(*map(float, center)),
+                   float(radius))
+        self._chk(_lib.mdg_set_thermal_bath(self._h, int(system), ctypes.byref(tb)))
+
+    def set_bath_atoms(self, apply):
+        """Atom::apply_ThermalBath per atom (None: all)."""
+        a = None if apply is None else np.ascontiguousarray(apply, np.uint8)
+        self._chk(_lib.mdg_set_bath_atoms(self._h, _p(a, ctypes.c_uint8)))
+
+    def bath_energy(self):
+        """Energy each system took from its bath so far (erg, per system)."""
+        out = np.zeros(self.nsys)
+        self._chk(_lib.mdg_bath_energy(self._h, _p(out)))
+        return out
 
     def params(self):
         """The NUM_PARAMS differentiable parameters (CGS), see PARAM_NAMES."""
