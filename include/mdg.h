@@ -188,7 +188,7 @@ int mdg_kernel_time(mdg_ctx* ctx, const char* kernel, double* total_ms, int64_t*
 /* Autotuner: times candidate launch shapes of the force kernels on the
  * context's own systems (reps force calls per candidate, which leave the
  * trajectory unchanged) and keeps the fastest for every later call: the
- * waves-per-SIMD variant of the bond-order kernels (1 or 2), the shape of
+ * waves-per-SIMD variant of the bond-order kernels (1-2; 1-4 in the mixed form), the shape of
  * the LJ kernel (lanes per atom 64, 64 held to 3 waves/SIMD, 32, 16, or 16
  * held to 4 waves/SIMD in the mixed form) and
  * the grid sizes of the bond, LJ, b* and torsion kernels.  report (may be

@@ -443,6 +443,10 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
       };
       if constexpr (!TH) {
         if (tu.occ_bonds == 2) return go(std::integral_constant<int, 2>());
+        if constexpr (std::is_same<R, float>::value) {  // 3-4 waves/SIMD: FP32 form only
+          if (tu.occ_bonds == 3) return go(std::integral_constant<int, 3>());
+          if (tu.occ_bonds == 4) return go(std::integral_constant<int, 4>());
+        }
       }
       go(std::integral_constant<int, 1>());
     });
@@ -491,6 +495,10 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
       };
       if constexpr (!TH) {
         if (tu.occ_bstar == 2) return go(std::integral_constant<int, 2>());
+        if constexpr (std::is_same<R, float>::value) {
+          if (tu.occ_bstar == 3) return go(std::integral_constant<int, 3>());
+          if (tu.occ_bstar == 4) return go(std::integral_constant<int, 4>());
+        }
       }
       go(std::integral_constant<int, 1>());
     });
@@ -1298,7 +1306,8 @@ int mdg_autotune(mdg_ctx* c, int reps, double* report) {
   };
   Tune& tu = c->tune;
   if (c->slot_rebo >= 0) {
-    tune_knob("k_rebo_bonds", tu.occ_bonds, {1, 2});
+    if (c->mixed) tune_knob("k_rebo_bonds", tu.occ_bonds, {1, 2, 3, 4});
+    else tune_knob("k_rebo_bonds", tu.occ_bonds, {1, 2});
     tune_knob("k_rebo_bonds", tu.grid_bonds, grids(nb, 256));
   }
   if (c->slot_lj >= 0) {
@@ -1306,7 +1315,8 @@ int mdg_autotune(mdg_ctx* c, int reps, double* report) {
     else tune_knob("k_lj_wave", tu.lj_shape, {0, 1, 2, 3});
     int lw = (tu.lj_shape >= 3) ? 16 : (tu.lj_shape == 2) ? 32 : 64;
     tune_knob("k_lj_wave", tu.grid_lj, grids(c->n, 4 * (64 / lw)));
-    tune_knob("k_bstar", tu.occ_bstar, {1, 2});
+    if (c->mixed) tune_knob("k_bstar", tu.occ_bstar, {1, 2, 3, 4});
+    else tune_knob("k_bstar", tu.occ_bstar, {1, 2});
     tune_knob("k_bstar", tu.grid_bstar, grids(nq, 256));
   }
   if (c->slot_tors >= 0) tune_knob("k_etors", tu.grid_tors, grids(nb, 256));

@@ -1479,5 +1479,7 @@ MDG_INSTANTIATE_OCC(double, false, 2)
 MDG_INSTANTIATE_OCC(double, true, 1)
 MDG_INSTANTIATE_OCC(float, false, 1)
 MDG_INSTANTIATE_OCC(float, false, 2)
+MDG_INSTANTIATE_OCC(float, false, 3)
+MDG_INSTANTIATE_OCC(float, false, 4)
 
 }  // namespace mdg
