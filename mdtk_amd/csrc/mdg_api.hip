@@ -55,6 +55,10 @@ struct mdg_ctx {
   TablesT<float>* dTf = nullptr;  // mixed precision: Angstrom/eV FP32 table
   bool mixed = false;
   hipStream_t own = nullptr, stream = nullptr;
+  // second stream (default priority) for the bond-centred force kernels, which
+  // run beside the LJ chain; joined back into `stream` by events
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::vector<void*> allocs;
 
   // one cell grid per system and list; cell ids offset per system
@@ -120,6 +124,9 @@ struct mdg_ctx {
     if (dT) hipFree(dT);
     if (dTf) hipFree(dTf);
     if (own) hipStreamDestroy(own);
+    if (side) hipStreamDestroy(side);
+    if (ev_fork) hipEventDestroy(ev_fork);
+    if (ev_join) hipEventDestroy(ev_join);
   }
 };
 
@@ -435,10 +442,21 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   int bgrid = tu.grid_bonds ? tu.grid_bonds : blocks(2LL * n, 256, 8192);
   int tgrid = tu.grid_tors ? tu.grid_tors : blocks(2LL * n, 256, 8192);
   int sgrid = tu.grid_bstar ? tu.grid_bstar : blocks(8LL * n, 256, 16384);
+  // The bond-centred kernels (REBO bonds, Brenner, torsion) only read the
+  // bond lists and add into grad, so they run on the side stream beside the
+  // LJ chain (pair rows -> LJ -> b*); an event joins them back.  Profiled
+  // calls stay on one stream so that every kernel's own time is measured.
+  const bool fork = !c->profiling && c->slot_lj >= 0 &&
+                    (c->slot_rebo >= 0 || c->slot_bren >= 0 || c->slot_tors >= 0);
+  hipStream_t sb = fork ? c->side : c->stream;
+  if (fork) {
+    hipEventRecord(c->ev_fork, c->stream);
+    hipStreamWaitEvent(c->side, c->ev_fork, 0);
+  }
   if (c->slot_rebo >= 0) {
     launch(c, "k_rebo_bonds", [&] {
       auto go = [&](auto occ) {
-        k_rebo_bonds<R, TH, decltype(occ)::value><<<bgrid, 256, 0, c->stream>>>(
+        k_rebo_bonds<R, TH, decltype(occ)::value><<<bgrid, 256, 0, sb>>>(
             A, T, c->dT, c->box, c->bonds, c->nbonds, A.grad, c->eatom + (size_t)c->slot_rebo * n, c->derr, dth);
       };
       if constexpr (!TH) {
@@ -453,8 +471,14 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   }
   if (c->slot_bren >= 0) {
     launch(c, "k_brenner_bonds", [&] {
-      k_brenner_bonds<<<bgrid, 256, 0, c->stream>>>(A, c->dT, c->box, c->bonds, c->nbonds, A.grad,
+      k_brenner_bonds<<<bgrid, 256, 0, sb>>>(A, c->dT, c->box, c->bonds, c->nbonds, A.grad,
                                                     c->eatom + (size_t)c->slot_bren * n, c->derr);
+    });
+  }
+  if (c->slot_tors >= 0) {
+    launch(c, "k_etors", [&] {
+      k_etors<R><<<tgrid, 256, 0, sb>>>(A, T, c->box, c->bonds, c->nbonds, c->cfg.etors_compat, A.grad,
+                                               c->eatom + (size_t)c->slot_tors * n, c->tally + 1);
     });
   }
   if (c->slot_lj >= 0) {
@@ -503,11 +527,9 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
       go(std::integral_constant<int, 1>());
     });
   }
-  if (c->slot_tors >= 0) {
-    launch(c, "k_etors", [&] {
-      k_etors<R><<<tgrid, 256, 0, c->stream>>>(A, T, c->box, c->bonds, c->nbonds, c->cfg.etors_compat, A.grad,
-                                               c->eatom + (size_t)c->slot_tors * n, c->tally + 1);
-    });
+  if (fork) {
+    hipEventRecord(c->ev_join, c->side);
+    hipStreamWaitEvent(c->stream, c->ev_join, 0);
   }
 }
 
@@ -607,6 +629,13 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
     return 1;
   }
   c->stream = c->own;
+  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+    g_create_err = "hipStreamCreate/hipEventCreate failed";
+    delete c;
+    return 1;
+  }
   Tables h;
   build_tables(h, cfg->brenner_paramset);
   c->host_T = h;
