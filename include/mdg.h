@@ -30,7 +30,8 @@ enum {
   MDG_POT_BRENNER = 2,
   MDG_POT_REBO = 4,
   MDG_POT_AIREBO = 8,
-  MDG_POT_ETORS = 16
+  MDG_POT_ETORS = 16,
+  MDG_POT_FLJ = 32      /* FLJ (pairwise/FLJ.cxx): {H,C} x {Cu,Au}, chained after FBZL (common.h:49) */
 };
 
 enum { MDG_PREC_FP64 = 0, MDG_PREC_MIXED = 1 };

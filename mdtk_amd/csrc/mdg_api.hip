@@ -40,7 +40,8 @@ struct mdg_ctx {
   mdg_config cfg{};
   std::string err;
   std::vector<int> kinds;
-  int slot_fbzl = -1, slot_rebo = -1, slot_bren = -1, slot_lj = -1, slot_tors = -1;
+  int slot_fbzl = -1, slot_flj = -1, slot_rebo = -1, slot_bren = -1, slot_lj = -1, slot_tors = -1;
+  FLJTable* dFLJ = nullptr;  // FLJ parameters (device)
   bool have_short = false, have_long = false;
   double rc_short = 0, rc_long = 0;
 
@@ -123,6 +124,7 @@ struct mdg_ctx {
     for (void* p : allocs) hipFree(p);
     if (dT) hipFree(dT);
     if (dTf) hipFree(dTf);
+    if (dFLJ) hipFree(dFLJ);
     if (own) hipStreamDestroy(own);
     if (side) hipStreamDestroy(side);
     if (ev_fork) hipEventDestroy(ev_fork);
@@ -551,6 +553,11 @@ void force_call(mdg_ctx* c) {
   if (c->mixed) force_terms<float, false>(c, c->dTf);
   else if (c->theta_mode) force_terms<double, true>(c, c->dT);
   else force_terms<double, false>(c, c->dT);
+  if (c->slot_flj >= 0) {
+    launch(c, "k_flj", [&] {
+      k_flj<<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->dFLJ, A.grad, c->eatom + (size_t)c->slot_flj * n);
+    });
+  }
   if (c->slot_fbzl >= 0) {
     launch(c, "k_fbzl", [&] {
       k_fbzl<<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->cfg.fbzl_r0, c->cfg.fbzl_r1, A.grad,
@@ -580,7 +587,7 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
     g_create_err = "Brenner replaces REBO: select one of them";
     return 1;
   }
-  if (!(m & (MDG_POT_REBO | MDG_POT_BRENNER | MDG_POT_ETORS | MDG_POT_FBZL))) {
+  if (!(m & (MDG_POT_REBO | MDG_POT_BRENNER | MDG_POT_ETORS | MDG_POT_FBZL | MDG_POT_FLJ))) {
     g_create_err = "no potentials selected";
     return 1;
   }
@@ -613,16 +620,18 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
   }
   // chain order of applications/common.h:46-68
   if (m & MDG_POT_FBZL) { c->slot_fbzl = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_FBZL); }
+  if (m & MDG_POT_FLJ) { c->slot_flj = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_FLJ); }
   if (m & MDG_POT_BRENNER) { c->slot_bren = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_BRENNER); }
   if (m & MDG_POT_REBO) { c->slot_rebo = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_REBO); }
   if (m & MDG_POT_AIREBO) { c->slot_lj = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_AIREBO); }
   if (m & MDG_POT_ETORS) { c->slot_tors = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_ETORS); }
   c->have_short = (m & (MDG_POT_REBO | MDG_POT_BRENNER | MDG_POT_ETORS)) != 0;
-  c->have_long = (m & (MDG_POT_AIREBO | MDG_POT_FBZL)) != 0;
+  c->have_long = (m & (MDG_POT_AIREBO | MDG_POT_FBZL | MDG_POT_FLJ)) != 0;
   c->rc_short = 2.0 * kAo;  // REBO/Brenner/ETors getRcutoff (REBO.hpp:130)
   c->rc_long = 0;
   if (m & MDG_POT_AIREBO) c->rc_long = 5.5 * kAo;  // AIREBO_LJ.hpp:81
   if (m & MDG_POT_FBZL) c->rc_long = std::max(c->rc_long, c->cfg.fbzl_r1);
+  if (m & MDG_POT_FLJ) c->rc_long = std::max(c->rc_long, 5.5 * kAo);  // common.h:49
   if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
     g_create_err = "hipStreamCreate failed";
     delete c;
@@ -651,6 +660,16 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
     convert_tables(h, hf);
     if (hipMalloc(&c->dTf, sizeof(hf)) != hipSuccess ||
         hipMemcpy(c->dTf, &hf, sizeof(hf), hipMemcpyHostToDevice) != hipSuccess) {
+      g_create_err = "table upload failed";
+      delete c;
+      return 1;
+    }
+  }
+  if (c->slot_flj >= 0) {
+    FLJTable ft;
+    build_flj_table(ft);
+    if (hipMalloc(&c->dFLJ, sizeof(ft)) != hipSuccess ||
+        hipMemcpy(c->dFLJ, &ft, sizeof(ft), hipMemcpyHostToDevice) != hipSuccess) {
       g_create_err = "table upload failed";
       delete c;
       return 1;
@@ -1057,17 +1076,20 @@ int mdg_nl_export(mdg_ctx* c, int term, int32_t* counts, int32_t* idx, int64_t c
   HIPCHK(c, hipMemcpy(ty.data(), c->A.type, n * 4, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(el.data(), c->A.elem, n * 4, hipMemcpyDeviceToHost));
   auto zbl_handled = [](int e) { return e == 1 || e == 12 || e == 64 || e == 197 || e == 108 || e == 40 || e == 131; };
+  auto flj_handled = [](int e) { return e == 1 || e == 12 || e == 64 || e == 197; };  // FLJ.cxx:38-52
   int64_t pos = 0;
   for (int i = 0; i < n; i++) {
     int m = 0;
     bool row_ok = true;
     if (kind == MDG_POT_AIREBO) row_ok = ty[i] <= T_H;
     if (kind == MDG_POT_FBZL) row_ok = zbl_handled(el[i]);
+    if (kind == MDG_POT_FLJ) row_ok = flj_handled(el[i]);
     if (row_ok)
       for (int q = 0; q < cnt[i]; q++) {
         int j = lst[(size_t)i * stride + q];
         if (kind == MDG_POT_AIREBO && ty[j] > T_H) continue;
         if (kind == MDG_POT_FBZL && !zbl_handled(el[j])) continue;
+        if (kind == MDG_POT_FLJ && !flj_handled(el[j])) continue;
         if (pos >= cap) {
           c->err = "nl_export capacity";
           return 1;
@@ -1252,7 +1274,8 @@ int mdg_neighbor_counts(mdg_ctx* c, int32_t* counts) {
   if (c->have_long) HIPCHK(c, hipMemcpy(l.data(), c->A.nll_cnt, n * 4, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(ty.data(), c->A.type, n * 4, hipMemcpyDeviceToHost));
   for (int i = 0; i < n; i++) {
-    bool long_handled = (c->slot_fbzl >= 0) || (c->slot_lj >= 0 && ty[i] <= T_H);
+    bool long_handled = (c->slot_fbzl >= 0) || (c->slot_lj >= 0 && ty[i] <= T_H) ||
+                        (c->slot_flj >= 0 && ty[i] != T_ION);  // FLJ handles H, C, Cu, Au
     counts[i] = s[i] + (long_handled ? l[i] : 0);
   }
   return 0;

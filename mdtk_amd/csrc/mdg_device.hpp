@@ -86,6 +86,17 @@ struct TablesT {
   R BH[2][5][5][3];            // Brenner H_CC, H_CH
   R BF[5][5][6][4];            // Brenner F (k nodes 0..5)
 };
+// FLJ (pairwise/FLJ.cxx): element index e2i (Cu 0, H 1, C 2, Au 3), LB-mixed
+// sigma/epsilon, the ZBL-to-LJ cubic bridge ends x[0..1] with the values and
+// slopes of each element pair, and the cosine cutoff R0..R1 (cm, erg).
+struct FLJTable {
+  double sigma[4][4], eps[4][4];
+  double x[2];
+  double v[4][4][2], d[4][4][2];
+  double R0, R1;
+};
+void build_flj_table(FLJTable& t);
+
 using Tables = TablesT<double>;
 using ReboParams = ReboParamsT<double>;
 using LJParams = LJParamsT<double>;

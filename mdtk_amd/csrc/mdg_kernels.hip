@@ -442,6 +442,78 @@ __global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* __restrict
   if (e != 0.0) unsafeAtomicAdd(eatom + i, e);
 }
 
+// FLJ::operator() (FLJ.cxx:256-341) for the pairs {H,C} x {Cu,Au}: each metal
+// atom takes its hydrocarbon partners from the long list (so every pair once);
+// ZBL below x[0], the cubic bridge to x[1], LJ times the cosine cutoff beyond.
+// The pair's energy goes to its lower-index atom, as the reference's i < j
+// loop sums it.
+__device__ __forceinline__ int flj_index(int elem) {
+  switch (elem) {
+    case 64: return 0;   // Cu
+    case 1: return 1;    // H
+    case 12: return 2;   // C
+    case 197: return 3;  // Au
+    default: return -1;
+  }
+}
+
+__global__ void k_flj(Atoms A, Box bx, const FLJTable* __restrict__ T, double* __restrict__ grad,
+                      double* __restrict__ eatom) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= A.n) return;
+  const int em = A.elem[m];
+  if (em != 64 && em != 197) return;
+  const int im = flj_index(em);
+  const int cnt = A.nll_cnt[m];
+  const int* list = A.nll + (size_t)m * A.cap_l;
+  const double R1 = T->R1;
+  for (int q = 0; q < cnt; q++) {
+    const int h = list[q];
+    const int eh = A.elem[h];
+    if (eh != 1 && eh != 12) continue;
+    const int a = m < h ? m : h, b = m < h ? h : m;  // AtomsPair(atom_i, atom_j), i < j
+    const int ia = (a == m) ? im : flj_index(eh), ib = (a == m) ? flj_index(eh) : im;
+    D3 rv = depos(A.p, a, b, bx);
+    double d2 = norm2(rv);
+    if (d2 > R1 * R1) continue;  // FPairwise::probablyAreNeighbours
+    double r = sqrt(d2);
+    double val, der;
+    if (r < T->x[0]) {
+      double ZA = A.Z[a], ZB = A.Z[b];
+      double AS = 8.8534e-1 * 0.53e-8 / (pow(ZA / kECharge, 0.23) + pow(ZB / kECharge, 0.23));
+      double Y = r / AS;
+      double e1 = exp(-3.1998 * Y), e2 = exp(-0.94229 * Y), e3 = exp(-0.4029 * Y), e4 = exp(-0.20162 * Y);
+      double sum = 0.18175 * e1 + 0.50986 * e2 + 0.28022 * e3 + 0.02817 * e4;
+      val = ZA * ZB / r * sum;
+      der = -ZA * ZB / (r * r) * sum -
+            ZA * ZB / (r * AS) *
+                (0.18175 * 3.1998 * e1 + 0.50986 * 0.94229 * e2 + 0.28022 * 0.4029 * e3 + 0.02817 * 0.20162 * e4);
+    } else if (r < T->x[1]) {  // the cubic through both ends' values and slopes (Spline.cxx:72-98)
+      const double* v = T->v[ia][ib];
+      const double* dd = T->d[ia][ib];
+      double hh = T->x[1] - T->x[0], t = (r - T->x[0]) / hh;
+      val = (1 + 2 * t) * (1 - t) * (1 - t) * v[0] + t * (1 - t) * (1 - t) * hh * dd[0] + t * t * (3 - 2 * t) * v[1] +
+            t * t * (t - 1) * hh * dd[1];
+      der = (6 * t * (t - 1) * v[0] - 6 * t * (t - 1) * v[1]) / hh + (1 - t) * (1 - 3 * t) * dd[0] +
+            t * (3 * t - 2) * dd[1];
+    } else {
+      double s = T->sigma[ia][ib] / r, s6 = s;
+      for (int u = 2; u <= 6; u++) s6 *= s;
+      double s12 = s6 * s6;
+      double lv = 4.0 * T->eps[ia][ib] * (s12 - s6);
+      double ld = 4.0 * T->eps[ia][ib] * (-12.0 * s12 / r + 6.0 * s6 / r);
+      double f, df;
+      cutoff(r, T->R0, T->R1, f, df);
+      val = f * lv;
+      der = ld * f + lv * df;
+    }
+    D3 v = rv * (der / r);
+    atomic_add3(grad, a, v);
+    atomic_add3(grad, b, -v);
+    if (val != 0.0) unsafeAtomicAdd(eatom + a, val);
+  }
+}
+
 // ------------------------------------------------------------ Verlet
 // eq.1 of SimLoop::executeMain (SimLoop.cxx:323-332): x += v dt + a dt^2/2,
 // accumulated into both neighbour lists' (and the LJ pair rows') displacements.

@@ -121,6 +121,7 @@ template <class R>
 __global__ void k_etors(Atoms A, const TablesT<R>* T, Box bx, const int2* bonds, const int* nbonds, int compat,
                         double* grad, double* eatom, long long* nquads);
 __global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* grad, double* eatom);
+__global__ void k_flj(Atoms A, Box bx, const FLJTable* T, double* grad, double* eatom);
 __global__ void k_brenner_bonds(Atoms A, const Tables* T, Box bx, const int2* bonds, const int* nbonds,
                                 double* grad, double* eatom, unsigned* err);
 __global__ void k_drift(Atoms A, const double* dtp);

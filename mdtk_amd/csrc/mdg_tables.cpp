@@ -65,6 +65,53 @@ struct Grid {
 };
 }  // namespace
 
+// FLJ parameters (FLJ.cxx:107-140: Lorentz-Berthelot mixing of like-pair
+// values) and its ZBL-to-LJ bridge ends (fillR_concat_, FLJ.cxx:158-253).
+void build_flj_table(FLJTable& t) {
+  std::memset(&t, 0, sizeof(t));
+  const double Ao = kAo, eV = kEV, kb = 8.617343E-5 * kEV;
+  enum { Cu = 0, H = 1, C = 2, Au = 3 };
+  t.sigma[Cu][Cu] = 3.05 * Ao;
+  t.eps[Cu][Cu] = 0.165565 * eV;
+  t.sigma[C][C] = 3.4 * Ao;
+  t.eps[C][C] = 0.002413 * eV;
+  t.sigma[Au][Au] = 2.569 * Ao;
+  t.eps[Au][Au] = 0.408 * eV;
+  t.sigma[H][H] = 2.81 * Ao;
+  t.eps[H][H] = 8.6 * kb;
+  for (int m : {Cu, Au})
+    for (int q : {C, H}) {
+      t.sigma[m][q] = t.sigma[q][m] = (t.sigma[m][m] + t.sigma[q][q]) / 2;
+      t.eps[m][q] = t.eps[q][m] = std::sqrt(t.eps[m][m] * t.eps[q][q]);
+    }
+  t.x[0] = 1.2 * Ao;
+  t.x[1] = 1.9 * Ao;
+  t.R0 = 5.0 * Ao;  // common.h:49
+  t.R1 = 5.5 * Ao;
+  const double Z[4] = {29.0 * kECharge, 1.0 * kECharge, 6.0 * kECharge, 79.0 * kECharge};
+  for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 4; b++) {
+      // ZBL value and slope at x[0]
+      double r = t.x[0];
+      double AS = 8.8534e-1 * 0.53e-8 / (std::pow(Z[a] / kECharge, 0.23) + std::pow(Z[b] / kECharge, 0.23));
+      double Y = r / AS;
+      double e1 = std::exp(-3.1998 * Y), e2 = std::exp(-0.94229 * Y), e3 = std::exp(-0.4029 * Y),
+             e4 = std::exp(-0.20162 * Y);
+      double sum = 0.18175 * e1 + 0.50986 * e2 + 0.28022 * e3 + 0.02817 * e4;
+      t.v[a][b][0] = Z[a] * Z[b] / r * sum;
+      t.d[a][b][0] = -Z[a] * Z[b] / (r * r) * sum -
+                     Z[a] * Z[b] / (r * AS) *
+                         (0.18175 * 3.1998 * e1 + 0.50986 * 0.94229 * e2 + 0.28022 * 0.4029 * e3 + 0.02817 * 0.20162 * e4);
+      // LJ value and slope at x[1] (no cutoff there)
+      r = t.x[1];
+      double s = t.sigma[a][b] / r, s6 = s;
+      for (int u = 2; u <= 6; u++) s6 *= s;
+      double s12 = s6 * s6;
+      t.v[a][b][1] = 4.0 * t.eps[a][b] * (s12 - s6);
+      t.d[a][b][1] = 4.0 * t.eps[a][b] * (-12.0 * s12 / r + 6.0 * s6 / r);
+    }
+}
+
 void build_tables(Tables& T, int brenner_set) {
   std::memset(&T, 0, sizeof(T));
   T.lunit = 1.0;  // the FP64 table works in the reference's cm and erg

@@ -13,7 +13,7 @@ using namespace orc;
 namespace {
 thread_local std::string g_err;
 
-enum : int { POT_FBZL = 1, POT_BRENNER = 2, POT_REBO = 4, POT_AIREBO = 8, POT_ETORS = 16 };
+enum : int { POT_FBZL = 1, POT_BRENNER = 2, POT_REBO = 4, POT_AIREBO = 8, POT_ETORS = 16, POT_FLJ = 32 };
 
 struct Handle {
   SimLoop sl;
@@ -62,6 +62,10 @@ void* orc_create(int n, const int* elem, const double* xyz, const double* vel, c
     if (pot_mask & POT_FBZL) {
       pots.push_back(new FBZL(fbzl_r0, fbzl_r1));
       h->kinds.push_back(POT_FBZL);
+    }
+    if (pot_mask & POT_FLJ) {  // common.h:49: FLJ(Rcutoff(5.0 A, 5.5 A)) after FBZL
+      pots.push_back(new FLJ(5.0 * Ao, 5.5 * Ao));
+      h->kinds.push_back(POT_FLJ);
     }
     REBO* rebo = nullptr;
     if (pot_mask & POT_BRENNER) {

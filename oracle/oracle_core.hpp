@@ -354,6 +354,23 @@ struct FBZL : Potential {  // mdtk/potentials/pairwise/FBZL.cxx
   double compute(AtomsArray& atoms) override;
 };
 
+// mdtk/potentials/pairwise/FLJ.{hpp,cxx}: hydrocarbon-metal pairs {H,C}x{Cu,Au};
+// ZBL below 1.2 A, a cubic bridge (value and slope of both ends) to 1.9 A,
+// Lorentz-Berthelot LJ times the cosine cutoff beyond.
+struct FLJ : Potential {
+  enum { Cu = 0, H = 1, C = 2, Au = 3, ECOUNT = 4 };
+  double R0, R1;
+  double sigma_[ECOUNT][ECOUNT], epsilon_[ECOUNT][ECOUNT];
+  double x_[2] = {1.2 * Ao, 1.9 * Ao};
+  double v_[ECOUNT][ECOUNT][2], d_[ECOUNT][ECOUNT][2];  // bridge ends: values and slopes
+  FLJ(double r0, double r1);
+  double getRcutoff() const override { return R1; }
+  int e2i(int id) const;
+  bool isHandledPair(const Atom& a, const Atom& b) const;
+  double VLJ(Pair& ij);
+  double compute(AtomsArray& atoms) override;
+};
+
 // mdtk/potentials/FProxy.{hpp,cxx} + the NL update in NeighbourList.cxx:31-106
 struct Proxy {
   std::vector<Potential*> pots;

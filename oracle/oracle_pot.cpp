@@ -1153,6 +1153,136 @@ double FBZL::compute(AtomsArray& atoms) {  // FBZL.cxx:101-127
   return Ei;
 }
 
+// =================================================================== FLJ
+namespace {
+// ZBL universal screening (FLJ.cxx:198-224, the FBZL form): value and d/dr.
+void zbl_pair(double ZA, double ZB, double r, double& v, double& d) {
+  double AB_ = 0.53e-8;
+  double AS = 8.8534e-1 * AB_ / (std::pow(ZA / e_charge, 0.23) + std::pow(ZB / e_charge, 0.23));
+  double Y = r / AS;
+  v = ZA * ZB / r *
+      (0.18175 * std::exp(-3.1998 * Y) + 0.50986 * std::exp(-0.94229 * Y) + 0.28022 * std::exp(-0.4029 * Y) +
+       0.02817 * std::exp(-0.20162 * Y));
+  d = -ZA * ZB / (r * r) *
+          (0.18175 * std::exp(-3.1998 * Y) + 0.50986 * std::exp(-0.94229 * Y) + 0.28022 * std::exp(-0.4029 * Y) +
+           0.02817 * std::exp(-0.20162 * Y)) -
+      ZA * ZB / (r * AS) *
+          (0.18175 * 3.1998 * std::exp(-3.1998 * Y) + 0.50986 * 0.94229 * std::exp(-0.94229 * Y) +
+           0.28022 * 0.4029 * std::exp(-0.4029 * Y) + 0.02817 * 0.20162 * std::exp(-0.20162 * Y));
+}
+void lj_pair(double sigma, double epsilon, double r, double& v, double& d) {  // FLJ.cxx:294-307
+  double s_div_r = sigma / r;
+  double s6 = s_div_r;
+  for (int i = 2; i <= 6; i++) s6 *= s_div_r;
+  double s12 = s6 * s6;
+  v = 4.0 * epsilon * (s12 - s6);
+  d = 4.0 * epsilon * (-12.0 * s12 / r + 6.0 * s6 / r);
+}
+// The cubic through (x0, v0, d0) and (x1, v1, d1) (Spline.cxx:72-98), in
+// the local coordinate t = (x - x0) / (x1 - x0).
+void cubic_bridge(const double x[2], const double v[2], const double d[2], double r, double& val, double& der) {
+  double h = x[1] - x[0], t = (r - x[0]) / h;
+  double h00 = (1 + 2 * t) * (1 - t) * (1 - t), h10 = t * (1 - t) * (1 - t);
+  double h01 = t * t * (3 - 2 * t), h11 = t * t * (t - 1);
+  val = h00 * v[0] + h10 * h * d[0] + h01 * v[1] + h11 * h * d[1];
+  double g00 = 6 * t * (t - 1), g10 = (1 - t) * (1 - 3 * t), g01 = -6 * t * (t - 1), g11 = t * (3 * t - 2);
+  der = (g00 * v[0] + g01 * v[1]) / h + g10 * d[0] + g11 * d[1];
+}
+}  // namespace
+
+FLJ::FLJ(double r0, double r1) : R0(r0), R1(r1) {  // FLJ.cxx:34-156
+  handled = {H_EL, C_EL, Cu_EL, Au_EL};
+  nl.Rcutoff = getRcutoff();
+  // Lorentz-Berthelot mixing of the like-pair parameters (FLJ.cxx:107-140)
+  sigma_[Cu][Cu] = 3.05 * Ao;
+  epsilon_[Cu][Cu] = 0.165565 * eV;
+  sigma_[C][C] = 3.4 * Ao;
+  epsilon_[C][C] = 0.002413 * eV;
+  sigma_[Au][Au] = 2.569 * Ao;
+  epsilon_[Au][Au] = 0.408 * eV;
+  sigma_[H][H] = 2.81 * Ao;
+  epsilon_[H][H] = 8.6 * 1.0 * kb;  // 8.6 K
+  const int metal[2] = {Cu, Au}, hc[2] = {C, H};
+  for (int m : metal)
+    for (int q : hc) {
+      sigma_[m][q] = sigma_[q][m] = (sigma_[m][m] + sigma_[q][q]) / 2;
+      epsilon_[m][q] = epsilon_[q][m] = std::sqrt(epsilon_[m][m] * epsilon_[q][q]);
+    }
+  sigma_[Cu][Au] = sigma_[Au][Cu] = 0.0;  // never used: metal-metal pairs are not handled
+  epsilon_[Cu][Au] = epsilon_[Au][Cu] = 0.0;
+  sigma_[C][H] = sigma_[H][C] = 0.0;
+  epsilon_[C][H] = epsilon_[H][C] = 0.0;
+  // the bridge of every element pair (fillR_concat_, FLJ.cxx:167-253)
+  const int ids[ECOUNT] = {Cu_EL, H_EL, C_EL, Au_EL};
+  for (int a = 0; a < ECOUNT; a++)
+    for (int b = 0; b < ECOUNT; b++) {
+      Atom A1, A2;
+      A1.ID = ids[a];
+      A2.ID = ids[b];
+      A1.setAttributesByElementID();
+      A2.setAttributesByElementID();
+      zbl_pair(A1.Z, A2.Z, x_[0], v_[a][b][0], d_[a][b][0]);
+      lj_pair(sigma_[a][b], epsilon_[a][b], x_[1], v_[a][b][1], d_[a][b][1]);
+    }
+}
+
+int FLJ::e2i(int id) const {
+  switch (id) {
+    case H_EL: return H;
+    case C_EL: return C;
+    case Cu_EL: return Cu;
+    case Au_EL: return Au;
+    default: throw OracleError("e2i() : unknown element");
+  }
+}
+
+bool FLJ::isHandledPair(const Atom& a, const Atom& b) const {  // {H,C} x {Cu,Au}, both orders
+  auto hc = [](int id) { return id == H_EL || id == C_EL; };
+  auto me = [](int id) { return id == Cu_EL || id == Au_EL; };
+  return (hc(a.ID) && me(b.ID)) || (me(a.ID) && hc(b.ID));
+}
+
+double FLJ::VLJ(Pair& ij) {  // FLJ.cxx:256-322
+  double r = ij.r();
+  int a = e2i(ij.a1->ID), b = e2i(ij.a2->ID);
+  if (r < x_[0]) {
+    double v, d;
+    zbl_pair(ij.a1->Z, ij.a2->Z, r, v, d);
+    ij.r(d);
+    return v;
+  }
+  if (r < x_[1]) {
+    double v, d;
+    cubic_bridge(x_, v_[a][b], d_[a][b], r, v, d);
+    ij.r(d);
+    return v;
+  }
+  double v, d;
+  lj_pair(sigma_[a][b], epsilon_[a][b], r, v, d);
+  double f = ij.f();
+  ij.r(d * f);
+  ij.f(v);
+  return f * v;
+}
+
+double FLJ::compute(AtomsArray& atoms) {  // FLJ.cxx:324-341
+  double Ei = 0;
+  for (size_t i = 0; i < atoms.size(); i++) {
+    Atom& a = atoms[i];
+    auto& nl_ = NL(a);
+    for (size_t j = 0; j < nl_.size(); j++) {
+      Atom& aj = atoms[nl_[j]];
+      if (a.globalIndex > aj.globalIndex) continue;
+      if (!isHandledPair(a, aj)) continue;
+      if (&a == &aj) continue;
+      if (norm2(depos(a, aj)) > SQR(R1)) continue;  // FPairwise::probablyAreNeighbours
+      Pair ij(a, aj, R0, R1);
+      Ei += VLJ(ij);
+    }
+  }
+  return Ei;
+}
+
 // =================================================================== SimLoop
 void SimLoop::prepare() {  // AtomsContainer.cxx:106-123, SimLoop.cxx:236-250
   ORC_REQUIRE(!atoms.empty());
