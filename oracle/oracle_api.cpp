@@ -13,7 +13,7 @@ using namespace orc;
 namespace {
 thread_local std::string g_err;
 
-enum : int { POT_FBZL = 1, POT_BRENNER = 2, POT_REBO = 4, POT_AIREBO = 8, POT_ETORS = 16, POT_FLJ = 32 };
+enum : int { POT_FBZL = 1, POT_BRENNER = 2, POT_REBO = 4, POT_AIREBO = 8, POT_ETORS = 16, POT_FLJ = 32, POT_ACKLAND = 64 };
 
 struct Handle {
   SimLoop sl;
@@ -39,7 +39,7 @@ const char* orc_last_error() { return g_err.c_str(); }
 void orc_set_alter_exact(int on) { g_alter_exact = on != 0; }
 
 // Chain order follows applications/common.h:46-68: FBZL, (FLJ: inert for H/C/Ar),
-// REBO, AIREBO(LJ), ETors.  Brenner replaces REBO when selected.
+// REBO, AIREBO(LJ), ETors, Ackland.  Brenner replaces REBO when selected.
 void* orc_create(int n, const int* elem, const double* xyz, const double* vel, const double* pbc,
                  const unsigned char* fixed, int pot_mask, int brenner_set, int etors_compat,
                  double fbzl_r0, double fbzl_r1) {
@@ -86,6 +86,10 @@ void* orc_create(int n, const int* elem, const double* xyz, const double* vel, c
       t->compat = etors_compat != 0;
       pots.push_back(t);
       h->kinds.push_back(POT_ETORS);
+    }
+    if (pot_mask & POT_ACKLAND) {  // common.h:67: Ackland last
+      pots.push_back(new Ackland());
+      h->kinds.push_back(POT_ACKLAND);
     }
     if (pots.empty()) throw OracleError("no potentials selected");
   });

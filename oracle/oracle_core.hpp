@@ -23,6 +23,7 @@
 //    are the gradients of the same functions as mdtk/potentials/FGeneral.hpp:
 //    136-305, equal up to rounding.
 #pragma once
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -368,6 +369,32 @@ struct FLJ : Potential {
   int e2i(int id) const;
   bool isHandledPair(const Atom& a, const Atom& b) const;
   double VLJ(Pair& ij);
+  double compute(AtomsArray& atoms) override;
+};
+
+// mdtk/potentials/manybody/Ackland/Ackland.{hpp,cxx}: the Ackland-Tichy-
+// Vitek-Finnis EAM for Cu, Ag, Au.  E = sum_i -sqrt(rho_i) + sum_{i<j} V(r),
+// rho_i = sum_j phi(r_ij) (geometric mean of the like-pair phi for unlike
+// pairs), V a sum of cubic knots with ZBL below 1.0 A and a cubic bridge
+// (value and slope of both ends) to 1.4 A.
+struct Ackland : Potential {
+  enum { Cu = 0, Ag = 1, Au = 2, ECOUNT = 3 };
+  double rk_[ECOUNT][ECOUNT][7] = {}, ak_[ECOUNT][ECOUNT][7] = {};
+  double Rk_[ECOUNT][ECOUNT][3] = {}, Ak_[ECOUNT][ECOUNT][3] = {};
+  double x_[2] = {1.0 * Ao, 1.4 * Ao};
+  double v_[ECOUNT][ECOUNT][2], d_[ECOUNT][ECOUNT][2];  // bridge ends: values and slopes
+  Ackland();
+  double getRcutoff() const override {  // Ackland.hpp:58
+    return std::max(rk_[Au][Au][1], std::max(rk_[Ag][Ag][1], rk_[Cu][Cu][1]));
+  }
+  int e2i(int id) const;
+  double knots(int a, int b, double r, double& der) const;  // the rk/ak cubic sum
+  double Phi(Pair& ij);
+  double PhiCap(int a, int b, double r) const;
+  double dPhiCap(int a, int b, double r) const;
+  double g(Pair& ij, double V = 0.0);
+  double rho(Atom& a, AtomsArray& atoms, double V = 0.0);
+  double F(Atom& a, AtomsArray& atoms);
   double compute(AtomsArray& atoms) override;
 };
 

@@ -1283,6 +1283,185 @@ double FLJ::compute(AtomsArray& atoms) {  // FLJ.cxx:324-341
   return Ei;
 }
 
+// =================================================================== Ackland
+// The published parameter tables (Ackland, Tichy, Vitek, Finnis, Phil. Mag. A
+// 56 (1987) 735) as setupPotential lists them (Ackland.cxx:278-373): like
+// pairs in units of the lattice constant a (eV/a^3, eV^2/a^3, knots in a),
+// unlike pairs in eV/A^3 and A.
+Ackland::Ackland() {
+  handled = {Cu_EL, Ag_EL, Au_EL};  // Ackland.cxx:236-267
+  struct Unlike {
+    int a, b;
+    double r[3], k[3];
+  };
+  const Unlike un[3] = {
+      {Ag, Au, {4.40856, 3.46970, 3.00000}, {0.003566503525, 1.015075326, 0.0}},
+      {Cu, Au, {4.309816, 4.047479, 3.297946}, {-0.08554551660, 0.192835880, 0.75932286}},
+      {Ag, Cu, {4.15800, 3.15700, 3.00000}, {0.03533037752, 0.8518466353, 0.0}},
+  };
+  for (const auto& u : un)
+    for (int q = 0; q < 3; q++) {
+      rk_[u.a][u.b][q + 1] = rk_[u.b][u.a][q + 1] = u.r[q] * Ao;
+      ak_[u.a][u.b][q + 1] = ak_[u.b][u.a][q + 1] = u.k[q] * eV / (Ao * Ao * Ao);
+    }
+  struct Like {
+    int e;
+    double lat, k[6], r[6], K[2], R[2];
+  };
+  const Like lk[3] = {
+      {Cu, 3.615,
+       {61.73525861, -108.18467800, 57.00053948, -12.88796578, 39.16381901, 0.0},
+       {1.225, 1.202, 1.154, 1.050, 0.866, 0.707},
+       {10.03718305, 17.06363299},
+       {1.225, 0.990}},
+      {Ag, 4.086,
+       {20.368404, -102.36075, 94.31277, -6.220051, 31.080889, 175.56047},
+       {1.2247449, 1.1547054, 1.1180065, 1.0000000, 0.8660254, 0.7071068},
+       {1.458761, 42.946555},
+       {1.2247449, 1.0000000}},
+      {Au, 4.078,
+       {29.059066, -153.14779, 148.17881, -22.20508, 72.71465, 199.26269},
+       {1.2247449, 1.1547054, 1.1180065, 1.0000000, 0.8660254, 0.7071068},
+       {21.930125, 284.99631},
+       {1.1180065, 0.8660254}},
+  };
+  for (const auto& l : lk) {
+    double a = l.lat * Ao;
+    for (int q = 0; q < 6; q++) {
+      ak_[l.e][l.e][q + 1] = l.k[q] * eV / (a * a * a);
+      rk_[l.e][l.e][q + 1] = l.r[q] * a;
+    }
+    for (int q = 0; q < 2; q++) {
+      Ak_[l.e][l.e][q + 1] = l.K[q] * (eV * eV) / (a * a * a);
+      Rk_[l.e][l.e][q + 1] = l.R[q] * a;
+    }
+  }
+  nl.Rcutoff = getRcutoff();
+  // the bridge of every element pair (fillR_concat_, Ackland.cxx:377-457):
+  // ZBL at 1.0 A, the knot sum at 1.4 A
+  const int ids[ECOUNT] = {Cu_EL, Ag_EL, Au_EL};
+  for (int a = 0; a < ECOUNT; a++)
+    for (int b = 0; b < ECOUNT; b++) {
+      Atom A1, A2;
+      A1.ID = ids[a];
+      A2.ID = ids[b];
+      A1.setAttributesByElementID();
+      A2.setAttributesByElementID();
+      zbl_pair(A1.Z, A2.Z, x_[0], v_[a][b][0], d_[a][b][0]);
+      v_[a][b][1] = knots(a, b, x_[1], d_[a][b][1]);
+    }
+}
+
+int Ackland::e2i(int id) const {  // Ackland.hpp:75-84
+  switch (id) {
+    case Cu_EL: return Cu;
+    case Ag_EL: return Ag;
+    case Au_EL: return Au;
+    default: throw OracleError("e2i() : unknown element");
+  }
+}
+
+// sum_k ak (rk - r)^3 over the knots beyond r: 6 knots for like pairs, 3 for
+// unlike (Ackland.cxx:107-128)
+double Ackland::knots(int a, int b, double r, double& der) const {
+  int nk = (a == b) ? 6 : 3;
+  double val = 0.0;
+  der = 0.0;
+  for (int k = 1; k <= nk; k++) {
+    double rt = rk_[a][b][k] - r;
+    if (rt <= 0) continue;
+    val += ak_[a][b][k] * rt * rt * rt;
+    der += ak_[a][b][k] * 3.0 * rt * rt * (-1.0);
+  }
+  return val;
+}
+
+double Ackland::Phi(Pair& ij) {  // Ackland.cxx:59-129
+  double r = ij.r();
+  int a = e2i(ij.a1->ID), b = e2i(ij.a2->ID);
+  if (r >= rk_[a][b][1]) return 0.0;
+  double v, d;
+  if (r < x_[0]) {
+    zbl_pair(ij.a1->Z, ij.a2->Z, r, v, d);
+  } else if (r < x_[1]) {
+    cubic_bridge(x_, v_[a][b], d_[a][b], r, v, d);
+  } else {
+    v = knots(a, b, r, d);
+  }
+  ij.r(d);
+  return v;
+}
+
+double Ackland::PhiCap(int a, int b, double r) const {  // Ackland.cxx:131-148
+  if (r >= Rk_[a][b][1]) return 0.0;
+  double val = 0.0;
+  for (int k = 1; k <= 2; k++) {
+    double rt = Rk_[a][b][k] - r;
+    if (rt <= 0) continue;
+    val += Ak_[a][b][k] * rt * rt * rt;
+  }
+  return val;
+}
+
+double Ackland::dPhiCap(int a, int b, double r) const {  // Ackland.cxx:150-166
+  if (r >= Rk_[a][b][1]) return 0.0;
+  double val = 0.0;
+  for (int k = 1; k <= 2; k++) {
+    double rt = Rk_[a][b][k] - r;
+    if (rt <= 0) continue;
+    val += Ak_[a][b][k] * 3.0 * rt * rt * (-1.0);
+  }
+  return val;
+}
+
+double Ackland::g(Pair& ij, double V) {  // Ackland.cxx:168-208
+  double r = ij.r();
+  int a = e2i(ij.a1->ID), b = e2i(ij.a2->ID);
+  if (a != b) {
+    double p1 = PhiCap(a, a, r), p2 = PhiCap(b, b, r);
+    double s = std::sqrt(p1 * p2);
+    ORC_REQUIRE(p1 * p2 >= 0);
+    if (r < Rk_[a][a][1] && r < Rk_[b][b][1])
+      ij.r(0.5 / s * (dPhiCap(a, a, r) * p2 + p1 * dPhiCap(b, b, r)) * V);
+    return s;
+  }
+  if (r < Rk_[a][b][1]) ij.r(dPhiCap(a, b, r) * V);
+  return PhiCap(a, b, r);
+}
+
+double Ackland::rho(Atom& ai, AtomsArray& atoms, double V) {  // Ackland.cxx:223-234
+  double s = 0.0;
+  for (int j : NL(ai)) {
+    Pair ij(ai, atoms[j], 10.0 * Ao, 20.0 * Ao);
+    s += g(ij, V);
+  }
+  return s;
+}
+
+double Ackland::F(Atom& ai, AtomsArray& atoms) {  // Ackland.cxx:210-221
+  double r = rho(ai, atoms);
+  ORC_REQUIRE(r >= 0.0);
+  double Fv = -std::sqrt(r);
+  if (r != 0) rho(ai, atoms, 0.5 / Fv);
+  return Fv;
+}
+
+double Ackland::compute(AtomsArray& atoms) {  // Ackland.cxx:31-57
+  double Ei = 0;
+  for (auto& a : atoms) {
+    if (!isHandled(a)) continue;
+    Ei += F(a, atoms);
+    for (int j : NL(a)) {
+      Atom& aj = atoms[j];
+      if (a.globalIndex > aj.globalIndex) continue;
+      if (&a == &aj) continue;
+      Pair ij(a, aj, 10.0 * Ao, 20.0 * Ao);
+      Ei += Phi(ij);
+    }
+  }
+  return Ei;
+}
+
 // =================================================================== SimLoop
 void SimLoop::prepare() {  // AtomsContainer.cxx:106-123, SimLoop.cxx:236-250
   ORC_REQUIRE(!atoms.empty());

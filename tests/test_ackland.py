@@ -1,0 +1,156 @@
+"""Ackland EAM for Cu, Ag, Au (Ackland.cxx) of the oracle against what the
+physics fixes (CPU).
+
+E = sum_i -sqrt(rho_i) + sum_{i<j} V(r_ij), rho_i = sum_j phi(r_ij).  The
+pins do not retype the parameter tables:
+* the equilateral triangle and the dimer of one element separate phi and V
+  at any r (dimer: -2 sqrt(phi) + V, triangle: -3 sqrt(2 phi) + 3 V), so
+  V below 1.0 A must be the Ziegler-Biersack-Littmark universal potential
+  (a0 = 0.53 A, e = 4.8e-10 esu, the reference's constants), V and phi must
+  vanish at each element's own range, and V must be C1 across the 1.0-1.4 A
+  bridge;
+* for an unlike pair the linear trimer A-B-A (2r beyond every range) gives
+  the pair's phi, which must be the geometric mean of the like-pair phis;
+* the perfect fcc crystal at the tabulated lattice constant is at zero
+  pressure, and its C11 and C12 (from volume and tetragonal strains of a
+  periodic block) are the room-temperature values the 1987 fit reproduces
+  (Cu 168.4/121.4, Ag 124.0/93.4, Au 186.0/157.0 GPa, to 1%);
+* forces are -dE/dx (central differences on a disordered three-metal
+  cluster), and H/C atoms take no part.
+"""
+import numpy as np
+import pytest
+
+from mdtk_amd import systems as S
+from oracle.oracle import Oracle, POT_ACKLAND
+
+EV = S.EV
+ANG = S.ANG
+E_ESU = 4.8e-10
+ZNUM = {64: 29, 108: 47, 197: 79}
+LAT = {64: 3.615, 108: 4.086, 197: 4.078}
+RANGE_PHI = {64: 1.225 * 3.615, 108: 1.2247449 * 4.086, 197: 1.1180065 * 4.078}  # density range, A
+RANGE_V = {64: 1.225 * 3.615, 108: 1.2247449 * 4.086, 197: 1.2247449 * 4.078}    # pair-term range, A
+RANGE_V_UNLIKE = {(108, 197): 4.40856, (64, 197): 4.309816, (64, 108): 4.15800}
+
+
+def zbl(z1, z2, r_cm):
+    a = 0.88534 * 0.53e-8 / (z1 ** 0.23 + z2 ** 0.23)
+    x = r_cm / a
+    phi = 0.18175 * np.exp(-3.1998 * x) + 0.50986 * np.exp(-0.94229 * x) + 0.28022 * np.exp(-0.4029 * x) \
+        + 0.02817 * np.exp(-0.20162 * x)
+    return (z1 * E_ESU) * (z2 * E_ESU) / r_cm * phi
+
+
+def energy(elem, pos_a, pbc=None):
+    pbc = np.array([S.NO_PBC] * 3) if pbc is None else pbc
+    s = S.System(np.asarray(elem, np.int32), np.asarray(pos_a, float) + 20.0 * (pbc == S.NO_PBC), pbc, name="x")
+    o = Oracle.from_system(s, pots=POT_ACKLAND)
+    e, g = o.energy_grad()
+    return float(e.sum()), g
+
+
+def like_terms(el, r):
+    """(phi, V) of one element at distance r (A), from the dimer and the
+    equilateral triangle."""
+    e2 = energy([el, el], [[0, 0, 0], [r, 0, 0]])[0]
+    e3 = energy([el] * 3, [[0, 0, 0], [r, 0, 0], [r / 2, r * np.sqrt(3) / 2, 0]])[0]
+    s = (e3 - 3 * e2) / (6 - 3 * np.sqrt(2))  # sqrt(phi)
+    return s * s, e2 + 2 * s
+
+
+@pytest.mark.parametrize("el", [64, 108, 197])
+def test_short_range_is_zbl_and_ranges_are_the_elements_own(el):
+    z = ZNUM[el]
+    for r in (0.3, 0.6, 0.95):
+        phi, v = like_terms(el, r)
+        assert abs(v / zbl(z, z, r * ANG) - 1) < 1e-10
+        assert phi > 0
+    rp, rv = RANGE_PHI[el], RANGE_V[el]
+    s_in, s_out = np.sqrt(like_terms(el, rp - 0.02)[0]), np.sqrt(like_terms(el, rp + 1e-6)[0])
+    assert s_in > 0 and s_out <= 1e-9 * s_in  # the triangle's rounding is all that is left beyond
+    inside = energy([el, el], [[0, 0, 0], [rv - 0.01, 0, 0]])[0] + 2 * np.sqrt(like_terms(el, rv - 0.01)[0])
+    assert inside != 0.0
+    e_out = energy([el, el], [[0, 0, 0], [max(rp, rv) + 1e-6, 0, 0]])[0]
+    assert e_out == 0.0
+
+
+@pytest.mark.parametrize("el", [64, 197])
+def test_pair_term_is_c1_across_the_bridge(el):
+    h = 1e-6
+    for x in (1.0, 1.4):
+        vm = [like_terms(el, x - k * h)[1] for k in (2, 1)]
+        vp = [like_terms(el, x + k * h)[1] for k in (1, 2)]
+        assert abs(vp[0] / vm[1] - 1) < 1e-5
+        slope_in, slope_out = (vm[1] - vm[0]) / h, (vp[1] - vp[0]) / h
+        assert abs(slope_in / slope_out - 1) < 1e-3
+
+
+@pytest.mark.parametrize("a,b", [(64, 197), (108, 197), (64, 108)])
+def test_unlike_density_is_the_geometric_mean(a, b):
+    for r in (2.6, 3.0, 3.6, 4.2):
+        e2 = energy([a, b], [[0, 0, 0], [r, 0, 0]])[0]
+        e3 = energy([a, b, a], [[0, 0, 0], [r, 0, 0], [2 * r, 0, 0]])[0]
+        # dimer: -2 q + V; trimer: -2 q - sqrt(2) q + 2 V, q = sqrt(phi_ab)
+        q = (2 * e2 - e3) / (np.sqrt(2) - 2)
+        want = np.sqrt(np.sqrt(like_terms(a, r)[0] * like_terms(b, r)[0]))
+        assert abs(q - want) <= 1e-9 * want + 1e-30
+    rv = RANGE_V_UNLIKE[(a, b)]
+    e_in = energy([a, b], [[0, 0, 0], [rv - 1e-3, 0, 0]])[0]
+    q_in = np.sqrt(np.sqrt(like_terms(a, rv - 1e-3)[0] * like_terms(b, rv - 1e-3)[0]))
+    assert e_in + 2 * q_in != 0.0  # V_ab still nonzero just inside its range
+    e_out = energy([a, b], [[0, 0, 0], [rv + 1e-6, 0, 0]])[0]
+    q_out = np.sqrt(np.sqrt(like_terms(a, rv + 1e-6)[0] * like_terms(b, rv + 1e-6)[0]))
+    assert abs(e_out + 2 * q_out) <= 1e-12 * abs(e_out) + 1e-30  # and zero beyond
+
+
+def _fcc(n=4):
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    cells = np.array([[i, j, k] for i in range(n) for j in range(n) for k in range(n)], float)
+    return (cells[:, None, :] + base[None, :, :]).reshape(-1, 3)
+
+
+def _crystal_energy(el, a, strain=(1.0, 1.0, 1.0), n=5):
+    st = np.array(strain)
+    p = _fcc(n) * a * st + 1e-3
+    e, _ = energy(np.full(len(p), el), p, pbc=n * a * st)
+    return e / len(p)
+
+
+@pytest.mark.parametrize("el,c11,c12", [(64, 168.4, 121.4), (108, 124.0, 93.4), (197, 186.0, 157.0)])
+def test_fcc_crystal_zero_pressure_and_elastic_constants(el, c11, c12):
+    a = LAT[el]
+    acm = a * ANG
+    omega = acm ** 3 / 4
+    h = 1e-4
+    e0, ep, em = (_crystal_energy(el, a * f) for f in (1.0, 1 + h, 1 - h))
+    dEda = (ep - em) / (2 * h * acm)
+    d2 = (ep - 2 * e0 + em) / (h * acm) ** 2
+    bulk = 4 * d2 / (9 * acm) * 1e-10  # GPa
+    pressure = -dEda / (3 * acm ** 2 / 4) * 1e-10
+    assert abs(pressure) < 1e-5 * bulk
+    eps = 1e-3
+    cdiff = (_crystal_energy(el, a, (1 + eps, 1 - eps, 1)) - e0) / (omega * eps ** 2) * 1e-10  # C11 - C12
+    assert abs((bulk + 2 * cdiff / 3) / c11 - 1) < 0.01
+    assert abs((bulk - cdiff / 3) / c12 - 1) < 0.01
+
+
+def test_forces_are_gradients_and_hydrocarbons_take_no_part():
+    rng = np.random.default_rng(3)
+    p = _fcc(2) * 3.9 + rng.normal(scale=0.15, size=(32, 3))
+    el = rng.choice([64, 108, 197], 32)
+    e0, g = energy(el, p)
+    h = 1e-5
+    for i in (0, 7, 19, 31):
+        for d in range(3):
+            q = p.copy()
+            q[i, d] += h
+            ep = energy(el, q)[0]
+            q[i, d] -= 2 * h
+            em = energy(el, q)[0]
+            fd = (ep - em) / (2 * h * ANG)
+            assert abs(g[i, d] - fd) <= 1e-6 * np.abs(g).max()
+    # a C and an H atom among the metal atoms change nothing
+    q = np.vstack([p, p[5] + [0.0, 0.0, 1.6], p[9] + [1.2, 0.0, 0.0]])
+    e1, g1 = energy(np.concatenate([el, [12, 1]]), q)
+    assert e1 == e0 and np.array_equal(g1[:32], g) and not g1[32:].any()
