@@ -23,15 +23,16 @@ extern "C" {
 typedef struct mdg_ctx mdg_ctx;
 
 /* Potential chain members, in the order applications/common.h:46-68 adds
- * them: FBZL (ZBL ion-target), Brenner (replaces REBO), REBO, AIREBO-LJ
- * (requires REBO), ETors. */
+ * them: FBZL (ZBL ion-target), FLJ, Brenner (replaces REBO), REBO,
+ * AIREBO-LJ (requires REBO), ETors, Ackland. */
 enum {
   MDG_POT_FBZL = 1,
   MDG_POT_BRENNER = 2,
   MDG_POT_REBO = 4,
   MDG_POT_AIREBO = 8,
   MDG_POT_ETORS = 16,
-  MDG_POT_FLJ = 32      /* FLJ (pairwise/FLJ.cxx): {H,C} x {Cu,Au}, chained after FBZL (common.h:49) */
+  MDG_POT_FLJ = 32,     /* FLJ (pairwise/FLJ.cxx): {H,C} x {Cu,Au}, chained after FBZL (common.h:49) */
+  MDG_POT_ACKLAND = 64  /* Ackland EAM (manybody/Ackland/Ackland.cxx): Cu, Ag, Au; last in the chain (common.h:67); FP64 in both precisions */
 };
 
 enum { MDG_PREC_FP64 = 0, MDG_PREC_MIXED = 1 };

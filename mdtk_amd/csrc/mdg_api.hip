@@ -40,8 +40,10 @@ struct mdg_ctx {
   mdg_config cfg{};
   std::string err;
   std::vector<int> kinds;
-  int slot_fbzl = -1, slot_flj = -1, slot_rebo = -1, slot_bren = -1, slot_lj = -1, slot_tors = -1;
+  int slot_fbzl = -1, slot_flj = -1, slot_rebo = -1, slot_bren = -1, slot_lj = -1, slot_tors = -1, slot_eam = -1;
   FLJTable* dFLJ = nullptr;  // FLJ parameters (device)
+  EAMTable* dEAM = nullptr;  // Ackland parameters (device)
+  double* eam_fp = nullptr;  // per atom: F'(rho) of the Ackland density pass
   bool have_short = false, have_long = false;
   double rc_short = 0, rc_long = 0;
 
@@ -125,6 +127,7 @@ struct mdg_ctx {
     if (dT) hipFree(dT);
     if (dTf) hipFree(dTf);
     if (dFLJ) hipFree(dFLJ);
+    if (dEAM) hipFree(dEAM);
     if (own) hipStreamDestroy(own);
     if (side) hipStreamDestroy(side);
     if (ev_fork) hipEventDestroy(ev_fork);
@@ -558,6 +561,15 @@ void force_call(mdg_ctx* c) {
       k_flj<<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->dFLJ, A.grad, c->eatom + (size_t)c->slot_flj * n);
     });
   }
+  if (c->slot_eam >= 0) {
+    double* ea = c->eatom + (size_t)c->slot_eam * n;
+    launch(c, "k_eam_density", [&] {
+      k_eam_density<<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dEAM, c->eam_fp, ea);
+    });
+    launch(c, "k_eam_force", [&] {
+      k_eam_force<<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dEAM, c->eam_fp, A.grad, ea);
+    });
+  }
   if (c->slot_fbzl >= 0) {
     launch(c, "k_fbzl", [&] {
       k_fbzl<<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->cfg.fbzl_r0, c->cfg.fbzl_r1, A.grad,
@@ -587,7 +599,7 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
     g_create_err = "Brenner replaces REBO: select one of them";
     return 1;
   }
-  if (!(m & (MDG_POT_REBO | MDG_POT_BRENNER | MDG_POT_ETORS | MDG_POT_FBZL | MDG_POT_FLJ))) {
+  if (!(m & (MDG_POT_REBO | MDG_POT_BRENNER | MDG_POT_ETORS | MDG_POT_FBZL | MDG_POT_FLJ | MDG_POT_ACKLAND))) {
     g_create_err = "no potentials selected";
     return 1;
   }
@@ -625,13 +637,18 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
   if (m & MDG_POT_REBO) { c->slot_rebo = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_REBO); }
   if (m & MDG_POT_AIREBO) { c->slot_lj = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_AIREBO); }
   if (m & MDG_POT_ETORS) { c->slot_tors = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_ETORS); }
+  if (m & MDG_POT_ACKLAND) { c->slot_eam = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_ACKLAND); }
   c->have_short = (m & (MDG_POT_REBO | MDG_POT_BRENNER | MDG_POT_ETORS)) != 0;
-  c->have_long = (m & (MDG_POT_AIREBO | MDG_POT_FBZL | MDG_POT_FLJ)) != 0;
+  c->have_long = (m & (MDG_POT_AIREBO | MDG_POT_FBZL | MDG_POT_FLJ | MDG_POT_ACKLAND)) != 0;
   c->rc_short = 2.0 * kAo;  // REBO/Brenner/ETors getRcutoff (REBO.hpp:130)
   c->rc_long = 0;
   if (m & MDG_POT_AIREBO) c->rc_long = 5.5 * kAo;  // AIREBO_LJ.hpp:81
   if (m & MDG_POT_FBZL) c->rc_long = std::max(c->rc_long, c->cfg.fbzl_r1);
   if (m & MDG_POT_FLJ) c->rc_long = std::max(c->rc_long, 5.5 * kAo);  // common.h:49
+  EAMTable et;
+  build_eam_table(et);
+  // Ackland::getRcutoff (Ackland.hpp:58): the largest like-pair first knot
+  if (m & MDG_POT_ACKLAND) c->rc_long = std::max(c->rc_long, std::max(et.rk[0][0][1], std::max(et.rk[1][1][1], et.rk[2][2][1])));
   if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
     g_create_err = "hipStreamCreate failed";
     delete c;
@@ -674,6 +691,12 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
       delete c;
       return 1;
     }
+  }
+  if (c->slot_eam >= 0 && (hipMalloc(&c->dEAM, sizeof(et)) != hipSuccess ||
+                           hipMemcpy(c->dEAM, &et, sizeof(et), hipMemcpyHostToDevice) != hipSuccess)) {
+    g_create_err = "table upload failed";
+    delete c;
+    return 1;
   }
   if (dalloc(c, c->flags, 3) || dalloc(c, c->counters, 8) || dalloc(c, c->tally, kTallySlots * kTallyStride) || dalloc(c, c->derr, 1) || dalloc(c, c->nbonds, 1) ||
       dalloc(c, c->qcount, kQueues * kQStride)) {
@@ -820,6 +843,8 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   if (c->slot_lj >= 0)
     r |= dalloc(c, A.pl, (size_t)n * A.cap_p) | dalloc(c, A.pl_cnt, n) | dalloc(c, A.disp_p, 3 * (size_t)n);
   r |= dalloc(c, c->handled_s, n) | dalloc(c, c->handled_l, n);
+  c->eam_fp = nullptr;
+  if (c->slot_eam >= 0) r |= dalloc(c, c->eam_fp, n);
   if (r) return 1;
   // cell grids, one per system and list
   auto grids = [&](double rc, CellGrid*& gd, int*& coff, int& ncell) -> int {
@@ -1077,6 +1102,7 @@ int mdg_nl_export(mdg_ctx* c, int term, int32_t* counts, int32_t* idx, int64_t c
   HIPCHK(c, hipMemcpy(el.data(), c->A.elem, n * 4, hipMemcpyDeviceToHost));
   auto zbl_handled = [](int e) { return e == 1 || e == 12 || e == 64 || e == 197 || e == 108 || e == 40 || e == 131; };
   auto flj_handled = [](int e) { return e == 1 || e == 12 || e == 64 || e == 197; };  // FLJ.cxx:38-52
+  auto eam_handled = [](int e) { return e == 64 || e == 108 || e == 197; };             // Ackland.cxx:236-262
   int64_t pos = 0;
   for (int i = 0; i < n; i++) {
     int m = 0;
@@ -1084,12 +1110,14 @@ int mdg_nl_export(mdg_ctx* c, int term, int32_t* counts, int32_t* idx, int64_t c
     if (kind == MDG_POT_AIREBO) row_ok = ty[i] <= T_H;
     if (kind == MDG_POT_FBZL) row_ok = zbl_handled(el[i]);
     if (kind == MDG_POT_FLJ) row_ok = flj_handled(el[i]);
+    if (kind == MDG_POT_ACKLAND) row_ok = eam_handled(el[i]);
     if (row_ok)
       for (int q = 0; q < cnt[i]; q++) {
         int j = lst[(size_t)i * stride + q];
         if (kind == MDG_POT_AIREBO && ty[j] > T_H) continue;
         if (kind == MDG_POT_FBZL && !zbl_handled(el[j])) continue;
         if (kind == MDG_POT_FLJ && !flj_handled(el[j])) continue;
+        if (kind == MDG_POT_ACKLAND && !eam_handled(el[j])) continue;
         if (pos >= cap) {
           c->err = "nl_export capacity";
           return 1;
@@ -1275,7 +1303,8 @@ int mdg_neighbor_counts(mdg_ctx* c, int32_t* counts) {
   HIPCHK(c, hipMemcpy(ty.data(), c->A.type, n * 4, hipMemcpyDeviceToHost));
   for (int i = 0; i < n; i++) {
     bool long_handled = (c->slot_fbzl >= 0) || (c->slot_lj >= 0 && ty[i] <= T_H) ||
-                        (c->slot_flj >= 0 && ty[i] != T_ION);  // FLJ handles H, C, Cu, Au
+                        (c->slot_flj >= 0 && ty[i] != T_ION) ||  // FLJ handles H, C, Cu, Au
+                        (c->slot_eam >= 0 && ty[i] == T_OTHER);  // Ackland handles Cu, Ag, Au
     counts[i] = s[i] + (long_handled ? l[i] : 0);
   }
   return 0;

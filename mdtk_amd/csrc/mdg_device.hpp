@@ -97,6 +97,21 @@ struct FLJTable {
 };
 void build_flj_table(FLJTable& t);
 
+// Ackland EAM (manybody/Ackland/Ackland.{hpp,cxx}) for Cu, Ag, Au, indexed
+// Cu = 0, Ag = 1, Au = 2.  Pair term V: knots rk/ak (k = 1..6 like, 1..3
+// unlike; V = 0 from rk[1] on), ZBL below x[0] with the screening length
+// AS, the cubic through both ends' values and slopes on [x[0], x[1]).
+// Density phi_e: knots Rk/Ak (k = 1, 2) of the element; unlike pairs take
+// the geometric mean of the two elements' phi.
+struct EAMTable {
+  double rk[3][3][7], ak[3][3][7];
+  double Rk[3][2], Ak[3][2];
+  double AS[3][3];
+  double x[2], v[3][3][2], d[3][3][2];
+  double rmax;  // no pair term or density beyond this distance
+};
+void build_eam_table(EAMTable& t);
+
 using Tables = TablesT<double>;
 using ReboParams = ReboParamsT<double>;
 using LJParams = LJParamsT<double>;

@@ -514,6 +514,166 @@ __global__ void k_flj(Atoms A, Box bx, const FLJTable* __restrict__ T, double* _
   }
 }
 
+// ------------------------------------------------------------ Ackland EAM
+// Ackland::operator() (Ackland.cxx:31-57): E = sum_i F(rho_i) + sum_{i<j} V,
+// F = -sqrt(rho), rho_i = sum_j g(r_ij).  Two passes over the long list with
+// 16 lanes per metal atom: the density pass stores F'(rho_i) (0 where rho is
+// 0, as F() applies no gradient there, :210-221) and F_i; the force pass
+// gathers, per ordered pair, dV/dr + (F'_i + F'_j) dg/dr along x_i - x_j into
+// atom i alone (no scatter to j), and V of the pairs j > i into i's energy.
+__device__ __forceinline__ int eam_index(int elem) {
+  switch (elem) {
+    case 64: return 0;   // Cu
+    case 108: return 1;  // Ag
+    case 197: return 2;  // Au
+    default: return -1;
+  }
+}
+
+// The element's density knot sum phi_e(r) and its slope (PhiCap/dPhiCap,
+// Ackland.cxx:131-166).
+__device__ __forceinline__ void eam_phi(const EAMTable* __restrict__ T, int e, double r, double& p, double& dp) {
+  p = 0.0;
+  dp = 0.0;
+  if (r >= T->Rk[e][0]) return;
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const double rt = T->Rk[e][k] - r;
+    if (rt <= 0) continue;
+    p += T->Ak[e][k] * rt * rt * rt;
+    dp += T->Ak[e][k] * 3.0 * rt * rt * (-1.0);
+  }
+}
+
+// g of the pair and dg/dr (Ackland.cxx:168-208): the like phi, or the
+// geometric mean of both like phi, whose slope is applied only inside both
+// elements' ranges.
+__device__ __forceinline__ void eam_g(const EAMTable* __restrict__ T, int a, int b, double r, double& g, double& dg) {
+  if (a == b) {
+    eam_phi(T, a, r, g, dg);
+    return;
+  }
+  double p1, d1, p2, d2;
+  eam_phi(T, a, r, p1, d1);
+  eam_phi(T, b, r, p2, d2);
+  g = sqrt(p1 * p2);
+  dg = (r < T->Rk[a][0] && r < T->Rk[b][0]) ? 0.5 / g * (d1 * p2 + p1 * d2) : 0.0;
+}
+
+// V of the pair and dV/dr (Phi, Ackland.cxx:59-129).
+__device__ __forceinline__ void eam_pair(const EAMTable* __restrict__ T, int a, int b, double ZA, double ZB, double r,
+                                         double& v, double& dv) {
+  v = 0.0;
+  dv = 0.0;
+  if (r >= T->rk[a][b][1]) return;
+  if (r < T->x[0]) {
+    const double AS = T->AS[a][b], Y = r / AS;
+    const double e1 = exp(-3.1998 * Y), e2 = exp(-0.94229 * Y), e3 = exp(-0.4029 * Y), e4 = exp(-0.20162 * Y);
+    const double sum = 0.18175 * e1 + 0.50986 * e2 + 0.28022 * e3 + 0.02817 * e4;
+    v = ZA * ZB / r * sum;
+    dv = -ZA * ZB / (r * r) * sum -
+         ZA * ZB / (r * AS) * (0.18175 * 3.1998 * e1 + 0.50986 * 0.94229 * e2 + 0.28022 * 0.4029 * e3 + 0.02817 * 0.20162 * e4);
+  } else if (r < T->x[1]) {  // the cubic through both ends' values and slopes (Spline.cxx:72-98)
+    const double* vv = T->v[a][b];
+    const double* dd = T->d[a][b];
+    const double hh = T->x[1] - T->x[0], t = (r - T->x[0]) / hh;
+    v = (1 + 2 * t) * (1 - t) * (1 - t) * vv[0] + t * (1 - t) * (1 - t) * hh * dd[0] + t * t * (3 - 2 * t) * vv[1] +
+        t * t * (t - 1) * hh * dd[1];
+    dv = (6 * t * (t - 1) * vv[0] - 6 * t * (t - 1) * vv[1]) / hh + (1 - t) * (1 - 3 * t) * dd[0] + t * (3 * t - 2) * dd[1];
+  } else {
+    const int nk = (a == b) ? 6 : 3;
+    for (int k = 1; k <= nk; k++) {
+      const double rt = T->rk[a][b][k] - r;
+      if (rt <= 0) continue;
+      v += T->ak[a][b][k] * rt * rt * rt;
+      dv += T->ak[a][b][k] * 3.0 * rt * rt * (-1.0);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_eam_density(Atoms A, Box bx, const EAMTable* __restrict__ T,
+                                                     double* __restrict__ fp, double* __restrict__ eatom) {
+  constexpr int LW = 16;
+  const int sl = threadIdx.x % LW;
+  const int i = blockIdx.x * (blockDim.x / LW) + threadIdx.x / LW;
+  if (i >= A.n) return;  // group-uniform
+  const int ei = eam_index(A.elem[i]);
+  if (ei < 0) {
+    if (sl == 0) fp[i] = 0.0;
+    return;
+  }
+  const int cnt = A.nll_cnt[i];
+  const int* L = A.nll + (size_t)i * A.cap_l;
+  const double4 pi = A.p[i];
+  const double* Lb = box_of(bx, i);
+  const double rmax = T->rmax;
+  double rho = 0.0;
+  for (int q = sl; q < cnt; q += LW) {
+    const int j = L[q];
+    const int ej = eam_index(A.elem[j]);
+    if (ej < 0) continue;
+    const double4 pj = A.p[j];
+    const double dx = mic(pi.x - pj.x, Lb[0]), dy = mic(pi.y - pj.y, Lb[1]), dz = mic(pi.z - pj.z, Lb[2]);
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    if (r >= rmax) continue;
+    double g, dg;
+    eam_g(T, ei, ej, r, g, dg);
+    rho += g;
+  }
+#pragma unroll
+  for (int o = LW / 2; o > 0; o >>= 1) rho += __shfl_xor(rho, o, LW);
+  if (sl == 0) {
+    const double F = -sqrt(rho);
+    fp[i] = (rho != 0.0) ? 0.5 / F : 0.0;
+    eatom[i] = F;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_eam_force(Atoms A, Box bx, const EAMTable* __restrict__ T,
+                                                   const double* __restrict__ fp, double* __restrict__ grad,
+                                                   double* __restrict__ eatom) {
+  constexpr int LW = 16;
+  const int sl = threadIdx.x % LW;
+  const int i = blockIdx.x * (blockDim.x / LW) + threadIdx.x / LW;
+  if (i >= A.n) return;  // group-uniform
+  const int ei = eam_index(A.elem[i]);
+  if (ei < 0) return;
+  const int cnt = A.nll_cnt[i];
+  const int* L = A.nll + (size_t)i * A.cap_l;
+  const double4 pi = A.p[i];
+  const double* Lb = box_of(bx, i);
+  const double rmax = T->rmax, fpi = fp[i], Zi = A.Z[i];
+  double gx = 0.0, gy = 0.0, gz = 0.0, ev = 0.0;
+  for (int q = sl; q < cnt; q += LW) {
+    const int j = L[q];
+    const int ej = eam_index(A.elem[j]);
+    if (ej < 0) continue;
+    const double4 pj = A.p[j];
+    const double dx = mic(pi.x - pj.x, Lb[0]), dy = mic(pi.y - pj.y, Lb[1]), dz = mic(pi.z - pj.z, Lb[2]);
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    if (r >= rmax) continue;
+    double v, dv, g, dg;
+    eam_pair(T, ei, ej, Zi, A.Z[j], r, v, dv);
+    eam_g(T, ei, ej, r, g, dg);
+    const double c = (dv + (fpi + fp[j]) * dg) / r;
+    gx += c * dx;
+    gy += c * dy;
+    gz += c * dz;
+    if (j > i) ev += v;
+  }
+#pragma unroll
+  for (int o = LW / 2; o > 0; o >>= 1) {
+    gx += __shfl_xor(gx, o, LW);
+    gy += __shfl_xor(gy, o, LW);
+    gz += __shfl_xor(gz, o, LW);
+    ev += __shfl_xor(ev, o, LW);
+  }
+  if (sl == 0) {
+    atomic_add3(grad, i, D3{gx, gy, gz});
+    eatom[i] += ev;
+  }
+}
+
 // ------------------------------------------------------------ Verlet
 // eq.1 of SimLoop::executeMain (SimLoop.cxx:323-332): x += v dt + a dt^2/2,
 // accumulated into both neighbour lists' (and the LJ pair rows') displacements.

@@ -122,6 +122,10 @@ __global__ void k_etors(Atoms A, const TablesT<R>* T, Box bx, const int2* bonds,
                         double* grad, double* eatom, long long* nquads);
 __global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* grad, double* eatom);
 __global__ void k_flj(Atoms A, Box bx, const FLJTable* T, double* grad, double* eatom);
+// Ackland EAM: the density pass (F'(rho) into fp, F into eatom), then the
+// force pass (gradient and pair energies); 16 lanes per atom, blocks of 256.
+__global__ void k_eam_density(Atoms A, Box bx, const EAMTable* T, double* fp, double* eatom);
+__global__ void k_eam_force(Atoms A, Box bx, const EAMTable* T, const double* fp, double* grad, double* eatom);
 __global__ void k_brenner_bonds(Atoms A, const Tables* T, Box bx, const int2* bonds, const int* nbonds,
                                 double* grad, double* eatom, unsigned* err);
 __global__ void k_drift(Atoms A, const double* dtp);

@@ -12,6 +12,7 @@
 // The interpolants are the unique Hermite polynomials those nodes define
 // (value and first derivatives, zero cross derivatives); the kernels evaluate
 // them in cell-local coordinates.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -109,6 +110,78 @@ void build_flj_table(FLJTable& t) {
       double s12 = s6 * s6;
       t.v[a][b][1] = 4.0 * t.eps[a][b] * (s12 - s6);
       t.d[a][b][1] = 4.0 * t.eps[a][b] * (-12.0 * s12 / r + 6.0 * s6 / r);
+    }
+}
+
+// Ackland, Tichy, Vitek, Finnis (Phil. Mag. A 56 (1987) 735) and the unlike
+// pairs of Ackland.cxx:282-299, as setupPotential sets them (Ackland.cxx:
+// 278-373); the bridge ends as fillR_concat_ computes them (:377-457).
+void build_eam_table(EAMTable& t) {
+  std::memset(&t, 0, sizeof(t));
+  const double Ao = kAo, eV = kEV;
+  enum { Cu = 0, Ag = 1, Au = 2 };
+  // unlike pairs: knots in A, amplitudes in eV/A^3 (the third Ag ones are 0)
+  const struct { int a, b; double r[3], k[3]; } un[3] = {
+      {Ag, Au, {4.40856, 3.46970, 3.00000}, {0.003566503525, 1.015075326, 0.0}},
+      {Cu, Au, {4.309816, 4.047479, 3.297946}, {-0.08554551660, 0.192835880, 0.75932286}},
+      {Cu, Ag, {4.15800, 3.15700, 3.00000}, {0.03533037752, 0.8518466353, 0.0}},
+  };
+  for (const auto& u : un)
+    for (int k = 1; k <= 3; k++) {
+      t.rk[u.a][u.b][k] = t.rk[u.b][u.a][k] = u.r[k - 1] * Ao;
+      t.ak[u.a][u.b][k] = t.ak[u.b][u.a][k] = u.k[k - 1] * eV / (Ao * Ao * Ao);
+    }
+  // like pairs: lattice constant a; V knots in a, amplitudes in eV/a^3;
+  // density knots in a, amplitudes in eV^2/a^3
+  const struct { int e; double lat, k[6], r[6], K[2], R[2]; } lk[3] = {
+      {Cu, 3.615, {61.73525861, -108.18467800, 57.00053948, -12.88796578, 39.16381901, 0.0},
+       {1.225, 1.202, 1.154, 1.050, 0.866, 0.707}, {10.03718305, 17.06363299}, {1.225, 0.990}},
+      {Ag, 4.086, {20.368404, -102.36075, 94.31277, -6.220051, 31.080889, 175.56047},
+       {1.2247449, 1.1547054, 1.1180065, 1.0000000, 0.8660254, 0.7071068}, {1.458761, 42.946555},
+       {1.2247449, 1.0000000}},
+      {Au, 4.078, {29.059066, -153.14779, 148.17881, -22.20508, 72.71465, 199.26269},
+       {1.2247449, 1.1547054, 1.1180065, 1.0000000, 0.8660254, 0.7071068}, {21.930125, 284.99631},
+       {1.1180065, 0.8660254}},
+  };
+  for (const auto& l : lk) {
+    const double a = l.lat * Ao, a3 = a * a * a;
+    for (int k = 1; k <= 6; k++) {
+      t.ak[l.e][l.e][k] = l.k[k - 1] * eV / a3;
+      t.rk[l.e][l.e][k] = l.r[k - 1] * a;
+    }
+    for (int k = 0; k < 2; k++) {
+      t.Ak[l.e][k] = l.K[k] * (eV * eV) / a3;
+      t.Rk[l.e][k] = l.R[k] * a;
+    }
+  }
+  t.x[0] = 1.0 * Ao;
+  t.x[1] = 1.4 * Ao;
+  const double Z[3] = {29.0 * kECharge, 47.0 * kECharge, 79.0 * kECharge};
+  t.rmax = 0;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      t.rmax = std::max(t.rmax, std::max(t.rk[a][b][1], t.Rk[a][0]));
+      t.AS[a][b] = 8.8534e-1 * 0.53e-8 / (std::pow(Z[a] / kECharge, 0.23) + std::pow(Z[b] / kECharge, 0.23));
+      // ZBL value and slope at x[0]
+      double r = t.x[0], Y = r / t.AS[a][b];
+      double e1 = std::exp(-3.1998 * Y), e2 = std::exp(-0.94229 * Y), e3 = std::exp(-0.4029 * Y),
+             e4 = std::exp(-0.20162 * Y);
+      double sum = 0.18175 * e1 + 0.50986 * e2 + 0.28022 * e3 + 0.02817 * e4;
+      t.v[a][b][0] = Z[a] * Z[b] / r * sum;
+      t.d[a][b][0] = -Z[a] * Z[b] / (r * r) * sum -
+                     Z[a] * Z[b] / (r * t.AS[a][b]) *
+                         (0.18175 * 3.1998 * e1 + 0.50986 * 0.94229 * e2 + 0.28022 * 0.4029 * e3 + 0.02817 * 0.20162 * e4);
+      // knot sum value and slope at x[1]
+      r = t.x[1];
+      double val = 0, der = 0;
+      for (int k = 1; k <= (a == b ? 6 : 3); k++) {
+        double rt = t.rk[a][b][k] - r;
+        if (rt <= 0) continue;
+        val += t.ak[a][b][k] * rt * rt * rt;
+        der += t.ak[a][b][k] * 3.0 * rt * rt * (-1.0);
+      }
+      t.v[a][b][1] = val;
+      t.d[a][b][1] = der;
     }
 }
 

@@ -187,6 +187,21 @@ def graphite(nx: int, ny: int, nz: int, a: float = 2.46, d: float = 3.35) -> Sys
     return s.wrapped()
 
 
+FCC_A = {64: 3.615, 108: 4.086, 197: 4.078}  # Cu, Ag, Au lattice constants, A (Ackland.cxx:304,327,350)
+
+
+def fcc(elem: int, nx: int, ny: int, nz: int, a: float | None = None, periodic=(True, True, True)) -> System:
+    """fcc crystal of one metal, cubic 4-atom cell, periodic along the flagged
+    axes (an open axis keeps the block's own extent)."""
+    a = FCC_A[elem] if a is None else a
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]]) * a
+    g = np.stack(np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij"), -1).reshape(-1, 3)
+    pos = (g[:, None, :] * a + base[None]).reshape(-1, 3) + 0.25 * a
+    pbc = np.array([n * a if per else NO_PBC for n, per in zip((nx, ny, nz), periodic)])
+    s = System(np.full(len(pos), elem, np.int32), pos, pbc, name=f"fcc{elem}-{nx}x{ny}x{nz}")
+    return s.wrapped()
+
+
 def ethane() -> System:
     """C2H6 (8 atoms, open boundaries): staggered ethane, C-C 1.54 A, C-H 1.09 A."""
     cc, ch = 1.54, 1.09
