@@ -39,6 +39,7 @@ POTS = ("FBZL", "REBO", "AIREBO", "ETors")
 KERNELS = ("k_bstar", "k_rebo_bonds", "k_lj_wave", "k_pair_rows", "k_etors", "k_coord", "k_bond_enum", "k_nl_build_long",
            "k_nl_build_short", "k_kick", "k_drift", "k_sys_disp", "k_sys_step_end", "k_zero", "k_fbzl",
            "k_cell_assign", "k_cell_scan", "k_cell_fill", "k_cell_clear")
+EAM_KERNELS = ("k_eam_density", "k_eam_force", "k_nl_build_long", "k_kick", "k_drift")
 
 
 def log(msg):
@@ -128,6 +129,50 @@ def roofline(kt, cnt, natoms, tune):
                    algorithmic_bytes=alg,
                    note="gather-latency bound (dependent list -> position -> bond-record loads); "
                         "traffic = bytes beyond L2 from rocprofv3 FETCH_SIZE/WRITE_SIZE")
+    return out
+
+
+def eam_algorithmic_bytes(entries, natoms):
+    """Compulsory traffic of the two Ackland passes: one 4-byte index per long
+    list entry in each pass; per atom, density pass: row count, element,
+    packed position (4 + 4 + 32 B) and F' and F written (16 B); force pass:
+    row count, element, packed position, F', charge (4 + 4 + 32 + 8 + 8 B),
+    gradient read + write (48 B), energy read + write (16 B)."""
+    dens = 4.0 * entries + natoms * (40 + 16)
+    force = 4.0 * entries + natoms * (56 + 48 + 16)
+    return dens, force
+
+
+def metal_block(steps, warmup):
+    """Ackland EAM alone on a periodic Cu crystal (40^3 cells, 256,000 atoms,
+    300 K, dt 1 fs): atom-steps/s and both passes against the HBM peak."""
+    import numpy as np
+    from mdtk_amd import Engine
+    from mdtk_amd import systems as S
+    cu = S.fcc(64, 40, 40, 40)
+    cu.vel = S.maxwell_velocities(cu, 300.0, stream=3)
+    e = Engine.from_system(cu, potentials=("Ackland",))
+    ms, res = timed(e, steps, warmup, 1e-15)
+    e.profile(True)
+    e.run(steps, fixed_dt=1e-15)
+    e.synchronize()
+    kt = {k: e.kernel_time(k) for k in EAM_KERNELS}
+    e.profile(False)
+    entries = float(e.counters()["long_entries"])
+    dens, force = eam_algorithmic_bytes(entries, cu.n)
+    rl = {}
+    for k, alg in (("k_eam_density", dens), ("k_eam_force", force)):
+        t_ms, nl = kt[k]
+        avg = t_ms / max(nl, 1)
+        rl[k] = {"bound": "hbm", "algorithmic_bytes": alg, "avg_launch_ms": avg,
+                 "achieved": alg / (avg / 1e3) / 1e9, "peak": 8000.0, "unit": "GB/s",
+                 "frac": alg / (avg / 1e3) / 1e9 / 8000.0}
+    et = (res["epot"] + res["ekin"]) / S.EV
+    out = {"workload": f"Cu fcc 40x40x40 ({cu.n} atoms), periodic, 300 K, Ackland EAM alone, dt 1 fs, FP64",
+           "value": cu.n * steps / (ms / 1e3), "unit": "atom-steps/s", "ms_per_step": ms / steps,
+           "long_entries_per_atom": entries / cu.n,
+           "kernels_ms_per_step": {k: v[0] / max(steps, 1) for k, v in kt.items()},
+           "roofline": rl, "max_abs_dE_tot_eV": float(np.abs(et - et[0]).max())}
     return out
 
 
@@ -302,6 +347,9 @@ def main():
             drift[prec] = {"slope_eV_per_atom_per_ps": slope, "max_abs_dE_eV_per_atom": dmax}
         out["nve_drift"] = {"system": f"PE 8x12x8 ({small.n} atoms), 300 K, 2000 steps of 0.2 fs", **drift}
         log(f"nve drift {drift}")
+        out["metal_ackland"] = metal_block(a.steps, a.warmup)
+        log(f"metal block (Ackland): {out['metal_ackland']['value']:.4e} atom-steps/s, "
+            f"{out['metal_ackland']['kernels_ms_per_step']}")
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(min(batch, key=lambda s: s.n), a.cpu_seconds)
     print(json.dumps(out), flush=True)

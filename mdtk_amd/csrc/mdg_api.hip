@@ -43,7 +43,12 @@ struct mdg_ctx {
   int slot_fbzl = -1, slot_flj = -1, slot_rebo = -1, slot_bren = -1, slot_lj = -1, slot_tors = -1, slot_eam = -1;
   FLJTable* dFLJ = nullptr;  // FLJ parameters (device)
   EAMTable* dEAM = nullptr;  // Ackland parameters (device)
-  double* eam_fp = nullptr;  // per atom: F'(rho) of the Ackland density pass
+  int2* rb = nullptr;          // reduction tiles {system, first atom}
+  int* rb_off = nullptr;       // per system: its first tile (nsys + 1)
+  double* red_part = nullptr;  // 8 partial results per tile
+  int nred = 0;
+  signed char* eam_e = nullptr;  // per atom: Ackland element index (Cu 0, Ag 1, Au 2) or -1
+  double2* eam_fe = nullptr;      // per atom: F'(rho) of the Ackland density pass and the element index
   bool have_short = false, have_long = false;
   double rc_short = 0, rc_long = 0;
 
@@ -429,8 +434,6 @@ __global__ void k_zero_counts(long long* tally) {
   tally[threadIdx.x * kTallyStride + 1] = 0;
 }
 
-// One force call (FProxy::operator(), FProxy.cxx:45-61): lists, coordination,
-// then every chain member adds into grad and its energy slot.
 // The bond-order chain members, in the geometry type R of the context.
 template <class R, bool TH>
 void force_terms(mdg_ctx* c, const TablesT<R>* T) {
@@ -538,6 +541,14 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   }
 }
 
+// The skin test of every system (both stages).
+void sys_disp(mdg_ctx* c, double lim_s, double lim_l) {
+  Atoms& A = c->A;
+  k_sys_disp_part<<<c->nred, 256, 0, c->stream>>>(c->rb, A.sys_off, A.disp_s, A.disp_l, A.disp_p, c->red_part);
+  k_sys_disp_merge<<<c->nsys, 64, 0, c->stream>>>(c->rb_off, c->red_part, lim_s, lim_l, c->pair_margin,
+                                                  A.disp_p ? 1 : 0, c->sflags, c->pflags, c->flags, c->counters);
+}
+
 // One force call (FProxy::operator(), FProxy.cxx:45-61): lists, coordination,
 // then every chain member adds into grad and its energy slot.
 void force_call(mdg_ctx* c) {
@@ -564,10 +575,10 @@ void force_call(mdg_ctx* c) {
   if (c->slot_eam >= 0) {
     double* ea = c->eatom + (size_t)c->slot_eam * n;
     launch(c, "k_eam_density", [&] {
-      k_eam_density<<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dEAM, c->eam_fp, ea);
+      k_eam_density<<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dEAM, c->eam_e, c->eam_fe, ea);
     });
     launch(c, "k_eam_force", [&] {
-      k_eam_force<<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dEAM, c->eam_fp, A.grad, ea);
+      k_eam_force<<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dEAM, c->eam_e, c->eam_fe, A.grad, ea);
     });
   }
   if (c->slot_fbzl >= 0) {
@@ -843,8 +854,26 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   if (c->slot_lj >= 0)
     r |= dalloc(c, A.pl, (size_t)n * A.cap_p) | dalloc(c, A.pl_cnt, n) | dalloc(c, A.disp_p, 3 * (size_t)n);
   r |= dalloc(c, c->handled_s, n) | dalloc(c, c->handled_l, n);
-  c->eam_fp = nullptr;
-  if (c->slot_eam >= 0) r |= dalloc(c, c->eam_fp, n);
+  {  // reduction tiles
+    std::vector<int2> rb;
+    std::vector<int> ro(nsys + 1, 0);
+    for (int s = 0; s < nsys; s++) {
+      ro[s] = (int)rb.size();
+      int m = off[s + 1] - off[s];
+      for (int t = 0; t < std::max(1, (m + kRedTile - 1) / kRedTile); t++)
+        rb.push_back(make_int2(s, off[s] + t * kRedTile));
+    }
+    ro[nsys] = (int)rb.size();
+    c->nred = (int)rb.size();
+    r |= dalloc(c, c->rb, rb.size()) | dalloc(c, c->rb_off, nsys + 1) | dalloc(c, c->red_part, 8 * rb.size());
+    if (!r) {
+      HIPCHK(c, hipMemcpy(c->rb, rb.data(), rb.size() * sizeof(int2), hipMemcpyHostToDevice));
+      HIPCHK(c, hipMemcpy(c->rb_off, ro.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice));
+    }
+  }
+  c->eam_e = nullptr;
+  c->eam_fe = nullptr;
+  if (c->slot_eam >= 0) r |= dalloc(c, c->eam_e, n) | dalloc(c, c->eam_fe, n);
   if (r) return 1;
   // cell grids, one per system and list
   auto grids = [&](double rc, CellGrid*& gd, int*& coff, int& ncell) -> int {
@@ -924,6 +953,11 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   HIPCHK(c, hipMemset(A.bcnt, 0, (size_t)n * 4));
   HIPCHK(c, up(c->handled_s, hs.data(), n));
   HIPCHK(c, up(c->handled_l, hl.data(), n));
+  if (c->eam_e) {
+    std::vector<signed char> ek(n);
+    for (int i = 0; i < n; i++) ek[i] = el[i] == 64 ? 0 : el[i] == 108 ? 1 : el[i] == 197 ? 2 : -1;
+    HIPCHK(c, up(c->eam_e, ek.data(), n));
+  }
   int fl[3] = {1, 1, 1};
   std::vector<int> sfl(2 * (size_t)nsys, 1);
   HIPCHK(c, up(c->flags, fl, sizeof(fl)));
@@ -1161,17 +1195,22 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
   // captured once and replayed.
   auto body = [&]() {
     if (c->bath_on)
-      launch(c, "k_bath_temp", [&] { k_bath_temp<<<nsys, 1024, 0, c->stream>>>(A, c->box); });
-    launch(c, "k_drift", [&] { k_drift<<<blocks(n), 256, 0, c->stream>>>(A, c->dtp); });
+      launch(c, "k_bath_temp", [&] {
+        k_bath_temp_part<<<c->nred, 256, 0, c->stream>>>(A, c->box, c->rb, c->red_part);
+        k_bath_temp_merge<<<nsys, 64, 0, c->stream>>>(A, c->rb_off, c->red_part);
+      });
+    launch(c, "k_drift", [&] { k_drift_tile<<<c->nred, 256, 0, c->stream>>>(A, c->rb, c->dtp, c->red_part); });
     launch(c, "k_sys_disp", [&] {
-      k_sys_disp<<<nsys, 1024, 0, c->stream>>>(A.sys_off, A.disp_s, A.disp_l, A.disp_p, lim_s, lim_l,
-                                              c->pair_margin, c->sflags, c->pflags, c->flags, c->counters);
+      k_sys_disp_merge<<<nsys, 64, 0, c->stream>>>(c->rb_off, c->red_part, lim_s, lim_l, c->pair_margin,
+                                                   A.disp_p ? 1 : 0, c->sflags, c->pflags, c->flags, c->counters);
     });
     force_call(c);
-    launch(c, "k_kick", [&] { k_kick<<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->dtp); });
+    launch(c, "k_kick", [&] {
+      k_kick_tile<<<c->nred, 256, 0, c->stream>>>(A, c->box, c->rb, c->dtp, nterms, c->eatom, c->red_part);
+    });
     launch(c, "k_sys_step_end", [&] {
-      k_sys_step_end<<<nsys, 1024, 0, c->stream>>>(A, nterms, c->eatom, c->dtp, fixed, c->stepout, c->stepidx,
-                                                  nsys, c->ebath);
+      k_sys_step_merge<<<nsys, 64, 0, c->stream>>>(c->rb_off, c->red_part, c->dtp, fixed, c->stepout, c->stepidx,
+                                                   nsys, A.tb_dE ? c->ebath : nullptr);
     });
     k_step_advance<<<1, 64, 0, c->stream>>>(c->stepidx);
   };
@@ -1288,8 +1327,7 @@ int mdg_set_positions(mdg_ctx* c, const double* xyz, const int32_t* pbc_count, c
   }
   double lim_s = c->have_short ? 0.44 * c->rc_short : 1e300;
   double lim_l = c->have_long ? 0.44 * c->rc_long : 1e300;
-  k_sys_disp<<<c->nsys, 1024, 0, c->stream>>>(c->A.sys_off, c->A.disp_s, c->A.disp_l, c->A.disp_p, lim_s, lim_l,
-                                             c->pair_margin, c->sflags, c->pflags, c->flags, c->counters);
+  sys_disp(c, lim_s, lim_l);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }

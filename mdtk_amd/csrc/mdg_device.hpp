@@ -107,6 +107,7 @@ struct EAMTable {
   double rk[3][3][7], ak[3][3][7];
   double Rk[3][2], Ak[3][2];
   double AS[3][3];
+  double Z[3];  // nuclear charges (atomic number x e)
   double x[2], v[3][3][2], d[3][3][2];
   double rmax;  // no pair term or density beyond this distance
 };

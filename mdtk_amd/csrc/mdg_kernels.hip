@@ -280,12 +280,8 @@ template __global__ void k_nl_build<512>(const int*, const int*, int, int, const
 // 108-133): rebuild iff d1 + d2 > 0.44 Rc.  The two largest values of a set
 // are exact whatever the merge order.
 __device__ __forceinline__ void top2_insert(double v, double& d1, double& d2) {
-  if (v >= d1) {
-    d2 = d1;
-    d1 = v;
-  } else if (v >= d2) {
-    d2 = v;
-  }
+  d2 = fmax(d2, fmin(v, d1));  // v >= d1: d1 moves down; else v competes for second place
+  d1 = fmax(d1, v);
 }
 
 __device__ __forceinline__ void top2_merge(double& d1, double& d2, double e1, double e2) {
@@ -300,16 +296,22 @@ __device__ __forceinline__ void top2_wave(double& d1, double& d2) {
   }
 }
 
-// One block per system; sflags[2*s+{0,1}], pflags[s] and the any-system
-// flags[0..2].  The LJ pair rows are redone with the long list or when their
-// own displacements pass the margin.
-__global__ void k_sys_disp(const int* __restrict__ sys_off, const double* __restrict__ ds,
-                           const double* __restrict__ dl, const double* __restrict__ dp, double lim_s, double lim_l,
-                           double lim_p, int* __restrict__ sflags, int* __restrict__ pflags, int* __restrict__ flags,
-                           long long* __restrict__ rebuilds) {
-  __shared__ double sh[16][6];  // up to 1024 threads
-  int s = blockIdx.x;
-  int b = sys_off[s], e = sys_off[s + 1];
+// Per-system reductions run in two stages: a 256-thread block per tile of
+// at most kRedTile atoms of one system (rb[tile] = {system, first atom})
+// writes its partial results to part[8 * tile + k]; then a wave per system
+// merges that system's tiles, rb_off[s] .. rb_off[s+1], in tile order, so a
+// system's result does not depend on the other systems of the batch.
+__device__ __forceinline__ int tile_end(const int2 t, const int* __restrict__ sys_off) {
+  return min(t.y + kRedTile, sys_off[t.x + 1]);
+}
+
+// The two largest displacement lengths of each list over one tile.
+__global__ void __launch_bounds__(256) k_sys_disp_part(const int2* __restrict__ rb, const int* __restrict__ sys_off,
+                                                       const double* __restrict__ ds, const double* __restrict__ dl,
+                                                       const double* __restrict__ dp, double* __restrict__ part) {
+  __shared__ double sh[4][6];
+  const int2 t = rb[blockIdx.x];
+  const int b = t.y, e = tile_end(t, sys_off);
   double a1 = 0, a2 = 0, b1 = 0, b2 = 0, c1 = 0, c2 = 0;
   auto len = [](const double* d, int i) {
     return sqrt(d[3 * i] * d[3 * i] + d[3 * i + 1] * d[3 * i + 1] + d[3 * i + 2] * d[3 * i + 2]);
@@ -322,7 +324,7 @@ __global__ void k_sys_disp(const int* __restrict__ sys_off, const double* __rest
   top2_wave(a1, a2);
   top2_wave(b1, b2);
   top2_wave(c1, c2);
-  int w = threadIdx.x >> 6;
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
     sh[w][0] = a1; sh[w][1] = a2; sh[w][2] = b1; sh[w][3] = b2; sh[w][4] = c1; sh[w][5] = c2;
   }
@@ -334,9 +336,34 @@ __global__ void k_sys_disp(const int* __restrict__ sys_off, const double* __rest
       top2_merge(y1, y2, sh[q][2], sh[q][3]);
       top2_merge(z1, z2, sh[q][4], sh[q][5]);
     }
+    double* o = part + 8 * (size_t)blockIdx.x;
+    o[0] = x1; o[1] = x2; o[2] = y1; o[3] = y2; o[4] = z1; o[5] = z2;
+  }
+}
+
+// One wave per system, the skin test of NeighbourList::MovedTooMuch (the
+// two largest displacements): sflags[2*s+{0,1}], pflags[s] and the
+// any-system flags[0..2].  The LJ pair rows are redone with the long list or when their
+// own displacements pass the margin (only when have_p).
+__global__ void __launch_bounds__(64) k_sys_disp_merge(const int* __restrict__ rb_off, const double* __restrict__ part,
+                                                       double lim_s, double lim_l, double lim_p, int have_p,
+                                                       int* __restrict__ sflags, int* __restrict__ pflags,
+                                                       int* __restrict__ flags, long long* __restrict__ rebuilds) {
+  const int s = blockIdx.x;
+  double x1 = 0, x2 = 0, y1 = 0, y2 = 0, z1 = 0, z2 = 0;
+  for (int q = rb_off[s] + threadIdx.x; q < rb_off[s + 1]; q += 64) {
+    const double* o = part + 8 * (size_t)q;
+    top2_merge(x1, x2, o[0], o[1]);
+    top2_merge(y1, y2, o[2], o[3]);
+    top2_merge(z1, z2, o[4], o[5]);
+  }
+  top2_wave(x1, x2);
+  top2_wave(y1, y2);
+  top2_wave(z1, z2);
+  if (threadIdx.x == 0) {
     int fs = (x1 + x2 > lim_s) ? 1 : 0;
     int fl = (y1 + y2 > lim_l) ? 1 : 0;
-    int fp = (dp && (fl || z1 + z2 > lim_p)) ? 1 : 0;
+    int fp = (have_p && (fl || z1 + z2 > lim_p)) ? 1 : 0;
     sflags[2 * s] |= fs;
     sflags[2 * s + 1] |= fl;
     if (fs) {
@@ -518,7 +545,8 @@ __global__ void k_flj(Atoms A, Box bx, const FLJTable* __restrict__ T, double* _
 // Ackland::operator() (Ackland.cxx:31-57): E = sum_i F(rho_i) + sum_{i<j} V,
 // F = -sqrt(rho), rho_i = sum_j g(r_ij).  Two passes over the long list with
 // 16 lanes per metal atom: the density pass stores F'(rho_i) (0 where rho is
-// 0, as F() applies no gradient there, :210-221) and F_i; the force pass
+// 0, as F() applies no gradient there, :210-221) beside the atom's element
+// and F_i; the force pass
 // gathers, per ordered pair, dV/dr + (F'_i + F'_j) dg/dr along x_i - x_j into
 // atom i alone (no scatter to j), and V of the pairs j > i into i's energy.
 __device__ __forceinline__ int eam_index(int elem) {
@@ -592,14 +620,15 @@ __device__ __forceinline__ void eam_pair(const EAMTable* __restrict__ T, int a, 
 }
 
 __global__ void __launch_bounds__(256) k_eam_density(Atoms A, Box bx, const EAMTable* __restrict__ T,
-                                                     double* __restrict__ fp, double* __restrict__ eatom) {
+                                                     const signed char* __restrict__ ek, double2* __restrict__ fe,
+                                                     double* __restrict__ eatom) {
   constexpr int LW = 16;
   const int sl = threadIdx.x % LW;
   const int i = blockIdx.x * (blockDim.x / LW) + threadIdx.x / LW;
   if (i >= A.n) return;  // group-uniform
-  const int ei = eam_index(A.elem[i]);
+  const int ei = ek[i];
   if (ei < 0) {
-    if (sl == 0) fp[i] = 0.0;
+    if (sl == 0) fe[i] = make_double2(0.0, -1.0);
     return;
   }
   const int cnt = A.nll_cnt[i];
@@ -610,52 +639,53 @@ __global__ void __launch_bounds__(256) k_eam_density(Atoms A, Box bx, const EAMT
   double rho = 0.0;
   for (int q = sl; q < cnt; q += LW) {
     const int j = L[q];
-    const int ej = eam_index(A.elem[j]);
-    if (ej < 0) continue;
     const double4 pj = A.p[j];
+    if ((int)pj.w != T_OTHER) continue;  // the metals are the atoms of type T_OTHER
     const double dx = mic(pi.x - pj.x, Lb[0]), dy = mic(pi.y - pj.y, Lb[1]), dz = mic(pi.z - pj.z, Lb[2]);
     const double r = sqrt(dx * dx + dy * dy + dz * dz);
     if (r >= rmax) continue;
     double g, dg;
-    eam_g(T, ei, ej, r, g, dg);
+    eam_g(T, ei, ek[j], r, g, dg);
     rho += g;
   }
 #pragma unroll
   for (int o = LW / 2; o > 0; o >>= 1) rho += __shfl_xor(rho, o, LW);
   if (sl == 0) {
     const double F = -sqrt(rho);
-    fp[i] = (rho != 0.0) ? 0.5 / F : 0.0;
+    fe[i] = make_double2((rho != 0.0) ? 0.5 / F : 0.0, double(ei));
     eatom[i] = F;
   }
 }
 
 __global__ void __launch_bounds__(256) k_eam_force(Atoms A, Box bx, const EAMTable* __restrict__ T,
-                                                   const double* __restrict__ fp, double* __restrict__ grad,
+                                                   const signed char* __restrict__ ek,
+                                                   const double2* __restrict__ fe, double* __restrict__ grad,
                                                    double* __restrict__ eatom) {
   constexpr int LW = 16;
   const int sl = threadIdx.x % LW;
   const int i = blockIdx.x * (blockDim.x / LW) + threadIdx.x / LW;
   if (i >= A.n) return;  // group-uniform
-  const int ei = eam_index(A.elem[i]);
+  const int ei = ek[i];
   if (ei < 0) return;
   const int cnt = A.nll_cnt[i];
   const int* L = A.nll + (size_t)i * A.cap_l;
   const double4 pi = A.p[i];
   const double* Lb = box_of(bx, i);
-  const double rmax = T->rmax, fpi = fp[i], Zi = A.Z[i];
+  const double rmax = T->rmax, fpi = fe[i].x, Zi = T->Z[ei];
   double gx = 0.0, gy = 0.0, gz = 0.0, ev = 0.0;
   for (int q = sl; q < cnt; q += LW) {
     const int j = L[q];
-    const int ej = eam_index(A.elem[j]);
-    if (ej < 0) continue;
     const double4 pj = A.p[j];
+    if ((int)pj.w != T_OTHER) continue;
     const double dx = mic(pi.x - pj.x, Lb[0]), dy = mic(pi.y - pj.y, Lb[1]), dz = mic(pi.z - pj.z, Lb[2]);
     const double r = sqrt(dx * dx + dy * dy + dz * dz);
     if (r >= rmax) continue;
+    const double2 fj = fe[j];  // F'(rho_j) and j's element
+    const int ej = (int)fj.y;
     double v, dv, g, dg;
-    eam_pair(T, ei, ej, Zi, A.Z[j], r, v, dv);
+    eam_pair(T, ei, ej, Zi, T->Z[ej], r, v, dv);
     eam_g(T, ei, ej, r, g, dg);
-    const double c = (dv + (fpi + fp[j]) * dg) / r;
+    const double c = (dv + (fpi + fj.x) * dg) / r;
     gx += c * dx;
     gy += c * dy;
     gz += c * dz;
@@ -676,29 +706,86 @@ __global__ void __launch_bounds__(256) k_eam_force(Atoms A, Box bx, const EAMTab
 
 // ------------------------------------------------------------ Verlet
 // eq.1 of SimLoop::executeMain (SimLoop.cxx:323-332): x += v dt + a dt^2/2,
-// accumulated into both neighbour lists' (and the LJ pair rows') displacements.
-__global__ void k_drift(Atoms A, const double* __restrict__ dtp) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.n || A.fixed[i]) return;
-  double dt = dtp[A.sys[i]];
-  double dx = A.vx[i] * dt + A.ax[i] * dt * dt / 2.0;
-  double dy = A.vy[i] * dt + A.ay[i] * dt * dt / 2.0;
-  double dz = A.vz[i] * dt + A.az[i] * dt * dt / 2.0;
-  double x = A.x[i] + dx, y = A.y[i] + dy, z = A.z[i] + dz;
-  A.x[i] = x;
-  A.y[i] = y;
-  A.z[i] = z;
-  A.p[i] = make_double4(x, y, z, double(A.type[i]));
-  A.disp_s[3 * i] += dx;
-  A.disp_s[3 * i + 1] += dy;
-  A.disp_s[3 * i + 2] += dz;
-  A.disp_l[3 * i] += dx;
-  A.disp_l[3 * i + 1] += dy;
-  A.disp_l[3 * i + 2] += dz;
+// accumulated into both neighbour lists' (and the LJ pair rows') displacements;
+// the lengths of the atom's accumulated displacements come back for the skin
+// test.
+__device__ __forceinline__ void drift_atom(const Atoms& A, int i, double dt, double& ls, double& ll, double& lp) {
+  double sx = A.disp_s[3 * i], sy = A.disp_s[3 * i + 1], sz = A.disp_s[3 * i + 2];
+  double lx = A.disp_l[3 * i], ly = A.disp_l[3 * i + 1], lz = A.disp_l[3 * i + 2];
+  double px = 0, py = 0, pz = 0;
   if (A.disp_p) {
-    A.disp_p[3 * i] += dx;
-    A.disp_p[3 * i + 1] += dy;
-    A.disp_p[3 * i + 2] += dz;
+    px = A.disp_p[3 * i];
+    py = A.disp_p[3 * i + 1];
+    pz = A.disp_p[3 * i + 2];
+  }
+  if (!A.fixed[i]) {
+    double dx = A.vx[i] * dt + A.ax[i] * dt * dt / 2.0;
+    double dy = A.vy[i] * dt + A.ay[i] * dt * dt / 2.0;
+    double dz = A.vz[i] * dt + A.az[i] * dt * dt / 2.0;
+    double x = A.x[i] + dx, y = A.y[i] + dy, z = A.z[i] + dz;
+    A.x[i] = x;
+    A.y[i] = y;
+    A.z[i] = z;
+    A.p[i] = make_double4(x, y, z, double(A.type[i]));
+    sx += dx;
+    sy += dy;
+    sz += dz;
+    lx += dx;
+    ly += dy;
+    lz += dz;
+    A.disp_s[3 * i] = sx;
+    A.disp_s[3 * i + 1] = sy;
+    A.disp_s[3 * i + 2] = sz;
+    A.disp_l[3 * i] = lx;
+    A.disp_l[3 * i + 1] = ly;
+    A.disp_l[3 * i + 2] = lz;
+    if (A.disp_p) {
+      px += dx;
+      py += dy;
+      pz += dz;
+      A.disp_p[3 * i] = px;
+      A.disp_p[3 * i + 1] = py;
+      A.disp_p[3 * i + 2] = pz;
+    }
+  }
+  ls = sqrt(sx * sx + sy * sy + sz * sz);
+  ll = sqrt(lx * lx + ly * ly + lz * lz);
+  lp = sqrt(px * px + py * py + pz * pz);
+}
+
+// The drift of one tile's atoms and the tile's part of the skin test (the
+// two largest displacement lengths per list), as k_sys_disp_part writes it.
+__global__ void __launch_bounds__(256) k_drift_tile(Atoms A, const int2* __restrict__ rb,
+                                                    const double* __restrict__ dtp, double* __restrict__ part) {
+  __shared__ double sh[4][6];
+  const int2 t = rb[blockIdx.x];
+  const int e = tile_end(t, A.sys_off);
+  const double dt = dtp[t.x];
+  double a1 = 0, a2 = 0, b1 = 0, b2 = 0, c1 = 0, c2 = 0;
+  for (int i = t.y + threadIdx.x; i < e; i += blockDim.x) {
+    double ls, ll, lp;
+    drift_atom(A, i, dt, ls, ll, lp);
+    top2_insert(ls, a1, a2);
+    top2_insert(ll, b1, b2);
+    if (A.disp_p) top2_insert(lp, c1, c2);
+  }
+  top2_wave(a1, a2);
+  top2_wave(b1, b2);
+  top2_wave(c1, c2);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sh[w][0] = a1; sh[w][1] = a2; sh[w][2] = b1; sh[w][3] = b2; sh[w][4] = c1; sh[w][5] = c2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x1 = 0, x2 = 0, y1 = 0, y2 = 0, z1 = 0, z2 = 0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) {
+      top2_merge(x1, x2, sh[q][0], sh[q][1]);
+      top2_merge(y1, y2, sh[q][2], sh[q][3]);
+      top2_merge(z1, z2, sh[q][4], sh[q][5]);
+    }
+    double* o = part + 8 * (size_t)blockIdx.x;
+    o[0] = x1; o[1] = x2; o[2] = y1; o[3] = y2; o[4] = z1; o[5] = z2;
   }
 }
 
@@ -734,53 +821,69 @@ __device__ __forceinline__ bool bath_inside(const Bath& b, double x, double y, d
   }
 }
 
-// actualTemperatureOfThermalBath (SimLoop.cxx:516-535), one block per system,
-// at the start of the step (positions and velocities at t).
-__global__ void k_bath_temp(Atoms A, Box bx) {
-  int s = blockIdx.x;
+// actualTemperatureOfThermalBath (SimLoop.cxx:516-535), tiled per system, at
+// the start of the step (positions and velocities at t).
+__global__ void __launch_bounds__(256) k_bath_temp_part(Atoms A, Box bx, const int2* __restrict__ rb,
+                                                        double* __restrict__ part) {
+  const int2 t = rb[blockIdx.x];
+  const int s = t.x;
   const Bath b = A.bath[s];
-  if (b.geom == TB_NONE) {
-    if (threadIdx.x == 0) A.tb_T[s] = 0.0;
-    return;
-  }
-  const double* L = bx.L + 3 * s;
   double e = 0.0, cnt = 0.0;
-  for (int i = A.sys_off[s] + threadIdx.x; i < A.sys_off[s + 1]; i += blockDim.x) {
-    if (A.fixed[i] || !A.tb_apply[i] || !bath_inside(b, A.x[i], A.y[i], A.z[i], L)) continue;
-    double vv = sqrt(A.vx[i] * A.vx[i] + A.vy[i] * A.vy[i] + A.vz[i] * A.vz[i]);
-    e += A.mass[i] * (vv * vv) / 2.0;
-    cnt += 1.0;
+  if (b.geom != TB_NONE) {
+    const double* L = bx.L + 3 * s;
+    for (int i = t.y + threadIdx.x; i < tile_end(t, A.sys_off); i += blockDim.x) {
+      if (A.fixed[i] || !A.tb_apply[i] || !bath_inside(b, A.x[i], A.y[i], A.z[i], L)) continue;
+      double vv = sqrt(A.vx[i] * A.vx[i] + A.vy[i] * A.vy[i] + A.vz[i] * A.vz[i]);
+      e += A.mass[i] * (vv * vv) / 2.0;
+      cnt += 1.0;
+    }
   }
   e = block_sum(e);
   cnt = block_sum(cnt);
-  if (threadIdx.x == 0) A.tb_T[s] = (cnt == 0.0) ? 0.0 : e / (3.0 / 2.0 * (8.617343E-5 * kEV) * cnt);
+  if (threadIdx.x == 0) {
+    part[8 * (size_t)blockIdx.x] = e;
+    part[8 * (size_t)blockIdx.x + 1] = cnt;
+  }
+}
+
+__global__ void __launch_bounds__(64) k_bath_temp_merge(Atoms A, const int* __restrict__ rb_off,
+                                                        const double* __restrict__ part) {
+  const int s = blockIdx.x;
+  double e = 0.0, cnt = 0.0;
+  for (int q = rb_off[s] + threadIdx.x; q < rb_off[s + 1]; q += 64) {
+    e += part[8 * (size_t)q];
+    cnt += part[8 * (size_t)q + 1];
+  }
+  e = wave_sum(e);
+  cnt = wave_sum(cnt);
+  if (threadIdx.x == 0)
+    A.tb_T[s] = (A.bath[s].geom == TB_NONE || cnt == 0.0) ? 0.0 : e / (3.0 / 2.0 * (8.617343E-5 * kEV) * cnt);
 }
 
 // eq.2-4 of SimLoop::executeMain (SimLoop.cxx:338-399) with the thermal
 // bath's friction (:352-386) on atoms inside its geometry at x(t+dt), then
 // the PBC wrap (:401, Atom.cxx:93-125).
-__global__ void k_kick(Atoms A, Box bx, const double* __restrict__ dtp) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.n) return;
+// One atom: v (in: v(t), out: v(t+dt)) from grad g; dE = the energy taken
+// from the bath (0 outside it).
+__device__ __forceinline__ void kick_atom(const Atoms& A, const Box& bx, int i, double dt, double gx, double gy,
+                                          double gz, double& vx, double& vy, double& vz, double& dE) {
   const double* L = box_of(bx, i);
   double x = A.x[i], y = A.y[i], z = A.z[i];
+  dE = 0.0;
   if (!A.fixed[i]) {
     const int s = A.sys[i];
-    double dt = dtp[s];
     double M = A.mass[i];
-    double fx = -A.grad[3 * i], fy = -A.grad[3 * i + 1], fz = -A.grad[3 * i + 2];
-    double hx = A.vx[i] + A.ax[i] * dt / 2.0;
-    double hy = A.vy[i] + A.ay[i] * dt / 2.0;
-    double hz = A.vz[i] + A.az[i] * dt / 2.0;
+    double fx = -gx, fy = -gy, fz = -gz;
+    double hx = vx + A.ax[i] * dt / 2.0;
+    double hy = vy + A.ay[i] * dt / 2.0;
+    double hz = vz + A.az[i] * dt / 2.0;
     if (A.bath) {
-      double dE = 0.0;
       const Bath& b = A.bath[s];
       if (b.geom != TB_NONE && A.tb_apply[i] && bath_inside(b, x, y, z, L)) {
         const double T = A.tb_T[s];
         double To_by_T = (fabs(T) < 1e-5) ? 0 : (b.To / T);
         if (To_by_T < -5.0) To_by_T = -5.0;
         if (To_by_T > 5.0) To_by_T = 5.0;
-        const double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i];
         const double k = 1.0 - sqrt(To_by_T);
         const double dfx = -vx * M * b.gamma * k, dfy = -vy * M * b.gamma * k, dfz = -vz * M * b.gamma * k;
         // energy taken from the bath: kinetic energy after the step with and
@@ -809,9 +912,12 @@ __global__ void k_kick(Atoms A, Box bx, const double* __restrict__ dtp) {
     A.ax[i] = ax;
     A.ay[i] = ay;
     A.az[i] = az;
-    A.vx[i] = hx + ax * dt / 2.0;
-    A.vy[i] = hy + ay * dt / 2.0;
-    A.vz[i] = hz + az * dt / 2.0;
+    vx = hx + ax * dt / 2.0;
+    vy = hy + ay * dt / 2.0;
+    vz = hz + az * dt / 2.0;
+    A.vx[i] = vx;
+    A.vy[i] = vy;
+    A.vz[i] = vz;
   }
   int cx = A.pcx[i], cy = A.pcy[i], cz = A.pcz[i];
   wrap(x, L[0], cx);
@@ -824,6 +930,48 @@ __global__ void k_kick(Atoms A, Box bx, const double* __restrict__ dtp) {
   A.pcy[i] = cy;
   A.pcz[i] = cz;
   A.p[i] = make_double4(x, y, z, double(A.type[i]));
+}
+
+// The kick of one tile's atoms and the tile's partial step outputs: the
+// potential energy (all chain members), the kinetic energy after the kick,
+// the net force, the bath's energy, and the largest speed of a free atom with
+// neighbours (the dt rule); merged per system by k_sys_step_merge.
+__global__ void __launch_bounds__(256) k_kick_tile(Atoms A, Box bx, const int2* __restrict__ rb,
+                                                   const double* __restrict__ dtp, int nterms,
+                                                   const double* __restrict__ eatom, double* __restrict__ part) {
+  const int2 t = rb[blockIdx.x];
+  const int e = tile_end(t, A.sys_off);
+  const double dt = dtp[t.x];
+  const int n = A.n;
+  double ep = 0, ek = 0, fx = 0, fy = 0, fz = 0, vm = 0, eb = 0;
+  for (int i = t.y + threadIdx.x; i < e; i += blockDim.x) {
+    const double gx = A.grad[3 * i], gy = A.grad[3 * i + 1], gz = A.grad[3 * i + 2];
+    double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i], dE;
+    kick_atom(A, bx, i, dt, gx, gy, gz, vx, vy, vz, dE);
+    for (int q = 0; q < nterms; q++) ep += eatom[(size_t)q * n + i];
+    double vv = sqrt(vx * vx + vy * vy + vz * vz);
+    ek += A.mass[i] * (vv * vv) / 2.0;
+    fx -= gx;
+    fy -= gy;
+    fz -= gz;
+    if (!A.fixed[i] && (A.nls_cnt[i] > 0 || A.nll_cnt[i] > 0) && vv > vm) vm = vv;
+    eb += dE;  // energy taken from the bath this step (SimLoop.cxx:384)
+  }
+  ep = block_sum(ep);
+  ek = block_sum(ek);
+  fx = block_sum(fx);
+  fy = block_sum(fy);
+  fz = block_sum(fz);
+  eb = block_sum(eb);
+  for (int o = 32; o > 0; o >>= 1) vm = fmax(vm, __shfl_xor(vm, o, 64));
+  __shared__ double vms[4];
+  if ((threadIdx.x & 63) == 0) vms[threadIdx.x >> 6] = vm;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++) vm = fmax(vm, vms[q]);
+    double* o = part + 8 * (size_t)blockIdx.x;
+    o[0] = ep; o[1] = ek; o[2] = fx; o[3] = fy; o[4] = fz; o[5] = vm; o[6] = eb;
+  }
 }
 
 // Refreshes the packed positions after the host wrote x, y, z.
@@ -861,43 +1009,29 @@ __global__ void k_sys_energy(int nterms, const int* __restrict__ sys_off, int n,
 
 // Per-system end of an iteration: E_pot (t+dt), E_kin after the kick
 // (SimLoop.cxx:500-514), |sum F| (:418-430), the dt rule over atoms with
-// neighbours (:394-398, :409-416), one block per system.
-// out[(step*nsys + s)*4 + {0,1,2,3}] = epot, ekin, dt used, |sum F|.
-__global__ void k_sys_step_end(Atoms A, int nterms, const double* __restrict__ eatom, double* __restrict__ dtp,
-                               int fixed_dt, double* __restrict__ out, const int* __restrict__ stepidx, int nsys,
-                               double* __restrict__ ebath) {
-  int s = blockIdx.x;
-  int b = A.sys_off[s], e = A.sys_off[s + 1];
-  int n = A.n;
-  double ep = 0, ek = 0, fx = 0, fy = 0, fz = 0, vm = 0;
-  for (int i = b + threadIdx.x; i < e; i += blockDim.x) {
-    for (int t = 0; t < nterms; t++) ep += eatom[(size_t)t * n + i];
-    double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i];
-    double vv = sqrt(vx * vx + vy * vy + vz * vz);
-    ek += A.mass[i] * (vv * vv) / 2.0;
-    fx -= A.grad[3 * i];
-    fy -= A.grad[3 * i + 1];
-    fz -= A.grad[3 * i + 2];
-    if (!A.fixed[i] && (A.nls_cnt[i] > 0 || A.nll_cnt[i] > 0) && vv > vm) vm = vv;
+// neighbours (:394-398, :409-416), tiled per system: the tile pass is
+// k_kick_tile.
+// One wave per system: out[(step*nsys + s)*4 + {0,1,2,3}] = epot, ekin, dt
+// used, |sum F|; the bath's energy; the next dt.
+__global__ void __launch_bounds__(64) k_sys_step_merge(const int* __restrict__ rb_off, const double* __restrict__ part,
+                                                       double* __restrict__ dtp, int fixed_dt, double* __restrict__ out,
+                                                       const int* __restrict__ stepidx, int nsys,
+                                                       double* __restrict__ ebath) {
+  const int s = blockIdx.x;
+  double ep = 0, ek = 0, fx = 0, fy = 0, fz = 0, vm = 0, eb = 0;
+  for (int q = rb_off[s] + threadIdx.x; q < rb_off[s + 1]; q += 64) {
+    const double* o = part + 8 * (size_t)q;
+    ep += o[0]; ek += o[1]; fx += o[2]; fy += o[3]; fz += o[4]; vm = fmax(vm, o[5]); eb += o[6];
   }
-  ep = block_sum(ep);
-  ek = block_sum(ek);
-  fx = block_sum(fx);
-  fy = block_sum(fy);
-  fz = block_sum(fz);
-  if (A.tb_dE) {  // energy taken from the bath this step (SimLoop.cxx:384)
-    double eb = 0.0;
-    for (int i = b + threadIdx.x; i < e; i += blockDim.x) eb += A.tb_dE[i];
-    eb = block_sum(eb);
-    if (threadIdx.x == 0) ebath[s] += eb;
-  }
+  ep = wave_sum(ep);
+  ek = wave_sum(ek);
+  fx = wave_sum(fx);
+  fy = wave_sum(fy);
+  fz = wave_sum(fz);
+  eb = wave_sum(eb);
   for (int o = 32; o > 0; o >>= 1) vm = fmax(vm, __shfl_xor(vm, o, 64));
-  __shared__ double vms[16];
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) vms[threadIdx.x >> 6] = vm;
-  __syncthreads();
   if (threadIdx.x == 0) {
-    for (int q = 0; q < (int)(blockDim.x >> 6); q++) vm = fmax(vm, vms[q]);
+    if (ebath) ebath[s] += eb;
     int step = *stepidx;
     double* o = out + ((size_t)step * nsys + s) * 4;
     o[0] = ep;

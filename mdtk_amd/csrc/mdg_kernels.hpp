@@ -81,10 +81,17 @@ __global__ void k_nl_build(const int* flag, const int* sflags, int list_id, int 
                            const unsigned char* handled, Box bx, const CellGrid* grids,
                            const int* cell_off, double range2, const int* cell_of, const int* start,
                            const int* cell_atoms, int* cnt, int* list, double* disp, unsigned* err, int* split);
+// Per-system reductions are tiled: tile t = rb[t] = {system, first atom},
+// at most kRedTile atoms, one 256-thread block each, writing 8 doubles to
+// part[8 t ..]; a 64-thread block per system then merges the tiles
+// rb_off[s] .. rb_off[s+1] in order.
+constexpr int kRedTile = 512;
 // flags[0..2]: some system rebuilds its short list / long list / LJ pair
 // rows; sflags[2*s+{0,1}] and pflags[s] per system.
-__global__ void k_sys_disp(const int* sys_off, const double* ds, const double* dl, const double* dp, double lim_s,
-                           double lim_l, double lim_p, int* sflags, int* pflags, int* flags, long long* rebuilds);
+__global__ void k_sys_disp_part(const int2* rb, const int* sys_off, const double* ds, const double* dl,
+                                const double* dp, double* part);
+__global__ void k_sys_disp_merge(const int* rb_off, const double* part, double lim_s, double lim_l, double lim_p,
+                                 int have_p, int* sflags, int* pflags, int* flags, long long* rebuilds);
 __global__ void k_clear_flags(int nsys, int* flags, int* sflags, int* pflags);
 __global__ void k_bond_enum(Atoms A, int2* out, int* count, int cap, unsigned* err);
 // Energies go to eatom (per-atom, per chain member), owned by the first atom
@@ -122,20 +129,28 @@ __global__ void k_etors(Atoms A, const TablesT<R>* T, Box bx, const int2* bonds,
                         double* grad, double* eatom, long long* nquads);
 __global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* grad, double* eatom);
 __global__ void k_flj(Atoms A, Box bx, const FLJTable* T, double* grad, double* eatom);
-// Ackland EAM: the density pass (F'(rho) into fp, F into eatom), then the
+// Ackland EAM: the density pass (F'(rho) into fe, F into eatom), then the
 // force pass (gradient and pair energies); 16 lanes per atom, blocks of 256.
-__global__ void k_eam_density(Atoms A, Box bx, const EAMTable* T, double* fp, double* eatom);
-__global__ void k_eam_force(Atoms A, Box bx, const EAMTable* T, const double* fp, double* grad, double* eatom);
+// ek: per atom the EAM element index (Cu 0, Ag 1, Au 2) or -1; fe: per atom
+// (F'(rho), element index) written by the density pass.
+__global__ void k_eam_density(Atoms A, Box bx, const EAMTable* T, const signed char* ek, double2* fe, double* eatom);
+__global__ void k_eam_force(Atoms A, Box bx, const EAMTable* T, const signed char* ek, const double2* fe,
+                            double* grad, double* eatom);
 __global__ void k_brenner_bonds(Atoms A, const Tables* T, Box bx, const int2* bonds, const int* nbonds,
                                 double* grad, double* eatom, unsigned* err);
-__global__ void k_drift(Atoms A, const double* dtp);
-__global__ void k_kick(Atoms A, Box bx, const double* dtp);
+// Verlet halves over the reduction tiles: the drift writes the skin test's
+// tile part (as k_sys_disp_part), the kick the step outputs' tile part
+// (merged by k_sys_step_merge).
+__global__ void k_drift_tile(Atoms A, const int2* rb, const double* dtp, double* part);
+__global__ void k_kick_tile(Atoms A, Box bx, const int2* rb, const double* dtp, int nterms, const double* eatom,
+                            double* part);
 __global__ void k_init_acc(Atoms A);
 __global__ void k_pack_pos(Atoms A);
 __global__ void k_sys_energy(int nterms, const int* sys_off, int n, const double* eatom, double* eacc);
-__global__ void k_sys_step_end(Atoms A, int nterms, const double* eatom, double* dtp, int fixed_dt, double* out,
-                               const int* stepidx, int nsys, double* ebath);
-__global__ void k_bath_temp(Atoms A, Box bx);
+__global__ void k_sys_step_merge(const int* rb_off, const double* part, double* dtp, int fixed_dt, double* out,
+                                 const int* stepidx, int nsys, double* ebath);
+__global__ void k_bath_temp_part(Atoms A, Box bx, const int2* rb, double* part);
+__global__ void k_bath_temp_merge(Atoms A, const int* rb_off, const double* part);
 __global__ void k_step_advance(int* stepidx);
 __global__ void k_zero(double* p, size_t n);
 __global__ void k_zero_i(int* p, int n);

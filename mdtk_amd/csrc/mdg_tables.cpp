@@ -157,6 +157,7 @@ void build_eam_table(EAMTable& t) {
   t.x[0] = 1.0 * Ao;
   t.x[1] = 1.4 * Ao;
   const double Z[3] = {29.0 * kECharge, 47.0 * kECharge, 79.0 * kECharge};
+  for (int a = 0; a < 3; a++) t.Z[a] = Z[a];
   t.rmax = 0;
   for (int a = 0; a < 3; a++)
     for (int b = 0; b < 3; b++) {
