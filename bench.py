@@ -132,15 +132,26 @@ def roofline(kt, cnt, natoms, tune):
     return out
 
 
-def eam_algorithmic_bytes(entries, natoms):
-    """Compulsory traffic of the two Ackland passes: one 4-byte index per long
-    list entry in each pass; per atom, density pass: row count, element,
-    packed position (4 + 4 + 32 B) and F' and F written (16 B); force pass:
-    row count, element, packed position, F', charge (4 + 4 + 32 + 8 + 8 B),
-    gradient read + write (48 B), energy read + write (16 B)."""
-    dens = 4.0 * entries + natoms * (40 + 16)
-    force = 4.0 * entries + natoms * (56 + 48 + 16)
+def eam_algorithmic_bytes(entries, inrange, natoms):
+    """Compulsory traffic of the two Ackland passes.  Density pass: a 4-byte
+    index per long-list entry, per atom the row count, element and packed
+    position (4 + 1 + 32 B) read and F', F, the in-range count (16 + 8 + 4 B)
+    written, and a 4-byte index per in-range partner written.  Force pass: a
+    4-byte index per in-range partner, per atom the count, element, packed
+    position and (F', element) (4 + 1 + 32 + 16 B), gradient read + write
+    (48 B) and energy read + write (16 B)."""
+    dens = 4.0 * entries + natoms * (37 + 28) + 4.0 * inrange
+    force = 4.0 * inrange + natoms * (53 + 48 + 16)
     return dens, force
+
+
+def inrange_pairs(pos_a, box_a, r_a):
+    """Ordered pairs (i != j) closer than r_a in a periodic box."""
+    from scipy.spatial import cKDTree
+    import numpy as np
+    p = np.mod(pos_a, box_a)
+    t = cKDTree(p, boxsize=box_a)
+    return float(t.count_neighbors(t, r_a) - len(p))
 
 
 def metal_block(steps, warmup):
@@ -159,7 +170,9 @@ def metal_block(steps, warmup):
     kt = {k: e.kernel_time(k) for k in EAM_KERNELS}
     e.profile(False)
     entries = float(e.counters()["long_entries"])
-    dens, force = eam_algorithmic_bytes(entries, cu.n)
+    x, _, _, _ = e.state()
+    inrange = inrange_pairs(x / S.ANG, cu.pbc, 1.2247449 * 4.086)  # Ackland's largest range (Ag), A
+    dens, force = eam_algorithmic_bytes(entries, inrange, cu.n)
     rl = {}
     for k, alg in (("k_eam_density", dens), ("k_eam_force", force)):
         t_ms, nl = kt[k]
@@ -170,7 +183,7 @@ def metal_block(steps, warmup):
     et = (res["epot"] + res["ekin"]) / S.EV
     out = {"workload": f"Cu fcc 40x40x40 ({cu.n} atoms), periodic, 300 K, Ackland EAM alone, dt 1 fs, FP64",
            "value": cu.n * steps / (ms / 1e3), "unit": "atom-steps/s", "ms_per_step": ms / steps,
-           "long_entries_per_atom": entries / cu.n,
+           "long_entries_per_atom": entries / cu.n, "inrange_partners_per_atom": inrange / cu.n,
            "kernels_ms_per_step": {k: v[0] / max(steps, 1) for k, v in kt.items()},
            "roofline": rl, "max_abs_dE_tot_eV": float(np.abs(et - et[0]).max())}
     return out

@@ -49,6 +49,8 @@ struct mdg_ctx {
   int nred = 0;
   signed char* eam_e = nullptr;  // per atom: Ackland element index (Cu 0, Ag 1, Au 2) or -1
   double2* eam_fe = nullptr;      // per atom: F'(rho) of the Ackland density pass and the element index
+  int* eam_row = nullptr;         // per atom: partners inside the Ackland range this step (kEamRow slots)
+  int* eam_cnt = nullptr;
   bool have_short = false, have_long = false;
   double rc_short = 0, rc_long = 0;
 
@@ -276,6 +278,7 @@ int check_err(mdg_ctx* c) {
     if (h & ERR_NEG_COORD) m += " coordination below -1e-15 (REBO.cxx:434);";
     if (h & ERR_QUEUE_OVERFLOW) m += " work-queue overflow;";
     if (h & ERR_SPLINE_DOMAIN) m += " spline argument outside its domain;";
+    if (h & ERR_EAM_ROW_OVERFLOW) m += " more than 128 metal atoms inside the Ackland range of one atom;";
     unsigned z = 0;
     hipMemcpy(c->derr, &z, sizeof(unsigned), hipMemcpyHostToDevice);
     c->err = m;
@@ -575,10 +578,12 @@ void force_call(mdg_ctx* c) {
   if (c->slot_eam >= 0) {
     double* ea = c->eatom + (size_t)c->slot_eam * n;
     launch(c, "k_eam_density", [&] {
-      k_eam_density<<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dEAM, c->eam_e, c->eam_fe, ea);
+      k_eam_density<<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dEAM, c->eam_e, c->eam_fe, ea, c->eam_row,
+                                                          c->eam_cnt, c->derr);
     });
     launch(c, "k_eam_force", [&] {
-      k_eam_force<<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dEAM, c->eam_e, c->eam_fe, A.grad, ea);
+      k_eam_force<<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dEAM, c->eam_e, c->eam_fe, c->eam_row,
+                                                        c->eam_cnt, A.grad, ea);
     });
   }
   if (c->slot_fbzl >= 0) {
@@ -873,7 +878,11 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   }
   c->eam_e = nullptr;
   c->eam_fe = nullptr;
-  if (c->slot_eam >= 0) r |= dalloc(c, c->eam_e, n) | dalloc(c, c->eam_fe, n);
+  c->eam_row = nullptr;
+  c->eam_cnt = nullptr;
+  if (c->slot_eam >= 0)
+    r |= dalloc(c, c->eam_e, n) | dalloc(c, c->eam_fe, n) | dalloc(c, c->eam_row, (size_t)n * kEamRow) |
+         dalloc(c, c->eam_cnt, n);
   if (r) return 1;
   // cell grids, one per system and list
   auto grids = [&](double rc, CellGrid*& gd, int*& coff, int& ncell) -> int {

@@ -36,6 +36,7 @@ enum : unsigned {
   ERR_NEG_COORD = 16u,
   ERR_QUEUE_OVERFLOW = 32u,
   ERR_SPLINE_DOMAIN = 64u,
+  ERR_EAM_ROW_OVERFLOW = 128u,
 };
 
 // Quintic Hermite pieces of G(cos theta), AIREBO/cubici.hpp:39-80.  Each piece

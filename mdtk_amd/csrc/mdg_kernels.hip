@@ -621,32 +621,56 @@ __device__ __forceinline__ void eam_pair(const EAMTable* __restrict__ T, int a, 
 
 __global__ void __launch_bounds__(256) k_eam_density(Atoms A, Box bx, const EAMTable* __restrict__ T,
                                                      const signed char* __restrict__ ek, double2* __restrict__ fe,
-                                                     double* __restrict__ eatom) {
+                                                     double* __restrict__ eatom, int* __restrict__ row,
+                                                     int* __restrict__ rcnt, unsigned* err) {
   constexpr int LW = 16;
-  const int sl = threadIdx.x % LW;
+  const int lane = threadIdx.x & 63;
+  const int sl = lane % LW;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const unsigned long long mine = ((1ull << LW) - 1ull) << (lane / LW * LW);
   const int i = blockIdx.x * (blockDim.x / LW) + threadIdx.x / LW;
   if (i >= A.n) return;  // group-uniform
   const int ei = ek[i];
   if (ei < 0) {
-    if (sl == 0) fe[i] = make_double2(0.0, -1.0);
+    if (sl == 0) {
+      fe[i] = make_double2(0.0, -1.0);
+      rcnt[i] = 0;
+    }
     return;
   }
   const int cnt = A.nll_cnt[i];
   const int* L = A.nll + (size_t)i * A.cap_l;
+  int* my = row + (size_t)i * kEamRow;
   const double4 pi = A.p[i];
   const double* Lb = box_of(bx, i);
   const double rmax = T->rmax;
   double rho = 0.0;
-  for (int q = sl; q < cnt; q += LW) {
-    const int j = L[q];
-    const double4 pj = A.p[j];
-    if ((int)pj.w != T_OTHER) continue;  // the metals are the atoms of type T_OTHER
-    const double dx = mic(pi.x - pj.x, Lb[0]), dy = mic(pi.y - pj.y, Lb[1]), dz = mic(pi.z - pj.z, Lb[2]);
-    const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    if (r >= rmax) continue;
-    double g, dg;
-    eam_g(T, ei, ek[j], r, g, dg);
-    rho += g;
+  int m = 0;
+  for (int b0 = 0; b0 < cnt; b0 += LW) {  // group-uniform trip count
+    const int q = b0 + sl;
+    bool in = false;
+    int j = 0;
+    if (q < cnt) {
+      j = L[q];
+      const double4 pj = A.p[j];
+      if ((int)pj.w == T_OTHER) {  // the metals are the atoms of type T_OTHER
+        const double dx = mic(pi.x - pj.x, Lb[0]), dy = mic(pi.y - pj.y, Lb[1]), dz = mic(pi.z - pj.z, Lb[2]);
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        if (r < rmax) {
+          in = true;
+          double g, dg;
+          eam_g(T, ei, ek[j], r, g, dg);
+          rho += g;
+        }
+      }
+    }
+    // the in-range partners, in list order, for the force pass
+    const unsigned long long msk = __ballot(in) & mine;
+    if (in) {
+      const int slot = m + __popcll(msk & below);
+      if (slot < kEamRow) my[slot] = j;
+    }
+    m += __popcll(msk);
   }
 #pragma unroll
   for (int o = LW / 2; o > 0; o >>= 1) rho += __shfl_xor(rho, o, LW);
@@ -654,12 +678,15 @@ __global__ void __launch_bounds__(256) k_eam_density(Atoms A, Box bx, const EAMT
     const double F = -sqrt(rho);
     fe[i] = make_double2((rho != 0.0) ? 0.5 / F : 0.0, double(ei));
     eatom[i] = F;
+    if (m > kEamRow) raise(err, ERR_EAM_ROW_OVERFLOW);
+    rcnt[i] = m < kEamRow ? m : kEamRow;
   }
 }
 
 __global__ void __launch_bounds__(256) k_eam_force(Atoms A, Box bx, const EAMTable* __restrict__ T,
                                                    const signed char* __restrict__ ek,
-                                                   const double2* __restrict__ fe, double* __restrict__ grad,
+                                                   const double2* __restrict__ fe, const int* __restrict__ row,
+                                                   const int* __restrict__ rcnt, double* __restrict__ grad,
                                                    double* __restrict__ eatom) {
   constexpr int LW = 16;
   const int sl = threadIdx.x % LW;
@@ -667,20 +694,18 @@ __global__ void __launch_bounds__(256) k_eam_force(Atoms A, Box bx, const EAMTab
   if (i >= A.n) return;  // group-uniform
   const int ei = ek[i];
   if (ei < 0) return;
-  const int cnt = A.nll_cnt[i];
-  const int* L = A.nll + (size_t)i * A.cap_l;
+  const int cnt = rcnt[i];
+  const int* L = row + (size_t)i * kEamRow;
   const double4 pi = A.p[i];
   const double* Lb = box_of(bx, i);
-  const double rmax = T->rmax, fpi = fe[i].x, Zi = T->Z[ei];
+  const double fpi = fe[i].x, Zi = T->Z[ei];
   double gx = 0.0, gy = 0.0, gz = 0.0, ev = 0.0;
   for (int q = sl; q < cnt; q += LW) {
     const int j = L[q];
     const double4 pj = A.p[j];
-    if ((int)pj.w != T_OTHER) continue;
+    const double2 fj = fe[j];  // F'(rho_j) and j's element
     const double dx = mic(pi.x - pj.x, Lb[0]), dy = mic(pi.y - pj.y, Lb[1]), dz = mic(pi.z - pj.z, Lb[2]);
     const double r = sqrt(dx * dx + dy * dy + dz * dz);
-    if (r >= rmax) continue;
-    const double2 fj = fe[j];  // F'(rho_j) and j's element
     const int ej = (int)fj.y;
     double v, dv, g, dg;
     eam_pair(T, ei, ej, Zi, T->Z[ej], r, v, dv);

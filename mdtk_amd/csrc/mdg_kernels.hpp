@@ -133,9 +133,13 @@ __global__ void k_flj(Atoms A, Box bx, const FLJTable* T, double* grad, double* 
 // force pass (gradient and pair energies); 16 lanes per atom, blocks of 256.
 // ek: per atom the EAM element index (Cu 0, Ag 1, Au 2) or -1; fe: per atom
 // (F'(rho), element index) written by the density pass.
-__global__ void k_eam_density(Atoms A, Box bx, const EAMTable* T, const signed char* ek, double2* fe, double* eatom);
+// row/rcnt: per atom its partners inside the EAM range this step (kEamRow
+// slots each), written by the density pass and read by the force pass.
+constexpr int kEamRow = 128;
+__global__ void k_eam_density(Atoms A, Box bx, const EAMTable* T, const signed char* ek, double2* fe, double* eatom,
+                              int* row, int* rcnt, unsigned* err);
 __global__ void k_eam_force(Atoms A, Box bx, const EAMTable* T, const signed char* ek, const double2* fe,
-                            double* grad, double* eatom);
+                            const int* row, const int* rcnt, double* grad, double* eatom);
 __global__ void k_brenner_bonds(Atoms A, const Tables* T, Box bx, const int2* bonds, const int* nbonds,
                                 double* grad, double* eatom, unsigned* err);
 // Verlet halves over the reduction tiles: the drift writes the skin test's
