@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
 POT_FBZL, POT_BRENNER, POT_REBO, POT_AIREBO, POT_ETORS, POT_FLJ, POT_ACKLAND = 1, 2, 4, 8, 16, 32, 64
+POT_FBM, POT_TB = 128, 256
 NUM_PARAMS = 33  # REBO pair {Q, A, alpha, B1-3, beta1-3} x (CC, CH, HH), then LJ {eps, sigma} x 3
 _lib = None
 

@@ -13,7 +13,8 @@ using namespace orc;
 namespace {
 thread_local std::string g_err;
 
-enum : int { POT_FBZL = 1, POT_BRENNER = 2, POT_REBO = 4, POT_AIREBO = 8, POT_ETORS = 16, POT_FLJ = 32, POT_ACKLAND = 64 };
+enum : int { POT_FBZL = 1, POT_BRENNER = 2, POT_REBO = 4, POT_AIREBO = 8, POT_ETORS = 16, POT_FLJ = 32, POT_ACKLAND = 64,
+            POT_FBM = 128, POT_TB = 256 };
 
 struct Handle {
   SimLoop sl;
@@ -90,6 +91,16 @@ void* orc_create(int n, const int* elem, const double* xyz, const double* vel, c
     if (pot_mask & POT_ACKLAND) {  // common.h:67: Ackland last
       pots.push_back(new Ackland());
       h->kinds.push_back(POT_ACKLAND);
+    }
+    // FBM and TightBinding are not registered by common.h (:69-72); selected,
+    // they follow the chain in this order.
+    if (pot_mask & POT_FBM) {  // FPairwise's default Rcutoff (FPairwise.hpp:34): 3.1-3.5 A
+      pots.push_back(new FBM(3.1 * Ao, 3.5 * Ao));
+      h->kinds.push_back(POT_FBM);
+    }
+    if (pot_mask & POT_TB) {
+      pots.push_back(new TightBinding());
+      h->kinds.push_back(POT_TB);
     }
     if (pots.empty()) throw OracleError("no potentials selected");
   });

@@ -398,6 +398,37 @@ struct Ackland : Potential {
   double compute(AtomsArray& atoms) override;
 };
 
+// mdtk/potentials/pairwise/FBM.{hpp,cxx}: Born-Mayer A exp(-b r) for Cu-Cu
+// and Cu-Ar, times the cosine cutoff (R0, R1); not in mdtrajsim's chain.
+struct FBM : Potential {
+  double R0, R1;
+  FBM(double r0, double r1);
+  double getRcutoff() const override { return R1; }
+  bool isHandledPair(const Atom& a, const Atom& b) const;
+  void params(const Atom& a, const Atom& b, double& A, double& B) const;
+  double F11(Pair& ij, double V = 0.0);
+  double compute(AtomsArray& atoms) override;
+};
+
+// mdtk/potentials/manybody/TightBinding/TightBinding.{hpp,cxx}: Cu second-
+// moment tight binding, E = sum_i -c sqrt(rho_i) + sum_{i<j} V, rho_i =
+// sum_j f(r) exp(-beta r), V = Born-Mayer below 1.0 A, a cubic bridge to
+// 1.2 A, f(r) Phi0 exp(-alpha r) beyond; f the cosine cutoff 5.0-5.5 A.
+// Not in mdtrajsim's chain.
+struct TightBinding : Potential {
+  double alpha_, beta_, c_, Phi0_, R_[2], BM_A, BM_B;
+  double x_[2] = {1.0 * Ao, 1.2 * Ao};
+  double v_[2], d_[2];  // bridge ends: values and slopes
+  TightBinding();
+  double getRcutoff() const override { return R_[1]; }
+  bool near(const Atom& a, const Atom& b) const { return !(norm2(depos(a, b)) > SQR(R_[1])); }
+  double Phi(Pair& ij);
+  double g(Pair& ij, double V = 0.0);
+  double rho(Atom& a, AtomsArray& atoms, double V = 0.0);
+  double F(Atom& a, AtomsArray& atoms);
+  double compute(AtomsArray& atoms) override;
+};
+
 // mdtk/potentials/FProxy.{hpp,cxx} + the NL update in NeighbourList.cxx:31-106
 struct Proxy {
   std::vector<Potential*> pots;

@@ -1462,6 +1462,134 @@ double Ackland::compute(AtomsArray& atoms) {  // Ackland.cxx:31-57
   return Ei;
 }
 
+// =================================================================== FBM
+FBM::FBM(double r0, double r1) : R0(r0), R1(r1) {  // FBM.cxx:34-53
+  handled = {Cu_EL, Ar_EL};
+  nl.Rcutoff = getRcutoff();
+}
+
+bool FBM::isHandledPair(const Atom& a, const Atom& b) const {  // Cu-Cu, Cu-Ar, Ar-Cu
+  return (a.ID == Cu_EL && (b.ID == Cu_EL || b.ID == Ar_EL)) || (a.ID == Ar_EL && b.ID == Cu_EL);
+}
+
+void FBM::params(const Atom& a, const Atom& b, double& A, double& B) const {  // FBM.cxx:43-52
+  bool ar = a.ID == Ar_EL || b.ID == Ar_EL;
+  A = (ar ? 59.874 : 22.565) * 1000.0 * eV;
+  B = (ar ? 72.0 : 50.88) / (10.0 * Ao);
+}
+
+double FBM::F11(Pair& ij, double V) {  // FBM.cxx:55-68
+  double A, B;
+  params(*ij.a1, *ij.a2, A, B);
+  double R = ij.r();
+  if (V != 0.0) ij.r(-A * B * std::exp(-B * R) * V);
+  return A * std::exp(-B * R);
+}
+
+double FBM::compute(AtomsArray& atoms) {  // FBM.cxx:70-94
+  double Ei = 0;
+  for (auto& a : atoms) {
+    for (int j : NL(a)) {
+      Atom& aj = atoms[j];
+      if (a.globalIndex > aj.globalIndex) continue;
+      if (!isHandledPair(a, aj)) continue;
+      if (&a == &aj) continue;
+      if (norm2(depos(a, aj)) > SQR(R1)) continue;  // FPairwise::probablyAreNeighbours
+      Pair ij(a, aj, R0, R1);
+      double f = ij.f();
+      double v = F11(ij, f);
+      ij.f(v);
+      Ei += v * f;
+    }
+  }
+  return Ei;
+}
+
+// =================================================================== TightBinding
+TightBinding::TightBinding() {  // TightBinding.cxx:161-189
+  handled = {Cu_EL};
+  alpha_ = 42.87 / (10.0 * Ao);
+  beta_ = 18.00 / (10.0 * Ao);
+  c_ = 12.17 * eV;
+  Phi0_ = 9.892 * 1000.0 * eV;
+  R_[0] = 5.0 * Ao;
+  R_[1] = 5.5 * Ao;
+  BM_A = 22.565 * 1000.0 * eV;
+  BM_B = 50.88 / (10.0 * Ao);
+  nl.Rcutoff = getRcutoff();
+  // bridge ends (fillR_concat_, TightBinding.cxx:191-231): Born-Mayer at 1.0 A,
+  // Phi0 exp(-alpha r) at 1.2 A
+  v_[0] = BM_A * std::exp(-BM_B * x_[0]);
+  d_[0] = -BM_B * BM_A * std::exp(-BM_B * x_[0]);
+  v_[1] = Phi0_ * std::exp(-alpha_ * x_[1]);
+  d_[1] = -alpha_ * Phi0_ * std::exp(-alpha_ * x_[1]);
+}
+
+double TightBinding::Phi(Pair& ij) {  // TightBinding.cxx:59-102
+  double fvar = ij.f();
+  if (fvar == 0.0) return 0.0;
+  double r = ij.r();
+  if (r < x_[0]) {
+    ij.r(-BM_B * BM_A * std::exp(-BM_B * r));
+    return BM_A * std::exp(-BM_B * r);
+  }
+  if (r < x_[1]) {
+    double v, d;
+    cubic_bridge(x_, v_, d_, r, v, d);
+    ij.r(d);
+    return v;
+  }
+  double Val = Phi0_ * std::exp(-alpha_ * r);
+  ij.r(Val * (-alpha_) * fvar);
+  ij.f(Val);
+  return fvar * Val;
+}
+
+double TightBinding::g(Pair& ij, double V) {  // TightBinding.cxx:105-130
+  double fvar = ij.f();
+  if (fvar == 0.0) return 0.0;
+  double r = ij.r();
+  double Val = std::exp(-beta_ * r);
+  ij.r(Val * (-beta_) * fvar * V);
+  ij.f(Val * V);
+  return fvar * Val;
+}
+
+double TightBinding::rho(Atom& ai, AtomsArray& atoms, double V) {  // TightBinding.cxx:145-159
+  double s = 0.0;
+  for (int j : NL(ai)) {
+    Atom& aj = atoms[j];
+    if (!near(ai, aj)) continue;
+    Pair ij(ai, aj, R_[0], R_[1]);
+    s += g(ij, V);
+  }
+  return s;
+}
+
+double TightBinding::F(Atom& ai, AtomsArray& atoms) {  // TightBinding.cxx:133-143
+  double r = rho(ai, atoms);
+  ORC_REQUIRE(r >= 0.0);
+  if (r != 0) rho(ai, atoms, -c_ / (2.0 * std::sqrt(r)));
+  return -c_ * std::sqrt(r);
+}
+
+double TightBinding::compute(AtomsArray& atoms) {  // TightBinding.cxx:30-56
+  double Ei = 0;
+  for (auto& a : atoms) {
+    if (!isHandled(a)) continue;
+    Ei += F(a, atoms);
+    for (int j : NL(a)) {
+      Atom& aj = atoms[j];
+      if (a.globalIndex > aj.globalIndex) continue;
+      if (&a == &aj) continue;
+      if (!near(a, aj)) continue;
+      Pair ij(a, aj, R_[0], R_[1]);
+      Ei += Phi(ij);
+    }
+  }
+  return Ei;
+}
+
 // =================================================================== SimLoop
 void SimLoop::prepare() {  // AtomsContainer.cxx:106-123, SimLoop.cxx:236-250
   ORC_REQUIRE(!atoms.empty());
