@@ -24,7 +24,8 @@ typedef struct mdg_ctx mdg_ctx;
 
 /* Potential chain members, in the order applications/common.h:46-68 adds
  * them: FBZL (ZBL ion-target), FLJ, Brenner (replaces REBO), REBO,
- * AIREBO-LJ (requires REBO), ETors, Ackland. */
+ * AIREBO-LJ (requires REBO), ETors, Ackland; then FBM and TightBinding,
+ * which the reference ships but does not register (common.h:69-72). */
 enum {
   MDG_POT_FBZL = 1,
   MDG_POT_BRENNER = 2,
@@ -32,7 +33,9 @@ enum {
   MDG_POT_AIREBO = 8,
   MDG_POT_ETORS = 16,
   MDG_POT_FLJ = 32,     /* FLJ (pairwise/FLJ.cxx): {H,C} x {Cu,Au}, chained after FBZL (common.h:49) */
-  MDG_POT_ACKLAND = 64  /* Ackland EAM (manybody/Ackland/Ackland.cxx): Cu, Ag, Au; last in the chain (common.h:67); FP64 in both precisions */
+  MDG_POT_ACKLAND = 64, /* Ackland EAM (manybody/Ackland/Ackland.cxx): Cu, Ag, Au; last in the chain (common.h:67); FP64 in both precisions */
+  MDG_POT_FBM = 128,    /* FBM (pairwise/FBM.cxx): Born-Mayer Cu-Cu, Cu-Ar, cutoff 3.1-3.5 A; not in common.h's chain, follows it; FP64 */
+  MDG_POT_TB = 256      /* TightBinding (manybody/TightBinding/TightBinding.cxx): Cu second moment; not in common.h's chain, follows FBM; FP64 */
 };
 
 enum { MDG_PREC_FP64 = 0, MDG_PREC_MIXED = 1 };

@@ -40,17 +40,25 @@ struct mdg_ctx {
   mdg_config cfg{};
   std::string err;
   std::vector<int> kinds;
-  int slot_fbzl = -1, slot_flj = -1, slot_rebo = -1, slot_bren = -1, slot_lj = -1, slot_tors = -1, slot_eam = -1;
+  int slot_fbzl = -1, slot_flj = -1, slot_rebo = -1, slot_bren = -1, slot_lj = -1, slot_tors = -1, slot_eam = -1,
+      slot_fbm = -1, slot_tb = -1;
   FLJTable* dFLJ = nullptr;  // FLJ parameters (device)
   EAMTable* dEAM = nullptr;  // Ackland parameters (device)
   int2* rb = nullptr;          // reduction tiles {system, first atom}
   int* rb_off = nullptr;       // per system: its first tile (nsys + 1)
   double* red_part = nullptr;  // 8 partial results per tile
   int nred = 0;
-  signed char* eam_e = nullptr;  // per atom: Ackland element index (Cu 0, Ag 1, Au 2) or -1
-  double2* eam_fe = nullptr;      // per atom: F'(rho) of the Ackland density pass and the element index
-  int* eam_row = nullptr;         // per atom: partners inside the Ackland range this step (kEamRow slots)
-  int* eam_cnt = nullptr;
+  // per embedded-atom member (0 Ackland, 1 TightBinding): per atom its element
+  // index or -1, (F'(rho), element) of the density pass, and its partners
+  // inside the range this step (kEamRow slots + count)
+  struct EmbedSet {
+    signed char* e = nullptr;
+    double2* fe = nullptr;
+    int* row = nullptr;
+    int* cnt = nullptr;
+  } emb[2];
+  TBTable* dTB = nullptr;    // TightBinding parameters (device)
+  FBMTable* dFBM = nullptr;  // FBM parameters (device)
   bool have_short = false, have_long = false;
   double rc_short = 0, rc_long = 0;
 
@@ -135,6 +143,8 @@ struct mdg_ctx {
     if (dTf) hipFree(dTf);
     if (dFLJ) hipFree(dFLJ);
     if (dEAM) hipFree(dEAM);
+    if (dTB) hipFree(dTB);
+    if (dFBM) hipFree(dFBM);
     if (own) hipStreamDestroy(own);
     if (side) hipStreamDestroy(side);
     if (ev_fork) hipEventDestroy(ev_fork);
@@ -577,13 +587,31 @@ void force_call(mdg_ctx* c) {
   }
   if (c->slot_eam >= 0) {
     double* ea = c->eatom + (size_t)c->slot_eam * n;
+    const auto& E = c->emb[0];
     launch(c, "k_eam_density", [&] {
-      k_eam_density<<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dEAM, c->eam_e, c->eam_fe, ea, c->eam_row,
-                                                          c->eam_cnt, c->derr);
+      k_eam_density<AcklandModel><<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dEAM, E.e, E.fe, ea, E.row,
+                                                                        E.cnt, c->derr);
     });
     launch(c, "k_eam_force", [&] {
-      k_eam_force<<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dEAM, c->eam_e, c->eam_fe, c->eam_row,
-                                                        c->eam_cnt, A.grad, ea);
+      k_eam_force<AcklandModel><<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dEAM, E.e, E.fe, E.row, E.cnt,
+                                                                      A.grad, ea);
+    });
+  }
+  if (c->slot_fbm >= 0) {
+    launch(c, "k_fbm", [&] {
+      k_fbm<<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dFBM, A.grad, c->eatom + (size_t)c->slot_fbm * n);
+    });
+  }
+  if (c->slot_tb >= 0) {
+    double* ea = c->eatom + (size_t)c->slot_tb * n;
+    const auto& E = c->emb[1];
+    launch(c, "k_tb_density", [&] {
+      k_eam_density<TBModel><<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dTB, E.e, E.fe, ea, E.row, E.cnt,
+                                                                   c->derr);
+    });
+    launch(c, "k_tb_force", [&] {
+      k_eam_force<TBModel><<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dTB, E.e, E.fe, E.row, E.cnt,
+                                                                 A.grad, ea);
     });
   }
   if (c->slot_fbzl >= 0) {
@@ -615,7 +643,8 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
     g_create_err = "Brenner replaces REBO: select one of them";
     return 1;
   }
-  if (!(m & (MDG_POT_REBO | MDG_POT_BRENNER | MDG_POT_ETORS | MDG_POT_FBZL | MDG_POT_FLJ | MDG_POT_ACKLAND))) {
+  if (!(m & (MDG_POT_REBO | MDG_POT_BRENNER | MDG_POT_ETORS | MDG_POT_FBZL | MDG_POT_FLJ | MDG_POT_ACKLAND |
+             MDG_POT_FBM | MDG_POT_TB))) {
     g_create_err = "no potentials selected";
     return 1;
   }
@@ -654,8 +683,11 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
   if (m & MDG_POT_AIREBO) { c->slot_lj = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_AIREBO); }
   if (m & MDG_POT_ETORS) { c->slot_tors = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_ETORS); }
   if (m & MDG_POT_ACKLAND) { c->slot_eam = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_ACKLAND); }
+  // FBM and TightBinding are not registered by common.h (:69-72); selected, they follow the chain
+  if (m & MDG_POT_FBM) { c->slot_fbm = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_FBM); }
+  if (m & MDG_POT_TB) { c->slot_tb = (int)c->kinds.size(); c->kinds.push_back(MDG_POT_TB); }
   c->have_short = (m & (MDG_POT_REBO | MDG_POT_BRENNER | MDG_POT_ETORS)) != 0;
-  c->have_long = (m & (MDG_POT_AIREBO | MDG_POT_FBZL | MDG_POT_FLJ | MDG_POT_ACKLAND)) != 0;
+  c->have_long = (m & (MDG_POT_AIREBO | MDG_POT_FBZL | MDG_POT_FLJ | MDG_POT_ACKLAND | MDG_POT_FBM | MDG_POT_TB)) != 0;
   c->rc_short = 2.0 * kAo;  // REBO/Brenner/ETors getRcutoff (REBO.hpp:130)
   c->rc_long = 0;
   if (m & MDG_POT_AIREBO) c->rc_long = 5.5 * kAo;  // AIREBO_LJ.hpp:81
@@ -665,6 +697,12 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
   build_eam_table(et);
   // Ackland::getRcutoff (Ackland.hpp:58): the largest like-pair first knot
   if (m & MDG_POT_ACKLAND) c->rc_long = std::max(c->rc_long, std::max(et.rk[0][0][1], std::max(et.rk[1][1][1], et.rk[2][2][1])));
+  TBTable tt;
+  build_tb_table(tt);
+  FBMTable ft2;
+  build_fbm_table(ft2);
+  if (m & MDG_POT_FBM) c->rc_long = std::max(c->rc_long, ft2.R1);  // FPairwise::getRcutoff
+  if (m & MDG_POT_TB) c->rc_long = std::max(c->rc_long, tt.R1);    // TightBinding.hpp:55
   if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
     g_create_err = "hipStreamCreate failed";
     delete c;
@@ -708,8 +746,12 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
       return 1;
     }
   }
-  if (c->slot_eam >= 0 && (hipMalloc(&c->dEAM, sizeof(et)) != hipSuccess ||
-                           hipMemcpy(c->dEAM, &et, sizeof(et), hipMemcpyHostToDevice) != hipSuccess)) {
+  if ((c->slot_eam >= 0 && (hipMalloc(&c->dEAM, sizeof(et)) != hipSuccess ||
+                            hipMemcpy(c->dEAM, &et, sizeof(et), hipMemcpyHostToDevice) != hipSuccess)) ||
+      (c->slot_tb >= 0 && (hipMalloc(&c->dTB, sizeof(tt)) != hipSuccess ||
+                           hipMemcpy(c->dTB, &tt, sizeof(tt), hipMemcpyHostToDevice) != hipSuccess)) ||
+      (c->slot_fbm >= 0 && (hipMalloc(&c->dFBM, sizeof(ft2)) != hipSuccess ||
+                            hipMemcpy(c->dFBM, &ft2, sizeof(ft2), hipMemcpyHostToDevice) != hipSuccess))) {
     g_create_err = "table upload failed";
     delete c;
     return 1;
@@ -876,13 +918,12 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
       HIPCHK(c, hipMemcpy(c->rb_off, ro.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice));
     }
   }
-  c->eam_e = nullptr;
-  c->eam_fe = nullptr;
-  c->eam_row = nullptr;
-  c->eam_cnt = nullptr;
-  if (c->slot_eam >= 0)
-    r |= dalloc(c, c->eam_e, n) | dalloc(c, c->eam_fe, n) | dalloc(c, c->eam_row, (size_t)n * kEamRow) |
-         dalloc(c, c->eam_cnt, n);
+  for (int k = 0; k < 2; k++) {
+    auto& E = c->emb[k];
+    E = mdg_ctx::EmbedSet{};
+    if ((k == 0 ? c->slot_eam : c->slot_tb) >= 0)
+      r |= dalloc(c, E.e, n) | dalloc(c, E.fe, n) | dalloc(c, E.row, (size_t)n * kEamRow) | dalloc(c, E.cnt, n);
+  }
   if (r) return 1;
   // cell grids, one per system and list
   auto grids = [&](double rc, CellGrid*& gd, int*& coff, int& ncell) -> int {
@@ -962,11 +1003,13 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   HIPCHK(c, hipMemset(A.bcnt, 0, (size_t)n * 4));
   HIPCHK(c, up(c->handled_s, hs.data(), n));
   HIPCHK(c, up(c->handled_l, hl.data(), n));
-  if (c->eam_e) {
-    std::vector<signed char> ek(n);
-    for (int i = 0; i < n; i++) ek[i] = el[i] == 64 ? 0 : el[i] == 108 ? 1 : el[i] == 197 ? 2 : -1;
-    HIPCHK(c, up(c->eam_e, ek.data(), n));
-  }
+  for (int k = 0; k < 2; k++)
+    if (c->emb[k].e) {  // Ackland: Cu 0, Ag 1, Au 2; TightBinding: Cu 0
+      std::vector<signed char> ek(n);
+      for (int i = 0; i < n; i++)
+        ek[i] = el[i] == 64 ? 0 : (k == 0 && el[i] == 108) ? 1 : (k == 0 && el[i] == 197) ? 2 : -1;
+      HIPCHK(c, up(c->emb[k].e, ek.data(), n));
+    }
   int fl[3] = {1, 1, 1};
   std::vector<int> sfl(2 * (size_t)nsys, 1);
   HIPCHK(c, up(c->flags, fl, sizeof(fl)));
@@ -1146,6 +1189,8 @@ int mdg_nl_export(mdg_ctx* c, int term, int32_t* counts, int32_t* idx, int64_t c
   auto zbl_handled = [](int e) { return e == 1 || e == 12 || e == 64 || e == 197 || e == 108 || e == 40 || e == 131; };
   auto flj_handled = [](int e) { return e == 1 || e == 12 || e == 64 || e == 197; };  // FLJ.cxx:38-52
   auto eam_handled = [](int e) { return e == 64 || e == 108 || e == 197; };             // Ackland.cxx:236-262
+  auto fbm_handled = [](int e) { return e == 64 || e == 40; };                          // FBM.cxx:37-38
+  auto tb_handled = [](int e) { return e == 64; };                                      // TightBinding.cxx:163
   int64_t pos = 0;
   for (int i = 0; i < n; i++) {
     int m = 0;
@@ -1154,6 +1199,8 @@ int mdg_nl_export(mdg_ctx* c, int term, int32_t* counts, int32_t* idx, int64_t c
     if (kind == MDG_POT_FBZL) row_ok = zbl_handled(el[i]);
     if (kind == MDG_POT_FLJ) row_ok = flj_handled(el[i]);
     if (kind == MDG_POT_ACKLAND) row_ok = eam_handled(el[i]);
+    if (kind == MDG_POT_FBM) row_ok = fbm_handled(el[i]);
+    if (kind == MDG_POT_TB) row_ok = tb_handled(el[i]);
     if (row_ok)
       for (int q = 0; q < cnt[i]; q++) {
         int j = lst[(size_t)i * stride + q];
@@ -1161,6 +1208,8 @@ int mdg_nl_export(mdg_ctx* c, int term, int32_t* counts, int32_t* idx, int64_t c
         if (kind == MDG_POT_FBZL && !zbl_handled(el[j])) continue;
         if (kind == MDG_POT_FLJ && !flj_handled(el[j])) continue;
         if (kind == MDG_POT_ACKLAND && !eam_handled(el[j])) continue;
+        if (kind == MDG_POT_FBM && !fbm_handled(el[j])) continue;
+        if (kind == MDG_POT_TB && !tb_handled(el[j])) continue;
         if (pos >= cap) {
           c->err = "nl_export capacity";
           return 1;
@@ -1344,14 +1393,16 @@ int mdg_set_positions(mdg_ctx* c, const double* xyz, const int32_t* pbc_count, c
 int mdg_neighbor_counts(mdg_ctx* c, int32_t* counts) {
   int n = c->n;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  std::vector<int> s(n, 0), l(n, 0), ty(n);
+  std::vector<int> s(n, 0), l(n, 0), ty(n), el(n);
   if (c->have_short) HIPCHK(c, hipMemcpy(s.data(), c->A.nls_cnt, n * 4, hipMemcpyDeviceToHost));
   if (c->have_long) HIPCHK(c, hipMemcpy(l.data(), c->A.nll_cnt, n * 4, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(ty.data(), c->A.type, n * 4, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(el.data(), c->A.elem, n * 4, hipMemcpyDeviceToHost));
   for (int i = 0; i < n; i++) {
     bool long_handled = (c->slot_fbzl >= 0) || (c->slot_lj >= 0 && ty[i] <= T_H) ||
                         (c->slot_flj >= 0 && ty[i] != T_ION) ||  // FLJ handles H, C, Cu, Au
-                        (c->slot_eam >= 0 && ty[i] == T_OTHER);  // Ackland handles Cu, Ag, Au
+                        (c->slot_eam >= 0 && ty[i] == T_OTHER) ||  // Ackland handles Cu, Ag, Au
+                        (c->slot_fbm >= 0 && (el[i] == 64 || el[i] == 40)) || (c->slot_tb >= 0 && el[i] == 64);
     counts[i] = s[i] + (long_handled ? l[i] : 0);
   }
   return 0;

@@ -114,6 +114,24 @@ struct EAMTable {
 };
 void build_eam_table(EAMTable& t);
 
+// TightBinding (manybody/TightBinding/TightBinding.cxx), Cu only: density
+// pair function f(r) exp(-beta r); pair term Born-Mayer A exp(-B r) below
+// x[0], the cubic through both ends' values and slopes to x[1], f(r) Phi0
+// exp(-alpha r) beyond; f the cosine cutoff R0..R1; embedding -c sqrt(rho).
+struct TBTable {
+  double alpha, beta, c, Phi0, R0, R1, A, B;
+  double x[2], v[2], d[2];
+};
+void build_tb_table(TBTable& t);
+
+// FBM (pairwise/FBM.cxx): A exp(-B r) times the cosine cutoff R0..R1 for
+// Cu-Cu and Cu-Ar (index Cu = 0, Ar = 1; Ar-Ar is not a pair of it).
+struct FBMTable {
+  double A[2][2], B[2][2];
+  double R0, R1;
+};
+void build_fbm_table(FBMTable& t);
+
 using Tables = TablesT<double>;
 using ReboParams = ReboParamsT<double>;
 using LJParams = LJParamsT<double>;

@@ -619,7 +619,58 @@ __device__ __forceinline__ void eam_pair(const EAMTable* __restrict__ T, int a, 
   }
 }
 
-__global__ void __launch_bounds__(256) k_eam_density(Atoms A, Box bx, const EAMTable* __restrict__ T,
+__device__ __forceinline__ double AcklandModel::rmax(const Table* T) { return T->rmax; }
+__device__ __forceinline__ void AcklandModel::g(const Table* T, int a, int b, double r, double& g, double& dg) {
+  eam_g(T, a, b, r, g, dg);
+}
+__device__ __forceinline__ void AcklandModel::pair(const Table* T, int a, int b, double r, double& v, double& dv) {
+  eam_pair(T, a, b, T->Z[a], T->Z[b], r, v, dv);
+}
+__device__ __forceinline__ void AcklandModel::embed(const Table*, double rho, double& F, double& Fp) {
+  F = -sqrt(rho);
+  Fp = (rho != 0.0) ? 0.5 / F : 0.0;
+}
+
+__device__ __forceinline__ double TBModel::rmax(const Table* T) { return T->R1; }
+// g = f(r) exp(-beta r) (TightBinding.cxx:105-130)
+__device__ __forceinline__ void TBModel::g(const Table* T, int, int, double r, double& g, double& dg) {
+  double f, df;
+  cutoff(r, T->R0, T->R1, f, df);
+  const double e = exp(-T->beta * r);
+  g = f * e;
+  dg = e * (-T->beta) * f + df * e;
+}
+// V (TightBinding.cxx:59-102): 0 where f is 0; Born-Mayer, the bridge, then
+// f Phi0 exp(-alpha r)
+__device__ __forceinline__ void TBModel::pair(const Table* T, int, int, double r, double& v, double& dv) {
+  double f, df;
+  cutoff(r, T->R0, T->R1, f, df);
+  v = 0.0;
+  dv = 0.0;
+  if (f == 0.0) return;
+  if (r < T->x[0]) {
+    v = T->A * exp(-T->B * r);
+    dv = -T->B * T->A * exp(-T->B * r);
+  } else if (r < T->x[1]) {  // the cubic through both ends' values and slopes (Spline.cxx:72-98)
+    const double hh = T->x[1] - T->x[0], t = (r - T->x[0]) / hh;
+    v = (1 + 2 * t) * (1 - t) * (1 - t) * T->v[0] + t * (1 - t) * (1 - t) * hh * T->d[0] +
+        t * t * (3 - 2 * t) * T->v[1] + t * t * (t - 1) * hh * T->d[1];
+    dv = (6 * t * (t - 1) * T->v[0] - 6 * t * (t - 1) * T->v[1]) / hh + (1 - t) * (1 - 3 * t) * T->d[0] +
+         t * (3 * t - 2) * T->d[1];
+  } else {
+    const double e = T->Phi0 * exp(-T->alpha * r);
+    v = f * e;
+    dv = e * (-T->alpha) * f + df * e;
+  }
+}
+// F = -c sqrt(rho) (TightBinding.cxx:133-143)
+__device__ __forceinline__ void TBModel::embed(const Table* T, double rho, double& F, double& Fp) {
+  F = -T->c * sqrt(rho);
+  Fp = (rho != 0.0) ? -T->c / (2.0 * sqrt(rho)) : 0.0;
+}
+
+template <class M>
+__global__ void __launch_bounds__(256) k_eam_density(Atoms A, Box bx, const typename M::Table* __restrict__ T,
                                                      const signed char* __restrict__ ek, double2* __restrict__ fe,
                                                      double* __restrict__ eatom, int* __restrict__ row,
                                                      int* __restrict__ rcnt, unsigned* err) {
@@ -643,7 +694,7 @@ __global__ void __launch_bounds__(256) k_eam_density(Atoms A, Box bx, const EAMT
   int* my = row + (size_t)i * kEamRow;
   const double4 pi = A.p[i];
   const double* Lb = box_of(bx, i);
-  const double rmax = T->rmax;
+  const double rmax = M::rmax(T);
   double rho = 0.0;
   int m = 0;
   for (int b0 = 0; b0 < cnt; b0 += LW) {  // group-uniform trip count
@@ -657,10 +708,13 @@ __global__ void __launch_bounds__(256) k_eam_density(Atoms A, Box bx, const EAMT
         const double dx = mic(pi.x - pj.x, Lb[0]), dy = mic(pi.y - pj.y, Lb[1]), dz = mic(pi.z - pj.z, Lb[2]);
         const double r = sqrt(dx * dx + dy * dy + dz * dz);
         if (r < rmax) {
-          in = true;
-          double g, dg;
-          eam_g(T, ei, ek[j], r, g, dg);
-          rho += g;
+          const int ej = ek[j];
+          if (ej >= 0) {
+            in = true;
+            double g, dg;
+            M::g(T, ei, ej, r, g, dg);
+            rho += g;
+          }
         }
       }
     }
@@ -675,15 +729,17 @@ __global__ void __launch_bounds__(256) k_eam_density(Atoms A, Box bx, const EAMT
 #pragma unroll
   for (int o = LW / 2; o > 0; o >>= 1) rho += __shfl_xor(rho, o, LW);
   if (sl == 0) {
-    const double F = -sqrt(rho);
-    fe[i] = make_double2((rho != 0.0) ? 0.5 / F : 0.0, double(ei));
+    double F, Fp;
+    M::embed(T, rho, F, Fp);
+    fe[i] = make_double2(Fp, double(ei));
     eatom[i] = F;
     if (m > kEamRow) raise(err, ERR_EAM_ROW_OVERFLOW);
     rcnt[i] = m < kEamRow ? m : kEamRow;
   }
 }
 
-__global__ void __launch_bounds__(256) k_eam_force(Atoms A, Box bx, const EAMTable* __restrict__ T,
+template <class M>
+__global__ void __launch_bounds__(256) k_eam_force(Atoms A, Box bx, const typename M::Table* __restrict__ T,
                                                    const signed char* __restrict__ ek,
                                                    const double2* __restrict__ fe, const int* __restrict__ row,
                                                    const int* __restrict__ rcnt, double* __restrict__ grad,
@@ -698,7 +754,7 @@ __global__ void __launch_bounds__(256) k_eam_force(Atoms A, Box bx, const EAMTab
   const int* L = row + (size_t)i * kEamRow;
   const double4 pi = A.p[i];
   const double* Lb = box_of(bx, i);
-  const double fpi = fe[i].x, Zi = T->Z[ei];
+  const double fpi = fe[i].x;
   double gx = 0.0, gy = 0.0, gz = 0.0, ev = 0.0;
   for (int q = sl; q < cnt; q += LW) {
     const int j = L[q];
@@ -708,8 +764,8 @@ __global__ void __launch_bounds__(256) k_eam_force(Atoms A, Box bx, const EAMTab
     const double r = sqrt(dx * dx + dy * dy + dz * dz);
     const int ej = (int)fj.y;
     double v, dv, g, dg;
-    eam_pair(T, ei, ej, Zi, T->Z[ej], r, v, dv);
-    eam_g(T, ei, ej, r, g, dg);
+    M::pair(T, ei, ej, r, v, dv);
+    M::g(T, ei, ej, r, g, dg);
     const double c = (dv + (fpi + fj.x) * dg) / r;
     gx += c * dx;
     gy += c * dy;
@@ -726,6 +782,66 @@ __global__ void __launch_bounds__(256) k_eam_force(Atoms A, Box bx, const EAMTab
   if (sl == 0) {
     atomic_add3(grad, i, D3{gx, gy, gz});
     eatom[i] += ev;
+  }
+}
+
+template __global__ void k_eam_density<AcklandModel>(Atoms, Box, const EAMTable*, const signed char*, double2*,
+                                                     double*, int*, int*, unsigned*);
+template __global__ void k_eam_force<AcklandModel>(Atoms, Box, const EAMTable*, const signed char*, const double2*,
+                                                   const int*, const int*, double*, double*);
+template __global__ void k_eam_density<TBModel>(Atoms, Box, const TBTable*, const signed char*, double2*, double*,
+                                                int*, int*, unsigned*);
+template __global__ void k_eam_force<TBModel>(Atoms, Box, const TBTable*, const signed char*, const double2*,
+                                              const int*, const int*, double*, double*);
+
+// FBM::operator() (FBM.cxx:70-94) for the pairs Cu-Cu and Cu-Ar: 16 lanes
+// per Cu or Ar atom gather its handled partners from the long list (both
+// directions, no scatter to j); the pair's energy goes to its lower-index
+// atom, as the reference's i < j loop sums it.
+__device__ __forceinline__ int fbm_index(int elem) { return elem == 64 ? 0 : elem == 40 ? 1 : -1; }
+
+__global__ void __launch_bounds__(256) k_fbm(Atoms A, Box bx, const FBMTable* __restrict__ T,
+                                             double* __restrict__ grad, double* __restrict__ eatom) {
+  constexpr int LW = 16;
+  const int sl = threadIdx.x % LW;
+  const int i = blockIdx.x * (blockDim.x / LW) + threadIdx.x / LW;
+  if (i >= A.n) return;  // group-uniform
+  const int ei = fbm_index(A.elem[i]);
+  if (ei < 0) return;
+  const int cnt = A.nll_cnt[i];
+  const int* L = A.nll + (size_t)i * A.cap_l;
+  const double4 pi = A.p[i];
+  const double* Lb = box_of(bx, i);
+  const double R0 = T->R0, R1 = T->R1;
+  double gx = 0.0, gy = 0.0, gz = 0.0, ev = 0.0;
+  for (int q = sl; q < cnt; q += LW) {
+    const int j = L[q];
+    const double4 pj = A.p[j];
+    const double dx = mic(pi.x - pj.x, Lb[0]), dy = mic(pi.y - pj.y, Lb[1]), dz = mic(pi.z - pj.z, Lb[2]);
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 > R1 * R1) continue;  // FPairwise::probablyAreNeighbours
+    const int ej = fbm_index(A.elem[j]);
+    if (ej < 0 || (ei == 1 && ej == 1)) continue;  // Cu-Cu and Cu-Ar only
+    const double r = sqrt(d2);
+    double f, df;
+    cutoff(r, R0, R1, f, df);
+    const double e = T->A[ei][ej] * exp(-T->B[ei][ej] * r);
+    const double c = (-T->B[ei][ej] * e * f + e * df) / r;
+    gx += c * dx;
+    gy += c * dy;
+    gz += c * dz;
+    if (j > i) ev += e * f;
+  }
+#pragma unroll
+  for (int o = LW / 2; o > 0; o >>= 1) {
+    gx += __shfl_xor(gx, o, LW);
+    gy += __shfl_xor(gy, o, LW);
+    gz += __shfl_xor(gz, o, LW);
+    ev += __shfl_xor(ev, o, LW);
+  }
+  if (sl == 0) {
+    atomic_add3(grad, i, D3{gx, gy, gz});
+    eatom[i] = ev;
   }
 }
 

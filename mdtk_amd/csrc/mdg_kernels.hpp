@@ -136,10 +136,32 @@ __global__ void k_flj(Atoms A, Box bx, const FLJTable* T, double* grad, double* 
 // row/rcnt: per atom its partners inside the EAM range this step (kEamRow
 // slots each), written by the density pass and read by the force pass.
 constexpr int kEamRow = 128;
-__global__ void k_eam_density(Atoms A, Box bx, const EAMTable* T, const signed char* ek, double2* fe, double* eatom,
-                              int* row, int* rcnt, unsigned* err);
-__global__ void k_eam_force(Atoms A, Box bx, const EAMTable* T, const signed char* ek, const double2* fe,
+// The two embedded-atom forms, as policies of the two passes: the element
+// pair's density function g and slope, its pair term V and slope, and the
+// embedding F(rho) with F'(rho) (0 where rho is 0: F() applies no gradient
+// there).  Members are defined with the kernels.
+struct AcklandModel {  // Ackland.cxx:59-221
+  using Table = EAMTable;
+  static __device__ double rmax(const Table* T);
+  static __device__ void g(const Table* T, int a, int b, double r, double& g, double& dg);
+  static __device__ void pair(const Table* T, int a, int b, double r, double& v, double& dv);
+  static __device__ void embed(const Table* T, double rho, double& F, double& Fp);
+};
+struct TBModel {  // TightBinding.cxx:59-159
+  using Table = TBTable;
+  static __device__ double rmax(const Table* T);
+  static __device__ void g(const Table* T, int a, int b, double r, double& g, double& dg);
+  static __device__ void pair(const Table* T, int a, int b, double r, double& v, double& dv);
+  static __device__ void embed(const Table* T, double rho, double& F, double& Fp);
+};
+template <class M>
+__global__ void k_eam_density(Atoms A, Box bx, const typename M::Table* T, const signed char* ek, double2* fe,
+                              double* eatom, int* row, int* rcnt, unsigned* err);
+template <class M>
+__global__ void k_eam_force(Atoms A, Box bx, const typename M::Table* T, const signed char* ek, const double2* fe,
                             const int* row, const int* rcnt, double* grad, double* eatom);
+// FBM pairs (Cu-Cu, Cu-Ar), 16 lanes per Cu or Ar atom, blocks of 256.
+__global__ void k_fbm(Atoms A, Box bx, const FBMTable* T, double* grad, double* eatom);
 __global__ void k_brenner_bonds(Atoms A, const Tables* T, Box bx, const int2* bonds, const int* nbonds,
                                 double* grad, double* eatom, unsigned* err);
 // Verlet halves over the reduction tiles: the drift writes the skin test's

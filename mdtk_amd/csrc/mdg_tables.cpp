@@ -186,6 +186,40 @@ void build_eam_table(EAMTable& t) {
     }
 }
 
+// TightBinding::setupPotential (TightBinding.cxx:175-189) and the bridge
+// ends of fillR_concat_ (:191-231).
+void build_tb_table(TBTable& t) {
+  std::memset(&t, 0, sizeof(t));
+  const double Ao = kAo, eV = kEV;
+  t.alpha = 42.87 / (10.0 * Ao);
+  t.beta = 18.00 / (10.0 * Ao);
+  t.c = 12.17 * eV;
+  t.Phi0 = 9.892 * 1000.0 * eV;
+  t.R0 = 5.0 * Ao;
+  t.R1 = 5.5 * Ao;
+  t.A = 22.565 * 1000.0 * eV;
+  t.B = 50.88 / (10.0 * Ao);
+  t.x[0] = 1.0 * Ao;
+  t.x[1] = 1.2 * Ao;
+  t.v[0] = t.A * std::exp(-t.B * t.x[0]);
+  t.d[0] = -t.B * t.A * std::exp(-t.B * t.x[0]);
+  t.v[1] = t.Phi0 * std::exp(-t.alpha * t.x[1]);
+  t.d[1] = -t.alpha * t.Phi0 * std::exp(-t.alpha * t.x[1]);
+}
+
+// FBM::FBM (FBM.cxx:34-53) with FPairwise's default cutoff 3.1-3.5 A
+// (FPairwise.hpp:34).
+void build_fbm_table(FBMTable& t) {
+  std::memset(&t, 0, sizeof(t));
+  const double Ao = kAo, eV = kEV;
+  t.A[0][0] = 22.565 * 1000.0 * eV;
+  t.B[0][0] = 50.88 / (10.0 * Ao);
+  t.A[0][1] = t.A[1][0] = 59.874 * 1000.0 * eV;
+  t.B[0][1] = t.B[1][0] = 72.0 / (10.0 * Ao);
+  t.R0 = 3.1 * Ao;
+  t.R1 = 3.5 * Ao;
+}
+
 void build_tables(Tables& T, int brenner_set) {
   std::memset(&T, 0, sizeof(T));
   T.lunit = 1.0;  // the FP64 table works in the reference's cm and erg

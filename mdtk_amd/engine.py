@@ -18,13 +18,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmdg.so")
 
 POT_FBZL, POT_BRENNER, POT_REBO, POT_AIREBO, POT_ETORS, POT_FLJ, POT_ACKLAND = 1, 2, 4, 8, 16, 32, 64
+POT_FBM, POT_TB = 128, 256
 NUM_PARAMS = 33  # REBO pair {Q, A, alpha, B1-3, beta1-3} x (CC, CH, HH), then LJ {eps, sigma} x (CC, CH, HH)
 PARAM_NAMES = [f"{n}_{p}" for p in ("CC", "CH", "HH")
                for n in ("Q", "A", "alpha", "B1", "B2", "B3", "beta1", "beta2", "beta3")] + \
               [f"{n}_{p}" for p in ("CC", "CH", "HH") for n in ("eps", "sigma")]
 PREC_FP64, PREC_MIXED = 0, 1
 _NAMES = {"FBZL": POT_FBZL, "FLJ": POT_FLJ, "Brenner": POT_BRENNER, "REBO": POT_REBO, "AIREBO": POT_AIREBO,
-          "ETors": POT_ETORS, "Ackland": POT_ACKLAND}
+          "ETors": POT_ETORS, "Ackland": POT_ACKLAND, "FBM": POT_FBM, "TightBinding": POT_TB}
 
 
 class EngineError(RuntimeError):
