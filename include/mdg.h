@@ -173,6 +173,18 @@ int mdg_set_thermal_bath(mdg_ctx* ctx, int system, const mdg_thermal_bath* tb);
 int mdg_set_bath_atoms(mdg_ctx* ctx, const uint8_t* apply);
 int mdg_bath_energy(mdg_ctx* ctx, double* out);
 
+/* SimLoop's clock per system (SimLoop.hpp fields simTime, simTimeFinal, dt;
+ * loaded from a state's .time file by SimLoopSaver::load): mdg_set_time
+ * sets them (system -1: every system; dt > 0); a system whose simTime has
+ * reached simTimeFinal stands still in mdg_step_nve -- the loop condition
+ * while (simTime < simTimeFinal) of SimLoop.cxx:252 -- and reports dt_used
+ * 0, so the systems of a batch stop on their own clocks.  Each step adds its
+ * dt to simTime (:405).  After mdg_set_systems: simTime 0, no final time,
+ * dt 1e-20 s (SimLoop.cxx:46-50).  mdg_get_time returns per system
+ * {simTime, simTimeFinal, next dt} (3*nsys values). */
+int mdg_set_time(mdg_ctx* ctx, int system, double sim_time, double sim_time_final, double dt);
+int mdg_get_time(mdg_ctx* ctx, double* out);
+
 /* Current state: replaces reading Atom::coords/V/an/PBC_count. */
 int mdg_get_state(mdg_ctx* ctx, double* xyz, double* vel, double* acc, int32_t* pbc_count);
 

@@ -47,6 +47,8 @@ struct mdg_ctx {
   int2* rb = nullptr;          // reduction tiles {system, first atom}
   int* rb_off = nullptr;       // per system: its first tile (nsys + 1)
   double* red_part = nullptr;  // 8 partial results per tile
+  double* simt = nullptr;      // per system: simulated time (s)
+  double* tfin = nullptr;      // per system: final time (s); a system past it stands still
   int nred = 0;
   // per embedded-atom member (0 Ackland, 1 TightBinding): per atom its element
   // index or -1, (F'(rho), element) of the density pass, and its partners
@@ -912,7 +914,8 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
     }
     ro[nsys] = (int)rb.size();
     c->nred = (int)rb.size();
-    r |= dalloc(c, c->rb, rb.size()) | dalloc(c, c->rb_off, nsys + 1) | dalloc(c, c->red_part, 8 * rb.size());
+    r |= dalloc(c, c->rb, rb.size()) | dalloc(c, c->rb_off, nsys + 1) | dalloc(c, c->red_part, 8 * rb.size()) |
+         dalloc(c, c->simt, nsys) | dalloc(c, c->tfin, nsys);
     if (!r) {
       HIPCHK(c, hipMemcpy(c->rb, rb.data(), rb.size() * sizeof(int2), hipMemcpyHostToDevice));
       HIPCHK(c, hipMemcpy(c->rb_off, ro.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice));
@@ -1016,6 +1019,11 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   HIPCHK(c, up(c->sflags, sfl.data(), sfl.size() * 4));
   HIPCHK(c, up(c->pflags, sfl.data(), (size_t)nsys * 4));
   HIPCHK(c, up(c->dtp, c->dt_host.data(), (size_t)nsys * 8));
+  {  // simTime 0, no final time (SimLoop.cxx:46)
+    std::vector<double> t0(nsys, 0.0), tf(nsys, 1e300);
+    HIPCHK(c, up(c->simt, t0.data(), (size_t)nsys * 8));
+    HIPCHK(c, up(c->tfin, tf.data(), (size_t)nsys * 8));
+  }
   c->iteration = 0;
   return 0;
 }
@@ -1082,6 +1090,47 @@ int mdg_set_bath_atoms(mdg_ctx* c, const uint8_t* apply) {
     for (int i = 0; i < c->n; i++) a[i] = apply[i] ? 1 : 0;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(c->tb_apply_d, a.data(), c->n, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int mdg_set_time(mdg_ctx* c, int system, double sim_time, double sim_time_final, double dt) {
+  if (c->n == 0) {
+    c->err = "no system";
+    return 1;
+  }
+  if (system < -1 || system >= c->nsys) {
+    c->err = "bad system index";
+    return 1;
+  }
+  if (!(dt > 0.0)) {
+    c->err = "dt must be positive";
+    return 1;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int s0 = system < 0 ? 0 : system, s1 = system < 0 ? c->nsys : system + 1;
+  std::vector<double> t(s1 - s0, sim_time), tf(s1 - s0, sim_time_final), d(s1 - s0, dt);
+  HIPCHK(c, hipMemcpy(c->simt + s0, t.data(), t.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->tfin + s0, tf.data(), tf.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->dtp + s0, d.data(), d.size() * 8, hipMemcpyHostToDevice));
+  for (int s = s0; s < s1; s++) c->dt_host[s] = dt;
+  return 0;
+}
+
+int mdg_get_time(mdg_ctx* c, double* out) {
+  if (c->n == 0) {
+    c->err = "no system";
+    return 1;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<double> t(c->nsys), tf(c->nsys), d(c->nsys);
+  HIPCHK(c, hipMemcpy(t.data(), c->simt, c->nsys * 8, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(tf.data(), c->tfin, c->nsys * 8, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(d.data(), c->dtp, c->nsys * 8, hipMemcpyDeviceToHost));
+  for (int s = 0; s < c->nsys; s++) {
+    out[3 * s] = t[s];
+    out[3 * s + 1] = tf[s];
+    out[3 * s + 2] = d[s];
+  }
   return 0;
 }
 
@@ -1257,18 +1306,21 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
         k_bath_temp_part<<<c->nred, 256, 0, c->stream>>>(A, c->box, c->rb, c->red_part);
         k_bath_temp_merge<<<nsys, 64, 0, c->stream>>>(A, c->rb_off, c->red_part);
       });
-    launch(c, "k_drift", [&] { k_drift_tile<<<c->nred, 256, 0, c->stream>>>(A, c->rb, c->dtp, c->red_part); });
+    launch(c, "k_drift", [&] {
+      k_drift_tile<<<c->nred, 256, 0, c->stream>>>(A, c->rb, c->dtp, c->simt, c->tfin, c->red_part);
+    });
     launch(c, "k_sys_disp", [&] {
       k_sys_disp_merge<<<nsys, 64, 0, c->stream>>>(c->rb_off, c->red_part, lim_s, lim_l, c->pair_margin,
                                                    A.disp_p ? 1 : 0, c->sflags, c->pflags, c->flags, c->counters);
     });
     force_call(c);
     launch(c, "k_kick", [&] {
-      k_kick_tile<<<c->nred, 256, 0, c->stream>>>(A, c->box, c->rb, c->dtp, nterms, c->eatom, c->red_part);
+      k_kick_tile<<<c->nred, 256, 0, c->stream>>>(A, c->box, c->rb, c->dtp, c->simt, c->tfin, nterms, c->eatom,
+                                                  c->red_part);
     });
     launch(c, "k_sys_step_end", [&] {
       k_sys_step_merge<<<nsys, 64, 0, c->stream>>>(c->rb_off, c->red_part, c->dtp, fixed, c->stepout, c->stepidx,
-                                                   nsys, A.tb_dE ? c->ebath : nullptr);
+                                                   nsys, A.tb_dE ? c->ebath : nullptr, c->simt, c->tfin);
     });
     k_step_advance<<<1, 64, 0, c->stream>>>(c->stepidx);
   };

@@ -850,7 +850,8 @@ __global__ void __launch_bounds__(256) k_fbm(Atoms A, Box bx, const FBMTable* __
 // accumulated into both neighbour lists' (and the LJ pair rows') displacements;
 // the lengths of the atom's accumulated displacements come back for the skin
 // test.
-__device__ __forceinline__ void drift_atom(const Atoms& A, int i, double dt, double& ls, double& ll, double& lp) {
+__device__ __forceinline__ void drift_atom(const Atoms& A, int i, double dt, bool live, double& ls, double& ll,
+                                           double& lp) {
   double sx = A.disp_s[3 * i], sy = A.disp_s[3 * i + 1], sz = A.disp_s[3 * i + 2];
   double lx = A.disp_l[3 * i], ly = A.disp_l[3 * i + 1], lz = A.disp_l[3 * i + 2];
   double px = 0, py = 0, pz = 0;
@@ -859,7 +860,7 @@ __device__ __forceinline__ void drift_atom(const Atoms& A, int i, double dt, dou
     py = A.disp_p[3 * i + 1];
     pz = A.disp_p[3 * i + 2];
   }
-  if (!A.fixed[i]) {
+  if (live && !A.fixed[i]) {
     double dx = A.vx[i] * dt + A.ax[i] * dt * dt / 2.0;
     double dy = A.vy[i] * dt + A.ay[i] * dt * dt / 2.0;
     double dz = A.vz[i] * dt + A.az[i] * dt * dt / 2.0;
@@ -896,16 +897,20 @@ __device__ __forceinline__ void drift_atom(const Atoms& A, int i, double dt, dou
 
 // The drift of one tile's atoms and the tile's part of the skin test (the
 // two largest displacement lengths per list), as k_sys_disp_part writes it.
+// A system whose simulated time has reached its final time stands still
+// (the loop condition of SimLoop.cxx:252).
 __global__ void __launch_bounds__(256) k_drift_tile(Atoms A, const int2* __restrict__ rb,
-                                                    const double* __restrict__ dtp, double* __restrict__ part) {
+                                                    const double* __restrict__ dtp, const double* __restrict__ simt,
+                                                    const double* __restrict__ tfin, double* __restrict__ part) {
   __shared__ double sh[4][6];
   const int2 t = rb[blockIdx.x];
   const int e = tile_end(t, A.sys_off);
   const double dt = dtp[t.x];
+  const bool live = simt[t.x] < tfin[t.x];
   double a1 = 0, a2 = 0, b1 = 0, b2 = 0, c1 = 0, c2 = 0;
   for (int i = t.y + threadIdx.x; i < e; i += blockDim.x) {
     double ls, ll, lp;
-    drift_atom(A, i, dt, ls, ll, lp);
+    drift_atom(A, i, dt, live, ls, ll, lp);
     top2_insert(ls, a1, a2);
     top2_insert(ll, b1, b2);
     if (A.disp_p) top2_insert(lp, c1, c2);
@@ -1078,17 +1083,19 @@ __device__ __forceinline__ void kick_atom(const Atoms& A, const Box& bx, int i, 
 // the net force, the bath's energy, and the largest speed of a free atom with
 // neighbours (the dt rule); merged per system by k_sys_step_merge.
 __global__ void __launch_bounds__(256) k_kick_tile(Atoms A, Box bx, const int2* __restrict__ rb,
-                                                   const double* __restrict__ dtp, int nterms,
+                                                   const double* __restrict__ dtp, const double* __restrict__ simt,
+                                                   const double* __restrict__ tfin, int nterms,
                                                    const double* __restrict__ eatom, double* __restrict__ part) {
   const int2 t = rb[blockIdx.x];
   const int e = tile_end(t, A.sys_off);
   const double dt = dtp[t.x];
+  const bool live = simt[t.x] < tfin[t.x];
   const int n = A.n;
   double ep = 0, ek = 0, fx = 0, fy = 0, fz = 0, vm = 0, eb = 0;
   for (int i = t.y + threadIdx.x; i < e; i += blockDim.x) {
     const double gx = A.grad[3 * i], gy = A.grad[3 * i + 1], gz = A.grad[3 * i + 2];
-    double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i], dE;
-    kick_atom(A, bx, i, dt, gx, gy, gz, vx, vy, vz, dE);
+    double vx = A.vx[i], vy = A.vy[i], vz = A.vz[i], dE = 0.0;
+    if (live) kick_atom(A, bx, i, dt, gx, gy, gz, vx, vy, vz, dE);
     for (int q = 0; q < nterms; q++) ep += eatom[(size_t)q * n + i];
     double vv = sqrt(vx * vx + vy * vy + vz * vz);
     ek += A.mass[i] * (vv * vv) / 2.0;
@@ -1153,11 +1160,13 @@ __global__ void k_sys_energy(int nterms, const int* __restrict__ sys_off, int n,
 // neighbours (:394-398, :409-416), tiled per system: the tile pass is
 // k_kick_tile.
 // One wave per system: out[(step*nsys + s)*4 + {0,1,2,3}] = epot, ekin, dt
-// used, |sum F|; the bath's energy; the next dt.
+// used (0 once the system has reached its final time), |sum F|; the bath's
+// energy; the simulated time and the next dt.
 __global__ void __launch_bounds__(64) k_sys_step_merge(const int* __restrict__ rb_off, const double* __restrict__ part,
                                                        double* __restrict__ dtp, int fixed_dt, double* __restrict__ out,
                                                        const int* __restrict__ stepidx, int nsys,
-                                                       double* __restrict__ ebath) {
+                                                       double* __restrict__ ebath, double* __restrict__ simt,
+                                                       const double* __restrict__ tfin) {
   const int s = blockIdx.x;
   double ep = 0, ek = 0, fx = 0, fy = 0, fz = 0, vm = 0, eb = 0;
   for (int q = rb_off[s] + threadIdx.x; q < rb_off[s + 1]; q += 64) {
@@ -1177,16 +1186,20 @@ __global__ void __launch_bounds__(64) k_sys_step_merge(const int* __restrict__ r
     double* o = out + ((size_t)step * nsys + s) * 4;
     o[0] = ep;
     o[1] = ek;
-    o[2] = dtp[s];
+    const bool live = simt[s] < tfin[s];  // a system past its final time stood still this step
+    o[2] = live ? dtp[s] : 0.0;
     o[3] = sqrt(fx * fx + fy * fy + fz * fz);
-    if (!fixed_dt) {
-      const double dt_max = 5e-16, dt_min = 1e-20;
-      double dt;
-      if (vm != 0.0) dt = 0.05 * (1.0 * kAo) / vm;
-      else dt = dt_max;
-      if (dt > dt_max) dt = dt_max;
-      if (dt < dt_min) dt = dt_min;
-      dtp[s] = dt;
+    if (live) {
+      simt[s] += dtp[s];  // SimLoop.cxx:405
+      if (!fixed_dt) {
+        const double dt_max = 5e-16, dt_min = 1e-20;
+        double dt;
+        if (vm != 0.0) dt = 0.05 * (1.0 * kAo) / vm;
+        else dt = dt_max;
+        if (dt > dt_max) dt = dt_max;
+        if (dt < dt_min) dt = dt_min;
+        dtp[s] = dt;
+      }
     }
   }
 }

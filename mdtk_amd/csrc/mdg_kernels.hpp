@@ -167,14 +167,17 @@ __global__ void k_brenner_bonds(Atoms A, const Tables* T, Box bx, const int2* bo
 // Verlet halves over the reduction tiles: the drift writes the skin test's
 // tile part (as k_sys_disp_part), the kick the step outputs' tile part
 // (merged by k_sys_step_merge).
-__global__ void k_drift_tile(Atoms A, const int2* rb, const double* dtp, double* part);
-__global__ void k_kick_tile(Atoms A, Box bx, const int2* rb, const double* dtp, int nterms, const double* eatom,
-                            double* part);
+// simt/tfin: per system the simulated time and the final time; a system
+// with simt >= tfin stands still (no drift, no kick, dt and time kept).
+__global__ void k_drift_tile(Atoms A, const int2* rb, const double* dtp, const double* simt, const double* tfin,
+                             double* part);
+__global__ void k_kick_tile(Atoms A, Box bx, const int2* rb, const double* dtp, const double* simt,
+                            const double* tfin, int nterms, const double* eatom, double* part);
 __global__ void k_init_acc(Atoms A);
 __global__ void k_pack_pos(Atoms A);
 __global__ void k_sys_energy(int nterms, const int* sys_off, int n, const double* eatom, double* eacc);
 __global__ void k_sys_step_merge(const int* rb_off, const double* part, double* dtp, int fixed_dt, double* out,
-                                 const int* stepidx, int nsys, double* ebath);
+                                 const int* stepidx, int nsys, double* ebath, double* simt, const double* tfin);
 __global__ void k_bath_temp_part(Atoms A, Box bx, const int2* rb, double* part);
 __global__ void k_bath_temp_merge(Atoms A, const int* rb_off, const double* part);
 __global__ void k_step_advance(int* stepidx);

@@ -78,6 +78,8 @@ def load_library(path: str = LIB_PATH):
     L.mdg_set_thermal_bath.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Bath)]
     L.mdg_set_bath_atoms.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8)]
     L.mdg_bath_energy.argtypes = [vp, dp]
+    L.mdg_set_time.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    L.mdg_get_time.argtypes = [vp, dp]
     _lib = L
     return L
 
@@ -207,6 +209,17 @@ class Engine:
         out = np.zeros(self.nsys)
         self._chk(_lib.mdg_bath_energy(self._h, _p(out)))
         return out
+
+    def set_time(self, sim_time: float = 0.0, sim_time_final: float = 1e300, dt: float = 1e-20, system: int = -1):
+        """SimLoop's simTime, simTimeFinal and dt (s) of one system or all (-1);
+        a system past its final time stands still in run() (dt_used 0)."""
+        self._chk(_lib.mdg_set_time(self._h, system, sim_time, sim_time_final, dt))
+
+    def time(self):
+        """Per system (simTime, simTimeFinal, next dt), shape (nsys, 3)."""
+        out = np.zeros(3 * max(self.nsys, 1))
+        self._chk(_lib.mdg_get_time(self._h, _p(out)))
+        return out.reshape(-1, 3)
 
     def params(self):
         """The NUM_PARAMS differentiable parameters (CGS), see PARAM_NAMES."""

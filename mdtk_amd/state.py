@@ -55,7 +55,8 @@ class State:
     indices: np.ndarray              # uint32 [n]
     tags: dict = field(default_factory=dict)   # name -> bool [n]
     bath: dict | None = None         # geometry (1-3), To, gamma, z_min, d_boundary, z_min_free, center, radius
-    time: dict = field(default_factory=lambda: dict(sim_time=0.0, sim_time_final=0.0, dt=1e-20, dt_prev=1e-20))
+    # SimLoop's defaults (SimLoop.cxx:46-50): simTime 0, simTimeFinal 4 ps, dt 1e-20 s
+    time: dict = field(default_factory=lambda: dict(sim_time=0.0, sim_time_final=4e-12, dt=1e-20, dt_prev=1e-20))
     check: dict | None = None
 
     @property
@@ -187,12 +188,14 @@ def write_state(prefix: str, st: State, compress: str | None = "xz"):
                             c["initial_energy"], c["current_energy"], c["energy_from_bath"], c["temperature"]))
 
 
-def engine_from_state(st: State, **engine_kw):
+def engine_from_state(st: State, clock: bool = True, **engine_kw):
     """An Engine on the state's atoms: elements, positions (with their PBC
     counters), velocities, fixed atoms, the thermal bath and its per-atom
-    opt-in.  The engine keeps one box per system, so every atom must have
-    the same PBC axes enabled.  Accelerations start from the forces at the
-    loaded positions (the reference's iteration 0)."""
+    opt-in, and (clock=True) SimLoop's clock from the .time record: simTime,
+    simTimeFinal and dt, so that run() stops where the reference's loop would.
+    The engine keeps one box per system, so every atom must have the same PBC
+    axes enabled.  Accelerations start from the forces at the loaded
+    positions (the reference's iteration 0)."""
     from .engine import Engine
     en = np.asarray(st.pbc_enabled, bool)
     if st.n and not (en == en[0]).all():
@@ -207,6 +210,9 @@ def engine_from_state(st: State, **engine_kw):
                              d_boundary=b["d_boundary"], z_min_free=b["z_min_free"], center=b["center"],
                              radius=b["radius"])
         eng.set_bath_atoms(st.tags.get("thermal_bath_applicable", np.ones(st.n, bool)).astype(np.uint8))
+    if clock:
+        t = st.time
+        eng.set_time(t["sim_time"], t["sim_time_final"], t["dt"])
     return eng
 
 

@@ -1,0 +1,57 @@
+"""A batched impact sequence on the GPU against the oracle run impact by
+impact: every impact stops on its own clock at simTimeFinal (adaptive dt),
+after as many iterations as the reference's loop, in the state the oracle
+reaches; the directories hold what mdtrajsim leaves there."""
+import os
+
+import numpy as np
+import pytest
+
+from mdtk_amd import impacts as I
+from mdtk_amd import state as ST
+from mdtk_amd import systems as S
+
+pytestmark = pytest.mark.gpu
+
+POTS = ("FBZL", "REBO", "AIREBO", "ETors")
+
+
+def test_impact_batch_matches_oracle(tmp_path):
+    from oracle.oracle import Oracle, POT_FBZL, POT_REBO, POT_AIREBO, POT_ETORS
+    b = S.bombard_target(300, 41, energy_ev=60.0, height=2.0)  # the ion meets the surface within the run
+    e, x, v, pbc, fx = b.to_cgs()
+    n = b.n
+    base = ST.State(e, x.reshape(n, 3), v.reshape(n, 3), np.zeros((n, 3)), pbc, np.zeros((n, 3), np.int32),
+                    np.tile(pbc != ST.NO_PBC, (n, 1)), np.arange(n, dtype=np.uint32), {"fixed": fx.astype(bool)},
+                    time=dict(sim_time=0.0, sim_time_final=40e-15, dt=1e-20, dt_prev=1e-20))
+    ST.write_state(str(tmp_path / "base"), base)
+    rng = np.random.default_rng(3)
+    xy = rng.random((3, 2)) * pbc[:2]
+    xy[1] = base.pos[5, :2]  # one impact straight onto a surface atom
+    I.write_ionpos(str(tmp_path / "ionpos.bin"), xy)
+    out = str(tmp_path / "run")
+    rep = I.run_impacts(str(tmp_path / "base"), str(tmp_path / "ionpos.bin"), out, potentials=POTS, batch=2,
+                        chunk=40, etors_compat=True)
+    assert [r["impact"] for r in rep] == [0, 1, 2]
+    assert len({r["iterations"] for r in rep}) > 1  # the impacts ran on clocks of their own
+    for r in rep:
+        k = r["impact"]
+        st = I.impact_state(base, xy[k, 0], xy[k, 1])
+        o = Oracle(st.elem, st.pos, st.vel, pbc, fx, pots=POT_FBZL | POT_REBO | POT_AIREBO | POT_ETORS)
+        t, its = 0.0, 0
+        while t < 40e-15:
+            t += o.run(1)["dt"][0]
+            its += 1
+        xo, vo = o.state()[:2]
+        print(f"impact {k}: {its} oracle iterations, {r['iterations']} GPU, t={r['sim_time']:.4e}", flush=True)
+        assert r["iterations"] == its
+        assert abs(r["sim_time"] - t) <= 1e-12 * t
+        d = r["dir"]
+        assert os.path.exists(os.path.join(d, "completed.ok"))
+        assert np.array_equal(np.fromfile(os.path.join(d, "ionpos_used.bin"), "<f8"), xy[k])
+        fin = ST.read_state(os.path.join(d, f"md{its:010d}"))
+        assert np.abs(fin.pos - xo).max() <= 1e-6 * 1e-8
+        assert np.abs(fin.vel - vo).max() <= 1e-8 * np.abs(vo).max()
+        assert abs(fin.time["sim_time"] - t) <= 1e-12 * t
+    # a finished impact is not run again
+    assert I.run_impacts(str(tmp_path / "base"), str(tmp_path / "ionpos.bin"), out, potentials=POTS) == []
