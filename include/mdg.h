@@ -185,6 +185,43 @@ int mdg_bath_energy(mdg_ctx* ctx, double* out);
 int mdg_set_time(mdg_ctx* ctx, int system, double sim_time, double sim_time_final, double dt);
 int mdg_get_time(mdg_ctx* ctx, double* out);
 
+/* Trajectory output captured on the GPU inside mdg_step_nve, drained to host
+ * memory after every chunk of steps (replaces SnapshotList::getSnapshot,
+ * mdtrajsim/main.cpp:58-83 and SnapshotList.hpp:101-131, and
+ * SimLoop::writetrajXVA, SimLoop.cxx:271-277,821-832).  At the start of
+ * every iteration of a running system: a MDG_FRAME_SNAPSHOT frame with the
+ * atoms flagged in snapshot_atoms (the reference selects those tagged
+ * projectile or cluster; null = none) when CustomSimLoop's cadence is due
+ * (t == 0 or int(t/I) != int((t - dt_prev)/I), I = 1e-14 s below 2 ps,
+ * 5e-14 s below 4 ps, 1e-13 s beyond), and, with xva_frames != 0, a
+ * MDG_FRAME_XVA frame with every atom of the system at iteration 0 and every
+ * 1 ps below 2 ps, 5 ps beyond.  Records hold FP64 positions (cm),
+ * velocities (cm/s), PBC counters and the atom's index in its system; the
+ * SnapshotList keeps them as float32 (SnapshotList.hpp:38-48).
+ * mdg_trajectory_size gives the frames and records gathered so far;
+ * mdg_take_trajectory copies them out (frames[k].first indexes records) and
+ * clears them.  Output stays on until mdg_set_trajectory_output(ctx, NULL, 0)
+ * or mdg_set_systems. */
+enum { MDG_FRAME_SNAPSHOT = 1, MDG_FRAME_XVA = 2 };
+typedef struct mdg_frame {
+  int32_t kind;      /* MDG_FRAME_* */
+  int32_t system;
+  int64_t iteration; /* SimLoop::iteration of the system */
+  double sim_time;   /* s */
+  double dt;         /* s, the dt of the iteration */
+  int64_t first;     /* first record */
+  int64_t count;     /* records */
+} mdg_frame;
+typedef struct mdg_atom_record {
+  double x[3];
+  double v[3];
+  int32_t pbc_count[3];
+  int32_t atom;      /* index within the system */
+} mdg_atom_record;
+int mdg_set_trajectory_output(mdg_ctx* ctx, const uint8_t* snapshot_atoms, int xva_frames);
+int mdg_trajectory_size(mdg_ctx* ctx, int64_t* nframes, int64_t* nrecords);
+int mdg_take_trajectory(mdg_ctx* ctx, mdg_frame* frames, mdg_atom_record* records);
+
 /* Current state: replaces reading Atom::coords/V/an/PBC_count. */
 int mdg_get_state(mdg_ctx* ctx, double* xyz, double* vel, double* acc, int32_t* pbc_count);
 

@@ -49,6 +49,13 @@ struct mdg_ctx {
   double* red_part = nullptr;  // 8 partial results per tile
   double* simt = nullptr;      // per system: simulated time (s)
   double* tfin = nullptr;      // per system: final time (s); a system past it stands still
+  double* dtprev = nullptr;    // per system: dt of the previous iteration (SimLoop::dt_prev)
+  long long* iter = nullptr;   // per system: iterations done (SimLoop::iteration)
+  // trajectory output (mdg_set_trajectory_output)
+  bool traj_on = false;
+  TrajBuf tb{};
+  std::vector<mdg_frame> frames_h;
+  std::vector<mdg_atom_record> recs_h;
   int nred = 0;
   // per embedded-atom member (0 Ackland, 1 TightBinding): per atom its element
   // index or -1, (F'(rho), element) of the density pass, and its partners
@@ -556,6 +563,28 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   }
 }
 
+static_assert(sizeof(mdg_frame) == sizeof(TrajFrame), "mdg_frame layout");
+static_assert(sizeof(mdg_atom_record) == sizeof(TrajRec), "mdg_atom_record layout");
+constexpr int kTrajChunk = 256;  // steps between drains of the trajectory buffers
+
+// Appends the frames and records captured since the chunk started to the
+// host lists (frames' first indexes re-based on the host list).
+int drain_trajectory(mdg_ctx* c) {
+  unsigned long long cnt[3];
+  HIPCHK(c, hipMemcpy(cnt, c->tb.cnt, sizeof(cnt), hipMemcpyDeviceToHost));
+  if (cnt[2]) {
+    c->err = "trajectory output buffer overflow";
+    return 1;
+  }
+  size_t nf = cnt[0], nr = cnt[1], f0 = c->frames_h.size(), r0 = c->recs_h.size();
+  c->frames_h.resize(f0 + nf);
+  c->recs_h.resize(r0 + nr);
+  if (nf) HIPCHK(c, hipMemcpy(&c->frames_h[f0], c->tb.frm, nf * sizeof(mdg_frame), hipMemcpyDeviceToHost));
+  if (nr) HIPCHK(c, hipMemcpy(&c->recs_h[r0], c->tb.rec, nr * sizeof(mdg_atom_record), hipMemcpyDeviceToHost));
+  for (size_t k = f0; k < f0 + nf; k++) c->frames_h[k].first += (int64_t)r0;
+  return 0;
+}
+
 // The skin test of every system (both stages).
 void sys_disp(mdg_ctx* c, double lim_s, double lim_l) {
   Atoms& A = c->A;
@@ -915,7 +944,12 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
     ro[nsys] = (int)rb.size();
     c->nred = (int)rb.size();
     r |= dalloc(c, c->rb, rb.size()) | dalloc(c, c->rb_off, nsys + 1) | dalloc(c, c->red_part, 8 * rb.size()) |
-         dalloc(c, c->simt, nsys) | dalloc(c, c->tfin, nsys);
+         dalloc(c, c->simt, nsys) | dalloc(c, c->tfin, nsys) | dalloc(c, c->dtprev, nsys) |
+         dalloc(c, c->iter, nsys);
+    c->traj_on = false;
+    c->tb = TrajBuf{};
+    c->frames_h.clear();
+    c->recs_h.clear();
     if (!r) {
       HIPCHK(c, hipMemcpy(c->rb, rb.data(), rb.size() * sizeof(int2), hipMemcpyHostToDevice));
       HIPCHK(c, hipMemcpy(c->rb_off, ro.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice));
@@ -1021,8 +1055,11 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   HIPCHK(c, up(c->dtp, c->dt_host.data(), (size_t)nsys * 8));
   {  // simTime 0, no final time (SimLoop.cxx:46)
     std::vector<double> t0(nsys, 0.0), tf(nsys, 1e300);
+    std::vector<long long> it0(nsys, 0);
     HIPCHK(c, up(c->simt, t0.data(), (size_t)nsys * 8));
     HIPCHK(c, up(c->tfin, tf.data(), (size_t)nsys * 8));
+    HIPCHK(c, up(c->dtprev, c->dt_host.data(), (size_t)nsys * 8));  // dt_prev = dt (SimLoop.cxx:248)
+    HIPCHK(c, up(c->iter, it0.data(), (size_t)nsys * 8));
   }
   c->iteration = 0;
   return 0;
@@ -1112,6 +1149,7 @@ int mdg_set_time(mdg_ctx* c, int system, double sim_time, double sim_time_final,
   HIPCHK(c, hipMemcpy(c->simt + s0, t.data(), t.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->tfin + s0, tf.data(), tf.size() * 8, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->dtp + s0, d.data(), d.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->dtprev + s0, d.data(), d.size() * 8, hipMemcpyHostToDevice));  // SimLoop.cxx:248
   for (int s = s0; s < s1; s++) c->dt_host[s] = dt;
   return 0;
 }
@@ -1131,6 +1169,61 @@ int mdg_get_time(mdg_ctx* c, double* out) {
     out[3 * s + 1] = tf[s];
     out[3 * s + 2] = d[s];
   }
+  return 0;
+}
+
+int mdg_set_trajectory_output(mdg_ctx* c, const uint8_t* snapshot_atoms, int xva_frames) {
+  if (c->n == 0) {
+    c->err = "no system";
+    return 1;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);  // the step's graph gains or loses the capture kernel
+  c->traj_on = false;
+  c->tb = TrajBuf{};
+  c->frames_h.clear();
+  c->recs_h.clear();
+  if (!snapshot_atoms && !xva_frames) return 0;
+  const int nsys = c->nsys, n = c->n;
+  std::vector<int> sel, off(nsys + 1, 0);
+  for (int s = 0; s < nsys; s++) {
+    off[s] = (int)sel.size();
+    if (snapshot_atoms)
+      for (int i = c->off_h[s]; i < c->off_h[s + 1]; i++)
+        if (snapshot_atoms[i]) sel.push_back(i);
+  }
+  off[nsys] = (int)sel.size();
+  TrajBuf& t = c->tb;
+  t.cap_frm = 2LL * kTrajChunk * nsys + 64;
+  t.cap_rec = (long long)kTrajChunk * (long long)sel.size() + (xva_frames ? 2LL * n : 0) + 64;
+  int* seld = nullptr;
+  int* offd = nullptr;
+  if (dalloc(c, t.rec, (size_t)t.cap_rec) || dalloc(c, t.frm, (size_t)t.cap_frm) || dalloc(c, t.cnt, 3))
+    return 1;
+  if (!sel.empty()) {
+    if (dalloc(c, seld, sel.size()) || dalloc(c, offd, nsys + 1)) return 1;
+    HIPCHK(c, hipMemcpy(seld, sel.data(), sel.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(offd, off.data(), (nsys + 1) * 4, hipMemcpyHostToDevice));
+  }
+  t.sel = seld;
+  t.sel_off = offd;
+  t.xva = xva_frames ? 1 : 0;
+  c->traj_on = true;
+  return 0;
+}
+
+int mdg_trajectory_size(mdg_ctx* c, int64_t* nframes, int64_t* nrecords) {
+  if (nframes) *nframes = (int64_t)c->frames_h.size();
+  if (nrecords) *nrecords = (int64_t)c->recs_h.size();
+  return 0;
+}
+
+int mdg_take_trajectory(mdg_ctx* c, mdg_frame* frames, mdg_atom_record* records) {
+  if (frames && !c->frames_h.empty()) std::memcpy(frames, c->frames_h.data(), c->frames_h.size() * sizeof(mdg_frame));
+  if (records && !c->recs_h.empty())
+    std::memcpy(records, c->recs_h.data(), c->recs_h.size() * sizeof(mdg_atom_record));
+  c->frames_h.clear();
+  c->recs_h.clear();
   return 0;
 }
 
@@ -1281,7 +1374,8 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
   Atoms& A = c->A;
   int n = c->n, nsys = c->nsys;
   // per-step, per-system outputs are staged in chunks of at most 2^20 entries
-  const int chunk = std::max(1, std::min(4096, (1 << 20) / nsys));
+  // with trajectory output, at most kTrajChunk steps between drains of its buffers
+  const int chunk = std::max(1, std::min(c->traj_on ? kTrajChunk : 4096, (1 << 20) / nsys));
   if (c->step_cap < chunk) {
     if (dalloc(c, c->stepout, (size_t)chunk * nsys * 4)) return 1;
     if (!c->stepidx && dalloc(c, c->stepidx, 1)) return 1;
@@ -1301,6 +1395,10 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
   // and flag it needs lives in device memory, so the same launches can be
   // captured once and replayed.
   auto body = [&]() {
+    if (c->traj_on)
+      launch(c, "k_traj_capture", [&] {
+        k_traj_capture<<<nsys, 256, 0, c->stream>>>(A, c->tb, c->simt, c->tfin, c->dtprev, c->iter, c->dtp);
+      });
     if (c->bath_on)
       launch(c, "k_bath_temp", [&] {
         k_bath_temp_part<<<c->nred, 256, 0, c->stream>>>(A, c->box, c->rb, c->red_part);
@@ -1320,7 +1418,8 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
     });
     launch(c, "k_sys_step_end", [&] {
       k_sys_step_merge<<<nsys, 64, 0, c->stream>>>(c->rb_off, c->red_part, c->dtp, fixed, c->stepout, c->stepidx,
-                                                   nsys, A.tb_dE ? c->ebath : nullptr, c->simt, c->tfin);
+                                                   nsys, A.tb_dE ? c->ebath : nullptr, c->simt, c->tfin, c->dtprev,
+                                                   c->iter);
     });
     k_step_advance<<<1, 64, 0, c->stream>>>(c->stepidx);
   };
@@ -1337,11 +1436,13 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
   for (int s0 = 0; s0 < nsteps; s0 += chunk) {
     int m = std::min(chunk, nsteps - s0);
     k_zero_i<<<1, 64, 0, c->stream>>>(c->stepidx, 1);
+    if (c->traj_on) k_zero<<<1, 64, 0, c->stream>>>(reinterpret_cast<double*>(c->tb.cnt), 3);  // all-zero bits
     for (int s = 0; s < m; s++) {
       if (use_graph) HIPCHK(c, hipGraphLaunch(c->graph_exec[fixed], c->stream));
       else body();
     }
     if (check_err(c)) return 1;
+    if (c->traj_on && drain_trajectory(c)) return 1;
     HIPCHK(c, hipMemcpy(o.data(), c->stepout, (size_t)m * nsys * 4 * sizeof(double), hipMemcpyDeviceToHost));
     for (int s = 0; s < m; s++)
       for (int q = 0; q < nsys; q++) {

@@ -1166,7 +1166,8 @@ __global__ void __launch_bounds__(64) k_sys_step_merge(const int* __restrict__ r
                                                        double* __restrict__ dtp, int fixed_dt, double* __restrict__ out,
                                                        const int* __restrict__ stepidx, int nsys,
                                                        double* __restrict__ ebath, double* __restrict__ simt,
-                                                       const double* __restrict__ tfin) {
+                                                       const double* __restrict__ tfin, double* __restrict__ dtprev,
+                                                       long long* __restrict__ iter) {
   const int s = blockIdx.x;
   double ep = 0, ek = 0, fx = 0, fy = 0, fz = 0, vm = 0, eb = 0;
   for (int q = rb_off[s] + threadIdx.x; q < rb_off[s + 1]; q += 64) {
@@ -1191,6 +1192,8 @@ __global__ void __launch_bounds__(64) k_sys_step_merge(const int* __restrict__ r
     o[3] = sqrt(fx * fx + fy * fy + fz * fz);
     if (live) {
       simt[s] += dtp[s];  // SimLoop.cxx:405
+      dtprev[s] = dtp[s];  // :407
+      iter[s] += 1;
       if (!fixed_dt) {
         const double dt_max = 5e-16, dt_min = 1e-20;
         double dt;
@@ -1202,6 +1205,71 @@ __global__ void __launch_bounds__(64) k_sys_step_merge(const int* __restrict__ r
       }
     }
   }
+}
+
+// A block per system, before the iteration's drift (doBeforeIteration and
+// the XVA check run first in the loop body, SimLoop.cxx:252-277).  Due
+// tests on the clock at the iteration's start, t, and the previous step's
+// dt: t == 0 or int(t / I) != int((t - dt_prev) / I), with I the SnapshotList
+// interval of CustomSimLoop::doBeforeIteration (mdtrajsim/main.cpp:58-83: 1e-14
+// s below 2 ps, 5e-14 s below 4 ps, 1e-13 s beyond) or the XVA interval it
+// sets (1 ps below 2 ps, 5 ps beyond; iteration 0 always writes one,
+// SimLoop.cxx:271-277).
+__global__ void __launch_bounds__(256) k_traj_capture(Atoms A, TrajBuf tb, const double* __restrict__ simt,
+                                                      const double* __restrict__ tfin,
+                                                      const double* __restrict__ dtprev,
+                                                      const long long* __restrict__ iter,
+                                                      const double* __restrict__ dtp) {
+  __shared__ long long s_first[2];
+  __shared__ int s_n[2];
+  const int s = blockIdx.x;
+  const double t = simt[s];
+  if (!(t < tfin[s])) return;  // block-uniform: the system has stopped
+  if (threadIdx.x == 0) {
+    const double dp = dtprev[s];
+    auto due = [&](double I) { return t == 0.0 || (long long)(t / I) != (long long)((t - dp) / I); };
+    const double isnap = t < 2e-12 ? 5e-16 * 5 * 4 : t < 4e-12 ? 5e-16 * 25 * 4 : 5e-16 * 50 * 4;
+    const double ixva = t < 2e-12 ? 1e-12 : 5e-12;
+    const int nsel = tb.sel_off ? tb.sel_off[s + 1] - tb.sel_off[s] : 0;
+    const int ns = A.sys_off[s + 1] - A.sys_off[s];
+    const int want[2] = {(nsel > 0 && due(isnap)) ? nsel : 0, (tb.xva && (iter[s] == 0 || due(ixva))) ? ns : 0};
+    for (int k = 0; k < 2; k++) {
+      s_n[k] = 0;
+      if (!want[k]) continue;
+      const unsigned long long f = atomicAdd(&tb.cnt[0], 1ull);
+      const unsigned long long r = atomicAdd(&tb.cnt[1], (unsigned long long)want[k]);
+      if ((long long)f >= tb.cap_frm || (long long)(r + want[k]) > tb.cap_rec) {
+        atomicMax(&tb.cnt[2], 1ull);
+        continue;
+      }
+      TrajFrame& F = tb.frm[f];
+      F.kind = k == 0 ? TRAJ_SNAPSHOT : TRAJ_XVA;
+      F.system = s;
+      F.iteration = iter[s];
+      F.sim_time = t;
+      F.dt = dtp[s];
+      F.first = (long long)r;
+      F.count = want[k];
+      s_n[k] = want[k];
+      s_first[k] = (long long)r;
+    }
+  }
+  __syncthreads();
+  for (int k = 0; k < 2; k++)
+    for (int q = threadIdx.x; q < s_n[k]; q += blockDim.x) {
+      const int i = (k == 0) ? tb.sel[tb.sel_off[s] + q] : A.sys_off[s] + q;
+      TrajRec& R = tb.rec[s_first[k] + q];
+      R.x[0] = A.x[i];
+      R.x[1] = A.y[i];
+      R.x[2] = A.z[i];
+      R.v[0] = A.vx[i];
+      R.v[1] = A.vy[i];
+      R.v[2] = A.vz[i];
+      R.pc[0] = A.pcx[i];
+      R.pc[1] = A.pcy[i];
+      R.pc[2] = A.pcz[i];
+      R.atom = i - A.sys_off[s];
+    }
 }
 
 __global__ void k_step_advance(int* __restrict__ stepidx) {

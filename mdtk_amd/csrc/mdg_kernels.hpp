@@ -177,10 +177,39 @@ __global__ void k_init_acc(Atoms A);
 __global__ void k_pack_pos(Atoms A);
 __global__ void k_sys_energy(int nterms, const int* sys_off, int n, const double* eatom, double* eacc);
 __global__ void k_sys_step_merge(const int* rb_off, const double* part, double* dtp, int fixed_dt, double* out,
-                                 const int* stepidx, int nsys, double* ebath, double* simt, const double* tfin);
+                                 const int* stepidx, int nsys, double* ebath, double* simt, const double* tfin,
+                                 double* dtprev, long long* iter);
 __global__ void k_bath_temp_part(Atoms A, Box bx, const int2* rb, double* part);
 __global__ void k_bath_temp_merge(Atoms A, const int* rb_off, const double* part);
 __global__ void k_step_advance(int* stepidx);
+
+// Trajectory output captured inside the step (the layouts of mdg_frame and
+// mdg_atom_record in mdg.h): at the start of an iteration a system due for a
+// SnapshotList record (mdtrajsim's CustomSimLoop cadence) or an XVA frame
+// (SimLoop::writetrajXVA cadence) reserves a frame and its atom records.
+enum : int { TRAJ_SNAPSHOT = 1, TRAJ_XVA = 2 };
+struct TrajRec {
+  double x[3], v[3];
+  int pc[3];
+  int atom;
+};
+struct TrajFrame {
+  int kind, system;
+  long long iteration;
+  double sim_time, dt;
+  long long first, count;
+};
+struct TrajBuf {
+  TrajRec* rec;
+  TrajFrame* frm;
+  unsigned long long* cnt;  // [0] frames, [1] records, [2] overflow
+  long long cap_rec, cap_frm;
+  const int* sel;      // selected atoms (SnapshotList), per system sel[sel_off[s] ..]
+  const int* sel_off;  // nsys + 1, or null: no SnapshotList
+  int xva;             // XVA frames wanted
+};
+__global__ void k_traj_capture(Atoms A, TrajBuf tb, const double* simt, const double* tfin, const double* dtprev,
+                               const long long* iter, const double* dtp);
 __global__ void k_zero(double* p, size_t n);
 __global__ void k_zero_i(int* p, int n);
 

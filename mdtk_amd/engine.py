@@ -80,8 +80,18 @@ def load_library(path: str = LIB_PATH):
     L.mdg_bath_energy.argtypes = [vp, dp]
     L.mdg_set_time.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
     L.mdg_get_time.argtypes = [vp, dp]
+    L.mdg_set_trajectory_output.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8), ctypes.c_int]
+    L.mdg_trajectory_size.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+    L.mdg_take_trajectory.argtypes = [vp, ctypes.c_void_p, ctypes.c_void_p]
     _lib = L
     return L
+
+
+# mdg_frame and mdg_atom_record of mdg.h
+FRAME_DTYPE = np.dtype([("kind", "<i4"), ("system", "<i4"), ("iteration", "<i8"), ("sim_time", "<f8"),
+                        ("dt", "<f8"), ("first", "<i8"), ("count", "<i8")])
+RECORD_DTYPE = np.dtype([("x", "<f8", 3), ("v", "<f8", 3), ("pbc_count", "<i4", 3), ("atom", "<i4")])
+FRAME_SNAPSHOT, FRAME_XVA = 1, 2
 
 
 class _Bath(ctypes.Structure):
@@ -100,8 +110,9 @@ def _p(a, t=ctypes.c_double):
 class Engine:
     """One GPU context: a potential chain over one system.
 
-    potentials: names from {"FBZL", "Brenner", "REBO", "AIREBO", "ETors"}; the
-    chain order is the reference's (applications/common.h:46-68).
+    potentials: names from {"FBZL", "FLJ", "Brenner", "REBO", "AIREBO", "ETors",
+    "Ackland", "FBM", "TightBinding"}; the chain order is the reference's
+    (applications/common.h:46-68), FBM and TightBinding after it.
     """
 
     def __init__(self, potentials=("REBO", "AIREBO", "ETors"), precision: str = "fp64",
@@ -214,6 +225,24 @@ class Engine:
         """SimLoop's simTime, simTimeFinal and dt (s) of one system or all (-1);
         a system past its final time stands still in run() (dt_used 0)."""
         self._chk(_lib.mdg_set_time(self._h, system, sim_time, sim_time_final, dt))
+
+    def set_trajectory_output(self, snapshot_atoms=None, xva: bool = False):
+        """Capture SnapshotList records of the flagged atoms (mdtrajsim's
+        cadence) and/or XVA frames of whole systems inside run(); None and
+        False turn it off."""
+        sel = None if snapshot_atoms is None else np.ascontiguousarray(snapshot_atoms, np.uint8)
+        self._chk(_lib.mdg_set_trajectory_output(self._h, _p(sel, ctypes.c_uint8), int(bool(xva))))
+
+    def take_trajectory(self):
+        """The frames (structured array, FRAME_DTYPE) and atom records
+        (RECORD_DTYPE) captured since the last call."""
+        nf, nr = ctypes.c_int64(0), ctypes.c_int64(0)
+        self._chk(_lib.mdg_trajectory_size(self._h, ctypes.byref(nf), ctypes.byref(nr)))
+        fr = np.zeros(nf.value, FRAME_DTYPE)
+        rec = np.zeros(nr.value, RECORD_DTYPE)
+        self._chk(_lib.mdg_take_trajectory(self._h, fr.ctypes.data if nf.value else None,
+                                           rec.ctypes.data if nr.value else None))
+        return fr, rec
 
     def time(self):
         """Per system (simTime, simTimeFinal, next dt), shape (nsys, 3)."""
