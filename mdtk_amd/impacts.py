@@ -74,10 +74,15 @@ def _write_used(d: str, x: float, y: float):
 
 def run_impacts(base: str, ionpos: str = "ionpos.bin", out: str = ".", start: int = 0, stop: int | None = None,
                 potentials=CHAIN, precision: str = "fp64", batch: int = 256, chunk: int = 1000,
-                compress: str | None = "xz", etors_compat: bool = True, max_steps: int = 10_000_000) -> list[dict]:
+                compress: str | None = "xz", etors_compat: bool = True, max_steps: int = 10_000_000,
+                trajectory: bool = True) -> list[dict]:
     """Runs impacts [start, stop) of the ionpos file from the base state
     (prefix `base` of its SimLoopSaver files) in batches of `batch`, `chunk`
-    steps per engine call, and writes each impact's directory under `out`.
+    steps per engine call, and writes each impact's directory under `out`;
+    with `trajectory`, also the SnapshotList of the atoms tagged projectile
+    or cluster (the last atom when none is tagged) as snapshots.conf and the
+    XVA frames as mde%010d.xva, as mdtrajsim's run leaves them
+    (main.cpp:155-167, SnapshotList.hpp:85-91, SimLoop.cxx:271-277).
     Returns per impact {impact, dir, iterations, sim_time, epot, ekin}."""
     from .engine import Engine
     st = ST.read_state(base)
@@ -108,6 +113,12 @@ def run_impacts(base: str, ionpos: str = "ionpos.bin", out: str = ".", start: in
             app = st.tags.get("thermal_bath_applicable", np.ones(st.n, bool)).astype(np.uint8)
             eng.set_bath_atoms(np.tile(app, len(group)))
         eng.set_time(t["sim_time"], t["sim_time_final"], t["dt"])
+        if trajectory:
+            tag = st.tags.get("projectile", np.zeros(st.n, bool)) | st.tags.get("cluster", np.zeros(st.n, bool))
+            if not tag.any():
+                tag = np.eye(1, st.n, st.n - 1, dtype=bool)[0]
+            eng.set_trajectory_output(np.tile(tag.astype(np.uint8), len(group)), xva=True)
+            frames, recs = [], []
         its = np.zeros(len(group), np.int64)
         last_e = np.zeros((len(group), 2))
         steps = 0
@@ -119,6 +130,11 @@ def run_impacts(base: str, ionpos: str = "ionpos.bin", out: str = ".", start: in
                 raise RuntimeError(f"impacts {group[0]}..{group[-1]}: {max_steps} steps without reaching simTimeFinal")
             r = eng.run(chunk)
             steps += chunk
+            if trajectory:
+                f, rc = eng.take_trajectory()
+                f["first"] += sum(len(q) for q in recs)
+                frames.append(f)
+                recs.append(rc)
             live = r["dt"].reshape(chunk, len(group)) > 0
             its += live.sum(axis=0)
             ep, ek = r["epot"].reshape(chunk, -1), r["ekin"].reshape(chunk, -1)
@@ -128,6 +144,11 @@ def run_impacts(base: str, ionpos: str = "ionpos.bin", out: str = ".", start: in
                     last_e[s] = ep[q, s], ek[q, s]
         clock = eng.time()
         x, v, a, c = eng.state()
+        if trajectory:
+            from . import trajectory as TR
+            from .engine import FRAME_DTYPE, RECORD_DTYPE
+            fa = np.concatenate(frames) if frames else np.zeros(0, FRAME_DTYPE)
+            ra = np.concatenate(recs) if recs else np.zeros(0, RECORD_DTYPE)
         for s, k in enumerate(group):
             d = impact_dir(out, k)
             os.makedirs(d, exist_ok=True)
@@ -137,6 +158,12 @@ def run_impacts(base: str, ionpos: str = "ionpos.bin", out: str = ".", start: in
                            st.pbc_enabled, st.indices, st.tags, st.bath,
                            dict(t, sim_time=float(clock[s, 0]), dt=float(clock[s, 2])), st.check)
             ST.write_state(os.path.join(d, f"md{int(t.get('iteration', 0)) + int(its[s]):010d}"), fin, compress)
+            if trajectory:
+                sel, shots = TR.snapshot_list(fa, ra, s)
+                TR.write_snapshots_conf(os.path.join(d, "snapshots.conf"), sel, shots)
+                for f in TR.xva_frames(fa, s):
+                    with open(os.path.join(d, TR.xva_name(f)), "w") as fo:
+                        fo.write(TR.xva_text(f, ra, st.check))
             with open(os.path.join(d, "completed.ok"), "w"):
                 pass
             report.append(dict(impact=k, dir=d, iterations=int(its[s]), sim_time=float(clock[s, 0]),

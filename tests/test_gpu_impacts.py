@@ -53,5 +53,9 @@ def test_impact_batch_matches_oracle(tmp_path):
         assert np.abs(fin.pos - xo).max() <= 1e-6 * 1e-8
         assert np.abs(fin.vel - vo).max() <= 1e-8 * np.abs(vo).max()
         assert abs(fin.time["sim_time"] - t) <= 1e-12 * t
+        from mdtk_amd import trajectory as TR
+        sel, shots = TR.read_snapshots_conf(os.path.join(d, "snapshots.conf"))
+        assert list(sel) == [n - 1] and len(shots) >= 3 and shots[0][0] == 0.0
+        assert os.path.exists(os.path.join(d, "mde0000000000.xva"))
     # a finished impact is not run again
     assert I.run_impacts(str(tmp_path / "base"), str(tmp_path / "ionpos.bin"), out, potentials=POTS) == []
