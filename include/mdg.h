@@ -200,8 +200,8 @@ int mdg_get_time(mdg_ctx* ctx, double* out);
  * SnapshotList keeps them as float32 (SnapshotList.hpp:38-48).
  * mdg_trajectory_size gives the frames and records gathered so far;
  * mdg_take_trajectory copies them out (frames[k].first indexes records) and
- * clears them.  Output stays on until mdg_set_trajectory_output(ctx, NULL, 0)
- * or mdg_set_systems. */
+ * clears them.  Output stays on until mdg_set_trajectory_output(ctx, NULL, 0);
+ * every call replaces the previous buffers and drops what they held. */
 enum { MDG_FRAME_SNAPSHOT = 1, MDG_FRAME_XVA = 2 };
 typedef struct mdg_frame {
   int32_t kind;      /* MDG_FRAME_* */

@@ -185,6 +185,19 @@ int dalloc(mdg_ctx* c, T*& p, size_t count) {
   return 0;
 }
 
+// Frees a buffer that dalloc made (and forgets it); null is a no-op.
+template <class T>
+void dfree(mdg_ctx* c, T*& p) {
+  if (!p) return;
+  void* q = const_cast<void*>(static_cast<const void*>(p));
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), q);
+  if (it != c->allocs.end()) {
+    hipFree(q);
+    c->allocs.erase(it);
+  }
+  p = nullptr;
+}
+
 int type_of(int elem) {
   switch (elem) {
     case 12: return T_C;
@@ -1180,6 +1193,11 @@ int mdg_set_trajectory_output(mdg_ctx* c, const uint8_t* snapshot_atoms, int xva
   HIPCHK(c, hipStreamSynchronize(c->stream));
   drop_graphs(c);  // the step's graph gains or loses the capture kernel
   c->traj_on = false;
+  dfree(c, c->tb.rec);
+  dfree(c, c->tb.frm);
+  dfree(c, c->tb.cnt);
+  dfree(c, c->tb.sel);
+  dfree(c, c->tb.sel_off);
   c->tb = TrajBuf{};
   c->frames_h.clear();
   c->recs_h.clear();
