@@ -170,3 +170,68 @@ def run_impacts(base: str, ionpos: str = "ionpos.bin", out: str = ".", start: in
                                epot=float(last_e[s, 0]), ekin=float(last_e[s, 1])))
         del eng
     return report
+
+
+def run_common(input_id: str = "base", out: str = ".", potentials=CHAIN, precision: str = "fp64",
+               chunk: int = 1000, compress: str | None = "xz", etors_compat: bool = True) -> dict | None:
+    """mdtrajsim's common usage (main.cpp:115-191): one trajectory from the
+    state `input_id` to its simTimeFinal, in `out`; nothing when `out` already
+    holds a finished run.  Writes the final state, snapshots.conf, the XVA
+    frames and completed.ok there."""
+    if is_finished(out):
+        return None
+    os.makedirs(out, exist_ok=True)
+    xy = read_state_projectile(input_id)
+    path = os.path.join(out, ".ionpos_common.bin")
+    write_ionpos(path, xy[None, :])
+    try:
+        rep = run_impacts(input_id, path, out=os.path.join(out, ".common"), potentials=potentials,
+                          precision=precision, batch=1, chunk=chunk, compress=compress, etors_compat=etors_compat)
+    finally:
+        os.remove(path)
+    d = impact_dir(os.path.join(out, ".common"), 0)
+    for f in os.listdir(d):
+        if not f.startswith("ionpos_used"):
+            os.replace(os.path.join(d, f), os.path.join(out, f))
+    for f in os.listdir(d):
+        os.remove(os.path.join(d, f))
+    os.rmdir(d)
+    os.rmdir(os.path.join(out, ".common"))
+    r = rep[0]
+    r["dir"] = out
+    return r
+
+
+def read_state_projectile(input_id: str) -> np.ndarray:
+    """(x, y) of the state's last atom, so that a common run keeps it in place."""
+    st = ST.read_state(input_id)
+    return st.pos[-1, :2].copy()
+
+
+def main(argv=None):
+    """`python -m mdtk_amd.impacts [-c | -i N] [FILE]`: mdtrajsim's two modes
+    (main.cpp:193-348) on the GPU; FILE is the state id (default base)."""
+    import argparse
+    ap = argparse.ArgumentParser(prog="python -m mdtk_amd.impacts")
+    ap.add_argument("file", nargs="?", default="base")
+    ap.add_argument("-c", "--common-usage", action="store_true")
+    ap.add_argument("-i", "--start-from-impact", type=int, default=None)
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--batch", type=int, default=256)
+    a = ap.parse_args(argv)
+    if a.start_from_impact is not None and not a.common_usage:
+        if a.start_from_impact < 0:
+            ap.error("Wrong starting impact number")
+        rep = run_impacts(a.file, "ionpos.bin", ".", start=a.start_from_impact, precision=a.precision,
+                          batch=a.batch)
+    else:
+        r = run_common(a.file, ".", precision=a.precision)
+        rep = [] if r is None else [r]
+    for r in rep:
+        print(f"{r['dir']}: {r['iterations']} iterations, simTime {r['sim_time']:.6e} s", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
+

@@ -59,3 +59,25 @@ def test_impact_batch_matches_oracle(tmp_path):
         assert os.path.exists(os.path.join(d, "mde0000000000.xva"))
     # a finished impact is not run again
     assert I.run_impacts(str(tmp_path / "base"), str(tmp_path / "ionpos.bin"), out, potentials=POTS) == []
+
+
+def test_common_usage_runs_one_trajectory(tmp_path):
+    from oracle.oracle import Oracle, POT_FBZL, POT_REBO, POT_AIREBO, POT_ETORS
+    b = S.bombard_target(300, 43, energy_ev=50.0, height=2.0)
+    e, x, v, pbc, fx = b.to_cgs()
+    n = b.n
+    base = ST.State(e, x.reshape(n, 3), v.reshape(n, 3), np.zeros((n, 3)), pbc, np.zeros((n, 3), np.int32),
+                    np.tile(pbc != ST.NO_PBC, (n, 1)), np.arange(n, dtype=np.uint32), {"fixed": fx.astype(bool)},
+                    time=dict(sim_time=0.0, sim_time_final=20e-15, dt=1e-20, dt_prev=1e-20))
+    ST.write_state(str(tmp_path / "base"), base)
+    out = str(tmp_path / "common")
+    r = I.run_common(str(tmp_path / "base"), out, potentials=POTS, chunk=30)
+    o = Oracle(e, x, v, pbc, fx, pots=POT_FBZL | POT_REBO | POT_AIREBO | POT_ETORS)
+    t, its = 0.0, 0
+    while t < 20e-15:
+        t += o.run(1)["dt"][0]
+        its += 1
+    assert r["iterations"] == its and os.path.exists(os.path.join(out, "completed.ok"))
+    fin = ST.read_state(os.path.join(out, f"md{its:010d}"))
+    assert np.abs(fin.pos - o.state()[0]).max() <= 1e-6 * 1e-8
+    assert sorted(os.listdir(out))[0] != ".common" and I.run_common(str(tmp_path / "base"), out) is None
