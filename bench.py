@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the single-system PE and drift figures")
+    ap.add_argument("--workload", choices=("bombard", "metal"), default="bombard",
+                    help="metal: only the Ackland Cu-crystal figure, as its own JSON line")
     return ap.parse_args()
 
 
@@ -86,8 +88,9 @@ def kernel_times(eng, steps, dt):
     return kt
 
 
-LJ_SHAPES = {0: (1, 64), 1: (3, 64), 2: (1, 32), 3: (1, 16), 4: (4, 16)}  # autotuner's lj_shape -> (OCC, LW)
+LJ_SHAPES = {0: (1, 64), 1: (3, 64), 2: (1, 32), 3: (1, 16), 4: (4, 16), 5: (3, 16)}  # autotuner's lj_shape -> (OCC, LW)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_bombard256.json")
+METAL_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_cu256k.json")
 
 
 def lj_algorithmic_bytes(cnt, natoms):
@@ -101,11 +104,11 @@ def lj_algorithmic_bytes(cnt, natoms):
             + 2.0 * cnt["bonds"] * 36 + 16.0 * cnt["bstar"])
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, path=TRAFFIC_FILE):
     """Bytes beyond L2 per launch (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
-    from the committed rocprofv3 counter summary of this workload."""
+    from the committed rocprofv3 counter summary of the workload."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             t = json.load(f)["bytes_per_launch"].get(kernel)
     except (OSError, ValueError, KeyError):
         return None
@@ -174,12 +177,13 @@ def metal_block(steps, warmup):
     inrange = inrange_pairs(x / S.ANG, cu.pbc, 1.2247449 * 4.086)  # Ackland's largest range (Ag), A
     dens, force = eam_algorithmic_bytes(entries, inrange, cu.n)
     rl = {}
-    for k, alg in (("k_eam_density", dens), ("k_eam_force", force)):
+    for k, alg, full in (("k_eam_density", dens, "k_eam_density<mdg::AcklandModel>"),
+                         ("k_eam_force", force, "k_eam_force<mdg::AcklandModel>")):
         t_ms, nl = kt[k]
         avg = t_ms / max(nl, 1)
         rl[k] = {"bound": "hbm", "algorithmic_bytes": alg, "avg_launch_ms": avg,
                  "achieved": alg / (avg / 1e3) / 1e9, "peak": 8000.0, "unit": "GB/s",
-                 "frac": alg / (avg / 1e3) / 1e9 / 8000.0}
+                 "frac": alg / (avg / 1e3) / 1e9 / 8000.0, "traffic": measured_traffic(full, METAL_TRAFFIC_FILE)}
     et = (res["epot"] + res["ekin"]) / S.EV
     out = {"workload": f"Cu fcc 40x40x40 ({cu.n} atoms), periodic, 300 K, Ackland EAM alone, dt 1 fs, FP64",
            "value": cu.n * steps / (ms / 1e3), "unit": "atom-steps/s", "ms_per_step": ms / steps,
@@ -260,6 +264,13 @@ def main():
     import numpy as np
     from mdtk_amd import Engine
     from mdtk_amd import systems as S
+
+    if a.workload == "metal":
+        m = metal_block(a.steps, a.warmup)
+        print(json.dumps({"metric": "atom-steps/s on 1 GPU, Ackland EAM (Cu crystal)", "n_gpus": 1, "steps": a.steps,
+                          "warmup": a.warmup, "higher_is_better": True, "dtype": "f64", "data": "synthetic", **m}),
+              flush=True)
+        return
 
     dt = 0.1e-15
     batch = S.bombard_batch(a.ntraj)

@@ -545,9 +545,11 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
           case 0: go(I1(), std::integral_constant<int, 64>()); break;
           case 1: go(std::integral_constant<int, 3>(), std::integral_constant<int, 64>()); break;
           case 2: go(I1(), std::integral_constant<int, 32>()); break;
-          case 4:  // 4 waves/SIMD (FP32 form only: the FP64 form's LDS allows 2)
+          case 4:  // 16 lanes at 4 or 3 waves/SIMD (FP32 form only: the FP64 form's LDS allows 2)
+          case 5:
             if constexpr (std::is_same<R, float>::value) {
-              go(std::integral_constant<int, 4>(), std::integral_constant<int, 16>());
+              if (tu.lj_shape == 4) go(std::integral_constant<int, 4>(), std::integral_constant<int, 16>());
+              else go(std::integral_constant<int, 3>(), std::integral_constant<int, 16>());
               break;
             }
             [[fallthrough]];
@@ -1661,7 +1663,7 @@ int mdg_autotune(mdg_ctx* c, int reps, double* report) {
     tune_knob("k_rebo_bonds", tu.grid_bonds, grids(nb, 256));
   }
   if (c->slot_lj >= 0) {
-    if (c->mixed) tune_knob("k_lj_wave", tu.lj_shape, {0, 1, 2, 3, 4});
+    if (c->mixed) tune_knob("k_lj_wave", tu.lj_shape, {0, 1, 2, 3, 4, 5});
     else tune_knob("k_lj_wave", tu.lj_shape, {0, 1, 2, 3});
     int lw = (tu.lj_shape >= 3) ? 16 : (tu.lj_shape == 2) ? 32 : 64;
     tune_knob("k_lj_wave", tu.grid_lj, grids(c->n, 4 * (64 / lw)));

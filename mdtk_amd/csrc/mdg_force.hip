@@ -1474,6 +1474,7 @@ MDG_INSTANTIATE_TH(float, false, 3, 64)
 MDG_INSTANTIATE_TH(float, false, 1, 32)
 MDG_INSTANTIATE_TH(float, false, 1, 16)
 MDG_INSTANTIATE_TH(float, false, 4, 16)
+MDG_INSTANTIATE_TH(float, false, 3, 16)
 MDG_INSTANTIATE_OCC(double, false, 1)
 MDG_INSTANTIATE_OCC(double, false, 2)
 MDG_INSTANTIATE_OCC(double, true, 1)

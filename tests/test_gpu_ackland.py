@@ -113,3 +113,25 @@ def test_ackland_batch_matches_single():
         a, b = bat.offsets[k], bat.offsets[k + 1]
         assert abs(eb[k][0] - e1[0]) <= 1e-12 * abs(e1[0])
         assert force_rel_err(gb[a:b], g1) <= 1e-12
+
+
+def test_copper_slab_in_a_thermal_bath():
+    """Ackland with SimLoop's box bath (To = 600 K on the slab's lower half and
+    its borders) against the oracle, adaptive dt: energies and the energy
+    taken from the bath."""
+    from mdtk_amd import Engine
+    from oracle.oracle import Oracle, POT_ACKLAND
+    s = S.jitter(S.fcc(64, 5, 5, 4, periodic=(True, True, False)), 0.05, 9)
+    s.vel = S.maxwell_velocities(s, 150.0, stream=10)
+    zmid = (s.pos[:, 2].min() + s.pos[:, 2].max()) / 2
+    kw = dict(To=600.0, gamma=5.0e13, z_min=zmid * S.ANG, d_boundary=2.0 * S.ANG, z_min_free=-5.0 * S.ANG)
+    o = Oracle.from_system(s, pots=POT_ACKLAND)
+    o.set_bath(2, kw["To"], kw["gamma"], kw["z_min"], kw["d_boundary"], kw["z_min_free"])
+    g = Engine.from_system(s, potentials=("Ackland",))
+    g.set_thermal_bath(geometry="box", **kw)
+    ro, rg = o.run(60), g.run(60)
+    assert np.abs(rg["epot"] - ro["epot"]).max() <= 1e-10 * np.abs(ro["epot"]).max()
+    assert np.abs(rg["ekin"] - ro["ekin"]).max() <= 1e-10 * np.abs(ro["ekin"]).max()
+    eb_o, eb_g = o.bath_energy(), g.bath_energy()[0]
+    print(f"bath energy oracle {eb_o / EV:.6e} eV, gpu {eb_g / EV:.6e} eV", flush=True)
+    assert eb_o > 0 and abs(eb_g - eb_o) <= 1e-9 * abs(eb_o)
