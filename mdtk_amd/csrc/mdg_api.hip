@@ -95,10 +95,13 @@ struct mdg_ctx {
   int *cell_of_s = nullptr, *cnt_s = nullptr, *start_s = nullptr, *atoms_s = nullptr;
   int *cell_of_l = nullptr, *cnt_l = nullptr, *start_l = nullptr, *atoms_l = nullptr;
   unsigned char *handled_s = nullptr, *handled_l = nullptr;
-  int* flags = nullptr;   // [0] short, [1] long list, [2] LJ pair rows: rebuild requested by some system
+  int* flags = nullptr;   // [0] short, [1] long list, [2] LJ pair rows, [3] metal rows: rebuild requested by some system
   int* sflags = nullptr;  // [2*s + list] rebuild requested by system s
   int* pflags = nullptr;  // [s] LJ pair rows of system s to be rebuilt
   double pair_margin = 1.0e-8;  // LJ pair-row margin (cm)
+  int* eflags = nullptr;  // [s] metal rows of system s to be rebuilt
+  double metal_skin = 1.0e-8;   // metal-row skin (cm)
+  double metal_rcand = 0.0;     // metal-row radius: the embedded-atom members' largest range + skin
   int2* bonds = nullptr;
   int* nbonds = nullptr;
   int bond_cap = 0;
@@ -335,13 +338,20 @@ hipError_t queued(mdg_ctx* c, int& nq) {
   return e;
 }
 
-// Requests a rebuild of every system's LJ pair rows at the next force call.
+// Requests a rebuild of every system's LJ pair rows and metal rows at the
+// next force call (the rows are filtered from a long list just rebuilt).
 int mark_pair_rows(mdg_ctx* c) {
-  if (!c->A.pl || c->nsys == 0) return 0;
+  if (c->nsys == 0 || (!c->A.pl && !c->A.ml)) return 0;
   std::vector<int> one((size_t)c->nsys, 1);
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  HIPCHK(c, hipMemcpy(c->pflags, one.data(), one.size() * 4, hipMemcpyHostToDevice));
-  HIPCHK(c, hipMemcpy(c->flags + 2, one.data(), 4, hipMemcpyHostToDevice));
+  if (c->A.pl) {
+    HIPCHK(c, hipMemcpy(c->pflags, one.data(), one.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->flags + 2, one.data(), 4, hipMemcpyHostToDevice));
+  }
+  if (c->A.ml) {
+    HIPCHK(c, hipMemcpy(c->eflags, one.data(), one.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->flags + 3, one.data(), 4, hipMemcpyHostToDevice));
+  }
   return 0;
 }
 
@@ -603,9 +613,11 @@ int drain_trajectory(mdg_ctx* c) {
 // The skin test of every system (both stages).
 void sys_disp(mdg_ctx* c, double lim_s, double lim_l) {
   Atoms& A = c->A;
-  k_sys_disp_part<<<c->nred, 256, 0, c->stream>>>(c->rb, A.sys_off, A.disp_s, A.disp_l, A.disp_p, c->red_part);
+  k_sys_disp_part<<<c->nred, 256, 0, c->stream>>>(c->rb, A.sys_off, A.disp_s, A.disp_l, A.disp_p, A.disp_e,
+                                                 c->red_part);
   k_sys_disp_merge<<<c->nsys, 64, 0, c->stream>>>(c->rb_off, c->red_part, lim_s, lim_l, c->pair_margin,
-                                                  A.disp_p ? 1 : 0, c->sflags, c->pflags, c->flags, c->counters);
+                                                  A.disp_p ? 1 : 0, c->metal_skin, A.disp_e ? 1 : 0, c->sflags,
+                                                  c->pflags, c->eflags, c->flags, c->counters);
 }
 
 // One force call (FProxy::operator(), FProxy.cxx:45-61): lists, coordination,
@@ -629,6 +641,11 @@ void force_call(mdg_ctx* c) {
   if (c->slot_flj >= 0) {
     launch(c, "k_flj", [&] {
       k_flj<<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->dFLJ, A.grad, c->eatom + (size_t)c->slot_flj * n);
+    });
+  }
+  if (A.ml) {
+    launch(c, "k_metal_rows", [&] {
+      k_metal_rows<<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->flags + 3, c->eflags, c->metal_rcand);
     });
   }
   if (c->slot_eam >= 0) {
@@ -666,7 +683,7 @@ void force_call(mdg_ctx* c) {
                                                c->eatom + (size_t)c->slot_fbzl * n);
     });
   }
-  k_clear_flags<<<blocks(2LL * c->nsys), 256, 0, c->stream>>>(c->nsys, c->flags, c->sflags, c->pflags);
+  k_clear_flags<<<blocks(2LL * c->nsys), 256, 0, c->stream>>>(c->nsys, c->flags, c->sflags, c->pflags, c->eflags);
 }
 }  // namespace
 
@@ -749,6 +766,7 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
   build_fbm_table(ft2);
   if (m & MDG_POT_FBM) c->rc_long = std::max(c->rc_long, ft2.R1);  // FPairwise::getRcutoff
   if (m & MDG_POT_TB) c->rc_long = std::max(c->rc_long, tt.R1);    // TightBinding.hpp:55
+  c->metal_rcand = std::max((m & MDG_POT_ACKLAND) ? et.rmax : 0.0, (m & MDG_POT_TB) ? tt.R1 : 0.0) + c->metal_skin;
   if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
     g_create_err = "hipStreamCreate failed";
     delete c;
@@ -802,7 +820,7 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
     delete c;
     return 1;
   }
-  if (dalloc(c, c->flags, 3) || dalloc(c, c->counters, 8) || dalloc(c, c->tally, kTallySlots * kTallyStride) || dalloc(c, c->derr, 1) || dalloc(c, c->nbonds, 1) ||
+  if (dalloc(c, c->flags, 4) || dalloc(c, c->counters, 8) || dalloc(c, c->tally, kTallySlots * kTallyStride) || dalloc(c, c->derr, 1) || dalloc(c, c->nbonds, 1) ||
       dalloc(c, c->qcount, kQueues * kQStride)) {
     g_create_err = c->err;
     delete c;
@@ -970,6 +988,13 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
       HIPCHK(c, hipMemcpy(c->rb_off, ro.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice));
     }
   }
+  A.ml = nullptr;
+  A.ml_cnt = nullptr;
+  A.disp_e = nullptr;
+  c->eflags = nullptr;
+  if (c->slot_eam >= 0 || c->slot_tb >= 0)
+    r |= dalloc(c, A.ml, (size_t)n * kMetalRow) | dalloc(c, A.ml_cnt, n) | dalloc(c, A.disp_e, 3 * (size_t)n) |
+         dalloc(c, c->eflags, nsys);
   for (int k = 0; k < 2; k++) {
     auto& E = c->emb[k];
     E = mdg_ctx::EmbedSet{};
@@ -1048,6 +1073,8 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   HIPCHK(c, hipMemset(A.disp_s, 0, 3 * (size_t)n * 8));
   HIPCHK(c, hipMemset(A.disp_l, 0, 3 * (size_t)n * 8));
   if (A.disp_p) HIPCHK(c, hipMemset(A.disp_p, 0, 3 * (size_t)n * 8));
+  if (A.disp_e) HIPCHK(c, hipMemset(A.disp_e, 0, 3 * (size_t)n * 8));
+  if (A.ml_cnt) HIPCHK(c, hipMemset(A.ml_cnt, 0xff, (size_t)n * 4));  // -1: no metal row yet
   if (A.pl_cnt) HIPCHK(c, hipMemset(A.pl_cnt, 0xff, (size_t)n * 4));  // -1: no pair row yet
   HIPCHK(c, hipMemset(A.nls_cnt, 0, (size_t)n * 4));
   HIPCHK(c, hipMemset(A.nll_cnt, 0, (size_t)n * 4));
@@ -1062,9 +1089,10 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
         ek[i] = el[i] == 64 ? 0 : (k == 0 && el[i] == 108) ? 1 : (k == 0 && el[i] == 197) ? 2 : -1;
       HIPCHK(c, up(c->emb[k].e, ek.data(), n));
     }
-  int fl[3] = {1, 1, 1};
+  int fl[4] = {1, 1, 1, 1};
   std::vector<int> sfl(2 * (size_t)nsys, 1);
   HIPCHK(c, up(c->flags, fl, sizeof(fl)));
+  if (c->eflags) HIPCHK(c, up(c->eflags, sfl.data(), (size_t)nsys * 4));
   HIPCHK(c, up(c->sflags, sfl.data(), sfl.size() * 4));
   HIPCHK(c, up(c->pflags, sfl.data(), (size_t)nsys * 4));
   HIPCHK(c, up(c->dtp, c->dt_host.data(), (size_t)nsys * 8));
@@ -1429,7 +1457,8 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
     });
     launch(c, "k_sys_disp", [&] {
       k_sys_disp_merge<<<nsys, 64, 0, c->stream>>>(c->rb_off, c->red_part, lim_s, lim_l, c->pair_margin,
-                                                   A.disp_p ? 1 : 0, c->sflags, c->pflags, c->flags, c->counters);
+                                                   A.disp_p ? 1 : 0, c->metal_skin, A.disp_e ? 1 : 0, c->sflags,
+                                                   c->pflags, c->eflags, c->flags, c->counters);
     });
     force_call(c);
     launch(c, "k_kick", [&] {
@@ -1534,9 +1563,10 @@ int mdg_set_positions(mdg_ctx* c, const double* xyz, const int32_t* pbc_count, c
     }
   }
   if (!dr) {
-    int fl[3] = {1, 1, 1};
+    int fl[4] = {1, 1, 1, 1};
     std::vector<int> sfl(2 * (size_t)c->nsys, 1);
     HIPCHK(c, hipMemcpy(c->flags, fl, sizeof(fl), hipMemcpyHostToDevice));
+    if (c->eflags) HIPCHK(c, hipMemcpy(c->eflags, sfl.data(), (size_t)c->nsys * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->sflags, sfl.data(), sfl.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(c->pflags, sfl.data(), (size_t)c->nsys * 4, hipMemcpyHostToDevice));
     return 0;
@@ -1551,11 +1581,12 @@ int mdg_set_positions(mdg_ctx* c, const double* xyz, const int32_t* pbc_count, c
   }
   HIPCHK(c, hipMemcpy(c->A.disp_s, s.data(), 3 * (size_t)n * 8, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(c->A.disp_l, l.data(), 3 * (size_t)n * 8, hipMemcpyHostToDevice));
-  if (c->A.disp_p) {
-    HIPCHK(c, hipMemcpy(s.data(), c->A.disp_p, 3 * (size_t)n * 8, hipMemcpyDeviceToHost));
-    for (size_t q = 0; q < (size_t)3 * n; q++) s[q] += dr[q];
-    HIPCHK(c, hipMemcpy(c->A.disp_p, s.data(), 3 * (size_t)n * 8, hipMemcpyHostToDevice));
-  }
+  for (double* dd : {c->A.disp_p, c->A.disp_e})
+    if (dd) {
+      HIPCHK(c, hipMemcpy(s.data(), dd, 3 * (size_t)n * 8, hipMemcpyDeviceToHost));
+      for (size_t q = 0; q < (size_t)3 * n; q++) s[q] += dr[q];
+      HIPCHK(c, hipMemcpy(dd, s.data(), 3 * (size_t)n * 8, hipMemcpyHostToDevice));
+    }
   double lim_s = c->have_short ? 0.44 * c->rc_short : 1e300;
   double lim_l = c->have_long ? 0.44 * c->rc_long : 1e300;
   sys_disp(c, lim_s, lim_l);

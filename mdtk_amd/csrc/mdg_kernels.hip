@@ -306,39 +306,46 @@ __device__ __forceinline__ int tile_end(const int2 t, const int* __restrict__ sy
 }
 
 // The two largest displacement lengths of each list over one tile.
+// The two largest values per list over a 256-thread block -- v holds this
+// thread's (d1, d2) of the short list, the long list, the LJ pair rows and
+// the metal rows -- written to part[8 * blockIdx.x ..].
+__device__ __forceinline__ void top2_tile_out(double (&v)[8], double* __restrict__ part) {
+  __shared__ double sh[4][8];
+#pragma unroll
+  for (int k = 0; k < 4; k++) top2_wave(v[2 * k], v[2 * k + 1]);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < 8; k++) sh[w][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < (int)(blockDim.x >> 6); q++)
+#pragma unroll
+      for (int k = 0; k < 4; k++) top2_merge(x[2 * k], x[2 * k + 1], sh[q][2 * k], sh[q][2 * k + 1]);
+    double* o = part + 8 * (size_t)blockIdx.x;
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = x[k];
+  }
+}
+
 __global__ void __launch_bounds__(256) k_sys_disp_part(const int2* __restrict__ rb, const int* __restrict__ sys_off,
                                                        const double* __restrict__ ds, const double* __restrict__ dl,
-                                                       const double* __restrict__ dp, double* __restrict__ part) {
-  __shared__ double sh[4][6];
+                                                       const double* __restrict__ dp, const double* __restrict__ de,
+                                                       double* __restrict__ part) {
   const int2 t = rb[blockIdx.x];
   const int b = t.y, e = tile_end(t, sys_off);
-  double a1 = 0, a2 = 0, b1 = 0, b2 = 0, c1 = 0, c2 = 0;
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   auto len = [](const double* d, int i) {
     return sqrt(d[3 * i] * d[3 * i] + d[3 * i + 1] * d[3 * i + 1] + d[3 * i + 2] * d[3 * i + 2]);
   };
   for (int i = b + threadIdx.x; i < e; i += blockDim.x) {
-    top2_insert(len(ds, i), a1, a2);
-    top2_insert(len(dl, i), b1, b2);
-    if (dp) top2_insert(len(dp, i), c1, c2);
+    top2_insert(len(ds, i), v[0], v[1]);
+    top2_insert(len(dl, i), v[2], v[3]);
+    if (dp) top2_insert(len(dp, i), v[4], v[5]);
+    if (de) top2_insert(len(de, i), v[6], v[7]);
   }
-  top2_wave(a1, a2);
-  top2_wave(b1, b2);
-  top2_wave(c1, c2);
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    sh[w][0] = a1; sh[w][1] = a2; sh[w][2] = b1; sh[w][3] = b2; sh[w][4] = c1; sh[w][5] = c2;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double x1 = 0, x2 = 0, y1 = 0, y2 = 0, z1 = 0, z2 = 0;
-    for (int q = 0; q < (int)(blockDim.x >> 6); q++) {
-      top2_merge(x1, x2, sh[q][0], sh[q][1]);
-      top2_merge(y1, y2, sh[q][2], sh[q][3]);
-      top2_merge(z1, z2, sh[q][4], sh[q][5]);
-    }
-    double* o = part + 8 * (size_t)blockIdx.x;
-    o[0] = x1; o[1] = x2; o[2] = y1; o[3] = y2; o[4] = z1; o[5] = z2;
-  }
+  top2_tile_out(v, part);
 }
 
 // One wave per system, the skin test of NeighbourList::MovedTooMuch (the
@@ -347,19 +354,22 @@ __global__ void __launch_bounds__(256) k_sys_disp_part(const int2* __restrict__ 
 // own displacements pass the margin (only when have_p).
 __global__ void __launch_bounds__(64) k_sys_disp_merge(const int* __restrict__ rb_off, const double* __restrict__ part,
                                                        double lim_s, double lim_l, double lim_p, int have_p,
-                                                       int* __restrict__ sflags, int* __restrict__ pflags,
+                                                       double lim_e, int have_e, int* __restrict__ sflags,
+                                                       int* __restrict__ pflags, int* __restrict__ eflags,
                                                        int* __restrict__ flags, long long* __restrict__ rebuilds) {
   const int s = blockIdx.x;
-  double x1 = 0, x2 = 0, y1 = 0, y2 = 0, z1 = 0, z2 = 0;
+  double x1 = 0, x2 = 0, y1 = 0, y2 = 0, z1 = 0, z2 = 0, u1 = 0, u2 = 0;
   for (int q = rb_off[s] + threadIdx.x; q < rb_off[s + 1]; q += 64) {
     const double* o = part + 8 * (size_t)q;
     top2_merge(x1, x2, o[0], o[1]);
     top2_merge(y1, y2, o[2], o[3]);
     top2_merge(z1, z2, o[4], o[5]);
+    top2_merge(u1, u2, o[6], o[7]);
   }
   top2_wave(x1, x2);
   top2_wave(y1, y2);
   top2_wave(z1, z2);
+  top2_wave(u1, u2);
   if (threadIdx.x == 0) {
     int fs = (x1 + x2 > lim_s) ? 1 : 0;
     int fl = (y1 + y2 > lim_l) ? 1 : 0;
@@ -378,14 +388,70 @@ __global__ void __launch_bounds__(64) k_sys_disp_merge(const int* __restrict__ r
       pflags[s] = 1;
       atomicOr(&flags[2], 1);
     }
+    if (have_e && (fl || u1 + u2 > lim_e)) {
+      eflags[s] = 1;
+      atomicOr(&flags[3], 1);
+    }
   }
 }
 
-__global__ void k_clear_flags(int nsys, int* __restrict__ flags, int* __restrict__ sflags, int* __restrict__ pflags) {
+__global__ void k_clear_flags(int nsys, int* __restrict__ flags, int* __restrict__ sflags, int* __restrict__ pflags,
+                              int* __restrict__ eflags) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 3) flags[i] = 0;
+  if (i < 4) flags[i] = 0;
   if (i < 2 * nsys) sflags[i] = 0;
   if (i < nsys) pflags[i] = 0;
+  if (eflags && i < nsys) eflags[i] = 0;
+}
+
+// The metal rows of the flagged systems (their skin test restarts: disp_e
+// of a rebuilt atom is zeroed).  A row keeps the long row's order.
+__global__ void __launch_bounds__(256) k_metal_rows(Atoms A, Box bx, const int* __restrict__ flag,
+                                                    const int* __restrict__ eflags, double rcand) {
+  if (!*flag) return;
+  constexpr int LW = 16;
+  const int lane = threadIdx.x & 63;
+  const int sl = lane % LW;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const unsigned long long mine = ((1ull << LW) - 1ull) << (lane / LW * LW);
+  const int i = blockIdx.x * (blockDim.x / LW) + threadIdx.x / LW;
+  if (i >= A.n || !eflags[A.sys[i]]) return;  // group-uniform
+  if (sl == 0) {
+    A.disp_e[3 * i] = 0.0;
+    A.disp_e[3 * i + 1] = 0.0;
+    A.disp_e[3 * i + 2] = 0.0;
+  }
+  const double4 pi = A.p[i];
+  if ((int)pi.w != T_OTHER) {
+    if (sl == 0) A.ml_cnt[i] = 0;
+    return;
+  }
+  const int cnt = A.nll_cnt[i];
+  const int* L = A.nll + (size_t)i * A.cap_l;
+  int* my = A.ml + (size_t)i * kMetalRow;
+  const double* Lb = box_of(bx, i);
+  const double rc2 = rcand * rcand;
+  int m = 0;
+  for (int b0 = 0; b0 < cnt; b0 += LW) {  // group-uniform trip count
+    const int q = b0 + sl;
+    bool in = false;
+    int j = 0;
+    if (q < cnt) {
+      j = L[q];
+      const double4 pj = A.p[j];
+      if ((int)pj.w == T_OTHER) {
+        const double dx = mic(pi.x - pj.x, Lb[0]), dy = mic(pi.y - pj.y, Lb[1]), dz = mic(pi.z - pj.z, Lb[2]);
+        in = dx * dx + dy * dy + dz * dz < rc2;
+      }
+    }
+    const unsigned long long msk = __ballot(in) & mine;
+    if (in) {
+      const int slot = m + __popcll(msk & below);
+      if (slot < kMetalRow) my[slot] = j;
+    }
+    m += __popcll(msk);
+  }
+  if (sl == 0) A.ml_cnt[i] = (m <= kMetalRow) ? m : -1;  // -1: the density pass sweeps the long row
 }
 
 // Bonds (i<j) in the order of i's list, appended to one array; one atomic
@@ -689,8 +755,12 @@ __global__ void __launch_bounds__(256) k_eam_density(Atoms A, Box bx, const type
     }
     return;
   }
-  const int cnt = A.nll_cnt[i];
-  const int* L = A.nll + (size_t)i * A.cap_l;
+  int cnt = A.ml_cnt[i];  // the metal row, or the long row when it did not fit
+  const int* L = A.ml + (size_t)i * kMetalRow;
+  if (cnt < 0) {
+    cnt = A.nll_cnt[i];
+    L = A.nll + (size_t)i * A.cap_l;
+  }
   int* my = row + (size_t)i * kEamRow;
   const double4 pi = A.p[i];
   const double* Lb = box_of(bx, i);
@@ -851,14 +921,19 @@ __global__ void __launch_bounds__(256) k_fbm(Atoms A, Box bx, const FBMTable* __
 // the lengths of the atom's accumulated displacements come back for the skin
 // test.
 __device__ __forceinline__ void drift_atom(const Atoms& A, int i, double dt, bool live, double& ls, double& ll,
-                                           double& lp) {
+                                           double& lp, double& le) {
   double sx = A.disp_s[3 * i], sy = A.disp_s[3 * i + 1], sz = A.disp_s[3 * i + 2];
   double lx = A.disp_l[3 * i], ly = A.disp_l[3 * i + 1], lz = A.disp_l[3 * i + 2];
-  double px = 0, py = 0, pz = 0;
+  double px = 0, py = 0, pz = 0, ex = 0, ey = 0, ez = 0;
   if (A.disp_p) {
     px = A.disp_p[3 * i];
     py = A.disp_p[3 * i + 1];
     pz = A.disp_p[3 * i + 2];
+  }
+  if (A.disp_e) {
+    ex = A.disp_e[3 * i];
+    ey = A.disp_e[3 * i + 1];
+    ez = A.disp_e[3 * i + 2];
   }
   if (live && !A.fixed[i]) {
     double dx = A.vx[i] * dt + A.ax[i] * dt * dt / 2.0;
@@ -889,10 +964,19 @@ __device__ __forceinline__ void drift_atom(const Atoms& A, int i, double dt, boo
       A.disp_p[3 * i + 1] = py;
       A.disp_p[3 * i + 2] = pz;
     }
+    if (A.disp_e) {
+      ex += dx;
+      ey += dy;
+      ez += dz;
+      A.disp_e[3 * i] = ex;
+      A.disp_e[3 * i + 1] = ey;
+      A.disp_e[3 * i + 2] = ez;
+    }
   }
   ls = sqrt(sx * sx + sy * sy + sz * sz);
   ll = sqrt(lx * lx + ly * ly + lz * lz);
   lp = sqrt(px * px + py * py + pz * pz);
+  le = sqrt(ex * ex + ey * ey + ez * ez);
 }
 
 // The drift of one tile's atoms and the tile's part of the skin test (the
@@ -902,37 +986,20 @@ __device__ __forceinline__ void drift_atom(const Atoms& A, int i, double dt, boo
 __global__ void __launch_bounds__(256) k_drift_tile(Atoms A, const int2* __restrict__ rb,
                                                     const double* __restrict__ dtp, const double* __restrict__ simt,
                                                     const double* __restrict__ tfin, double* __restrict__ part) {
-  __shared__ double sh[4][6];
   const int2 t = rb[blockIdx.x];
   const int e = tile_end(t, A.sys_off);
   const double dt = dtp[t.x];
   const bool live = simt[t.x] < tfin[t.x];
-  double a1 = 0, a2 = 0, b1 = 0, b2 = 0, c1 = 0, c2 = 0;
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int i = t.y + threadIdx.x; i < e; i += blockDim.x) {
-    double ls, ll, lp;
-    drift_atom(A, i, dt, live, ls, ll, lp);
-    top2_insert(ls, a1, a2);
-    top2_insert(ll, b1, b2);
-    if (A.disp_p) top2_insert(lp, c1, c2);
+    double ls, ll, lp, le;
+    drift_atom(A, i, dt, live, ls, ll, lp, le);
+    top2_insert(ls, v[0], v[1]);
+    top2_insert(ll, v[2], v[3]);
+    if (A.disp_p) top2_insert(lp, v[4], v[5]);
+    if (A.disp_e) top2_insert(le, v[6], v[7]);
   }
-  top2_wave(a1, a2);
-  top2_wave(b1, b2);
-  top2_wave(c1, c2);
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-    sh[w][0] = a1; sh[w][1] = a2; sh[w][2] = b1; sh[w][3] = b2; sh[w][4] = c1; sh[w][5] = c2;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double x1 = 0, x2 = 0, y1 = 0, y2 = 0, z1 = 0, z2 = 0;
-    for (int q = 0; q < (int)(blockDim.x >> 6); q++) {
-      top2_merge(x1, x2, sh[q][0], sh[q][1]);
-      top2_merge(y1, y2, sh[q][2], sh[q][3]);
-      top2_merge(z1, z2, sh[q][4], sh[q][5]);
-    }
-    double* o = part + 8 * (size_t)blockIdx.x;
-    o[0] = x1; o[1] = x2; o[2] = y1; o[3] = y2; o[4] = z1; o[5] = z2;
-  }
+  top2_tile_out(v, part);
 }
 
 __device__ __forceinline__ void wrap(double& c, double L, int& cnt) {

@@ -47,6 +47,13 @@ struct Atoms {
   int cap_p;
   int *pl_cnt, *pl;
   double* disp_p;
+  // Metal rows (embedded-atom members): the metal partners of each metal
+  // atom within the members' largest range plus a skin when the row was
+  // built, in long-row order; ml_cnt[i] = -1 when they did not fit
+  // kMetalRow.  Rebuilt with the long list and when the two largest
+  // displacements since the last build (disp_e) add up to more than the skin.
+  int *ml_cnt, *ml;
+  double* disp_e;
   int maxb;                   // bond-list capacity per atom
   int bond_cap;               // capacity of the (i<j) bond array
   int* bcnt;
@@ -89,10 +96,16 @@ constexpr int kRedTile = 512;
 // flags[0..2]: some system rebuilds its short list / long list / LJ pair
 // rows; sflags[2*s+{0,1}] and pflags[s] per system.
 __global__ void k_sys_disp_part(const int2* rb, const int* sys_off, const double* ds, const double* dl,
-                                const double* dp, double* part);
+                                const double* dp, const double* de, double* part);
+// flags[3] / eflags[s]: metal rows to rebuild (with the long list, or past lim_e)
 __global__ void k_sys_disp_merge(const int* rb_off, const double* part, double lim_s, double lim_l, double lim_p,
-                                 int have_p, int* sflags, int* pflags, int* flags, long long* rebuilds);
-__global__ void k_clear_flags(int nsys, int* flags, int* sflags, int* pflags);
+                                 int have_p, double lim_e, int have_e, int* sflags, int* pflags, int* eflags,
+                                 int* flags, long long* rebuilds);
+__global__ void k_clear_flags(int nsys, int* flags, int* sflags, int* pflags, int* eflags);
+// Metal rows of the systems flagged in eflags (any flagged: *flag), partners
+// of type T_OTHER within rcand; 16 lanes per atom, blocks of 256.
+constexpr int kMetalRow = 128;
+__global__ void k_metal_rows(Atoms A, Box bx, const int* flag, const int* eflags, double rcand);
 __global__ void k_bond_enum(Atoms A, int2* out, int* count, int cap, unsigned* err);
 // Energies go to eatom (per-atom, per chain member), owned by the first atom
 // of the bond, pair or quadruplet.
@@ -154,6 +167,7 @@ struct TBModel {  // TightBinding.cxx:59-159
   static __device__ void pair(const Table* T, int a, int b, double r, double& v, double& dv);
   static __device__ void embed(const Table* T, double rho, double& F, double& Fp);
 };
+// The density pass sweeps the metal rows (the long row where ml_cnt < 0).
 template <class M>
 __global__ void k_eam_density(Atoms A, Box bx, const typename M::Table* T, const signed char* ek, double2* fe,
                               double* eatom, int* row, int* rcnt, unsigned* err);

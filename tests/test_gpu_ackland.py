@@ -135,3 +135,22 @@ def test_copper_slab_in_a_thermal_bath():
     eb_o, eb_g = o.bath_energy(), g.bath_energy()[0]
     print(f"bath energy oracle {eb_o / EV:.6e} eV, gpu {eb_g / EV:.6e} eV", flush=True)
     assert eb_o > 0 and abs(eb_g - eb_o) <= 1e-9 * abs(eb_o)
+
+
+@pytest.mark.parametrize("names,mask", [(("Ackland",), 64), (("Ackland", "TightBinding"), 64 | 256)])
+def test_flying_atom_trajectory(names, mask):
+    """A Cu atom flying at 0.1 A/fs into a 600 K copper surface, 200 steps of
+    0.5 fs: its displacement rebuilds the metal rows on their own 1 A skin
+    between long-list rebuilds (2.2 A skin), before and after the impact;
+    energies stay with the oracle's."""
+    b = S.jitter(S.fcc(64, 5, 5, 3, periodic=(True, True, False)), 0.05, 11)
+    b.vel = S.maxwell_velocities(b, 600.0, stream=12)
+    top = b.pos[:, 2].max()
+    pos = np.concatenate([b.pos, [[b.pbc[0] / 2, b.pbc[1] / 2, top + 8.0]]])
+    vel = np.concatenate([b.vel, [[0.0, 0.0, -0.1]]])  # A/fs
+    s = S.System(np.concatenate([b.elem, [64]]).astype(np.int32), pos, b.pbc, vel, name="Cu -> Cu(001)")
+    o, g = _pair(s, names, mask)
+    ro, rg = o.run(200, fixed_dt=0.5e-15), g.run(200, fixed_dt=0.5e-15)
+    assert np.abs(rg["epot"] - ro["epot"]).max() <= 1e-10 * np.abs(ro["epot"]).max()
+    assert np.abs(rg["ekin"] - ro["ekin"]).max() <= 1e-10 * np.abs(ro["ekin"]).max()
+    assert g.state()[0][-1, 2] < (top + 2.0) * S.ANG  # the atom reached the surface
