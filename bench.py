@@ -397,6 +397,13 @@ def main():
             drift[prec] = {"slope_eV_per_atom_per_ps": slope, "max_abs_dE_eV_per_atom": dmax}
         out["nve_drift"] = {"system": f"PE 8x12x8 ({small.n} atoms), 300 K, 2000 steps of 0.2 fs", **drift}
         log(f"nve drift {drift}")
+        gr = S.graphite(64, 64, 32)  # 1,048,576 atoms, AB graphite, 3-D periodic
+        gr.vel = S.maxwell_velocities(gr, 300.0, stream=5)
+        e = Engine.from_system(gr, potentials=("REBO", "AIREBO", "ETors"), precision="mixed")
+        ms_gr, _ = timed(e, a.steps, a.warmup, 0.5e-15)
+        out["graphite1m_mixed_value"] = gr.n * a.steps / (ms_gr / 1e3)
+        del e
+        log(f"graphite 1M mixed {out['graphite1m_mixed_value']:.4e} atom-steps/s")
         out["metal_ackland"] = metal_block(a.steps, a.warmup)
         log(f"metal block (Ackland): {out['metal_ackland']['value']:.4e} atom-steps/s, "
             f"{out['metal_ackland']['kernels_ms_per_step']}")
