@@ -730,7 +730,7 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
   if (c->cfg.max_short_nb <= 0) c->cfg.max_short_nb = 64;
   if (c->cfg.max_long_nb <= 0) c->cfg.max_long_nb = 512;
   if (c->cfg.max_short_nb != 64 || c->cfg.max_long_nb != 512) {
-    g_create_err = "list capacities are fixed at 64 (short) and 512 (long) in this build";
+    g_create_err = "list capacities are fixed at 64 (short) and 512 (long) in this library";
     delete c;
     return 1;
   }

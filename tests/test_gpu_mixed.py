@@ -1,9 +1,9 @@
 """Mixed-precision form and graph replay, on the GPU.
 
 Mixed precision (FP32 displacements, cutoffs and splines in Angstrom/eV; FP64
-coordination and bond-order sums, energies and force accumulation) is judged
-against the FP64 engine: per-atom force RMS error <= 1e-5 eV/A (the bar the
-task states), energy within 1e-6 relative.
+coordination and bond-order sums, energies and force accumulation) is checked
+against the FP64 engine: per-atom force RMS error <= 1e-5 eV/A (the mixed
+form's accuracy bar), energy within 1e-6 relative.
 """
 import os
 
