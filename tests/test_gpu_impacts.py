@@ -81,3 +81,27 @@ def test_common_usage_runs_one_trajectory(tmp_path):
     fin = ST.read_state(os.path.join(out, f"md{its:010d}"))
     assert np.abs(fin.pos - o.state()[0]).max() <= 1e-6 * 1e-8
     assert sorted(os.listdir(out))[0] != ".common" and I.run_common(str(tmp_path / "base"), out) is None
+
+
+def test_clock_stops_each_system_on_its_own():
+    """A batch where one system's simTimeFinal is already reached: it stands
+    still (positions, velocities, clock; dt_used 0) while the other runs as it
+    does alone."""
+    from mdtk_amd import Engine
+    a = S.bombard_target(300, 44, energy_ev=60.0)
+    b = S.bombard_target(300, 45, energy_ev=60.0)
+    eng = Engine.from_systems([a, b], potentials=POTS)
+    eng.set_time(0.0, 0.0, 1e-20, system=0)
+    eng.set_time(0.0, 1e-12, 1e-20, system=1)
+    x0 = eng.state()[0][: a.n].copy()
+    r = eng.run(60)
+    assert not r["dt"][:, 0].any() and r["dt"][:, 1].all()
+    x, v = eng.state()[:2]
+    assert np.array_equal(x[: a.n], x0)
+    one = Engine.from_system(b, potentials=POTS)
+    one.set_time(0.0, 1e-12, 1e-20)
+    r1 = one.run(60)
+    assert np.abs(r["epot"][:, 1] - r1["epot"]).max() <= 1e-12 * np.abs(r1["epot"]).max()
+    assert np.abs(r["dt"][:, 1] - r1["dt"]).max() <= 1e-12 * r1["dt"].max()
+    t = eng.time()
+    assert t[0, 0] == 0.0 and abs(t[1, 0] - r1["dt"].sum()) <= 1e-12 * t[1, 0]
