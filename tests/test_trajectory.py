@@ -51,3 +51,12 @@ def test_xva_text():
     assert len(lines) == 3 + 3 * 5 + 7 + 4
     assert lines[-4] == f"{0.0:.10e}" and lines[-2] == "0"
     assert T.xva_name(f) == "mde0000000000.xva"
+
+
+def test_record_layouts_match_the_header():
+    """mdg_frame is 48 bytes and mdg_atom_record 64 (include/mdg.h); the
+    Python views must read them field for field."""
+    assert FRAME_DTYPE.itemsize == 48 and RECORD_DTYPE.itemsize == 64
+    assert [FRAME_DTYPE.fields[k][1] for k in ("kind", "system", "iteration", "sim_time", "dt", "first", "count")] \
+        == [0, 4, 8, 16, 24, 32, 40]
+    assert [RECORD_DTYPE.fields[k][1] for k in ("x", "v", "pbc_count", "atom")] == [0, 24, 48, 60]
