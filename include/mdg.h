@@ -25,7 +25,8 @@ typedef struct mdg_ctx mdg_ctx;
 /* Potential chain members, in the order applications/common.h:46-68 adds
  * them: FBZL (ZBL ion-target), FLJ, Brenner (replaces REBO), REBO,
  * AIREBO-LJ (requires REBO), ETors, Ackland; then FBM and TightBinding,
- * which the reference ships but does not register (common.h:69-72). */
+ * which the reference ships but does not register (TightBinding is
+ * commented out at common.h:68-71; FBM is not named there). */
 enum {
   MDG_POT_FBZL = 1,
   MDG_POT_BRENNER = 2,
