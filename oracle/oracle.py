@@ -55,6 +55,12 @@ def lib():
         L.orc_set_bath_atoms.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_ubyte)]
         L.orc_bath_energy.argtypes = [ctypes.c_void_p, dp]
         L.orc_get_params.argtypes = [ctypes.c_void_p, dp]
+        L.orc_depos.argtypes = [dp, dp, ip, dp, dp, ip, dp]
+        L.orc_apply_pbc.argtypes = [dp, dp, ip]
+        L.orc_cos_theta.argtypes = [dp, ctypes.c_double, dp]
+        L.orc_cos_dihedral.argtypes = [dp, ctypes.c_double, dp]
+        L.orc_moved_too_much.argtypes = [ctypes.c_int, dp, ctypes.c_double, ip]
+        L.orc_cij_path.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, dp, ip, ip]
         _lib = L
     return _lib
 
@@ -157,6 +163,16 @@ class Oracle:
         self._chk(lib().orc_run(self.h, nsteps, fixed_dt, _p(ep), _p(ek), _p(dt), _p(nf)))
         return dict(epot=ep, ekin=ek, dt=dt, netforce=nf)
 
+    def cij_path(self, i: int, j: int):
+        """(weight, nodes) of the C_ij path stored for the LJ pair (i, j) at
+        the last energy_grad (nodes[0] = i; empty when there is none)."""
+        w = np.zeros(1)
+        hops = np.zeros(1, np.int32)
+        nodes = np.zeros(4, np.int32)
+        self._chk(lib().orc_cij_path(self.h, int(i), int(j), _p(w), _p(hops, ctypes.c_int),
+                                     _p(nodes, ctypes.c_int)))
+        return float(w[0]), [int(q) for q in nodes[:hops[0] + 1]] if hops[0] else []
+
     def state(self):
         x, v, a = (np.zeros((self.n, 3)) for _ in range(3))
         c = np.zeros((self.n, 3), np.int32)
@@ -170,3 +186,49 @@ def spline_eval(table: int, x: float, y: float = 0.0, z: float = 0.0, which: int
     if rc:
         raise OracleError(lib().orc_last_error().decode())
     return float(out[0])
+
+
+def _chk(rc):
+    if rc:
+        raise OracleError(lib().orc_last_error().decode())
+
+
+def depos(xa, La, ca, xb, Lb, cb):
+    """depos(a, b) with per-atom PBC vectors and counters (cm)."""
+    a = [np.ascontiguousarray(v, t) for v, t in ((xa, np.float64), (La, np.float64), (ca, np.int32),
+                                                 (xb, np.float64), (Lb, np.float64), (cb, np.int32))]
+    out = np.zeros(3)
+    _chk(lib().orc_depos(_p(a[0]), _p(a[1]), _p(a[2], ctypes.c_int), _p(a[3]), _p(a[4]),
+                         _p(a[5], ctypes.c_int), _p(out)))
+    return out
+
+
+def apply_pbc(xyz, L, pc):
+    x = np.array(xyz, np.float64)
+    c = np.array(pc, np.int32)
+    Lb = np.ascontiguousarray(L, np.float64)
+    _chk(lib().orc_apply_pbc(_p(x), _p(Lb), _p(c, ctypes.c_int)))
+    return x, c
+
+
+def cos_theta(xyz3, V=0.0):
+    """(cos, grad[3,3]) of CosTheta(ij, ik, V) for atoms i, j, k."""
+    x = np.ascontiguousarray(xyz3, np.float64).reshape(9)
+    out = np.zeros(10)
+    _chk(lib().orc_cos_theta(_p(x), float(V), _p(out)))
+    return out[0], out[1:].reshape(3, 3)
+
+
+def cos_dihedral(xyz4, V=0.0):
+    """(cos, grad[4,3]) of CosDihedral(ij, ik, jl, V) for atoms i, j, k, l."""
+    x = np.ascontiguousarray(xyz4, np.float64).reshape(12)
+    out = np.zeros(13)
+    _chk(lib().orc_cos_dihedral(_p(x), float(V), _p(out)))
+    return out[0], out[1:].reshape(4, 3)
+
+
+def moved_too_much(disp, rc):
+    d = np.ascontiguousarray(disp, np.float64)
+    out = np.zeros(1, np.int32)
+    _chk(lib().orc_moved_too_much(len(d), _p(d), float(rc), _p(out, ctypes.c_int)))
+    return bool(out[0])

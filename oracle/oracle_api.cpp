@@ -312,4 +312,111 @@ int orc_spline_eval(int table, double x, double y, double z, int which, double* 
     *out = r;
   });
 }
+// ---------------------------------------------------------------- pieces
+// Single functions of the restatement, for checks against the reference's
+// own compiled code (oracle/_ref) and against closed forms.
+
+// depos(a1, a2) with per-atom PBC vectors and counters (Atom.hpp:115-140).
+int orc_depos(const double* xa, const double* La, const int* ca, const double* xb, const double* Lb, const int* cb,
+              double* out) {
+  return guarded([&] {
+    Atom a, b;
+    a.coords = V3(xa[0], xa[1], xa[2]);
+    b.coords = V3(xb[0], xb[1], xb[2]);
+    a.PBC = V3(La[0], La[1], La[2]);
+    b.PBC = V3(Lb[0], Lb[1], Lb[2]);
+    for (int d = 0; d < 3; d++) {
+      a.pbc_count[d] = ca[d];
+      b.pbc_count[d] = cb[d];
+    }
+    V3 r = depos(a, b);
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+  });
+}
+
+// Atom::applyPBC (Atom.cxx:93-125) on one atom.
+int orc_apply_pbc(double* xyz, const double* L, int* pc) {
+  return guarded([&] {
+    Atom a;
+    a.coords = V3(xyz[0], xyz[1], xyz[2]);
+    a.PBC = V3(L[0], L[1], L[2]);
+    for (int d = 0; d < 3; d++) a.pbc_count[d] = pc[d];
+    a.applyPBC();
+    xyz[0] = a.coords.x;
+    xyz[1] = a.coords.y;
+    xyz[2] = a.coords.z;
+    for (int d = 0; d < 3; d++) pc[d] = a.pbc_count[d];
+  });
+}
+
+// CosTheta(ij, ik, V) for atoms i, j, k (open boundaries): out[0] = cos,
+// out[1..9] the gradients added to i, j, k.
+int orc_cos_theta(const double* xyz, double V, double* out) {
+  return guarded([&] {
+    Atom at[3];
+    for (int q = 0; q < 3; q++) at[q].coords = V3(xyz[3 * q], xyz[3 * q + 1], xyz[3 * q + 2]);
+    Pair ij(at[0], at[1], true), ik(at[0], at[2], true);
+    out[0] = CosTheta(ij, ik, V);
+    for (int q = 0; q < 3; q++) {
+      out[1 + 3 * q] = at[q].grad.x;
+      out[2 + 3 * q] = at[q].grad.y;
+      out[3 + 3 * q] = at[q].grad.z;
+    }
+  });
+}
+
+// CosDihedral(ij, ik, jl, V) for atoms i, j, k, l: out[0] = cos, out[1..12]
+// the gradients added to i, j, k, l.
+int orc_cos_dihedral(const double* xyz, double V, double* out) {
+  return guarded([&] {
+    Atom at[4];
+    for (int q = 0; q < 4; q++) at[q].coords = V3(xyz[3 * q], xyz[3 * q + 1], xyz[3 * q + 2]);
+    Pair ij(at[0], at[1], true), ik(at[0], at[2], true), jl(at[1], at[3], true);
+    out[0] = CosDihedral(ij, ik, jl, V);
+    for (int q = 0; q < 4; q++) {
+      out[1 + 3 * q] = at[q].grad.x;
+      out[2 + 3 * q] = at[q].grad.y;
+      out[3 + 3 * q] = at[q].grad.z;
+    }
+  });
+}
+
+// NeighbourList::MovedTooMuch (NeighbourList.cxx:108-133) over disp[n*3].
+int orc_moved_too_much(int n, const double* disp, double rc, int* out) {
+  return guarded([&] {
+    NeighbourList nl;
+    nl.Rcutoff = rc;
+    nl.init(n);
+    for (int i = 0; i < n; i++) nl.displacements[i] = V3(disp[3 * i], disp[3 * i + 1], disp[3 * i + 2]);
+    *out = nl.movedTooMuch() ? 1 : 0;
+  });
+}
+
+// The C_ij path the AIREBO-LJ member stored for (i, j) at the last force call
+// (AIREBO_LJ.cxx:289-358): *w the path weight (0 when there is none) and
+// nodes[0..hops] the atoms along it (nodes[0] = i); *hops = 0 for none.
+int orc_cij_path(void* p, int i, int j, double* w, int* hops, int* nodes) {
+  Handle* h = static_cast<Handle*>(p);
+  return guarded([&] {
+    AIREBO_LJ* lj = nullptr;
+    for (auto* q : h->sl.fpot.pots)
+      if (auto* c = dynamic_cast<AIREBO_LJ*>(q)) lj = c;
+    if (!lj) throw OracleError("no AIREBO-LJ member");
+    if (i < 0 || j < 0 || (size_t)i >= lj->CA.size() || (size_t)j >= lj->CA.size())
+      throw OracleError("atom index out of range (call energy_grad first)");
+    *w = 0.0;
+    *hops = 0;
+    auto it = lj->CA[i].find((size_t)j);
+    if (it == lj->CA[i].end()) return;
+    *w = it->second.w;
+    const auto& path = it->second.path;
+    *hops = (int)path.size();
+    for (size_t q = 0; q < path.size(); q++) {
+      nodes[q] = (int)path[q].a1->globalIndex;
+      nodes[q + 1] = (int)path[q].a2->globalIndex;
+    }
+  });
+}
 }
