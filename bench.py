@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the single-system PE and drift figures")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="profiling runs: only the mixed batched engine (tuned, timed, kernel times), no CPU leg")
     ap.add_argument("--workload", choices=("bombard", "metal"), default="bombard",
                     help="metal: only the Ackland Cu-crystal figure, as its own JSON line")
     return ap.parse_args()
@@ -299,12 +301,13 @@ def main():
     natoms = sum(s.n for s in batch)
     log(f"batch: {len(batch)} trajectories, {natoms} atoms")
     cpu = None
-    if not a.no_cpu:  # before any GPU call: the oracle processes are forked from this one
+    if not a.no_cpu and not a.mixed_only:  # before any GPU call: the oracle processes are forked from this one
         cpu = cpu_baseline(batch, a.cpu_seconds)
         log(f"cpu baseline: {cpu['value']:.4e} atom-steps/s on {cpu['cores']} processes")
 
-    f_rms = force_rms(batch)
-    log(f"mixed force RMS error vs FP64: {f_rms:.3e} eV/A")
+    if not a.mixed_only:
+        f_rms = force_rms(batch)
+        log(f"mixed force RMS error vs FP64: {f_rms:.3e} eV/A")
 
     # headline: mixed + batched, one graph launch per step for all trajectories
     emx = Engine.from_systems(batch, potentials=POTS, etors_compat=False, precision="mixed")
@@ -316,6 +319,13 @@ def main():
     kt_mx = kernel_times(emx, a.steps, dt)
     cnt_mx = emx.counters()
     del emx
+    if a.mixed_only:
+        print(json.dumps({"metric": METRIC, "value": v_mx, "unit": "atom-steps/s", "n_gpus": 1, "steps": a.steps,
+                          "warmup": a.warmup, "ms_per_step": ms_mx / a.steps, "mode": "mixed batched only",
+                          "autotune": tune_mx, "roofline": roofline(kt_mx, cnt_mx, natoms, tune_mx),
+                          "kernels_ms_per_step": {k: v[0] / max(a.steps, 1) for k, v in kt_mx.items()},
+                          "counters": cnt_mx}), flush=True)
+        return
 
     # FP64 batched (same kernels in FP64)
     e64 = Engine.from_systems(batch, potentials=POTS, etors_compat=False)

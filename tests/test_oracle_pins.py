@@ -1,0 +1,354 @@
+"""Closed-form and brute-force pins of the oracle functions that finite
+differences alone cannot hold (CPU).
+
+Each test builds a geometry in which every factor of the energy is known
+independently of the oracle -- textbook constants, published node values of
+the reference's tables, a numpy dihedral, an enumeration of all bond paths --
+so that a wrong constant, a sign, an index order or a transposed argument in
+the oracle fails one of them:
+
+* FBZL: the universal ZBL screened Coulomb potential (Ziegler, Biersack and
+  Littmark 1985: coefficients 0.18175, 0.50986, 0.28022, 0.02817, exponents
+  3.1998, 0.94229, 0.4029, 0.20162, screening length 0.8854 a0 /
+  (Z1^0.23 + Z2^0.23)) with the reference's a0 = 0.53 A (FBZL.cxx:36) and
+  e = 4.8e-10 esu (consts.cxx:29), times the 5.0-5.5 A cosine switch.
+* ETors: one quad k-i-j-l at set dihedral angles, E = w_ki w_ij w_jl zeta
+  [(256/405) x^5 - 0.1], x = (1 + cos phi)/2 with phi the textbook dihedral
+  (0 = cis) and zeta_CC, zeta_CH, zeta_HH = 0.3079, 0.1787, 0.1250 eV
+  (Stuart, Tutein and Harrison 2000; AIREBO_ETors.cxx:100-133).
+* pi_dh: twisting ethylene about its C=C bond changes only the dihedral
+  term, E(tau) - E(0) = T_CC(2,2,1) * 4 sin^2(tau) * VA_CC(r).
+* The H-containing REBO terms: methane (G_C2 node, P_CH(3,0), the H side
+  without P), linear H3 (G_H(-1) and the sign and scale of the H-centred
+  ExpTerm).
+* C_ij: every walk of at most three REBO bonds enumerated in the reference's
+  loop order, the best weight and the tie rules of AIREBO_LJ.cxx:311,331,352.
+* The adaptive time step of SimLoop.cxx:394-416.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from mdtk_amd import systems as S
+from tests.helpers import ANG, EV
+from tests.test_params import REF, _pair_energy
+
+from oracle.oracle import (Oracle, POT_AIREBO, POT_ETORS, POT_FBZL, POT_REBO)
+
+NO = S.NO_PBC
+OPEN = np.array([NO, NO, NO])
+E_ESU = 4.8e-10
+ZBL_A = (0.18175, 0.50986, 0.28022, 0.02817)
+ZBL_B = (3.1998, 0.94229, 0.4029, 0.20162)
+ZNUM = {S.H: 1, S.C: 6, S.AR: 18, 64: 29, 131: 54, 108: 47, 197: 79}
+
+
+def _sys(elem, pos):
+    return S.System(np.array(elem, np.int32), np.array(pos, float) + 20.0, OPEN.copy())
+
+
+def _energy(sys_, pots, **kw):
+    e, g = Oracle.from_system(sys_, pots=pots, **kw).energy_grad()
+    return e / EV, g
+
+
+def _cos_switch(r, r1, r2):
+    if r < r1:
+        return 1.0
+    if r < r2:
+        return 0.5 * (1 + math.cos(math.pi * (r - r1) / (r2 - r1)))
+    return 0.0
+
+
+# ------------------------------------------------------------------ FBZL
+def zbl_ev(z1, z2, r):
+    """ZBL universal potential in eV at r Angstrom (textbook form)."""
+    a_s = 0.8854 * 0.53 / (z1 ** 0.23 + z2 ** 0.23)  # Angstrom
+    phi = sum(a * math.exp(-b * r / a_s) for a, b in zip(ZBL_A, ZBL_B))
+    return z1 * z2 * E_ESU ** 2 / (r * ANG) * phi / EV
+
+
+@pytest.mark.parametrize("pair", [(S.AR, S.H), (S.AR, S.C), (S.AR, S.AR), (131, 64)])
+@pytest.mark.parametrize("r", [0.6, 1.5, 3.0, 4.99, 5.2, 5.45, 5.6])
+def test_fbzl_zbl_closed_form(pair, r):
+    e, _ = _energy(_sys(pair, [[0, 0, 0], [r, 0, 0]]), POT_FBZL)
+    # the reference's screening constant is 0.88534 a0 (FBZL.cxx:81); the
+    # textbook's 0.8854 differs in the fifth digit, which moves the energy by
+    # up to a few 1e-4 through exp(-b r / a_s)
+    want = zbl_ev(ZNUM[pair[0]], ZNUM[pair[1]], r) * _cos_switch(r, 5.0, 5.5)
+    assert abs(e[0] - want) <= 2e-3 * abs(want) + 1e-30
+    # and exactly against the reference's own constant
+    a_s = 0.88534 * 0.53 / (ZNUM[pair[0]] ** 0.23 + ZNUM[pair[1]] ** 0.23)
+    phi = sum(a * math.exp(-b * r / a_s) for a, b in zip(ZBL_A, ZBL_B))
+    exact = ZNUM[pair[0]] * ZNUM[pair[1]] * E_ESU ** 2 / (r * ANG) * phi / EV * _cos_switch(r, 5.0, 5.5)
+    assert abs(e[0] - exact) <= 1e-12 * abs(exact) + 1e-30
+
+
+def test_fbzl_leaves_hydrocarbon_pairs_alone():
+    """FBZL handles ion-ion and ion-target pairs only (FBZL.cxx:59-68)."""
+    e, g = _energy(_sys([S.C, S.H, S.C], [[0, 0, 0], [1.1, 0, 0], [0, 1.5, 0]]), POT_FBZL)
+    assert e[0] == 0.0 and not g.any()
+
+
+# ----------------------------------------------------------------- ETors
+ZETA = {(S.C, S.C): 0.3079, (S.C, S.H): 0.1787, (S.H, S.C): 0.1787, (S.H, S.H): 0.1250}
+
+
+def _quad(ek, el, phi_deg, r_ik, r_jl, r_ij=1.54, theta_deg=110.0):
+    """Atoms [k, i, j, l]: i at 0, j on +x, k and l at angle theta to the
+    i-j axis, l rotated by the dihedral phi about it (phi = 0: cis)."""
+    th = math.radians(theta_deg)
+    ph = math.radians(phi_deg)
+    i = np.zeros(3)
+    j = np.array([r_ij, 0, 0])
+    k = i + r_ik * np.array([math.cos(th), math.sin(th), 0])
+    l = j + r_jl * np.array([-math.cos(th), math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph)])
+    return _sys([ek, S.C, S.C, el], [k, i, j, l]), (k, i, j, l)
+
+
+def _dihedral(k, i, j, l):
+    """Textbook dihedral k-i-j-l (0 = cis), from the two plane normals."""
+    b1, b2, b3 = i - k, j - i, l - j
+    n1, n2 = np.cross(b1, b2), np.cross(b2, b3)
+    m1 = np.cross(n1, b2 / np.linalg.norm(b2))
+    return math.atan2(np.dot(m1, n2), np.dot(n1, n2))
+
+
+@pytest.mark.parametrize("ends", [(S.C, S.C), (S.H, S.H), (S.C, S.H), (S.H, S.C)])
+@pytest.mark.parametrize("phi", [0.0, 60.0, 90.0, 120.0, 180.0, 250.0])
+def test_etors_single_quad_closed_form(ends, phi):
+    ek, el = ends
+    r_ik = 1.54 if ek == S.C else 1.09
+    r_jl = 1.54 if el == S.C else 1.09
+    sys_, pts = _quad(ek, el, phi, r_ik, r_jl)
+    x = 0.5 * (1 + math.cos(_dihedral(*pts)))
+    want = ZETA[(ek, el)] * ((256.0 / 405.0) * x ** 5 - 0.1)
+    for compat in (True, False):  # the compat flag changes the force, not the energy
+        e, _ = _energy(sys_, POT_ETORS, etors_compat=compat)
+        assert abs(e[0] - want) <= 1e-12, (ends, phi, compat)
+
+
+def test_etors_cutoff_weights():
+    """A C-C arm at 1.85 A sits half way through the 1.7-2.0 A switch
+    (w = 1/2); the quad's energy halves."""
+    sys_, pts = _quad(S.C, S.C, 75.0, 1.54, 1.85)
+    x = 0.5 * (1 + math.cos(_dihedral(*pts)))
+    w = _cos_switch(1.85, 1.7, 2.0)
+    assert abs(w - 0.5) < 1e-15
+    e, _ = _energy(sys_, POT_ETORS)
+    assert abs(e[0] - w * ZETA[(S.C, S.C)] * ((256.0 / 405.0) * x ** 5 - 0.1)) <= 1e-12
+
+
+def test_etors_skips_collinear_arms():
+    """|sin theta| < 0.1 skips the quad (AIREBO_ETors.cxx:68-69)."""
+    sys_, _ = _quad(S.C, S.C, 40.0, 1.54, 1.54, theta_deg=175.0)
+    e, _ = _energy(sys_, POT_ETORS)
+    assert e[0] == 0.0
+
+
+# ------------------------------------------------------------------ REBO
+def _va(r, p):
+    return _pair_energy(r, REF[p])[1]
+
+
+def _vr(r, p):
+    return _pair_energy(r, REF[p])[0]
+
+
+def _ethylene(tau_deg, r_cc=1.33, r_ch=1.09):
+    tau = math.radians(tau_deg)
+    c1, c2 = np.zeros(3), np.array([r_cc, 0, 0])
+    pos = [c1, c2]
+    for ph in (0.0, math.pi):
+        pos.append(c1 + r_ch * np.array([math.cos(math.radians(120)), math.sin(math.radians(120)) * math.cos(ph),
+                                         math.sin(math.radians(120)) * math.sin(ph)]))
+    for ph in (tau, tau + math.pi):
+        pos.append(c2 + r_ch * np.array([-math.cos(math.radians(120)), math.sin(math.radians(120)) * math.cos(ph),
+                                         math.sin(math.radians(120)) * math.sin(ph)]))
+    return _sys([S.C, S.C, S.H, S.H, S.H, S.H], pos)
+
+
+@pytest.mark.parametrize("tau", [30.0, 45.0, 90.0, 135.0])
+def test_pi_dh_twisted_ethylene(tau):
+    """Each C has Nt = 3 (N_ij = 2), no other carbon (Nconj = 1): twisting
+    by tau leaves every bond length, angle and coordination unchanged and
+    gives all four H-C=C-H dihedrals 1 - cos^2 = sin^2(tau), so only
+    pi_dh = T_CC(2,2,1) * 4 sin^2(tau) moves, with T_CC(2,2,1) = -0.035140
+    (AIREBO/cubici.cxx Func_Tij, Stuart 2000)."""
+    e0, _ = _energy(_ethylene(0.0), POT_REBO)
+    e1, _ = _energy(_ethylene(tau), POT_REBO)
+    want = -0.035140 * 4 * math.sin(math.radians(tau)) ** 2 * _va(1.33, 0)
+    assert want > 0  # twisting costs energy
+    assert abs((e1[0] - e0[0]) - want) <= 1e-10
+
+
+def test_methane_closed_form():
+    """CH4 (C-H 1.09 A, tetrahedral; H-H 1.78 A > R2): for each C-H bond the
+    C side has N_ij = 3, so G = G_C2(-1/3) = 0.097321 (Sprime(3; 2, 3) = 0),
+    P_CH(N_H = 3, N_C = 0) = -0.303928 (cubici.cxx HCH[3][0]); the H side has
+    no other neighbour and no P; pi_rc_CH(3, 0, 1) = 0.  E = 4 (VR + b VA)."""
+    d = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]]) / math.sqrt(3) * 1.09
+    sys_ = _sys([S.C, S.H, S.H, S.H, S.H], np.vstack([np.zeros(3), d]))
+    e, _ = _energy(sys_, POT_REBO)
+    D = 1 + 3 * 0.097321 - 0.303928
+    b = 0.5 * (D ** -0.5 + 1.0)
+    want = 4 * (_vr(1.09, 1) + b * _va(1.09, 1))
+    assert abs(e[0] - want) <= 1e-9 * abs(want)
+    # P_CH's arguments in the other order, P_CH(0, 3) = -0.307585, would move it
+    D2 = 1 + 3 * 0.097321 - 0.307585
+    assert abs(4 * (0.5 * (D2 ** -0.5 + 1.0) - b) * _va(1.09, 1)) > 1e-4
+
+
+def test_linear_h3_g_h_and_expterm():
+    """H-H-H on a line, 0.9 and 1.0 A (both inside R1 = 1.1 A; the ends 1.9 A
+    apart, beyond R2 = 1.7 A).  The centre sees its two partners at cos = -1,
+    G_H(-1) = 11.2357 (cubici.cxx FuncG_H), each weighted by the H-centred
+    ExpTerm exp((4/A)((rho - r_ik) - (rho - r_ij))) = exp(4 (r_ij - r_ik)/A);
+    the ends have no other neighbour; pi_rc_HH(1, 0, 1) = 0."""
+    r1, r2 = 0.9, 1.0
+    sys_ = _sys([S.H, S.H, S.H], [[-r1, 0, 0], [0, 0, 0], [r2, 0, 0]])
+    e, _ = _energy(sys_, POT_REBO)
+    b1 = 0.5 * ((1 + 11.2357 * math.exp(4 * (r1 - r2))) ** -0.5 + 1.0)
+    b2 = 0.5 * ((1 + 11.2357 * math.exp(4 * (r2 - r1))) ** -0.5 + 1.0)
+    want = _vr(r1, 2) + b1 * _va(r1, 2) + _vr(r2, 2) + b2 * _va(r2, 2)
+    assert abs(e[0] - want) <= 1e-10 * abs(want)
+
+
+# ------------------------------------------------------------------- C_ij
+REBO_R = {(0, 0): (1.7, 2.0), (0, 1): (1.3, 1.8), (1, 0): (1.3, 1.8), (1, 1): (1.1, 1.7)}
+
+
+def _brute_paths(sys_):
+    """Per ordered pair (i, j): the C_ij path the reference keeps, from every
+    walk of <= 3 REBO bonds in its loop order (REBO list rows ascending, a
+    hop a->b kept when r_ab <= R2(a, b)): the largest weight, then the
+    fewest bonds, then the first met (AIREBO_LJ.cxx:289-358).  Distances and
+    switches in cm as AtomsPair computes them (AtomsPair.hpp:132-156)."""
+    _, x, _, _, _ = sys_.to_cgs()
+    el = sys_.elem
+    n = len(el)
+    t = [0 if e == S.C else 1 for e in el]
+
+    def dist(a, b):
+        dx, dy, dz = x[a, 0] - x[b, 0], x[a, 1] - x[b, 1], x[a, 2] - x[b, 2]
+        return math.sqrt(dx * dx + dy * dy + dz * dz)
+
+    d = np.array([[dist(a, b) for b in range(n)] for a in range(n)])
+    nl = [[b for b in range(n) if b != a and d[a, b] * d[a, b] < (1.44 * 2.0 * ANG) ** 2] for a in range(n)]
+
+    def hop(a, b):
+        r1, r2 = (v * ANG for v in REBO_R[(t[a], t[b])])
+        r = d[a, b]
+        if r > r2:
+            return None
+        if r < r1:
+            return 1.0
+        if r < r2:
+            return (1.0 + math.cos(math.pi * (r - r1) / (r2 - r1))) / 2.0
+        return 0.0
+
+    best = {}
+    order = 0
+    for i in range(n):
+        for a in nl[i]:
+            f1 = hop(i, a)
+            if f1 is None:
+                continue
+            cand = [(a, f1, (i, a))]
+            for b in nl[a]:
+                f2 = hop(a, b)
+                if f2 is None:
+                    continue
+                cand.append((b, f1 * f2, (i, a, b)))
+                for c in nl[b]:
+                    f3 = hop(b, c)
+                    if f3 is None:
+                        continue
+                    cand.append((c, f1 * f2 * f3, (i, a, b, c)))
+            for tgt, w, path in cand:
+                order += 1
+                if w <= 0.0:
+                    continue
+                key = (-w, len(path), order)
+                if (i, tgt) not in best or key < best[(i, tgt)][0]:
+                    best[(i, tgt)] = (key, w, list(path))
+    return best
+
+
+def _check_paths(sys_):
+    o = Oracle.from_system(sys_, pots=POT_REBO | POT_AIREBO)
+    o.energy_grad()
+    best = _brute_paths(sys_)
+    n = sys_.n
+    checked = 0
+    for i in range(n):
+        for j in range(i + 1, n):
+            w, nodes = o.cij_path(i, j)
+            if (i, j) in best:
+                _, wb, pb = best[(i, j)]
+                assert abs(w - wb) <= 1e-15, (i, j, w, wb)
+                assert nodes == pb, (i, j, nodes, pb)
+                checked += 1
+            else:
+                assert w == 0.0 and nodes == []
+    return checked
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_cij_paths_brute_force(seed):
+    """Jittered hydrocarbon clusters (many bonds in the switching zones)."""
+    s = S.jitter(S.hydrocarbon_cluster(40, seed), 0.12, seed + 10)
+    assert _check_paths(s) > 50
+
+
+def test_cij_tie_rules():
+    """Exact ties: a ring i-m1-j-m2 with the four bonds at 1.85 A (f = 1/2
+    each; m1-m2 2.63 A apart, no bond) gives two 2-bond paths of the same
+    weight 1/4: the first in loop order (through m1) must win.  A second
+    ring whose direct bond i-j sits in the switch with f = 1/4 ties the
+    2-bond paths' weight: the single bond must win."""
+    h = math.sqrt(1.85 ** 2 - 1.3 ** 2)
+    # mirror images y = +-h about y = 0 (no offset), so both arms round alike
+    ring = S.System(np.array([S.C] * 4, np.int32), np.array([[0, 0, 0], [1.3, h, 0], [2.6, 0, 0], [1.3, -h, 0]]),
+                    OPEN.copy())
+    assert _check_paths(ring) > 0
+    o = Oracle.from_system(ring, pots=POT_REBO | POT_AIREBO)
+    o.energy_grad()
+    w, nodes = o.cij_path(0, 2)
+    assert abs(w - 0.25) < 1e-12 and nodes == [0, 1, 2]
+    # direct bond with f = 1/4: r = 1.7 + 0.3 acos(-1/2)/pi = 1.9 A
+    r_dir = 1.7 + 0.3 * math.acos(2 * 0.25 - 1) / math.pi
+    h2 = math.sqrt(1.85 ** 2 - (r_dir / 2) ** 2)
+    ring2 = S.System(np.array([S.C] * 4, np.int32),
+                     np.array([[0, 0, 0], [r_dir / 2, h2, 0], [r_dir, 0, 0], [r_dir / 2, -h2, 0]]), OPEN.copy())
+    o2 = Oracle.from_system(ring2, pots=POT_REBO | POT_AIREBO)
+    o2.energy_grad()
+    w2, nodes2 = o2.cij_path(0, 2)
+    best = _brute_paths(ring2)[(0, 2)]
+    assert nodes2 == best[2] and abs(w2 - best[1]) <= 1e-15
+    assert _check_paths(ring2) > 0
+
+
+# ------------------------------------------------------------- time step
+@pytest.mark.parametrize("v", [1.0e4, 5.0e6, 3.0e7, 1.0e11, 1.0e14])
+def test_adaptive_dt_rule(v):
+    """dt = 0.05 A / v_max clamped to [1e-20, 5e-16] s, v_max over atoms
+    with neighbours only (SimLoop.cxx:394-416): a fast Ar atom that no chain
+    member handles does not count; the first step takes 1e-20 s
+    (SimLoop.cxx:50).  At 1e14 cm/s the dimer flies 100 A apart in that first
+    step, the skin test rebuilds the lists inside the same step, nobody has
+    neighbours when v_max is taken, and dt = 5e-16 s."""
+    sys_ = _sys([S.C, S.C, S.AR], [[0, 0, 0], [1.4, 0, 0], [30, 0, 0]])
+    vel = np.array([[-v, 0.3 * v, 0], [v, -0.3 * v, 0], [0, 0, 3.0e8]]) / (ANG / S.FS)  # A/fs
+    sys_.vel = vel
+    o = Oracle.from_system(sys_, pots=POT_REBO | POT_AIREBO)
+    r0 = o.run(1)
+    assert r0["dt"][0] == 1e-20
+    x, vcur, _, _ = o.state()
+    vmax = np.linalg.norm(vcur[:2], axis=1).max()
+    bonded = np.linalg.norm(x[0] - x[1]) < 1.44 * 2.0 * ANG
+    want = min(5e-16, max(1e-20, 0.05e-8 / vmax)) if bonded else 5e-16
+    assert bonded == (v < 1e13)
+    r1 = o.run(1)
+    assert r1["dt"][0] == want
