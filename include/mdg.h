@@ -243,13 +243,16 @@ int mdg_kernel_time(mdg_ctx* ctx, const char* kernel, double* total_ms, int64_t*
 /* Autotuner: times candidate launch shapes of the force kernels on the
  * context's own systems (reps force calls per candidate, which leave the
  * trajectory unchanged) and keeps the fastest for every later call: the
+ * order in which the scattering kernels hand work to the XCDs (dealt in
+ * turn, or a contiguous eighth of the work per XCD), the
  * waves-per-SIMD variant of the bond-order kernels (1-2; 1-4 in the mixed form), the shape of
  * the LJ kernel (shape 0-3: lanes per atom 64, 64 held to 3 waves/SIMD, 32,
  * 16; mixed form also 4-5: 16 lanes held to 4 or 3 waves/SIMD) and
  * the grid sizes of the bond, LJ, b* and torsion kernels.  report (may be
- * null) receives 9 values: occ_bonds, grid_bonds, occ_bstar, grid_bstar,
+ * null) receives 10 values: occ_bonds, grid_bonds, occ_bstar, grid_bstar,
  * grid_lj, grid_tors (0 = built-in heuristic), the tuned kernels' summed ms
- * per call before and after, and the LJ shape (0-5).  No reference
+ * per call before and after, the LJ shape (0-5) and the XCD order (0: dealt
+ * in turn, 1: contiguous eighths).  No reference
  * counterpart. */
 int mdg_autotune(mdg_ctx* ctx, int reps, double* report);
 

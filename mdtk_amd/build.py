@@ -3,34 +3,49 @@
 The library is built in-tree so that it travels with the source tree; the
 Python package loads it with ctypes.  FP64 kernels are compiled without FMA
 contraction (-ffp-contract=off) so that every a*b+c rounds twice, as in the
-reference's x86-64 build, and without packed-FP32 instructions.
+reference's x86-64 build, and without packed-FP32 instructions.  The sources
+compile in parallel into mdtk_amd/_build and link into one library.
 """
 from __future__ import annotations
 
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmdg.so")
 SOURCES = ["mdg_api.hip", "mdg_kernels.hip", "mdg_force.hip", "mdg_brenner.hip", "mdg_tables.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
+FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-         "-Wno-unused-result", "-Wno-unused-value", "-Wl,-z,defs",  # an unresolved symbol fails the link
+         "-Wno-unused-result", "-Wno-unused-value",
          "-I" + os.path.join(HERE, "..", "include")]
+LINK = ["--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,-z,defs"]  # an unresolved symbol fails the link
 
 
-def build(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    cmd = [HIPCC, *FLAGS, "-o", LIB + ".tmp", *srcs]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libmdg.so")
+    return r.returncode
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    objs = [os.path.join(OBJ, s.rsplit(".", 1)[0] + ".o") for s in SOURCES]
+    cmds = [[HIPCC, *FLAGS, "-c", os.path.join(CSRC, s), "-o", o]
+            for s, o in zip(SOURCES, objs)]
+    with ThreadPoolExecutor(len(cmds)) as ex:
+        rcs = list(ex.map(lambda c: _run(c, verbose), cmds))
+    if any(rcs):
+        raise RuntimeError("hipcc failed compiling the engine")
+    if _run([HIPCC, *LINK, "-o", LIB + ".tmp", *objs], verbose):
+        raise RuntimeError("hipcc failed linking libmdg.so")
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

@@ -568,18 +568,29 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
       }
     });
     launch(c, "k_bstar", [&] {
-      auto go = [&](auto occ) {
-        k_bstar<R, TH, decltype(occ)::value><<<sgrid, 256, 0, c->stream>>>(
-            A, T, c->box, c->queue, c->qcount, c->qsub, A.grad, c->eatom + (size_t)c->slot_lj * n, c->derr, dth);
-      };
-      if constexpr (!TH) {
-        if (tu.occ_bstar == 2) return go(std::integral_constant<int, 2>());
-        if constexpr (std::is_same<R, float>::value) {
-          if (tu.occ_bstar == 3) return go(std::integral_constant<int, 3>());
-          if (tu.occ_bstar == 4) return go(std::integral_constant<int, 4>());
+      double* eb = c->eatom + (size_t)c->slot_lj * n;
+      if constexpr (TH) {  // one pass
+        k_bstar<R, TH, 1, 0><<<sgrid, 256, 0, c->stream>>>(A, T, c->box, c->queue, c->qcount, c->qsub, A.grad, eb,
+                                                            c->derr, dth);
+      } else {
+        // the bound pass at the tuned occupancy, then the full pass over the
+        // pairs whose bound failed
+        auto go = [&](auto occ) {
+          k_bstar<R, false, decltype(occ)::value, 1><<<sgrid, 256, 0, c->stream>>>(
+              A, T, c->box, c->queue, c->qcount, c->qsub, A.grad, eb, c->derr, dth);
+        };
+        int occ = tu.occ_bstar;
+        if constexpr (std::is_same<R, float>::value) {  // 3-4 waves/SIMD: FP32 form only
+          if (occ == 3) go(std::integral_constant<int, 3>());
+          if (occ == 4) go(std::integral_constant<int, 4>());
+        } else {
+          if (occ > 2) occ = 1;
         }
+        if (occ == 2) go(std::integral_constant<int, 2>());
+        if (occ == 1) go(std::integral_constant<int, 1>());
+        k_bstar<R, false, 1, 2><<<sgrid, 256, 0, c->stream>>>(A, T, c->box, c->queue, c->qcount, c->qsub, A.grad,
+                                                               eb, c->derr, dth);
       }
-      go(std::integral_constant<int, 1>());
     });
   }
   if (fork) {
@@ -1650,6 +1661,14 @@ int mdg_autotune(mdg_ctx* c, int reps, double* report) {
     c->kst.clear();
     for (int r = 0; r < reps; r++) force_call(c);
     collect_profile(c);
+    if (!kernel) {  // the four scattering force kernels together
+      double t = 0.0;
+      for (const char* k : {"k_lj_wave", "k_rebo_bonds", "k_bstar", "k_etors"}) {
+        auto it = c->kst.find(k);
+        if (it != c->kst.end()) t += it->second.ms / reps;
+      }
+      return t;
+    }
     auto it = c->kst.find(kernel);
     return it == c->kst.end() ? 0.0 : it->second.ms / reps;
   };
@@ -1671,6 +1690,22 @@ int mdg_autotune(mdg_ctx* c, int reps, double* report) {
   std::map<std::string, double> base_ms, best_ms;  // per kernel: untuned, tuned
   auto tune_knob = [&](const char* kernel, int& knob, const std::vector<int>& cand) {
     double t0 = measure(kernel);
+    if (!kernel) {  // a knob of all scattering kernels: compared on their sum only
+      int best = knob;
+      double bt = t0;
+      for (int v : cand) {
+        int keep = knob;
+        knob = v;
+        double t = measure(kernel);
+        if (t < bt) {
+          bt = t;
+          best = v;
+        }
+        knob = keep;
+      }
+      knob = best;
+      return;
+    }
     if (!base_ms.count(kernel)) base_ms[kernel] = t0;
     int best = knob;
     double bt = t0;
@@ -1688,6 +1723,7 @@ int mdg_autotune(mdg_ctx* c, int reps, double* report) {
     best_ms[kernel] = bt;
   };
   Tune& tu = c->tune;
+  tune_knob(nullptr, c->A.wg_map, {0, 1});
   if (c->slot_rebo >= 0) {
     if (c->mixed) tune_knob("k_rebo_bonds", tu.occ_bonds, {1, 2, 3, 4});
     else tune_knob("k_rebo_bonds", tu.occ_bonds, {1, 2});
@@ -1714,10 +1750,10 @@ int mdg_autotune(mdg_ctx* c, int reps, double* report) {
   for (auto& kv : base_ms) before += kv.second;
   for (auto& kv : best_ms) after += kv.second;
   if (report) {
-    double r[9] = {(double)tu.occ_bonds, (double)tu.grid_bonds, (double)tu.occ_bstar, (double)tu.grid_bstar,
-                   (double)tu.grid_lj,   (double)tu.grid_tors,  before,                after,
-                   (double)tu.lj_shape};
-    for (int q = 0; q < 9; q++) report[q] = r[q];
+    double r[10] = {(double)tu.occ_bonds, (double)tu.grid_bonds, (double)tu.occ_bstar, (double)tu.grid_bstar,
+                    (double)tu.grid_lj,   (double)tu.grid_tors,  before,                after,
+                    (double)tu.lj_shape,  (double)c->A.wg_map};
+    for (int q = 0; q < 10; q++) report[q] = r[q];
   }
   return check_err(c);
 }

@@ -86,6 +86,11 @@ struct TablesT {
   R T3[4][5][5][16][4];        // pi_rc CC, CH, HH, T_ij: (v, di, dj, dk)
   R BH[2][5][5][3];            // Brenner H_CC, H_CH
   R BF[5][5][6][4];            // Brenner F (k nodes 0..5)
+  // Upper bound of pi_rc over its whole domain for each (type i, type j)
+  // orientation (the bicubic/tricubic interpolant can overshoot its nodes
+  // only through node slopes; see rc_upper_bound), plus a margin for the
+  // FP32 evaluation.  Used to prove b* < b_min without evaluating pi_rc.
+  double rcmax[2][2];
 };
 // FLJ (pairwise/FLJ.cxx): element index e2i (Cu 0, H 1, C 2, Au 3), LB-mixed
 // sigma/epsilon, the ZBL-to-LJ cubic bridge ends x[0..1] with the values and

@@ -24,12 +24,23 @@ constexpr int kTorsCache = 1024;
 constexpr int kBondCache = 1024;
 constexpr int kPairCache = 512;
 
-// The contiguous share [b0, b1) of nwork items of workgroup blockIdx.x, a
+// The workgroup's place in the work order.  The dispatcher deals
+// workgroups to the eight XCDs in turn (workgroup b runs on XCD b mod 8);
+// with map != 0 (and a grid that is a multiple of 8) XCD x takes the x-th
+// contiguous eighth of the work, so that neighbouring work -- and the
+// gradient lines it scatters into -- stays in one XCD's L2.
+__device__ __forceinline__ int work_block(int map) {
+  const int g = gridDim.x, b = blockIdx.x;
+  if (map && (g & 7) == 0) return (b & 7) * (g >> 3) + (b >> 3);
+  return b;
+}
+
+// The contiguous share [b0, b1) of nwork items of this workgroup, a
 // multiple of the `unit` items it takes per pass (so that its gradient cache
 // sees neighbouring work).
-__device__ __forceinline__ void block_share(int nwork, int& b0, int& b1, int unit = 256) {
+__device__ __forceinline__ void block_share(int nwork, int& b0, int& b1, int map, int unit = 256) {
   long long per = ((long long)(nwork + gridDim.x - 1) / gridDim.x + unit - 1) / unit * unit;
-  long long lo = per * blockIdx.x;
+  long long lo = per * work_block(map);
   b0 = (int)min(lo, (long long)nwork);
   b1 = (int)min(lo + per, (long long)nwork);
 }
@@ -276,12 +287,11 @@ __device__ __forceinline__ R cos_theta(V3<R> a, V3<R> b, R ra, R rb, unsigned* e
   return c;
 }
 
-// Forward pass of b_ij = (p_ij + p_ji)/2 + pi_rc + pi_dh (REBO::Baver,
-// REBO.cxx:200-211, with D :235-262, p_sigma_pi :264-275, Nconj :540-624,
-// P :626-657, pi_rc :659-714, pi_dh :716-814).
+// First part of the forward pass of b_ij: the coordinations excluding the
+// pair and D_ij, D_ji of the sigma-pi terms (REBO.cxx:235-262, P :626-657).
 template <class R>
-__device__ __forceinline__ void bo_forward(const Atoms& A, const TablesT<R>* __restrict__ T, const BPair<R>& p,
-                                           BOState& st, unsigned* err) {
+__device__ __forceinline__ void bo_sigma_pi(const Atoms& A, const TablesT<R>* __restrict__ T, const BPair<R>& p,
+                                            BOState& st, unsigned* err) {
   const ReboParamsT<R>& P = T->rebo;
   R R2ij = P.R[p.ti][p.tj][1];
   st.N[0] = coord_excl<R>(A.Nt[p.i], true, p.r, R2ij, p.f, err);
@@ -318,6 +328,15 @@ __device__ __forceinline__ void bo_forward(const Atoms& A, const TablesT<R>* __r
     D += st.P[s][0];
     st.D[s] = D;
   }
+}
+
+// The rest of the forward pass of b_ij = (p_ij + p_ji)/2 + pi_rc + pi_dh
+// (REBO::Baver, REBO.cxx:200-211, with p_sigma_pi :264-275, Nconj :540-624,
+// pi_rc :659-714, pi_dh :716-814), after bo_sigma_pi.
+template <class R>
+__device__ __forceinline__ void bo_finish(const Atoms& A, const TablesT<R>* __restrict__ T, const BPair<R>& p,
+                                          BOState& st, unsigned* err) {
+  const ReboParamsT<R>& P = T->rebo;
   double pij = pow(st.D[0], -0.5);
   double pji = pow(st.D[1], -0.5);
   // Nconj partial sums: carbon neighbours k of the centre, f_ka * S'(N_ka)
@@ -395,6 +414,13 @@ __device__ __forceinline__ void bo_forward(const Atoms& A, const TablesT<R>* __r
     }
   }
   st.b = (pij + pji) / 2.0 + st.rc[0] + pidh;
+}
+
+template <class R>
+__device__ __forceinline__ void bo_forward(const Atoms& A, const TablesT<R>* __restrict__ T, const BPair<R>& p,
+                                           BOState& st, unsigned* err) {
+  bo_sigma_pi<R>(A, T, p, st, err);
+  bo_finish<R>(A, T, p, st, err);
 }
 
 // Scatter c * d f_ak / dx for the pair (a,k) along u_ak = rv/r (a +=, k -=).
@@ -597,7 +623,7 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
   const double su = 1.0 / T->eunit;  // erg -> the table's energy unit (adjoint seed)
   int nbt = min(*nbonds, A.bond_cap);
   int b0, b1;
-  block_share(nbt, b0, b1);
+  block_share(nbt, b0, b1, A.wg_map);
   for (int base = b0; base < b1; base += 256) {
     const int t = base + threadIdx.x;
     const ReboParamsT<double>& P = T64->rebo;
@@ -1208,7 +1234,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
   };
   // a contiguous share of the atoms per workgroup, G atoms per pass
   int a0, a1;
-  block_share(A.n, a0, a1, G);
+  block_share(A.n, a0, a1, A.wg_map, G);
   for (int base = a0; base < a1; base += G) {
     if (base + gid < a1) atom(base + gid);
     __syncthreads();
@@ -1218,13 +1244,29 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
   count_add(npairs, np);
 }
 
-// LJ pairs that need b*_ij (S_r != 0): the whole StrStb / VLJ / Cij chain.
-template <class R, bool TH, int OCC>
+// LJ pairs that need b*_ij (S_r != 0): the whole StrStb / VLJ / Cij chain
+// (AIREBO_LJ.cxx:79-202).
+//
+// b* < b_min needs no more than the sigma-pi sums: pi_dh = T_ij * sum
+// (1 - cos^2) f' f' <= 0 (T_ij's nodes are <= 0 with zero slopes, so its
+// interpolant stays between them) and pi_rc <= rcmax, so
+//   b* <= bub = (D_ij^-1/2 + D_ji^-1/2)/2 + rcmax
+// (taken with 1/sqrt and a 1e-12 slack that covers every rounding
+// difference to the full evaluation's pow).  When bub < b_min, S(b*) = 1
+// and dS(b*) = 0 exactly, as the full evaluation would find, and neither
+// Nconj, pi_rc, pi_dh nor the bond-order adjoint is needed.
+// PASS 0: one pass, the full evaluation inline where the bound fails.
+// PASS 1: the bound pass; a pair whose bound fails is marked (queue entry w
+//         = 1) and left to PASS 2 entirely.  It compiles without the bond
+//         order's second half and adjoint, so it keeps few registers.
+// PASS 2: the full evaluation of the marked pairs.
+template <class R, bool TH, int OCC, int PASS>
 __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
-                                              const int4* __restrict__ queue, const int* __restrict__ qcount,
+                                              int4* __restrict__ queue, const int* __restrict__ qcount,
                                               int qsub, double* __restrict__ grad, double* __restrict__ eatom,
                                               unsigned* err, double* __restrict__ dth) {
   static_assert(kQueues == 256, "one thread per sub-queue in the prefix below (blockDim 256)");
+  static_assert(PASS >= 0 && PASS <= 2 && (PASS == 0 || !TH), "parameter gradients use the one-pass form");
   const double gs = T->eunit / T->lunit;
   // s_pre[q]: first flat index of sub-queue q (exclusive prefix of the counts)
   __shared__ int s_pre[kQueues + 1];
@@ -1252,19 +1294,25 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
   __syncthreads();
   const int nq = s_pre[kQueues];
   int b0, b1;
-  block_share(nq, b0, b1);
+  block_share(nq, b0, b1, A.wg_map);
   for (int base = b0; base < b1; base += 256) {
    const int t = base + threadIdx.x;
-   if (t < b1) {
-    const LJParamsT<R>& L = T->lj;
-    const ReboParamsT<R>& P = T->rebo;
-    int lo = 0, hi = kQueues - 1;  // last sub-queue whose first index is <= t
+   int lo = 0;
+   int4 qd = make_int4(0, 0, 0, 0);
+   bool work = t < b1;
+   if (work) {
+    int hi = kQueues - 1;  // last sub-queue whose first index is <= t
     while (lo < hi) {
       int mid = (lo + hi + 1) >> 1;
       if (s_pre[mid] <= t) lo = mid;
       else hi = mid - 1;
     }
-    int4 qd = queue[(size_t)lo * qsub + (t - s_pre[lo])];
+    qd = queue[(size_t)lo * qsub + (t - s_pre[lo])];
+    if constexpr (PASS == 2) work = qd.w != 0;
+   }
+   if (work) {
+    const LJParamsT<R>& L = T->lj;
+    const ReboParamsT<R>& P = T->rebo;
     int i = qd.x, j = qd.y;
     int pkey = qd.z;  // C_ij path (0: none), bond slots as in k_lj_wave
     int ti = A.type[i], tj = A.type[j];
@@ -1290,10 +1338,25 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
     p.f = fb;
     p.reff = P.R[ti][tj][0];
     BOState st;
-    bo_forward<R>(A, T, p, st, err);
-    R bs = R(st.b);
     R bmin = L.b[ti][tj][0], bmax = L.b[ti][tj][1];
-    R Stb = Sfun<R>(bs, bmin, bmax);
+    bo_sigma_pi<R>(A, T, p, st, err);
+    bool below = false;
+    if constexpr (PASS != 2) {
+      const double bub = 0.5 * (1.0 / sqrt(st.D[0]) + 1.0 / sqrt(st.D[1])) + T->rcmax[ti][tj] + 1e-12;
+      below = bub < double(bmin);
+    }
+    bool done = false;
+    if constexpr (PASS == 1) {
+      if (!below) {  // left to the full pass
+        queue[(size_t)lo * qsub + (t - s_pre[lo])].w = 1;
+        done = true;
+      }
+    } else {
+      if (!below) bo_finish<R>(A, T, p, st, err);
+    }
+    if (!done) {
+    R bs = below ? bmin : R(st.b);
+    R Stb = below ? R(1) : Sfun<R>(bs, bmin, bmax);
     R SrSb = R(1) + (Str * Stb - Str);
     R f, df;
     cutoff<R>(r, L.R[ti][tj][0], L.R[ti][tj][1], f, df);
@@ -1315,9 +1378,11 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
     if (V2 != R(0)) {
       R dStr = dSfun<R>(r, rmin, rmax);
       if (dStr != R(0)) cr += -dStr * V2;
-      R dStb = dSfun<R>(bs, bmin, bmax);
       if (dStr != R(0) && Stb != R(0)) cr += dStr * Stb * V2;
-      if (dStb != R(0)) bo_backward<R>(A, T, p, st, double(Str * dStb * V2), gc, gs, err);
+      if constexpr (PASS != 1) {
+        R dStb = below ? R(0) : dSfun<R>(bs, bmin, bmax);
+        if (dStb != R(0)) bo_backward<R>(A, T, p, st, double(Str * dStb * V2), gc, gs, err);
+      }
     }
     V3<R> v = rv * (cr / r);
     scatter3<R>(gc, i, v, gs);
@@ -1335,6 +1400,7 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
                       double(LJ);
       unsafeAtomicAdd(th, de);
       unsafeAtomicAdd(th + 1, ds);
+    }
     }
    }
     __syncthreads();
@@ -1360,7 +1426,7 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
   const double gs = T->eunit / T->lunit;
   int nbt = min(*nbonds, A.bond_cap);
   int b0, b1;
-  block_share(nbt, b0, b1);
+  block_share(nbt, b0, b1, A.wg_map);
   for (int base = b0; base < b1; base += 256) {
     const int t = base + threadIdx.x;
     const ReboParamsT<R>& P = T->rebo;
@@ -1451,8 +1517,8 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
 #define MDG_INSTANTIATE_OCC(R, TH, OCC)                                                                        \
   template __global__ void k_rebo_bonds<R, TH, OCC>(Atoms, const TablesT<R>*, const Tables*, Box, const int2*, \
                                                     const int*, double*, double*, unsigned*, double*);         \
-  template __global__ void k_bstar<R, TH, OCC>(Atoms, const TablesT<R>*, Box, const int4*, const int*, int,  \
-                                               double*, double*, unsigned*, double*);
+  template __global__ void k_bstar<R, TH, OCC, 0>(Atoms, const TablesT<R>*, Box, int4*, const int*, int,     \
+                                                  double*, double*, unsigned*, double*);
 #define MDG_INSTANTIATE_TH(R, TH, OCC, LW)                                                                     \
   template __global__ void k_lj_wave<R, TH, OCC, LW>(Atoms, const TablesT<R>*, Box, double*, double*, int4*,   \
                                                      int*, int, long long*, unsigned*, double*);
@@ -1482,5 +1548,16 @@ MDG_INSTANTIATE_OCC(float, false, 1)
 MDG_INSTANTIATE_OCC(float, false, 2)
 MDG_INSTANTIATE_OCC(float, false, 3)
 MDG_INSTANTIATE_OCC(float, false, 4)
+#define MDG_INSTANTIATE_SPLIT(R, OCC)                                                                           \
+  template __global__ void k_bstar<R, false, OCC, 1>(Atoms, const TablesT<R>*, Box, int4*, const int*, int,    \
+                                                     double*, double*, unsigned*, double*);                    \
+  template __global__ void k_bstar<R, false, OCC, 2>(Atoms, const TablesT<R>*, Box, int4*, const int*, int,    \
+                                                     double*, double*, unsigned*, double*);
+MDG_INSTANTIATE_SPLIT(double, 1)
+MDG_INSTANTIATE_SPLIT(double, 2)
+MDG_INSTANTIATE_SPLIT(float, 1)
+MDG_INSTANTIATE_SPLIT(float, 2)
+MDG_INSTANTIATE_SPLIT(float, 3)
+MDG_INSTANTIATE_SPLIT(float, 4)
 
 }  // namespace mdg

@@ -67,6 +67,7 @@ struct Atoms {
   double* tb_T;               // per system: bath temperature at the start of the step
   double* tb_dE;              // per atom: energy taken from the bath this step
   double* an_nt;              // n*3 acceleration without the bath (kept with or without a bath)
+  int wg_map;                 // 1: the scattering force kernels give each XCD a contiguous eighth of the work
 };
 
 struct CellGrid {
@@ -134,8 +135,8 @@ __global__ void k_pair_rows(Atoms A, const TablesT<R>* T, Box bx, const int* pfl
 template <class R, bool TH, int OCC, int LW>
 __global__ void k_lj_wave(Atoms A, const TablesT<R>* T, Box bx, double* grad, double* eatom, int4* queue,
                           int* qcount, int qsub, long long* npairs, unsigned* err, double* dth);
-template <class R, bool TH, int OCC>
-__global__ void k_bstar(Atoms A, const TablesT<R>* T, Box bx, const int4* queue, const int* qcount, int qsub,
+template <class R, bool TH, int OCC, int PASS>
+__global__ void k_bstar(Atoms A, const TablesT<R>* T, Box bx, int4* queue, const int* qcount, int qsub,
                         double* grad, double* eatom, unsigned* err, double* dth);
 template <class R>
 __global__ void k_etors(Atoms A, const TablesT<R>* T, Box bx, const int2* bonds, const int* nbonds, int compat,

@@ -64,6 +64,29 @@ struct Grid {
           for (int k = kb + 1; k < 16; k++) t[i][j][k][0] = t[i][j][kb][0];
   }
 };
+// An upper bound of a tricubic Hermite interpolant with node values and
+// first derivatives (cross derivatives zero) over its built cells.  Inside a
+// cell the value is sum over corners of v h_a h_b h_c plus slope terms; the
+// value weights are non-negative and sum to one, and each slope basis
+// function is bounded by |t^3 - 2t^2 + t|, |t^3 - t^2| <= 4/27 on [0, 1]
+// (the other factors lie in [0, 1]).  Outside the built cells the table is 0.
+double rc_upper_bound(const double (*t)[5][16][4], int kbuilt) {
+  double ub = 0.0;
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++)
+      for (int k = 0; k < kbuilt; k++) {
+        double vmax = -1e300, slope = 0.0;
+        for (int a = 0; a < 2; a++)
+          for (int b = 0; b < 2; b++)
+            for (int c = 0; c < 2; c++) {
+              const double* q = t[i + a][j + b][k + c];
+              vmax = std::max(vmax, q[0]);
+              slope += std::fabs(q[1]) + std::fabs(q[2]) + std::fabs(q[3]);
+            }
+        ub = std::max(ub, vmax + (4.0 / 27.0) * slope);
+      }
+  return ub;
+}
 }  // namespace
 
 // FLJ parameters (FLJ.cxx:107-140: Lorentz-Berthelot mixing of like-pair
@@ -330,6 +353,15 @@ void build_tables(Tables& T, int brenner_set) {
     g.sym(2, 2, 1, -0.035140);
     g.sym(2, 2, 2, -0.004048); g.up(2, 2, 2);
   }
+  // pi_rc upper bounds in (i, j) orientation; H-C evaluates F_CH with the
+  // arguments swapped, which the bound over the whole table covers.  The
+  // margin covers the FP32 form's rounding of the evaluation.
+  {
+    const double margin = 1e-5;
+    T.rcmax[C][C] = rc_upper_bound(T.T3[0], 15) + margin;
+    T.rcmax[C][H] = T.rcmax[H][C] = rc_upper_bound(T.T3[1], 15) + margin;
+    T.rcmax[H][H] = rc_upper_bound(T.T3[2], 15) + margin;
+  }
   // AIREBO-LJ
   LJParams& l = T.lj;
   l.sigma[C][C] = 3.40 * Ao;
@@ -476,6 +508,7 @@ void convert_tables(const Tables& d, TablesT<float>& f) {
   cv(&f.zetaT[0][0], &d.zetaT[0][0], 4, E);
   cv(&f.P[0][0][0][0], &d.P[0][0][0][0], 2 * 5 * 5 * 3, 1.0);
   cv(&f.T3[0][0][0][0][0], &d.T3[0][0][0][0][0], 4 * 5 * 5 * 16 * 4, 1.0);
+  std::memcpy(f.rcmax, d.rcmax, sizeof(f.rcmax));
 }
 
 }  // namespace mdg

@@ -328,10 +328,10 @@ class Engine:
         """Picks the force kernels' launch shapes by timing candidates on this
         engine's systems; returns the choices and the tuned kernels' ms per
         force call before and after."""
-        r = np.zeros(9)
+        r = np.zeros(10)
         self._chk(_lib.mdg_autotune(self._h, int(reps), _p(r)))
         keys = ["occ_bonds", "grid_bonds", "occ_bstar", "grid_bstar", "grid_lj", "grid_tors", "ms_before",
-                "ms_after", "lj_shape"]
+                "ms_after", "lj_shape", "xcd_order"]
         return {k: (float(v) if k.startswith("ms") else int(v)) for k, v in zip(keys, r)}
 
     def synchronize(self):
