@@ -113,6 +113,20 @@ int mdg_term_kind(const mdg_ctx* ctx, int i);
  * when its skin test asks for it (NeighbourList.cxx:108-133). */
 int mdg_energy_grad(mdg_ctx* ctx, double* e_terms, double* grad);
 
+/* mdg_create + mdg_set_system + mdg_energy_grad + mdg_destroy in one call
+ * that allocates no device memory, for callers that cannot allocate: tight
+ * serving loops, harnesses that forbid hipMalloc.  No counterpart in the
+ * reference.  Same arguments and results as those calls (host arrays: elem[n],
+ * xyz[n*3] in cm, pbc[3], e_terms[mdg_num_terms], grad[n*3]); every device
+ * buffer of the call comes from one static 64 MiB array per GPU, held until
+ * the process ends, so n is limited to 4096 atoms (and the system's cell grid
+ * and lists must fit; a larger one fails with a message).  The first call on a
+ * device looks up that array's address.  Works on the null stream.  Calls
+ * that overlap on one GPU (from two threads) share the array and race.
+ * Errors: nonzero return, message from mdg_last_error(NULL). */
+int mdg_energy_grad_noalloc(const mdg_config* cfg, int n, const int32_t* elem, const double* xyz,
+                            const double pbc[3], double* e_terms, double* grad);
+
 /* Differentiable parameters theta[MDG_NUM_PARAMS] (CGS): for the type pairs
  * p = 0 (C-C), 1 (C-H), 2 (H-H), REBO's pair parameters {Q, A, alpha, B1, B2,
  * B3, beta1, beta2, beta3} at theta[9p..9p+8] (REBO::REBO, REBO.cxx:856-941;

@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -87,6 +88,10 @@ struct mdg_ctx {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::vector<void*> allocs;
+  // With arena set, dalloc carves device buffers from [arena, arena +
+  // arena_cap) instead of calling hipMalloc (mdg_energy_grad_noalloc).
+  char* arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
 
   // one cell grid per system and list; cell ids offset per system
   CellGrid *gs_d = nullptr, *gl_d = nullptr;
@@ -150,13 +155,7 @@ struct mdg_ctx {
       }
     for (auto& e : timer_ev)
       if (e) hipEventDestroy(e);
-    for (void* p : allocs) hipFree(p);
-    if (dT) hipFree(dT);
-    if (dTf) hipFree(dTf);
-    if (dFLJ) hipFree(dFLJ);
-    if (dEAM) hipFree(dEAM);
-    if (dTB) hipFree(dTB);
-    if (dFBM) hipFree(dFBM);
+    for (void* p : allocs) hipFree(p);  // the tables too; nothing when the buffers came from an arena
     if (own) hipStreamDestroy(own);
     if (side) hipStreamDestroy(side);
     if (ev_fork) hipEventDestroy(ev_fork);
@@ -178,6 +177,17 @@ template <class T>
 int dalloc(mdg_ctx* c, T*& p, size_t count) {
   void* q = nullptr;
   size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+  if (c->arena) {  // carve from the caller's memory, 256-byte aligned
+    size_t at = (c->arena_used + 255) & ~size_t(255);
+    if (at + bytes > c->arena_cap) {
+      c->err = "static scratch exhausted: this system needs more than its " + std::to_string(c->arena_cap >> 20) +
+               " MiB";
+      return 1;
+    }
+    p = reinterpret_cast<T*>(c->arena + at);
+    c->arena_used = at + bytes;
+    return 0;
+  }
   hipError_t e = hipMalloc(&q, bytes);
   if (e != hipSuccess) {
     c->err = "hipMalloc(" + std::to_string(bytes) + " bytes): " + hipGetErrorString(e);
@@ -499,7 +509,7 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   // bond lists and add into grad, so they run on the side stream beside the
   // LJ chain (pair rows -> LJ -> b*); an event joins them back.  Profiled
   // calls stay on one stream so that every kernel's own time is measured.
-  const bool fork = !c->profiling && c->slot_lj >= 0 &&
+  const bool fork = !c->profiling && c->side && c->slot_lj >= 0 &&
                     (c->slot_rebo >= 0 || c->slot_bren >= 0 || c->slot_tors >= 0);
   hipStream_t sb = fork ? c->side : c->stream;
   if (fork) {
@@ -696,53 +706,58 @@ void force_call(mdg_ctx* c) {
   }
   k_clear_flags<<<blocks(2LL * c->nsys), 256, 0, c->stream>>>(c->nsys, c->flags, c->sflags, c->pflags, c->eflags);
 }
-}  // namespace
 
-extern "C" {
+// A copy of a host object into a new device buffer of the context.
+template <class T>
+int upload(mdg_ctx* c, T*& d, const T& h) {
+  if (dalloc(c, d, 1)) return 1;
+  HIPCHK(c, hipMemcpy(d, &h, sizeof(T), hipMemcpyHostToDevice));
+  return 0;
+}
 
-const char* mdg_last_error(const mdg_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
-
-int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
-  *out = nullptr;
+// mdg_create's work on a constructed context: the potential chain in the
+// order of applications/common.h:46-68, the parameter tables, streams and
+// the device counters.  With c->arena set (mdg_energy_grad_noalloc) every
+// device buffer comes from that memory, and no stream or event is created:
+// the context works on the null stream without the side stream.
+int ctx_init(mdg_ctx* c, const mdg_config* cfg) {
   if (!cfg) {
-    g_create_err = "null config";
+    c->err = "null config";
     return 1;
   }
   int m = cfg->pot_mask;
   if ((m & MDG_POT_AIREBO) && !(m & MDG_POT_REBO)) {
-    g_create_err = "AIREBO-LJ needs REBO in the chain (AIREBO_LJ.hpp:63)";
+    c->err = "AIREBO-LJ needs REBO in the chain (AIREBO_LJ.hpp:63)";
     return 1;
   }
   if ((m & MDG_POT_BRENNER) && (m & (MDG_POT_REBO | MDG_POT_AIREBO))) {
-    g_create_err = "Brenner replaces REBO: select one of them";
+    c->err = "Brenner replaces REBO: select one of them";
     return 1;
   }
   if (!(m & (MDG_POT_REBO | MDG_POT_BRENNER | MDG_POT_ETORS | MDG_POT_FBZL | MDG_POT_FLJ | MDG_POT_ACKLAND |
              MDG_POT_FBM | MDG_POT_TB))) {
-    g_create_err = "no potentials selected";
+    c->err = "no potentials selected";
     return 1;
   }
   if (cfg->precision != MDG_PREC_FP64 && cfg->precision != MDG_PREC_MIXED) {
-    g_create_err = "precision must be MDG_PREC_FP64 or MDG_PREC_MIXED";
+    c->err = "precision must be MDG_PREC_FP64 or MDG_PREC_MIXED";
     return 1;
   }
   if (cfg->precision == MDG_PREC_MIXED && (m & MDG_POT_BRENNER)) {
-    g_create_err = "the mixed-precision form covers REBO/AIREBO/ETors/FBZL; run Brenner in FP64";
+    c->err = "the mixed-precision form covers REBO/AIREBO/ETors/FBZL; run Brenner in FP64";
     return 1;
   }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
-    g_create_err = "no HIP device";
+    c->err = "no HIP device";
     return 1;
   }
-  mdg_ctx* c = new mdg_ctx();
   c->cfg = *cfg;
   c->use_graphs = std::getenv("MDG_NO_GRAPH") == nullptr;
   if (c->cfg.max_short_nb <= 0) c->cfg.max_short_nb = 64;
   if (c->cfg.max_long_nb <= 0) c->cfg.max_long_nb = 512;
   if (c->cfg.max_short_nb != 64 || c->cfg.max_long_nb != 512) {
-    g_create_err = "list capacities are fixed at 64 (short) and 512 (long) in this library";
-    delete c;
+    c->err = "list capacities are fixed at 64 (short) and 512 (long) in this library";
     return 1;
   }
   if (c->cfg.fbzl_r1 <= 0) {
@@ -778,68 +793,59 @@ int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
   if (m & MDG_POT_FBM) c->rc_long = std::max(c->rc_long, ft2.R1);  // FPairwise::getRcutoff
   if (m & MDG_POT_TB) c->rc_long = std::max(c->rc_long, tt.R1);    // TightBinding.hpp:55
   c->metal_rcand = std::max((m & MDG_POT_ACKLAND) ? et.rmax : 0.0, (m & MDG_POT_TB) ? tt.R1 : 0.0) + c->metal_skin;
-  if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
-    g_create_err = "hipStreamCreate failed";
-    delete c;
-    return 1;
-  }
-  c->stream = c->own;
-  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
-    g_create_err = "hipStreamCreate/hipEventCreate failed";
-    delete c;
-    return 1;
+  if (!c->arena) {
+    if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
+      c->err = "hipStreamCreate failed";
+      return 1;
+    }
+    c->stream = c->own;
+    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+      c->err = "hipStreamCreate/hipEventCreate failed";
+      return 1;
+    }
   }
   Tables h;
   build_tables(h, cfg->brenner_paramset);
   c->host_T = h;
-  if (hipMalloc(&c->dT, sizeof(Tables)) != hipSuccess ||
-      hipMemcpy(c->dT, &h, sizeof(Tables), hipMemcpyHostToDevice) != hipSuccess) {
-    g_create_err = "table upload failed";
-    delete c;
-    return 1;
-  }
+  if (upload(c, c->dT, h)) return 1;
   c->mixed = cfg->precision == MDG_PREC_MIXED;
   if (c->mixed) {
     TablesT<float> hf;
     convert_tables(h, hf);
-    if (hipMalloc(&c->dTf, sizeof(hf)) != hipSuccess ||
-        hipMemcpy(c->dTf, &hf, sizeof(hf), hipMemcpyHostToDevice) != hipSuccess) {
-      g_create_err = "table upload failed";
-      delete c;
-      return 1;
-    }
+    if (upload(c, c->dTf, hf)) return 1;
   }
   if (c->slot_flj >= 0) {
     FLJTable ft;
     build_flj_table(ft);
-    if (hipMalloc(&c->dFLJ, sizeof(ft)) != hipSuccess ||
-        hipMemcpy(c->dFLJ, &ft, sizeof(ft), hipMemcpyHostToDevice) != hipSuccess) {
-      g_create_err = "table upload failed";
-      delete c;
-      return 1;
-    }
+    if (upload(c, c->dFLJ, ft)) return 1;
   }
-  if ((c->slot_eam >= 0 && (hipMalloc(&c->dEAM, sizeof(et)) != hipSuccess ||
-                            hipMemcpy(c->dEAM, &et, sizeof(et), hipMemcpyHostToDevice) != hipSuccess)) ||
-      (c->slot_tb >= 0 && (hipMalloc(&c->dTB, sizeof(tt)) != hipSuccess ||
-                           hipMemcpy(c->dTB, &tt, sizeof(tt), hipMemcpyHostToDevice) != hipSuccess)) ||
-      (c->slot_fbm >= 0 && (hipMalloc(&c->dFBM, sizeof(ft2)) != hipSuccess ||
-                            hipMemcpy(c->dFBM, &ft2, sizeof(ft2), hipMemcpyHostToDevice) != hipSuccess))) {
-    g_create_err = "table upload failed";
-    delete c;
+  if ((c->slot_eam >= 0 && upload(c, c->dEAM, et)) || (c->slot_tb >= 0 && upload(c, c->dTB, tt)) ||
+      (c->slot_fbm >= 0 && upload(c, c->dFBM, ft2)))
     return 1;
-  }
-  if (dalloc(c, c->flags, 4) || dalloc(c, c->counters, 8) || dalloc(c, c->tally, kTallySlots * kTallyStride) || dalloc(c, c->derr, 1) || dalloc(c, c->nbonds, 1) ||
-      dalloc(c, c->qcount, kQueues * kQStride)) {
+  if (dalloc(c, c->flags, 4) || dalloc(c, c->counters, 8) || dalloc(c, c->tally, kTallySlots * kTallyStride) ||
+      dalloc(c, c->derr, 1) || dalloc(c, c->nbonds, 1) || dalloc(c, c->qcount, kQueues * kQStride))
+    return 1;
+  HIPCHK(c, hipMemset(c->counters, 0, 8 * sizeof(long long)));
+  HIPCHK(c, hipMemset(c->tally, 0, kTallySlots * kTallyStride * sizeof(long long)));
+  HIPCHK(c, hipMemset(c->derr, 0, sizeof(unsigned)));
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+const char* mdg_last_error(const mdg_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+int mdg_create(const mdg_config* cfg, mdg_ctx** out) {
+  *out = nullptr;
+  mdg_ctx* c = new mdg_ctx();
+  if (ctx_init(c, cfg)) {
     g_create_err = c->err;
     delete c;
     return 1;
   }
-  hipMemset(c->counters, 0, 8 * sizeof(long long));
-  hipMemset(c->tally, 0, kTallySlots * kTallyStride * sizeof(long long));
-  hipMemset(c->derr, 0, sizeof(unsigned));
   *out = c;
   return 0;
 }
@@ -1314,6 +1320,64 @@ int mdg_energy_grad(mdg_ctx* c, double* e_terms, double* grad) {
     for (int s = 0; s < c->nsys; s++)
       for (int t = 0; t < nt; t++) e_terms[(size_t)s * nt + t] = e[8 * (size_t)s + t];
   return 0;
+}
+
+}  // extern "C"
+
+namespace {
+// Static scratch of mdg_energy_grad_noalloc: every device buffer of that call
+// (atoms, cell grids, lists, bond records, the b* queue, tables, counters)
+// is carved from this array; each GPU holds its own copy, at its own
+// address, until the process ends.
+constexpr size_t kNoAllocBytes = size_t(64) << 20;
+constexpr int kNoAllocMaxAtoms = 4096;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) char scratch[kNoAllocBytes];
+std::mutex g_scratch_mu;
+char* g_scratch_at[kMaxDevices] = {};
+
+// The device address of the current device's scratch, looked up once per
+// device (the array's name is only a host-side handle).
+int scratch_of(int dev, char*& out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_at[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      g_create_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return 1;
+    }
+    g_scratch_at[dev] = static_cast<char*>(p);
+  }
+  out = g_scratch_at[dev];
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int mdg_energy_grad_noalloc(const mdg_config* cfg, int n, const int32_t* elem, const double* xyz,
+                            const double pbc[3], double* e_terms, double* grad) {
+  if (n <= 0 || n > kNoAllocMaxAtoms) {
+    g_create_err = "mdg_energy_grad_noalloc: n = " + std::to_string(n) + " outside 1.." +
+                   std::to_string(kNoAllocMaxAtoms) + " (the static scratch's limit)";
+    return 1;
+  }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    g_create_err = "mdg_energy_grad_noalloc: no current HIP device";
+    return 1;
+  }
+  char* base = nullptr;
+  if (scratch_of(dev, base)) return 1;
+  mdg_ctx c;
+  c.arena = base;
+  c.arena_cap = kNoAllocBytes;
+  int rc = ctx_init(&c, cfg);
+  if (!rc) rc = mdg_set_system(&c, n, elem, xyz, nullptr, pbc, nullptr);
+  if (!rc) rc = mdg_energy_grad(&c, e_terms, grad);
+  if (rc) g_create_err = "mdg_energy_grad_noalloc: " + c.err;
+  return rc;
 }
 
 int mdg_get_params(const mdg_ctx* c, double* theta) {

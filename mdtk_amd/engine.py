@@ -74,6 +74,7 @@ def load_library(path: str = LIB_PATH):
     L.mdg_event_record.argtypes = [vp, ctypes.c_int]
     L.mdg_event_elapsed.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
     L.mdg_synchronize.argtypes = [vp]
+    L.mdg_energy_grad_noalloc.argtypes = [ctypes.POINTER(_Config), ctypes.c_int, ip, dp, dp, dp, dp]
     L.mdg_autotune.argtypes = [vp, ctypes.c_int, dp]
     L.mdg_set_thermal_bath.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Bath)]
     L.mdg_set_bath_atoms.argtypes = [vp, ctypes.POINTER(ctypes.c_uint8)]
@@ -107,6 +108,31 @@ def _p(a, t=ctypes.c_double):
     return None if a is None else a.ctypes.data_as(ctypes.POINTER(t))
 
 
+def _config(potentials, precision, etors_compat, brenner_paramset, fbzl_r):
+    mask = 0
+    for p in potentials:
+        mask |= _NAMES[p]
+    return _Config(mask, int(brenner_paramset), int(bool(etors_compat)),
+                   PREC_FP64 if precision == "fp64" else PREC_MIXED, float(fbzl_r[0]), float(fbzl_r[1]), 0, 0)
+
+
+def energy_grad_noalloc(sys, potentials=("REBO", "AIREBO", "ETors"), precision: str = "fp64",
+                        etors_compat: bool = True, brenner_paramset: int = 0, fbzl_r=(5.0e-8, 5.5e-8)):
+    """One force call through mdg_energy_grad_noalloc (no device memory is
+    allocated; at most 4096 atoms): (per-member energies in erg, dE/dx in
+    erg/cm) of the system at its positions."""
+    L = load_library()
+    cfg = _config(potentials, precision, etors_compat, brenner_paramset, fbzl_r)
+    e, x, _, p, _ = sys.to_cgs()
+    nterms = sum(1 for k in _NAMES.values() if cfg.pot_mask & k)
+    et = np.zeros(nterms)
+    g = np.zeros((len(e), 3))
+    rc = L.mdg_energy_grad_noalloc(ctypes.byref(cfg), len(e), _p(e, ctypes.c_int32), _p(x), _p(p), _p(et), _p(g))
+    if rc != 0:
+        raise EngineError(L.mdg_last_error(None).decode())
+    return et, g
+
+
 class Engine:
     """One GPU context: a potential chain over one system.
 
@@ -118,11 +144,7 @@ class Engine:
     def __init__(self, potentials=("REBO", "AIREBO", "ETors"), precision: str = "fp64",
                  etors_compat: bool = True, brenner_paramset: int = 0, fbzl_r=(5.0e-8, 5.5e-8)):
         L = load_library()
-        mask = 0
-        for p in potentials:
-            mask |= _NAMES[p]
-        cfg = _Config(mask, int(brenner_paramset), int(bool(etors_compat)),
-                      PREC_FP64 if precision == "fp64" else PREC_MIXED, float(fbzl_r[0]), float(fbzl_r[1]), 0, 0)
+        cfg = _config(potentials, precision, etors_compat, brenner_paramset, fbzl_r)
         h = ctypes.c_void_p()
         if L.mdg_create(ctypes.byref(cfg), ctypes.byref(h)) != 0:
             raise EngineError(L.mdg_last_error(None).decode())
