@@ -143,6 +143,20 @@ int mdg_get_params(const mdg_ctx* ctx, double* theta);
 int mdg_set_params(mdg_ctx* ctx, const double* theta);
 int mdg_energy_grad_params(mdg_ctx* ctx, double* e_terms, double* grad, double* dtheta);
 
+/* Virial (no reference counterpart: MDTK computes none).  With
+ * mdg_set_virial(ctx, 1) every force call also reduces, inside the force
+ * kernels and in FP64, the per-system virial W = sum over the terms' pair
+ * pushes of r_ab (x) F_ab (each force term is a sum of equal and opposite
+ * pushes along its own minimum-image pair vectors), i.e. W = sum_i x_i (x) F_i
+ * for an open system; six components xx, yy, zz, xy, xz, yz in erg, system s
+ * at [6 s].  mdg_virial returns that of the last force call; mdg_step_virial
+ * that of every step of the last mdg_step_nve (W at x(t+dt), beside epot),
+ * step k of system s at [(k nsys + s) * 6].  Parameter-gradient calls
+ * (mdg_energy_grad_params) leave it out.  Call after mdg_set_system(s). */
+int mdg_set_virial(mdg_ctx* ctx, int on);
+int mdg_virial(mdg_ctx* ctx, double* out);
+int mdg_step_virial(mdg_ctx* ctx, double* out);
+
 /* Neighbour list of chain member `term` as the reference would hold it
  * (FGeneral::nl, NeighbourList.hpp:41-83): counts[n], then the lists
  * concatenated (ascending index per atom) into idx[cap]. */

@@ -114,6 +114,11 @@ struct mdg_ctx {
   int* qcount = nullptr;
   int qsub = 0;  // entries per b* sub-queue (kQueues of them)
   double* eatom = nullptr;  // nterms*n per-atom energies of the last force call
+  bool virial = false;      // force calls reduce the virial (mdg_set_virial)
+  double* vir = nullptr;    // 6*nsys virial of the last force call (erg)
+  double* stepvir = nullptr;  // chunk*nsys*6 per-step virials of mdg_step_nve
+  int stepvir_cap = 0;
+  std::vector<double> vir_steps;  // per-step virials of the last mdg_step_nve
   double* dtheta = nullptr;  // kNumTheta*nsys parameter gradients (allocated on first use)
   bool theta_mode = false;
   Tables host_T;             // parameter tables as uploaded (for parameter overrides)
@@ -489,12 +494,16 @@ __global__ void k_zero_counts(long long* tally) {
   tally[threadIdx.x * kTallyStride + 1] = 0;
 }
 
-// The bond-order chain members, in the geometry type R of the context.
-template <class R, bool TH>
+// The bond-order chain members, in the geometry type R of the context.  VIR:
+// the kernels' virial variants (one launch shape each), which also add the
+// per-system virial into c->vir.
+template <class R, bool TH, bool VIR>
 void force_terms(mdg_ctx* c, const TablesT<R>* T) {
+  static_assert(!(TH && VIR), "parameter-gradient calls do not reduce the virial");
   int n = c->n;
   Atoms& A = c->A;
   double* dth = TH ? c->dtheta : nullptr;
+  double* vir = VIR ? c->vir : nullptr;
   if (c->have_short) {
     launch(c, "k_coord", [&] { k_coord<R><<<blocks(n, 16), 256, 0, c->stream>>>(A, T, c->box, c->derr); });
     launch(c, "k_bond_enum", [&] {
@@ -519,10 +528,11 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   if (c->slot_rebo >= 0) {
     launch(c, "k_rebo_bonds", [&] {
       auto go = [&](auto occ) {
-        k_rebo_bonds<R, TH, decltype(occ)::value><<<bgrid, 256, 0, sb>>>(
-            A, T, c->dT, c->box, c->bonds, c->nbonds, A.grad, c->eatom + (size_t)c->slot_rebo * n, c->derr, dth);
+        k_rebo_bonds<R, TH, decltype(occ)::value, VIR><<<bgrid, 256, 0, sb>>>(
+            A, T, c->dT, c->box, c->bonds, c->nbonds, A.grad, c->eatom + (size_t)c->slot_rebo * n, c->derr, dth,
+            vir);
       };
-      if constexpr (!TH) {
+      if constexpr (!TH && !VIR) {
         if (tu.occ_bonds == 2) return go(std::integral_constant<int, 2>());
         if constexpr (std::is_same<R, float>::value) {  // 3-4 waves/SIMD: FP32 form only
           if (tu.occ_bonds == 3) return go(std::integral_constant<int, 3>());
@@ -534,14 +544,14 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   }
   if (c->slot_bren >= 0) {
     launch(c, "k_brenner_bonds", [&] {
-      k_brenner_bonds<<<bgrid, 256, 0, sb>>>(A, c->dT, c->box, c->bonds, c->nbonds, A.grad,
-                                                    c->eatom + (size_t)c->slot_bren * n, c->derr);
+      k_brenner_bonds<VIR><<<bgrid, 256, 0, sb>>>(A, c->dT, c->box, c->bonds, c->nbonds, A.grad,
+                                                  c->eatom + (size_t)c->slot_bren * n, c->derr, vir);
     });
   }
   if (c->slot_tors >= 0) {
     launch(c, "k_etors", [&] {
-      k_etors<R><<<tgrid, 256, 0, sb>>>(A, T, c->box, c->bonds, c->nbonds, c->cfg.etors_compat, A.grad,
-                                               c->eatom + (size_t)c->slot_tors * n, c->tally + 1);
+      k_etors<R, VIR><<<tgrid, 256, 0, sb>>>(A, T, c->box, c->bonds, c->nbonds, c->cfg.etors_compat, A.grad,
+                                             c->eatom + (size_t)c->slot_tors * n, c->tally + 1, vir);
     });
   }
   if (c->slot_lj >= 0) {
@@ -553,12 +563,12 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
       auto go = [&](auto occ, auto lw) {
         constexpr int LW = decltype(lw)::value;
         int grid = tu.grid_lj ? tu.grid_lj : blocks(n, 4 * (64 / LW), 65536);
-        k_lj_wave<R, TH, decltype(occ)::value, LW><<<grid, 256, 0, c->stream>>>(
+        k_lj_wave<R, TH, decltype(occ)::value, LW, VIR><<<grid, 256, 0, c->stream>>>(
             A, T, c->box, A.grad, c->eatom + (size_t)c->slot_lj * n, c->queue, c->qcount, c->qsub,
-            c->tally + 0, c->derr, dth);
+            c->tally + 0, c->derr, dth, vir);
       };
       using I1 = std::integral_constant<int, 1>;
-      if constexpr (TH) {
+      if constexpr (TH || VIR) {
         go(I1(), std::integral_constant<int, 16>());
       } else {
         switch (tu.lj_shape) {
@@ -579,15 +589,15 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
     });
     launch(c, "k_bstar", [&] {
       double* eb = c->eatom + (size_t)c->slot_lj * n;
-      if constexpr (TH) {  // one pass
-        k_bstar<R, TH, 1, 0><<<sgrid, 256, 0, c->stream>>>(A, T, c->box, c->queue, c->qcount, c->qsub, A.grad, eb,
-                                                            c->derr, dth);
+      if constexpr (TH || VIR) {  // one pass
+        k_bstar<R, TH, 1, 0, VIR><<<sgrid, 256, 0, c->stream>>>(A, T, c->box, c->queue, c->qcount, c->qsub, A.grad,
+                                                                 eb, c->derr, dth, vir);
       } else {
         // the bound pass at the tuned occupancy, then the full pass over the
         // pairs whose bound failed
         auto go = [&](auto occ) {
-          k_bstar<R, false, decltype(occ)::value, 1><<<sgrid, 256, 0, c->stream>>>(
-              A, T, c->box, c->queue, c->qcount, c->qsub, A.grad, eb, c->derr, dth);
+          k_bstar<R, false, decltype(occ)::value, 1, false><<<sgrid, 256, 0, c->stream>>>(
+              A, T, c->box, c->queue, c->qcount, c->qsub, A.grad, eb, c->derr, dth, nullptr);
         };
         int occ = tu.occ_bstar;
         if constexpr (std::is_same<R, float>::value) {  // 3-4 waves/SIMD: FP32 form only
@@ -598,8 +608,8 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
         }
         if (occ == 2) go(std::integral_constant<int, 2>());
         if (occ == 1) go(std::integral_constant<int, 1>());
-        k_bstar<R, false, 1, 2><<<sgrid, 256, 0, c->stream>>>(A, T, c->box, c->queue, c->qcount, c->qsub, A.grad,
-                                                               eb, c->derr, dth);
+        k_bstar<R, false, 1, 2, false><<<sgrid, 256, 0, c->stream>>>(A, T, c->box, c->queue, c->qcount, c->qsub,
+                                                                      A.grad, eb, c->derr, dth, nullptr);
       }
     });
   }
@@ -656,12 +666,23 @@ void force_call(mdg_ctx* c) {
   k_zero_i<<<1, 64, 0, c->stream>>>(c->nbonds, 1);
   k_zero_i<<<kQueues * kQStride / 256, 256, 0, c->stream>>>(c->qcount, kQueues * kQStride);
   k_zero_counts<<<1, kTallySlots, 0, c->stream>>>(c->tally);
-  if (c->mixed) force_terms<float, false>(c, c->dTf);
-  else if (c->theta_mode) force_terms<double, true>(c, c->dT);
-  else force_terms<double, false>(c, c->dT);
+  const bool vir = c->virial && !c->theta_mode;
+  double* vw = vir ? c->vir : nullptr;
+  if (vir) k_zero<<<1, 64, 0, c->stream>>>(c->vir, 6 * (size_t)c->nsys);
+  if (c->mixed) {
+    if (vir) force_terms<float, false, true>(c, c->dTf);
+    else force_terms<float, false, false>(c, c->dTf);
+  } else if (c->theta_mode) {
+    force_terms<double, true, false>(c, c->dT);
+  } else {
+    if (vir) force_terms<double, false, true>(c, c->dT);
+    else force_terms<double, false, false>(c, c->dT);
+  }
   if (c->slot_flj >= 0) {
     launch(c, "k_flj", [&] {
-      k_flj<<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->dFLJ, A.grad, c->eatom + (size_t)c->slot_flj * n);
+      double* ea = c->eatom + (size_t)c->slot_flj * n;
+      if (vir) k_flj<true><<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->dFLJ, A.grad, ea, vw);
+      else k_flj<false><<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->dFLJ, A.grad, ea, nullptr);
     });
   }
   if (A.ml) {
@@ -677,13 +698,19 @@ void force_call(mdg_ctx* c) {
                                                                         E.cnt, c->derr);
     });
     launch(c, "k_eam_force", [&] {
-      k_eam_force<AcklandModel><<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dEAM, E.e, E.fe, E.row, E.cnt,
-                                                                      A.grad, ea);
+      if (vir)
+        k_eam_force<AcklandModel, true><<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dEAM, E.e, E.fe, E.row,
+                                                                              E.cnt, A.grad, ea, vw);
+      else
+        k_eam_force<AcklandModel, false><<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dEAM, E.e, E.fe, E.row,
+                                                                               E.cnt, A.grad, ea, nullptr);
     });
   }
   if (c->slot_fbm >= 0) {
     launch(c, "k_fbm", [&] {
-      k_fbm<<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dFBM, A.grad, c->eatom + (size_t)c->slot_fbm * n);
+      double* ea = c->eatom + (size_t)c->slot_fbm * n;
+      if (vir) k_fbm<true><<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dFBM, A.grad, ea, vw);
+      else k_fbm<false><<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dFBM, A.grad, ea, nullptr);
     });
   }
   if (c->slot_tb >= 0) {
@@ -694,14 +721,20 @@ void force_call(mdg_ctx* c) {
                                                                    c->derr);
     });
     launch(c, "k_tb_force", [&] {
-      k_eam_force<TBModel><<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dTB, E.e, E.fe, E.row, E.cnt,
-                                                                 A.grad, ea);
+      if (vir)
+        k_eam_force<TBModel, true><<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dTB, E.e, E.fe, E.row, E.cnt,
+                                                                         A.grad, ea, vw);
+      else
+        k_eam_force<TBModel, false><<<blocks(n, 16), 256, 0, c->stream>>>(A, c->box, c->dTB, E.e, E.fe, E.row,
+                                                                          E.cnt, A.grad, ea, nullptr);
     });
   }
   if (c->slot_fbzl >= 0) {
     launch(c, "k_fbzl", [&] {
-      k_fbzl<<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->cfg.fbzl_r0, c->cfg.fbzl_r1, A.grad,
-                                               c->eatom + (size_t)c->slot_fbzl * n);
+      double* ea = c->eatom + (size_t)c->slot_fbzl * n;
+      if (vir) k_fbzl<true><<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->cfg.fbzl_r0, c->cfg.fbzl_r1, A.grad, ea, vw);
+      else k_fbzl<false><<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->cfg.fbzl_r0, c->cfg.fbzl_r1, A.grad, ea,
+                                                          nullptr);
     });
   }
   k_clear_flags<<<blocks(2LL * c->nsys), 256, 0, c->stream>>>(c->nsys, c->flags, c->sflags, c->pflags, c->eflags);
@@ -1380,6 +1413,36 @@ int mdg_energy_grad_noalloc(const mdg_config* cfg, int n, const int32_t* elem, c
   return rc;
 }
 
+int mdg_set_virial(mdg_ctx* c, int on) {
+  if (c->n == 0) {
+    c->err = "mdg_set_virial: set the system(s) first";
+    return 1;
+  }
+  if (on && !c->vir && dalloc(c, c->vir, 6 * (size_t)c->nsys)) return 1;
+  if ((on != 0) != c->virial) drop_graphs(c);  // the captured step's launches change
+  c->virial = on != 0;
+  return 0;
+}
+
+int mdg_virial(mdg_ctx* c, double* out) {
+  if (!c->virial) {
+    c->err = "mdg_virial: the virial is off (mdg_set_virial)";
+    return 1;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(out, c->vir, 6 * (size_t)c->nsys * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int mdg_step_virial(mdg_ctx* c, double* out) {
+  if (!c->virial) {
+    c->err = "mdg_step_virial: the virial is off (mdg_set_virial)";
+    return 1;
+  }
+  std::copy(c->vir_steps.begin(), c->vir_steps.end(), out);
+  return 0;
+}
+
 int mdg_get_params(const mdg_ctx* c, double* theta) {
   Tables t = c->host_T;
   theta_io(t, theta, false);
@@ -1504,6 +1567,11 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
     if (!c->stepidx && dalloc(c, c->stepidx, 1)) return 1;
     c->step_cap = chunk;
   }
+  if (c->virial && c->stepvir_cap < chunk) {
+    if (dalloc(c, c->stepvir, (size_t)chunk * nsys * 6)) return 1;
+    c->stepvir_cap = chunk;
+  }
+  c->vir_steps.clear();
   int fixed = fixed_dt > 0 ? 1 : 0;
   if (fixed) k_fill<<<blocks(nsys), 256, 0, c->stream>>>(c->dtp, nsys, fixed_dt);
   int nterms = (int)c->kinds.size();
@@ -1545,6 +1613,7 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
                                                    nsys, A.tb_dE ? c->ebath : nullptr, c->simt, c->tfin, c->dtprev,
                                                    c->iter);
     });
+    if (c->virial) k_step_virial<<<blocks(6LL * nsys), 256, 0, c->stream>>>(nsys, c->vir, c->stepvir, c->stepidx);
     k_step_advance<<<1, 64, 0, c->stream>>>(c->stepidx);
   };
   bool use_graph = !c->profiling && c->use_graphs;
@@ -1568,6 +1637,12 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
     if (check_err(c)) return 1;
     if (c->traj_on && drain_trajectory(c)) return 1;
     HIPCHK(c, hipMemcpy(o.data(), c->stepout, (size_t)m * nsys * 4 * sizeof(double), hipMemcpyDeviceToHost));
+    if (c->virial) {
+      size_t v0 = c->vir_steps.size();
+      c->vir_steps.resize(v0 + (size_t)m * nsys * 6);
+      HIPCHK(c, hipMemcpy(&c->vir_steps[v0], c->stepvir, (size_t)m * nsys * 6 * sizeof(double),
+                          hipMemcpyDeviceToHost));
+    }
     for (int s = 0; s < m; s++)
       for (int q = 0; q < nsys; q++) {
         const double* v = &o[((size_t)s * nsys + q) * 4];

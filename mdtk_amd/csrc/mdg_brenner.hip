@@ -40,8 +40,9 @@ __device__ __forceinline__ double nsum(const Atoms& A, int a, int b, int only_ty
 }
 
 // Scatter c * d f_ak/dx over a's neighbours other than b (optionally one type).
+template <class Wv>
 __device__ __forceinline__ void nsum_back(const Atoms& A, int a, int b, int only_type, double c, D3& ga,
-                                          double* grad) {
+                                          double* grad, Wv& vw) {
   if (c == 0.0) return;
   int nb = A.bcnt[a];
   for (int t = 0; t < nb; t++) {
@@ -50,6 +51,7 @@ __device__ __forceinline__ void nsum_back(const Atoms& A, int a, int b, int only
     if (only_type >= 0 && A.type[k.k] != only_type) continue;
     D3 v = k.rv * (k.df * c / k.r);
     ga = ga + v;
+    vw.push(k.rv, v);
     atomic_add3(grad, k.k, -v);
   }
 }
@@ -65,15 +67,19 @@ __device__ __forceinline__ double dFsw(double x) {
 }
 }  // namespace
 
+template <bool VIR>
 __global__ void __launch_bounds__(256) k_brenner_bonds(Atoms A, const Tables* __restrict__ T, Box bx,
                                                       const int2* __restrict__ bonds,
                                                       const int* __restrict__ nbonds, double* __restrict__ grad,
-                                                      double* __restrict__ eatom, unsigned* err) {
+                                                      double* __restrict__ eatom, unsigned* err,
+                                                      double* __restrict__ vir) {
   const BrennerParams& P = T->bren;
+  Virial<VIR> vw(vir, 1.0);
   int nbt = min(*nbonds, A.bond_cap);
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nbt; t += gridDim.x * blockDim.x) {
     int2 bd = bonds[t];
     int i = bd.x, j = bd.y;
+    if constexpr (VIR) vw.at(A.sys[i]);
     int ti = A.type[i], tj = A.type[j];
     D3 rv = depos(A.p, i, j, bx);
     double r = sqrt(norm2(rv));
@@ -182,6 +188,8 @@ __global__ void __launch_bounds__(256) k_brenner_bonds(Atoms A, const Tables* __
             ga = ga + (da + db) * cc;
             gb = gb - da * cc;
             gk = gk - db * cc;
+            vw.push(ra, da * cc);
+            vw.push(k.rv, db * cc);
           } else {
             G = P.G_HH;
           }
@@ -189,6 +197,7 @@ __global__ void __launch_bounds__(256) k_brenner_bonds(Atoms A, const Tables* __
             D3 v = k.rv * (k.df * (G * E * seedD) / k.r);
             ga = ga + v;
             gk = gk - v;
+            vw.push(k.rv, v);
           }
           if (al != 0.0) {
             double dt = E * al;
@@ -199,18 +208,20 @@ __global__ void __launch_bounds__(256) k_brenner_bonds(Atoms A, const Tables* __
             D3 v = k.rv * (-dt * V3 / k.r);
             ga = ga + v;
             gk = gk - v;
+            vw.push(ra, u);
+            vw.push(k.rv, v);
           }
           atomic_add3(grad, k.k, gk);
         }
         if (ta == T_C && tb <= T_H) {
-          nsum_back(A, a, b, T_H, HN[s][1] * seedD, ga, grad);
-          nsum_back(A, a, b, T_C, HN[s][2] * seedD, ga, grad);
+          nsum_back(A, a, b, T_H, HN[s][1] * seedD, ga, grad, vw);
+          nsum_back(A, a, b, T_C, HN[s][2] * seedD, ga, grad, vw);
         }
       }
       if (ti == T_C && tj == T_C) {
         double Wf = W / 2.0;
-        nsum_back(A, i, j, -1, FN[1] * Wf, gi, grad);
-        nsum_back(A, j, i, -1, FN[2] * Wf, gj, grad);
+        nsum_back(A, i, j, -1, FN[1] * Wf, gi, grad, vw);
+        nsum_back(A, j, i, -1, FN[2] * Wf, gj, grad, vw);
         double Vc = FN[3] * Wf;
         if (Vc != 0.0) {
           for (int s = 0; s < 2; s++) {
@@ -224,12 +235,13 @@ __global__ void __launch_bounds__(256) k_brenner_bonds(Atoms A, const Tables* __
               if (k.df != 0.0) {
                 D3 v = k.rv * (k.df * (Fsw(x) * Vc) / k.r);
                 ga = ga + v;
+                vw.push(k.rv, v);
                 atomic_add3(grad, k.k, -v);
               }
               double c2 = k.f * dFsw(x) * Vc;
               if (c2 != 0.0) {
                 D3 gk = d3(0, 0, 0);
-                nsum_back(A, k.k, a, -1, c2, gk, grad);
+                nsum_back(A, k.k, a, -1, c2, gk, grad, vw);
                 atomic_add3(grad, k.k, gk);
               }
             }
@@ -243,10 +255,17 @@ __global__ void __launch_bounds__(256) k_brenner_bonds(Atoms A, const Tables* __
     D3 v = rv * (cr / r);
     gi = gi + v;
     gj = gj - v;
+    vw.push(rv, v);
     atomic_add3(grad, i, gi);
     atomic_add3(grad, j, gj);
     if (e != 0.0) unsafeAtomicAdd(eatom + i, e);
   }
+  vw.flush();
 }
+
+template __global__ void k_brenner_bonds<false>(Atoms, const Tables*, Box, const int2*, const int*, double*,
+                                                double*, unsigned*, double*);
+template __global__ void k_brenner_bonds<true>(Atoms, const Tables*, Box, const int2*, const int*, double*,
+                                               double*, unsigned*, double*);
 
 }  // namespace mdg

@@ -426,4 +426,95 @@ __device__ __forceinline__ void scatter3(const G& g, int a, V3<R> v, double gs) 
   }
 }
 
+// Virial of one kernel's terms, W_ab = sum over pair pushes of -r_a v_b,
+// where a push adds +v to atom a and -v to atom b and r = x_a - x_b (the
+// minimum-image vector the term itself uses): every force term here is a
+// sum of such pushes along its own pair vectors, so W is the virial of the
+// forces as computed, for any box.  Components xx, yy, zz, xy, xz, yz; the
+// kernel's energy unit, FP64.  Accumulated per thread for the system of the
+// current term and added into vir[6 s + c] (erg) when the system changes and
+// at the end.  Virial<false> compiles to nothing.
+template <bool ON>
+struct Virial {
+  __device__ __forceinline__ Virial(double* = nullptr, double = 1.0) {}
+  __device__ __forceinline__ void at(int) {}
+  template <class R>
+  __device__ __forceinline__ void push(V3<R>, V3<R>) {}
+  template <class R>
+  __device__ __forceinline__ void push_half(V3<R>, V3<R>) {}
+  __device__ __forceinline__ void flush() {}
+  __device__ __forceinline__ void flush_wave() {}
+  template <int LW>
+  __device__ __forceinline__ void flush_group(int) {}
+};
+
+template <>
+struct Virial<true> {
+  double w[6] = {0, 0, 0, 0, 0, 0};
+  int sys = -1;
+  double* out = nullptr;
+  double scale = 1.0;
+  __device__ __forceinline__ Virial(double* o = nullptr, double sc = 1.0) : out(o), scale(sc) {}
+  __device__ __forceinline__ void add_out() {
+    if (sys < 0) return;
+#pragma unroll
+    for (int c = 0; c < 6; c++)
+      if (w[c] != 0.0) unsafeAtomicAdd(out + 6 * sys + c, w[c] * scale);
+#pragma unroll
+    for (int c = 0; c < 6; c++) w[c] = 0.0;
+  }
+  __device__ __forceinline__ void at(int s) {
+    if (s != sys) {
+      add_out();
+      sys = s;
+    }
+  }
+  template <class R>
+  __device__ __forceinline__ void push(V3<R> r, V3<R> v) {
+    const double rx = double(r.x), ry = double(r.y), rz = double(r.z);
+    const double vx = double(v.x), vy = double(v.y), vz = double(v.z);
+    w[0] -= rx * vx;
+    w[1] -= ry * vy;
+    w[2] -= rz * vz;
+    w[3] -= rx * vy;
+    w[4] -= rx * vz;
+    w[5] -= ry * vz;
+  }
+  // A pair visited from both of its atoms (gather kernels): each visit
+  // pushes half.
+  template <class R>
+  __device__ __forceinline__ void push_half(V3<R> r, V3<R> v) {
+    push<R>(r * R(0.5), v);
+  }
+  __device__ __forceinline__ void flush() { add_out(); }
+  // A group of LW consecutive lanes working on one atom (same system): sum
+  // over the group, lane sl == 0 adds it.
+  template <int LW>
+  __device__ __forceinline__ void flush_group(int sl) {
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+      double v = w[c];
+      for (int o = LW / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, LW);
+      w[c] = (sl == 0) ? v : 0.0;
+    }
+    add_out();
+  }
+  // Whole-wave flush (every lane of the wave must call it): one set of
+  // atomics per wave when the lanes that hold terms share one system.
+  __device__ __forceinline__ void flush_wave() {
+    int s0 = sys;
+    for (int o = 32; o > 0; o >>= 1) s0 = max(s0, __shfl_xor(s0, o, 64));
+    if (s0 >= 0 && __all(sys == s0 || sys < 0)) {
+#pragma unroll
+      for (int c = 0; c < 6; c++) {
+        double v = w[c];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        w[c] = ((threadIdx.x & 63) == 0) ? v : 0.0;
+      }
+      sys = ((threadIdx.x & 63) == 0) ? s0 : -1;
+    }
+    add_out();
+  }
+};
+
 }  // namespace mdg

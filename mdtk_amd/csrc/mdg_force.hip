@@ -424,17 +424,21 @@ __device__ __forceinline__ void bo_forward(const Atoms& A, const TablesT<R>* __r
 }
 
 // Scatter c * d f_ak / dx for the pair (a,k) along u_ak = rv/r (a +=, k -=).
-template <class R, class G>
-__device__ __forceinline__ void pair_push(const G& g, int k, V3<R> rv, R r, R c, V3<R>& ga, double gs) {
+template <class R, class G, class W>
+__device__ __forceinline__ void pair_push(const G& g, int k, V3<R> rv, R r, R c, V3<R>& ga, double gs, W& vw) {
   V3<R> v = rv * (c / r);
   ga = ga + v;
+  vw.push(rv, v);
   scatter3<R>(g, k, -v, gs);
 }
 
-// Adjoint pass: adds W * db_ij/dx to grad (W in the kernel's energy unit).
-template <class R, class G>
+// Adjoint pass: adds W * db_ij/dx to grad (W in the kernel's energy unit);
+// every contribution is a push +v, -v along one of the term's pair vectors,
+// which vw adds to the virial.
+template <class R, class G, class Wv>
 __device__ __forceinline__ void bo_backward(const Atoms& A, const TablesT<R>* __restrict__ T, const BPair<R>& p,
-                                            const BOState& st, double W, G grad, double gs, unsigned* err) {
+                                            const BOState& st, double W, G grad, double gs, unsigned* err,
+                                            Wv& vw) {
   const ReboParamsT<R>& P = T->rebo;
   V3<R> gi = v3<R>(0, 0, 0), gj = v3<R>(0, 0, 0);
   double dN[2] = {0, 0}, dNH[2] = {0, 0}, dNC[2] = {0, 0}, dNconj = 0.0;
@@ -466,12 +470,15 @@ __device__ __forceinline__ void bo_backward(const Atoms& A, const TablesT<R>* __
         ga = ga + (da + db) * cc;
         gb = gb - da * cc;
         gk = gk - db * cc;
+        vw.push(q.rv, da * cc);
+        vw.push(k.rv, db * cc);
       }
       if (g.has_dN) dN[s] += double(g.dGdN * V1);
       if (k.df != R(0)) {
         V3<R> v = k.rv * (k.df * (g.G * E * seedR) / k.r);
         ga = ga + v;
         gk = gk - v;
+        vw.push(k.rv, v);
       }
       if (q.ta == T_H) {
         R dt = E * P.expscale;
@@ -483,6 +490,8 @@ __device__ __forceinline__ void bo_backward(const Atoms& A, const TablesT<R>* __
         V3<R> v = k.rv * (-dt * V3s / k.r);
         ga = ga + v;
         gk = gk - v;
+        vw.push(q.rv, u);
+        vw.push(k.rv, v);
       }
       scatter3<R>(grad, k.k, gk, gs);
     }
@@ -540,18 +549,23 @@ __device__ __forceinline__ void bo_backward(const Atoms& A, const TablesT<R>* __
           gj = gj + (dc - da) * V;
           gk = gk - db * V;
           gl = gl - dc * V;
+          vw.push(a_, da * V);
+          vw.push(b_, db * V);
+          vw.push(c_, dc * V);
         }
         R Vk = (R(1) - c * c) * fjl * Ws;
         if (Vk != R(0) && k.r >= R1k && k.r <= R2k) {
           V3<R> v = k.rv * (dsprime<R>(k.r, R1k, R2k) * Vk / k.r);
           gi = gi + v;
           gk = gk - v;
+          vw.push(k.rv, v);
         }
         R Vl = (R(1) - c * c) * fik * Ws;
         if (Vl != R(0) && l.r >= R1l && l.r <= R2l) {
           V3<R> v = l.rv * (dsprime<R>(l.r, R1l, R2l) * Vl / l.r);
           gj = gj + v;
           gl = gl - v;
+          vw.push(l.rv, v);
         }
         scatter3<R>(grad, k.k, gk, gs);
         scatter3<R>(grad, l.k, gl, gs);
@@ -579,7 +593,7 @@ __device__ __forceinline__ void bo_backward(const Atoms& A, const TablesT<R>* __
         c += Sp * sconj;
         coefk = double(k.f) * double(dsprime<R>(R(Nk), P.Ntot[0], P.Ntot[1])) * sconj;
       }
-      if (c != 0.0 && k.df != R(0)) pair_push<R>(grad, k.k, k.rv, k.r, R(double(k.df) * c), ga, gs);
+      if (c != 0.0 && k.df != R(0)) pair_push<R>(grad, k.k, k.rv, k.r, R(double(k.df) * c), ga, gs, vw);
       if (coefk != 0.0) {
         // d N_ka: neighbours m of k other than a
         V3<R> gk = v3<R>(0, 0, 0);
@@ -589,6 +603,7 @@ __device__ __forceinline__ void bo_backward(const Atoms& A, const TablesT<R>* __
           if (m.k == q.a || m.df == R(0)) continue;
           V3<R> v = m.rv * R(double(m.df) * coefk / double(m.r));
           gk = gk + v;
+          vw.push(m.rv, v);
           scatter3<R>(grad, m.k, -v, gs);
         }
         scatter3<R>(grad, k.k, gk, gs);
@@ -607,12 +622,14 @@ __device__ __forceinline__ void bo_backward(const Atoms& A, const TablesT<R>* __
 // ~1e-5 eV/A per bond.  The bond order b_ij (and its adjoint) runs in R.
 // TH: also accumulate dE/dtheta of the pair parameters into dth (FP64 only).
 // OCC: waves per SIMD the register allocation is held to (a tuning choice).
-template <class R, bool TH, int OCC>
+// VIR: also accumulate the per-system virial into vir (erg).
+template <class R, bool TH, int OCC, bool VIR>
 __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<R>* __restrict__ T,
                                                    const Tables* __restrict__ T64, Box bx,
                                                    const int2* __restrict__ bonds, const int* __restrict__ nbonds,
                                                    double* __restrict__ grad, double* __restrict__ eatom,
-                                                   unsigned* err, double* __restrict__ dth) {
+                                                   unsigned* err, double* __restrict__ dth,
+                                                   double* __restrict__ vir) {
   __shared__ int s_key[kBondCache];
   __shared__ double s_val[3 * kBondCache];
   __shared__ int s_fill;
@@ -621,6 +638,7 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
   __syncthreads();
   const double gs = T->eunit / T->lunit;
   const double su = 1.0 / T->eunit;  // erg -> the table's energy unit (adjoint seed)
+  Virial<VIR> vw(vir, T->eunit);
   int nbt = min(*nbonds, A.bond_cap);
   int b0, b1;
   block_share(nbt, b0, b1, A.wg_map);
@@ -641,6 +659,9 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
     double f, df;
     cutoff<double>(r, P.R[a][b][0], P.R[a][b][1], f, df);
     if (t >= b1) f = 0.0;  // past the share: no work this pass
+    if constexpr (VIR) {
+      if (t < b1) vw.at(A.sys[p.i]);
+    }
     {  // f_ij inside the bond order in R, as the coordination pass summed it
       R fr, dfr;
       cutoff<R>(p.r, T->rebo.R[a][b][0], T->rebo.R[a][b][1], fr, dfr);
@@ -667,7 +688,7 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
         bo_forward<R>(A, T, p, st, err);
         bb = st.b;
         if (VA != 0.0) {
-          bo_backward<R>(A, T, p, st, VA * su, gc, gs, err);
+          bo_backward<R>(A, T, p, st, VA * su, gc, gs, err, vw);
           double vder = (-P.beta[0][a][b]) * u1 + (-P.beta[1][a][b]) * u2 + (-P.beta[2][a][b]) * u3;
           cr += (-vder * f * bb) + (-vsum * bb) * df;
           eb += bb * VA;
@@ -691,11 +712,13 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
       D3 v = rd * (cr / r);
       atomic_add3(gc, p.i, v);
       atomic_add3(gc, p.j, -v);
+      vw.push(rd * su, v);  // FP64 cm and erg/cm, in the kernel's energy unit
     }
     __syncthreads();
     gc.drain_if_crowded();
   }
   gc.flush();
+  vw.flush_wave();
 }
 
 // ------------------------------------------------------------ C_ij paths
@@ -758,8 +781,8 @@ __device__ __forceinline__ R path_weight(const PathBonds<R>& pb) {
 }
 
 // Gradient of C_ij = 1 - prod f over the path's bonds, seeded with V.
-template <class R, class G>
-__device__ __forceinline__ void cij_backward(const PathBonds<R>& pb, R V, const G& grad, double gs) {
+template <class R, class G, class Wv>
+__device__ __forceinline__ void cij_backward(const PathBonds<R>& pb, R V, const G& grad, double gs, Wv& vw) {
 #pragma unroll
   for (int q = 0; q < 3; q++) {
     if (q >= pb.hops || pb.df[q] == R(0)) continue;
@@ -769,6 +792,7 @@ __device__ __forceinline__ void cij_backward(const PathBonds<R>& pb, R V, const 
       if (u != q && u < pb.hops) mm *= pb.f[u];
     R c = pb.df[q] * (-mm * V);
     V3<R> v = pb.rv[q] * (c / pb.r[q]);
+    vw.push(pb.rv[q], v);
     scatter3<R>(grad, pb.nodes[q], v, gs);
     scatter3<R>(grad, pb.nodes[q + 1], -v, gs);
   }
@@ -918,12 +942,12 @@ __global__ void __launch_bounds__(256) k_pair_rows(Atoms A, const TablesT<R>* __
 // fire-and-forget scatter atomics waits for them).
 // All LDS arrays are per group, so the phases are ordered by wavefront-scope
 // fences (a wave's LDS operations complete in order).
-template <class R, bool TH, int OCC, int LW>
+template <class R, bool TH, int OCC, int LW, bool VIR>
 __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
                                                      double* __restrict__ grad, double* __restrict__ eatom,
                                                      int4* __restrict__ queue, int* __restrict__ qcount, int qsub,
                                                      long long* __restrict__ npairs, unsigned* err,
-                                                     double* __restrict__ dth) {
+                                                     double* __restrict__ dth, double* __restrict__ vir) {
   constexpr int G = 4 * (64 / LW);  // atoms in flight per workgroup
   // this workgroup's b* sub-queue
   int* const qc = qcount + (blockIdx.x & (kQueues - 1)) * kQStride;
@@ -958,10 +982,12 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
   gc.clear();
   __syncthreads();
   long long np = 0;
+  Virial<VIR> vw(vir, T->eunit);
   // one atom per group (returns are group-uniform)
   auto atom = [&](const int i) {
     int ti = A.type[i];
     if (ti > T_H) return;
+    if constexpr (VIR) vw.at(A.sys[i]);
     const double* Lb = box_of(bx, i);
     double4 pi4 = A.p[i];
     double xi = pi4.x, yi = pi4.y, zi = pi4.z;
@@ -1185,8 +1211,9 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       R cr = der * f * V + val * V * df;
       V3<R> v = rv * (cr / r);
       gi = gi + d3(double(v.x), double(v.y), double(v.z));
+      vw.push(rv, v);
       scatter3<R>(gc, j, -v, gs);
-      if (pkey) cij_backward<R>(pb, R(1) * LJ, gc, gs);
+      if (pkey) cij_backward<R>(pb, R(1) * LJ, gc, gs, vw);
       e += to_erg<R>(double(R(1) * C * LJ), T->eunit);
       if constexpr (TH) {  // S_r = 0: E = C f 4 eps ((sigma/r)^12 - (sigma/r)^6)
         double de = double(C) * double(f) * 4.0 * double(s12 - s6);
@@ -1241,6 +1268,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     gc.drain_if_crowded();
   }
   gc.flush();
+  vw.flush_wave();
   count_add(npairs, np);
 }
 
@@ -1260,11 +1288,11 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
 //         = 1) and left to PASS 2 entirely.  It compiles without the bond
 //         order's second half and adjoint, so it keeps few registers.
 // PASS 2: the full evaluation of the marked pairs.
-template <class R, bool TH, int OCC, int PASS>
+template <class R, bool TH, int OCC, int PASS, bool VIR>
 __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
                                               int4* __restrict__ queue, const int* __restrict__ qcount,
                                               int qsub, double* __restrict__ grad, double* __restrict__ eatom,
-                                              unsigned* err, double* __restrict__ dth) {
+                                              unsigned* err, double* __restrict__ dth, double* __restrict__ vir) {
   static_assert(kQueues == 256, "one thread per sub-queue in the prefix below (blockDim 256)");
   static_assert(PASS >= 0 && PASS <= 2 && (PASS == 0 || !TH), "parameter gradients use the one-pass form");
   const double gs = T->eunit / T->lunit;
@@ -1293,6 +1321,7 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
   gc.clear();
   __syncthreads();
   const int nq = s_pre[kQueues];
+  Virial<VIR> vw(vir, T->eunit);
   int b0, b1;
   block_share(nq, b0, b1, A.wg_map);
   for (int base = b0; base < b1; base += 256) {
@@ -1355,6 +1384,7 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
       if (!below) bo_finish<R>(A, T, p, st, err);
     }
     if (!done) {
+    if constexpr (VIR) vw.at(A.sys[i]);
     R bs = below ? bmin : R(st.b);
     R Stb = below ? R(1) : Sfun<R>(bs, bmin, bmax);
     R SrSb = R(1) + (Str * Stb - Str);
@@ -1381,13 +1411,14 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
       if (dStr != R(0) && Stb != R(0)) cr += dStr * Stb * V2;
       if constexpr (PASS != 1) {
         R dStb = below ? R(0) : dSfun<R>(bs, bmin, bmax);
-        if (dStb != R(0)) bo_backward<R>(A, T, p, st, double(Str * dStb * V2), gc, gs, err);
+        if (dStb != R(0)) bo_backward<R>(A, T, p, st, double(Str * dStb * V2), gc, gs, err, vw);
       }
     }
     V3<R> v = rv * (cr / r);
     scatter3<R>(gc, i, v, gs);
     scatter3<R>(gc, j, -v, gs);
-    if (pkey && SrSb * LJ != R(0)) cij_backward<R>(pb, SrSb * LJ, gc, gs);
+    vw.push(rv, v);
+    if (pkey && SrSb * LJ != R(0)) cij_backward<R>(pb, SrSb * LJ, gc, gs, vw);
     add_energy(eatom, i, to_erg<R>(double(SrSb * C * LJ), T->eunit));
     if constexpr (TH) {
       // S_r depends on r/sigma only: dS_r/dsigma = -(r/sigma) dS_r/dr
@@ -1407,15 +1438,16 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
     gc.drain_if_crowded();
   }
   gc.flush();
+  vw.flush_wave();
 }
 
 // ------------------------------------------------------------ torsion
 // ETors::operator() (AIREBO_ETors.cxx:32-115) over C-C bonds.
-template <class R>
+template <class R, bool VIR>
 __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
                                               const int2* __restrict__ bonds, const int* __restrict__ nbonds,
                                               int compat, double* __restrict__ grad, double* __restrict__ eatom,
-                                              long long* __restrict__ nquads) {
+                                              long long* __restrict__ nquads, double* __restrict__ vir) {
   __shared__ int s_key[kTorsCache];
   __shared__ double s_val[3 * kTorsCache];
   __shared__ int s_fill;
@@ -1424,6 +1456,7 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
   __syncthreads();
   long long nq = 0;
   const double gs = T->eunit / T->lunit;
+  Virial<VIR> vw(vir, T->eunit);
   int nbt = min(*nbonds, A.bond_cap);
   int b0, b1;
   block_share(nbt, b0, b1, A.wg_map);
@@ -1433,6 +1466,7 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
     int2 bd = (t < b1) ? bonds[t] : make_int2(0, 0);
     int i = bd.x, j = bd.y;
     if (t < b1 && A.type[i] == T_C && A.type[j] == T_C) {
+      if constexpr (VIR) vw.at(A.sys[i]);
       V3<R> a_ = to_local<R>(depos(A.p, i, j, bx));
       R rij = sqrt(norm2(a_));
       R wij, dwij;
@@ -1480,6 +1514,9 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
               gj = gj + (dc - da) * V;
               gk = gk - db * V;
               gl = gl - dc * V;
+              vw.push(a_, da * V);
+              vw.push(b_, db * V);
+              vw.push(c_, dc * V);
             }
           }
           // cutoff-weight derivatives
@@ -1487,16 +1524,19 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
             V3<R> v = k.rv * (k.df * (wij * wjl * vt) / k.r);
             gi = gi + v;
             gk = gk - v;
+            vw.push(k.rv, v);
           }
           if (dwij != R(0)) {
             V3<R> v = a_ * (dwij * (wki * wjl * vt) / rij);
             gi = gi + v;
             gj = gj - v;
+            vw.push(a_, v);
           }
           if (l.df != R(0)) {
             V3<R> v = l.rv * (l.df * (wki * wij * vt) / l.r);
             gj = gj + v;
             gl = gl - v;
+            vw.push(l.rv, v);
           }
           scatter3<R>(gc, k.k, gk, gs);
           scatter3<R>(gc, l.k, gl, gs);
@@ -1511,21 +1551,34 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
     gc.drain_if_crowded();
   }
   gc.flush();
+  vw.flush_wave();
   count_add(nquads, nq);
 }
 
 #define MDG_INSTANTIATE_OCC(R, TH, OCC)                                                                        \
-  template __global__ void k_rebo_bonds<R, TH, OCC>(Atoms, const TablesT<R>*, const Tables*, Box, const int2*, \
-                                                    const int*, double*, double*, unsigned*, double*);         \
-  template __global__ void k_bstar<R, TH, OCC, 0>(Atoms, const TablesT<R>*, Box, int4*, const int*, int,     \
-                                                  double*, double*, unsigned*, double*);
+  template __global__ void k_rebo_bonds<R, TH, OCC, false>(Atoms, const TablesT<R>*, const Tables*, Box,          \
+                                                           const int2*, const int*, double*, double*, unsigned*, \
+                                                           double*, double*);                                    \
+  template __global__ void k_bstar<R, TH, OCC, 0, false>(Atoms, const TablesT<R>*, Box, int4*, const int*, int,  \
+                                                         double*, double*, unsigned*, double*, double*);
 #define MDG_INSTANTIATE_TH(R, TH, OCC, LW)                                                                     \
-  template __global__ void k_lj_wave<R, TH, OCC, LW>(Atoms, const TablesT<R>*, Box, double*, double*, int4*,   \
-                                                     int*, int, long long*, unsigned*, double*);
+  template __global__ void k_lj_wave<R, TH, OCC, LW, false>(Atoms, const TablesT<R>*, Box, double*, double*,   \
+                                                            int4*, int*, int, long long*, unsigned*, double*,   \
+                                                            double*);
 #define MDG_INSTANTIATE(R)                                                                                     \
   template __global__ void k_coord<R>(Atoms, const TablesT<R>*, Box, unsigned*);                              \
-  template __global__ void k_etors<R>(Atoms, const TablesT<R>*, Box, const int2*, const int*, int, double*,   \
-                                      double*, long long*);                                                  \
+  template __global__ void k_etors<R, false>(Atoms, const TablesT<R>*, Box, const int2*, const int*, int,     \
+                                             double*, double*, long long*, double*);                          \
+  template __global__ void k_etors<R, true>(Atoms, const TablesT<R>*, Box, const int2*, const int*, int,      \
+                                            double*, double*, long long*, double*);                           \
+  template __global__ void k_rebo_bonds<R, false, 1, true>(Atoms, const TablesT<R>*, const Tables*, Box,      \
+                                                           const int2*, const int*, double*, double*,         \
+                                                           unsigned*, double*, double*);                      \
+  template __global__ void k_lj_wave<R, false, 1, 16, true>(Atoms, const TablesT<R>*, Box, double*, double*,  \
+                                                            int4*, int*, int, long long*, unsigned*, double*,  \
+                                                            double*);                                          \
+  template __global__ void k_bstar<R, false, 1, 0, true>(Atoms, const TablesT<R>*, Box, int4*, const int*,    \
+                                                         int, double*, double*, unsigned*, double*, double*);  \
   template __global__ void k_pair_rows<R>(Atoms, const TablesT<R>*, Box, const int*, const int*, double,      \
                                           long long*);
 MDG_INSTANTIATE(double)
@@ -1549,10 +1602,12 @@ MDG_INSTANTIATE_OCC(float, false, 2)
 MDG_INSTANTIATE_OCC(float, false, 3)
 MDG_INSTANTIATE_OCC(float, false, 4)
 #define MDG_INSTANTIATE_SPLIT(R, OCC)                                                                           \
-  template __global__ void k_bstar<R, false, OCC, 1>(Atoms, const TablesT<R>*, Box, int4*, const int*, int,    \
-                                                     double*, double*, unsigned*, double*);                    \
-  template __global__ void k_bstar<R, false, OCC, 2>(Atoms, const TablesT<R>*, Box, int4*, const int*, int,    \
-                                                     double*, double*, unsigned*, double*);
+  template __global__ void k_bstar<R, false, OCC, 1, false>(Atoms, const TablesT<R>*, Box, int4*, const int*,  \
+                                                            int, double*, double*, unsigned*, double*,         \
+                                                            double*);                                          \
+  template __global__ void k_bstar<R, false, OCC, 2, false>(Atoms, const TablesT<R>*, Box, int4*, const int*,  \
+                                                            int, double*, double*, unsigned*, double*,         \
+                                                            double*);
 MDG_INSTANTIATE_SPLIT(double, 1)
 MDG_INSTANTIATE_SPLIT(double, 2)
 MDG_INSTANTIATE_SPLIT(float, 1)

@@ -498,9 +498,13 @@ __device__ __forceinline__ bool zbl_target(int elem) {
 }
 __device__ __forceinline__ bool zbl_ion(int elem) { return elem == 40 || elem == 131; }
 
-__global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* __restrict__ grad, double* __restrict__ eatom) {
+template <bool VIR>
+__global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* __restrict__ grad, double* __restrict__ eatom,
+                       double* __restrict__ vir) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n || !zbl_ion(A.elem[i])) return;
+  Virial<VIR> vw(vir, 1.0);
+  if constexpr (VIR) vw.at(A.sys[i]);
   double e = 0.0;
   int cnt = A.nll_cnt[i];
   const int* list = A.nll + (size_t)i * A.cap_l;
@@ -530,10 +534,14 @@ __global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* __restrict
     D3 v = rv * (cr / r);
     atomic_add3(grad, a, v);
     atomic_add3(grad, b, -v);
+    vw.push(rv, v);
     e += val * f;
   }
   if (e != 0.0) unsafeAtomicAdd(eatom + i, e);
+  vw.flush();
 }
+template __global__ void k_fbzl<false>(Atoms, Box, double, double, double*, double*, double*);
+template __global__ void k_fbzl<true>(Atoms, Box, double, double, double*, double*, double*);
 
 // FLJ::operator() (FLJ.cxx:256-341) for the pairs {H,C} x {Cu,Au}: each metal
 // atom takes its hydrocarbon partners from the long list (so every pair once);
@@ -550,12 +558,15 @@ __device__ __forceinline__ int flj_index(int elem) {
   }
 }
 
+template <bool VIR>
 __global__ void k_flj(Atoms A, Box bx, const FLJTable* __restrict__ T, double* __restrict__ grad,
-                      double* __restrict__ eatom) {
+                      double* __restrict__ eatom, double* __restrict__ vir) {
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= A.n) return;
   const int em = A.elem[m];
   if (em != 64 && em != 197) return;
+  Virial<VIR> vw(vir, 1.0);
+  if constexpr (VIR) vw.at(A.sys[m]);
   const int im = flj_index(em);
   const int cnt = A.nll_cnt[m];
   const int* list = A.nll + (size_t)m * A.cap_l;
@@ -603,9 +614,13 @@ __global__ void k_flj(Atoms A, Box bx, const FLJTable* __restrict__ T, double* _
     D3 v = rv * (der / r);
     atomic_add3(grad, a, v);
     atomic_add3(grad, b, -v);
+    vw.push(rv, v);
     if (val != 0.0) unsafeAtomicAdd(eatom + a, val);
   }
+  vw.flush();
 }
+template __global__ void k_flj<false>(Atoms, Box, const FLJTable*, double*, double*, double*);
+template __global__ void k_flj<true>(Atoms, Box, const FLJTable*, double*, double*, double*);
 
 // ------------------------------------------------------------ Ackland EAM
 // Ackland::operator() (Ackland.cxx:31-57): E = sum_i F(rho_i) + sum_{i<j} V,
@@ -808,12 +823,14 @@ __global__ void __launch_bounds__(256) k_eam_density(Atoms A, Box bx, const type
   }
 }
 
-template <class M>
+// Each pair is visited from both atoms (gather), so each visit adds half of
+// the pair's virial.
+template <class M, bool VIR>
 __global__ void __launch_bounds__(256) k_eam_force(Atoms A, Box bx, const typename M::Table* __restrict__ T,
                                                    const signed char* __restrict__ ek,
                                                    const double2* __restrict__ fe, const int* __restrict__ row,
                                                    const int* __restrict__ rcnt, double* __restrict__ grad,
-                                                   double* __restrict__ eatom) {
+                                                   double* __restrict__ eatom, double* __restrict__ vir) {
   constexpr int LW = 16;
   const int sl = threadIdx.x % LW;
   const int i = blockIdx.x * (blockDim.x / LW) + threadIdx.x / LW;
@@ -826,6 +843,8 @@ __global__ void __launch_bounds__(256) k_eam_force(Atoms A, Box bx, const typena
   const double* Lb = box_of(bx, i);
   const double fpi = fe[i].x;
   double gx = 0.0, gy = 0.0, gz = 0.0, ev = 0.0;
+  Virial<VIR> vw(vir, 1.0);
+  if constexpr (VIR) vw.at(A.sys[i]);
   for (int q = sl; q < cnt; q += LW) {
     const int j = L[q];
     const double4 pj = A.p[j];
@@ -840,8 +859,10 @@ __global__ void __launch_bounds__(256) k_eam_force(Atoms A, Box bx, const typena
     gx += c * dx;
     gy += c * dy;
     gz += c * dz;
+    vw.push_half(D3{dx, dy, dz}, D3{c * dx, c * dy, c * dz});
     if (j > i) ev += v;
   }
+  vw.template flush_group<LW>(sl);
 #pragma unroll
   for (int o = LW / 2; o > 0; o >>= 1) {
     gx += __shfl_xor(gx, o, LW);
@@ -857,12 +878,20 @@ __global__ void __launch_bounds__(256) k_eam_force(Atoms A, Box bx, const typena
 
 template __global__ void k_eam_density<AcklandModel>(Atoms, Box, const EAMTable*, const signed char*, double2*,
                                                      double*, int*, int*, unsigned*);
-template __global__ void k_eam_force<AcklandModel>(Atoms, Box, const EAMTable*, const signed char*, const double2*,
-                                                   const int*, const int*, double*, double*);
+template __global__ void k_eam_force<AcklandModel, false>(Atoms, Box, const EAMTable*, const signed char*,
+                                                          const double2*, const int*, const int*, double*, double*,
+                                                          double*);
+template __global__ void k_eam_force<AcklandModel, true>(Atoms, Box, const EAMTable*, const signed char*,
+                                                         const double2*, const int*, const int*, double*, double*,
+                                                         double*);
 template __global__ void k_eam_density<TBModel>(Atoms, Box, const TBTable*, const signed char*, double2*, double*,
                                                 int*, int*, unsigned*);
-template __global__ void k_eam_force<TBModel>(Atoms, Box, const TBTable*, const signed char*, const double2*,
-                                              const int*, const int*, double*, double*);
+template __global__ void k_eam_force<TBModel, false>(Atoms, Box, const TBTable*, const signed char*,
+                                                     const double2*, const int*, const int*, double*, double*,
+                                                     double*);
+template __global__ void k_eam_force<TBModel, true>(Atoms, Box, const TBTable*, const signed char*,
+                                                    const double2*, const int*, const int*, double*, double*,
+                                                    double*);
 
 // FBM::operator() (FBM.cxx:70-94) for the pairs Cu-Cu and Cu-Ar: 16 lanes
 // per Cu or Ar atom gather its handled partners from the long list (both
@@ -870,8 +899,10 @@ template __global__ void k_eam_force<TBModel>(Atoms, Box, const TBTable*, const 
 // atom, as the reference's i < j loop sums it.
 __device__ __forceinline__ int fbm_index(int elem) { return elem == 64 ? 0 : elem == 40 ? 1 : -1; }
 
+template <bool VIR>
 __global__ void __launch_bounds__(256) k_fbm(Atoms A, Box bx, const FBMTable* __restrict__ T,
-                                             double* __restrict__ grad, double* __restrict__ eatom) {
+                                             double* __restrict__ grad, double* __restrict__ eatom,
+                                             double* __restrict__ vir) {
   constexpr int LW = 16;
   const int sl = threadIdx.x % LW;
   const int i = blockIdx.x * (blockDim.x / LW) + threadIdx.x / LW;
@@ -884,6 +915,8 @@ __global__ void __launch_bounds__(256) k_fbm(Atoms A, Box bx, const FBMTable* __
   const double* Lb = box_of(bx, i);
   const double R0 = T->R0, R1 = T->R1;
   double gx = 0.0, gy = 0.0, gz = 0.0, ev = 0.0;
+  Virial<VIR> vw(vir, 1.0);
+  if constexpr (VIR) vw.at(A.sys[i]);
   for (int q = sl; q < cnt; q += LW) {
     const int j = L[q];
     const double4 pj = A.p[j];
@@ -900,8 +933,10 @@ __global__ void __launch_bounds__(256) k_fbm(Atoms A, Box bx, const FBMTable* __
     gx += c * dx;
     gy += c * dy;
     gz += c * dz;
+    vw.push_half(D3{dx, dy, dz}, D3{c * dx, c * dy, c * dz});
     if (j > i) ev += e * f;
   }
+  vw.template flush_group<LW>(sl);
 #pragma unroll
   for (int o = LW / 2; o > 0; o >>= 1) {
     gx += __shfl_xor(gx, o, LW);
@@ -914,6 +949,8 @@ __global__ void __launch_bounds__(256) k_fbm(Atoms A, Box bx, const FBMTable* __
     eatom[i] = ev;
   }
 }
+template __global__ void k_fbm<false>(Atoms, Box, const FBMTable*, double*, double*, double*);
+template __global__ void k_fbm<true>(Atoms, Box, const FBMTable*, double*, double*, double*);
 
 // ------------------------------------------------------------ Verlet
 // eq.1 of SimLoop::executeMain (SimLoop.cxx:323-332): x += v dt + a dt^2/2,
@@ -1337,6 +1374,13 @@ __global__ void __launch_bounds__(256) k_traj_capture(Atoms A, TrajBuf tb, const
       R.pc[2] = A.pcz[i];
       R.atom = i - A.sys_off[s];
     }
+}
+
+// The force call's per-system virial into this step's output slot.
+__global__ void k_step_virial(int nsys, const double* __restrict__ vir, double* __restrict__ stepvir,
+                              const int* __restrict__ stepidx) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < 6 * nsys) stepvir[(size_t)(*stepidx) * 6 * nsys + q] = vir[q];
 }
 
 __global__ void k_step_advance(int* __restrict__ stepidx) {

@@ -115,9 +115,10 @@ __global__ void k_coord(Atoms A, const TablesT<R>* T, Box bx, unsigned* err);
 // TH = true (FP64 only): also accumulate dE/dtheta per system into dth
 // (kNumTheta doubles per system).  OCC: waves per SIMD the bond-order
 // kernels' register allocation is held to (1 or 2; chosen by the autotuner).
-template <class R, bool TH, int OCC>
+template <class R, bool TH, int OCC, bool VIR>
 __global__ void k_rebo_bonds(Atoms A, const TablesT<R>* T, const Tables* T64, Box bx, const int2* bonds,
-                             const int* nbonds, double* grad, double* eatom, unsigned* err, double* dth);
+                             const int* nbonds, double* grad, double* eatom, unsigned* err, double* dth,
+                             double* vir);
 // The b* work queue is kQueues sub-queues of qsub entries each; sub-queue q
 // holds entries [q * qsub, (q + 1) * qsub) and its count sits at
 // qcount[q * kQStride], one count per 128-byte line, so the LJ kernel's
@@ -132,17 +133,21 @@ constexpr int kTallyStride = 16;
 template <class R>
 __global__ void k_pair_rows(Atoms A, const TablesT<R>* T, Box bx, const int* pflag, const int* pflags,
                             double margin, long long* rebuilt);
-template <class R, bool TH, int OCC, int LW>
+// VIR (template flag of every force kernel): also add the per-system virial
+// of the kernel's terms into vir[6 s + c] (xx, yy, zz, xy, xz, yz; erg).
+template <class R, bool TH, int OCC, int LW, bool VIR>
 __global__ void k_lj_wave(Atoms A, const TablesT<R>* T, Box bx, double* grad, double* eatom, int4* queue,
-                          int* qcount, int qsub, long long* npairs, unsigned* err, double* dth);
-template <class R, bool TH, int OCC, int PASS>
+                          int* qcount, int qsub, long long* npairs, unsigned* err, double* dth, double* vir);
+template <class R, bool TH, int OCC, int PASS, bool VIR>
 __global__ void k_bstar(Atoms A, const TablesT<R>* T, Box bx, int4* queue, const int* qcount, int qsub,
-                        double* grad, double* eatom, unsigned* err, double* dth);
-template <class R>
+                        double* grad, double* eatom, unsigned* err, double* dth, double* vir);
+template <class R, bool VIR>
 __global__ void k_etors(Atoms A, const TablesT<R>* T, Box bx, const int2* bonds, const int* nbonds, int compat,
-                        double* grad, double* eatom, long long* nquads);
-__global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* grad, double* eatom);
-__global__ void k_flj(Atoms A, Box bx, const FLJTable* T, double* grad, double* eatom);
+                        double* grad, double* eatom, long long* nquads, double* vir);
+template <bool VIR>
+__global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* grad, double* eatom, double* vir);
+template <bool VIR>
+__global__ void k_flj(Atoms A, Box bx, const FLJTable* T, double* grad, double* eatom, double* vir);
 // Ackland EAM: the density pass (F'(rho) into fe, F into eatom), then the
 // force pass (gradient and pair energies); 16 lanes per atom, blocks of 256.
 // ek: per atom the EAM element index (Cu 0, Ag 1, Au 2) or -1; fe: per atom
@@ -172,13 +177,15 @@ struct TBModel {  // TightBinding.cxx:59-159
 template <class M>
 __global__ void k_eam_density(Atoms A, Box bx, const typename M::Table* T, const signed char* ek, double2* fe,
                               double* eatom, int* row, int* rcnt, unsigned* err);
-template <class M>
+template <class M, bool VIR>
 __global__ void k_eam_force(Atoms A, Box bx, const typename M::Table* T, const signed char* ek, const double2* fe,
-                            const int* row, const int* rcnt, double* grad, double* eatom);
+                            const int* row, const int* rcnt, double* grad, double* eatom, double* vir);
 // FBM pairs (Cu-Cu, Cu-Ar), 16 lanes per Cu or Ar atom, blocks of 256.
-__global__ void k_fbm(Atoms A, Box bx, const FBMTable* T, double* grad, double* eatom);
+template <bool VIR>
+__global__ void k_fbm(Atoms A, Box bx, const FBMTable* T, double* grad, double* eatom, double* vir);
+template <bool VIR>
 __global__ void k_brenner_bonds(Atoms A, const Tables* T, Box bx, const int2* bonds, const int* nbonds,
-                                double* grad, double* eatom, unsigned* err);
+                                double* grad, double* eatom, unsigned* err, double* vir);
 // Verlet halves over the reduction tiles: the drift writes the skin test's
 // tile part (as k_sys_disp_part), the kick the step outputs' tile part
 // (merged by k_sys_step_merge).
@@ -196,6 +203,7 @@ __global__ void k_sys_step_merge(const int* rb_off, const double* part, double* 
                                  double* dtprev, long long* iter);
 __global__ void k_bath_temp_part(Atoms A, Box bx, const int2* rb, double* part);
 __global__ void k_bath_temp_merge(Atoms A, const int* rb_off, const double* part);
+__global__ void k_step_virial(int nsys, const double* vir, double* stepvir, const int* stepidx);
 __global__ void k_step_advance(int* stepidx);
 
 // Trajectory output captured inside the step (the layouts of mdg_frame and

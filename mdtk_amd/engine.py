@@ -74,6 +74,9 @@ def load_library(path: str = LIB_PATH):
     L.mdg_event_record.argtypes = [vp, ctypes.c_int]
     L.mdg_event_elapsed.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
     L.mdg_synchronize.argtypes = [vp]
+    L.mdg_set_virial.argtypes = [vp, ctypes.c_int]
+    L.mdg_virial.argtypes = [vp, dp]
+    L.mdg_step_virial.argtypes = [vp, dp]
     L.mdg_energy_grad_noalloc.argtypes = [ctypes.POINTER(_Config), ctypes.c_int, ip, dp, dp, dp, dp]
     L.mdg_autotune.argtypes = [vp, ctypes.c_int, dp]
     L.mdg_set_thermal_bath.argtypes = [vp, ctypes.c_int, ctypes.POINTER(_Bath)]
@@ -222,6 +225,19 @@ class Engine:
         self._chk(_lib.mdg_energy_grad(self._h, _p(e), _p(g)))
         return (e if self.nsys <= 1 else e.reshape(self.nsys, self.nterms)), g
 
+    def set_virial(self, on: bool = True):
+        """Every later force call also reduces the per-system virial."""
+        self._chk(_lib.mdg_set_virial(self._h, int(bool(on))))
+        self._virial = bool(on)
+
+    def virial(self):
+        """Virial of the last force call, (6,) or (nsys, 6): xx, yy, zz, xy,
+        xz, yz in erg."""
+        m = max(self.nsys, 1)
+        w = np.zeros(6 * m)
+        self._chk(_lib.mdg_virial(self._h, _p(w)))
+        return w if self.nsys <= 1 else w.reshape(m, 6)
+
     def set_thermal_bath(self, geometry="box", To=300.0, gamma=1.0e13, z_min=1.0e6 * 1e-8, d_boundary=0.0,
                          z_min_free=-5.0e-8, center=(0.0, 0.0, 0.0), radius=1.0e6 * 1e-8, system=-1):
         """Thermal bath of SimLoop (SimLoop.hpp:118-189) for one system or all
@@ -317,6 +333,10 @@ class Engine:
         out = dict(epot=ep, ekin=ek, dt=dt, netforce=nf)
         if m > 1:
             out = {k: v.reshape(nsteps, m) for k, v in out.items()}
+        if getattr(self, "_virial", False):
+            w = np.zeros(nsteps * m * 6)
+            self._chk(_lib.mdg_step_virial(self._h, _p(w)))
+            out["virial"] = w.reshape(nsteps, 6) if m == 1 else w.reshape(nsteps, m, 6)
         return out
 
     def state(self):

@@ -61,6 +61,7 @@ def lib():
         L.orc_cos_dihedral.argtypes = [dp, ctypes.c_double, dp]
         L.orc_moved_too_much.argtypes = [ctypes.c_int, dp, ctypes.c_double, ip]
         L.orc_cij_path.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, dp, ip, ip]
+        L.orc_virial.argtypes = [ctypes.c_void_p, dp]
         _lib = L
     return _lib
 
@@ -162,6 +163,13 @@ class Oracle:
         ep, ek, dt, nf = (np.zeros(nsteps) for _ in range(4))
         self._chk(lib().orc_run(self.h, nsteps, fixed_dt, _p(ep), _p(ek), _p(dt), _p(nf)))
         return dict(epot=ep, ekin=ek, dt=dt, netforce=nf)
+
+    def virial(self):
+        """Virial of one force call at the current positions: xx, yy, zz, xy,
+        xz, yz in erg (sum over the gradient pushes of -r (x) v)."""
+        w = np.zeros(6)
+        self._chk(lib().orc_virial(self.h, _p(w)))
+        return w
 
     def cij_path(self, i: int, j: int):
         """(weight, nodes) of the C_ij path stored for the LJ pair (i, j) at

@@ -116,6 +116,27 @@ void orc_destroy(void* p) { delete static_cast<Handle*>(p); }
 int orc_num_terms(void* p) { return (int)static_cast<Handle*>(p)->kinds.size(); }
 int orc_term_kind(void* p, int i) { return static_cast<Handle*>(p)->kinds[i]; }
 
+// The virial of one force call at the current positions (six components,
+// erg), from the pushes of every primitive (see g_vir in oracle_core.hpp).
+int orc_virial(void* p, double* out) {
+  Handle* h = static_cast<Handle*>(p);
+  return guarded([&] {
+    auto& sl = h->sl;
+    if (!sl.prepared) sl.prepare();
+    sl.fpot.NL_UpdateIfNeeded(sl.atoms);
+    for (double& v : g_vir) v = 0.0;
+    g_vir_on = true;
+    try {
+      sl.fpot.compute(sl.atoms);
+    } catch (...) {
+      g_vir_on = false;
+      throw;
+    }
+    g_vir_on = false;
+    for (int c = 0; c < 6; c++) out[c] = g_vir[c];
+  });
+}
+
 // Energy (per chain entry, erg) and dE/dx (erg/cm) at the current positions,
 // exactly as one FProxy call made from SimLoop::energy().
 int orc_energy_grad(void* p, double* e_terms, double* grad) {

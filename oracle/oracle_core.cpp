@@ -102,6 +102,8 @@ Pair Pair::neg() const {  // AtomsPair.hpp:103-116
 // separation, even though the value is the constant r_alter
 // (AtomsPair.hpp:53-63).
 bool g_alter_exact = false;
+bool g_vir_on = false;
+double g_vir[6] = {0, 0, 0, 0, 0, 0};
 
 V3 Pair::dr_alter(const Atom& d) const {
   if (!alt) return dr(d);
@@ -118,6 +120,7 @@ double Pair::r_alter(double V) {  // AtomsPair.hpp:44-52
   if (V != 0.0) {
     a1->grad += dr_alter(*a1) * V;
     a2->grad += dr_alter(*a2) * V;
+    if (g_vir_on) vir_push(rv, dr_alter(*a1) * V);
   }
   return ralt_;
 }
@@ -138,6 +141,10 @@ double CosTheta(Pair& ij, Pair& ik, double V) {
     ij.a1->grad += (ga + gb) * V;
     ij.a2->grad += (-ga) * V;
     ik.a2->grad += (-gb) * V;
+    if (g_vir_on) {
+      vir_push(ij.rv, ga * V);
+      vir_push(ik.rv, gb * V);
+    }
   }
   return cc;
 }
@@ -177,6 +184,11 @@ double CosDihedral(Pair& ij, Pair& ik, Pair& jl, double V) {
     ij.a2->grad += (dc - da) * V;
     ik.a2->grad += (-db) * V;
     jl.a2->grad += (-dc) * V;
+    if (g_vir_on) {
+      vir_push(a, da * V);
+      vir_push(b, db * V);
+      vir_push(c, dc * V);
+    }
   }
   return cc;
 }

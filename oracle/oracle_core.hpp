@@ -90,6 +90,21 @@ struct Atom {
 
 V3 depos(const Atom& a1, const Atom& a2);  // mdtk/Atom.hpp:115-140
 
+// Virial of the gradient pushes (no reference counterpart): while g_vir_on,
+// every primitive that adds V*d/dx into Atom::grad (Pair::r, Pair::r_alter,
+// CosTheta, CosDihedral) also adds -r (x) v for each of its pair vectors r
+// and the push v the pair carries; components xx, yy, zz, xy, xz, yz (erg).
+extern bool g_vir_on;
+extern double g_vir[6];
+inline void vir_push(const V3& r, const V3& v) {
+  g_vir[0] -= r.x * v.x;
+  g_vir[1] -= r.y * v.y;
+  g_vir[2] -= r.z * v.z;
+  g_vir[3] -= r.x * v.y;
+  g_vir[4] -= r.x * v.z;
+  g_vir[5] -= r.y * v.z;
+}
+
 // Test switch: when true, the b* pair's r_alter gets a zero derivative (the
 // exact derivative of its constant value) instead of the reference's
 // unit-vector derivative.  Default false (reference behaviour).
@@ -118,6 +133,7 @@ struct Pair {
     if (V != 0.0) {
       a1->grad += dr(*a1) * V;
       a2->grad += dr(*a2) * V;
+      if (g_vir_on) vir_push(rv, u * V);
     }
     return r_;
   }
