@@ -660,7 +660,13 @@ void force_call(mdg_ctx* c) {
   if (c->have_long) nl_build(c, false);
   size_t nt = c->kinds.size();
   launch(c, "k_zero", [&] {
-    k_zero<<<blocks(3LL * n, 256, 4096), 256, 0, c->stream>>>(A.grad, (size_t)3 * n);
+    if (A.gfix) {  // all-zero bits are integer zero too
+      k_zero<<<blocks(6LL * n, 256, 4096), 256, 0, c->stream>>>(reinterpret_cast<double*>(A.gfix), (size_t)6 * n);
+      k_zero<<<blocks(2LL * nt * n, 256, 4096), 256, 0, c->stream>>>(reinterpret_cast<double*>(A.efix),
+                                                                     2 * nt * n);
+    } else {
+      k_zero<<<blocks(3LL * n, 256, 4096), 256, 0, c->stream>>>(A.grad, (size_t)3 * n);
+    }
     k_zero<<<blocks((long long)nt * n, 256, 4096), 256, 0, c->stream>>>(c->eatom, nt * n);
   });
   k_zero_i<<<1, 64, 0, c->stream>>>(c->nbonds, 1);
@@ -735,6 +741,14 @@ void force_call(mdg_ctx* c) {
       if (vir) k_fbzl<true><<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->cfg.fbzl_r0, c->cfg.fbzl_r1, A.grad, ea, vw);
       else k_fbzl<false><<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->cfg.fbzl_r0, c->cfg.fbzl_r1, A.grad, ea,
                                                           nullptr);
+    });
+  }
+  if (A.gfix) {  // the fixed-point sums into grad and the atomically summed energy slots
+    unsigned fixmask = 0;
+    for (int t = 0; t < (int)nt; t++)
+      if (t != c->slot_eam && t != c->slot_fbm && t != c->slot_tb) fixmask |= 1u << t;  // those store directly
+    launch(c, "k_fix_join", [&] {
+      k_fix_join<<<blocks((3LL + nt) * n, 256, 8192), 256, 0, c->stream>>>(A, (int)nt, fixmask, c->eatom);
     });
   }
   k_clear_flags<<<blocks(2LL * c->nsys), 256, 0, c->stream>>>(c->nsys, c->flags, c->sflags, c->pflags, c->eflags);
@@ -975,6 +989,11 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   c->box.sys = sysd;
   r |= dalloc(c, c->sflags, 2 * (size_t)nsys) | dalloc(c, c->pflags, nsys) | dalloc(c, c->dtp, nsys) | dalloc(c, c->eacc, 8 * (size_t)nsys);
   r |= dalloc(c, c->eatom, c->kinds.size() * (size_t)n);
+  A.eatom0 = c->eatom;
+  A.gfix = nullptr;
+  A.efix = nullptr;
+  if (!c->mixed)  // the FP64 form accumulates in fixed point (deterministic)
+    r |= dalloc(c, A.gfix, 6 * (size_t)n) | dalloc(c, A.efix, 2 * c->kinds.size() * (size_t)n);
   r |= dalloc(c, A.x, n) | dalloc(c, A.y, n) | dalloc(c, A.z, n);
   r |= dalloc(c, A.vx, n) | dalloc(c, A.vy, n) | dalloc(c, A.vz, n);
   r |= dalloc(c, A.ax, n) | dalloc(c, A.ay, n) | dalloc(c, A.az, n);

@@ -52,7 +52,7 @@ __device__ __forceinline__ void nsum_back(const Atoms& A, int a, int b, int only
     D3 v = k.rv * (k.df * c / k.r);
     ga = ga + v;
     vw.push(k.rv, v);
-    atomic_add3(grad, k.k, -v);
+    atomic_add3(GradSink{grad, A.gfix}, k.k, -v);
   }
 }
 
@@ -75,6 +75,7 @@ __global__ void __launch_bounds__(256) k_brenner_bonds(Atoms A, const Tables* __
                                                       double* __restrict__ vir) {
   const BrennerParams& P = T->bren;
   Virial<VIR> vw(vir, 1.0);
+  const GradSink gsink{grad, A.gfix};
   int nbt = min(*nbonds, A.bond_cap);
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nbt; t += gridDim.x * blockDim.x) {
     int2 bd = bonds[t];
@@ -211,7 +212,7 @@ __global__ void __launch_bounds__(256) k_brenner_bonds(Atoms A, const Tables* __
             vw.push(ra, u);
             vw.push(k.rv, v);
           }
-          atomic_add3(grad, k.k, gk);
+          atomic_add3(gsink, k.k, gk);
         }
         if (ta == T_C && tb <= T_H) {
           nsum_back(A, a, b, T_H, HN[s][1] * seedD, ga, grad, vw);
@@ -236,13 +237,13 @@ __global__ void __launch_bounds__(256) k_brenner_bonds(Atoms A, const Tables* __
                 D3 v = k.rv * (k.df * (Fsw(x) * Vc) / k.r);
                 ga = ga + v;
                 vw.push(k.rv, v);
-                atomic_add3(grad, k.k, -v);
+                atomic_add3(gsink, k.k, -v);
               }
               double c2 = k.f * dFsw(x) * Vc;
               if (c2 != 0.0) {
                 D3 gk = d3(0, 0, 0);
                 nsum_back(A, k.k, a, -1, c2, gk, grad, vw);
-                atomic_add3(grad, k.k, gk);
+                atomic_add3(gsink, k.k, gk);
               }
             }
           }
@@ -256,9 +257,9 @@ __global__ void __launch_bounds__(256) k_brenner_bonds(Atoms A, const Tables* __
     gi = gi + v;
     gj = gj - v;
     vw.push(rv, v);
-    atomic_add3(grad, i, gi);
-    atomic_add3(grad, j, gj);
-    if (e != 0.0) unsafeAtomicAdd(eatom + i, e);
+    atomic_add3(gsink, i, gi);
+    atomic_add3(gsink, j, gj);
+    energy_add(A, eatom, i, e);
   }
   vw.flush();
 }

@@ -334,34 +334,105 @@ __device__ __forceinline__ void atomic_add3(double* g, int a, D3 v) {
   unsafeAtomicAdd(g + 3 * a + 2, v.z);
 }
 
+// Deterministic accumulation of the FP64 form.  Floating-point atomics add
+// in whatever order the hardware takes them, so FP64 runs would not repeat
+// bit for bit.  The FP64 form instead splits every contribution v exactly
+// into hi * 2^-30 + lo * 2^-60 (hi = trunc(v 2^30); the remainder is exact by
+// Sterbenz's lemma and lo rounds it once at 2^-60) and adds the two integers
+// with integer atomics, whose sums do not depend on the order; the gradient
+// (erg/cm) is read back as hi 2^-30 + lo 2^-60 once per force call: range
+// 2^33, resolution 2^-60 (5e-15 eV/A).  Energies (erg) are scaled by 2^40
+// first (resolution 8e-31 erg).
+constexpr double kFixHi = 1073741824.0;           // 2^30
+constexpr double kFixLo = 1152921504606846976.0;  // 2^60
+constexpr double kFixEnergy = 1099511627776.0;    // 2^40
+__device__ __forceinline__ void fix_split(double v, long long& hi, long long& lo) {
+  const double h = trunc(v * kFixHi);
+  hi = (long long)h;
+  lo = llrint((v - h * (1.0 / kFixHi)) * kFixLo);
+}
+__device__ __forceinline__ double fix_join(long long hi, long long lo) {
+  return double(hi) * (1.0 / kFixHi) + double(lo) * (1.0 / kFixLo);
+}
+__device__ __forceinline__ void fix_atomic(long long* p, double v) {
+  if (v == 0.0) return;
+  long long hi, lo;
+  fix_split(v, hi, lo);
+  if (hi) atomicAdd((unsigned long long*)p, (unsigned long long)hi);
+  if (lo) atomicAdd((unsigned long long*)(p + 1), (unsigned long long)lo);
+}
+// fixed-point gradient: per atom and component the pair (hi, lo)
+__device__ __forceinline__ void fix_add3(long long* g, int a, D3 v) {
+  fix_atomic(g + 6 * a + 0, v.x);
+  fix_atomic(g + 6 * a + 2, v.y);
+  fix_atomic(g + 6 * a + 4, v.z);
+}
+
+// The gradient of kernels without a workgroup cache: FP64 atomics into grad,
+// or the fixed-point pairs when fix is set (FP64 form).
+struct GradSink {
+  double* grad;
+  long long* fix;
+};
+__device__ __forceinline__ void atomic_add3(const GradSink& g, int a, D3 v) {
+  if (g.fix) fix_add3(g.fix, a, v);
+  else atomic_add3(g.grad, a, v);
+}
+
 // Workgroup gradient cache: S slots of LDS keyed by atom index (linear
 // probing from a multiplicative hash).  A scatter finds or claims its atom's
-// slot and adds there with FP64 LDS atomics; after 8 probes it adds into the
-// global gradient instead.  flush() moves every claimed slot into the global
-// gradient with one atomic per component, so the atoms that a workgroup's
-// bonds or pairs share cost one global update per flush instead of one per
-// contribution.  clear() and flush() are called by the whole workgroup with a
-// barrier between them and any add().
+// slot and adds there with LDS atomics (FP64, or the fixed-point pairs when
+// FIX); after 8 probes it adds into the global gradient instead.  flush()
+// moves every claimed slot into the global gradient with one atomic per
+// component, so the atoms that a workgroup's bonds or pairs share cost one
+// global update per flush instead of one per contribution.  clear() and
+// flush() are called by the whole workgroup with a barrier between them and
+// any add().
 #define MDG_LDS __attribute__((address_space(3)))
-template <int S>
+template <int S, bool FIX = false>
 struct GradCache {
   static_assert((S & (S - 1)) == 0 && S >= 64, "S: power of two");
+  using val_t = typename std::conditional<FIX, long long, double>::type;
+  static constexpr int kSlots = S;
+  static constexpr int kPer = FIX ? 6 : 3;  // values per slot
+  static constexpr int kVals = kPer * S;
   MDG_LDS int* key;     // [S], -1 when free
-  MDG_LDS double* val;  // [3 S]
+  MDG_LDS val_t* val;   // [kPer S]
   MDG_LDS int* fill;    // slots claimed since the last clear
   double* grad;         // global gradient, n*3
+  long long* gfix;      // fixed-point global gradient, n*6 (FIX)
 
-  __device__ __forceinline__ GradCache(int* k, double* v, int* f, double* g)
-      : key((MDG_LDS int*)k), val((MDG_LDS double*)v), fill((MDG_LDS int*)f), grad(g) {}
+  __device__ __forceinline__ GradCache(int* k, val_t* v, int* f, double* g, long long* gf)
+      : key((MDG_LDS int*)k), val((MDG_LDS val_t*)v), fill((MDG_LDS int*)f), grad(g), gfix(gf) {}
 
   __device__ __forceinline__ void clear() const {
     for (int q = threadIdx.x; q < S; q += blockDim.x) {
       key[q] = -1;
-      val[3 * q] = 0.0;
-      val[3 * q + 1] = 0.0;
-      val[3 * q + 2] = 0.0;
+#pragma unroll
+      for (int c = 0; c < kPer; c++) val[kPer * q + c] = val_t(0);
     }
     if (threadIdx.x == 0) *fill = 0;
+  }
+  __device__ __forceinline__ void lds_add(int h, double x, double y, double z) const {
+    if constexpr (FIX) {
+      const double v[3] = {x, y, z};
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        if (v[c] == 0.0) continue;
+        long long hi, lo;
+        fix_split(v[c], hi, lo);
+        if (hi) __hip_atomic_fetch_add(&val[6 * h + 2 * c], hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (lo) __hip_atomic_fetch_add(&val[6 * h + 2 * c + 1], lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+    } else {
+      __hip_atomic_fetch_add(&val[3 * h], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(&val[3 * h + 1], y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(&val[3 * h + 2], z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
+  __device__ __forceinline__ void global_add(int a, double x, double y, double z) const {
+    if constexpr (FIX) fix_add3(gfix, a, d3(x, y, z));
+    else atomic_add3(grad, a, d3(x, y, z));
   }
   __device__ __forceinline__ void add(int a, double x, double y, double z) const {
     constexpr int kBits = __builtin_ctz(S);
@@ -380,19 +451,25 @@ struct GradCache {
         }
       }
       if (k == a) {
-        __hip_atomic_fetch_add(&val[3 * h], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(&val[3 * h + 1], y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        __hip_atomic_fetch_add(&val[3 * h + 2], z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        lds_add(h, x, y, z);
         return;
       }
       h = (h + 1) & (S - 1);
     }
-    atomic_add3(grad, a, d3(x, y, z));
+    global_add(a, x, y, z);
   }
   __device__ __forceinline__ void flush() const {
     for (int q = threadIdx.x; q < S; q += blockDim.x) {
       int a = key[q];
-      if (a >= 0) atomic_add3(grad, a, d3(val[3 * q], val[3 * q + 1], val[3 * q + 2]));
+      if (a < 0) continue;
+      if constexpr (FIX) {
+#pragma unroll
+        for (int c = 0; c < 6; c++)
+          if (val[6 * q + c])
+            atomicAdd((unsigned long long*)(gfix + 6 * a + c), (unsigned long long)val[6 * q + c]);
+      } else {
+        atomic_add3(grad, a, d3(val[3 * q], val[3 * q + 1], val[3 * q + 2]));
+      }
     }
   }
   // Called by the whole workgroup after a barrier: empties the cache when it
@@ -409,10 +486,17 @@ struct GradCache {
   }
 };
 
-template <int S>
-__device__ __forceinline__ void atomic_add3(const GradCache<S>& g, int a, D3 v) {
+template <int S, bool FIX>
+__device__ __forceinline__ void atomic_add3(const GradCache<S, FIX>& g, int a, D3 v) {
   g.add(a, v.x, v.y, v.z);
 }
+
+// The cache a kernel of geometry type R uses, in the LDS of a cache of S FP64
+// slots: the FP64 form (R = double) accumulates in fixed point (half the
+// slots, each twice as wide).
+template <class R, int S>
+using CacheOf = typename std::conditional<std::is_same<R, double>::value, GradCache<S / 2, true>,
+                                          GradCache<S, false>>::type;
 
 // Gradient scatter from a kernel working in unit/type R into the FP64 CGS
 // gradient (global, or through a workgroup cache): gs = (erg per energy

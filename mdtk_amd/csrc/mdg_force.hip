@@ -64,8 +64,8 @@ __device__ __forceinline__ double* theta_of(double* dth, const Atoms& A, int own
 
 // Energy of one bond, pair or quadruplet, credited to its owner atom (the
 // per-system totals are summed over atoms afterwards).
-__device__ __forceinline__ void add_energy(double* eatom, int owner, double e) {
-  if (e != 0.0) unsafeAtomicAdd(eatom + owner, e);
+__device__ __forceinline__ void add_energy(const Atoms& A, double* eatom, int owner, double e) {
+  energy_add(A, eatom, owner, e);
 }
 
 // Energy of a kernel unit expressed in erg.
@@ -630,10 +630,11 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
                                                    double* __restrict__ grad, double* __restrict__ eatom,
                                                    unsigned* err, double* __restrict__ dth,
                                                    double* __restrict__ vir) {
-  __shared__ int s_key[kBondCache];
-  __shared__ double s_val[3 * kBondCache];
+  using GC = CacheOf<R, kBondCache>;
+  __shared__ int s_key[GC::kSlots];
+  __shared__ typename GC::val_t s_val[GC::kVals];
   __shared__ int s_fill;
-  GradCache<kBondCache> gc{s_key, s_val, &s_fill, grad};
+  GC gc{s_key, s_val, &s_fill, grad, A.gfix};
   gc.clear();
   __syncthreads();
   const double gs = T->eunit / T->lunit;
@@ -708,7 +709,7 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
           unsafeAtomicAdd(th + 6 + n, bb * f * P.B[n][a][b] * r * en);
         }
       }
-      add_energy(eatom, p.i, eb);
+      add_energy(A, eatom, p.i, eb);
       D3 v = rd * (cr / r);
       atomic_add3(gc, p.i, v);
       atomic_add3(gc, p.j, -v);
@@ -975,10 +976,11 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
   const unsigned long long below = (1ull << lane) - 1ull;
   const unsigned long long mine = (LW == 64) ? ~0ull : (((1ull << LW) - 1ull) << (lane / LW * LW));
   constexpr int CS = (OCC >= 4) ? kPairCache / 2 : kPairCache;  // 4 workgroups per CU must fit the LDS
-  __shared__ int s_key[CS];
-  __shared__ double s_val[3 * CS];
+  using GC = CacheOf<R, CS>;
+  __shared__ int s_key[GC::kSlots];
+  __shared__ typename GC::val_t s_val[GC::kVals];
   __shared__ int s_fill;
-  GradCache<CS> gc{s_key, s_val, &s_fill, grad};
+  GC gc{s_key, s_val, &s_fill, grad, A.gfix};
   gc.clear();
   __syncthreads();
   long long np = 0;
@@ -1243,7 +1245,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
         if constexpr (std::is_same<R, double>::value) atomic_add3(gc, i, d3(gx, gy, gz));
         else atomic_add3(gc, i, d3(gx * gs, gy * gs, gz * gs));
       }
-      add_energy(eatom, i, e);
+      add_energy(A, eatom, i, e);
     }
     if constexpr (TH) {
 #pragma unroll
@@ -1314,10 +1316,11 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
     if (q == 0) s_pre[0] = 0;
     __syncthreads();
   }
-  __shared__ int s_key[kBondCache];
-  __shared__ double s_val[3 * kBondCache];
+  using GC = CacheOf<R, kBondCache>;
+  __shared__ int s_key[GC::kSlots];
+  __shared__ typename GC::val_t s_val[GC::kVals];
   __shared__ int s_fill;
-  GradCache<kBondCache> gc{s_key, s_val, &s_fill, grad};
+  GC gc{s_key, s_val, &s_fill, grad, A.gfix};
   gc.clear();
   __syncthreads();
   const int nq = s_pre[kQueues];
@@ -1419,7 +1422,7 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
     scatter3<R>(gc, j, -v, gs);
     vw.push(rv, v);
     if (pkey && SrSb * LJ != R(0)) cij_backward<R>(pb, SrSb * LJ, gc, gs, vw);
-    add_energy(eatom, i, to_erg<R>(double(SrSb * C * LJ), T->eunit));
+    add_energy(A, eatom, i, to_erg<R>(double(SrSb * C * LJ), T->eunit));
     if constexpr (TH) {
       // S_r depends on r/sigma only: dS_r/dsigma = -(r/sigma) dS_r/dr
       double* th = theta_of(dth, A, i) + 27 + 2 * (ti + tj);
@@ -1448,10 +1451,11 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
                                               const int2* __restrict__ bonds, const int* __restrict__ nbonds,
                                               int compat, double* __restrict__ grad, double* __restrict__ eatom,
                                               long long* __restrict__ nquads, double* __restrict__ vir) {
-  __shared__ int s_key[kTorsCache];
-  __shared__ double s_val[3 * kTorsCache];
+  using GC = CacheOf<R, kTorsCache>;
+  __shared__ int s_key[GC::kSlots];
+  __shared__ typename GC::val_t s_val[GC::kVals];
   __shared__ int s_fill;
-  GradCache<kTorsCache> gc{s_key, s_val, &s_fill, grad};
+  GC gc{s_key, s_val, &s_fill, grad, A.gfix};
   gc.clear();
   __syncthreads();
   long long nq = 0;
@@ -1543,7 +1547,7 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
           eb += double(wki * wij * wjl * vt);
         }
       }
-      add_energy(eatom, i, to_erg<R>(eb, T->eunit));
+      add_energy(A, eatom, i, to_erg<R>(eb, T->eunit));
       scatter3<R>(gc, i, gi, gs);
       scatter3<R>(gc, j, gj, gs);
     }

@@ -532,12 +532,12 @@ __global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* __restrict
                      (0.18175 * 3.1998 * e1 + 0.50986 * 0.94229 * e2 + 0.28022 * 0.4029 * e3 + 0.02817 * 0.20162 * e4);
     double cr = der * f + df * val;
     D3 v = rv * (cr / r);
-    atomic_add3(grad, a, v);
-    atomic_add3(grad, b, -v);
+    atomic_add3(GradSink{grad, A.gfix}, a, v);
+    atomic_add3(GradSink{grad, A.gfix}, b, -v);
     vw.push(rv, v);
     e += val * f;
   }
-  if (e != 0.0) unsafeAtomicAdd(eatom + i, e);
+  energy_add(A, eatom, i, e);
   vw.flush();
 }
 template __global__ void k_fbzl<false>(Atoms, Box, double, double, double*, double*, double*);
@@ -612,10 +612,10 @@ __global__ void k_flj(Atoms A, Box bx, const FLJTable* __restrict__ T, double* _
       der = ld * f + lv * df;
     }
     D3 v = rv * (der / r);
-    atomic_add3(grad, a, v);
-    atomic_add3(grad, b, -v);
+    atomic_add3(GradSink{grad, A.gfix}, a, v);
+    atomic_add3(GradSink{grad, A.gfix}, b, -v);
     vw.push(rv, v);
-    if (val != 0.0) unsafeAtomicAdd(eatom + a, val);
+    energy_add(A, eatom, a, val);
   }
   vw.flush();
 }
@@ -871,7 +871,7 @@ __global__ void __launch_bounds__(256) k_eam_force(Atoms A, Box bx, const typena
     ev += __shfl_xor(ev, o, LW);
   }
   if (sl == 0) {
-    atomic_add3(grad, i, D3{gx, gy, gz});
+    atomic_add3(GradSink{grad, A.gfix}, i, D3{gx, gy, gz});
     eatom[i] += ev;
   }
 }
@@ -945,7 +945,7 @@ __global__ void __launch_bounds__(256) k_fbm(Atoms A, Box bx, const FBMTable* __
     ev += __shfl_xor(ev, o, LW);
   }
   if (sl == 0) {
-    atomic_add3(grad, i, D3{gx, gy, gz});
+    atomic_add3(GradSink{grad, A.gfix}, i, D3{gx, gy, gz});
     eatom[i] = ev;
   }
 }
@@ -1374,6 +1374,21 @@ __global__ void __launch_bounds__(256) k_traj_capture(Atoms A, TrajBuf tb, const
       R.pc[2] = A.pcz[i];
       R.atom = i - A.sys_off[s];
     }
+}
+
+// The end of a force call of the FP64 form: the fixed-point gradient into
+// grad and the fixed-point energy slots (bit t of fixmask) into eatom.
+__global__ void k_fix_join(Atoms A, int nterms, unsigned fixmask, double* __restrict__ eatom) {
+  const size_t n = (size_t)A.n;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < 3 * n + nterms * n;
+       q += (size_t)gridDim.x * blockDim.x) {
+    if (q < 3 * n) {
+      A.grad[q] = fix_join(A.gfix[2 * q], A.gfix[2 * q + 1]);
+    } else {
+      const size_t e = q - 3 * n;
+      if ((fixmask >> (e / n)) & 1u) eatom[e] = fix_join(A.efix[2 * e], A.efix[2 * e + 1]) * (1.0 / kFixEnergy);
+    }
+  }
 }
 
 // The force call's per-system virial into this step's output slot.

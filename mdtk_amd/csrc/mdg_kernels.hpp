@@ -68,7 +68,22 @@ struct Atoms {
   double* tb_dE;              // per atom: energy taken from the bath this step
   double* an_nt;              // n*3 acceleration without the bath (kept with or without a bath)
   int wg_map;                 // 1: the scattering force kernels give each XCD a contiguous eighth of the work
+  // Deterministic accumulation (FP64 form; null in the mixed form): the
+  // gradient and the atomically summed per-atom energies as fixed-point
+  // pairs (hi, lo) -- gfix[6 a + 2 c], efix[2 (t n + a)] for the energy slot
+  // eatom0 + t n -- joined into grad and eatom at the end of each force call.
+  long long* gfix;
+  long long* efix;
+  double* eatom0;
 };
+
+// Adds e (erg) to the per-atom energy slot[owner]: fixed point in the FP64
+// form, an FP64 atomic otherwise.
+__device__ __forceinline__ void energy_add(const Atoms& A, double* slot, int owner, double e) {
+  if (e == 0.0) return;
+  if (A.efix) fix_atomic(A.efix + 2 * ((slot - A.eatom0) + owner), e * kFixEnergy);
+  else unsafeAtomicAdd(slot + owner, e);
+}
 
 struct CellGrid {
   double lo[3], w[3];
@@ -203,6 +218,7 @@ __global__ void k_sys_step_merge(const int* rb_off, const double* part, double* 
                                  double* dtprev, long long* iter);
 __global__ void k_bath_temp_part(Atoms A, Box bx, const int2* rb, double* part);
 __global__ void k_bath_temp_merge(Atoms A, const int* rb_off, const double* part);
+__global__ void k_fix_join(Atoms A, int nterms, unsigned fixmask, double* eatom);
 __global__ void k_step_virial(int nsys, const double* vir, double* stepvir, const int* stepidx);
 __global__ void k_step_advance(int* stepidx);
 
