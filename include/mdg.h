@@ -173,6 +173,33 @@ int mdg_nl_export(mdg_ctx* ctx, int term, int32_t* counts, int32_t* idx, int64_t
 int mdg_step_nve(mdg_ctx* ctx, int nsteps, double fixed_dt, double* epot, double* ekin, double* dt_used,
                  double* netforce);
 
+/* Device errors of one system (a capacity such as 16 bonds or 512 long-list
+ * neighbours of one atom, or a range check such as cos(theta) beyond +-1, the
+ * reference's REQUIREs) stop that system only: inside mdg_step_nve it stands
+ * still from the step after the error (its simTimeFinal is set to its
+ * simTime, as if its run had ended), the other systems run every step, every
+ * output is filled, and the call returns MDG_SYSTEMS_STOPPED with the
+ * systems and errors in mdg_last_error.  Errors of shared resources (the b*
+ * work queue) fail the call.  mdg_system_errors returns each system's error
+ * bits (MDG_ERR_*; 0 = none) seen since the last call and clears them; a
+ * stopped system stays stopped until its bits are cleared and mdg_set_time
+ * gives it a new final time.  No reference counterpart: the reference runs
+ * one trajectory, which a REQUIRE ends (SimLoop.cxx:176-205). */
+enum { MDG_SYSTEMS_STOPPED = 2 };
+enum {
+  MDG_ERR_SHORT_NL_OVERFLOW = 1,  /* an atom's short-list neighbours beyond max_short_nb */
+  MDG_ERR_LONG_NL_OVERFLOW = 2,   /* an atom's long-list neighbours beyond max_long_nb */
+  MDG_ERR_BOND_OVERFLOW = 4,      /* more than 16 bonds (inside the REBO/Brenner f_c range) on one atom */
+  MDG_ERR_COS_RANGE = 8,          /* cos(theta) outside [-1-1e-7, 1+1e-7] (FGeneral.hpp:147) */
+  MDG_ERR_NEG_COORD = 16,         /* coordination below -1e-15 (REBO.cxx:434) */
+  MDG_ERR_QUEUE_OVERFLOW = 32,    /* a shared work queue: fails the call */
+  MDG_ERR_SPLINE_DOMAIN = 64,     /* a spline argument outside its domain */
+  MDG_ERR_EAM_ROW_OVERFLOW = 128, /* more than 128 metal atoms inside the Ackland range of one atom */
+  MDG_ERR_PATH_OVERFLOW = 256,    /* an atom's LJ pairs or C_ij paths beyond their capacity */
+  MDG_ERR_TYPE_RANGE = 512        /* a packed position carries a type outside the tables */
+};
+int mdg_system_errors(mdg_ctx* ctx, uint32_t* out);
+
 /* Thermal bath of one system (SimLoop::thermalBathCommon / thermalBathGeom*,
  * SimLoop.hpp:118-189), applied inside mdg_step_nve's kick as the reference
  * applies it (SimLoop.cxx:280, 352-386): atoms (non-fixed, opted in, inside

@@ -70,6 +70,7 @@ struct mdg_ctx {
   TBTable* dTB = nullptr;    // TightBinding parameters (device)
   FBMTable* dFBM = nullptr;  // FBM parameters (device)
   bool have_short = false, have_long = false;
+  LongMembers lmem{};  // the long-range members for the dt rule's hasNeighbors
   double rc_short = 0, rc_long = 0;
 
   int n = 0;
@@ -128,7 +129,8 @@ struct mdg_ctx {
   int step_cap = 0;         // steps per output chunk
   long long* counters = nullptr;  // [0] short rebuilds [1] long rebuilds (per system and rebuild)
   long long* tally = nullptr;     // kTallySlots spread slots: [0] LJ pairs [1] quads [2] pair rows built
-  unsigned* derr = nullptr;
+  unsigned* derr = nullptr;  // device error words: [0] global resources, [1 + s] system s
+  std::vector<unsigned> failed;  // per system: error bits seen so far (mdg_system_errors)
   long iteration = 0;
 
   Tune tune;
@@ -315,26 +317,51 @@ CellGrid make_grid(const double L[3], const std::vector<double>& xyz, int n, dou
   return g;
 }
 
-int check_err(mdg_ctx* c) {
-  unsigned h = 0;
-  HIPCHK(c, hipMemcpyAsync(&h, c->derr, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+std::string err_text(unsigned h) {
+  std::string m;
+  if (h & ERR_SHORT_NL_OVERFLOW) m += " short-list overflow (raise max_short_nb);";
+  if (h & ERR_LONG_NL_OVERFLOW) m += " long-list overflow (raise max_long_nb);";
+  if (h & ERR_BOND_OVERFLOW) m += " more than 16 bonds on one atom;";
+  if (h & ERR_COS_RANGE) m += " cos(theta) outside [-1-1e-7, 1+1e-7] (FGeneral.hpp:147);";
+  if (h & ERR_NEG_COORD) m += " coordination below -1e-15 (REBO.cxx:434);";
+  if (h & ERR_QUEUE_OVERFLOW) m += " work-queue overflow;";
+  if (h & ERR_SPLINE_DOMAIN) m += " spline argument outside its domain;";
+  if (h & ERR_EAM_ROW_OVERFLOW) m += " more than 128 metal atoms inside the Ackland range of one atom;";
+  if (h & ERR_PATH_OVERFLOW) m += " an atom's LJ pairs or C_ij paths beyond their capacity (128 pairs, 32 two-bond "
+                                  "prefixes, 128 path records, 64 targets);";
+  if (h & ERR_TYPE_RANGE) m += " a packed position carries a type outside the tables;";
+  return m;
+}
+
+// Reads the device error words: w[0] (global resources) and w[1 + s] (system
+// s).  Errors of global resources always fail the call.  Errors of single
+// systems fail it too, unless keep_going: then the systems are recorded in
+// c->failed (their words stay set, so the step loop keeps them still) and the
+// call carries on.  Returns 1 on failure (message in c->err).
+int check_err(mdg_ctx* c, bool keep_going = false) {
+  const int ns = std::max(c->nsys, 0);
+  std::vector<unsigned> h(1 + (size_t)ns, 0u);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->derr, h.size() * sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (h) {
-    std::string m = "device error:";
-    if (h & ERR_SHORT_NL_OVERFLOW) m += " short-list overflow (raise max_short_nb);";
-    if (h & ERR_LONG_NL_OVERFLOW) m += " long-list overflow (raise max_long_nb);";
-    if (h & ERR_BOND_OVERFLOW) m += " bond-list overflow;";
-    if (h & ERR_COS_RANGE) m += " cos(theta) outside [-1-1e-7, 1+1e-7] (FGeneral.hpp:147);";
-    if (h & ERR_NEG_COORD) m += " coordination below -1e-15 (REBO.cxx:434);";
-    if (h & ERR_QUEUE_OVERFLOW) m += " work-queue overflow;";
-    if (h & ERR_SPLINE_DOMAIN) m += " spline argument outside its domain;";
-    if (h & ERR_EAM_ROW_OVERFLOW) m += " more than 128 metal atoms inside the Ackland range of one atom;";
-    unsigned z = 0;
-    hipMemcpy(c->derr, &z, sizeof(unsigned), hipMemcpyHostToDevice);
+  unsigned any = h[0];
+  for (int s = 0; s < ns; s++) any |= h[1 + s];
+  if (!any) return 0;
+  std::string m = "device error:";
+  if (h[0]) m += err_text(h[0]);
+  for (int s = 0; s < ns; s++)
+    if (h[1 + s]) {
+      if (ns > 1) m += " [system " + std::to_string(s) + "]";
+      m += err_text(h[1 + s]);
+      c->failed[s] |= h[1 + s];
+    }
+  if (!h[0] && keep_going) {
     c->err = m;
-    return 1;
+    return 0;
   }
-  return 0;
+  std::vector<unsigned> z(h.size(), 0u);  // a failed call leaves the words clear for the next
+  HIPCHK(c, hipMemcpy(c->derr, z.data(), z.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+  c->err = m;
+  return 1;
 }
 
 inline int blocks(long long work, int bs = 256, int cap = 1 << 20) {
@@ -440,6 +467,9 @@ void nl_build(mdg_ctx* c, bool shortlist) {
       k_nl_build<512><<<blocks(n, 4, 8192), 256, 0, c->stream>>>(flag, c->sflags, 1, n, A.p, c->handled_l,
                                                             c->box, g, coff, range2, cell_of, start, cat,
                                                             A.nll_cnt, A.nll, A.disp_l, c->derr, A.nll_split);
+    });
+    launch(c, "k_long_nbflag", [&] {
+      k_long_nbflag<<<blocks(n, 16), 256, 0, c->stream>>>(flag, c->sflags, A, c->box, c->lmem);
     });
   }
 }
@@ -620,6 +650,12 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
 }
 
 static_assert(sizeof(mdg_frame) == sizeof(TrajFrame), "mdg_frame layout");
+static_assert(MDG_ERR_SHORT_NL_OVERFLOW == ERR_SHORT_NL_OVERFLOW && MDG_ERR_LONG_NL_OVERFLOW == ERR_LONG_NL_OVERFLOW &&
+                  MDG_ERR_BOND_OVERFLOW == ERR_BOND_OVERFLOW && MDG_ERR_COS_RANGE == ERR_COS_RANGE &&
+                  MDG_ERR_NEG_COORD == ERR_NEG_COORD && MDG_ERR_QUEUE_OVERFLOW == ERR_QUEUE_OVERFLOW &&
+                  MDG_ERR_SPLINE_DOMAIN == ERR_SPLINE_DOMAIN && MDG_ERR_EAM_ROW_OVERFLOW == ERR_EAM_ROW_OVERFLOW &&
+                  MDG_ERR_PATH_OVERFLOW == ERR_PATH_OVERFLOW && MDG_ERR_TYPE_RANGE == ERR_TYPE_RANGE,
+              "public error bits");
 static_assert(sizeof(mdg_atom_record) == sizeof(TrajRec), "mdg_atom_record layout");
 constexpr int kTrajChunk = 256;  // steps between drains of the trajectory buffers
 
@@ -840,6 +876,20 @@ int ctx_init(mdg_ctx* c, const mdg_config* cfg) {
   if (m & MDG_POT_FBM) c->rc_long = std::max(c->rc_long, ft2.R1);  // FPairwise::getRcutoff
   if (m & MDG_POT_TB) c->rc_long = std::max(c->rc_long, tt.R1);    // TightBinding.hpp:55
   c->metal_rcand = std::max((m & MDG_POT_ACKLAND) ? et.rmax : 0.0, (m & MDG_POT_TB) ? tt.R1 : 0.0) + c->metal_skin;
+  {  // handled elements (H, C, Ar, Xe, Cu, Ag, Au bits) and list radius of each long-range member
+    auto add = [&](unsigned h, double rc) {
+      c->lmem.handled[c->lmem.m] = h;
+      c->lmem.r2[c->lmem.m] = (1.44 * rc) * (1.44 * rc);
+      c->lmem.m++;
+    };
+    if (m & MDG_POT_FBZL) add(0x7Fu, c->cfg.fbzl_r1);  // FBZL.cxx:36-58
+    if (m & MDG_POT_AIREBO) add(0x03u, 5.5 * kAo);     // AIREBO_LJ: H, C
+    if (m & MDG_POT_FLJ) add(0x53u, 5.5 * kAo);        // FLJ.cxx:38-52: H, C, Cu, Au
+    if (m & MDG_POT_ACKLAND)                           // Ackland.cxx:236-262: Cu, Ag, Au
+      add(0x70u, std::max(et.rk[0][0][1], std::max(et.rk[1][1][1], et.rk[2][2][1])));
+    if (m & MDG_POT_FBM) add(0x14u, ft2.R1);           // FBM.cxx:37-38: Cu, Ar
+    if (m & MDG_POT_TB) add(0x10u, tt.R1);             // TightBinding.cxx:163: Cu
+  }
   if (!c->arena) {
     if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess) {
       c->err = "hipStreamCreate failed";
@@ -972,6 +1022,12 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   c->n = n;
   c->nsys = nsys;
   c->off_h = off;
+  c->failed.assign(nsys, 0u);
+  {  // error words: [0] global resources, [1 + s] system s
+    dfree(c, c->derr);
+    if (dalloc(c, c->derr, 1 + (size_t)nsys)) return 1;
+    HIPCHK(c, hipMemset(c->derr, 0, (1 + (size_t)nsys) * sizeof(unsigned)));
+  }
   c->Lh.assign(pbc, pbc + 3 * (size_t)nsys);
   c->dt_host.assign(nsys, 1e-20);  // SimLoop.cxx:50
   Atoms& A = c->A;
@@ -1014,7 +1070,8 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   r |= dalloc(c, A.disp_s, 3 * (size_t)n) | dalloc(c, A.disp_l, 3 * (size_t)n);
   r |= dalloc(c, A.nls_cnt, n) | dalloc(c, A.nll_cnt, n);
   if (c->have_short) r |= dalloc(c, A.nls, (size_t)n * A.cap_s);
-  if (c->have_long) r |= dalloc(c, A.nll, (size_t)n * A.cap_l) | dalloc(c, A.nll_split, n);
+  A.nbl = nullptr;
+  if (c->have_long) r |= dalloc(c, A.nll, (size_t)n * A.cap_l) | dalloc(c, A.nll_split, n) | dalloc(c, A.nbl, n);
   r |= dalloc(c, A.bcnt, n) | dalloc(c, A.bj, (size_t)n * A.maxb);
   r |= dalloc(c, A.p, n);
   if (!c->mixed) r |= dalloc(c, A.b64, (size_t)n * A.maxb);
@@ -1148,6 +1205,7 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   HIPCHK(c, hipMemset(A.nls_cnt, 0, (size_t)n * 4));
   HIPCHK(c, hipMemset(A.nll_cnt, 0, (size_t)n * 4));
   if (A.nll_split) HIPCHK(c, hipMemset(A.nll_split, 0, (size_t)n * 4));
+  if (A.nbl) HIPCHK(c, hipMemset(A.nbl, 0, (size_t)n));
   HIPCHK(c, hipMemset(A.bcnt, 0, (size_t)n * 4));
   HIPCHK(c, up(c->handled_s, hs.data(), n));
   HIPCHK(c, up(c->handled_l, hl.data(), n));
@@ -1432,6 +1490,18 @@ int mdg_energy_grad_noalloc(const mdg_config* cfg, int n, const int32_t* elem, c
   return rc;
 }
 
+int mdg_system_errors(mdg_ctx* c, uint32_t* out) {
+  if (c->n == 0) {
+    c->err = "no system";
+    return 1;
+  }
+  if (check_err(c, true)) return 1;  // errors of global resources fail here
+  for (int s = 0; s < c->nsys; s++) out[s] = c->failed[s];
+  std::fill(c->failed.begin(), c->failed.end(), 0u);
+  HIPCHK(c, hipMemset(c->derr, 0, (1 + (size_t)c->nsys) * sizeof(unsigned)));
+  return 0;
+}
+
 int mdg_set_virial(mdg_ctx* c, int on) {
   if (c->n == 0) {
     c->err = "mdg_set_virial: set the system(s) first";
@@ -1578,6 +1648,7 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
   if (nsteps <= 0) return 0;
   Atoms& A = c->A;
   int n = c->n, nsys = c->nsys;
+  const std::vector<unsigned> failed0 = c->failed;
   // per-step, per-system outputs are staged in chunks of at most 2^20 entries
   // with trajectory output, at most kTrajChunk steps between drains of its buffers
   const int chunk = std::max(1, std::min(c->traj_on ? kTrajChunk : 4096, (1 << 20) / nsys));
@@ -1630,7 +1701,7 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
     launch(c, "k_sys_step_end", [&] {
       k_sys_step_merge<<<nsys, 64, 0, c->stream>>>(c->rb_off, c->red_part, c->dtp, fixed, c->stepout, c->stepidx,
                                                    nsys, A.tb_dE ? c->ebath : nullptr, c->simt, c->tfin, c->dtprev,
-                                                   c->iter);
+                                                   c->iter, c->derr + 1);
     });
     if (c->virial) k_step_virial<<<blocks(6LL * nsys), 256, 0, c->stream>>>(nsys, c->vir, c->stepvir, c->stepidx);
     k_step_advance<<<1, 64, 0, c->stream>>>(c->stepidx);
@@ -1653,7 +1724,7 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
       if (use_graph) HIPCHK(c, hipGraphLaunch(c->graph_exec[fixed], c->stream));
       else body();
     }
-    if (check_err(c)) return 1;
+    if (check_err(c, true)) return 1;  // systems with their own errors stand still, the rest go on
     if (c->traj_on && drain_trajectory(c)) return 1;
     HIPCHK(c, hipMemcpy(o.data(), c->stepout, (size_t)m * nsys * 4 * sizeof(double), hipMemcpyDeviceToHost));
     if (c->virial) {
@@ -1673,6 +1744,11 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
       }
   }
   c->iteration += nsteps;
+  for (int s = 0; s < nsys; s++)
+    if (c->failed[s] & ~failed0[s]) {  // c->err names the systems and errors
+      c->err = "some systems stopped on device errors (the others ran every step):" + c->err.substr(13);
+      return MDG_SYSTEMS_STOPPED;
+    }
   return 0;
 }
 

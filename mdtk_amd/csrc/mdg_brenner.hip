@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(256) k_brenner_bonds(Atoms A, const Tables* __
           double G;
           if (ta == T_C) {
             double c = dot(ra, k.rv) / (r * k.r);
-            if (!(c >= -1.0 - 1e-7 && c <= 1.0 + 1e-7)) atomicOr(err, ERR_COS_RANGE);
+            if (!(c >= -1.0 - 1e-7 && c <= 1.0 + 1e-7)) raise(sys_err(err, A.sys[i]), ERR_COS_RANGE);
             if (c < -1.0) c = -1.0;
             if (c > 1.0) c = 1.0;
             G = P.ao * (1.0 + P.co * P.co / (P.do_ * P.do_) - P.co * P.co / (P.do_ * P.do_ + (1.0 + c) * (1.0 + c)));

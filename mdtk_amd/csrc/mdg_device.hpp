@@ -37,7 +37,17 @@ enum : unsigned {
   ERR_QUEUE_OVERFLOW = 32u,
   ERR_SPLINE_DOMAIN = 64u,
   ERR_EAM_ROW_OVERFLOW = 128u,
+  ERR_PATH_OVERFLOW = 256u,   // an atom's LJ pairs or C_ij paths beyond their per-atom capacity
+  ERR_TYPE_RANGE = 512u,      // a packed position carries a type outside the tables
 };
+// Errors that a global resource (the b* queue, the bond array) raises; every
+// other bit belongs to one system.
+constexpr unsigned kGlobalErrors = ERR_QUEUE_OVERFLOW;
+
+// Device error words: w[0] collects errors of global resources, w[1 + s]
+// those of system s (checked by the host after each call).
+__device__ __forceinline__ unsigned* sys_err(unsigned* w, int s) { return w + 1 + s; }
+__device__ __forceinline__ void raise(unsigned* e, unsigned bit) { atomicOr(e, bit); }
 
 // Quintic Hermite pieces of G(cos theta), AIREBO/cubici.hpp:39-80.  Each piece
 // is stored as local monomials p(t) = sum c_n t^n on [xa, xb], t = (x-xa)/h.

@@ -12,7 +12,6 @@
 namespace mdg {
 
 namespace {
-__device__ __forceinline__ void raise(unsigned* err, unsigned bit) { atomicOr(err, bit); }
 
 __device__ __forceinline__ double wave_sum(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -138,6 +137,7 @@ __global__ void __launch_bounds__(256) k_coord(Atoms A, const TablesT<R>* __rest
   if (i >= A.n) return;  // group-uniform
   BondRec<R>* B = brec<R>(A);
   const int ti = A.type[i];
+  unsigned* const er = sys_err(err, A.sys[i]);
   double Nt = 0, NH = 0, NC = 0;
   int nb = 0;
   if (ti <= T_H) {
@@ -155,6 +155,7 @@ __global__ void __launch_bounds__(256) k_coord(Atoms A, const TablesT<R>* __rest
         k = L[q];
         const double4 pk = A.p[k];
         tk = (int)pk.w;  // the packed position carries the type
+        if ((unsigned)tk > (unsigned)T_OTHER) raise(er, ERR_TYPE_RANGE);
         rv = to_local<R>(d3(mic(pi.x - pk.x, Lb[0]), mic(pi.y - pk.y, Lb[1]), mic(pi.z - pk.z, Lb[2])));
         r = sqrt(rv.x * rv.x + rv.y * rv.y + rv.z * rv.z);
         if ((unsigned)tk <= (unsigned)T_H && !(r > T->rebo.R[ti][tk][1])) {
@@ -183,7 +184,7 @@ __global__ void __launch_bounds__(256) k_coord(Atoms A, const TablesT<R>* __rest
     }
   }
   if (sl == 0) {
-    if (nb > A.maxb) raise(err, ERR_BOND_OVERFLOW);
+    if (nb > A.maxb) raise(er, ERR_BOND_OVERFLOW);
     A.Nt[i] = Nt;
     A.NH[i] = NH;
     A.NC[i] = NC;
@@ -663,6 +664,7 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
     if constexpr (VIR) {
       if (t < b1) vw.at(A.sys[p.i]);
     }
+    unsigned* const er = sys_err(err, A.sys[p.i]);
     {  // f_ij inside the bond order in R, as the coordination pass summed it
       R fr, dfr;
       cutoff<R>(p.r, T->rebo.R[a][b][0], T->rebo.R[a][b][1], fr, dfr);
@@ -686,10 +688,10 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
       double bb = 0.0;
       if (VA != 0.0 || TH) {
         BOState st;
-        bo_forward<R>(A, T, p, st, err);
+        bo_forward<R>(A, T, p, st, er);
         bb = st.b;
         if (VA != 0.0) {
-          bo_backward<R>(A, T, p, st, VA * su, gc, gs, err, vw);
+          bo_backward<R>(A, T, p, st, VA * su, gc, gs, er, vw);
           double vder = (-P.beta[0][a][b]) * u1 + (-P.beta[1][a][b]) * u2 + (-P.beta[2][a][b]) * u3;
           cr += (-vder * f * bb) + (-vsum * bb) * df;
           eb += bb * VA;
@@ -990,6 +992,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     int ti = A.type[i];
     if (ti > T_H) return;
     if constexpr (VIR) vw.at(A.sys[i]);
+    unsigned* const er = sys_err(err, A.sys[i]);
     const double* Lb = box_of(bx, i);
     double4 pi4 = A.p[i];
     double xi = pi4.x, yi = pi4.y, zi = pi4.z;
@@ -1036,7 +1039,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       }
     }
     if (npair > kPairCap) {
-      if (sl == 0) raise(err, ERR_QUEUE_OVERFLOW);
+      if (sl == 0) raise(er, ERR_PATH_OVERFLOW);
       npair = kPairCap;
     }
     if (npair == 0) return;
@@ -1060,7 +1063,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     __builtin_amdgcn_wave_barrier();
     int tot2 = s_off[gid][nb1];
     if (tot2 > kL2) {
-      if (sl == 0) raise(err, ERR_QUEUE_OVERFLOW);
+      if (sl == 0) raise(er, ERR_PATH_OVERFLOW);
       tot2 = kL2;
     }
     for (int c = sl; c < tot2; c += LW) {  // 2-bond prefixes (s1, s2)
@@ -1087,7 +1090,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     __builtin_amdgcn_wave_barrier();
     int nrec = nb1 + tot2 + s_o3[gid][tot2];
     if (!PACK && nrec > kPathCap) {
-      if (sl == 0) raise(err, ERR_QUEUE_OVERFLOW);
+      if (sl == 0) raise(er, ERR_PATH_OVERFLOW);
       nrec = kPathCap;
     }
     for (int r = sl; r < nrec; r += LW) {
@@ -1121,7 +1124,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
         h = hslot(target);
         for (int probes = 0;; probes++) {
           if (probes == kHash) {
-            raise(err, ERR_QUEUE_OVERFLOW);
+            raise(er, ERR_PATH_OVERFLOW);
             h = -1;
             break;
           }
@@ -1158,6 +1161,10 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     // pairs are issued together below
     auto pair = [&](const int j, const double4 pj) {
       const int tj = (int)pj.w;  // phase 0 kept C/H partners only
+      if ((unsigned)tj > (unsigned)T_H) {  // a stale packed copy: never index the tables with it
+        raise(er, ERR_TYPE_RANGE);
+        return;
+      }
       D3 rd = d3(mic(xi - pj.x, Lb[0]), mic(yi - pj.y, Lb[1]), mic(zi - pj.z, Lb[2]));
       R w = R(0);
       int bkey = 0x7fffffff;
@@ -1348,6 +1355,7 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
     int i = qd.x, j = qd.y;
     int pkey = qd.z;  // C_ij path (0: none), bond slots as in k_lj_wave
     int ti = A.type[i], tj = A.type[j];
+    unsigned* const er = sys_err(err, A.sys[i]);
     V3<R> rv = to_local<R>(depos(A.p, i, j, bx));
     R r = sqrt(norm2(rv));
     PathBonds<R> pb;
@@ -1371,7 +1379,7 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
     p.reff = P.R[ti][tj][0];
     BOState st;
     R bmin = L.b[ti][tj][0], bmax = L.b[ti][tj][1];
-    bo_sigma_pi<R>(A, T, p, st, err);
+    bo_sigma_pi<R>(A, T, p, st, er);
     bool below = false;
     if constexpr (PASS != 2) {
       const double bub = 0.5 * (1.0 / sqrt(st.D[0]) + 1.0 / sqrt(st.D[1])) + T->rcmax[ti][tj] + 1e-12;
@@ -1384,7 +1392,7 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
         done = true;
       }
     } else {
-      if (!below) bo_finish<R>(A, T, p, st, err);
+      if (!below) bo_finish<R>(A, T, p, st, er);
     }
     if (!done) {
     if constexpr (VIR) vw.at(A.sys[i]);
@@ -1414,7 +1422,7 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
       if (dStr != R(0) && Stb != R(0)) cr += dStr * Stb * V2;
       if constexpr (PASS != 1) {
         R dStb = below ? R(0) : dSfun<R>(bs, bmin, bmax);
-        if (dStb != R(0)) bo_backward<R>(A, T, p, st, double(Str * dStb * V2), gc, gs, err, vw);
+        if (dStb != R(0)) bo_backward<R>(A, T, p, st, double(Str * dStb * V2), gc, gs, er, vw);
       }
     }
     V3<R> v = rv * (cr / r);

@@ -8,7 +8,6 @@
 namespace mdg {
 
 // ----------------------------------------------------------------- helpers
-__device__ __forceinline__ void raise(unsigned* err, unsigned bit) { atomicOr(err, bit); }
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
@@ -230,7 +229,7 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
       }
     }
     if (m > CAP) {
-      if (lane == 0) raise(err, CAP >= 256 ? ERR_LONG_NL_OVERFLOW : ERR_SHORT_NL_OVERFLOW);
+      if (lane == 0) raise(sys_err(err, sy), CAP >= 256 ? ERR_LONG_NL_OVERFLOW : ERR_SHORT_NL_OVERFLOW);
       m = CAP;
     }
     // pad to a power of two and sort ascending
@@ -818,7 +817,7 @@ __global__ void __launch_bounds__(256) k_eam_density(Atoms A, Box bx, const type
     M::embed(T, rho, F, Fp);
     fe[i] = make_double2(Fp, double(ei));
     eatom[i] = F;
-    if (m > kEamRow) raise(err, ERR_EAM_ROW_OVERFLOW);
+    if (m > kEamRow) raise(sys_err(err, A.sys[i]), ERR_EAM_ROW_OVERFLOW);
     rcnt[i] = m < kEamRow ? m : kEamRow;
   }
 }
@@ -1206,7 +1205,10 @@ __global__ void __launch_bounds__(256) k_kick_tile(Atoms A, Box bx, const int2* 
     fx -= gx;
     fy -= gy;
     fz -= gz;
-    if (!A.fixed[i] && (A.nls_cnt[i] > 0 || A.nll_cnt[i] > 0) && vv > vm) vm = vv;
+    // FProxy::hasNeighbors (SimLoop.cxx:394-398): the short list's members
+    // handle H and C only, the long-range members by their own element sets
+    const bool nb = A.nls_cnt[i] > 0 || (A.nbl ? A.nbl[i] != 0 : A.nll_cnt[i] > 0);
+    if (!A.fixed[i] && nb && vv > vm) vm = vv;
     eb += dE;  // energy taken from the bath this step (SimLoop.cxx:384)
   }
   ep = block_sum(ep);
@@ -1270,8 +1272,8 @@ __global__ void __launch_bounds__(64) k_sys_step_merge(const int* __restrict__ r
                                                        double* __restrict__ dtp, int fixed_dt, double* __restrict__ out,
                                                        const int* __restrict__ stepidx, int nsys,
                                                        double* __restrict__ ebath, double* __restrict__ simt,
-                                                       const double* __restrict__ tfin, double* __restrict__ dtprev,
-                                                       long long* __restrict__ iter) {
+                                                       double* __restrict__ tfin, double* __restrict__ dtprev,
+                                                       long long* __restrict__ iter, const unsigned* __restrict__ serr) {
   const int s = blockIdx.x;
   double ep = 0, ek = 0, fx = 0, fy = 0, fz = 0, vm = 0, eb = 0;
   for (int q = rb_off[s] + threadIdx.x; q < rb_off[s + 1]; q += 64) {
@@ -1308,6 +1310,9 @@ __global__ void __launch_bounds__(64) k_sys_step_merge(const int* __restrict__ r
         dtp[s] = dt;
       }
     }
+    // a device error in this system's terms (a capacity or a range check)
+    // stops it here: it stands still from the next step on, the others go on
+    if (serr[s]) tfin[s] = simt[s];
   }
 }
 
@@ -1374,6 +1379,46 @@ __global__ void __launch_bounds__(256) k_traj_capture(Atoms A, TrajBuf tb, const
       R.pc[2] = A.pcz[i];
       R.atom = i - A.sys_off[s];
     }
+}
+
+// hasNeighbors for the long-range members after a long-list build: 16 lanes
+// per atom scan its row (positions as the build saw them) for a partner that
+// some member handling both elements lists.
+__global__ void __launch_bounds__(256) k_long_nbflag(const int* __restrict__ flag, const int* __restrict__ sflags,
+                                                     Atoms A, Box bx, LongMembers lm) {
+  if (!*flag) return;  // the long list's rebuild flag
+  constexpr int LW = 16;
+  const int sl = threadIdx.x % LW;
+  const int i = blockIdx.x * (blockDim.x / LW) + threadIdx.x / LW;
+  if (i >= A.n) return;  // group-uniform
+  if (!sflags[2 * A.sys[i] + 1]) return;
+  const unsigned ei = elem_bit(A.elem[i]);
+  unsigned mine = 0;  // members that handle atom i
+  for (int q = 0; q < lm.m; q++)
+    if (lm.handled[q] & ei) mine |= 1u << q;
+  bool found = false;
+  if (mine) {
+    const int cnt = A.nll_cnt[i];
+    const int* L = A.nll + (size_t)i * A.cap_l;
+    const double4 pi = A.p[i];
+    const double* Lb = box_of(bx, i);
+    for (int b0 = 0; b0 < cnt && !found; b0 += LW) {
+      const int q = b0 + sl;
+      bool hit = false;
+      if (q < cnt) {
+        const int j = L[q];
+        const unsigned ej = elem_bit(A.elem[j]);
+        const double4 pj = A.p[j];
+        const double dx = mic(pi.x - pj.x, Lb[0]), dy = mic(pi.y - pj.y, Lb[1]), dz = mic(pi.z - pj.z, Lb[2]);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        for (int m = 0; m < lm.m; m++)
+          if (((mine >> m) & 1u) && (lm.handled[m] & ej) && d2 < lm.r2[m]) hit = true;
+      }
+      const unsigned long long grp = ((1ull << LW) - 1ull) << ((threadIdx.x & 63) / LW * LW);
+      found = (__ballot(hit) & grp) != 0ull;
+    }
+  }
+  if (sl == 0) A.nbl[i] = found ? 1 : 0;
 }
 
 // The end of a force call of the FP64 form: the fixed-point gradient into

@@ -75,7 +75,35 @@ struct Atoms {
   long long* gfix;
   long long* efix;
   double* eatom0;
+  // Per atom: 1 when some long-range chain member lists a neighbour of it
+  // (FProxy::hasNeighbors, FProxy.hpp:99-104, for those members), refreshed
+  // at each long-list build; null without long-range members.
+  unsigned char* nbl;
 };
+
+// The long-range chain members as FProxy::hasNeighbors sees them: member q
+// lists atom i's partner j iff both elements are in handled[q] (bit per
+// element index: H, C, Ar, Xe, Cu, Ag, Au) and d^2 < r2[q] = ((1 + 0.44)
+// Rc_q)^2 (NeighbourList.cxx:83-102).
+struct LongMembers {
+  int m;
+  unsigned handled[6];
+  double r2[6];
+};
+__device__ __forceinline__ unsigned elem_bit(int e) {
+  switch (e) {
+    case 1: return 1u;
+    case 12: return 2u;
+    case 40: return 4u;
+    case 131: return 8u;
+    case 64: return 16u;
+    case 108: return 32u;
+    case 197: return 64u;
+    default: return 0u;
+  }
+}
+// flag: the long list's rebuild flag; sflags[2 s + 1]: system s rebuilt it.
+__global__ void k_long_nbflag(const int* flag, const int* sflags, Atoms A, Box bx, LongMembers lm);
 
 // Adds e (erg) to the per-atom energy slot[owner]: fixed point in the FP64
 // form, an FP64 atomic otherwise.
@@ -213,9 +241,11 @@ __global__ void k_kick_tile(Atoms A, Box bx, const int2* rb, const double* dtp, 
 __global__ void k_init_acc(Atoms A);
 __global__ void k_pack_pos(Atoms A);
 __global__ void k_sys_energy(int nterms, const int* sys_off, int n, const double* eatom, double* eacc);
+// serr[s]: system s's device error word; a system with an error stops (its
+// final time is set to its current time).
 __global__ void k_sys_step_merge(const int* rb_off, const double* part, double* dtp, int fixed_dt, double* out,
-                                 const int* stepidx, int nsys, double* ebath, double* simt, const double* tfin,
-                                 double* dtprev, long long* iter);
+                                 const int* stepidx, int nsys, double* ebath, double* simt, double* tfin,
+                                 double* dtprev, long long* iter, const unsigned* serr);
 __global__ void k_bath_temp_part(Atoms A, Box bx, const int2* rb, double* part);
 __global__ void k_bath_temp_merge(Atoms A, const int* rb_off, const double* part);
 __global__ void k_fix_join(Atoms A, int nterms, unsigned fixmask, double* eatom);

@@ -24,6 +24,10 @@ PARAM_NAMES = [f"{n}_{p}" for p in ("CC", "CH", "HH")
                for n in ("Q", "A", "alpha", "B1", "B2", "B3", "beta1", "beta2", "beta3")] + \
               [f"{n}_{p}" for p in ("CC", "CH", "HH") for n in ("eps", "sigma")]
 PREC_FP64, PREC_MIXED = 0, 1
+SYSTEMS_STOPPED = 2  # mdg_step_nve: some systems stopped on their own device errors, the rest ran
+ERR_BITS = {1: "short-list overflow", 2: "long-list overflow", 4: "bond overflow", 8: "cos range",
+            16: "negative coordination", 32: "queue overflow", 64: "spline domain", 128: "Ackland row overflow",
+            256: "path overflow", 512: "type range"}  # MDG_ERR_*
 _NAMES = {"FBZL": POT_FBZL, "FLJ": POT_FLJ, "Brenner": POT_BRENNER, "REBO": POT_REBO, "AIREBO": POT_AIREBO,
           "ETors": POT_ETORS, "Ackland": POT_ACKLAND, "FBM": POT_FBM, "TightBinding": POT_TB}
 
@@ -74,6 +78,7 @@ def load_library(path: str = LIB_PATH):
     L.mdg_event_record.argtypes = [vp, ctypes.c_int]
     L.mdg_event_elapsed.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
     L.mdg_synchronize.argtypes = [vp]
+    L.mdg_system_errors.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32)]
     L.mdg_set_virial.argtypes = [vp, ctypes.c_int]
     L.mdg_virial.argtypes = [vp, dp]
     L.mdg_step_virial.argtypes = [vp, dp]
@@ -324,12 +329,20 @@ class Engine:
         off = np.concatenate([[0], np.cumsum(counts)])
         return [idx[off[i]:off[i + 1]].copy() for i in range(self.n)]
 
-    def run(self, nsteps: int, fixed_dt: float = 0.0):
+    def run(self, nsteps: int, fixed_dt: float = 0.0, allow_stopped: bool = False):
         """nsteps Verlet iterations; per-step arrays of shape (nsteps,), or
-        (nsteps, nsys) for a batch."""
+        (nsteps, nsys) for a batch.  A system whose own terms raise a device
+        error stands still from then on while the others run; with
+        allow_stopped the outputs come back with out["stopped"] (the error
+        bits per system, 0 = ran every step), otherwise EngineError."""
         m = max(self.nsys, 1)
         ep, ek, dt, nf = (np.zeros(nsteps * m) for _ in range(4))
-        self._chk(_lib.mdg_step_nve(self._h, int(nsteps), float(fixed_dt), _p(ep), _p(ek), _p(dt), _p(nf)))
+        rc = _lib.mdg_step_nve(self._h, int(nsteps), float(fixed_dt), _p(ep), _p(ek), _p(dt), _p(nf))
+        stopped = None
+        if rc == SYSTEMS_STOPPED and allow_stopped:
+            stopped = self.system_errors()
+        else:
+            self._chk(rc)
         out = dict(epot=ep, ekin=ek, dt=dt, netforce=nf)
         if m > 1:
             out = {k: v.reshape(nsteps, m) for k, v in out.items()}
@@ -337,6 +350,15 @@ class Engine:
             w = np.zeros(nsteps * m * 6)
             self._chk(_lib.mdg_step_virial(self._h, _p(w)))
             out["virial"] = w.reshape(nsteps, 6) if m == 1 else w.reshape(nsteps, m, 6)
+        if stopped is not None:
+            out["stopped"] = stopped
+        return out
+
+    def system_errors(self):
+        """Device error bits of each system since the last query (0 = none),
+        cleared by the query."""
+        out = np.zeros(max(self.nsys, 1), np.uint32)
+        self._chk(_lib.mdg_system_errors(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
         return out
 
     def state(self):

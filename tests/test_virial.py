@@ -173,3 +173,21 @@ def test_gpu_virial_batch_and_steps():
     assert r["virial"].shape == (3, 3, 6)
     b.energy_grad()  # at the positions after the third step
     assert np.abs(r["virial"][-1] - b.virial()).max() <= 1e-12 * np.abs(b.virial()).max()
+
+
+@pytest.mark.gpu
+def test_gpu_virial_fresh_engines_first_call():
+    """The first force call of each new context (lists built, cold caches)
+    gives the oracle's virial, and the next call repeats its gradient bit for
+    bit (the FP64 form is deterministic)."""
+    from mdtk_amd import Engine
+    s, _, _ = _case("pe")
+    wo = O.Oracle.from_system(s, pots=O.POT_REBO, etors_compat=False).virial()
+    for _ in range(6):
+        e = Engine.from_system(s, potentials=("REBO",), etors_compat=False)
+        e.set_virial(True)
+        _, g1 = e.energy_grad()
+        w1 = e.virial()
+        _, g2 = e.energy_grad()
+        assert np.abs(w1 - wo).max() <= 1e-12 * np.abs(wo).max()
+        assert np.array_equal(g1, g2)
