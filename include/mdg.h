@@ -73,7 +73,7 @@ int mdg_set_system(mdg_ctx* ctx, int n, const int32_t* elem, const double* xyz, 
 /* A batch of nsys independent systems in one context, advanced together by
  * every call (one launch per kernel for the whole batch): the reference runs
  * one SimLoop per trajectory (e.g. one per bombardment in
- * applications/mdtk-bombard, SimLoop.cxx:252-445); this replaces nsys of
+ * applications/mdtrajsim/main.cpp:280-348, SimLoop.cxx:252-445); this replaces nsys of
  * them.  counts[nsys] atoms per system; elem/xyz/vel/fixed hold the systems'
  * atoms one after another (sum(counts) entries, as mdg_set_system); pbc[3*nsys]
  * each system's box.  Systems never interact; each keeps its own neighbour
@@ -280,6 +280,24 @@ int mdg_take_trajectory(mdg_ctx* ctx, mdg_frame* frames, mdg_atom_record* record
 
 /* Current state: replaces reading Atom::coords/V/an/PBC_count. */
 int mdg_get_state(mdg_ctx* ctx, double* xyz, double* vel, double* acc, int32_t* pbc_count);
+
+/* Accelerations acc[n*3] (cm/s^2) of loaded states, Atom::an as
+ * SimLoopSaver::load reads it from <id>.a (SimLoopSaver.cxx:232-625), for the
+ * systems with keep[s] = 1 (keep null: every system): the first
+ * mdg_step_nve starts those systems from them instead of from the forces at
+ * their positions, as SimLoop::execute does past iteration 0
+ * (SimLoop.cxx:305-321 runs only at iteration 0), so that a run resumed from
+ * a flushed state continues the interrupted one; the other systems start at
+ * iteration 0 as usual.  Fixed atoms keep 0.  Only before the first
+ * mdg_step_nve after mdg_set_systems. */
+int mdg_set_accelerations(mdg_ctx* ctx, const double* acc, const uint8_t* keep);
+
+/* SimLoop::iteration of one system or all (-1), as SimLoopSaver::loadIteration
+ * sets it from the state id md%010d (SimLoopSaver.cxx:850-854); it counts
+ * each system's own steps and names its XVA frames.  0 after
+ * mdg_set_systems.  mdg_get_iteration returns nsys values. */
+int mdg_set_iteration(mdg_ctx* ctx, int system, int64_t iteration);
+int mdg_get_iteration(mdg_ctx* ctx, int64_t* out);
 
 /* Runs the context's kernels on the given HIP stream (hipStream_t passed as
  * void*; null restores the context's own stream).  The stream must come from

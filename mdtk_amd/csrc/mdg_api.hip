@@ -135,6 +135,7 @@ struct mdg_ctx {
 
   Tune tune;
   bool profiling = false;
+  unsigned char* acc_keep = nullptr;  // per system: 1 = keep the accelerations set (a resumed state)
   std::map<std::string, KStat> kst;
   hipEvent_t timer_ev[8] = {};
   std::vector<hipEvent_t> event_pool;
@@ -1232,6 +1233,7 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
     HIPCHK(c, up(c->iter, it0.data(), (size_t)nsys * 8));
   }
   c->iteration = 0;
+  c->acc_keep = nullptr;
   return 0;
 }
 
@@ -1665,9 +1667,9 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
   int fixed = fixed_dt > 0 ? 1 : 0;
   if (fixed) k_fill<<<blocks(nsys), 256, 0, c->stream>>>(c->dtp, nsys, fixed_dt);
   int nterms = (int)c->kinds.size();
-  if (c->iteration == 0) {  // SimLoop.cxx:305-321
+  if (c->iteration == 0) {  // SimLoop.cxx:305-321, for the systems at their iteration 0
     force_call(c);
-    launch(c, "k_init_acc", [&] { k_init_acc<<<blocks(n), 256, 0, c->stream>>>(A); });
+    launch(c, "k_init_acc", [&] { k_init_acc<<<blocks(n), 256, 0, c->stream>>>(A, c->acc_keep); });
   }
   double lim_s = 0.44 * c->rc_short, lim_l = 0.44 * c->rc_long;
   if (!c->have_short) lim_s = 1e300;
@@ -1778,6 +1780,69 @@ int mdg_get_state(mdg_ctx* c, double* xyz, double* vel, double* acc, int32_t* pb
       for (int i = 0; i < n; i++) pbc_count[3 * i + dd] = q[i];
     }
   }
+  return 0;
+}
+
+int mdg_set_accelerations(mdg_ctx* c, const double* acc, const uint8_t* keep) {
+  int n = c->n;
+  if (n == 0) {
+    c->err = "no system";
+    return 1;
+  }
+  if (c->iteration != 0) {
+    c->err = "mdg_set_accelerations: only before the first mdg_step_nve of the systems";
+    return 1;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<double> a(n), b(n), d(n), nt(3 * (size_t)n);
+  std::vector<unsigned char> fx(n);
+  HIPCHK(c, hipMemcpy(fx.data(), c->A.fixed, n, hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; i++) {
+    const bool f = fx[i] != 0;  // a fixed atom keeps a = 0 (Atom::fix, Atom.hpp:58)
+    a[i] = f ? 0.0 : acc[3 * i];
+    b[i] = f ? 0.0 : acc[3 * i + 1];
+    d[i] = f ? 0.0 : acc[3 * i + 2];
+    nt[3 * i] = a[i];
+    nt[3 * i + 1] = b[i];
+    nt[3 * i + 2] = d[i];
+  }
+  HIPCHK(c, hipMemcpy(c->A.ax, a.data(), n * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->A.ay, b.data(), n * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->A.az, d.data(), n * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->A.an_nt, nt.data(), nt.size() * 8, hipMemcpyHostToDevice));
+  std::vector<unsigned char> k(c->nsys, 1);
+  if (keep)
+    for (int s = 0; s < c->nsys; s++) k[s] = keep[s] ? 1 : 0;
+  if (!c->acc_keep && dalloc(c, c->acc_keep, c->nsys)) return 1;
+  HIPCHK(c, hipMemcpy(c->acc_keep, k.data(), c->nsys, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int mdg_set_iteration(mdg_ctx* c, int system, int64_t iteration) {
+  if (c->n == 0) {
+    c->err = "no system";
+    return 1;
+  }
+  if (system < -1 || system >= c->nsys || iteration < 0) {
+    c->err = "mdg_set_iteration: bad system index or negative iteration";
+    return 1;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int s0 = system < 0 ? 0 : system, s1 = system < 0 ? c->nsys : system + 1;
+  std::vector<long long> it(s1 - s0, (long long)iteration);
+  HIPCHK(c, hipMemcpy(c->iter + s0, it.data(), it.size() * 8, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int mdg_get_iteration(mdg_ctx* c, int64_t* out) {
+  if (c->n == 0) {
+    c->err = "no system";
+    return 1;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<long long> it(c->nsys);
+  HIPCHK(c, hipMemcpy(it.data(), c->iter, it.size() * 8, hipMemcpyDeviceToHost));
+  for (int s = 0; s < c->nsys; s++) out[s] = it[s];
   return 0;
 }
 

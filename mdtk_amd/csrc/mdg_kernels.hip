@@ -1234,10 +1234,12 @@ __global__ void k_pack_pos(Atoms A) {
   if (i < A.n) A.p[i] = make_double4(A.x[i], A.y[i], A.z[i], double(A.type[i]));
 }
 
-// First force evaluation (iteration 0, SimLoop.cxx:305-321): a = -grad/M.
-__global__ void k_init_acc(Atoms A) {
+// First force evaluation (iteration 0, SimLoop.cxx:305-321): a = -grad/M,
+// except in the systems whose accelerations were loaded (keep[s] = 1).
+__global__ void k_init_acc(Atoms A, const unsigned char* __restrict__ keep) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n || A.fixed[i]) return;
+  if (keep && keep[A.sys[i]]) return;
   double M = A.mass[i];
   A.ax[i] = -A.grad[3 * i] / M;
   A.ay[i] = -A.grad[3 * i + 1] / M;

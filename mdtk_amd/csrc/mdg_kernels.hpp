@@ -238,7 +238,7 @@ __global__ void k_drift_tile(Atoms A, const int2* rb, const double* dtp, const d
                              double* part);
 __global__ void k_kick_tile(Atoms A, Box bx, const int2* rb, const double* dtp, const double* simt,
                             const double* tfin, int nterms, const double* eatom, double* part);
-__global__ void k_init_acc(Atoms A);
+__global__ void k_init_acc(Atoms A, const unsigned char* keep);
 __global__ void k_pack_pos(Atoms A);
 __global__ void k_sys_energy(int nterms, const int* sys_off, int n, const double* eatom, double* eacc);
 // serr[s]: system s's device error word; a system with an error stops (its

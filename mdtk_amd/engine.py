@@ -79,6 +79,9 @@ def load_library(path: str = LIB_PATH):
     L.mdg_event_elapsed.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
     L.mdg_synchronize.argtypes = [vp]
     L.mdg_system_errors.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32)]
+    L.mdg_set_accelerations.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_uint8)]
+    L.mdg_set_iteration.argtypes = [vp, ctypes.c_int, ctypes.c_int64]
+    L.mdg_get_iteration.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     L.mdg_set_virial.argtypes = [vp, ctypes.c_int]
     L.mdg_virial.argtypes = [vp, dp]
     L.mdg_step_virial.argtypes = [vp, dp]
@@ -359,6 +362,24 @@ class Engine:
         cleared by the query."""
         out = np.zeros(max(self.nsys, 1), np.uint32)
         self._chk(_lib.mdg_system_errors(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+        return out
+
+    def set_accelerations(self, acc, keep=None):
+        """Accelerations (cm/s^2) of loaded states: the first run() starts the
+        systems with keep[s] (None: all) from them instead of from the forces
+        at their positions."""
+        a = np.ascontiguousarray(acc, np.float64).reshape(self.n, 3)
+        k = None if keep is None else np.ascontiguousarray(keep, np.uint8).reshape(max(self.nsys, 1))
+        self._chk(_lib.mdg_set_accelerations(self._h, _p(a), _p(k, ctypes.c_uint8)))
+
+    def set_iteration(self, iteration: int, system: int = -1):
+        """SimLoop::iteration of one system or all (-1)."""
+        self._chk(_lib.mdg_set_iteration(self._h, int(system), int(iteration)))
+
+    def iterations(self):
+        """Per system: SimLoop::iteration (steps done since 0 or the value set)."""
+        out = np.zeros(max(self.nsys, 1), np.int64)
+        self._chk(_lib.mdg_get_iteration(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
         return out
 
     def state(self):

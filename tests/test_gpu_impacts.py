@@ -105,3 +105,99 @@ def test_clock_stops_each_system_on_its_own():
     assert np.abs(r["dt"][:, 1] - r1["dt"]).max() <= 1e-12 * r1["dt"].max()
     t = eng.time()
     assert t[0, 0] == 0.0 and abs(t[1, 0] - r1["dt"].sum()) <= 1e-12 * t[1, 0]
+
+
+def _base(tmp_path, seed, t_final, n=300):
+    b = S.bombard_target(n, seed, energy_ev=60.0, height=2.0)
+    e, x, v, pbc, fx = b.to_cgs()
+    n = b.n
+    base = ST.State(e, x.reshape(n, 3), v.reshape(n, 3), np.zeros((n, 3)), pbc, np.zeros((n, 3), np.int32),
+                    np.tile(pbc != ST.NO_PBC, (n, 1)), np.arange(n, dtype=np.uint32), {"fixed": fx.astype(bool)},
+                    time=dict(sim_time=0.0, sim_time_final=t_final, dt=1e-20, dt_prev=1e-20))
+    ST.write_state(str(tmp_path / "base"), base)
+    return base, pbc
+
+
+def test_interrupted_batch_resumes_to_the_same_final_state(tmp_path):
+    """mdtrajsim's run control (main.cpp:115-191, SimLoop.cxx:263-268): a
+    batch cut short after its flushes resumes, from the latest md%010d state
+    of each impact (accelerations, clock, iteration, snapshots.conf), beside
+    impacts that start fresh, and ends in the state an uninterrupted batch
+    reaches, bit for bit (the FP64 form is deterministic)."""
+    base, pbc = _base(tmp_path, 47, 50e-15)
+    xy = np.random.default_rng(8).random((4, 2)) * pbc[:2]
+    I.write_ionpos(str(tmp_path / "ionpos.bin"), xy)
+    kw = dict(potentials=POTS, flush_interval=40, chunk=1000, compress=None)
+    a = I.run_impacts(str(tmp_path / "base"), str(tmp_path / "ionpos.bin"), str(tmp_path / "A"), batch=4, **kw)
+    with pytest.raises(RuntimeError):  # killed after 100 steps: flushes at 40 and 80 stay
+        I.run_impacts(str(tmp_path / "base"), str(tmp_path / "ionpos.bin"), str(tmp_path / "B"), stop=2, batch=2,
+                      max_steps=100, **kw)
+    for k in (0, 1):
+        d = I.impact_dir(str(tmp_path / "B"), k)
+        assert I.list_iterations(d) == [0, 40, 80] and not I.is_finished(d) and I.is_clean(d)
+        assert os.path.exists(os.path.join(d, "snapshots.conf"))
+    b = I.run_impacts(str(tmp_path / "base"), str(tmp_path / "ionpos.bin"), str(tmp_path / "B"), batch=4, **kw)
+    assert [r["impact"] for r in a] == [r["impact"] for r in b] == [0, 1, 2, 3]
+    from mdtk_amd import trajectory as TR
+    for ra, rb in zip(a, b):
+        print(f"impact {ra['impact']}: {ra['iterations']} / {rb['iterations']} iterations", flush=True)
+        assert ra["iterations"] == rb["iterations"] > 80 and ra["sim_time"] == rb["sim_time"]
+        da, db = ra["dir"], rb["dir"]
+        assert I.list_iterations(da) == I.list_iterations(db) == [ra["iterations"]]
+        fa = ST.read_state(os.path.join(da, I.state_id(ra["iterations"])))
+        fb = ST.read_state(os.path.join(db, I.state_id(rb["iterations"])))
+        for q in ("pos", "vel", "acc", "pbc_count"):
+            assert np.array_equal(getattr(fa, q), getattr(fb, q)), q
+        assert fa.time == fb.time
+        assert os.path.exists(os.path.join(db, "completed.ok")) and I.is_clean(db)
+        ta = [t for t, _ in TR.read_snapshots_conf(os.path.join(da, "snapshots.conf"))[1]]
+        tb = [t for t, _ in TR.read_snapshots_conf(os.path.join(db, "snapshots.conf"))[1]]
+        # the resumed iteration's own due test uses dt_prev = dt (SimLoop.cxx:247),
+        # so at most the snapshot at the resume time may differ
+        assert len(set(ta) ^ set(tb)) <= 1 and len(ta) > 3
+        assert sorted(f for f in os.listdir(da) if f.endswith(".xva")) == \
+            sorted(f for f in os.listdir(db) if f.endswith(".xva"))
+
+
+def test_device_error_ends_one_trajectory(tmp_path):
+    """A trajectory whose own terms raise a device error ends as SimLoop's
+    catch ends it (SimLoop.cxx:176-197): completed.error, its state at that
+    iteration, in.mde.after_crash and mde_state.after_crash; the other
+    trajectory of the batch finishes as it does alone, and a later run skips
+    the failed one (main.cpp:95-112)."""
+    from tests.test_gpu_errors import _hedgehog
+    good = S.jitter(S.polyethylene(2, 3, 5), 0.05, 3)
+    good.vel = S.maxwell_velocities(good, 300.0, stream=4)
+    trs = []
+    for k, s in enumerate([good, _hedgehog()]):
+        e, x, v, pbc, fx = s.to_cgs()
+        n = s.n
+        st = ST.State(e, x.reshape(n, 3), v.reshape(n, 3), np.zeros((n, 3)), pbc, np.zeros((n, 3), np.int32),
+                      np.tile(pbc != ST.NO_PBC, (n, 1)), np.arange(n, dtype=np.uint32), {},
+                      time=dict(sim_time=0.0, sim_time_final=5e-15, dt=1e-20, dt_prev=1e-20))
+        d = str(tmp_path / f"t{k}")
+        os.makedirs(d)
+        I.write_state_guarded(d, 0, st, None)
+        trs.append(I._start(d, k, st))
+    rep = I.run_trajectories(trs, potentials=("REBO", "AIREBO", "ETors"), compress=None, etors_compat=False,
+                             flush_interval=1000)
+    ok, bad = rep
+    assert ok["error"] == 0 and bad["error"] & 4  # MDG_ERR_BOND_OVERFLOW
+    assert os.path.exists(os.path.join(ok["dir"], "completed.ok"))
+    for f in ("completed.error", "in.mde.after_crash", "mde_state.after_crash"):
+        assert os.path.exists(os.path.join(bad["dir"], f)), f
+    assert not os.path.exists(os.path.join(bad["dir"], "completed.ok")) and I.is_finished(bad["dir"])
+    assert I.list_iterations(bad["dir"]) == [0, 1] and bad["iterations"] == 1
+    lines = open(os.path.join(bad["dir"], "in.mde.after_crash")).read().split("\n")
+    assert lines[0] == "19"
+    # the healthy trajectory equals a run of its own
+    d = str(tmp_path / "alone")
+    os.makedirs(d)
+    st = ST.read_state(os.path.join(ok["dir"], "md0000000000"))
+    I.write_state_guarded(d, 0, st, None)
+    one = I.run_trajectories([I._start(d, 0, st)], potentials=("REBO", "AIREBO", "ETors"), compress=None,
+                             etors_compat=False)[0]
+    assert one["iterations"] == ok["iterations"]
+    fa = ST.read_state(os.path.join(d, I.state_id(one["iterations"])))
+    fb = ST.read_state(os.path.join(ok["dir"], I.state_id(ok["iterations"])))
+    assert np.array_equal(fa.pos, fb.pos) and np.array_equal(fa.vel, fb.vel)
