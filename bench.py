@@ -9,7 +9,9 @@ trajectory energies and adaptive-dt bookkeeping):
   open, bottom chain row fixed), 300 K Maxwell-Boltzmann velocities, one Ar
   ion 8 A above the surface at 100 eV, normal incidence, seeded impact point;
   dt = 0.1 fs fixed.  All trajectories are advanced by one hipGraph launch
-  per step, in the mixed FP32/FP64 form.
+  per step, in the mixed FP32/FP64 form.  An untimed approach (set-up,
+  before the warm-up) brings the ions to 2 A above their surfaces, so the
+  timed window holds the impacts and their list rebuilds.
 The plain FP64 drop-in runs the same trajectories the way the reference
 does, one trajectory per context and one after another; it is timed on a
 sample of the trajectories.  Synthetic input; positions and velocities are
@@ -56,6 +58,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the single-system PE and drift figures")
+    ap.add_argument("--approach", type=int, default=-1,
+                    help="untimed steps before the warm-up (-1: until the ions are 2 A above their surfaces)")
     ap.add_argument("--mixed-only", action="store_true",
                     help="profiling runs: only the mixed batched engine (tuned, timed, kernel times), no CPU leg")
     ap.add_argument("--workload", choices=("bombard", "metal"), default="bombard",
@@ -63,10 +67,26 @@ def parse():
     return ap.parse_args()
 
 
-def timed(eng, steps, warmup, dt):
+def approach_steps(batch, dt, above=2.0):
+    """Steps of dt until the ions, moving along -z at their initial speed,
+    are `above` A over the top surface atom of their slab (the untimed
+    approach, so that the timed window holds the impacts)."""
+    import numpy as np
+    n = []
+    for s in batch:
+        h = s.pos[-1, 2] - s.pos[:-1, 2].max()
+        v = abs(s.vel[-1, 2]) * (dt / 1e-15)  # A/fs x fs per step
+        n.append(max(0.0, (h - above) / v) if v > 0 else 0.0)
+    return int(np.median(n))
+
+
+def timed(eng, steps, warmup, dt, approach=0):
     """Milliseconds for `steps` Verlet iterations between two HIP events on the
     engine's stream, with the device synchronised before and after; the
-    per-step outputs and the list rebuilds that fell inside the window."""
+    per-step outputs and the list rebuilds that fell inside the window.
+    `approach` untimed steps run first (set-up), then `warmup`."""
+    if approach:
+        eng.run(approach, fixed_dt=dt)
     eng.run(max(warmup, 1), fixed_dt=dt)
     eng.synchronize()
     c0 = eng.counters()
@@ -92,6 +112,7 @@ def kernel_times(eng, steps, dt):
 
 LJ_SHAPES = {0: (1, 64), 1: (3, 64), 2: (1, 32), 3: (1, 16), 4: (4, 16), 5: (3, 16)}  # autotuner's lj_shape -> (OCC, LW)
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_bombard256.json")
+VALU_FILE = os.path.join(ROOT, "profiles", "valu_bombard256.json")
 METAL_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_cu256k.json")
 
 
@@ -117,9 +138,21 @@ def measured_traffic(kernel, path=TRAFFIC_FILE):
     return None if t is None else t["read_bytes"] + t["write_bytes"]
 
 
+def valu_profile(kernel, path=VALU_FILE):
+    """FP32/FP64 FLOPs and VALU activity per launch of `kernel` from the
+    committed rocprofv3 counter summary of the workload (or None)."""
+    try:
+        with open(path) as f:
+            return json.load(f)["per_launch"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def roofline(kt, cnt, natoms, tune):
     """The dominant kernel against MI355X's HBM peak (8 TB/s): algorithmic
-    bytes per launch over the launch's average HIP-event duration."""
+    bytes per launch over the launch's average HIP-event duration; beside it
+    the same launch's VALU rate (FLOPs per launch from the committed counter
+    summary over the live duration) against the FP32 and FP64 vector peaks."""
     dom = max(kt, key=lambda k: kt[k][0])
     dom_ms, dom_n = kt[dom]
     avg_s = dom_ms / max(dom_n, 1) / 1e3
@@ -127,13 +160,25 @@ def roofline(kt, cnt, natoms, tune):
            "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": dom_n}
     if dom == "k_lj_wave":
         occ, lw = LJ_SHAPES[int(tune.get("lj_shape", 3))]
-        name = f"k_lj_wave<float, false, {occ}, {lw}>"
+        name = f"k_lj_wave<float, false, {occ}, {lw}, false>"
         alg = lj_algorithmic_bytes(cnt, natoms)
         ach = alg / avg_s / 1e9
         out.update(kernel=name, achieved=ach, frac=ach / 8000.0, traffic=measured_traffic(name),
                    algorithmic_bytes=alg,
-                   note="gather-latency bound (dependent list -> position -> bond-record loads); "
-                        "traffic = bytes beyond L2 from rocprofv3 FETCH_SIZE/WRITE_SIZE")
+                   traffic_source=("profiles/traffic_bombard256.json: rocprofv3 FETCH_SIZE + WRITE_SIZE of this "
+                                   "kernel on this workload, copy-calibrated (measured once, not in this run)"),
+                   note="latency bound: waves wait on dependent list -> position -> bond-record gathers and on "
+                        "the LDS gradient cache (see valu.busy)")
+        v = valu_profile(name)
+        if v:
+            f32 = v["fp32_flops"] / avg_s / 1e12
+            f64 = v["fp64_flops"] / avg_s / 1e12
+            out["valu"] = {"achieved_fp32": f32, "peak_fp32": 157.3, "frac_fp32": f32 / 157.3,
+                           "achieved_fp64": f64, "peak_fp64": 78.6, "frac_fp64": f64 / 78.6, "unit": "TFLOP/s",
+                           "busy": v["valu_busy"], "lane_utilisation": v["lane_utilisation"],
+                           "wait_fraction": v["wait_fraction"],
+                           "source": "profiles/valu_bombard256.json (rocprofv3 SQ counters of this kernel)"}
+            out["binding"] = "neither: latency (dependent gathers, LDS atomics)"
     return out
 
 
@@ -200,7 +245,7 @@ def metal_block(steps, warmup):
 
 
 def force_rms(systems):
-    """RMS over atoms of |F_mixed - F_fp64| in eV/A at the initial configuration."""
+    """RMS over atoms of |F_mixed - F_fp64| in eV/A at the given configuration."""
     import numpy as np
     from mdtk_amd import Engine
     from mdtk_amd import systems as S
@@ -258,6 +303,18 @@ def _cpu_worker(args):
     return traj.n, steps, time.perf_counter() - t0
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(batch, budget_s: float):
     """The oracle (single-threaded CPU restatement, as the reference runs one
     trajectory per process) on the batch's smallest trajectories, one process
@@ -271,7 +328,7 @@ def cpu_baseline(batch, budget_s: float):
         res = pool.map(_cpu_worker, [(t, budget_s) for t in sample])
     rate = sum(n * k / dt for n, k, dt in res)
     one = [n * k / dt for n, k, dt in res]
-    return {"value": rate, "unit": "atom-steps/s", "cores": procs, "kind": "port",
+    return {"value": rate, "unit": "atom-steps/s", "cores": procs, "kind": "port", "cpu_model": cpu_model(),
             "per_process_value": sum(one) / len(one),
             "sample": f"the {procs} smallest bombardment trajectories of the batch "
                       f"({min(t.n for t in sample)}-{max(t.n for t in sample)} atoms: PE slab + Ar at 100 eV), "
@@ -305,15 +362,23 @@ def main():
         cpu = cpu_baseline(batch, a.cpu_seconds)
         log(f"cpu baseline: {cpu['value']:.4e} atom-steps/s on {cpu['cores']} processes")
 
-    if not a.mixed_only:
-        f_rms = force_rms(batch)
-        log(f"mixed force RMS error vs FP64: {f_rms:.3e} eV/A")
+    napp = approach_steps(batch, dt) if a.approach < 0 else a.approach
+    log(f"untimed approach: {napp} steps")
 
     # headline: mixed + batched, one graph launch per step for all trajectories
     emx = Engine.from_systems(batch, potentials=POTS, etors_compat=False, precision="mixed")
     tune_mx = emx.autotune(reps=2)
     log(f"mixed batched autotune: {tune_mx}")
-    ms_mx, res_mx = timed(emx, a.steps, a.warmup, dt)
+    if not a.mixed_only and napp:  # the configuration the timed window starts from
+        emx.run(napp, fixed_dt=dt)
+        x0 = emx.state()[0]
+        off = emx.offsets
+        at_window = [S.System(s.elem, x0[off[k]:off[k + 1]] / S.ANG, s.pbc, None, s.fixed, s.name)
+                     for k, s in enumerate(batch)]
+        ms_mx, res_mx = timed(emx, a.steps, a.warmup, dt)
+    else:
+        at_window = batch
+        ms_mx, res_mx = timed(emx, a.steps, a.warmup, dt, napp)
     v_mx = natoms * a.steps / (ms_mx / 1e3)
     log(f"mixed batched: {ms_mx / a.steps:.3f} ms/step, {v_mx:.4e} atom-steps/s")
     kt_mx = kernel_times(emx, a.steps, dt)
@@ -327,11 +392,14 @@ def main():
                           "counters": cnt_mx}), flush=True)
         return
 
+    f_rms = force_rms(at_window)
+    log(f"mixed force RMS error vs FP64 at the window's start: {f_rms:.3e} eV/A")
+
     # FP64 batched (same kernels in FP64)
     e64 = Engine.from_systems(batch, potentials=POTS, etors_compat=False)
     tune_64 = e64.autotune(reps=2)
     log(f"fp64 batched autotune: {tune_64}")
-    ms_64b, _ = timed(e64, a.steps, a.warmup, dt)
+    ms_64b, _ = timed(e64, a.steps, a.warmup, dt, napp)
     v_64b = natoms * a.steps / (ms_64b / 1e3)
     log(f"fp64 batched: {ms_64b / a.steps:.3f} ms/step, {v_64b:.4e} atom-steps/s")
     del e64
@@ -341,7 +409,7 @@ def main():
     tot_ms, tot_as = 0.0, 0
     for s in sample:
         e = Engine.from_system(s, potentials=POTS, etors_compat=False)
-        ms, _ = timed(e, a.steps, a.warmup, dt)
+        ms, _ = timed(e, a.steps, a.warmup, dt, napp)
         tot_ms += ms
         tot_as += s.n * a.steps
         del e
@@ -370,6 +438,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": (f"batched bombardment: {len(batch)} PE slabs ~U[2000,12000] atoms + Ar 100 eV, "
                                 "FBZL+REBO+AIREBO-LJ+ETors, dt 0.1 fs, one hipGraph launch per step"),
+                   "timed_window": (f"after an untimed approach of {napp} steps (the ions 2 A above their "
+                                    f"surfaces) and {a.warmup} warm-up steps: the impacts"),
                    "trajectories": len(batch), "atoms": natoms,
                    "atoms_min": min(s.n for s in batch), "atoms_max": max(s.n for s in batch),
                    "precision": "mixed" if use_mixed else "fp64", "batched": True,

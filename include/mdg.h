@@ -328,6 +328,11 @@ int mdg_kernel_time(mdg_ctx* ctx, const char* kernel, double* total_ms, int64_t*
  * in turn, 1: contiguous eighths).  No reference
  * counterpart. */
 int mdg_autotune(mdg_ctx* ctx, int reps, double* report);
+/* Sets the launch shapes directly from a report of mdg_autotune (the same 10
+ * values; the two timings are ignored), e.g. one saved from an earlier run
+ * on the same workload.  A knob outside the autotuner's candidates is an
+ * error.  No reference counterpart. */
+int mdg_set_tune(mdg_ctx* ctx, const double* knobs);
 
 /* Timing with HIP events on the context's stream: mdg_event_record records
  * event `slot` (0-7); mdg_event_elapsed waits for event b and returns the

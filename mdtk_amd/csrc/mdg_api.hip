@@ -130,6 +130,8 @@ struct mdg_ctx {
   long long* counters = nullptr;  // [0] short rebuilds [1] long rebuilds (per system and rebuild)
   long long* tally = nullptr;     // kTallySlots spread slots: [0] LJ pairs [1] quads [2] pair rows built
   unsigned* derr = nullptr;  // device error words: [0] global resources, [1 + s] system s
+  int* ions = nullptr;       // the atoms the ZBL term centres on (Ar, Xe), ascending
+  int nions = 0;
   std::vector<unsigned> failed;  // per system: error bits seen so far (mdg_system_errors)
   long iteration = 0;
 
@@ -775,9 +777,14 @@ void force_call(mdg_ctx* c) {
   if (c->slot_fbzl >= 0) {
     launch(c, "k_fbzl", [&] {
       double* ea = c->eatom + (size_t)c->slot_fbzl * n;
-      if (vir) k_fbzl<true><<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->cfg.fbzl_r0, c->cfg.fbzl_r1, A.grad, ea, vw);
-      else k_fbzl<false><<<blocks(n), 256, 0, c->stream>>>(A, c->box, c->cfg.fbzl_r0, c->cfg.fbzl_r1, A.grad, ea,
-                                                          nullptr);
+      if (c->nions == 0) return;
+      const int g = blocks(64LL * c->nions);  // a wave per ion
+      if (vir)
+        k_fbzl<true><<<g, 256, 0, c->stream>>>(A, c->box, c->ions, c->nions, c->cfg.fbzl_r0, c->cfg.fbzl_r1, A.grad,
+                                               ea, vw);
+      else
+        k_fbzl<false><<<g, 256, 0, c->stream>>>(A, c->box, c->ions, c->nions, c->cfg.fbzl_r0, c->cfg.fbzl_r1,
+                                                A.grad, ea, nullptr);
     });
   }
   if (A.gfix) {  // the fixed-point sums into grad and the atomically summed energy slots
@@ -1067,6 +1074,17 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   c->bath_h.assign(nsys, Bath{TB_NONE, 0.0, 1.0e13, 1000000.0 * kAo, 0.0, -5.0 * kAo, 0.0, 0.0, 0.0,
                               1000000.0 * kAo});
   r |= dalloc(c, A.type, n) | dalloc(c, A.elem, n) | dalloc(c, A.fixed, n);
+  {  // the ZBL term's centres
+    std::vector<int> ions;
+    for (int i = 0; i < n; i++)
+      if (el[i] == 40 || el[i] == 131) ions.push_back(i);
+    c->nions = (int)ions.size();
+    c->ions = nullptr;
+    if (c->nions) {
+      r |= dalloc(c, c->ions, ions.size());
+      if (!r) HIPCHK(c, hipMemcpy(c->ions, ions.data(), ions.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+  }
   r |= dalloc(c, A.pcx, n) | dalloc(c, A.pcy, n) | dalloc(c, A.pcz, n);
   r |= dalloc(c, A.disp_s, 3 * (size_t)n) | dalloc(c, A.disp_l, 3 * (size_t)n);
   r |= dalloc(c, A.nls_cnt, n) | dalloc(c, A.nll_cnt, n);
@@ -2055,6 +2073,32 @@ int mdg_autotune(mdg_ctx* c, int reps, double* report) {
     for (int q = 0; q < 10; q++) report[q] = r[q];
   }
   return check_err(c);
+}
+
+int mdg_set_tune(mdg_ctx* c, const double* knobs) {
+  const int occ_bonds = (int)knobs[0], occ_bstar = (int)knobs[2], lj_shape = (int)knobs[8], xcd = (int)knobs[9];
+  const int max_occ = c->mixed ? 4 : 2;
+  if (occ_bonds < 1 || occ_bonds > max_occ || occ_bstar < 1 || occ_bstar > max_occ || lj_shape < 0 ||
+      lj_shape > (c->mixed ? 5 : 3) || (xcd != 0 && xcd != 1)) {
+    c->err = "mdg_set_tune: a knob outside the autotuner's candidates";
+    return 1;
+  }
+  for (int q : {1, 3, 4, 5})
+    if (knobs[q] < 0 || knobs[q] > (1 << 20)) {
+      c->err = "mdg_set_tune: grid size outside [0, 2^20]";
+      return 1;
+    }
+  Tune& tu = c->tune;
+  tu.occ_bonds = occ_bonds;
+  tu.grid_bonds = (int)knobs[1];
+  tu.occ_bstar = occ_bstar;
+  tu.grid_bstar = (int)knobs[3];
+  tu.grid_lj = (int)knobs[4];
+  tu.grid_tors = (int)knobs[5];
+  tu.lj_shape = lj_shape;
+  c->A.wg_map = xcd;
+  drop_graphs(c);  // captured launch shapes are stale now
+  return 0;
 }
 
 int mdg_event_record(mdg_ctx* c, int slot) {

@@ -497,17 +497,22 @@ __device__ __forceinline__ bool zbl_target(int elem) {
 }
 __device__ __forceinline__ bool zbl_ion(int elem) { return elem == 40 || elem == 131; }
 
+// One wavefront per ion (ions[w]): its lanes share the ion's long-list row;
+// the pair energies go to the ion's slot, one sum per wave.
 template <bool VIR>
-__global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* __restrict__ grad, double* __restrict__ eatom,
-                       double* __restrict__ vir) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.n || !zbl_ion(A.elem[i])) return;
+__global__ void __launch_bounds__(256) k_fbzl(Atoms A, Box bx, const int* __restrict__ ions, int nions, double R0,
+                                              double R1, double* __restrict__ grad, double* __restrict__ eatom,
+                                              double* __restrict__ vir) {
+  const int w = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (w >= nions) return;  // wave-uniform
+  const int i = ions[w];
   Virial<VIR> vw(vir, 1.0);
   if constexpr (VIR) vw.at(A.sys[i]);
   double e = 0.0;
-  int cnt = A.nll_cnt[i];
+  const int cnt = A.nll_cnt[i];
   const int* list = A.nll + (size_t)i * A.cap_l;
-  for (int q = 0; q < cnt; q++) {
+  for (int q = lane; q < cnt; q += 64) {
     int j = list[q];
     bool jion = zbl_ion(A.elem[j]);
     if (!(jion || zbl_target(A.elem[j]))) continue;
@@ -536,11 +541,12 @@ __global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* __restrict
     vw.push(rv, v);
     e += val * f;
   }
-  energy_add(A, eatom, i, e);
-  vw.flush();
+  e = wave_sum(e);
+  if (lane == 0) energy_add(A, eatom, i, e);
+  vw.flush_wave();
 }
-template __global__ void k_fbzl<false>(Atoms, Box, double, double, double*, double*, double*);
-template __global__ void k_fbzl<true>(Atoms, Box, double, double, double*, double*, double*);
+template __global__ void k_fbzl<false>(Atoms, Box, const int*, int, double, double, double*, double*, double*);
+template __global__ void k_fbzl<true>(Atoms, Box, const int*, int, double, double, double*, double*, double*);
 
 // FLJ::operator() (FLJ.cxx:256-341) for the pairs {H,C} x {Cu,Au}: each metal
 // atom takes its hydrocarbon partners from the long list (so every pair once);

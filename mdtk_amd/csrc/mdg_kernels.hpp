@@ -188,7 +188,8 @@ template <class R, bool VIR>
 __global__ void k_etors(Atoms A, const TablesT<R>* T, Box bx, const int2* bonds, const int* nbonds, int compat,
                         double* grad, double* eatom, long long* nquads, double* vir);
 template <bool VIR>
-__global__ void k_fbzl(Atoms A, Box bx, double R0, double R1, double* grad, double* eatom, double* vir);
+__global__ void k_fbzl(Atoms A, Box bx, const int* ions, int nions, double R0, double R1, double* grad, double* eatom,
+                       double* vir);
 template <bool VIR>
 __global__ void k_flj(Atoms A, Box bx, const FLJTable* T, double* grad, double* eatom, double* vir);
 // Ackland EAM: the density pass (F'(rho) into fe, F into eatom), then the

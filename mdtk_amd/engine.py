@@ -79,6 +79,7 @@ def load_library(path: str = LIB_PATH):
     L.mdg_event_elapsed.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp]
     L.mdg_synchronize.argtypes = [vp]
     L.mdg_system_errors.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32)]
+    L.mdg_set_tune.argtypes = [vp, dp]
     L.mdg_set_accelerations.argtypes = [vp, dp, ctypes.POINTER(ctypes.c_uint8)]
     L.mdg_set_iteration.argtypes = [vp, ctypes.c_int, ctypes.c_int64]
     L.mdg_get_iteration.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
@@ -418,6 +419,14 @@ class Engine:
         keys = ["occ_bonds", "grid_bonds", "occ_bstar", "grid_bstar", "grid_lj", "grid_tors", "ms_before",
                 "ms_after", "lj_shape", "xcd_order"]
         return {k: (float(v) if k.startswith("ms") else int(v)) for k, v in zip(keys, r)}
+
+    def set_tune(self, tune: dict):
+        """Launch shapes as autotune() returns them (e.g. saved from an earlier
+        run on the same workload)."""
+        keys = ["occ_bonds", "grid_bonds", "occ_bstar", "grid_bstar", "grid_lj", "grid_tors", "ms_before",
+                "ms_after", "lj_shape", "xcd_order"]
+        r = np.array([float(tune.get(k, 0.0)) for k in keys])
+        self._chk(_lib.mdg_set_tune(self._h, _p(r)))
 
     def synchronize(self):
         self._chk(_lib.mdg_synchronize(self._h))
