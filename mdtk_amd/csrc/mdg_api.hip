@@ -461,15 +461,19 @@ void nl_build(mdg_ctx* c, bool shortlist) {
   });
   if (shortlist) {
     launch(c, "k_nl_build_short", [&] {
-      k_nl_build<64><<<blocks(n, 4, 8192), 256, 0, c->stream>>>(flag, c->sflags, 0, n, A.p, c->handled_s,
+      k_nl_build<64, true><<<blocks(n, 4, 8192), 256, 0, c->stream>>>(flag, c->sflags, 0, n, A.p, c->handled_s,
                                                            c->box, g, coff, range2, cell_of, start, cat,
                                                            A.nls_cnt, A.nls, A.disp_s, c->derr, nullptr);
     });
   } else {
     launch(c, "k_nl_build_long", [&] {
-      k_nl_build<512><<<blocks(n, 4, 8192), 256, 0, c->stream>>>(flag, c->sflags, 1, n, A.p, c->handled_l,
-                                                            c->box, g, coff, range2, cell_of, start, cat,
-                                                            A.nll_cnt, A.nll, A.disp_l, c->derr, A.nll_split);
+      auto go = [&](auto sorted) {
+        k_nl_build<512, decltype(sorted)::value><<<blocks(n, 4, 8192), 256, 0, c->stream>>>(
+            flag, c->sflags, 1, n, A.p, c->handled_l, c->box, g, coff, range2, cell_of, start, cat, A.nll_cnt,
+            A.nll, A.disp_l, c->derr, A.nll_split);
+      };
+      if (c->mixed) go(std::false_type());  // rows split at j > i only (no sort)
+      else go(std::true_type());
     });
     launch(c, "k_long_nbflag", [&] {
       k_long_nbflag<<<blocks(n, 16), 256, 0, c->stream>>>(flag, c->sflags, A, c->box, c->lmem);
@@ -1654,6 +1658,9 @@ int mdg_nl_export(mdg_ctx* c, int term, int32_t* counts, int32_t* idx, int64_t c
         idx[pos++] = j;
         m++;
       }
+    // the mixed form's long rows hold the j < i, then the j > i entries in
+    // candidate order; the reference's lists are ascending
+    if (!shortlist) std::sort(idx + (pos - m), idx + pos);
     counts[i] = m;
   }
   return 0;

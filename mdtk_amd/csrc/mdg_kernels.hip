@@ -117,12 +117,16 @@ __global__ void k_cell_fill(const int* __restrict__ flag, const int* __restrict_
 // with d^2 = dx*dx+dy*dy+dz*dz of the minimum-image displacement.  The
 // candidates of all stencil cells are swept 64 per round (a prefix over the
 // cells' counts maps a lane to its cell), compacted with a ballot/popcount
-// prefix into LDS, then sorted ascending (the reference's list order) with a
-// bitonic sort.
+// prefix into LDS, then (SORT) sorted ascending, the reference's list order,
+// with a bitonic sort, or (!SORT, the long list of the mixed form, whose
+// users need only the split) written as the j < i entries, then the j > i
+// entries, each in candidate order.  The FP64 form keeps every list sorted,
+// so its sums run in an order that no rebuild schedule changes (a run
+// resumed from a saved state repeats the uninterrupted one bit for bit).
 // Only atoms whose system asked for a rebuild (sflags[2*s+list]) are redone;
 // the grid strides over the atoms, so a launch with nothing to rebuild costs
 // a few thousand empty waves, not one per atom.
-template <int CAP>
+template <int CAP, bool SORT>
 __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, const int* __restrict__ sflags,
                                                   int list_id, int n, const double4* __restrict__ pos,
                                                   const unsigned char* __restrict__ handled, Box bx,
@@ -232,6 +236,32 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
       if (lane == 0) raise(sys_err(err, sy), CAP >= 256 ? ERR_LONG_NL_OVERFLOW : ERR_SHORT_NL_OVERFLOW);
       m = CAP;
     }
+    if constexpr (!SORT) {  // j < i entries first, then j > i, each in candidate order
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      int lt = 0;
+      for (int q = lane; q < m; q += 64) lt += (my[q] < i);
+      lt = wave_sum(lt);
+      int* row = list + (size_t)i * CAP;
+      const unsigned long long lanes_below = (1ull << lane) - 1ull;
+      for (int b = 0, a0 = 0, a1 = lt; b < m; b += 64) {
+        const int q = b + lane;
+        const int v = (q < m) ? my[q] : 0;
+        const bool low = q < m && v < i, high = q < m && v > i;
+        const unsigned long long ml = __ballot(low), mh = __ballot(high);
+        if (low) row[a0 + __popcll(ml & lanes_below)] = v;
+        if (high) row[a1 + __popcll(mh & lanes_below)] = v;
+        a0 += __popcll(ml);
+        a1 += __popcll(mh);
+      }
+      if (lane == 0) {
+        cnt[i] = m;
+        if (split) split[i] = lt;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      continue;
+    }
     // pad to a power of two and sort ascending
     int p2 = 1;
     while (p2 < m) p2 <<= 1;
@@ -267,12 +297,18 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
   }
 }
 
-template __global__ void k_nl_build<64>(const int*, const int*, int, int, const double4*, const unsigned char*,
-                                        Box, const CellGrid*, const int*, double, const int*, const int*,
-                                        const int*, int*, int*, double*, unsigned*, int*);
-template __global__ void k_nl_build<512>(const int*, const int*, int, int, const double4*, const unsigned char*,
-                                         Box, const CellGrid*, const int*, double, const int*, const int*,
-                                         const int*, int*, int*, double*, unsigned*, int*);
+template __global__ void k_nl_build<64, true>(const int*, const int*, int, int, const double4*,
+                                              const unsigned char*, Box, const CellGrid*, const int*, double,
+                                              const int*, const int*, const int*, int*, int*, double*, unsigned*,
+                                              int*);
+template __global__ void k_nl_build<512, true>(const int*, const int*, int, int, const double4*,
+                                               const unsigned char*, Box, const CellGrid*, const int*, double,
+                                               const int*, const int*, const int*, int*, int*, double*, unsigned*,
+                                               int*);
+template __global__ void k_nl_build<512, false>(const int*, const int*, int, int, const double4*,
+                                                const unsigned char*, Box, const CellGrid*, const int*, double,
+                                                const int*, const int*, const int*, int*, int*, double*, unsigned*,
+                                                int*);
 
 // ------------------------------------------------------------ skin test
 // Top-2 accumulated displacement per system and list (NeighbourList.cxx:

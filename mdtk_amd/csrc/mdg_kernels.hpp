@@ -127,7 +127,7 @@ __global__ void k_cell_scan(const int* flag, const int* sflags, int list_id, con
                             int* cnt, int* start);
 __global__ void k_cell_fill(const int* flag, const int* sflags, int list_id, int n, const int* sys,
                             const int* cell_of, const int* start, int* cursor, int* cell_atoms);
-template <int CAP>
+template <int CAP, bool SORT>
 __global__ void k_nl_build(const int* flag, const int* sflags, int list_id, int n, const double4* p,
                            const unsigned char* handled, Box bx, const CellGrid* grids,
                            const int* cell_off, double range2, const int* cell_of, const int* start,
