@@ -12,6 +12,7 @@ non-fixed atoms removed (mdtk/AtomsContainer.cxx:205-219,268-278).
 """
 from __future__ import annotations
 
+import itertools
 import math
 from dataclasses import dataclass, field
 
@@ -222,58 +223,112 @@ def ethane() -> System:
 def hydrocarbon_cluster(n_target: int, stream: int, jitter: float = 0.05, seed: int = BASE_SEED) -> System:
     """Random sp3 CH2 chain fragment with ~n_target atoms (C:H ~ 1:2), open boundaries.
 
-    A zig-zag backbone is folded by seeded dihedral choices (trans/gauche) with
-    a simple overlap rejection; each C carries two H (ends carry three), then
-    every coordinate is jittered by N(0, jitter) Angstrom.
+    A zig-zag backbone is folded by seeded dihedral choices (trans/gauche);
+    each C carries two H (ends carry three), then every coordinate is jittered
+    by N(0, jitter) Angstrom.  A dihedral is rejected when the new C comes
+    within 2.4 A of a non-neighbour C or 2.0 A of a placed H, or when the H it
+    fixes on the previous C come within 2.0 A of a C other than their own or
+    1.8 A of another C's H: no H sits inside a foreign atom's REBO cutoff
+    (C-H 1.8 A, H-H 1.7 A) before the jitter.  The three dihedrals are tried
+    in a random order and the first accepted is kept; when none is, the
+    growing end steps back 1, 2, 4, ... C and grows again (at most 4 nC
+    times); past that the candidate with the largest clearance is kept.
     """
-    cc, ch = 1.54, 1.09
+    cc = 1.54
     nC = max(2, (n_target - 2) // 3)
-    u = uniforms(8 * nC + 64, stream, seed)
-    ui = iter(u)
+    ui = _uniform_stream(stream, seed)
     th = math.radians(112.0)
     Cs = [np.zeros(3), np.array([cc, 0, 0]), np.array([cc + cc * math.cos(math.pi - th), cc * math.sin(math.pi - th), 0])]
+    Hs = [_hydrogens(Cs, 0, 3), _hydrogens(Cs, 1, 3)]  # H of every C whose neighbours are placed
+    carr = np.zeros((max(nC, 3), 3))
+    carr[:3] = Cs
+    harr = np.zeros((3 * max(nC, 3) + 2, 3))
+    nh = 5
+    harr[:nh] = np.concatenate(Hs)
+    dihs = (math.pi, math.pi / 3, -math.pi / 3)
+    orders = list(itertools.permutations(range(3)))
+    backtracks, fails, longest = 0, 0, 3
     while len(Cs) < nC:
-        for _ in range(12):
-            dih = [math.pi, math.pi / 3, -math.pi / 3][int(next(ui) * 3) % 3]
-            p = _place(Cs[-3], Cs[-2], Cs[-1], cc, th, dih)
-            if min(np.linalg.norm(p - q) for q in Cs[:-2]) > 2.4:
+        k = len(Cs)
+        best, best_clear = None, -np.inf
+        for d in orders[int(next(ui) * 6) % 6]:  # the dihedrals in a random order
+            p = _place(Cs[-3], Cs[-2], Cs[-1], cc, th, dihs[d])
+            h_prev = _hydrogens((Cs[-2], Cs[-1], p), 1, 3)
+            clear = min(np.sqrt(((carr[:k - 2] - p) ** 2).sum(1).min()) - 2.4,
+                        np.sqrt(((harr[:nh] - p) ** 2).sum(1).min()) - 2.0,
+                        np.sqrt(((carr[None, :k - 1] - h_prev[:, None]) ** 2).sum(2).min()) - 2.0,
+                        np.sqrt(((harr[None, :nh] - h_prev[:, None]) ** 2).sum(2).min()) - 1.8)
+            if clear > best_clear:
+                best, best_clear = (p, h_prev), clear
+            if clear > 0:
                 break
-        Cs.append(p)
+        if best_clear <= 0 and k > 3 and backtracks < 4 * nC:
+            # a trapped end: take back the last 1, 2, 4, ... C (with the H
+            # they fixed) and grow again
+            backtracks += 1
+            fails += 1
+            for _ in range(min(1 << min(fails - 1, 10), k - 3)):
+                Cs.pop()
+                Hs.pop()
+                nh -= 2
+            continue
+        Cs.append(best[0])
+        Hs.append(best[1])
+        carr[k] = best[0]
+        harr[nh:nh + 2] = best[1]
+        nh += 2
+        if len(Cs) > longest:
+            longest, fails = len(Cs), 0
     Cs = Cs[:nC]
+    Hs = Hs[:nC - 1] + [_hydrogens(Cs, nC - 1, nC)]
     el, pos = [], []
-    hch = math.radians(109.47)
-    for i, cpos in enumerate(Cs):
+    for cpos, hs in zip(Cs, Hs):
         el.append(C)
         pos.append(cpos)
-        nb = [Cs[j] for j in (i - 1, i + 1) if 0 <= j < nC]
-        bis = -sum((q - cpos) / np.linalg.norm(q - cpos) for q in nb)
-        if np.linalg.norm(bis) < 1e-6:
-            bis = np.array([0, 0, 1.0])
-        bis /= np.linalg.norm(bis)
-        ax = (nb[0] - cpos) if len(nb) == 1 else (nb[1] - nb[0])
-        perp = np.cross(bis, ax)
-        for trial in ([1.0, 0, 0], [0, 1.0, 0]):
-            if np.linalg.norm(perp) > 1e-6:
-                break
-            perp = np.cross(bis, trial)
-        perp /= np.linalg.norm(perp)
-        nh = 3 if len(nb) == 1 else 2
-        if nh == 2:
-            dirs = [math.cos(hch / 2) * bis + s * math.sin(hch / 2) * perp for s in (-1, 1)]
-        else:
-            axn = -(nb[0] - cpos) / np.linalg.norm(nb[0] - cpos)
-            e1 = perp
-            e2 = np.cross(axn, e1)
-            dirs = [math.cos(math.radians(70.5)) * axn + math.sin(math.radians(70.5)) *
-                    (math.cos(2 * math.pi * k / 3) * e1 + math.sin(2 * math.pi * k / 3) * e2) for k in range(3)]
-        for d in dirs:
-            el.append(H)
-            pos.append(cpos + ch * d / np.linalg.norm(d))
+        el += [H] * len(hs)
+        pos += list(hs)
     pos = np.array(pos)
     pos += jitter * normals(pos.size, stream + 1000003, seed).reshape(pos.shape)
     pos -= pos.min(0)
     pos += 10.0
     return System(np.array(el, np.int32), pos, np.array([NO_PBC] * 3), name=f"cluster{stream}")
+
+
+def _uniform_stream(stream, seed):
+    """Uniforms of one Philox stream, 4096 at a time, for as long as asked."""
+    block = 0
+    while True:
+        yield from uniforms(4096, stream + 7919 * block, seed)
+        block += 1
+
+
+def _hydrogens(Cs, i, nC):
+    """The H of backbone C i (of the first nC): two in the H-C-H plane
+    normal to the C-C-C plane, H-C-H 109.47 deg; three, staggered, at a chain
+    end.  C-H 1.09 A."""
+    ch, hch = 1.09, math.radians(109.47)
+    cpos = Cs[i]
+    nb = [Cs[j] for j in (i - 1, i + 1) if 0 <= j < nC]
+    bis = -sum((q - cpos) / np.linalg.norm(q - cpos) for q in nb)
+    if np.linalg.norm(bis) < 1e-6:
+        bis = np.array([0, 0, 1.0])
+    bis /= np.linalg.norm(bis)
+    ax = (nb[0] - cpos) if len(nb) == 1 else (nb[1] - nb[0])
+    perp = np.cross(bis, ax)
+    for trial in ([1.0, 0, 0], [0, 1.0, 0]):
+        if np.linalg.norm(perp) > 1e-6:
+            break
+        perp = np.cross(bis, trial)
+    perp /= np.linalg.norm(perp)
+    if len(nb) == 2:
+        dirs = [math.cos(hch / 2) * bis + s * math.sin(hch / 2) * perp for s in (-1, 1)]
+    else:
+        axn = -(nb[0] - cpos) / np.linalg.norm(nb[0] - cpos)
+        e1 = perp
+        e2 = np.cross(axn, e1)
+        dirs = [math.cos(math.radians(70.5)) * axn + math.sin(math.radians(70.5)) *
+                (math.cos(2 * math.pi * k / 3) * e1 + math.sin(2 * math.pi * k / 3) * e2) for k in range(3)]
+    return np.array([cpos + ch * d / np.linalg.norm(d) for d in dirs])
 
 
 def _place(a, b, c, bond, theta, dih):

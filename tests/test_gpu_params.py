@@ -141,3 +141,43 @@ def test_force_matching_loss_gradient():
         assert rel <= 1e-5
         checked += 1
     assert checked >= 20
+
+
+def test_c4_scale_dE_dtheta():
+    """The differentiation config at its own size: 512 seeded hydrocarbon
+    clusters of 64-512 atoms in one batch.  The adjoint dE_s/dtheta of every
+    system equals the central difference of the batched GPU energy (h = 1e-6
+    relative, as above) for every parameter, and two members' energies and
+    forces equal the oracle's."""
+    from mdtk_amd import Engine
+    from mdtk_amd.engine import PARAM_NAMES
+    from oracle.oracle import Oracle, POT_REBO, POT_AIREBO
+    mem = _clusters(512, 9, lo=64, hi=512)
+    eng = Engine.from_systems(mem, potentials=POTS)
+    th0 = eng.params()
+    e, g, d = eng.energy_grad_params()
+    off = eng.offsets
+    for k in (0, 511):
+        eo, go = Oracle.from_system(mem[k], pots=POT_REBO | POT_AIREBO).energy_grad()
+        assert np.abs(e[k] - eo).max() <= 1e-12 * np.abs(eo).max()
+        assert force_rel_err(g[off[k]:off[k + 1]], go) <= 1e-10
+    abs_steps = {"B": 1e-6 * EV, "beta": 1e-6 / ANG}
+    steps = np.array([_fd_step(v) or abs_steps["beta" if n.startswith("beta") else "B"]
+                      for v, n in zip(th0, PARAM_NAMES)])
+    sens = np.abs(d) * (steps * 1e6)
+    scale = sens.max(1)
+    worst = 0.0
+    for k, name in enumerate(PARAM_NAMES):
+        h = steps[k]
+        tp, tm = th0.copy(), th0.copy()
+        tp[k] += h
+        tm[k] -= h
+        eng.set_params(tp)
+        ep = eng.energy_grad()[0].sum(1)
+        eng.set_params(tm)
+        em = eng.energy_grad()[0].sum(1)
+        err = np.abs((ep - em) / (2 * h) - d[:, k]) * (h * 1e6) / scale
+        worst = max(worst, err.max())
+        assert err.max() <= 1e-6, name
+    eng.set_params(th0)
+    print(f"512 clusters, {sum(s.n for s in mem)} atoms: worst adjoint-FD error {worst:.2e}", flush=True)

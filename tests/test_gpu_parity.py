@@ -117,3 +117,25 @@ def test_adaptive_dt_parity():
     rg = g.run(10)
     assert np.allclose(ro["dt"], rg["dt"], rtol=1e-12, atol=0)
     assert abs(ro["epot"][-1] - rg["epot"][-1]) <= 1e-11 * abs(ro["epot"][-1])
+
+
+def test_brenner_c1_trajectory():
+    """The plain Brenner NVE config at its own size: periodic diamond, 8 x 8 x 8
+    conventional cells (4096 C), 300 K Maxwell-Boltzmann velocities with zero
+    net momentum, dt 0.5 fs: per-step energies and the state after 20 steps
+    equal the oracle's."""
+    s = S.diamond(8, 8, 8)
+    assert s.n == 4096
+    s.vel = S.maxwell_velocities(s, 300.0, stream=1)
+    assert np.abs((s.vel * S.MASS_AMU[S.C]).sum(0)).max() <= 1e-12 * np.abs(s.vel).max() * s.n
+    o, g = _pair(s, ("Brenner",))
+    ro = o.run(20, fixed_dt=0.5e-15)
+    rg = g.run(20, fixed_dt=0.5e-15)
+    scale = np.abs(ro["epot"]).max()
+    assert np.abs(ro["epot"] - rg["epot"]).max() <= 1e-10 * scale
+    assert np.abs(ro["ekin"] - rg["ekin"]).max() <= 1e-9 * np.abs(ro["ekin"]).max()
+    xo, vo = o.state()[:2]
+    xg, vg = g.state()[:2]
+    print(f"diamond 4096 Brenner: max |dx| {np.abs(xo - xg).max() / S.ANG:.2e} A", flush=True)
+    assert np.abs(xo - xg).max() <= 1e-9 * S.ANG
+    assert np.abs(vo - vg).max() <= 1e-8 * np.abs(vo).max()
