@@ -96,6 +96,20 @@ int mdg_num_systems(const mdg_ctx* ctx);
  * forces a rebuild of every neighbour list. */
 int mdg_set_positions(mdg_ctx* ctx, const double* xyz, const int32_t* pbc_count, const double* dr);
 
+/* Each atom's own PBC vector (Atom::PBC, mdtk/Atom.hpp:35-82), for systems
+ * whose atoms do not all share their box: pbc[n*3] (cm, NO_PBC for an open
+ * axis), with positions xyz[n*3] and image counters pbc_count[n*3] (may be
+ * null: zero) as the caller's atoms hold them; each atom is wrapped by its
+ * own PBC (Atom::applyPBC, Atom.cxx:93-125; a nonzero counter on an open
+ * axis is an error, Atom.cxx:105).  Every pair then takes depos with both
+ * atoms' PBC (Atom.hpp:115-140): an open atom sees a periodic one at its
+ * unwrapped coordinate, x + PBC*PBC_count.  On each axis of a system the
+ * atoms may hold the box's value (mdg_set_system's pbc) and one other (a
+ * SimLoopSaver state's per-atom enabled flags give exactly box or NO_PBC);
+ * such a system's neighbour lists test all its pairs, as the reference's do.
+ * Call after mdg_set_system(s); every list is rebuilt. */
+int mdg_set_atom_pbc(mdg_ctx* ctx, const double* pbc, const double* xyz, const int32_t* pbc_count);
+
 /* Per-atom neighbour counts over all chain members' lists, as
  * FProxy::hasNeighbors tests them (FProxy.hpp:99-104): counts[n] > 0 iff
  * the atom has a neighbour in some list. */

@@ -78,7 +78,14 @@ struct mdg_ctx {
   std::vector<double> Lh;      // 3*nsys box lengths (host copy)
   std::vector<int> off_h;      // nsys+1 atom offsets (host copy)
   std::vector<double> dt_host;  // per-system dt carried between calls
-  Box box{};                   // device: L[3*nsys], sys[n]
+  Box box{};                   // device: L[3*nsys], sys[n]; per-atom PBC of mixed systems
+  // Systems whose atoms carry PBC vectors other than their box (mdg_set_atom_pbc)
+  // Systems whose atoms carry their own PBC vectors (mdg_set_atom_pbc): per
+  // system a flag, per atom the periodic axes of the box on which the atom
+  // is open (host copies; empty: no such system)
+  std::vector<unsigned char> apbc_h, open_h;
+  std::vector<unsigned char> tb_apply_h;  // per atom: Atom::apply_ThermalBath (host copy; empty: all)
+  unsigned char *apbc_d = nullptr, *open_d = nullptr;
   Atoms A{};
   Tables* dT = nullptr;
   TablesT<float>* dTf = nullptr;  // mixed precision: Angstrom/eV FP32 table
@@ -117,6 +124,7 @@ struct mdg_ctx {
   double* eatom = nullptr;  // nterms*n per-atom energies of the last force call
   bool virial = false;      // force calls reduce the virial (mdg_set_virial)
   double* vir = nullptr;    // 6*nsys virial of the last force call (erg)
+  long long* virfix = nullptr;  // 12*nsys: its six components as fixed-point (hi, lo) sums
   double* stepvir = nullptr;  // chunk*nsys*6 per-step virials of mdg_step_nve
   int stepvir_cap = 0;
   std::vector<double> vir_steps;  // per-step virials of the last mdg_step_nve
@@ -320,6 +328,50 @@ CellGrid make_grid(const double L[3], const std::vector<double>& xyz, int n, dou
   return g;
 }
 
+// The cell grids of every system for both lists, from host positions p
+// (3n): a system whose atoms carry their own PBC vectors gets one cell, so
+// its list build tests every pair of the system with depos, as the
+// reference's all-pairs build does (NeighbourList.cxx:31-106).
+int setup_grids(mdg_ctx* c, const std::vector<double>& p) {
+  const int nsys = c->nsys;
+  const std::vector<int>& off = c->off_h;
+  auto one = [&](double rc, CellGrid*& gd, int*& coff, int& ncell, int*& cnt, int*& start) -> int {
+    std::vector<CellGrid> g(nsys);
+    std::vector<int> co(nsys + 1, 0);
+    for (int s = 0; s < nsys; s++) {
+      if (!c->apbc_h.empty() && c->apbc_h[s]) {
+        g[s] = CellGrid{};
+        for (int d = 0; d < 3; d++) {
+          g[s].n[d] = 1;
+          g[s].w[d] = 1e300;
+        }
+      } else {
+        std::vector<double> ps(p.begin() + 3 * (size_t)off[s], p.begin() + 3 * (size_t)off[s + 1]);
+        g[s] = make_grid(&c->Lh[3 * s], ps, off[s + 1] - off[s], (1.0 + 0.44) * rc);
+      }
+      long long cells = (long long)g[s].n[0] * g[s].n[1] * g[s].n[2];
+      if (co[s] + cells > (1LL << 28)) {
+        c->err = "cell grid too large";
+        return 1;
+      }
+      co[s + 1] = co[s] + (int)cells;
+    }
+    ncell = co[nsys];
+    dfree(c, gd);
+    dfree(c, coff);
+    dfree(c, cnt);
+    dfree(c, start);
+    if (dalloc(c, gd, nsys) || dalloc(c, coff, nsys + 1) || dalloc(c, cnt, ncell) || dalloc(c, start, ncell + 1))
+      return 1;
+    HIPCHK(c, hipMemcpy(gd, g.data(), nsys * sizeof(CellGrid), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(coff, co.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice));
+    return 0;
+  };
+  if (c->have_short && one(c->rc_short, c->gs_d, c->coff_s, c->ncs, c->cnt_s, c->start_s)) return 1;
+  if (c->have_long && one(c->rc_long, c->gl_d, c->coff_l, c->ncl, c->cnt_l, c->start_l)) return 1;
+  return 0;
+}
+
 std::string err_text(unsigned h) {
   std::string m;
   if (h & ERR_SHORT_NL_OVERFLOW) m += " short-list overflow (raise max_short_nb);";
@@ -459,21 +511,33 @@ void nl_build(mdg_ctx* c, bool shortlist) {
   launch(c, "k_cell_fill", [&] {
     k_cell_fill<<<blocks(n), 256, 0, c->stream>>>(flag, c->sflags, L, n, A.sys, cell_of, start, cnt, cat);
   });
+  // systems whose atoms carry their own PBC vectors get their lists from
+  // the APBC instantiation (all pairs, each atom's own depos)
+  const bool apbc = c->box.apbc != nullptr;
   if (shortlist) {
     launch(c, "k_nl_build_short", [&] {
-      k_nl_build<64, true><<<blocks(n, 4, 8192), 256, 0, c->stream>>>(flag, c->sflags, 0, n, A.p, c->handled_s,
-                                                           c->box, g, coff, range2, cell_of, start, cat,
-                                                           A.nls_cnt, A.nls, A.disp_s, c->derr, nullptr);
+      auto go = [&](auto ap) {
+        k_nl_build<64, true, decltype(ap)::value><<<blocks(n, 4, 8192), 256, 0, c->stream>>>(
+            flag, c->sflags, 0, n, A.p, c->handled_s, c->box, g, coff, range2, cell_of, start, cat, A.nls_cnt,
+            A.nls, A.disp_s, c->derr, nullptr, A.x, A.y, A.z);
+      };
+      go(std::false_type());
+      if (apbc) go(std::true_type());
     });
   } else {
     launch(c, "k_nl_build_long", [&] {
-      auto go = [&](auto sorted) {
-        k_nl_build<512, decltype(sorted)::value><<<blocks(n, 4, 8192), 256, 0, c->stream>>>(
+      auto go = [&](auto sorted, auto ap) {
+        k_nl_build<512, decltype(sorted)::value, decltype(ap)::value><<<blocks(n, 4, 8192), 256, 0, c->stream>>>(
             flag, c->sflags, 1, n, A.p, c->handled_l, c->box, g, coff, range2, cell_of, start, cat, A.nll_cnt,
-            A.nll, A.disp_l, c->derr, A.nll_split);
+            A.nll, A.disp_l, c->derr, A.nll_split, A.x, A.y, A.z);
       };
-      if (c->mixed) go(std::false_type());  // rows split at j > i only (no sort)
-      else go(std::true_type());
+      if (c->mixed) {  // rows split at j > i only (no sort)
+        go(std::false_type(), std::false_type());
+        if (apbc) go(std::false_type(), std::true_type());
+      } else {
+        go(std::true_type(), std::false_type());
+        if (apbc) go(std::true_type(), std::true_type());
+      }
     });
     launch(c, "k_long_nbflag", [&] {
       k_long_nbflag<<<blocks(n, 16), 256, 0, c->stream>>>(flag, c->sflags, A, c->box, c->lmem);
@@ -540,7 +604,7 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   int n = c->n;
   Atoms& A = c->A;
   double* dth = TH ? c->dtheta : nullptr;
-  double* vir = VIR ? c->vir : nullptr;
+  long long* vir = VIR ? c->virfix : nullptr;
   if (c->have_short) {
     launch(c, "k_coord", [&] { k_coord<R><<<blocks(n, 16), 256, 0, c->stream>>>(A, T, c->box, c->derr); });
     launch(c, "k_bond_enum", [&] {
@@ -716,8 +780,8 @@ void force_call(mdg_ctx* c) {
   k_zero_i<<<kQueues * kQStride / 256, 256, 0, c->stream>>>(c->qcount, kQueues * kQStride);
   k_zero_counts<<<1, kTallySlots, 0, c->stream>>>(c->tally);
   const bool vir = c->virial && !c->theta_mode;
-  double* vw = vir ? c->vir : nullptr;
-  if (vir) k_zero<<<1, 64, 0, c->stream>>>(c->vir, 6 * (size_t)c->nsys);
+  long long* vw = vir ? c->virfix : nullptr;
+  if (vir) k_zero<<<1, 64, 0, c->stream>>>(reinterpret_cast<double*>(c->virfix), 12 * (size_t)c->nsys);  // all-zero bits
   if (c->mixed) {
     if (vir) force_terms<float, false, true>(c, c->dTf);
     else force_terms<float, false, false>(c, c->dTf);
@@ -799,6 +863,7 @@ void force_call(mdg_ctx* c) {
       k_fix_join<<<blocks((3LL + nt) * n, 256, 8192), 256, 0, c->stream>>>(A, (int)nt, fixmask, c->eatom);
     });
   }
+  if (vir) k_vir_join<<<blocks(6LL * c->nsys), 256, 0, c->stream>>>(c->nsys, c->virfix, c->vir);
   k_clear_flags<<<blocks(2LL * c->nsys), 256, 0, c->stream>>>(c->nsys, c->flags, c->sflags, c->pflags, c->eflags);
 }
 
@@ -1151,37 +1216,9 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
       r |= dalloc(c, E.e, n) | dalloc(c, E.fe, n) | dalloc(c, E.row, (size_t)n * kEamRow) | dalloc(c, E.cnt, n);
   }
   if (r) return 1;
-  // cell grids, one per system and list
-  auto grids = [&](double rc, CellGrid*& gd, int*& coff, int& ncell) -> int {
-    std::vector<CellGrid> g(nsys);
-    std::vector<int> co(nsys + 1, 0);
-    for (int s = 0; s < nsys; s++) {
-      std::vector<double> ps(p.begin() + 3 * (size_t)off[s], p.begin() + 3 * (size_t)off[s + 1]);
-      g[s] = make_grid(&c->Lh[3 * s], ps, off[s + 1] - off[s], (1.0 + 0.44) * rc);
-      long long cells = (long long)g[s].n[0] * g[s].n[1] * g[s].n[2];
-      if (co[s] + cells > (1LL << 28)) {
-        c->err = "cell grid too large";
-        return 1;
-      }
-      co[s + 1] = co[s] + (int)cells;
-    }
-    ncell = co[nsys];
-    if (dalloc(c, gd, nsys) || dalloc(c, coff, nsys + 1)) return 1;
-    HIPCHK(c, hipMemcpy(gd, g.data(), nsys * sizeof(CellGrid), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(coff, co.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice));
-    return 0;
-  };
-  if (c->have_short) {
-    if (grids(c->rc_short, c->gs_d, c->coff_s, c->ncs)) return 1;
-    r |= dalloc(c, c->cell_of_s, n) | dalloc(c, c->cnt_s, c->ncs) | dalloc(c, c->start_s, c->ncs + 1) |
-         dalloc(c, c->atoms_s, n);
-  }
-  if (c->have_long) {
-    if (grids(c->rc_long, c->gl_d, c->coff_l, c->ncl)) return 1;
-    r |= dalloc(c, c->cell_of_l, n) | dalloc(c, c->cnt_l, c->ncl) | dalloc(c, c->start_l, c->ncl + 1) |
-         dalloc(c, c->atoms_l, n);
-  }
-  if (r) return 1;
+  r |= dalloc(c, c->cell_of_s, n) | dalloc(c, c->atoms_s, n) | dalloc(c, c->cell_of_l, n) |
+       dalloc(c, c->atoms_l, n);
+  if (r || setup_grids(c, p)) return 1;
   std::vector<double> xs(n), ys(n), zs2(n), vxs(n, 0.0), vys(n, 0.0), vzs(n, 0.0);
   std::vector<int> px(n), py(n), pz(n);
   for (int i = 0; i < n; i++) {
@@ -1272,6 +1309,29 @@ int mdg_set_system(mdg_ctx* c, int n, const int32_t* elem, const double* xyz, co
 int mdg_num_systems(const mdg_ctx* c) { return c->nsys; }
 
 
+// The PBC the thermal bath's geometry REQUIREs of every atom it applies to
+// (SimLoop.hpp:141 lateral PBC for the box, :166 none for the sphere), each
+// atom with its own PBC vector.
+int bath_pbc_check(mdg_ctx* c, int s, int geom) {
+  if (geom != MDG_TB_BOX && geom != MDG_TB_SPHERE) return 0;
+  const bool mixed = !c->apbc_h.empty() && c->apbc_h[s];
+  for (int i = c->off_h[s]; i < (mixed ? c->off_h[s + 1] : c->off_h[s] + 1); i++) {
+    if (mixed && !c->tb_apply_h.empty() && !c->tb_apply_h[i]) continue;
+    double L[3];
+    for (int d = 0; d < 3; d++)
+      L[d] = (mixed && ((c->open_h[i] >> d) & 1)) ? kNoPbc : c->Lh[3 * s + d];
+    if (geom == MDG_TB_BOX && (L[0] == kNoPbc || L[1] == kNoPbc)) {
+      c->err = "thermal bath box needs lateral PBC (SimLoop.hpp:141) [system " + std::to_string(s) + "]";
+      return 1;
+    }
+    if (geom == MDG_TB_SPHERE && (L[0] != kNoPbc || L[1] != kNoPbc || L[2] != kNoPbc)) {
+      c->err = "thermal bath sphere needs open boundaries (SimLoop.hpp:166) [system " + std::to_string(s) + "]";
+      return 1;
+    }
+  }
+  return 0;
+}
+
 int mdg_set_thermal_bath(mdg_ctx* c, int system, const mdg_thermal_bath* tb) {
   if (c->n == 0) {
     c->err = "no system";
@@ -1287,17 +1347,8 @@ int mdg_set_thermal_bath(mdg_ctx* c, int system, const mdg_thermal_bath* tb) {
     return 1;
   }
   int s0 = system < 0 ? 0 : system, s1 = system < 0 ? c->nsys : system + 1;
-  for (int s = s0; s < s1; s++) {
-    const double* L = &c->Lh[3 * s];
-    if (tb->geometry == MDG_TB_BOX && (L[0] == kNoPbc || L[1] == kNoPbc)) {  // SimLoop.hpp:141
-      c->err = "thermal bath box needs lateral PBC (SimLoop.hpp:141) [system " + std::to_string(s) + "]";
-      return 1;
-    }
-    if (tb->geometry == MDG_TB_SPHERE && (L[0] != kNoPbc || L[1] != kNoPbc || L[2] != kNoPbc)) {  // :166
-      c->err = "thermal bath sphere needs open boundaries (SimLoop.hpp:166) [system " + std::to_string(s) + "]";
-      return 1;
-    }
-  }
+  for (int s = s0; s < s1; s++)
+    if (bath_pbc_check(c, s, tb->geometry)) return 1;
   if (bath_alloc(c)) return 1;
   for (int s = s0; s < s1; s++)
     c->bath_h[s] = Bath{tb->geometry, tb->To,        tb->gamma,      tb->z_min,  tb->d_boundary, tb->z_min_free,
@@ -1319,6 +1370,13 @@ int mdg_set_bath_atoms(mdg_ctx* c, const uint8_t* apply) {
   std::vector<unsigned char> a(c->n, 1);
   if (apply)
     for (int i = 0; i < c->n; i++) a[i] = apply[i] ? 1 : 0;
+  std::vector<unsigned char> prev = c->tb_apply_h;
+  c->tb_apply_h = a;
+  for (int s = 0; s < c->nsys; s++)
+    if (bath_pbc_check(c, s, c->bath_h[s].geom)) {
+      c->tb_apply_h = prev;
+      return 1;
+    }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(c->tb_apply_d, a.data(), c->n, hipMemcpyHostToDevice));
   return 0;
@@ -1531,7 +1589,8 @@ int mdg_set_virial(mdg_ctx* c, int on) {
     c->err = "mdg_set_virial: set the system(s) first";
     return 1;
   }
-  if (on && !c->vir && dalloc(c, c->vir, 6 * (size_t)c->nsys)) return 1;
+  if (on && !c->vir && (dalloc(c, c->vir, 6 * (size_t)c->nsys) || dalloc(c, c->virfix, 12 * (size_t)c->nsys)))
+    return 1;
   if ((on != 0) != c->virial) drop_graphs(c);  // the captured step's launches change
   c->virial = on != 0;
   return 0;
@@ -1927,6 +1986,82 @@ int mdg_set_positions(mdg_ctx* c, const double* xyz, const int32_t* pbc_count, c
   sys_disp(c, lim_s, lim_l);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
+}
+
+int mdg_set_atom_pbc(mdg_ctx* c, const double* pbc, const double* xyz, const int32_t* pbc_count) {
+  const int n = c->n, nsys = c->nsys;
+  if (n == 0) {
+    c->err = "no system";
+    return 1;
+  }
+  if (!pbc || !xyz) {
+    c->err = "mdg_set_atom_pbc: null pbc or positions";
+    return 1;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // every list's minimum image must be the reference's depos for the pairs it
+  // lists: L >= 2 (1 + 0.44) Rc on the axes where a system's atoms differ
+  const double span = 2.0 * (1.0 + 0.44) * std::max(c->rc_short, c->rc_long);
+  std::vector<unsigned char> open(n, 0), ap(nsys, 0);
+  for (int s = 0; s < nsys; s++)
+    for (int i = c->off_h[s]; i < c->off_h[s + 1]; i++)
+      for (int d = 0; d < 3; d++) {
+        const double L0 = c->Lh[3 * s + d], v = pbc[3 * i + d];
+        if (v == L0) continue;
+        if (v != kNoPbc) {
+          c->err = "mdg_set_atom_pbc: an atom's PBC on each axis is its system's box value or NO_PBC (atom " +
+                   std::to_string(i) + ")";
+          return 1;
+        }
+        if (L0 < span) {
+          c->err = "mdg_set_atom_pbc: a box with atoms open on its axis " + std::to_string(d) +
+                   " must be at least 2 (1 + 0.44) Rc = " + std::to_string(span / kAo) + " A wide [system " +
+                   std::to_string(s) + "]";
+          return 1;
+        }
+        open[i] |= (unsigned char)(1u << d);
+        ap[s] = 1;
+      }
+  // each atom wrapped by its own PBC (Atom::applyPBC, Atom.cxx:93-125)
+  std::vector<double> q(xyz, xyz + 3 * (size_t)n);
+  std::vector<int32_t> pc(3 * (size_t)n, 0);
+  if (pbc_count) pc.assign(pbc_count, pbc_count + 3 * (size_t)n);
+  for (int i = 0; i < n; i++)
+    for (int d = 0; d < 3; d++) {
+      const double L = pbc[3 * i + d];
+      double& x = q[3 * i + d];
+      int32_t& k = pc[3 * i + d];
+      if (L == kNoPbc) {
+        if (k != 0) {  // Atom.cxx:105,114,123
+          c->err = "mdg_set_atom_pbc: nonzero PBC_count on an open axis (Atom.cxx:105) [atom " + std::to_string(i) +
+                   "]";
+          return 1;
+        }
+        continue;
+      }
+      int guard = 0;
+      while (x < 0 && guard++ < 1000000) { x += L; --k; }
+      while (x >= L && guard++ < 1000000) { x -= L; ++k; }
+    }
+  bool any = false;
+  for (unsigned char m : ap) any = any || m;
+  c->apbc_h = ap;
+  c->open_h = open;
+  for (int s = 0; s < nsys; s++)
+    if (bath_pbc_check(c, s, c->bath_h[s].geom)) return 1;
+  if (!c->apbc_d && (dalloc(c, c->apbc_d, nsys) || dalloc(c, c->open_d, n))) return 1;
+  HIPCHK(c, hipMemcpy(c->apbc_d, ap.data(), nsys, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->open_d, open.data(), n, hipMemcpyHostToDevice));
+  c->box.apbc = any ? c->apbc_d : nullptr;
+  c->box.open_axes = c->open_d;
+  c->box.pcx = c->A.pcx;
+  c->box.pcy = c->A.pcy;
+  c->box.pcz = c->A.pcz;
+  c->A.open_axes = any ? c->open_d : nullptr;
+  c->A.boxL = c->box.L;
+  if (setup_grids(c, q)) return 1;
+  drop_graphs(c);  // the kernels' Box and Atoms arguments changed
+  return mdg_set_positions(c, q.data(), pc.data(), nullptr);  // every list is rebuilt
 }
 
 int mdg_neighbor_counts(mdg_ctx* c, int32_t* counts) {

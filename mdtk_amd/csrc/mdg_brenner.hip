@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(256) k_brenner_bonds(Atoms A, const Tables* __
                                                       const int2* __restrict__ bonds,
                                                       const int* __restrict__ nbonds, double* __restrict__ grad,
                                                       double* __restrict__ eatom, unsigned* err,
-                                                      double* __restrict__ vir) {
+                                                      long long* __restrict__ vir) {
   const BrennerParams& P = T->bren;
   Virial<VIR> vw(vir, 1.0);
   const GradSink gsink{grad, A.gfix};
@@ -265,8 +265,8 @@ __global__ void __launch_bounds__(256) k_brenner_bonds(Atoms A, const Tables* __
 }
 
 template __global__ void k_brenner_bonds<false>(Atoms, const Tables*, Box, const int2*, const int*, double*,
-                                                double*, unsigned*, double*);
+                                                double*, unsigned*, long long*);
 template __global__ void k_brenner_bonds<true>(Atoms, const Tables*, Box, const int2*, const int*, double*,
-                                               double*, unsigned*, double*);
+                                               double*, unsigned*, long long*);
 
 }  // namespace mdg

@@ -206,18 +206,89 @@ __device__ __forceinline__ double mic(double r, double L) {
 
 // Boxes of a batch of independent systems: L[3*s..3*s+2] is system s's box,
 // sys[a] the system of atom a (all zero for a single system).
+// Atoms carry their own PBC vectors in the reference (Atom::PBC).  A system
+// whose atoms do not all share its box is "mixed": on each periodic axis of
+// the box some of its atoms are open (NO_PBC), as a state file's per-atom
+// enabled flags allow; open_axes[a] bit d marks them.  Only the list build,
+// the packed copy and the PBC wrap look at this (see packed_coord below);
+// apbc == null when no system is mixed.
 struct Box {
   const double* L;
   const int* sys;
+  const unsigned char* apbc;       // per system
+  const unsigned char* open_axes;  // per atom
+  const int *pcx, *pcy, *pcz;  // Atom::PBC_count, for the unwrap of mixed pairs
 };
 __device__ __forceinline__ const double* box_of(const Box& bx, int a) { return bx.L + 3 * bx.sys[a]; }
+
+// One system's periodicity as its atom a sees it.
+struct Frame {
+  const double* L;
+};
+__device__ __forceinline__ Frame frame_of(const Box& bx, int a) { return Frame{bx.L + 3 * bx.sys[a]}; }
+
+// The PBC value of atom a's axis d (its own Atom::PBC).
+__device__ __forceinline__ double pbc_of(const Box& bx, const double* L, int a, int d) {
+  return (bx.apbc && bx.apbc[bx.sys[a]] && ((bx.open_axes[a] >> d) & 1)) ? kNoPbc : L[d];
+}
+
+// Atom a's own PBC vector.
+struct Pbc3 {
+  double v[3];
+};
+__device__ __forceinline__ Pbc3 own_pbc(const Box& bx, int a) {
+  const double* L = box_of(bx, a);
+  Pbc3 o;
+#pragma unroll
+  for (int d = 0; d < 3; d++) o.v[d] = pbc_of(bx, L, a, d);
+  return o;
+}
+
+// One axis of depos for atoms with PBC values La, Lb and image counters ca,
+// cb (mdtk/Atom.hpp:121-133): an open atom sees a periodic one at its
+// unwrapped coordinate; two atoms of one periodicity take the minimum image.
+__device__ __forceinline__ double depos_axis(double r, double La, double Lb, const int* pc, int a, int b) {
+  if (La == kNoPbc && Lb != kNoPbc) r -= Lb * pc[b];
+  if (Lb == kNoPbc && La != kNoPbc) r += La * pc[a];
+  if (La == Lb && La != kNoPbc && fabs(r) > La * 0.5) r += (r > 0) ? (-La) : (La);
+  return r;
+}
+
+// depos with both atoms' own PBC vectors from their true coordinates
+// (A.x/y/z, not the packed copy): the list build of mixed systems.
+__device__ __forceinline__ D3 depos_atoms(const Box& bx, const double* x, const double* y, const double* z, int a,
+                                          int b) {
+  const double* L = box_of(bx, a);
+  return d3(depos_axis(x[a] - x[b], pbc_of(bx, L, a, 0), pbc_of(bx, L, b, 0), bx.pcx, a, b),
+            depos_axis(y[a] - y[b], pbc_of(bx, L, a, 1), pbc_of(bx, L, b, 1), bx.pcy, a, b),
+            depos_axis(z[a] - z[b], pbc_of(bx, L, a, 2), pbc_of(bx, L, b, 2), bx.pcz, a, b));
+}
+
+// The packed copy holds every coordinate reduced into [0, L) on each
+// periodic axis of the system's box.  For the atoms whose own PBC is the box
+// that is their wrapped coordinate; an atom of a mixed system that is open
+// on such an axis keeps its true coordinate in A.x and its reduced one here.
+// The force kernels then take plain minimum images, and for every pair the
+// mixed list build lists, that is the reference's depos: the two differ by a
+// multiple of L, and a listed pair stays within 2 (1 + 0.44) Rc of its
+// partner, so the minimum image cannot pick another copy within a cutoff
+// when L >= 2 (1 + 0.44) Rc (mdg_set_atom_pbc requires it of mixed systems).
+__device__ __forceinline__ double packed_coord(double x, double L) {
+  if (L == kNoPbc || (x >= 0.0 && x < L)) return x;
+  double q = x - L * floor(x / L);
+  return (q >= L) ? q - L : q;
+}
+
+// x_a - x_b (mdtk/Atom.hpp:115-140) for atoms a, b of the system of frame f,
+// from their packed positions: the minimum image.
+__device__ __forceinline__ D3 disp(const Box&, const Frame& f, int, int, double4 pa, double4 pb) {
+  return d3(mic(pa.x - pb.x, f.L[0]), mic(pa.y - pb.y, f.L[1]), mic(pa.z - pb.z, f.L[2]));
+}
 
 // Positions are gathered from a packed copy p[a] = (x, y, z, type): one
 // 32-byte load per atom instead of three scattered ones.
 __device__ __forceinline__ D3 depos(const double4* __restrict__ p, int a, int b, const Box& bx) {
-  const double* L = box_of(bx, a);
-  double4 pa = p[a], pb = p[b];
-  return d3(mic(pa.x - pb.x, L[0]), mic(pa.y - pb.y, L[1]), mic(pa.z - pb.z, L[2]));
+  return disp(bx, frame_of(bx, a), a, b, p[a], p[b]);
 }
 
 // A displacement taken in FP64 cm, expressed in the kernel's unit and type:
@@ -530,7 +601,7 @@ __device__ __forceinline__ void scatter3(const G& g, int a, V3<R> v, double gs) 
 // at the end.  Virial<false> compiles to nothing.
 template <bool ON>
 struct Virial {
-  __device__ __forceinline__ Virial(double* = nullptr, double = 1.0) {}
+  __device__ __forceinline__ Virial(long long* = nullptr, double = 1.0) {}
   __device__ __forceinline__ void at(int) {}
   template <class R>
   __device__ __forceinline__ void push(V3<R>, V3<R>) {}
@@ -546,14 +617,16 @@ template <>
 struct Virial<true> {
   double w[6] = {0, 0, 0, 0, 0, 0};
   int sys = -1;
-  double* out = nullptr;
+  long long* out = nullptr;  // per system six fixed-point (hi, lo) pairs of the energy scale
   double scale = 1.0;
-  __device__ __forceinline__ Virial(double* o = nullptr, double sc = 1.0) : out(o), scale(sc) {}
+  __device__ __forceinline__ Virial(long long* o = nullptr, double sc = 1.0) : out(o), scale(sc) {}
+  // Integer sums, so the virial of a force call does not depend on the order
+  // in which waves add their parts (as the fixed-point gradient).
   __device__ __forceinline__ void add_out() {
     if (sys < 0) return;
 #pragma unroll
     for (int c = 0; c < 6; c++)
-      if (w[c] != 0.0) unsafeAtomicAdd(out + 6 * sys + c, w[c] * scale);
+      if (w[c] != 0.0) fix_atomic(out + 12 * sys + 2 * c, w[c] * scale * kFixEnergy);
 #pragma unroll
     for (int c = 0; c < 6; c++) w[c] = 0.0;
   }

@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(256) k_coord(Atoms A, const TablesT<R>* __rest
     const int cnt = A.nls_cnt[i];
     const int* L = A.nls + (size_t)i * A.cap_s;
     const double4 pi = A.p[i];
-    const double* Lb = box_of(bx, i);
+    const Frame fr = frame_of(bx, i);
     for (int b0 = 0; b0 < cnt; b0 += LW) {
       const int q = b0 + sl;
       bool ok = false;
@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(256) k_coord(Atoms A, const TablesT<R>* __rest
         const double4 pk = A.p[k];
         tk = (int)pk.w;  // the packed position carries the type
         if ((unsigned)tk > (unsigned)T_OTHER) raise(er, ERR_TYPE_RANGE);
-        rv = to_local<R>(d3(mic(pi.x - pk.x, Lb[0]), mic(pi.y - pk.y, Lb[1]), mic(pi.z - pk.z, Lb[2])));
+        rv = to_local<R>(disp(bx, fr, i, k, pi, pk));
         r = sqrt(rv.x * rv.x + rv.y * rv.y + rv.z * rv.z);
         if ((unsigned)tk <= (unsigned)T_H && !(r > T->rebo.R[ti][tk][1])) {
           ok = true;
@@ -630,7 +630,7 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
                                                    const int2* __restrict__ bonds, const int* __restrict__ nbonds,
                                                    double* __restrict__ grad, double* __restrict__ eatom,
                                                    unsigned* err, double* __restrict__ dth,
-                                                   double* __restrict__ vir) {
+                                                   long long* __restrict__ vir) {
   using GC = CacheOf<R, kBondCache>;
   __shared__ int s_key[GC::kSlots];
   __shared__ typename GC::val_t s_val[GC::kVals];
@@ -887,7 +887,7 @@ __global__ void __launch_bounds__(256) k_pair_rows(Atoms A, const TablesT<R>* __
         lim2[t] = rc * rc;
       }
       double4 pi4 = A.p[i];
-      const double* Lb = box_of(bx, i);
+      const Frame fr = frame_of(bx, i);
       int cnt = A.nll_cnt[i];
       const int* list = A.nll + (size_t)i * A.cap_l;
       int* out = A.pl + (size_t)i * A.cap_p;
@@ -907,9 +907,8 @@ __global__ void __launch_bounds__(256) k_pair_rows(Atoms A, const TablesT<R>* __
           int tj = (int)pj[u].w;
           bool ok = false;
           if (jj[u] > i && (unsigned)tj <= (unsigned)T_H) {
-            double dx = mic(pi4.x - pj[u].x, Lb[0]), dy = mic(pi4.y - pj[u].y, Lb[1]),
-                   dz = mic(pi4.z - pj[u].z, Lb[2]);
-            ok = dx * dx + dy * dy + dz * dz <= lim2[tj];
+            const D3 d = disp(bx, fr, i, jj[u], pi4, pj[u]);
+            ok = d.x * d.x + d.y * d.y + d.z * d.z <= lim2[tj];
           }
           unsigned long long b = __ballot(ok) & mine;
           if (ok) {
@@ -950,7 +949,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
                                                      double* __restrict__ grad, double* __restrict__ eatom,
                                                      int4* __restrict__ queue, int* __restrict__ qcount, int qsub,
                                                      long long* __restrict__ npairs, unsigned* err,
-                                                     double* __restrict__ dth, double* __restrict__ vir) {
+                                                     double* __restrict__ dth, long long* __restrict__ vir) {
   constexpr int G = 4 * (64 / LW);  // atoms in flight per workgroup
   // this workgroup's b* sub-queue
   int* const qc = qcount + (blockIdx.x & (kQueues - 1)) * kQStride;
@@ -993,9 +992,8 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     if (ti > T_H) return;
     if constexpr (VIR) vw.at(A.sys[i]);
     unsigned* const er = sys_err(err, A.sys[i]);
-    const double* Lb = box_of(bx, i);
-    double4 pi4 = A.p[i];
-    double xi = pi4.x, yi = pi4.y, zi = pi4.z;
+    const Frame fr = frame_of(bx, i);
+    const double4 pi4 = A.p[i];
     // phase 0: compact the LJ pairs
     // the pair row, or the j > i part of the long row when the pair row overflowed
     int cnt = A.pl_cnt[i];
@@ -1026,8 +1024,8 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
         int tj = (int)pj[u].w;  // the packed position carries the type
         bool ok = false;
         if (jj[u] > i && (unsigned)tj <= (unsigned)T_H) {
-          double dx = mic(xi - pj[u].x, Lb[0]), dy = mic(yi - pj[u].y, Lb[1]), dz = mic(zi - pj[u].z, Lb[2]);
-          double d2 = dx * dx + dy * dy + dz * dz;
+          const D3 d = disp(bx, fr, i, jj[u], pi4, pj[u]);
+          double d2 = d.x * d.x + d.y * d.y + d.z * d.z;
           ok = !(d2 > double(L.R2sq[ti][tj]) * cut2);
         }
         unsigned long long m = __ballot(ok) & mine;
@@ -1165,7 +1163,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
         raise(er, ERR_TYPE_RANGE);
         return;
       }
-      D3 rd = d3(mic(xi - pj.x, Lb[0]), mic(yi - pj.y, Lb[1]), mic(zi - pj.z, Lb[2]));
+      D3 rd = disp(bx, fr, i, j, pi4, pj);
       R w = R(0);
       int bkey = 0x7fffffff;
       int h = hslot(j);
@@ -1271,11 +1269,11 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
   // a contiguous share of the atoms per workgroup, G atoms per pass
   int a0, a1;
   block_share(A.n, a0, a1, A.wg_map, G);
-  for (int base = a0; base < a1; base += G) {
-    if (base + gid < a1) atom(base + gid);
-    __syncthreads();
-    gc.drain_if_crowded();
-  }
+    for (int base = a0; base < a1; base += G) {
+      if (base + gid < a1) atom(base + gid);
+      __syncthreads();
+      gc.drain_if_crowded();
+    }
   gc.flush();
   vw.flush_wave();
   count_add(npairs, np);
@@ -1301,7 +1299,7 @@ template <class R, bool TH, int OCC, int PASS, bool VIR>
 __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
                                               int4* __restrict__ queue, const int* __restrict__ qcount,
                                               int qsub, double* __restrict__ grad, double* __restrict__ eatom,
-                                              unsigned* err, double* __restrict__ dth, double* __restrict__ vir) {
+                                              unsigned* err, double* __restrict__ dth, long long* __restrict__ vir) {
   static_assert(kQueues == 256, "one thread per sub-queue in the prefix below (blockDim 256)");
   static_assert(PASS >= 0 && PASS <= 2 && (PASS == 0 || !TH), "parameter gradients use the one-pass form");
   const double gs = T->eunit / T->lunit;
@@ -1458,7 +1456,7 @@ template <class R, bool VIR>
 __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
                                               const int2* __restrict__ bonds, const int* __restrict__ nbonds,
                                               int compat, double* __restrict__ grad, double* __restrict__ eatom,
-                                              long long* __restrict__ nquads, double* __restrict__ vir) {
+                                              long long* __restrict__ nquads, long long* __restrict__ vir) {
   using GC = CacheOf<R, kTorsCache>;
   __shared__ int s_key[GC::kSlots];
   __shared__ typename GC::val_t s_val[GC::kVals];
@@ -1570,27 +1568,27 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
 #define MDG_INSTANTIATE_OCC(R, TH, OCC)                                                                        \
   template __global__ void k_rebo_bonds<R, TH, OCC, false>(Atoms, const TablesT<R>*, const Tables*, Box,          \
                                                            const int2*, const int*, double*, double*, unsigned*, \
-                                                           double*, double*);                                    \
+                                                           double*, long long*);                                    \
   template __global__ void k_bstar<R, TH, OCC, 0, false>(Atoms, const TablesT<R>*, Box, int4*, const int*, int,  \
-                                                         double*, double*, unsigned*, double*, double*);
+                                                         double*, double*, unsigned*, double*, long long*);
 #define MDG_INSTANTIATE_TH(R, TH, OCC, LW)                                                                     \
   template __global__ void k_lj_wave<R, TH, OCC, LW, false>(Atoms, const TablesT<R>*, Box, double*, double*,   \
                                                             int4*, int*, int, long long*, unsigned*, double*,   \
-                                                            double*);
+                                                            long long*);
 #define MDG_INSTANTIATE(R)                                                                                     \
   template __global__ void k_coord<R>(Atoms, const TablesT<R>*, Box, unsigned*);                              \
   template __global__ void k_etors<R, false>(Atoms, const TablesT<R>*, Box, const int2*, const int*, int,     \
-                                             double*, double*, long long*, double*);                          \
+                                             double*, double*, long long*, long long*);                          \
   template __global__ void k_etors<R, true>(Atoms, const TablesT<R>*, Box, const int2*, const int*, int,      \
-                                            double*, double*, long long*, double*);                           \
+                                            double*, double*, long long*, long long*);                           \
   template __global__ void k_rebo_bonds<R, false, 1, true>(Atoms, const TablesT<R>*, const Tables*, Box,      \
                                                            const int2*, const int*, double*, double*,         \
-                                                           unsigned*, double*, double*);                      \
+                                                           unsigned*, double*, long long*);                      \
   template __global__ void k_lj_wave<R, false, 1, 16, true>(Atoms, const TablesT<R>*, Box, double*, double*,  \
                                                             int4*, int*, int, long long*, unsigned*, double*,  \
-                                                            double*);                                          \
+                                                            long long*);                                          \
   template __global__ void k_bstar<R, false, 1, 0, true>(Atoms, const TablesT<R>*, Box, int4*, const int*,    \
-                                                         int, double*, double*, unsigned*, double*, double*);  \
+                                                         int, double*, double*, unsigned*, double*, long long*);  \
   template __global__ void k_pair_rows<R>(Atoms, const TablesT<R>*, Box, const int*, const int*, double,      \
                                           long long*);
 MDG_INSTANTIATE(double)
@@ -1616,10 +1614,10 @@ MDG_INSTANTIATE_OCC(float, false, 4)
 #define MDG_INSTANTIATE_SPLIT(R, OCC)                                                                           \
   template __global__ void k_bstar<R, false, OCC, 1, false>(Atoms, const TablesT<R>*, Box, int4*, const int*,  \
                                                             int, double*, double*, unsigned*, double*,         \
-                                                            double*);                                          \
+                                                            long long*);                                          \
   template __global__ void k_bstar<R, false, OCC, 2, false>(Atoms, const TablesT<R>*, Box, int4*, const int*,  \
                                                             int, double*, double*, unsigned*, double*,         \
-                                                            double*);
+                                                            long long*);
 MDG_INSTANTIATE_SPLIT(double, 1)
 MDG_INSTANTIATE_SPLIT(double, 2)
 MDG_INSTANTIATE_SPLIT(float, 1)

@@ -7,6 +7,22 @@
 
 namespace mdg {
 
+// The packed copy of atom i at (x, y, z): reduced into the box on the
+// periodic axes on which the atom itself is open (packed_coord), else as is.
+__device__ __forceinline__ double4 packed_pos(const Atoms& A, int i, double x, double y, double z) {
+  if (A.open_axes) {
+    const unsigned m = A.open_axes[i];
+    if (m) {
+      const double* L = A.boxL + 3 * A.sys[i];
+      if (m & 1u) x = packed_coord(x, L[0]);
+      if (m & 2u) y = packed_coord(y, L[1]);
+      if (m & 4u) z = packed_coord(z, L[2]);
+    }
+  }
+  return make_double4(x, y, z, double(A.type[i]));
+}
+
+
 // ----------------------------------------------------------------- helpers
 
 template <typename T>
@@ -126,7 +142,7 @@ __global__ void k_cell_fill(const int* __restrict__ flag, const int* __restrict_
 // Only atoms whose system asked for a rebuild (sflags[2*s+list]) are redone;
 // the grid strides over the atoms, so a launch with nothing to rebuild costs
 // a few thousand empty waves, not one per atom.
-template <int CAP, bool SORT>
+template <int CAP, bool SORT, bool APBC>
 __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, const int* __restrict__ sflags,
                                                   int list_id, int n, const double4* __restrict__ pos,
                                                   const unsigned char* __restrict__ handled, Box bx,
@@ -135,7 +151,8 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
                                                   const int* __restrict__ cell_of, const int* __restrict__ start,
                                                   const int* __restrict__ cell_atoms, int* __restrict__ cnt,
                                                   int* __restrict__ list, double* __restrict__ disp, unsigned* err,
-                                                  int* __restrict__ split) {
+                                                  int* __restrict__ split, const double* __restrict__ xs,
+                                                  const double* __restrict__ ys, const double* __restrict__ zs) {
   if (!*flag) return;
   __shared__ int buf[4][CAP];
   __shared__ int s_pre[4][32], s_st[4][32];  // stencil cells: first candidate, first atom
@@ -143,8 +160,10 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
   for (int i = blockIdx.x * 4 + w; i < n; i += gridDim.x * 4) {
     int sy = bx.sys[i];
     if (!sflags[2 * sy + list_id]) continue;  // wave-uniform
+    // APBC: the systems whose atoms carry their own PBC vectors, and only those
+    if (APBC != (bx.apbc != nullptr && bx.apbc[sy] != 0)) continue;
     const CellGrid& g = grids[sy];
-    const double* L = box_of(bx, i);
+    const Frame fr = frame_of(bx, i);
     int base_cell = cell_off[sy];
     int* my = buf[w];
     if (lane == 0) {
@@ -155,7 +174,6 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
     int m = 0;
     if (handled[i]) {
       double4 pi = pos[i];
-      double xi = pi.x, yi = pi.y, zi = pi.z;
       int ci = cell_of[i] - base_cell;
       int c2 = ci % g.n[2], c1 = (ci / g.n[2]) % g.n[1], c0 = ci / (g.n[1] * g.n[2]);
       int lo[3], hi[3], cc[3] = {c0, c1, c2};
@@ -216,10 +234,10 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
           j = cell_atoms[s_st[w][lo_c] + (t - s_pre[w][lo_c])];
           if (j != i && handled[j]) {
             double4 pj = pos[j];
-            double dx = mic(xi - pj.x, L[0]);
-            double dy = mic(yi - pj.y, L[1]);
-            double dz = mic(zi - pj.z, L[2]);
-            double d2 = dx * dx + dy * dy + dz * dz;
+            D3 d;
+            if constexpr (APBC) d = depos_atoms(bx, xs, ys, zs, i, j);  // each atom's own PBC
+            else d = mdg::disp(bx, fr, i, j, pi, pj);  // qualified: the argument disp names the displacements
+            double d2 = d.x * d.x + d.y * d.y + d.z * d.z;
             ok = d2 < range2;
           }
         }
@@ -297,18 +315,17 @@ __global__ void __launch_bounds__(256) k_nl_build(const int* __restrict__ flag, 
   }
 }
 
-template __global__ void k_nl_build<64, true>(const int*, const int*, int, int, const double4*,
-                                              const unsigned char*, Box, const CellGrid*, const int*, double,
-                                              const int*, const int*, const int*, int*, int*, double*, unsigned*,
-                                              int*);
-template __global__ void k_nl_build<512, true>(const int*, const int*, int, int, const double4*,
-                                               const unsigned char*, Box, const CellGrid*, const int*, double,
-                                               const int*, const int*, const int*, int*, int*, double*, unsigned*,
-                                               int*);
-template __global__ void k_nl_build<512, false>(const int*, const int*, int, int, const double4*,
-                                                const unsigned char*, Box, const CellGrid*, const int*, double,
-                                                const int*, const int*, const int*, int*, int*, double*, unsigned*,
-                                                int*);
+#define MDG_NL_BUILD(CAP, SORT, APBC)                                                                          \
+  template __global__ void k_nl_build<CAP, SORT, APBC>(                                                      \
+      const int*, const int*, int, int, const double4*, const unsigned char*, Box, const CellGrid*, const int*, \
+      double, const int*, const int*, const int*, int*, int*, double*, unsigned*, int*, const double*,         \
+      const double*, const double*);
+MDG_NL_BUILD(64, true, false)
+MDG_NL_BUILD(512, true, false)
+MDG_NL_BUILD(512, false, false)
+MDG_NL_BUILD(64, true, true)
+MDG_NL_BUILD(512, true, true)
+MDG_NL_BUILD(512, false, true)
 
 // ------------------------------------------------------------ skin test
 // Top-2 accumulated displacement per system and list (NeighbourList.cxx:
@@ -464,7 +481,7 @@ __global__ void __launch_bounds__(256) k_metal_rows(Atoms A, Box bx, const int* 
   const int cnt = A.nll_cnt[i];
   const int* L = A.nll + (size_t)i * A.cap_l;
   int* my = A.ml + (size_t)i * kMetalRow;
-  const double* Lb = box_of(bx, i);
+  const Frame fr = frame_of(bx, i);
   const double rc2 = rcand * rcand;
   int m = 0;
   for (int b0 = 0; b0 < cnt; b0 += LW) {  // group-uniform trip count
@@ -475,7 +492,8 @@ __global__ void __launch_bounds__(256) k_metal_rows(Atoms A, Box bx, const int* 
       j = L[q];
       const double4 pj = A.p[j];
       if ((int)pj.w == T_OTHER) {
-        const double dx = mic(pi.x - pj.x, Lb[0]), dy = mic(pi.y - pj.y, Lb[1]), dz = mic(pi.z - pj.z, Lb[2]);
+        const D3 dd = disp(bx, fr, i, j, pi, pj);
+        const double dx = dd.x, dy = dd.y, dz = dd.z;
         in = dx * dx + dy * dy + dz * dz < rc2;
       }
     }
@@ -538,7 +556,7 @@ __device__ __forceinline__ bool zbl_ion(int elem) { return elem == 40 || elem ==
 template <bool VIR>
 __global__ void __launch_bounds__(256) k_fbzl(Atoms A, Box bx, const int* __restrict__ ions, int nions, double R0,
                                               double R1, double* __restrict__ grad, double* __restrict__ eatom,
-                                              double* __restrict__ vir) {
+                                              long long* __restrict__ vir) {
   const int w = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6);
   const int lane = threadIdx.x & 63;
   if (w >= nions) return;  // wave-uniform
@@ -581,8 +599,8 @@ __global__ void __launch_bounds__(256) k_fbzl(Atoms A, Box bx, const int* __rest
   if (lane == 0) energy_add(A, eatom, i, e);
   vw.flush_wave();
 }
-template __global__ void k_fbzl<false>(Atoms, Box, const int*, int, double, double, double*, double*, double*);
-template __global__ void k_fbzl<true>(Atoms, Box, const int*, int, double, double, double*, double*, double*);
+template __global__ void k_fbzl<false>(Atoms, Box, const int*, int, double, double, double*, double*, long long*);
+template __global__ void k_fbzl<true>(Atoms, Box, const int*, int, double, double, double*, double*, long long*);
 
 // FLJ::operator() (FLJ.cxx:256-341) for the pairs {H,C} x {Cu,Au}: each metal
 // atom takes its hydrocarbon partners from the long list (so every pair once);
@@ -601,7 +619,7 @@ __device__ __forceinline__ int flj_index(int elem) {
 
 template <bool VIR>
 __global__ void k_flj(Atoms A, Box bx, const FLJTable* __restrict__ T, double* __restrict__ grad,
-                      double* __restrict__ eatom, double* __restrict__ vir) {
+                      double* __restrict__ eatom, long long* __restrict__ vir) {
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= A.n) return;
   const int em = A.elem[m];
@@ -660,8 +678,8 @@ __global__ void k_flj(Atoms A, Box bx, const FLJTable* __restrict__ T, double* _
   }
   vw.flush();
 }
-template __global__ void k_flj<false>(Atoms, Box, const FLJTable*, double*, double*, double*);
-template __global__ void k_flj<true>(Atoms, Box, const FLJTable*, double*, double*, double*);
+template __global__ void k_flj<false>(Atoms, Box, const FLJTable*, double*, double*, long long*);
+template __global__ void k_flj<true>(Atoms, Box, const FLJTable*, double*, double*, long long*);
 
 // ------------------------------------------------------------ Ackland EAM
 // Ackland::operator() (Ackland.cxx:31-57): E = sum_i F(rho_i) + sum_{i<j} V,
@@ -819,7 +837,7 @@ __global__ void __launch_bounds__(256) k_eam_density(Atoms A, Box bx, const type
   }
   int* my = row + (size_t)i * kEamRow;
   const double4 pi = A.p[i];
-  const double* Lb = box_of(bx, i);
+  const Frame fr = frame_of(bx, i);
   const double rmax = M::rmax(T);
   double rho = 0.0;
   int m = 0;
@@ -831,7 +849,8 @@ __global__ void __launch_bounds__(256) k_eam_density(Atoms A, Box bx, const type
       j = L[q];
       const double4 pj = A.p[j];
       if ((int)pj.w == T_OTHER) {  // the metals are the atoms of type T_OTHER
-        const double dx = mic(pi.x - pj.x, Lb[0]), dy = mic(pi.y - pj.y, Lb[1]), dz = mic(pi.z - pj.z, Lb[2]);
+        const D3 dd = disp(bx, fr, i, j, pi, pj);
+        const double dx = dd.x, dy = dd.y, dz = dd.z;
         const double r = sqrt(dx * dx + dy * dy + dz * dz);
         if (r < rmax) {
           const int ej = ek[j];
@@ -871,7 +890,7 @@ __global__ void __launch_bounds__(256) k_eam_force(Atoms A, Box bx, const typena
                                                    const signed char* __restrict__ ek,
                                                    const double2* __restrict__ fe, const int* __restrict__ row,
                                                    const int* __restrict__ rcnt, double* __restrict__ grad,
-                                                   double* __restrict__ eatom, double* __restrict__ vir) {
+                                                   double* __restrict__ eatom, long long* __restrict__ vir) {
   constexpr int LW = 16;
   const int sl = threadIdx.x % LW;
   const int i = blockIdx.x * (blockDim.x / LW) + threadIdx.x / LW;
@@ -881,7 +900,7 @@ __global__ void __launch_bounds__(256) k_eam_force(Atoms A, Box bx, const typena
   const int cnt = rcnt[i];
   const int* L = row + (size_t)i * kEamRow;
   const double4 pi = A.p[i];
-  const double* Lb = box_of(bx, i);
+  const Frame fr = frame_of(bx, i);
   const double fpi = fe[i].x;
   double gx = 0.0, gy = 0.0, gz = 0.0, ev = 0.0;
   Virial<VIR> vw(vir, 1.0);
@@ -890,7 +909,8 @@ __global__ void __launch_bounds__(256) k_eam_force(Atoms A, Box bx, const typena
     const int j = L[q];
     const double4 pj = A.p[j];
     const double2 fj = fe[j];  // F'(rho_j) and j's element
-    const double dx = mic(pi.x - pj.x, Lb[0]), dy = mic(pi.y - pj.y, Lb[1]), dz = mic(pi.z - pj.z, Lb[2]);
+    const D3 dd = disp(bx, fr, i, j, pi, pj);
+        const double dx = dd.x, dy = dd.y, dz = dd.z;
     const double r = sqrt(dx * dx + dy * dy + dz * dz);
     const int ej = (int)fj.y;
     double v, dv, g, dg;
@@ -921,18 +941,18 @@ template __global__ void k_eam_density<AcklandModel>(Atoms, Box, const EAMTable*
                                                      double*, int*, int*, unsigned*);
 template __global__ void k_eam_force<AcklandModel, false>(Atoms, Box, const EAMTable*, const signed char*,
                                                           const double2*, const int*, const int*, double*, double*,
-                                                          double*);
+                                                          long long*);
 template __global__ void k_eam_force<AcklandModel, true>(Atoms, Box, const EAMTable*, const signed char*,
                                                          const double2*, const int*, const int*, double*, double*,
-                                                         double*);
+                                                         long long*);
 template __global__ void k_eam_density<TBModel>(Atoms, Box, const TBTable*, const signed char*, double2*, double*,
                                                 int*, int*, unsigned*);
 template __global__ void k_eam_force<TBModel, false>(Atoms, Box, const TBTable*, const signed char*,
                                                      const double2*, const int*, const int*, double*, double*,
-                                                     double*);
+                                                     long long*);
 template __global__ void k_eam_force<TBModel, true>(Atoms, Box, const TBTable*, const signed char*,
                                                     const double2*, const int*, const int*, double*, double*,
-                                                    double*);
+                                                    long long*);
 
 // FBM::operator() (FBM.cxx:70-94) for the pairs Cu-Cu and Cu-Ar: 16 lanes
 // per Cu or Ar atom gather its handled partners from the long list (both
@@ -943,7 +963,7 @@ __device__ __forceinline__ int fbm_index(int elem) { return elem == 64 ? 0 : ele
 template <bool VIR>
 __global__ void __launch_bounds__(256) k_fbm(Atoms A, Box bx, const FBMTable* __restrict__ T,
                                              double* __restrict__ grad, double* __restrict__ eatom,
-                                             double* __restrict__ vir) {
+                                             long long* __restrict__ vir) {
   constexpr int LW = 16;
   const int sl = threadIdx.x % LW;
   const int i = blockIdx.x * (blockDim.x / LW) + threadIdx.x / LW;
@@ -953,7 +973,7 @@ __global__ void __launch_bounds__(256) k_fbm(Atoms A, Box bx, const FBMTable* __
   const int cnt = A.nll_cnt[i];
   const int* L = A.nll + (size_t)i * A.cap_l;
   const double4 pi = A.p[i];
-  const double* Lb = box_of(bx, i);
+  const Frame fr = frame_of(bx, i);
   const double R0 = T->R0, R1 = T->R1;
   double gx = 0.0, gy = 0.0, gz = 0.0, ev = 0.0;
   Virial<VIR> vw(vir, 1.0);
@@ -961,7 +981,8 @@ __global__ void __launch_bounds__(256) k_fbm(Atoms A, Box bx, const FBMTable* __
   for (int q = sl; q < cnt; q += LW) {
     const int j = L[q];
     const double4 pj = A.p[j];
-    const double dx = mic(pi.x - pj.x, Lb[0]), dy = mic(pi.y - pj.y, Lb[1]), dz = mic(pi.z - pj.z, Lb[2]);
+    const D3 dd = disp(bx, fr, i, j, pi, pj);
+        const double dx = dd.x, dy = dd.y, dz = dd.z;
     const double d2 = dx * dx + dy * dy + dz * dz;
     if (d2 > R1 * R1) continue;  // FPairwise::probablyAreNeighbours
     const int ej = fbm_index(A.elem[j]);
@@ -990,8 +1011,8 @@ __global__ void __launch_bounds__(256) k_fbm(Atoms A, Box bx, const FBMTable* __
     eatom[i] = ev;
   }
 }
-template __global__ void k_fbm<false>(Atoms, Box, const FBMTable*, double*, double*, double*);
-template __global__ void k_fbm<true>(Atoms, Box, const FBMTable*, double*, double*, double*);
+template __global__ void k_fbm<false>(Atoms, Box, const FBMTable*, double*, double*, long long*);
+template __global__ void k_fbm<true>(Atoms, Box, const FBMTable*, double*, double*, long long*);
 
 // ------------------------------------------------------------ Verlet
 // eq.1 of SimLoop::executeMain (SimLoop.cxx:323-332): x += v dt + a dt^2/2,
@@ -1021,7 +1042,7 @@ __device__ __forceinline__ void drift_atom(const Atoms& A, int i, double dt, boo
     A.x[i] = x;
     A.y[i] = y;
     A.z[i] = z;
-    A.p[i] = make_double4(x, y, z, double(A.type[i]));
+    A.p[i] = packed_pos(A, i, x, y, z);
     sx += dx;
     sy += dy;
     sz += dz;
@@ -1121,9 +1142,8 @@ __global__ void __launch_bounds__(256) k_bath_temp_part(Atoms A, Box bx, const i
   const Bath b = A.bath[s];
   double e = 0.0, cnt = 0.0;
   if (b.geom != TB_NONE) {
-    const double* L = bx.L + 3 * s;
     for (int i = t.y + threadIdx.x; i < tile_end(t, A.sys_off); i += blockDim.x) {
-      if (A.fixed[i] || !A.tb_apply[i] || !bath_inside(b, A.x[i], A.y[i], A.z[i], L)) continue;
+      if (A.fixed[i] || !A.tb_apply[i] || !bath_inside(b, A.x[i], A.y[i], A.z[i], own_pbc(bx, i).v)) continue;
       double vv = sqrt(A.vx[i] * A.vx[i] + A.vy[i] * A.vy[i] + A.vz[i] * A.vz[i]);
       e += A.mass[i] * (vv * vv) / 2.0;
       cnt += 1.0;
@@ -1158,7 +1178,8 @@ __global__ void __launch_bounds__(64) k_bath_temp_merge(Atoms A, const int* __re
 // from the bath (0 outside it).
 __device__ __forceinline__ void kick_atom(const Atoms& A, const Box& bx, int i, double dt, double gx, double gy,
                                           double gz, double& vx, double& vy, double& vz, double& dE) {
-  const double* L = box_of(bx, i);
+  const Pbc3 own = own_pbc(bx, i);  // the atom's own PBC (Atom::applyPBC wraps by it)
+  const double* L = own.v;
   double x = A.x[i], y = A.y[i], z = A.z[i];
   dE = 0.0;
   if (!A.fixed[i]) {
@@ -1220,7 +1241,7 @@ __device__ __forceinline__ void kick_atom(const Atoms& A, const Box& bx, int i, 
   A.pcx[i] = cx;
   A.pcy[i] = cy;
   A.pcz[i] = cz;
-  A.p[i] = make_double4(x, y, z, double(A.type[i]));
+  A.p[i] = packed_pos(A, i, x, y, z);
 }
 
 // The kick of one tile's atoms and the tile's partial step outputs: the
@@ -1273,7 +1294,7 @@ __global__ void __launch_bounds__(256) k_kick_tile(Atoms A, Box bx, const int2* 
 // Refreshes the packed positions after the host wrote x, y, z.
 __global__ void k_pack_pos(Atoms A) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < A.n) A.p[i] = make_double4(A.x[i], A.y[i], A.z[i], double(A.type[i]));
+  if (i < A.n) A.p[i] = packed_pos(A, i, A.x[i], A.y[i], A.z[i]);
 }
 
 // First force evaluation (iteration 0, SimLoop.cxx:305-321): a = -grad/M,
@@ -1445,7 +1466,7 @@ __global__ void __launch_bounds__(256) k_long_nbflag(const int* __restrict__ fla
     const int cnt = A.nll_cnt[i];
     const int* L = A.nll + (size_t)i * A.cap_l;
     const double4 pi = A.p[i];
-    const double* Lb = box_of(bx, i);
+    const Frame fr = frame_of(bx, i);
     for (int b0 = 0; b0 < cnt && !found; b0 += LW) {
       const int q = b0 + sl;
       bool hit = false;
@@ -1453,7 +1474,8 @@ __global__ void __launch_bounds__(256) k_long_nbflag(const int* __restrict__ fla
         const int j = L[q];
         const unsigned ej = elem_bit(A.elem[j]);
         const double4 pj = A.p[j];
-        const double dx = mic(pi.x - pj.x, Lb[0]), dy = mic(pi.y - pj.y, Lb[1]), dz = mic(pi.z - pj.z, Lb[2]);
+        const D3 dd = disp(bx, fr, i, j, pi, pj);
+        const double dx = dd.x, dy = dd.y, dz = dd.z;
         const double d2 = dx * dx + dy * dy + dz * dz;
         for (int m = 0; m < lm.m; m++)
           if (((mine >> m) & 1u) && (lm.handled[m] & ej) && d2 < lm.r2[m]) hit = true;
@@ -1485,6 +1507,12 @@ __global__ void k_step_virial(int nsys, const double* __restrict__ vir, double* 
                               const int* __restrict__ stepidx) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q < 6 * nsys) stepvir[(size_t)(*stepidx) * 6 * nsys + q] = vir[q];
+}
+
+// The virial of a force call from its fixed-point sums (erg).
+__global__ void k_vir_join(int nsys, const long long* __restrict__ fix, double* __restrict__ vir) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < 6 * nsys) vir[q] = fix_join(fix[2 * q], fix[2 * q + 1]) * (1.0 / kFixEnergy);
 }
 
 __global__ void k_step_advance(int* __restrict__ stepidx) {

@@ -79,6 +79,11 @@ struct Atoms {
   // (FProxy::hasNeighbors, FProxy.hpp:99-104, for those members), refreshed
   // at each long-list build; null without long-range members.
   unsigned char* nbl;
+  // Mixed systems (atoms open on a periodic axis of their box, mdg_set_atom_pbc):
+  // per atom the bits of those axes, null when no system is mixed; boxL = the
+  // systems' boxes (3 per system).
+  const unsigned char* open_axes;
+  const double* boxL;
 };
 
 // The long-range chain members as FProxy::hasNeighbors sees them: member q
@@ -127,11 +132,15 @@ __global__ void k_cell_scan(const int* flag, const int* sflags, int list_id, con
                             int* cnt, int* start);
 __global__ void k_cell_fill(const int* flag, const int* sflags, int list_id, int n, const int* sys,
                             const int* cell_of, const int* start, int* cursor, int* cell_atoms);
-template <int CAP, bool SORT>
+// APBC: the instantiation for the systems whose atoms carry their own PBC
+// vectors (every pair of such a system is a candidate, tested with depos of
+// the true coordinates xs/ys/zs); the other one skips those systems.
+template <int CAP, bool SORT, bool APBC>
 __global__ void k_nl_build(const int* flag, const int* sflags, int list_id, int n, const double4* p,
                            const unsigned char* handled, Box bx, const CellGrid* grids,
                            const int* cell_off, double range2, const int* cell_of, const int* start,
-                           const int* cell_atoms, int* cnt, int* list, double* disp, unsigned* err, int* split);
+                           const int* cell_atoms, int* cnt, int* list, double* disp, unsigned* err, int* split,
+                           const double* xs, const double* ys, const double* zs);
 // Per-system reductions are tiled: tile t = rb[t] = {system, first atom},
 // at most kRedTile atoms, one 256-thread block each, writing 8 doubles to
 // part[8 t ..]; a 64-thread block per system then merges the tiles
@@ -161,7 +170,7 @@ __global__ void k_coord(Atoms A, const TablesT<R>* T, Box bx, unsigned* err);
 template <class R, bool TH, int OCC, bool VIR>
 __global__ void k_rebo_bonds(Atoms A, const TablesT<R>* T, const Tables* T64, Box bx, const int2* bonds,
                              const int* nbonds, double* grad, double* eatom, unsigned* err, double* dth,
-                             double* vir);
+                             long long* vir);
 // The b* work queue is kQueues sub-queues of qsub entries each; sub-queue q
 // holds entries [q * qsub, (q + 1) * qsub) and its count sits at
 // qcount[q * kQStride], one count per 128-byte line, so the LJ kernel's
@@ -180,18 +189,18 @@ __global__ void k_pair_rows(Atoms A, const TablesT<R>* T, Box bx, const int* pfl
 // of the kernel's terms into vir[6 s + c] (xx, yy, zz, xy, xz, yz; erg).
 template <class R, bool TH, int OCC, int LW, bool VIR>
 __global__ void k_lj_wave(Atoms A, const TablesT<R>* T, Box bx, double* grad, double* eatom, int4* queue,
-                          int* qcount, int qsub, long long* npairs, unsigned* err, double* dth, double* vir);
+                          int* qcount, int qsub, long long* npairs, unsigned* err, double* dth, long long* vir);
 template <class R, bool TH, int OCC, int PASS, bool VIR>
 __global__ void k_bstar(Atoms A, const TablesT<R>* T, Box bx, int4* queue, const int* qcount, int qsub,
-                        double* grad, double* eatom, unsigned* err, double* dth, double* vir);
+                        double* grad, double* eatom, unsigned* err, double* dth, long long* vir);
 template <class R, bool VIR>
 __global__ void k_etors(Atoms A, const TablesT<R>* T, Box bx, const int2* bonds, const int* nbonds, int compat,
-                        double* grad, double* eatom, long long* nquads, double* vir);
+                        double* grad, double* eatom, long long* nquads, long long* vir);
 template <bool VIR>
 __global__ void k_fbzl(Atoms A, Box bx, const int* ions, int nions, double R0, double R1, double* grad, double* eatom,
-                       double* vir);
+                       long long* vir);
 template <bool VIR>
-__global__ void k_flj(Atoms A, Box bx, const FLJTable* T, double* grad, double* eatom, double* vir);
+__global__ void k_flj(Atoms A, Box bx, const FLJTable* T, double* grad, double* eatom, long long* vir);
 // Ackland EAM: the density pass (F'(rho) into fe, F into eatom), then the
 // force pass (gradient and pair energies); 16 lanes per atom, blocks of 256.
 // ek: per atom the EAM element index (Cu 0, Ag 1, Au 2) or -1; fe: per atom
@@ -223,13 +232,13 @@ __global__ void k_eam_density(Atoms A, Box bx, const typename M::Table* T, const
                               double* eatom, int* row, int* rcnt, unsigned* err);
 template <class M, bool VIR>
 __global__ void k_eam_force(Atoms A, Box bx, const typename M::Table* T, const signed char* ek, const double2* fe,
-                            const int* row, const int* rcnt, double* grad, double* eatom, double* vir);
+                            const int* row, const int* rcnt, double* grad, double* eatom, long long* vir);
 // FBM pairs (Cu-Cu, Cu-Ar), 16 lanes per Cu or Ar atom, blocks of 256.
 template <bool VIR>
-__global__ void k_fbm(Atoms A, Box bx, const FBMTable* T, double* grad, double* eatom, double* vir);
+__global__ void k_fbm(Atoms A, Box bx, const FBMTable* T, double* grad, double* eatom, long long* vir);
 template <bool VIR>
 __global__ void k_brenner_bonds(Atoms A, const Tables* T, Box bx, const int2* bonds, const int* nbonds,
-                                double* grad, double* eatom, unsigned* err, double* vir);
+                                double* grad, double* eatom, unsigned* err, long long* vir);
 // Verlet halves over the reduction tiles: the drift writes the skin test's
 // tile part (as k_sys_disp_part), the kick the step outputs' tile part
 // (merged by k_sys_step_merge).
@@ -251,6 +260,7 @@ __global__ void k_bath_temp_part(Atoms A, Box bx, const int2* rb, double* part);
 __global__ void k_bath_temp_merge(Atoms A, const int* rb_off, const double* part);
 __global__ void k_fix_join(Atoms A, int nterms, unsigned fixmask, double* eatom);
 __global__ void k_step_virial(int nsys, const double* vir, double* stepvir, const int* stepidx);
+__global__ void k_vir_join(int nsys, const long long* fix, double* vir);
 __global__ void k_step_advance(int* stepidx);
 
 // Trajectory output captured inside the step (the layouts of mdg_frame and

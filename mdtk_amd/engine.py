@@ -62,6 +62,7 @@ def load_library(path: str = LIB_PATH):
     L.mdg_set_systems.argtypes = [vp, ctypes.c_int, ip, ip, dp, dp, dp, ctypes.POINTER(ctypes.c_uint8)]
     L.mdg_num_systems.argtypes = [vp]
     L.mdg_set_positions.argtypes = [vp, dp, ip, dp]
+    L.mdg_set_atom_pbc.argtypes = [vp, dp, dp, ip]
     L.mdg_get_params.argtypes = [vp, dp]
     L.mdg_set_params.argtypes = [vp, dp]
     L.mdg_energy_grad_params.argtypes = [vp, dp, dp, dp]
@@ -324,6 +325,15 @@ class Engine:
         c = None if pbc_count is None else np.ascontiguousarray(pbc_count, np.int32).reshape(-1)
         d = None if dr_cm is None else np.ascontiguousarray(dr_cm, np.float64).reshape(-1)
         self._chk(_lib.mdg_set_positions(self._h, _p(x), _p(c, ctypes.c_int32), _p(d)))
+
+    def set_atom_pbc(self, pbc_cm, xyz_cm, pbc_count=None):
+        """Each atom's own PBC vector (n x 3, cm; NO_PBC for an open axis),
+        with the positions and image counters the caller's atoms hold
+        (Atom::PBC, Atom::PBC_count); every list is rebuilt."""
+        p = np.ascontiguousarray(pbc_cm, np.float64).reshape(-1)
+        x = np.ascontiguousarray(xyz_cm, np.float64).reshape(-1)
+        c = None if pbc_count is None else np.ascontiguousarray(pbc_count, np.int32).reshape(-1)
+        self._chk(_lib.mdg_set_atom_pbc(self._h, _p(p), _p(x), _p(c, ctypes.c_int32)))
 
     def neighbour_lists(self, term: int):
         counts = np.zeros(self.n, np.int32)

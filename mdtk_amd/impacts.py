@@ -248,10 +248,15 @@ def _start(d: str, label: int, fallback: ST.State | None = None) -> _Traj:
 
 
 def _box(st: ST.State) -> np.ndarray:
-    en = np.asarray(st.pbc_enabled, bool)
-    if st.n and not (en == en[0]).all():
-        raise ValueError("atoms with different PBC axes in one state: the engine takes one box per system")
-    return np.where(en[0] if st.n else np.zeros(3, bool), st.pbc, ST.NO_PBC)
+    """The system's box: the state's box on every axis some atom has enabled."""
+    en = np.asarray(st.pbc_enabled, bool).reshape(-1, 3)
+    return np.where(en.any(axis=0) if st.n else np.zeros(3, bool), st.pbc, ST.NO_PBC)
+
+
+def _atom_pbc(st: ST.State) -> np.ndarray:
+    """Each atom's own PBC vector (Atom::PBC): the box on its enabled axes."""
+    en = np.asarray(st.pbc_enabled, bool).reshape(-1, 3)
+    return np.where(en, st.pbc[None, :], ST.NO_PBC)
 
 
 def _snap_tag(st: ST.State) -> np.ndarray:
@@ -277,7 +282,12 @@ def run_trajectories(trajs: list[_Traj], potentials=CHAIN, precision: str = "fp6
     eng.set_systems([(s.elem, s.pos, s.vel, _box(s), s.tags.get("fixed", np.zeros(s.n, bool)).astype(np.uint8))
                      for s in sts])
     off = eng.offsets
-    eng.set_positions(np.concatenate([s.pos for s in sts]), pbc_count=np.concatenate([s.pbc_count for s in sts]))
+    pos = np.concatenate([s.pos for s in sts])
+    cnt = np.concatenate([s.pbc_count for s in sts])
+    if any(s.n and not (np.asarray(s.pbc_enabled, bool) == np.asarray(s.pbc_enabled, bool)[0]).all() for s in sts):
+        eng.set_atom_pbc(np.concatenate([_atom_pbc(s) for s in sts]), pos, cnt)  # per-atom PBC in some state
+    else:
+        eng.set_positions(pos, pbc_count=cnt)
     b = sts[0].bath
     if b and b["geometry"] != 0:
         eng.set_thermal_bath(geometry=b["geometry"], To=b["To"], gamma=b["gamma"], z_min=b["z_min"],

@@ -193,17 +193,19 @@ def engine_from_state(st: State, clock: bool = True, **engine_kw):
     counters), velocities, fixed atoms, the thermal bath and its per-atom
     opt-in, and (clock=True) SimLoop's clock from the .time record: simTime,
     simTimeFinal and dt, so that run() stops where the reference's loop would.
-    The engine keeps one box per system, so every atom must have the same PBC
-    axes enabled.  Accelerations start from the forces at the loaded
+    Each atom's PBC is the box on its enabled axes and NO_PBC on the others
+    (Atom::PBC as SimLoopSaver stores it); atoms whose axes differ take
+    mdg_set_atom_pbc.  Accelerations start from the forces at the loaded
     positions (the reference's iteration 0)."""
     from .engine import Engine
-    en = np.asarray(st.pbc_enabled, bool)
-    if st.n and not (en == en[0]).all():
-        raise ValueError("atoms with different PBC axes in one state: the engine takes one box per system")
-    pbc = np.where(en[0] if st.n else np.zeros(3, bool), st.pbc, NO_PBC)
+    en = np.asarray(st.pbc_enabled, bool).reshape(-1, 3)
+    pbc = np.where(en.any(axis=0) if st.n else np.zeros(3, bool), st.pbc, NO_PBC)
     eng = Engine(**engine_kw)
     eng.set_system(st.elem, st.pos, st.vel, pbc, st.tags.get("fixed", np.zeros(st.n, bool)).astype(np.uint8))
-    eng.set_positions(st.pos, pbc_count=st.pbc_count)
+    if st.n and not (en == en[0]).all():
+        eng.set_atom_pbc(np.where(en, st.pbc[None, :], NO_PBC), st.pos, st.pbc_count)
+    else:
+        eng.set_positions(st.pos, pbc_count=st.pbc_count)
     b = st.bath
     if b and b["geometry"] != 0:
         eng.set_thermal_bath(geometry=b["geometry"], To=b["To"], gamma=b["gamma"], z_min=b["z_min"],

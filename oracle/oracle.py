@@ -62,6 +62,7 @@ def lib():
         L.orc_moved_too_much.argtypes = [ctypes.c_int, dp, ctypes.c_double, ip]
         L.orc_cij_path.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, dp, ip, ip]
         L.orc_virial.argtypes = [ctypes.c_void_p, dp]
+        L.orc_set_atom_pbc.argtypes = [ctypes.c_void_p, dp, ip]
         _lib = L
     return _lib
 
@@ -180,6 +181,14 @@ class Oracle:
         self._chk(lib().orc_cij_path(self.h, int(i), int(j), _p(w), _p(hops, ctypes.c_int),
                                      _p(nodes, ctypes.c_int)))
         return float(w[0]), [int(q) for q in nodes[:hops[0] + 1]] if hops[0] else []
+
+    def set_atom_pbc(self, pbc_cm, pbc_count=None):
+        """Each atom's own PBC vector (n x 3, cm; NO_PBC for an open axis)
+        and image counters, before the first call (Atom::PBC, PBC_count)."""
+        p = np.ascontiguousarray(pbc_cm, np.float64).reshape(self.n, 3)
+        c = None if pbc_count is None else np.ascontiguousarray(pbc_count, np.int32).reshape(self.n, 3)
+        self._keep += [p, c]
+        self._chk(lib().orc_set_atom_pbc(self.h, _p(p), _p(c, ctypes.c_int)))
 
     def state(self):
         x, v, a = (np.zeros((self.n, 3)) for _ in range(3))

@@ -59,6 +59,7 @@ void* orc_create(int n, const int* elem, const double* xyz, const double* vel, c
       at[i].globalIndex = i;
       at[i].setAttributesByElementID();
     }
+    h->sl.arrayPBC = V3(pbc[0], pbc[1], pbc[2]);
     auto& pots = h->sl.fpot.pots;
     if (pot_mask & POT_FBZL) {
       pots.push_back(new FBZL(fbzl_r0, fbzl_r1));
@@ -281,6 +282,24 @@ int orc_run(void* p, int nsteps, double fixed_dt, double* epot, double* ekin, do
       if (ekin) ekin[s] = sl.Ekin;
       if (dts) dts[s] = dt_used;
       if (netforce) netforce[s] = norm(sl.netForce);
+    }
+  });
+}
+
+// Each atom's own PBC vector and image counters (Atom::PBC, Atom::PBC_count;
+// mdtk/Atom.hpp:35-82), set before the first call: prepare() wraps every
+// atom by its own PBC (AtomsContainer.cxx:106-123) and depos takes both
+// atoms' PBC (Atom.hpp:115-140).  The array box stays the one given to
+// orc_create (checkMIC).
+int orc_set_atom_pbc(void* p, const double* pbc, const int* pbc_count) {
+  Handle* h = static_cast<Handle*>(p);
+  return guarded([&] {
+    if (h->sl.prepared) throw OracleError("orc_set_atom_pbc: set before the first call");
+    auto& at = h->sl.atoms;
+    for (size_t i = 0; i < at.size(); i++) {
+      at[i].PBC = V3(pbc[3 * i], pbc[3 * i + 1], pbc[3 * i + 2]);
+      if (pbc_count)
+        for (int d = 0; d < 3; d++) at[i].pbc_count[d] = pbc_count[3 * i + d];
     }
   });
 }

@@ -494,6 +494,7 @@ struct ThermalBath {
 // mdtk/SimLoop.cxx:225-474 (no file output)
 struct SimLoop {
   AtomsArray atoms;
+  V3 arrayPBC{NO_PBC_L, NO_PBC_L, NO_PBC_L};  // AtomsArray::arrayPBC (AtomsContainer.hpp:34), checkMIC's box
   Proxy fpot;
   ThermalBath bath;
   double energyTransferredFromBath = 0.0;  // SimLoop.hpp:65

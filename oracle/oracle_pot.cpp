@@ -1604,7 +1604,7 @@ void SimLoop::prepare() {  // AtomsContainer.cxx:106-123, SimLoop.cxx:236-250
     if (a.PBC.z != NO_PBC_L && (a.coords.z < 0 || a.coords.z >= a.PBC.z)) throw OracleError("Atoms do not fit given PBC cell !");
   }
   double rc2 = fpot.getRcutoff() * 2.0;  // checkMIC, AtomsContainer.cxx:68-78
-  const V3& L = atoms[0].PBC;
+  const V3& L = arrayPBC;
   if (L.x <= rc2 || L.y <= rc2 || L.z <= rc2) throw OracleError("Rcutoff is too large for given PBC !");
   fpot.NL_init(atoms);
   fpot.NL_UpdateIfNeeded(atoms);

@@ -352,3 +352,48 @@ def test_adaptive_dt_rule(v):
     assert bonded == (v < 1e13)
     r1 = o.run(1)
     assert r1["dt"][0] == want
+
+
+def _mixed_oracles():
+    """The mixed-PBC frame three ways: (a) with each atom's own PBC vector,
+    (b) the slab alone, periodic, (c) every atom open at its unwrapped
+    coordinate x + PBC*PBC_count, (d) every atom periodic (the ion too)."""
+    from tests.helpers import mixed_pbc_case
+    sys_, pbc, cnt = mixed_pbc_case()
+    pots = POT_FBZL | POT_REBO | POT_AIREBO | POT_ETORS
+    e, x, v, p, f = sys_.to_cgs()
+    a = Oracle(e, x, v, p, f, pots=pots)
+    a.set_atom_pbc(pbc, cnt)
+    b = Oracle(e[:-1], x[:-1], v[:-1], p, f[:-1], pots=POT_REBO | POT_AIREBO | POT_ETORS)
+    xu = x + np.where(pbc != S.NO_PBC * ANG, pbc, 0.0) * cnt
+    c = Oracle(e, xu, v, OPEN * ANG, f, pots=pots)
+    d = Oracle(e, x, v, p, f, pots=pots)
+    return a, b, c, d
+
+
+def test_mixed_pbc_chain():
+    """Per-atom PBC (Atom.hpp:115-140) through the whole chain: the ion, open
+    on every axis, meets each slab atom at its unwrapped coordinate only --
+    FBZL equals the all-open system's built from x + PBC*PBC_count (a slab
+    atom with PBC_count.x = -1 next to the far face sits beside the ion),
+    and differs from the all-periodic one (the geometry needs the rule);
+    REBO, AIREBO-LJ and ETors see only the periodic slab, as without the
+    ion."""
+    a, b, c, d = _mixed_oracles()
+    ea, ga = a.energy_grad()
+    eb, gb = b.energy_grad()
+    ec, gc = c.energy_grad()
+    ed, _ = d.energy_grad()
+    assert ea[0] != 0.0
+    assert abs(ea[0] - ec[0]) <= 1e-12 * abs(ec[0])
+    assert abs(ea[0] - ed[0]) > 1e-3 * abs(ed[0])
+    np.testing.assert_allclose(ea[1:], eb, rtol=1e-14, atol=0)
+    zbl_a = ga - np.vstack([gb, np.zeros((1, 3))])  # the FBZL part of the gradient
+    oc = Oracle(*c._keep[:3], OPEN * ANG, c._keep[4], pots=POT_FBZL)
+    _, gzc = oc.energy_grad()
+    assert np.abs(zbl_a - gzc).max() <= 1e-9 * np.abs(gzc).max()
+    # the neighbour far across the x face is in the ion's FBZL list only
+    # through its PBC_count
+    nl = a.neighbour_lists(0)[a.n - 1]
+    x, _, _, cnt = a.state()
+    assert any(cnt[j, 0] == -1 for j in nl)
