@@ -86,9 +86,24 @@ class PotentialChain {
         vel[3 * i + 2] = atoms[i].V.z;
         fixed[i] = atoms[i].isFixed() ? 1 : 0;
       }
-      const double pbc[3] = {atoms[0].PBC.x, atoms[0].PBC.y, atoms[0].PBC.z};
+      // the box: per axis the periodic value the atoms carry (the array's
+      // PBC); atoms open on a periodic axis keep their own PBC vectors
+      // through mdg_set_atom_pbc (Atom.hpp:115-140)
+      const double no_pbc = 10000.0 * 1e-8;  // NO_PBC, mdtk/consts.hpp:96
+      double pbc[3] = {no_pbc, no_pbc, no_pbc};
+      std::vector<double> own(3 * n);
+      for (std::size_t i = 0; i < n; i++) {
+        own[3 * i] = atoms[i].PBC.x;
+        own[3 * i + 1] = atoms[i].PBC.y;
+        own[3 * i + 2] = atoms[i].PBC.z;
+        for (int d = 0; d < 3; d++)
+          if (own[3 * i + d] != no_pbc) pbc[d] = own[3 * i + d];
+      }
+      bool per_atom = false;
+      for (std::size_t q = 0; q < 3 * n; q++) per_atom = per_atom || own[q] != pbc[q % 3];
       check(mdg_set_system(ctx_, static_cast<int>(n), elem.data(), xyz_.data(), vel.data(), pbc, fixed.data()));
-      check(mdg_set_positions(ctx_, xyz_.data(), pc_.data(), nullptr));
+      if (per_atom) check(mdg_set_atom_pbc(ctx_, own.data(), xyz_.data(), pc_.data()));
+      else check(mdg_set_positions(ctx_, xyz_.data(), pc_.data(), nullptr));
       dr_.assign(3 * n, 0.0);
       ready_ = true;
     } else {
