@@ -9,7 +9,7 @@ trajectory energies and adaptive-dt bookkeeping):
   open, bottom chain row fixed), 300 K Maxwell-Boltzmann velocities, one Ar
   ion 8 A above the surface at 100 eV, normal incidence, seeded impact point;
   dt = 0.1 fs fixed.  All trajectories are advanced by one hipGraph launch
-  per step, in the mixed FP32/FP64 form.  An untimed approach (set-up,
+  per 32 steps, in the mixed FP32/FP64 form.  An untimed approach (set-up,
   before the warm-up) brings the ions to 2 A above their surfaces, so the
   timed window holds the impacts and their list rebuilds.
 The plain FP64 drop-in runs the same trajectories the way the reference
@@ -365,7 +365,7 @@ def main():
     napp = approach_steps(batch, dt) if a.approach < 0 else a.approach
     log(f"untimed approach: {napp} steps")
 
-    # headline: mixed + batched, one graph launch per step for all trajectories
+    # headline: mixed + batched, one graph launch per 32 steps for all trajectories
     emx = Engine.from_systems(batch, potentials=POTS, etors_compat=False, precision="mixed")
     tune_mx = emx.autotune(reps=2)
     log(f"mixed batched autotune: {tune_mx}")
@@ -437,7 +437,7 @@ def main():
         "dtype": "f32/f64 mixed" if use_mixed else "f64",
         "data": "synthetic",
         "config": {"workload": (f"batched bombardment: {len(batch)} PE slabs ~U[2000,12000] atoms + Ar 100 eV, "
-                                "FBZL+REBO+AIREBO-LJ+ETors, dt 0.1 fs, one hipGraph launch per step"),
+                                "FBZL+REBO+AIREBO-LJ+ETors, dt 0.1 fs, one hipGraph launch per 32 steps"),
                    "timed_window": (f"after an untimed approach of {napp} steps (the ions 2 A above their "
                                     f"surfaces) and {a.warmup} warm-up steps: the impacts"),
                    "trajectories": len(batch), "atoms": natoms,

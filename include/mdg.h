@@ -103,10 +103,11 @@ int mdg_set_positions(mdg_ctx* ctx, const double* xyz, const int32_t* pbc_count,
  * own PBC (Atom::applyPBC, Atom.cxx:93-125; a nonzero counter on an open
  * axis is an error, Atom.cxx:105).  Every pair then takes depos with both
  * atoms' PBC (Atom.hpp:115-140): an open atom sees a periodic one at its
- * unwrapped coordinate, x + PBC*PBC_count.  On each axis of a system the
- * atoms may hold the box's value (mdg_set_system's pbc) and one other (a
- * SimLoopSaver state's per-atom enabled flags give exactly box or NO_PBC);
- * such a system's neighbour lists test all its pairs, as the reference's do.
+ * unwrapped coordinate, x + PBC*PBC_count.  On each axis an atom holds its
+ * system's box value (mdg_set_system's pbc) or NO_PBC, as a SimLoopSaver
+ * state's per-atom enabled flags express it, and a box with open atoms on
+ * an axis must be at least 2 (1 + 0.44) Rc wide there (errors otherwise).
+ * Such a system's neighbour lists test all its pairs, as the reference's do.
  * Call after mdg_set_system(s); every list is rebuilt. */
 int mdg_set_atom_pbc(mdg_ctx* ctx, const double* pbc, const double* xyz, const int32_t* pbc_count);
 
@@ -347,6 +348,14 @@ int mdg_autotune(mdg_ctx* ctx, int reps, double* report);
  * on the same workload.  A knob outside the autotuner's candidates is an
  * error.  No reference counterpart. */
 int mdg_set_tune(mdg_ctx* ctx, const double* knobs);
+/* Steps per graph launch in mdg_step_nve (default 32; 1..4096).  The whole
+ * step of every system lives on the device (skin tests, rebuild flags,
+ * clocks, per-step outputs), so a block of steps is captured into one
+ * hipGraph and one launch advances the batch by that many steps; a chunk's
+ * remainder runs one step per launch.  Results do not depend on it.  No
+ * reference counterpart (SimLoop::executeMain's loop, SimLoop.cxx:252-445,
+ * runs one iteration at a time). */
+int mdg_set_graph_block(mdg_ctx* ctx, int steps);
 
 /* Timing with HIP events on the context's stream: mdg_event_record records
  * event `slot` (0-7); mdg_event_elapsed waits for event b and returns the

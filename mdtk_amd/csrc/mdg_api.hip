@@ -37,6 +37,8 @@ struct Tune {
 };
 }  // namespace
 
+constexpr int kGraphBlock = 32;  // steps per launch of the multi-step graph
+
 struct mdg_ctx {
   mdg_config cfg{};
   std::string err;
@@ -158,7 +160,10 @@ struct mdg_ctx {
   double* ebath = nullptr;  // per system: energy taken from the bath so far (erg)
   bool bath_on = false;     // some system has a bath geometry
   bool use_graphs = true;
-  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // one step, adaptive / fixed dt
+  int graph_block = kGraphBlock;  // steps per captured multi-step graph (1: one step per launch)
+  // [fixed + 2 * block]: one step, or kGraphBlock steps in one graph, with
+  // adaptive / fixed dt
+  hipGraphExec_t graph_exec[4] = {nullptr, nullptr, nullptr, nullptr};
   std::vector<hipGraph_t> graphs;
 
   ~mdg_ctx() {
@@ -1793,22 +1798,33 @@ int mdg_step_nve(mdg_ctx* c, int nsteps, double fixed_dt, double* epot, double* 
     k_step_advance<<<1, 64, 0, c->stream>>>(c->stepidx);
   };
   bool use_graph = !c->profiling && c->use_graphs;
-  if (use_graph && !c->graph_exec[fixed]) {
+  // The whole step lives on the device, so `reps` steps capture into one
+  // graph: one launch advances every system of the batch by kGraphBlock
+  // steps, the rest of a chunk goes one step per launch.
+  auto capture = [&](int idx, int reps) -> int {
+    if (c->graph_exec[idx]) return 0;
     hipGraph_t g = nullptr;
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    body();
+    for (int r = 0; r < reps; r++) body();
     HIPCHK(c, hipStreamEndCapture(c->stream, &g));
-    HIPCHK(c, hipGraphInstantiate(&c->graph_exec[fixed], g, nullptr, nullptr, 0));
+    HIPCHK(c, hipGraphInstantiate(&c->graph_exec[idx], g, nullptr, nullptr, 0));
     c->graphs.push_back(g);
-  }
+    return 0;
+  };
+  const int block = c->graph_block;
   std::vector<double> o((size_t)chunk * nsys * 4);
   for (int s0 = 0; s0 < nsteps; s0 += chunk) {
     int m = std::min(chunk, nsteps - s0);
     k_zero_i<<<1, 64, 0, c->stream>>>(c->stepidx, 1);
     if (c->traj_on) k_zero<<<1, 64, 0, c->stream>>>(reinterpret_cast<double*>(c->tb.cnt), 3);  // all-zero bits
-    for (int s = 0; s < m; s++) {
-      if (use_graph) HIPCHK(c, hipGraphLaunch(c->graph_exec[fixed], c->stream));
-      else body();
+    int s = 0;
+    if (use_graph) {
+      if (block > 1 && m >= block && capture(fixed + 2, block)) return 1;
+      for (; block > 1 && s + block <= m; s += block) HIPCHK(c, hipGraphLaunch(c->graph_exec[fixed + 2], c->stream));
+      if (s < m && capture(fixed, 1)) return 1;
+      for (; s < m; s++) HIPCHK(c, hipGraphLaunch(c->graph_exec[fixed], c->stream));
+    } else {
+      for (; s < m; s++) body();
     }
     if (check_err(c, true)) return 1;  // systems with their own errors stand still, the rest go on
     if (c->traj_on && drain_trajectory(c)) return 1;
@@ -2215,6 +2231,23 @@ int mdg_autotune(mdg_ctx* c, int reps, double* report) {
     for (int q = 0; q < 10; q++) report[q] = r[q];
   }
   return check_err(c);
+}
+
+int mdg_set_graph_block(mdg_ctx* c, int steps) {
+  if (steps < 1 || steps > 4096) {
+    c->err = "mdg_set_graph_block: steps per launch must be in [1, 4096]";
+    return 1;
+  }
+  if (steps != c->graph_block) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 2; k < 4; k++)
+      if (c->graph_exec[k]) {
+        hipGraphExecDestroy(c->graph_exec[k]);
+        c->graph_exec[k] = nullptr;
+      }
+  }
+  c->graph_block = steps;
+  return 0;
 }
 
 int mdg_set_tune(mdg_ctx* c, const double* knobs) {

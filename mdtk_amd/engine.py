@@ -63,6 +63,7 @@ def load_library(path: str = LIB_PATH):
     L.mdg_num_systems.argtypes = [vp]
     L.mdg_set_positions.argtypes = [vp, dp, ip, dp]
     L.mdg_set_atom_pbc.argtypes = [vp, dp, dp, ip]
+    L.mdg_set_graph_block.argtypes = [vp, ctypes.c_int]
     L.mdg_get_params.argtypes = [vp, dp]
     L.mdg_set_params.argtypes = [vp, dp]
     L.mdg_energy_grad_params.argtypes = [vp, dp, dp, dp]
@@ -437,6 +438,10 @@ class Engine:
                 "ms_after", "lj_shape", "xcd_order"]
         r = np.array([float(tune.get(k, 0.0)) for k in keys])
         self._chk(_lib.mdg_set_tune(self._h, _p(r)))
+
+    def set_graph_block(self, steps: int):
+        """Steps per graph launch in run() (default 32; results do not depend on it)."""
+        self._chk(_lib.mdg_set_graph_block(self._h, int(steps)))
 
     def synchronize(self):
         self._chk(_lib.mdg_synchronize(self._h))
