@@ -148,6 +148,72 @@ def valu_profile(kernel, path=VALU_FILE):
         return None
 
 
+# Algorithmic FLOPs per unit of work of each class of the AIREBO path, one
+# forward and one adjoint pass as the kernels compute them (mdg_force.hip),
+# tallied from the source with add/sub/mul/compare = 1, div = 8, sqrt = 8,
+# exp/sin/cos = 20, pow = 40 (no FMA: the kernels build without contraction);
+# integer and address work, loads and atomics are not counted, nor the
+# reference's seeded re-evaluations.  Units and counts of the last force call
+# come from mdg_counters / mdg_work_counts.
+FLOPS_PER_UNIT = {
+    # k_rebo_bonds outside the bond order: displacement and r in FP32 and
+    # FP64 (35), VR with its derivative (55), the three VA exponentials (73),
+    # b VA and its radial derivative (19), the gradient push (17); cutoffs
+    # inside R1 are compares
+    "rebo_pair (bond)": 204,
+    # one (bond, side, neighbour k) of the sigma-pi sums: forward cos(theta)
+    # (18) and G with both quintic pieces and the S'(N) switch (82), D += G f
+    # (3); adjoint cos(theta) and G again (100), d cos / d(r_ij, r_ik) and the
+    # three pushes (101), dG/dN and the scatter (13)
+    "sigma_pi (triplet)": 318,
+    # the rest of one bond order: six coordination exclusions (24), P bicubics
+    # (4/3 per bond in PE, 140 each), D^-1/2 twice (80), the Nconj sums (26),
+    # pi_rc tricubic (600), T_ij tricubic on the C-C third (200), the adjoint
+    # seeds D^-3/2 (88) and the coordination and Nconj derivatives (~190)
+    "bond_order_rest (bond)": 1300,
+    # one ETors quad: the dihedral (two cross products, norms, the cosine and
+    # sines, ~140), V(omega) and the cutoff weights (~40), the adjoint of the
+    # dihedral into four atoms and the weight derivatives (~220)
+    "etors (quad)": 400,
+    # one LJ pair inside the cutoff: the pair-row sweep's distance tests
+    # (15 per row entry, ~1.65 entries per pair, 25), displacement and r
+    # (25), S_r, the cutoff (20 averaged over the 5.0-5.5 A shell),
+    # (sigma/r)^6, ^12, VLJ and its derivative (40), the gradient push and
+    # j-side scatter (20)
+    "lj (pair)": 135,
+    # one queued b* pair in the bound pass, besides its triplets: r, S_r,
+    # C_ij weight, six coordination exclusions, the P bicubics (180), the
+    # bound (37) and the LJ evaluation with S_r S_b and the C_ij adjoint (120)
+    "bstar_bound (pair)": 400,
+    # one b* triplet of the bound pass (forward sigma-pi only)
+    "bstar_triplet (triplet)": 104,
+    # one b* pair given the full bond order in the second pass: the bond
+    # order's remainder and adjoint (1300) plus ~8 adjoint triplets (214 each)
+    "bstar_full (pair)": 3000,
+}
+
+
+def flop_tally(cnt, work, step_ms):
+    """Per-class algorithmic FLOPs of one force call (the last of the window)
+    and the step's rate against the FP32 vector peak (the mixed form's
+    bond-order, LJ and torsion algebra is FP32; its FP64 parts, the pair
+    terms and the sums, are under a tenth of the tally)."""
+    units = {"rebo_pair (bond)": cnt["bonds"], "sigma_pi (triplet)": work["rebo_triplets"],
+             "bond_order_rest (bond)": cnt["bonds"], "etors (quad)": cnt["quads"], "lj (pair)": cnt["lj_pairs"],
+             "bstar_bound (pair)": cnt["bstar"], "bstar_triplet (triplet)": work["bstar_triplets"],
+             "bstar_full (pair)": work["bstar_full"]}
+    rows = {k: {"units": int(units[k]), "flops_per_unit": f, "gflop": units[k] * f / 1e9}
+            for k, f in FLOPS_PER_UNIT.items()}
+    tot = sum(r["gflop"] for r in rows.values()) * 1e9
+    rate = tot / (step_ms / 1e3) / 1e12
+    return {"classes": rows, "gflop_per_step": tot / 1e9, "achieved": rate, "unit": "TFLOP/s",
+            "peak_fp32": 157.3, "frac_fp32": rate / 157.3,
+            "pi_dh": "not tallied: T_ij vanishes for the chains' saturated C-C bonds; the impacts' few "
+                     "unsaturated ones are left out",
+            "weights": "add/mul/compare 1, div 8, sqrt 8, exp/sin/cos 20, pow 40; forward + adjoint; "
+                       "tallied from the kernel source"}
+
+
 def roofline(kt, cnt, natoms, tune):
     """The dominant kernel against MI355X's HBM peak (8 TB/s): algorithmic
     bytes per launch over the launch's average HIP-event duration; beside it
@@ -170,10 +236,13 @@ def roofline(kt, cnt, natoms, tune):
                    note="latency bound: waves wait on dependent list -> position -> bond-record gathers and on "
                         "the LDS gradient cache (see valu.busy)")
         v = valu_profile(name)
+        lj_alg = cnt["lj_pairs"] * FLOPS_PER_UNIT["lj (pair)"] / avg_s / 1e12
         if v:
             f32 = v["fp32_flops"] / avg_s / 1e12
             f64 = v["fp64_flops"] / avg_s / 1e12
-            out["valu"] = {"achieved_fp32": f32, "peak_fp32": 157.3, "frac_fp32": f32 / 157.3,
+            out["valu"] = {"algorithmic": lj_alg, "algorithmic_frac_fp32": lj_alg / 157.3,
+                           "algorithmic_source": "the LJ class of the FLOP tally (flops.classes) over this launch",
+                           "achieved_fp32": f32, "peak_fp32": 157.3, "frac_fp32": f32 / 157.3,
                            "achieved_fp64": f64, "peak_fp64": 78.6, "frac_fp64": f64 / 78.6, "unit": "TFLOP/s",
                            "busy": v["valu_busy"], "lane_utilisation": v["lane_utilisation"],
                            "wait_fraction": v["wait_fraction"],
@@ -383,13 +452,17 @@ def main():
     log(f"mixed batched: {ms_mx / a.steps:.3f} ms/step, {v_mx:.4e} atom-steps/s")
     kt_mx = kernel_times(emx, a.steps, dt)
     cnt_mx = emx.counters()
+    work_mx = emx.work_counts()
+    flops_mx = flop_tally(cnt_mx, work_mx, ms_mx / a.steps)
+    log(f"algorithmic FLOPs: {flops_mx['gflop_per_step']:.2f} GFLOP/step, {flops_mx['achieved']:.2f} TFLOP/s "
+        f"({100 * flops_mx['frac_fp32']:.1f}% of the FP32 vector peak)")
     del emx
     if a.mixed_only:
         print(json.dumps({"metric": METRIC, "value": v_mx, "unit": "atom-steps/s", "n_gpus": 1, "steps": a.steps,
                           "warmup": a.warmup, "ms_per_step": ms_mx / a.steps, "mode": "mixed batched only",
                           "autotune": tune_mx, "roofline": roofline(kt_mx, cnt_mx, natoms, tune_mx),
                           "kernels_ms_per_step": {k: v[0] / max(a.steps, 1) for k, v in kt_mx.items()},
-                          "counters": cnt_mx}), flush=True)
+                          "counters": cnt_mx, "work": work_mx, "flops": flops_mx}), flush=True)
         return
 
     f_rms = force_rms(at_window)
@@ -457,6 +530,8 @@ def main():
         "roofline": rl,
         "kernels_ms_per_step": {k: v[0] / max(a.steps, 1) for k, v in kt_mx.items()},
         "counters": cnt_mx,
+        "work": work_mx,
+        "flops": flops_mx,
         "energy_check_eV": {"max_abs_dE_tot_over_steps": float(np.abs(et - et[0]).max())},
         "rebuilds_in_timed_window": res_mx["rebuilds"],
     }

@@ -372,6 +372,13 @@ int mdg_synchronize(mdg_ctx* ctx);
  * (one per atom and build; the rows are the long-list entries within the LJ
  * cutoff plus a 1 A margin).  No reference counterpart. */
 int mdg_counters(mdg_ctx* ctx, int64_t out[8]);
+/* Work of the last force call for the FLOP tally of the bond-order terms:
+ * out[0] sigma-pi triplets of the REBO bonds (bond, side, neighbour k; per
+ * centre with nb bonds nb (nb - 1)), out[1] sigma-pi triplets of the b*
+ * bound pass (the neighbours of both atoms of every queued pair), out[2]
+ * queued b* pairs whose bound failed (the full bond order and adjoint; the
+ * mixed form's two-pass b*, else 0).  No reference counterpart. */
+int mdg_work_counts(mdg_ctx* ctx, int64_t out[3]);
 
 #ifdef __cplusplus
 }

@@ -138,7 +138,9 @@ struct mdg_ctx {
   double* stepout = nullptr;
   int step_cap = 0;         // steps per output chunk
   long long* counters = nullptr;  // [0] short rebuilds [1] long rebuilds (per system and rebuild)
-  long long* tally = nullptr;     // kTallySlots spread slots: [0] LJ pairs [1] quads [2] pair rows built
+  // kTallySlots spread slots: [0] LJ pairs [1] quads [2] pair rows built [3] REBO sigma-pi triplets
+  // [4] b* bound-pass sigma-pi triplets [5] b* pairs given the full bond order
+  long long* tally = nullptr;
   unsigned* derr = nullptr;  // device error words: [0] global resources, [1 + s] system s
   int* ions = nullptr;       // the atoms the ZBL term centres on (Ar, Xe), ascending
   int nions = 0;
@@ -596,8 +598,7 @@ __global__ void k_fill(double* p, int n, double v) {
 
 // Clears the per-call tallies (LJ pairs, quads) of every slot; blockDim kTallySlots.
 __global__ void k_zero_counts(long long* tally) {
-  tally[threadIdx.x * kTallyStride] = 0;
-  tally[threadIdx.x * kTallyStride + 1] = 0;
+  for (int k : {0, 1, 3, 4, 5}) tally[threadIdx.x * kTallyStride + k] = 0;  // [2] counts across calls
 }
 
 // The bond-order chain members, in the geometry type R of the context.  VIR:
@@ -613,7 +614,7 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
   if (c->have_short) {
     launch(c, "k_coord", [&] { k_coord<R><<<blocks(n, 16), 256, 0, c->stream>>>(A, T, c->box, c->derr); });
     launch(c, "k_bond_enum", [&] {
-      k_bond_enum<<<blocks(n), 256, 0, c->stream>>>(A, c->bonds, c->nbonds, c->bond_cap, c->derr);
+      k_bond_enum<<<blocks(n), 256, 0, c->stream>>>(A, c->bonds, c->nbonds, c->bond_cap, c->derr, c->tally + 3);
     });
   }
   const Tune& tu = c->tune;
@@ -697,13 +698,13 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
       double* eb = c->eatom + (size_t)c->slot_lj * n;
       if constexpr (TH || VIR) {  // one pass
         k_bstar<R, TH, 1, 0, VIR><<<sgrid, 256, 0, c->stream>>>(A, T, c->box, c->queue, c->qcount, c->qsub, A.grad,
-                                                                 eb, c->derr, dth, vir);
+                                                                 eb, c->derr, dth, vir, c->tally + 4);
       } else {
         // the bound pass at the tuned occupancy, then the full pass over the
         // pairs whose bound failed
         auto go = [&](auto occ) {
           k_bstar<R, false, decltype(occ)::value, 1, false><<<sgrid, 256, 0, c->stream>>>(
-              A, T, c->box, c->queue, c->qcount, c->qsub, A.grad, eb, c->derr, dth, nullptr);
+              A, T, c->box, c->queue, c->qcount, c->qsub, A.grad, eb, c->derr, dth, nullptr, c->tally + 4);
         };
         int occ = tu.occ_bstar;
         if constexpr (std::is_same<R, float>::value) {  // 3-4 waves/SIMD: FP32 form only
@@ -715,7 +716,7 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
         if (occ == 2) go(std::integral_constant<int, 2>());
         if (occ == 1) go(std::integral_constant<int, 1>());
         k_bstar<R, false, 1, 2, false><<<sgrid, 256, 0, c->stream>>>(A, T, c->box, c->queue, c->qcount, c->qsub,
-                                                                      A.grad, eb, c->derr, dth, nullptr);
+                                                                      A.grad, eb, c->derr, dth, nullptr, c->tally + 4);
       }
     });
   }
@@ -2300,6 +2301,18 @@ int mdg_event_elapsed(mdg_ctx* c, int a, int b, double* ms) {
 
 int mdg_synchronize(mdg_ctx* c) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mdg_work_counts(mdg_ctx* c, int64_t out[3]) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<long long> t((size_t)kTallySlots * kTallyStride);
+  HIPCHK(c, hipMemcpy(t.data(), c->tally, t.size() * sizeof(long long), hipMemcpyDeviceToHost));
+  for (int k = 0; k < 3; k++) {
+    long long v = 0;
+    for (int q = 0; q < kTallySlots; q++) v += t[(size_t)q * kTallyStride + 3 + k];
+    out[k] = v;
+  }
   return 0;
 }
 

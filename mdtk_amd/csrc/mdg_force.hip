@@ -44,15 +44,6 @@ __device__ __forceinline__ void block_share(int nwork, int& b0, int& b1, int map
   b1 = (int)min(lo + per, (long long)nwork);
 }
 
-// Adds a per-lane tally into this workgroup's slot of a spread tally (kind
-// pointer = tally + k) with one atomic per wave (all 64 lanes must be
-// active): one address bumped by every wave of a grid would serialise it.
-__device__ __forceinline__ void count_add(long long* kind, long long v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  if ((threadIdx.x & 63) == 0 && v)
-    atomicAdd((unsigned long long*)(kind + (blockIdx.x & (kTallySlots - 1)) * kTallyStride), (unsigned long long)v);
-}
-
 // Parameter-gradient layout (per system, kNumTheta doubles): for the type
 // pairs p = C-C, C-H, H-H (p = ti + tj), REBO's pair parameters at 9p + {Q, A,
 // alpha, B1, B2, B3, beta1, beta2, beta3} (REBO.cxx:856-941) and the LJ
@@ -1299,7 +1290,8 @@ template <class R, bool TH, int OCC, int PASS, bool VIR>
 __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* __restrict__ T, Box bx,
                                               int4* __restrict__ queue, const int* __restrict__ qcount,
                                               int qsub, double* __restrict__ grad, double* __restrict__ eatom,
-                                              unsigned* err, double* __restrict__ dth, long long* __restrict__ vir) {
+                                              unsigned* err, double* __restrict__ dth, long long* __restrict__ vir,
+                                              long long* __restrict__ work) {
   static_assert(kQueues == 256, "one thread per sub-queue in the prefix below (blockDim 256)");
   static_assert(PASS >= 0 && PASS <= 2 && (PASS == 0 || !TH), "parameter gradients use the one-pass form");
   const double gs = T->eunit / T->lunit;
@@ -1330,6 +1322,7 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
   __syncthreads();
   const int nq = s_pre[kQueues];
   Virial<VIR> vw(vir, T->eunit);
+  long long n_trip = 0, n_full = 0;  // PASS 1 work tallies
   int b0, b1;
   block_share(nq, b0, b1, A.wg_map);
   for (int base = b0; base < b1; base += 256) {
@@ -1385,9 +1378,11 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
     }
     bool done = false;
     if constexpr (PASS == 1) {
+      n_trip += A.bcnt[i] + A.bcnt[j];  // sigma-pi triplets of the bound (both sides, no partner excluded)
       if (!below) {  // left to the full pass
         queue[(size_t)lo * qsub + (t - s_pre[lo])].w = 1;
         done = true;
+        n_full++;
       }
     } else {
       if (!below) bo_finish<R>(A, T, p, st, er);
@@ -1448,6 +1443,10 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
   }
   gc.flush();
   vw.flush_wave();
+  if constexpr (PASS == 1) {
+    count_add(work + 0, n_trip);
+    count_add(work + 1, n_full);
+  }
 }
 
 // ------------------------------------------------------------ torsion
@@ -1570,7 +1569,7 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
                                                            const int2*, const int*, double*, double*, unsigned*, \
                                                            double*, long long*);                                    \
   template __global__ void k_bstar<R, TH, OCC, 0, false>(Atoms, const TablesT<R>*, Box, int4*, const int*, int,  \
-                                                         double*, double*, unsigned*, double*, long long*);
+                                                         double*, double*, unsigned*, double*, long long*, long long*);
 #define MDG_INSTANTIATE_TH(R, TH, OCC, LW)                                                                     \
   template __global__ void k_lj_wave<R, TH, OCC, LW, false>(Atoms, const TablesT<R>*, Box, double*, double*,   \
                                                             int4*, int*, int, long long*, unsigned*, double*,   \
@@ -1588,7 +1587,7 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
                                                             int4*, int*, int, long long*, unsigned*, double*,  \
                                                             long long*);                                          \
   template __global__ void k_bstar<R, false, 1, 0, true>(Atoms, const TablesT<R>*, Box, int4*, const int*,    \
-                                                         int, double*, double*, unsigned*, double*, long long*);  \
+                                                         int, double*, double*, unsigned*, double*, long long*, long long*);  \
   template __global__ void k_pair_rows<R>(Atoms, const TablesT<R>*, Box, const int*, const int*, double,      \
                                           long long*);
 MDG_INSTANTIATE(double)
@@ -1614,10 +1613,10 @@ MDG_INSTANTIATE_OCC(float, false, 4)
 #define MDG_INSTANTIATE_SPLIT(R, OCC)                                                                           \
   template __global__ void k_bstar<R, false, OCC, 1, false>(Atoms, const TablesT<R>*, Box, int4*, const int*,  \
                                                             int, double*, double*, unsigned*, double*,         \
-                                                            long long*);                                          \
+                                                            long long*, long long*);                                          \
   template __global__ void k_bstar<R, false, OCC, 2, false>(Atoms, const TablesT<R>*, Box, int4*, const int*,  \
                                                             int, double*, double*, unsigned*, double*,         \
-                                                            long long*);
+                                                            long long*, long long*);
 MDG_INSTANTIATE_SPLIT(double, 1)
 MDG_INSTANTIATE_SPLIT(double, 2)
 MDG_INSTANTIATE_SPLIT(float, 1)

@@ -511,12 +511,15 @@ __global__ void __launch_bounds__(256) k_metal_rows(Atoms A, Box bx, const int* 
 // per workgroup (blockDim 256) reserves the group's bonds, placed by an
 // inclusive scan of the per-atom counts.
 __global__ void __launch_bounds__(256) k_bond_enum(Atoms A, int2* __restrict__ out, int* __restrict__ count, int cap,
-                                                   unsigned* err) {
+                                                   unsigned* err, long long* __restrict__ ntrip) {
   __shared__ int s_w[4];
   __shared__ int s_base;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int nb = (i < A.n) ? A.bcnt[i] : 0;
+  // sigma-pi triplets (bond, side, neighbour) of this call's REBO bonds: a
+  // centre with nb bonds takes nb - 1 neighbours for each of its nb bonds
+  count_add(ntrip, (long long)nb * (nb > 0 ? nb - 1 : 0));
   int mine = 0;
   for (int s = 0; s < nb; s++) mine += (A.bj[i * A.maxb + s] > i);
   int incl = mine;

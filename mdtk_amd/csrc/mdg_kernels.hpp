@@ -159,7 +159,7 @@ __global__ void k_clear_flags(int nsys, int* flags, int* sflags, int* pflags, in
 // of type T_OTHER within rcand; 16 lanes per atom, blocks of 256.
 constexpr int kMetalRow = 128;
 __global__ void k_metal_rows(Atoms A, Box bx, const int* flag, const int* eflags, double rcand);
-__global__ void k_bond_enum(Atoms A, int2* out, int* count, int cap, unsigned* err);
+__global__ void k_bond_enum(Atoms A, int2* out, int* count, int cap, unsigned* err, long long* ntrip);
 // Energies go to eatom (per-atom, per chain member), owned by the first atom
 // of the bond, pair or quadruplet.
 template <class R>
@@ -182,6 +182,16 @@ constexpr int kQStride = 32;
 // of slot q is tally[q * kTallyStride + k]; the host adds the slots up.
 constexpr int kTallySlots = 256;
 constexpr int kTallyStride = 16;
+
+// Adds a per-lane tally into this workgroup's slot of a spread tally (kind
+// pointer = tally + k) with one atomic per wave (all 64 lanes must be
+// active): one address bumped by every wave of a grid would serialise it.
+__device__ __forceinline__ void count_add(long long* kind, long long v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0 && v)
+    atomicAdd((unsigned long long*)(kind + (blockIdx.x & (kTallySlots - 1)) * kTallyStride), (unsigned long long)v);
+}
+
 template <class R>
 __global__ void k_pair_rows(Atoms A, const TablesT<R>* T, Box bx, const int* pflag, const int* pflags,
                             double margin, long long* rebuilt);
@@ -192,7 +202,8 @@ __global__ void k_lj_wave(Atoms A, const TablesT<R>* T, Box bx, double* grad, do
                           int* qcount, int qsub, long long* npairs, unsigned* err, double* dth, long long* vir);
 template <class R, bool TH, int OCC, int PASS, bool VIR>
 __global__ void k_bstar(Atoms A, const TablesT<R>* T, Box bx, int4* queue, const int* qcount, int qsub,
-                        double* grad, double* eatom, unsigned* err, double* dth, long long* vir);
+                        double* grad, double* eatom, unsigned* err, double* dth, long long* vir,
+                        long long* work);
 template <class R, bool VIR>
 __global__ void k_etors(Atoms A, const TablesT<R>* T, Box bx, const int2* bonds, const int* nbonds, int compat,
                         double* grad, double* eatom, long long* nquads, long long* vir);

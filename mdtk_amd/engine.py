@@ -64,6 +64,7 @@ def load_library(path: str = LIB_PATH):
     L.mdg_set_positions.argtypes = [vp, dp, ip, dp]
     L.mdg_set_atom_pbc.argtypes = [vp, dp, dp, ip]
     L.mdg_set_graph_block.argtypes = [vp, ctypes.c_int]
+    L.mdg_work_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     L.mdg_get_params.argtypes = [vp, dp]
     L.mdg_set_params.argtypes = [vp, dp]
     L.mdg_energy_grad_params.argtypes = [vp, dp, dp, dp]
@@ -438,6 +439,13 @@ class Engine:
                 "ms_after", "lj_shape", "xcd_order"]
         r = np.array([float(tune.get(k, 0.0)) for k in keys])
         self._chk(_lib.mdg_set_tune(self._h, _p(r)))
+
+    def work_counts(self) -> dict:
+        """Bond-order work of the last force call: REBO sigma-pi triplets, the b*
+        bound pass's triplets and the b* pairs that took the full bond order."""
+        w = np.zeros(3, np.int64)
+        self._chk(_lib.mdg_work_counts(self._h, _p(w, ctypes.c_int64)))
+        return dict(rebo_triplets=int(w[0]), bstar_triplets=int(w[1]), bstar_full=int(w[2]))
 
     def set_graph_block(self, steps: int):
         """Steps per graph launch in run() (default 32; results do not depend on it)."""
