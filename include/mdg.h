@@ -333,7 +333,9 @@ int mdg_kernel_time(mdg_ctx* ctx, const char* kernel, double* total_ms, int64_t*
  * trajectory unchanged) and keeps the fastest for every later call: the
  * order in which the scattering kernels hand work to the XCDs (dealt in
  * turn, or a contiguous eighth of the work per XCD), the
- * waves-per-SIMD variant of the bond-order kernels (1-2; 1-4 in the mixed form), the shape of
+ * waves-per-SIMD variant of the bond-order kernels (1-2; 1-4 in the mixed form; for the REBO bonds
+ * also 11-12: LDS-staged shells, one wavefront per tile of 64 bonds, at 1 or 2 waves/SIMD; 12 in the
+ * mixed form only), the shape of
  * the LJ kernel (shape 0-3: lanes per atom 64, 64 held to 3 waves/SIMD, 32,
  * 16; mixed form also 4-5: 16 lanes held to 4 or 3 waves/SIMD) and
  * the grid sizes of the bond, LJ, b* and torsion kernels.  report (may be

@@ -29,7 +29,9 @@ struct KStat {
 // Launch shapes of the force kernels (0 = the built-in heuristic); set by
 // mdg_autotune from timings on the context's own system.
 struct Tune {
-  int occ_bonds = 1, occ_bstar = 1;  // waves/SIMD variant of the bond-order kernels
+  // waves/SIMD variant of the bond-order kernels; occ_bonds 11 and 12: the
+  // REBO bonds with LDS-staged shells (k_rebo_shell) at 1 or 2 waves/SIMD
+  int occ_bonds = 1, occ_bstar = 1;
   // LJ kernel shape: 0 = 64 lanes per atom, 1 = the same held to 3 waves/SIMD,
   // 2 = 32 lanes per atom, 3 = 16 lanes per atom (more atoms in flight)
   int lj_shape = 3;
@@ -640,6 +642,17 @@ void force_terms(mdg_ctx* c, const TablesT<R>* T) {
             vir);
       };
       if constexpr (!TH && !VIR) {
+        if (tu.occ_bonds >= 11) {  // LDS-staged shells, one wavefront per tile of 64 bonds
+          auto sh = [&](auto occ) {
+            k_rebo_shell<R, decltype(occ)::value><<<bgrid, 256, 0, sb>>>(A, T, c->dT, c->box, c->bonds, c->nbonds,
+                                                                         A.grad,
+                                                                         c->eatom + (size_t)c->slot_rebo * n, c->derr);
+          };
+          if constexpr (std::is_same<R, float>::value) {
+            if (tu.occ_bonds == 12) return sh(std::integral_constant<int, 2>());
+          }
+          return sh(std::integral_constant<int, 1>());
+        }
         if (tu.occ_bonds == 2) return go(std::integral_constant<int, 2>());
         if constexpr (std::is_same<R, float>::value) {  // 3-4 waves/SIMD: FP32 form only
           if (tu.occ_bonds == 3) return go(std::integral_constant<int, 3>());
@@ -2201,8 +2214,8 @@ int mdg_autotune(mdg_ctx* c, int reps, double* report) {
   Tune& tu = c->tune;
   tune_knob(nullptr, c->A.wg_map, {0, 1});
   if (c->slot_rebo >= 0) {
-    if (c->mixed) tune_knob("k_rebo_bonds", tu.occ_bonds, {1, 2, 3, 4});
-    else tune_knob("k_rebo_bonds", tu.occ_bonds, {1, 2});
+    if (c->mixed) tune_knob("k_rebo_bonds", tu.occ_bonds, {1, 2, 3, 4, 11, 12});
+    else tune_knob("k_rebo_bonds", tu.occ_bonds, {1, 2, 11});
     tune_knob("k_rebo_bonds", tu.grid_bonds, grids(nb, 256));
   }
   if (c->slot_lj >= 0) {
@@ -2254,7 +2267,8 @@ int mdg_set_graph_block(mdg_ctx* c, int steps) {
 int mdg_set_tune(mdg_ctx* c, const double* knobs) {
   const int occ_bonds = (int)knobs[0], occ_bstar = (int)knobs[2], lj_shape = (int)knobs[8], xcd = (int)knobs[9];
   const int max_occ = c->mixed ? 4 : 2;
-  if (occ_bonds < 1 || occ_bonds > max_occ || occ_bstar < 1 || occ_bstar > max_occ || lj_shape < 0 ||
+  const bool shell = occ_bonds == 11 || (c->mixed && occ_bonds == 12);
+  if (((occ_bonds < 1 || occ_bonds > max_occ) && !shell) || occ_bstar < 1 || occ_bstar > max_occ || lj_shape < 0 ||
       lj_shape > (c->mixed ? 5 : 3) || (xcd != 0 && xcd != 1)) {
     c->err = "mdg_set_tune: a knob outside the autotuner's candidates";
     return 1;

@@ -73,6 +73,8 @@ struct BondRef {
   int k;
   R r, f, df;
   V3<R> rv;  // x_a - x_k (minimum image)
+  int tk;    // k's type and total coordination (LDS shells only)
+  double Nk;
 };
 
 // The bond records of the kernel's type.
@@ -97,6 +99,96 @@ __device__ __forceinline__ BondRef<R> bond_at(const Atoms& A, int a, int s) {
   b.rv = v3<R>(q.x, q.y, q.z);
   return b;
 }
+
+// The bond-order code reads each atom's shell -- its bond records, its
+// coordination sums, and each partner's type and coordination -- through an
+// accessor: GlobalShell reads global memory (the per-step bond lists), the
+// LDS shells of k_rebo_shell read a wavefront's staged copy.  view(a)
+// resolves an atom once; nb/bond/Nt/NH/NC read through the view.
+struct GView {
+  int a;
+};
+template <class R>
+struct GlobalShell {
+  const Atoms& A;
+  __device__ __forceinline__ GView view(int a) const { return GView{a}; }
+  __device__ __forceinline__ int nb(GView v) const { return A.bcnt[v.a]; }
+  __device__ __forceinline__ BondRef<R> bond(GView v, int t) const { return bond_at<R>(A, v.a, t); }
+  __device__ __forceinline__ double Nt(GView v) const { return A.Nt[v.a]; }
+  __device__ __forceinline__ double NH(GView v) const { return A.NH[v.a]; }
+  __device__ __forceinline__ double NC(GView v) const { return A.NC[v.a]; }
+  // a bond partner's type and total coordination
+  __device__ __forceinline__ int ktype(const BondRef<R>& k) const { return A.type[k.k]; }
+  __device__ __forceinline__ double kNt(const BondRef<R>& k) const { return A.Nt[k.k]; }
+};
+
+// A wavefront's staged shells (k_rebo_shell): the atoms of its tile of
+// bonds, each with its coordination sums and its bond records, every record
+// carrying the partner's index, type and total coordination, so that the
+// bond order reads LDS where the global form gathers.  An atom that did not
+// fit (more than kShAtoms atoms or kShRecs records in the tile) is read
+// from global memory.
+constexpr int kShAtoms = 96;   // atoms per wave
+constexpr int kShRecs = 352;   // bond records per wave
+constexpr int kShHash = 256;   // hash slots per wave (power of two)
+template <class R>
+struct ShellRec {
+  R x, y, z, r, f, df, Nk;
+  unsigned kt;  // partner index << 2 | partner type
+};
+struct LView {
+  int a, s;  // atom, staged slot (-1: global)
+};
+__device__ __forceinline__ int shell_hash(int a) { return (int)(((unsigned)a * 2654435761u) >> 24); }
+template <class R>
+struct LdsShell {
+  const Atoms& A;
+  const MDG_LDS int* key;    // [kShHash] atom, -1 free
+  const MDG_LDS short* slot; // [kShHash] slot of the atom, -1 not staged
+  const MDG_LDS int* nbs;    // [kShAtoms] bonds, -1 when the records did not fit
+  const MDG_LDS int* off;    // [kShAtoms] first record
+  const MDG_LDS double* N;   // [kShAtoms][3] Nt, NH, NC
+  const MDG_LDS ShellRec<R>* rec;
+  __device__ __forceinline__ LView view(int a) const {
+    int h = shell_hash(a);
+#pragma unroll 1
+    for (int p = 0; p < kShHash; p++) {
+      const int k = key[h];
+      if (k == a) {
+        const int sl = slot[h];
+        return LView{a, (sl >= 0 && nbs[sl] >= 0) ? sl : -1};
+      }
+      if (k == -1) break;
+      h = (h + 1) & (kShHash - 1);
+    }
+    return LView{a, -1};
+  }
+  __device__ __forceinline__ int nb(LView v) const { return v.s >= 0 ? nbs[v.s] : A.bcnt[v.a]; }
+  __device__ __forceinline__ BondRef<R> bond(LView v, int t) const {
+    BondRef<R> b;
+    if (v.s >= 0) {
+      const MDG_LDS ShellRec<R>* q = rec + off[v.s] + t;
+      const unsigned kt = q->kt;
+      b.k = (int)(kt >> 2);
+      b.tk = (int)(kt & 3u);
+      b.r = q->r;
+      b.f = q->f;
+      b.df = q->df;
+      b.rv = v3<R>(q->x, q->y, q->z);
+      b.Nk = double(q->Nk);
+    } else {
+      b = bond_at<R>(A, v.a, t);
+      b.tk = A.type[b.k];
+      b.Nk = A.Nt[b.k];
+    }
+    return b;
+  }
+  __device__ __forceinline__ double Nt(LView v) const { return v.s >= 0 ? N[3 * v.s] : A.Nt[v.a]; }
+  __device__ __forceinline__ double NH(LView v) const { return v.s >= 0 ? N[3 * v.s + 1] : A.NH[v.a]; }
+  __device__ __forceinline__ double NC(LView v) const { return v.s >= 0 ? N[3 * v.s + 2] : A.NC[v.a]; }
+  __device__ __forceinline__ int ktype(const BondRef<R>& k) const { return k.tk; }
+  __device__ __forceinline__ double kNt(const BondRef<R>& k) const { return k.Nk; }
+};
 
 // Coordination of a excluding b (REBO::Nt/NH/NC, REBO.cxx:423-493): the
 // cached FP64 sum minus f_ab when r_ab < R2(a,b), clamped at 0.
@@ -281,26 +373,28 @@ __device__ __forceinline__ R cos_theta(V3<R> a, V3<R> b, R ra, R rb, unsigned* e
 
 // First part of the forward pass of b_ij: the coordinations excluding the
 // pair and D_ij, D_ji of the sigma-pi terms (REBO.cxx:235-262, P :626-657).
-template <class R>
-__device__ __forceinline__ void bo_sigma_pi(const Atoms& A, const TablesT<R>* __restrict__ T, const BPair<R>& p,
+template <class R, class SH>
+__device__ __forceinline__ void bo_sigma_pi(const SH& sh, const TablesT<R>* __restrict__ T, const BPair<R>& p,
                                             BOState& st, unsigned* err) {
   const ReboParamsT<R>& P = T->rebo;
   R R2ij = P.R[p.ti][p.tj][1];
-  st.N[0] = coord_excl<R>(A.Nt[p.i], true, p.r, R2ij, p.f, err);
-  st.N[1] = coord_excl<R>(A.Nt[p.j], true, p.r, R2ij, p.f, err);
-  st.NHc[0] = coord_excl<R>(A.NH[p.i], p.tj == T_H, p.r, R2ij, p.f, err);
-  st.NCc[0] = coord_excl<R>(A.NC[p.i], p.tj == T_C, p.r, R2ij, p.f, err);
-  st.NHc[1] = coord_excl<R>(A.NH[p.j], p.ti == T_H, p.r, R2ij, p.f, err);
-  st.NCc[1] = coord_excl<R>(A.NC[p.j], p.ti == T_C, p.r, R2ij, p.f, err);
+  const auto vi = sh.view(p.i), vj = sh.view(p.j);
+  st.N[0] = coord_excl<R>(sh.Nt(vi), true, p.r, R2ij, p.f, err);
+  st.N[1] = coord_excl<R>(sh.Nt(vj), true, p.r, R2ij, p.f, err);
+  st.NHc[0] = coord_excl<R>(sh.NH(vi), p.tj == T_H, p.r, R2ij, p.f, err);
+  st.NCc[0] = coord_excl<R>(sh.NC(vi), p.tj == T_C, p.r, R2ij, p.f, err);
+  st.NHc[1] = coord_excl<R>(sh.NH(vj), p.ti == T_H, p.r, R2ij, p.f, err);
+  st.NCc[1] = coord_excl<R>(sh.NC(vj), p.ti == T_C, p.r, R2ij, p.f, err);
 #pragma unroll
   for (int s = 0; s < 2; s++) {
     Side<R> q = side_of(p, s);
     double D = 1.0;
-    int nb = A.bcnt[q.a];
+    const auto va = sh.view(q.a);
+    int nb = sh.nb(va);
     for (int t = 0; t < nb; t++) {
-      BondRef<R> k = bond_at<R>(A, q.a, t);
+      BondRef<R> k = sh.bond(va, t);
       if (k.k == q.b) continue;
-      int tk = A.type[k.k];
+      int tk = sh.ktype(k);
       R E = exp_term<R>(P, q, tk, k.r, p.reff);
       if (E != R(0)) {
         R raw;
@@ -325,8 +419,8 @@ __device__ __forceinline__ void bo_sigma_pi(const Atoms& A, const TablesT<R>* __
 // The rest of the forward pass of b_ij = (p_ij + p_ji)/2 + pi_rc + pi_dh
 // (REBO::Baver, REBO.cxx:200-211, with p_sigma_pi :264-275, Nconj :540-624,
 // pi_rc :659-714, pi_dh :716-814), after bo_sigma_pi.
-template <class R>
-__device__ __forceinline__ void bo_finish(const Atoms& A, const TablesT<R>* __restrict__ T, const BPair<R>& p,
+template <class R, class SH>
+__device__ __forceinline__ void bo_finish(const SH& sh, const TablesT<R>* __restrict__ T, const BPair<R>& p,
                                           BOState& st, unsigned* err) {
   const ReboParamsT<R>& P = T->rebo;
   double pij = pow(st.D[0], -0.5);
@@ -336,13 +430,14 @@ __device__ __forceinline__ void bo_finish(const Atoms& A, const TablesT<R>* __re
   for (int s = 0; s < 2; s++) {
     Side<R> q = side_of(p, s);
     double sum = 0.0;
-    int nb = A.bcnt[q.a];
+    const auto va = sh.view(q.a);
+    int nb = sh.nb(va);
     for (int t = 0; t < nb; t++) {
-      BondRef<R> k = bond_at<R>(A, q.a, t);
+      BondRef<R> k = sh.bond(va, t);
       if (k.k == q.b) continue;
-      int tk = A.type[k.k];
+      int tk = sh.ktype(k);
       if (tk != T_C) continue;
-      double Nk = coord_excl<R>(A.Nt[k.k], true, k.r, P.R[tk][q.ta][1], k.f, err);
+      double Nk = coord_excl<R>(sh.kNt(k), true, k.r, P.R[tk][q.ta][1], k.f, err);
       sum += double(k.f * sprime<R>(R(Nk), P.Ntot[0], P.Ntot[1]));
     }
     st.sum[s] = sum;
@@ -377,17 +472,18 @@ __device__ __forceinline__ void bo_finish(const Atoms& A, const TablesT<R>* __re
     for (int q = 0; q < 4; q++) st.T[q] = o[q];
     if (st.T[0] != 0.0) {
       double s = 0.0;
-      int nbi = A.bcnt[p.i], nbj = A.bcnt[p.j];
+      const auto vi = sh.view(p.i), vj = sh.view(p.j);
+      int nbi = sh.nb(vi), nbj = sh.nb(vj);
       for (int a = 0; a < nbi; a++) {
-        BondRef<R> k = bond_at<R>(A, p.i, a);
+        BondRef<R> k = sh.bond(vi, a);
         if (k.k == p.j) continue;
         for (int b = 0; b < nbj; b++) {
-          BondRef<R> l = bond_at<R>(A, p.j, b);
+          BondRef<R> l = sh.bond(vj, b);
           if (l.k == p.i) continue;
           if (k.k == l.k) continue;
           R sin_ik = sqrt(norm2(cross(p.rv, k.rv))) / (p.r * k.r);
           if (fabs(sin_ik) < R(0.1)) continue;
-          int tk = A.type[k.k], tl = A.type[l.k];
+          int tk = sh.ktype(k), tl = sh.ktype(l);
           R fik = sprime<R>(k.r, P.Rp[tk][p.ti][0], P.Rp[tk][p.ti][1]);
           R sin_jl = sqrt(norm2(cross(p.rv, l.rv))) / (p.r * l.r);
           if (fabs(sin_jl) < R(0.1)) continue;
@@ -408,11 +504,11 @@ __device__ __forceinline__ void bo_finish(const Atoms& A, const TablesT<R>* __re
   st.b = (pij + pji) / 2.0 + st.rc[0] + pidh;
 }
 
-template <class R>
-__device__ __forceinline__ void bo_forward(const Atoms& A, const TablesT<R>* __restrict__ T, const BPair<R>& p,
+template <class R, class SH>
+__device__ __forceinline__ void bo_forward(const SH& sh, const TablesT<R>* __restrict__ T, const BPair<R>& p,
                                            BOState& st, unsigned* err) {
-  bo_sigma_pi<R>(A, T, p, st, err);
-  bo_finish<R>(A, T, p, st, err);
+  bo_sigma_pi<R>(sh, T, p, st, err);
+  bo_finish<R>(sh, T, p, st, err);
 }
 
 // Scatter c * d f_ak / dx for the pair (a,k) along u_ak = rv/r (a +=, k -=).
@@ -427,8 +523,8 @@ __device__ __forceinline__ void pair_push(const G& g, int k, V3<R> rv, R r, R c,
 // Adjoint pass: adds W * db_ij/dx to grad (W in the kernel's energy unit);
 // every contribution is a push +v, -v along one of the term's pair vectors,
 // which vw adds to the virial.
-template <class R, class G, class Wv>
-__device__ __forceinline__ void bo_backward(const Atoms& A, const TablesT<R>* __restrict__ T, const BPair<R>& p,
+template <class R, class SH, class G, class Wv>
+__device__ __forceinline__ void bo_backward(const SH& sh, const TablesT<R>* __restrict__ T, const BPair<R>& p,
                                             const BOState& st, double W, G grad, double gs, unsigned* err,
                                             Wv& vw) {
   const ReboParamsT<R>& P = T->rebo;
@@ -442,11 +538,12 @@ __device__ __forceinline__ void bo_backward(const Atoms& A, const TablesT<R>* __
     V3<R>& gb = (s == 0) ? gj : gi;
     double seedD = (-0.5 * pow(st.D[s], -0.5 - 1.0)) * (W / 2.0);
     R seedR = R(seedD);
-    int nb = A.bcnt[q.a];
+    const auto va = sh.view(q.a);
+    int nb = sh.nb(va);
     for (int t = 0; t < nb; t++) {
-      BondRef<R> k = bond_at<R>(A, q.a, t);
+      BondRef<R> k = sh.bond(va, t);
       if (k.k == q.b) continue;
-      int tk = A.type[k.k];
+      int tk = sh.ktype(k);
       R E = exp_term<R>(P, q, tk, k.r, p.reff);
       if (E == R(0)) continue;
       R raw;
@@ -503,17 +600,18 @@ __device__ __forceinline__ void bo_backward(const Atoms& A, const TablesT<R>* __
       dNconj += st.T[3] * Wt;
     }
     R Ws = R(st.T[0] * W);
-    int nbi = A.bcnt[p.i], nbj = A.bcnt[p.j];
+    const auto vi = sh.view(p.i), vj = sh.view(p.j);
+      int nbi = sh.nb(vi), nbj = sh.nb(vj);
     for (int a = 0; a < nbi; a++) {
-      BondRef<R> k = bond_at<R>(A, p.i, a);
+      BondRef<R> k = sh.bond(vi, a);
       if (k.k == p.j) continue;
       for (int b = 0; b < nbj; b++) {
-        BondRef<R> l = bond_at<R>(A, p.j, b);
+        BondRef<R> l = sh.bond(vj, b);
         if (l.k == p.i) continue;
         if (k.k == l.k) continue;
         R sin_ik = sqrt(norm2(cross(p.rv, k.rv))) / (p.r * k.r);
         if (fabs(sin_ik) < R(0.1)) continue;
-        int tk = A.type[k.k], tl = A.type[l.k];
+        int tk = sh.ktype(k), tl = sh.ktype(l);
         R R1k = P.Rp[tk][p.ti][0], R2k = P.Rp[tk][p.ti][1];
         R fik = sprime<R>(k.r, R1k, R2k);
         R sin_jl = sqrt(norm2(cross(p.rv, l.rv))) / (p.r * l.r);
@@ -570,17 +668,18 @@ __device__ __forceinline__ void bo_backward(const Atoms& A, const TablesT<R>* __
     Side<R> q = side_of(p, s);
     V3<R>& ga = (s == 0) ? gi : gj;
     double sconj = (st.sum[s] != 0.0) ? 2.0 * st.sum[s] * dNconj : 0.0;
-    int nb = A.bcnt[q.a];
+    const auto va = sh.view(q.a);
+    int nb = sh.nb(va);
     for (int t = 0; t < nb; t++) {
-      BondRef<R> k = bond_at<R>(A, q.a, t);
+      BondRef<R> k = sh.bond(va, t);
       if (k.k == q.b) continue;
-      int tk = A.type[k.k];
+      int tk = sh.ktype(k);
       double c = dN[s];
       if (tk == T_H) c += dNH[s];
       if (tk == T_C) c += dNC[s];
       double coefk = 0.0;
       if (tk == T_C && sconj != 0.0) {
-        double Nk = coord_excl<R>(A.Nt[k.k], true, k.r, P.R[tk][q.ta][1], k.f, err);
+        double Nk = coord_excl<R>(sh.kNt(k), true, k.r, P.R[tk][q.ta][1], k.f, err);
         double Sp = sprime<R>(R(Nk), P.Ntot[0], P.Ntot[1]);
         c += Sp * sconj;
         coefk = double(k.f) * double(dsprime<R>(R(Nk), P.Ntot[0], P.Ntot[1])) * sconj;
@@ -589,9 +688,10 @@ __device__ __forceinline__ void bo_backward(const Atoms& A, const TablesT<R>* __
       if (coefk != 0.0) {
         // d N_ka: neighbours m of k other than a
         V3<R> gk = v3<R>(0, 0, 0);
-        int nbk = A.bcnt[k.k];
+        const auto vk = sh.view(k.k);
+        int nbk = sh.nb(vk);
         for (int u = 0; u < nbk; u++) {
-          BondRef<R> m = bond_at<R>(A, k.k, u);
+          BondRef<R> m = sh.bond(vk, u);
           if (m.k == q.a || m.df == R(0)) continue;
           V3<R> v = m.rv * R(double(m.df) * coefk / double(m.r));
           gk = gk + v;
@@ -679,10 +779,10 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
       double bb = 0.0;
       if (VA != 0.0 || TH) {
         BOState st;
-        bo_forward<R>(A, T, p, st, er);
+        bo_forward<R>(GlobalShell<R>{A}, T, p, st, er);
         bb = st.b;
         if (VA != 0.0) {
-          bo_backward<R>(A, T, p, st, VA * su, gc, gs, er, vw);
+          bo_backward<R>(GlobalShell<R>{A}, T, p, st, VA * su, gc, gs, er, vw);
           double vder = (-P.beta[0][a][b]) * u1 + (-P.beta[1][a][b]) * u2 + (-P.beta[2][a][b]) * u3;
           cr += (-vder * f * bb) + (-vsum * bb) * df;
           eb += bb * VA;
@@ -713,6 +813,191 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
   }
   gc.flush();
   vw.flush_wave();
+}
+
+// REBO bonds with LDS-staged shells: one wavefront per tile of 64
+// consecutive bonds of the bond array (consecutive centres i, since the
+// array is built in atom order).  The wave first stages the shells of the
+// tile's atoms (the i and j of its bonds) in LDS -- coordination sums and
+// bond records, each record with the partner's index, type and total
+// coordination -- then each lane evaluates one bond as k_rebo_bonds does,
+// reading every shell of i and j (and of a neighbour k in the Nconj
+// adjoint, when k is in the tile) from LDS.  FP32 and FP64 forms, no
+// parameter gradients and no virial (those stay with k_rebo_bonds).
+constexpr int kShellCache = 512;  // gradient-cache slots (LDS left by the shells)
+template <class R, int OCC>
+__global__ void __launch_bounds__(256, OCC) k_rebo_shell(Atoms A, const TablesT<R>* __restrict__ T,
+                                                   const Tables* __restrict__ T64, Box bx,
+                                                   const int2* __restrict__ bonds, const int* __restrict__ nbonds,
+                                                   double* __restrict__ grad, double* __restrict__ eatom,
+                                                   unsigned* err) {
+  using GC = CacheOf<R, kShellCache>;
+  __shared__ int s_key[GC::kSlots];
+  __shared__ typename GC::val_t s_val[GC::kVals];
+  __shared__ int s_fill;
+  __shared__ int sh_key[4][kShHash];
+  __shared__ short sh_slot[4][kShHash];
+  __shared__ int sh_atom[4][kShAtoms];
+  __shared__ int sh_nb[4][kShAtoms];
+  __shared__ int sh_off[4][kShAtoms + 1];
+  __shared__ double sh_N[4][kShAtoms][3];
+  __shared__ ShellRec<R> sh_rec[4][kShRecs];
+  __shared__ int sh_cnt[4];
+  GC gc{s_key, s_val, &s_fill, grad, A.gfix};
+  gc.clear();
+  __syncthreads();
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const double gs = T->eunit / T->lunit;
+  const double su = 1.0 / T->eunit;  // erg -> the table's energy unit (adjoint seed)
+  const BondRec<R>* B = brec<R>(A);
+  Virial<false> vw;
+  const LdsShell<R> lsh{A, (const MDG_LDS int*)sh_key[w], (const MDG_LDS short*)sh_slot[w],
+                        (const MDG_LDS int*)sh_nb[w], (const MDG_LDS int*)sh_off[w],
+                        (const MDG_LDS double*)&sh_N[w][0][0], (const MDG_LDS ShellRec<R>*)sh_rec[w]};
+  auto wave_sync = [] {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  };
+  int nbt = min(*nbonds, A.bond_cap);
+  int b0, b1;
+  block_share(nbt, b0, b1, A.wg_map);
+  for (int base = b0; base < b1; base += 256) {
+    const int t = base + threadIdx.x;
+    const ReboParamsT<double>& P = T64->rebo;
+    int2 bd = (t < b1) ? bonds[t] : make_int2(0, 0);
+    // ---- stage the shells of this wave's tile
+    for (int q = lane; q < kShHash; q += 64) {
+      sh_key[w][q] = -1;
+      sh_slot[w][q] = -1;
+    }
+    if (lane == 0) sh_cnt[w] = 0;
+    wave_sync();
+    if (t < b1) {
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        const int a = e ? bd.y : bd.x;
+        int h = shell_hash(a);
+#pragma unroll 1
+        for (int p = 0; p < kShHash; p++) {
+          const int old = atomicCAS(&sh_key[w][h], -1, a);
+          if (old == -1) {
+            const int sl = atomicAdd(&sh_cnt[w], 1);
+            sh_slot[w][h] = (short)(sl < kShAtoms ? sl : -1);
+            if (sl < kShAtoms) sh_atom[w][sl] = a;
+            break;
+          }
+          if (old == a) break;
+          h = (h + 1) & (kShHash - 1);
+        }
+      }
+    }
+    wave_sync();
+    const int na = min(sh_cnt[w], kShAtoms);
+    // each staged atom's bond count and coordination; record offsets by a
+    // prefix over the slots (an atom whose records do not fit stays global)
+    int run = 0;
+    for (int s0 = 0; s0 < na; s0 += 64) {
+      const int sl = s0 + lane;
+      int nbv = 0;
+      if (sl < na) {
+        const int a = sh_atom[w][sl];
+        nbv = A.bcnt[a];
+        sh_N[w][sl][0] = A.Nt[a];
+        sh_N[w][sl][1] = A.NH[a];
+        sh_N[w][sl][2] = A.NC[a];
+      }
+      int incl = nbv;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+      }
+      const int start = run + incl - nbv;
+      const bool fits = start + nbv <= kShRecs;
+      if (sl < na) {
+        sh_nb[w][sl] = fits ? nbv : -1;
+        sh_off[w][sl] = start;
+      }
+      run += __shfl(incl, 63, 64);
+    }
+    wave_sync();
+    const int nrec = min(run, kShRecs);
+    for (int r = lane; r < nrec; r += 64) {
+      int lo = 0, hi = na - 1;  // last slot whose first record is <= r
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sh_off[w][mid] <= r) lo = mid;
+        else hi = mid - 1;
+      }
+      const int tt = r - sh_off[w][lo];
+      if (sh_nb[w][lo] < 0 || tt >= sh_nb[w][lo]) continue;  // records of an atom that did not fit
+      const int a = sh_atom[w][lo];
+      const int o = a * A.maxb + tt;
+      const BondRec<R> q = B[o];
+      const int k = A.bj[o];
+      ShellRec<R> sr;
+      sr.x = q.x;
+      sr.y = q.y;
+      sr.z = q.z;
+      sr.r = q.r;
+      sr.f = q.f;
+      sr.df = q.df;
+      sr.Nk = R(A.Nt[k]);
+      sr.kt = ((unsigned)k << 2) | ((unsigned)A.type[k] & 3u);
+      sh_rec[w][r] = sr;
+    }
+    wave_sync();
+    // ---- the bond (as k_rebo_bonds, shells from LDS)
+    BPair<R> p;
+    p.i = bd.x;
+    p.j = bd.y;
+    p.ti = A.type[p.i];
+    p.tj = A.type[p.j];
+    D3 rd = depos(A.p, p.i, p.j, bx);
+    p.rv = to_local<R>(rd);
+    p.r = sqrt(norm2(p.rv));
+    int a = p.ti, b = p.tj;
+    double r = sqrt(norm2(rd));
+    double f, df;
+    cutoff<double>(r, P.R[a][b][0], P.R[a][b][1], f, df);
+    if (t >= b1) f = 0.0;  // past the share: no work this pass
+    unsigned* const er = sys_err(err, A.sys[p.i]);
+    {
+      R fr, dfr;
+      cutoff<R>(p.r, T->rebo.R[a][b][0], T->rebo.R[a][b][1], fr, dfr);
+      p.f = fr;
+    }
+    p.reff = p.r;
+    if (f != 0.0) {
+      double cr = 0.0;
+      double t1 = P.A[a][b] * exp(-P.alpha[a][b] * r);
+      double val = (1.0 + P.Q[a][b] / r) * t1;
+      double der = -t1 * (P.Q[a][b] + P.alpha[a][b] * r * r + P.alpha[a][b] * r * P.Q[a][b]) / (r * r);
+      double VR = f * val;
+      cr += der * f + val * df;
+      double u1 = P.B[0][a][b] * exp(-P.beta[0][a][b] * r);
+      double u2 = P.B[1][a][b] * exp(-P.beta[1][a][b] * r);
+      double u3 = P.B[2][a][b] * exp(-P.beta[2][a][b] * r);
+      double vsum = u1 + u2 + u3;
+      double VA = -f * vsum;
+      double eb = VR;
+      if (VA != 0.0) {
+        BOState st;
+        bo_forward<R>(lsh, T, p, st, er);
+        const double bb = st.b;
+        bo_backward<R>(lsh, T, p, st, VA * su, gc, gs, er, vw);
+        double vder = (-P.beta[0][a][b]) * u1 + (-P.beta[1][a][b]) * u2 + (-P.beta[2][a][b]) * u3;
+        cr += (-vder * f * bb) + (-vsum * bb) * df;
+        eb += bb * VA;
+      }
+      add_energy(A, eatom, p.i, eb);
+      D3 v = rd * (cr / r);
+      atomic_add3(gc, p.i, v);
+      atomic_add3(gc, p.j, -v);
+    }
+    __syncthreads();
+    gc.drain_if_crowded();
+  }
+  gc.flush();
 }
 
 // ------------------------------------------------------------ C_ij paths
@@ -1370,7 +1655,7 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
     p.reff = P.R[ti][tj][0];
     BOState st;
     R bmin = L.b[ti][tj][0], bmax = L.b[ti][tj][1];
-    bo_sigma_pi<R>(A, T, p, st, er);
+    bo_sigma_pi<R>(GlobalShell<R>{A}, T, p, st, er);
     bool below = false;
     if constexpr (PASS != 2) {
       const double bub = 0.5 * (1.0 / sqrt(st.D[0]) + 1.0 / sqrt(st.D[1])) + T->rcmax[ti][tj] + 1e-12;
@@ -1385,7 +1670,7 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
         n_full++;
       }
     } else {
-      if (!below) bo_finish<R>(A, T, p, st, er);
+      if (!below) bo_finish<R>(GlobalShell<R>{A}, T, p, st, er);
     }
     if (!done) {
     if constexpr (VIR) vw.at(A.sys[i]);
@@ -1415,7 +1700,7 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
       if (dStr != R(0) && Stb != R(0)) cr += dStr * Stb * V2;
       if constexpr (PASS != 1) {
         R dStb = below ? R(0) : dSfun<R>(bs, bmin, bmax);
-        if (dStb != R(0)) bo_backward<R>(A, T, p, st, double(Str * dStb * V2), gc, gs, er, vw);
+        if (dStb != R(0)) bo_backward<R>(GlobalShell<R>{A}, T, p, st, double(Str * dStb * V2), gc, gs, er, vw);
       }
     }
     V3<R> v = rv * (cr / r);
@@ -1592,6 +1877,12 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
                                           long long*);
 MDG_INSTANTIATE(double)
 MDG_INSTANTIATE(float)
+#define MDG_INSTANTIATE_SHELL(R, OCC)                                                                        \
+  template __global__ void k_rebo_shell<R, OCC>(Atoms, const TablesT<R>*, const Tables*, Box, const int2*,       \
+                                                const int*, double*, double*, unsigned*);
+MDG_INSTANTIATE_SHELL(double, 1)
+MDG_INSTANTIATE_SHELL(float, 1)
+MDG_INSTANTIATE_SHELL(float, 2)
 MDG_INSTANTIATE_TH(double, false, 1, 64)
 MDG_INSTANTIATE_TH(double, false, 3, 64)
 MDG_INSTANTIATE_TH(double, false, 1, 32)

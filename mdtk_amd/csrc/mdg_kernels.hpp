@@ -167,6 +167,11 @@ __global__ void k_coord(Atoms A, const TablesT<R>* T, Box bx, unsigned* err);
 // TH = true (FP64 only): also accumulate dE/dtheta per system into dth
 // (kNumTheta doubles per system).  OCC: waves per SIMD the bond-order
 // kernels' register allocation is held to (1 or 2; chosen by the autotuner).
+// REBO bonds with the shells of each wavefront's tile of 64 bonds staged in
+// LDS (FP32 and FP64 forms, no parameter gradients or virial).
+template <class R, int OCC>
+__global__ void k_rebo_shell(Atoms A, const TablesT<R>* T, const Tables* T64, Box bx, const int2* bonds,
+                             const int* nbonds, double* grad, double* eatom, unsigned* err);
 template <class R, bool TH, int OCC, bool VIR>
 __global__ void k_rebo_bonds(Atoms A, const TablesT<R>* T, const Tables* T64, Box bx, const int2* bonds,
                              const int* nbonds, double* grad, double* eatom, unsigned* err, double* dth,
