@@ -381,6 +381,11 @@ int mdg_counters(mdg_ctx* ctx, int64_t out[8]);
  * queued b* pairs whose bound failed (the full bond order and adjoint; the
  * mixed form's two-pass b*, else 0).  No reference counterpart. */
 int mdg_work_counts(mdg_ctx* ctx, int64_t out[3]);
+/* The table bounds behind the LJ pass's per-atom b* test: out[0] a lower
+ * bound of G for a C centre (G_C1 and G_C2), out[1] of G_H, out[2] of P_CC,
+ * out[3] of P_CH, each over its whole domain less a margin for the FP32
+ * form.  No reference counterpart (diagnostics). */
+int mdg_bstar_bounds(mdg_ctx* ctx, double out[4]);
 
 #ifdef __cplusplus
 }

@@ -1183,6 +1183,8 @@ int mdg_set_systems(mdg_ctx* c, int nsys, const int32_t* counts, const int32_t* 
   r |= dalloc(c, A.p, n);
   if (!c->mixed) r |= dalloc(c, A.b64, (size_t)n * A.maxb);
   r |= dalloc(c, A.Nt, n) | dalloc(c, A.NH, n) | dalloc(c, A.NC, n);
+  A.pmax = nullptr;
+  if (c->slot_lj >= 0) r |= dalloc(c, A.pmax, 2 * (size_t)n);
   if (c->mixed) {
     r |= dalloc(c, A.b32, (size_t)n * A.maxb);
   }
@@ -2315,6 +2317,14 @@ int mdg_event_elapsed(mdg_ctx* c, int a, int b, double* ms) {
 
 int mdg_synchronize(mdg_ctx* c) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int mdg_bstar_bounds(mdg_ctx* c, double out[4]) {
+  out[0] = c->host_T.glb[0];
+  out[1] = c->host_T.glb[1];
+  out[2] = c->host_T.plb[T_C];
+  out[3] = c->host_T.plb[T_H];
   return 0;
 }
 

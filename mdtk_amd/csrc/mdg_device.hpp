@@ -101,6 +101,14 @@ struct TablesT {
   // only through node slopes; see rc_upper_bound), plus a margin for the
   // FP32 evaluation.  Used to prove b* < b_min without evaluating pi_rc.
   double rcmax[2][2];
+  // Lower bounds over the whole domain, minus a margin for the FP32 form's
+  // evaluation: glb[0] of G for a C centre (both quintics G_C1, G_C2; their
+  // S'(N) blend lies between them), glb[1] of G_H; plb[t] of P_CC (t = C) and
+  // P_CH (t = H), the out-of-table zero included.  With them the coordination
+  // pass bounds, for every atom a and partner type t, D_a^-1/2 from above for
+  // any partner direction (pmax), so that b* < b_min is proven per pair
+  // without its sigma-pi sums.
+  double glb[2], plb[2];
 };
 // FLJ (pairwise/FLJ.cxx): element index e2i (Cu 0, H 1, C 2, Au 3), LB-mixed
 // sigma/epsilon, the ZBL-to-LJ cubic bridge ends x[0..1] with the values and

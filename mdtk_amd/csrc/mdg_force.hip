@@ -222,6 +222,7 @@ __global__ void __launch_bounds__(256) k_coord(Atoms A, const TablesT<R>* __rest
   const int ti = A.type[i];
   unsigned* const er = sys_err(err, A.sys[i]);
   double Nt = 0, NH = 0, NC = 0;
+  float eH[2] = {0.0f, 0.0f};  // an H centre's sum of f ExpTerm for a partner of type C, H (b* bound)
   int nb = 0;
   if (ti <= T_H) {
     const int cnt = A.nls_cnt[i];
@@ -246,6 +247,16 @@ __global__ void __launch_bounds__(256) k_coord(Atoms A, const TablesT<R>* __rest
           cutoff<R>(r, T->rebo.R[ti][tk][0], T->rebo.R[ti][tk][1], f, df);
         }
       }
+      // an H centre's f ExpTerm for a b* partner of type C, H (r_alter = R0,
+      // REBO.cxx:221-226), one bond per lane
+      float ex[2] = {0.0f, 0.0f};
+      if (ok && ti == T_H && A.pmax) {
+        const ReboParamsT<R>& P = T->rebo;
+#pragma unroll
+        for (int tb = 0; tb < 2; tb++)
+          ex[tb] = float(f) * expf(float(P.expscale) * ((float(P.rho[T_H][tk]) - float(r)) -
+                                                        (float(P.rho[T_H][tb]) - float(P.R[T_H][tb][0]))));
+      }
       const unsigned long long m = __ballot(ok) & mine;
       if (ok) {
         const int slot = nb + __popcll(m & below);
@@ -262,12 +273,29 @@ __global__ void __launch_bounds__(256) k_coord(Atoms A, const TablesT<R>* __rest
         Nt += fs;
         if (ts == T_H) NH += fs;
         if (ts == T_C) NC += fs;
+        if (ti == T_H && A.pmax) {  // group-uniform
+          eH[0] += __shfl(ex[0], src, 64);
+          eH[1] += __shfl(ex[1], src, 64);
+        }
       }
       nb += __popcll(m);
     }
   }
   if (sl == 0) {
     if (nb > A.maxb) raise(er, ERR_BOND_OVERFLOW);
+    if (A.pmax) {
+      // D_a = 1 + sum_k f G E (+ P for a C centre) over a's bonds, partner
+      // not among them: G >= glb, E = 1 for C; P >= plb.  In FP32 with
+      // relative margins of 1e-5 (the table bounds' own margins are 1e-4)
+#pragma unroll
+      for (int tb = 0; tb < 2; tb++) {
+        float D = 0.0f;
+        if (ti == T_C) D = 1.0f + float(T->plb[tb]) + float(T->glb[0]) * float(Nt);
+        else if (ti == T_H) D = 1.0f + float(T->glb[1]) * eH[tb];
+        D = D * (1.0f - 1e-5f) - 1e-5f;
+        A.pmax[2 * i + tb] = (D > 0.0f) ? rsqrtf(D) * (1.0f + 1e-5f) : 1e30f;
+      }
+    }
     A.Nt[i] = Nt;
     A.NH[i] = NH;
     A.NC[i] = NC;
@@ -1465,6 +1493,12 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       V3<R> rv = to_local<R>(rd);
       R r = sqrt(norm2(rv));
       R Str = Sfun<R>(r, L.RLJ[ti][tj][0], L.RLJ[ti][tj][1]);
+      // b* < b_min by the per-atom bounds (a pair that is no REBO bond:
+      // neither atom's sum holds the other, and pi_rc <= rcmax, pi_dh <= 0):
+      // then S_b = 1 and the pair is one of S_r = 0's form, finished here
+      if (Str != R(0) && A.pmax && r > T->rebo.R[ti][tj][1] &&
+          0.5f * (A.pmax[2 * i + tj] + A.pmax[2 * j + ti]) + float(T->rcmax[ti][tj]) + 1e-5f < float(L.b[ti][tj][0]))
+        Str = R(0);
       if (Str != R(0)) {  // b* needed: one queue atomic per wave
         unsigned long long m = __ballot(1);
         int leader = __ffsll((long long)m) - 1;

@@ -61,6 +61,10 @@ struct Atoms {
   BondRec<double>* b64;       // FP64 bond geometry (cm)
   BondRec<float>* b32;        // FP32 bond geometry (Angstrom), mixed precision
   double *Nt, *NH, *NC;       // REBO coordination sums (REBO.hpp:250-258)
+  // Per atom a and partner type t (C, H): an upper bound of D_a^-1/2 over
+  // every partner direction that is not a's bond (coordination pass; null
+  // without AIREBO-LJ), the b* bound of the LJ pass.
+  float* pmax;
   // thermal bath (null when no system of the context has one)
   const Bath* bath;           // per system
   const unsigned char* tb_apply;  // per atom: Atom::apply_ThermalBath

@@ -70,6 +70,43 @@ struct Grid {
 // value weights are non-negative and sum to one, and each slope basis
 // function is bounded by |t^3 - 2t^2 + t|, |t^3 - t^2| <= 4/27 on [0, 1]
 // (the other factors lie in [0, 1]).  Outside the built cells the table is 0.
+// A lower bound of a G spline over its pieces: each piece is a quintic in t
+// on [0, 1], and a polynomial lies above the smallest of its Bernstein
+// coefficients b_k = sum_{i<=k} C(k,i)/C(5,i) c_i.
+double g_lower_bound(const GSpline& g) {
+  static const double C5[6] = {1, 5, 10, 10, 5, 1};
+  double lb = 1e300;
+  for (int s = 0; s < g.count; s++)
+    for (int k = 0; k <= 5; k++) {
+      double b = 0.0, ck = 1.0;  // ck = C(k, i)
+      for (int i = 0; i <= k; i++) {
+        b += ck / C5[i] * g.c[s][i];
+        ck = ck * (k - i) / (i + 1);
+      }
+      lb = std::min(lb, b);
+    }
+  return lb;
+}
+
+// A lower bound of a bicubic Hermite interpolant (cross derivatives zero)
+// over its cells, as rc_upper_bound: the corner values' minimum less 4/27 of
+// the slopes' magnitudes; beyond the table the value is 0.
+double p_lower_bound(const double (*t)[5][3]) {
+  double lb = 0.0;
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) {
+      double vmin = 1e300, slope = 0.0;
+      for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+          const double* q = t[i + a][j + b];
+          vmin = std::min(vmin, q[0]);
+          slope += std::fabs(q[1]) + std::fabs(q[2]);
+        }
+      lb = std::min(lb, vmin - (4.0 / 27.0) * slope);
+    }
+  return lb;
+}
+
 double rc_upper_bound(const double (*t)[5][16][4], int kbuilt) {
   double ub = 0.0;
   for (int i = 0; i < 4; i++)
@@ -361,6 +398,12 @@ void build_tables(Tables& T, int brenner_set) {
     T.rcmax[C][C] = rc_upper_bound(T.T3[0], 15) + margin;
     T.rcmax[C][H] = T.rcmax[H][C] = rc_upper_bound(T.T3[1], 15) + margin;
     T.rcmax[H][H] = rc_upper_bound(T.T3[2], 15) + margin;
+    // lower bounds of G and P for the per-atom b* bound; the margins cover
+    // the FP32 evaluation (|G| <= 20, |P| < 1)
+    T.glb[0] = std::min(g_lower_bound(T.rebo.gC1), g_lower_bound(T.rebo.gC2)) - 1e-4;
+    T.glb[1] = g_lower_bound(T.rebo.gH) - 1e-3;
+    T.plb[C] = p_lower_bound(T.P[0]) - 1e-4;
+    T.plb[H] = p_lower_bound(T.P[1]) - 1e-4;
   }
   // AIREBO-LJ
   LJParams& l = T.lj;
@@ -509,6 +552,8 @@ void convert_tables(const Tables& d, TablesT<float>& f) {
   cv(&f.P[0][0][0][0], &d.P[0][0][0][0], 2 * 5 * 5 * 3, 1.0);
   cv(&f.T3[0][0][0][0][0], &d.T3[0][0][0][0][0], 4 * 5 * 5 * 16 * 4, 1.0);
   std::memcpy(f.rcmax, d.rcmax, sizeof(f.rcmax));
+  std::memcpy(f.glb, d.glb, sizeof(f.glb));
+  std::memcpy(f.plb, d.plb, sizeof(f.plb));
 }
 
 }  // namespace mdg

@@ -65,6 +65,7 @@ def load_library(path: str = LIB_PATH):
     L.mdg_set_atom_pbc.argtypes = [vp, dp, dp, ip]
     L.mdg_set_graph_block.argtypes = [vp, ctypes.c_int]
     L.mdg_work_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    L.mdg_bstar_bounds.argtypes = [vp, dp]
     L.mdg_get_params.argtypes = [vp, dp]
     L.mdg_set_params.argtypes = [vp, dp]
     L.mdg_energy_grad_params.argtypes = [vp, dp, dp, dp]
@@ -446,6 +447,12 @@ class Engine:
         w = np.zeros(3, np.int64)
         self._chk(_lib.mdg_work_counts(self._h, _p(w, ctypes.c_int64)))
         return dict(rebo_triplets=int(w[0]), bstar_triplets=int(w[1]), bstar_full=int(w[2]))
+
+    def bstar_bounds(self) -> dict:
+        """The table bounds of the per-atom b* test (G_C, G_H, P_CC, P_CH lower bounds)."""
+        b = np.zeros(4)
+        self._chk(_lib.mdg_bstar_bounds(self._h, _p(b)))
+        return dict(g_c=b[0], g_h=b[1], p_cc=b[2], p_ch=b[3])
 
     def set_graph_block(self, steps: int):
         """Steps per graph launch in run() (default 32; results do not depend on it)."""
