@@ -310,18 +310,89 @@ __device__ __forceinline__ V3<R> to_local(D3 d) {
   }
 }
 
+// cos(pi t) and sin(pi t) for t in [0, 1], FP32, for the mixed form's
+// switching functions.  With x = pi (t - 1/2), |x| <= pi/2:
+// cos(pi t) = -sin x and sin(pi t) = cos x, from the Taylor series of sin x
+// through x^13 and of cos x through x^14 by Horner's rule in x^2 (the first
+// omitted terms are below 1e-9, under an FP32 ulp of the results' scale).
+// The library's cosf/sinf reduce arbitrary arguments and cost about ten
+// times as many instructions in every wave that reaches a switch.
+__host__ __device__ __forceinline__ void cospi_sinpi_unit(float t, float& c, float& s) {
+  const float x = float(kPi) * (t - 0.5f);
+  const float x2 = x * x;
+  float ps = 1.6059043836821615e-10f;  // 1/13!
+  ps = ps * x2 - 2.5052108385441719e-08f;
+  ps = ps * x2 + 2.7557319223985891e-06f;
+  ps = ps * x2 - 1.9841269841269841e-04f;
+  ps = ps * x2 + 8.3333333333333333e-03f;
+  ps = ps * x2 - 1.6666666666666667e-01f;
+  ps = ps * x2 + 1.0f;
+  float pc = -1.1470745597729725e-11f;  // -1/14!
+  pc = pc * x2 + 2.0876756987868099e-09f;
+  pc = pc * x2 - 2.7557319223985891e-07f;
+  pc = pc * x2 + 2.4801587301587302e-05f;
+  pc = pc * x2 - 1.3888888888888889e-03f;
+  pc = pc * x2 + 4.1666666666666667e-02f;
+  pc = pc * x2 - 0.5f;
+  pc = pc * x2 + 1.0f;
+  // kept inside [-1, 1] and [0, 1] as the functions are (a switch f = (1 + c)/2
+  // never goes below zero)
+  c = fminf(fmaxf(-(x * ps), -1.0f), 1.0f);
+  s = fmaxf(pc, 0.0f);
+}
+// The same in FP64 (sin through x^21, cos through x^22; first omitted terms
+// below 2e-18), for the mixed form's FP64 pair terms.
+__host__ __device__ __forceinline__ void cospi_sinpi_unit(double t, double& c, double& s) {
+  const double x = kPi * (t - 0.5);
+  const double x2 = x * x;
+  double ps = 1.95729410633912626e-20;  // 1/21!
+  ps = ps * x2 - 8.22063524662432950e-18;
+  ps = ps * x2 + 2.81145725434552060e-15;
+  ps = ps * x2 - 7.64716373181981641e-13;
+  ps = ps * x2 + 1.60590438368216133e-10;
+  ps = ps * x2 - 2.50521083854417202e-08;
+  ps = ps * x2 + 2.75573192239858925e-06;
+  ps = ps * x2 - 1.98412698412698413e-04;
+  ps = ps * x2 + 8.33333333333333322e-03;
+  ps = ps * x2 - 1.66666666666666657e-01;
+  ps = ps * x2 + 1.0;
+  double pc = 8.89679139245057408e-22;  // 1/22!
+  pc = pc * x2 - 4.11031762331216484e-19;
+  pc = pc * x2 + 1.56192069685862253e-16;
+  pc = pc * x2 - 4.77947733238738525e-14;
+  pc = pc * x2 + 1.14707455977297245e-11;
+  pc = pc * x2 - 2.08767569878681002e-09;
+  pc = pc * x2 + 2.75573192239858883e-07;
+  pc = pc * x2 - 2.48015873015873016e-05;
+  pc = pc * x2 + 1.38888888888888894e-03;
+  pc = pc * x2 - 4.16666666666666644e-02;
+  pc = pc * x2 + 0.5;
+  pc = 1.0 - pc * x2;
+  c = fmin(fmax(-(x * ps), -1.0), 1.0);
+  s = fmax(pc, 0.0);
+}
+
 // Cosine cutoff of AtomsPair (AtomsPair.hpp:142-154): f=1 below R1, cosine
-// switch to 0 at R2, with df/dr.
-template <class R>
+// switch to 0 at R2, with df/dr.  POLY: the switch from cospi_sinpi_unit
+// (the mixed form; FP32 kernels, and their FP64 pair terms), else from the
+// library's cos and sin as the reference computes it (the FP64 form).
+template <class R, bool POLY = std::is_same<R, float>::value>
 __device__ __forceinline__ void cutoff(R r, R R1, R R2, R& f, R& df) {
   f = R(0);
   df = R(0);
   if (r < R1) {
     f = R(1);
   } else if (r < R2) {
-    R a = R(kPi) * (r - R1) / (R2 - R1);
-    f = (R(1) + cos(a)) / R(2);
-    df = (-(R(kPi) / (R2 - R1)) * sin(a)) / R(2);
+    if constexpr (POLY) {
+      R c, sn;
+      cospi_sinpi_unit((r - R1) / (R2 - R1), c, sn);
+      f = (R(1) + c) / R(2);
+      df = (-(R(kPi) / (R2 - R1)) * sn) / R(2);
+    } else {
+      R a = R(kPi) * (r - R1) / (R2 - R1);
+      f = (R(1) + cos(a)) / R(2);
+      df = (-(R(kPi) / (R2 - R1)) * sin(a)) / R(2);
+    }
   }
 }
 
@@ -330,12 +401,24 @@ template <class R>
 __device__ __forceinline__ R sprime(R a, R mn, R mx) {
   if (a < mn) return R(1);
   if (a > mx) return R(0);
-  return (R(1) + cos(R(kPi) * (a - mn) / (mx - mn))) / R(2);
+  if constexpr (std::is_same<R, float>::value) {
+    float c, sn;
+    cospi_sinpi_unit((a - mn) / (mx - mn), c, sn);
+    return (1.0f + c) / 2.0f;
+  } else {
+    return (R(1) + cos(R(kPi) * (a - mn) / (mx - mn))) / R(2);
+  }
 }
 template <class R>
 __device__ __forceinline__ R dsprime(R a, R mn, R mx) {
   if (a < mn || a > mx) return R(0);
-  return (-(R(kPi) / (mx - mn)) * sin(R(kPi) * (a - mn) / (mx - mn))) / R(2);
+  if constexpr (std::is_same<R, float>::value) {
+    float c, sn;
+    cospi_sinpi_unit((a - mn) / (mx - mn), c, sn);
+    return (-(float(kPi) / (mx - mn)) * sn) / 2.0f;
+  } else {
+    return (-(R(kPi) / (mx - mn)) * sin(R(kPi) * (a - mn) / (mx - mn))) / R(2);
+  }
 }
 
 // G(cos): first piece whose right end is >= c (cubici.hpp:82-100), Horner in t.

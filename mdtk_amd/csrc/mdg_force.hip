@@ -451,8 +451,15 @@ template <class R, class SH>
 __device__ __forceinline__ void bo_finish(const SH& sh, const TablesT<R>* __restrict__ T, const BPair<R>& p,
                                           BOState& st, unsigned* err) {
   const ReboParamsT<R>& P = T->rebo;
-  double pij = pow(st.D[0], -0.5);
-  double pji = pow(st.D[1], -0.5);
+  // D^-1/2: the library pow as the reference (FP64 form), 1/sqrt (mixed)
+  double pij, pji;
+  if constexpr (std::is_same<R, float>::value) {
+    pij = 1.0 / sqrt(st.D[0]);
+    pji = 1.0 / sqrt(st.D[1]);
+  } else {
+    pij = pow(st.D[0], -0.5);
+    pji = pow(st.D[1], -0.5);
+  }
   // Nconj partial sums: carbon neighbours k of the centre, f_ka * S'(N_ka)
 #pragma unroll
   for (int s = 0; s < 2; s++) {
@@ -564,7 +571,13 @@ __device__ __forceinline__ void bo_backward(const SH& sh, const TablesT<R>* __re
     Side<R> q = side_of(p, s);
     V3<R>& ga = (s == 0) ? gi : gj;
     V3<R>& gb = (s == 0) ? gj : gi;
-    double seedD = (-0.5 * pow(st.D[s], -0.5 - 1.0)) * (W / 2.0);
+    double seedD;
+    if constexpr (std::is_same<R, float>::value) {
+      const double q = 1.0 / sqrt(st.D[s]);
+      seedD = (-0.5 * (q * q * q)) * (W / 2.0);
+    } else {
+      seedD = (-0.5 * pow(st.D[s], -0.5 - 1.0)) * (W / 2.0);
+    }
     R seedR = R(seedD);
     const auto va = sh.view(q.a);
     int nb = sh.nb(va);
@@ -778,7 +791,7 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
     int a = p.ti, b = p.tj;
     double r = sqrt(norm2(rd));
     double f, df;
-    cutoff<double>(r, P.R[a][b][0], P.R[a][b][1], f, df);
+    cutoff<double, std::is_same<R, float>::value>(r, P.R[a][b][0], P.R[a][b][1], f, df);
     if (t >= b1) f = 0.0;  // past the share: no work this pass
     if constexpr (VIR) {
       if (t < b1) vw.at(A.sys[p.i]);
@@ -986,7 +999,7 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_shell(Atoms A, const TablesT<
     int a = p.ti, b = p.tj;
     double r = sqrt(norm2(rd));
     double f, df;
-    cutoff<double>(r, P.R[a][b][0], P.R[a][b][1], f, df);
+    cutoff<double, std::is_same<R, float>::value>(r, P.R[a][b][0], P.R[a][b][1], f, df);
     if (t >= b1) f = 0.0;  // past the share: no work this pass
     unsigned* const er = sys_err(err, A.sys[p.i]);
     {
@@ -1292,15 +1305,20 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
   Virial<VIR> vw(vir, T->eunit);
   // one atom per group (returns are group-uniform)
   auto atom = [&](const int i) {
-    int ti = A.type[i];
+    // the loads that need i alone go out together, phase 1's bond row
+    // (lane s holds bond s: partner and weight) among them
+    const int ti = A.type[i];
+    int cnt = A.pl_cnt[i];
+    const double4 pi4 = A.p[i];
+    const int nb1 = A.bcnt[i];
+    const int bjs = (sl < 16) ? A.bj[i * A.maxb + sl] : -1;
+    const R bfs = (sl < 16) ? B[i * A.maxb + sl].f : R(0);
     if (ti > T_H) return;
     if constexpr (VIR) vw.at(A.sys[i]);
     unsigned* const er = sys_err(err, A.sys[i]);
     const Frame fr = frame_of(bx, i);
-    const double4 pi4 = A.p[i];
     // phase 0: compact the LJ pairs
     // the pair row, or the j > i part of the long row when the pair row overflowed
-    int cnt = A.pl_cnt[i];
     const int* list = A.pl + (size_t)i * A.cap_p;
     int first = 0;
     if (cnt < 0) {
@@ -1335,7 +1353,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
         unsigned long long m = __ballot(ok) & mine;
         if (ok) {
           int slot = npair + __popcll(m & below);
-          if (slot < kPairCap) s_pair[gid][slot] = jj[u];
+          if (slot < kPairCap) s_pair[gid][slot] = jj[u] | (tj << 30);  // partner and its type
         }
         npair += __popcll(m);
       }
@@ -1352,9 +1370,8 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       s_hw[gid][q] = 0ull;
       if constexpr (!PACK) s_hk[gid][q] = 0x7fffffff;
     }
-    int nb1 = A.bcnt[i];
     int c2 = 0;
-    if (sl < nb1) c2 = A.bcnt[A.bj[i * A.maxb + sl]];
+    if (sl < nb1) c2 = A.bcnt[bjs];
     int incl = group_scan<LW>(c2, sl);
     if (sl < 16) s_off[gid][sl + 1] = incl;
     if (sl == 0) {
@@ -1368,16 +1385,22 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       if (sl == 0) raise(er, ERR_PATH_OVERFLOW);
       tot2 = kL2;
     }
-    for (int c = sl; c < tot2; c += LW) {  // 2-bond prefixes (s1, s2)
+    // 2-bond prefixes (s1, s2); the first bond's partner and weight come from
+    // lane s1 (tot2 is the same for the whole group: all its lanes shuffle)
+    for (int c0 = 0; c0 < tot2; c0 += LW) {
+      const int c = c0 + sl;
       int s1 = 0;
       while (s1 + 1 < nb1 && s_off[gid][s1 + 1] <= c) s1++;
-      int s2 = c - s_off[gid][s1];
-      int k = A.bj[i * A.maxb + s1];
-      int m = A.bj[k * A.maxb + s2];
-      s_m[gid][c] = m;
-      s_w2[gid][c] = B[i * A.maxb + s1].f * B[k * A.maxb + s2].f;
-      s_k2[gid][c] = (s1 << 16) | (s2 << 8);
-      s_o3[gid][c + 1] = A.bcnt[m];
+      const int k = __shfl(bjs, s1, LW);
+      const R f1 = __shfl(bfs, s1, LW);
+      if (c < tot2) {
+        int s2 = c - s_off[gid][s1];
+        int m = A.bj[k * A.maxb + s2];
+        s_m[gid][c] = m;
+        s_w2[gid][c] = f1 * B[k * A.maxb + s2].f;
+        s_k2[gid][c] = (s1 << 16) | (s2 << 8);
+        s_o3[gid][c + 1] = A.bcnt[m];
+      }
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -1398,9 +1421,9 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     for (int r = sl; r < nrec; r += LW) {
       int target, key;
       R w;
-      if (r < nb1) {
-        target = A.bj[i * A.maxb + r];
-        w = B[i * A.maxb + r].f;
+      if (r < nb1) {  // r < 16 <= LW: this lane's own bond
+        target = bjs;
+        w = bfs;
         key = (1 << 24) | (r << 16);
       } else if (r < nb1 + tot2) {
         int c = r - nb1;
@@ -1461,7 +1484,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     D3 gi = d3(0, 0, 0);  // FP64 in both forms: the lane width only reorders FP64 sums
     // one pair (j, its packed position); the loads of a lane's next two
     // pairs are issued together below
-    auto pair = [&](const int j, const double4 pj) {
+    auto pair = [&](const int j, const double4 pj, const float bsum) {
       const int tj = (int)pj.w;  // phase 0 kept C/H partners only
       if ((unsigned)tj > (unsigned)T_H) {  // a stale packed copy: never index the tables with it
         raise(er, ERR_TYPE_RANGE);
@@ -1497,7 +1520,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       // neither atom's sum holds the other, and pi_rc <= rcmax, pi_dh <= 0):
       // then S_b = 1 and the pair is one of S_r = 0's form, finished here
       if (Str != R(0) && A.pmax && r > T->rebo.R[ti][tj][1] &&
-          0.5f * (A.pmax[2 * i + tj] + A.pmax[2 * j + ti]) + float(T->rcmax[ti][tj]) + 1e-5f < float(L.b[ti][tj][0]))
+          0.5f * bsum + float(T->rcmax[ti][tj]) + 1e-5f < float(L.b[ti][tj][0]))
         Str = R(0);
       if (Str != R(0)) {  // b* needed: one queue atomic per wave
         unsigned long long m = __ballot(1);
@@ -1545,13 +1568,21 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
         }
       }
     };
+    // a lane's next two pairs: positions and the per-atom b* bounds
+    // (pmax[i][tj] + pmax[j][ti]) gathered together
     for (int q = sl; q < npair; q += 2 * LW) {
       const int q2 = q + LW;
-      const int ja = s_pair[gid][q];
-      const int jb = (q2 < npair) ? s_pair[gid][q2] : ja;
+      const int sa = s_pair[gid][q];
+      const int sb = (q2 < npair) ? s_pair[gid][q2] : sa;
+      const int ja = sa & 0x3fffffff, jb = sb & 0x3fffffff;
       const double4 pa = A.p[ja], pb4 = A.p[jb];
-      pair(ja, pa);
-      if (q2 < npair) pair(jb, pb4);
+      float ba = 0.f, bb = 0.f;
+      if (A.pmax) {
+        ba = A.pmax[2 * i + (sa >> 30)] + A.pmax[2 * ja + ti];
+        bb = A.pmax[2 * i + (sb >> 30)] + A.pmax[2 * jb + ti];
+      }
+      pair(ja, pa, ba);
+      if (q2 < npair) pair(jb, pb4, bb);
     }
     double gx = group_sum<LW>(gi.x), gy = group_sum<LW>(gi.y), gz = group_sum<LW>(gi.z);
     e = group_sum<LW>(e);
@@ -1826,10 +1857,21 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
           if (cd > R(1)) cd = R(1);
           R c = -cd;
           R x = R(0.5) * (R(1) + c);
-          R vt = R(256.0 / 405.0) * zt * pow(x, R(5)) - R(0.1) * zt;
+          // x^4 and x^5: the library pow in the FP64 form (as the reference),
+          // products in the FP32 one (x in [0, 1]: a few ulp)
+          R x4, x5;
+          if constexpr (std::is_same<R, float>::value) {
+            const R xx = x * x;
+            x4 = xx * xx;
+            x5 = x4 * x;
+          } else {
+            x5 = pow(x, R(5));
+            x4 = pow(x, R(5) - R(1));
+          }
+          R vt = R(256.0 / 405.0) * zt * x5 - R(0.1) * zt;
           V3<R> gk = v3<R>(0, 0, 0), gl = v3<R>(0, 0, 0);
           if (vt != R(0)) {
-            R der = R(256.0 / 405.0) * zt * pow(x, R(5) - R(1)) * R(0.5);
+            R der = R(256.0 / 405.0) * zt * x4 * R(0.5);
             if (!compat) der *= R(5);
             R V = R(-1) * der * (wki * wij * wjl);
             if (V != R(0) && e2 != R(0)) {
