@@ -3,8 +3,11 @@
 The library is built in-tree so that it travels with the source tree; the
 Python package loads it with ctypes.  FP64 kernels are compiled without FMA
 contraction (-ffp-contract=off) so that every a*b+c rounds twice, as in the
-reference's x86-64 build, and without packed-FP32 instructions.  The sources
-compile in parallel into mdtk_amd/_build and link into one library.
+reference's x86-64 build, and without packed-FP32 instructions.  FP32
+division and square root use the hardware's approximate instructions (a few
+ulp): FP32 arithmetic belongs to the mixed form alone, whose bar is a force
+error, not bit parity.  The sources compile in parallel into mdtk_amd/_build
+and link into one library.
 """
 from __future__ import annotations
 
@@ -20,6 +23,7 @@ LIB = os.path.join(HERE, "libmdg.so")
 SOURCES = ["mdg_api.hip", "mdg_kernels.hip", "mdg_force.hip", "mdg_brenner.hip", "mdg_tables.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+         "-fno-hip-fp32-correctly-rounded-divide-sqrt",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result", "-Wno-unused-value",
          "-I" + os.path.join(HERE, "..", "include")]
