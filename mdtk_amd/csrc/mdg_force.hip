@@ -1305,14 +1305,14 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
   Virial<VIR> vw(vir, T->eunit);
   // one atom per group (returns are group-uniform)
   auto atom = [&](const int i) {
-    // the loads that need i alone go out together, phase 1's bond row
-    // (lane s holds bond s: partner and weight) among them
+    // the loads that need i alone go out together, then phase 1's bond row
+    // (lane s holds bond s: partner and weight) beside phase 0's list
     const int ti = A.type[i];
     int cnt = A.pl_cnt[i];
     const double4 pi4 = A.p[i];
     const int nb1 = A.bcnt[i];
-    const int bjs = (sl < 16) ? A.bj[i * A.maxb + sl] : -1;
-    const R bfs = (sl < 16) ? B[i * A.maxb + sl].f : R(0);
+    const int bjs = (sl < nb1) ? A.bj[i * A.maxb + sl] : -1;  // nb1 <= maxb = 16 <= LW
+    const R bfs = (sl < nb1) ? B[i * A.maxb + sl].f : R(0);
     if (ti > T_H) return;
     if constexpr (VIR) vw.at(A.sys[i]);
     unsigned* const er = sys_err(err, A.sys[i]);
@@ -1328,6 +1328,12 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     }
     int npair = 0;
     const double cut2 = T->lunit * T->lunit;
+    // pairs that may have S_r != 0 (r < RLJ1, with a margin over the FP32
+    // distance the pair evaluation takes) are marked: only they need the b*
+    // bounds of phase 2
+    double rlj2[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) rlj2[t] = double(L.RLJ[ti][t][1]) * double(L.RLJ[ti][t][1]) * cut2 * 1.002;
     // U chunks of LW entries per round: every list load of the round is in
     // flight before the position gathers, and those before the compaction
     constexpr int U = 4;
@@ -1344,16 +1350,18 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
 #pragma unroll
       for (int u = 0; u < U; u++) {
         int tj = (int)pj[u].w;  // the packed position carries the type
-        bool ok = false;
+        bool ok = false, close = false;
         if (jj[u] > i && (unsigned)tj <= (unsigned)T_H) {
           const D3 d = disp(bx, fr, i, jj[u], pi4, pj[u]);
           double d2 = d.x * d.x + d.y * d.y + d.z * d.z;
           ok = !(d2 > double(L.R2sq[ti][tj]) * cut2);
+          close = d2 < rlj2[tj];
         }
         unsigned long long m = __ballot(ok) & mine;
         if (ok) {
           int slot = npair + __popcll(m & below);
-          if (slot < kPairCap) s_pair[gid][slot] = jj[u] | (tj << 30);  // partner and its type
+          if (slot < kPairCap)  // partner, its type (bit 30), the mark (bit 31)
+            s_pair[gid][slot] = (int)((unsigned)jj[u] | ((unsigned)tj << 30) | (close ? 0x80000000u : 0u));
         }
         npair += __popcll(m);
       }
@@ -1568,18 +1576,19 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
         }
       }
     };
-    // a lane's next two pairs: positions and the per-atom b* bounds
-    // (pmax[i][tj] + pmax[j][ti]) gathered together
+    // a lane's next two pairs: positions and, for marked pairs, the per-atom
+    // b* bounds (pmax[i][tj] + pmax[j][ti]) gathered together; an unmarked
+    // pair's bound sum stays 1e30 (nothing proven)
     for (int q = sl; q < npair; q += 2 * LW) {
       const int q2 = q + LW;
       const int sa = s_pair[gid][q];
       const int sb = (q2 < npair) ? s_pair[gid][q2] : sa;
       const int ja = sa & 0x3fffffff, jb = sb & 0x3fffffff;
       const double4 pa = A.p[ja], pb4 = A.p[jb];
-      float ba = 0.f, bb = 0.f;
+      float ba = 1e30f, bb = 1e30f;
       if (A.pmax) {
-        ba = A.pmax[2 * i + (sa >> 30)] + A.pmax[2 * ja + ti];
-        bb = A.pmax[2 * i + (sb >> 30)] + A.pmax[2 * jb + ti];
+        if (sa < 0) ba = A.pmax[2 * i + ((sa >> 30) & 1)] + A.pmax[2 * ja + ti];
+        if (sb < 0) bb = A.pmax[2 * i + ((sb >> 30) & 1)] + A.pmax[2 * jb + ti];
       }
       pair(ja, pa, ba);
       if (q2 < npair) pair(jb, pb4, bb);
