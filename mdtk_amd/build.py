@@ -6,7 +6,9 @@ contraction (-ffp-contract=off) so that every a*b+c rounds twice, as in the
 reference's x86-64 build, and without packed-FP32 instructions.  FP32
 division and square root use the hardware's approximate instructions (a few
 ulp): FP32 arithmetic belongs to the mixed form alone, whose bar is a force
-error, not bit parity.  The sources compile in parallel into mdtk_amd/_build
+error, not bit parity.  The force kernels' file compiles twice: its FP64-form
+kernels without contraction, its mixed-form kernels with FMA contraction
+(-ffp-contract=fast).  The units compile in parallel into mdtk_amd/_build
 and link into one library.
 """
 from __future__ import annotations
@@ -20,7 +22,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmdg.so")
-SOURCES = ["mdg_api.hip", "mdg_kernels.hip", "mdg_force.hip", "mdg_brenner.hip", "mdg_tables.cpp"]
+# (source, object, extra flags)
+UNITS = [("mdg_api.hip", "mdg_api", []), ("mdg_kernels.hip", "mdg_kernels", []),
+         ("mdg_force.hip", "mdg_force64", ["-DMDG_FORCE_FP64"]),
+         ("mdg_force.hip", "mdg_force32", ["-DMDG_FORCE_FP32", "-ffp-contract=fast"]),
+         ("mdg_brenner.hip", "mdg_brenner", []), ("mdg_tables.cpp", "mdg_tables", [])]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
          "-fno-hip-fp32-correctly-rounded-divide-sqrt",
@@ -41,9 +47,9 @@ def _run(cmd, verbose):
 
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
-    objs = [os.path.join(OBJ, s.rsplit(".", 1)[0] + ".o") for s in SOURCES]
-    cmds = [[HIPCC, *FLAGS, "-c", os.path.join(CSRC, s), "-o", o]
-            for s, o in zip(SOURCES, objs)]
+    objs = [os.path.join(OBJ, o + ".o") for _, o, _ in UNITS]
+    cmds = [[HIPCC, *FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
+            for (src, _, extra), obj in zip(UNITS, objs)]
     with ThreadPoolExecutor(len(cmds)) as ex:
         rcs = list(ex.map(lambda c: _run(c, verbose), cmds))
     if any(rcs):

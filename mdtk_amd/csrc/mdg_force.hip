@@ -1934,6 +1934,14 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
   count_add(nquads, nq);
 }
 
+// The FP64 form's kernels and the mixed form's are compiled in two passes
+// of this file (the build defines MDG_FORCE_FP64 or MDG_FORCE_FP32; both when
+// neither is given): the mixed form's with FMA contraction, the FP64 form's
+// without, as the reference's x86-64 build rounds.
+#if !defined(MDG_FORCE_FP64) && !defined(MDG_FORCE_FP32)
+#define MDG_FORCE_FP64
+#define MDG_FORCE_FP32
+#endif
 #define MDG_INSTANTIATE_OCC(R, TH, OCC)                                                                        \
   template __global__ void k_rebo_bonds<R, TH, OCC, false>(Atoms, const TablesT<R>*, const Tables*, Box,          \
                                                            const int2*, const int*, double*, double*, unsigned*, \
@@ -1960,32 +1968,9 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
                                                          int, double*, double*, unsigned*, double*, long long*, long long*);  \
   template __global__ void k_pair_rows<R>(Atoms, const TablesT<R>*, Box, const int*, const int*, double,      \
                                           long long*);
-MDG_INSTANTIATE(double)
-MDG_INSTANTIATE(float)
 #define MDG_INSTANTIATE_SHELL(R, OCC)                                                                        \
   template __global__ void k_rebo_shell<R, OCC>(Atoms, const TablesT<R>*, const Tables*, Box, const int2*,       \
                                                 const int*, double*, double*, unsigned*);
-MDG_INSTANTIATE_SHELL(double, 1)
-MDG_INSTANTIATE_SHELL(float, 1)
-MDG_INSTANTIATE_SHELL(float, 2)
-MDG_INSTANTIATE_TH(double, false, 1, 64)
-MDG_INSTANTIATE_TH(double, false, 3, 64)
-MDG_INSTANTIATE_TH(double, false, 1, 32)
-MDG_INSTANTIATE_TH(double, false, 1, 16)
-MDG_INSTANTIATE_TH(double, true, 1, 16)
-MDG_INSTANTIATE_TH(float, false, 1, 64)
-MDG_INSTANTIATE_TH(float, false, 3, 64)
-MDG_INSTANTIATE_TH(float, false, 1, 32)
-MDG_INSTANTIATE_TH(float, false, 1, 16)
-MDG_INSTANTIATE_TH(float, false, 4, 16)
-MDG_INSTANTIATE_TH(float, false, 3, 16)
-MDG_INSTANTIATE_OCC(double, false, 1)
-MDG_INSTANTIATE_OCC(double, false, 2)
-MDG_INSTANTIATE_OCC(double, true, 1)
-MDG_INSTANTIATE_OCC(float, false, 1)
-MDG_INSTANTIATE_OCC(float, false, 2)
-MDG_INSTANTIATE_OCC(float, false, 3)
-MDG_INSTANTIATE_OCC(float, false, 4)
 #define MDG_INSTANTIATE_SPLIT(R, OCC)                                                                           \
   template __global__ void k_bstar<R, false, OCC, 1, false>(Atoms, const TablesT<R>*, Box, int4*, const int*,  \
                                                             int, double*, double*, unsigned*, double*,         \
@@ -1993,11 +1978,38 @@ MDG_INSTANTIATE_OCC(float, false, 4)
   template __global__ void k_bstar<R, false, OCC, 2, false>(Atoms, const TablesT<R>*, Box, int4*, const int*,  \
                                                             int, double*, double*, unsigned*, double*,         \
                                                             long long*, long long*);
+#ifdef MDG_FORCE_FP64
+MDG_INSTANTIATE(double)
+MDG_INSTANTIATE_SHELL(double, 1)
+MDG_INSTANTIATE_TH(double, false, 1, 64)
+MDG_INSTANTIATE_TH(double, false, 3, 64)
+MDG_INSTANTIATE_TH(double, false, 1, 32)
+MDG_INSTANTIATE_TH(double, false, 1, 16)
+MDG_INSTANTIATE_TH(double, true, 1, 16)
+MDG_INSTANTIATE_OCC(double, false, 1)
+MDG_INSTANTIATE_OCC(double, false, 2)
+MDG_INSTANTIATE_OCC(double, true, 1)
 MDG_INSTANTIATE_SPLIT(double, 1)
 MDG_INSTANTIATE_SPLIT(double, 2)
+#endif
+#ifdef MDG_FORCE_FP32
+MDG_INSTANTIATE(float)
+MDG_INSTANTIATE_SHELL(float, 1)
+MDG_INSTANTIATE_SHELL(float, 2)
+MDG_INSTANTIATE_TH(float, false, 1, 64)
+MDG_INSTANTIATE_TH(float, false, 3, 64)
+MDG_INSTANTIATE_TH(float, false, 1, 32)
+MDG_INSTANTIATE_TH(float, false, 1, 16)
+MDG_INSTANTIATE_TH(float, false, 4, 16)
+MDG_INSTANTIATE_TH(float, false, 3, 16)
+MDG_INSTANTIATE_OCC(float, false, 1)
+MDG_INSTANTIATE_OCC(float, false, 2)
+MDG_INSTANTIATE_OCC(float, false, 3)
+MDG_INSTANTIATE_OCC(float, false, 4)
 MDG_INSTANTIATE_SPLIT(float, 1)
 MDG_INSTANTIATE_SPLIT(float, 2)
 MDG_INSTANTIATE_SPLIT(float, 3)
 MDG_INSTANTIATE_SPLIT(float, 4)
+#endif
 
 }  // namespace mdg
