@@ -13,9 +13,10 @@ stored between calls.
 differentiable to second order in the way a force-matching loss needs: the
 backward of F contracts the mixed derivative d^2E/dx dtheta (and the Hessian
 d^2E/dx^2) with the incoming force cotangent w.  That contraction is the
-directional derivative of the exact first-order adjoint along w, taken by a
-central difference over the positions, x +- h w with max|h w| = 1e-4
-Angstrom (error O(h^2) relative to the adjoint itself).
+directional derivative of the exact first-order adjoint along w, taken by
+the five-point central difference over the positions, x +- h w and
+x +- 2h w with max|h w| = 1e-4 Angstrom (truncation error O(h^4) relative to
+the adjoint itself).
 """
 from __future__ import annotations
 
@@ -98,9 +99,12 @@ class _EnergyForces(torch.autograd.Function):
                 h = FD_STEP_CM / wmax
                 _, gp, dp = _evaluate(eng, ctx.xn + h * w, ctx.tn)
                 _, gm, dm = _evaluate(eng, ctx.xn - h * w, ctx.tn)
-                # F = -dE/dx: d(w . F)/dtheta = -(w . d/dx) dE/dtheta
-                gth -= (dp - dm).sum(0) / (2 * h)
-                gx -= (gp - gm) / (2 * h)
+                _, gp2, dp2 = _evaluate(eng, ctx.xn + 2 * h * w, ctx.tn)
+                _, gm2, dm2 = _evaluate(eng, ctx.xn - 2 * h * w, ctx.tn)
+                # F = -dE/dx: d(w . F)/dtheta = -(w . d/dx) dE/dtheta;
+                # f'(0) = (8 (f(h) - f(-h)) - (f(2h) - f(-2h))) / 12h
+                gth -= (8 * (dp - dm) - (dp2 - dm2)).sum(0) / (12 * h)
+                gx -= (8 * (gp - gm) - (gp2 - gm2)) / (12 * h)
                 eng.set_positions(ctx.xn, dr_cm=None)
         return (torch.from_numpy(gx).to(ctx.dev), torch.from_numpy(gth).to(ctx.dev), None)
 

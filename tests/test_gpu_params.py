@@ -7,7 +7,8 @@
   of the GPU energy, h = 1e-6 relative (absolute for parameters that are 0):
   |adjoint - FD| <= 1e-6 max|dE/dtheta| per system.
 * The torch.autograd wrappers: E.sum().backward() and the force-matching loss
-  gradient against central differences of the loss over theta.
+  gradient against central differences of the loss over theta, and the
+  loss's mixed derivative against the closed form of an LJ-only system.
 """
 import numpy as np
 import pytest
@@ -141,6 +142,36 @@ def test_force_matching_loss_gradient():
         assert rel <= 1e-5
         checked += 1
     assert checked >= 20
+
+
+def test_force_matching_mixed_derivative_closed_form():
+    """The mixed derivative d(w . F)/dtheta against a closed form: carbon
+    atoms 3.9-5.3 A apart (no REBO bonds, S_r = 0, C_ij = 1) interact by
+    E = sum f(r) 4 eps ((s/r)^12 - (s/r)^6) alone, so F is proportional to
+    eps_CC and dF/deps_CC = F / eps_CC exactly.  With E_ref = E and F_ref = 0
+    the loss is L = sum |F|^2 and dL/deps_CC = 2 L / eps_CC; the op's value
+    (five-point difference of the exact adjoint) agrees to 1e-9."""
+    import torch
+    from mdtk_amd import Engine
+    from mdtk_amd.engine import PARAM_NAMES
+    from mdtk_amd.autograd import force_matching_loss
+    pos = np.array([[0.0, 0.0, 0.0], [3.9, 0.0, 0.0], [3.9, 4.2, 0.3], [0.2, 4.6, 2.9], [8.1, 1.0, 0.7]])
+    sys_ = S.System(np.full(len(pos), S.C, np.int32), pos, np.full(3, S.NO_PBC), None, None, "c5")
+    eng = Engine.from_system(sys_, potentials=POTS)
+    k = PARAM_NAMES.index("eps_CC")
+    th0 = eng.params()
+    x0 = sys_.to_cgs()[1].reshape(-1, 3)
+    e0, _ = eng.energy_grad()
+    e_ref = torch.tensor([np.sum(e0)])  # one system
+    f_ref = torch.zeros((len(pos), 3), dtype=torch.float64)
+    th = torch.tensor(th0, requires_grad=True)
+    L = force_matching_loss(eng, torch.tensor(x0), th, e_ref, f_ref)
+    L.backward()
+    want = 2.0 * L.item() / th0[k]
+    got = th.grad.numpy()[k]
+    print(f"  L = {L.item():.6e} (eV/A)^2, dL/deps_CC op {got:.12e} closed form {want:.12e}", flush=True)
+    assert L.item() > 1e-6
+    assert abs(got - want) <= 1e-9 * abs(want)
 
 
 def test_c4_scale_dE_dtheta():
