@@ -1331,9 +1331,12 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
     // pairs that may have S_r != 0 (r < RLJ1, with a margin over the FP32
     // distance the pair evaluation takes) are marked: only they need the b*
     // bounds of phase 2
-    double rlj2[2];
+    double rlj2[2], lim2[2];
 #pragma unroll
-    for (int t = 0; t < 2; t++) rlj2[t] = double(L.RLJ[ti][t][1]) * double(L.RLJ[ti][t][1]) * cut2 * 1.002;
+    for (int t = 0; t < 2; t++) {
+      rlj2[t] = double(L.RLJ[ti][t][1]) * double(L.RLJ[ti][t][1]) * cut2 * 1.002;
+      lim2[t] = double(L.R2sq[ti][t]) * cut2;
+    }
     // U chunks of LW entries per round: every list load of the round is in
     // flight before the position gathers, and those before the compaction
     constexpr int U = 4;
@@ -1354,7 +1357,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
         if (jj[u] > i && (unsigned)tj <= (unsigned)T_H) {
           const D3 d = disp(bx, fr, i, jj[u], pi4, pj[u]);
           double d2 = d.x * d.x + d.y * d.y + d.z * d.z;
-          ok = !(d2 > double(L.R2sq[ti][tj]) * cut2);
+          ok = !(d2 > lim2[tj]);
           close = d2 < rlj2[tj];
         }
         unsigned long long m = __ballot(ok) & mine;
