@@ -1300,6 +1300,17 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
   __shared__ int s_fill;
   GC gc{s_key, s_val, &s_fill, grad, A.gfix};
   gc.clear();
+  // the pair parameters of the four C/H type pairs, read from LDS per pair
+  struct PairP {
+    R rlj0, rlj1, rebo_r1, r0, r1, sigma, zeta;
+    float rcmax, b0;
+  };
+  __shared__ PairP s_pp[4];
+  if (threadIdx.x < 4) {
+    const int a = threadIdx.x >> 1, b = threadIdx.x & 1;
+    s_pp[threadIdx.x] = PairP{L.RLJ[a][b][0], L.RLJ[a][b][1], T->rebo.R[a][b][1], L.R[a][b][0], L.R[a][b][1],
+                              L.sigma[a][b],   L.zeta[a][b],   float(T->rcmax[a][b]), float(L.b[a][b][0])};
+  }
   __syncthreads();
   long long np = 0;
   Virial<VIR> vw(vir, T->eunit);
@@ -1501,6 +1512,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
         raise(er, ERR_TYPE_RANGE);
         return;
       }
+      const PairP& pp = s_pp[2 * ti + tj];
       D3 rd = disp(bx, fr, i, j, pi4, pj);
       R w = R(0);
       int bkey = 0x7fffffff;
@@ -1526,12 +1538,12 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       int pkey = (w > R(0)) ? bkey : 0;  // C_ij path (0: none)
       V3<R> rv = to_local<R>(rd);
       R r = sqrt(norm2(rv));
-      R Str = Sfun<R>(r, L.RLJ[ti][tj][0], L.RLJ[ti][tj][1]);
+      R Str = Sfun<R>(r, pp.rlj0, pp.rlj1);
       // b* < b_min by the per-atom bounds (a pair that is no REBO bond:
       // neither atom's sum holds the other, and pi_rc <= rcmax, pi_dh <= 0):
       // then S_b = 1 and the pair is one of S_r = 0's form, finished here
-      if (Str != R(0) && A.pmax && r > T->rebo.R[ti][tj][1] &&
-          0.5f * bsum + float(T->rcmax[ti][tj]) + 1e-5f < float(L.b[ti][tj][0]))
+      if (Str != R(0) && A.pmax && r > pp.rebo_r1 &&
+          0.5f * bsum + pp.rcmax + 1e-5f < pp.b0)
         Str = R(0);
       if (Str != R(0)) {  // b* needed: one queue atomic per wave
         unsigned long long m = __ballot(1);
@@ -1548,13 +1560,13 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       pb.hops = 0;
       if (pkey) pb = load_path<R>(A, i, pkey);
       R f, df;
-      cutoff<R>(r, L.R[ti][tj][0], L.R[ti][tj][1], f, df);
+      cutoff<R>(r, pp.r0, pp.r1, f, df);
       if (f == R(0)) return;
-      R s = L.sigma[ti][tj] / r;
+      R s = pp.sigma / r;
       R s6 = s;
       for (int u = 2; u <= 6; u++) s6 *= s;
       R s12 = s6 * s6;
-      R z = L.zeta[ti][tj];
+      R z = pp.zeta;
       R val = R(4) * z * (s12 - s6);
       R der = R(4) * z * (R(-12) * s12 / r + R(6) * s6 / r);
       R LJ = f * val;
@@ -1569,7 +1581,7 @@ __global__ void __launch_bounds__(256, OCC) k_lj_wave(Atoms A, const TablesT<R>*
       if constexpr (TH) {  // S_r = 0: E = C f 4 eps ((sigma/r)^12 - (sigma/r)^6)
         double de = double(C) * double(f) * 4.0 * double(s12 - s6);
         double ds = double(C) * double(f) * 4.0 * double(z) * double(R(12) * s12 - R(6) * s6) /
-                    double(L.sigma[ti][tj]);
+                    double(pp.sigma);
         if (tj == T_C) {
           th_e[0] += de;
           th_s[0] += ds;
