@@ -229,6 +229,11 @@ __global__ void __launch_bounds__(256) k_coord(Atoms A, const TablesT<R>* __rest
     const int* L = A.nls + (size_t)i * A.cap_s;
     const double4 pi = A.p[i];
     const Frame fr = frame_of(bx, i);
+    // the centre's cutoffs and H ExpTerm densities for partners C and H,
+    // loaded once per atom and selected by the partner's type
+    const R rlo[2] = {T->rebo.R[ti][T_C][0], T->rebo.R[ti][T_H][0]};
+    const R rhi[2] = {T->rebo.R[ti][T_C][1], T->rebo.R[ti][T_H][1]};
+    const float rhoH[2] = {float(T->rebo.rho[T_H][T_C]), float(T->rebo.rho[T_H][T_H])};
     for (int b0 = 0; b0 < cnt; b0 += LW) {
       const int q = b0 + sl;
       bool ok = false;
@@ -242,9 +247,12 @@ __global__ void __launch_bounds__(256) k_coord(Atoms A, const TablesT<R>* __rest
         if ((unsigned)tk > (unsigned)T_OTHER) raise(er, ERR_TYPE_RANGE);
         rv = to_local<R>(disp(bx, fr, i, k, pi, pk));
         r = sqrt(rv.x * rv.x + rv.y * rv.y + rv.z * rv.z);
-        if ((unsigned)tk <= (unsigned)T_H && !(r > T->rebo.R[ti][tk][1])) {
-          ok = true;
-          cutoff<R>(r, T->rebo.R[ti][tk][0], T->rebo.R[ti][tk][1], f, df);
+        if ((unsigned)tk <= (unsigned)T_H) {
+          const R lo = (tk == T_C) ? rlo[0] : rlo[1], hi = (tk == T_C) ? rhi[0] : rhi[1];
+          if (!(r > hi)) {
+            ok = true;
+            cutoff<R>(r, lo, hi, f, df);
+          }
         }
       }
       // an H centre's f ExpTerm for a b* partner of type C, H (r_alter = R0,
@@ -254,7 +262,7 @@ __global__ void __launch_bounds__(256) k_coord(Atoms A, const TablesT<R>* __rest
         const ReboParamsT<R>& P = T->rebo;
 #pragma unroll
         for (int tb = 0; tb < 2; tb++)
-          ex[tb] = float(f) * expf(float(P.expscale) * ((float(P.rho[T_H][tk]) - float(r)) -
+          ex[tb] = float(f) * expf(float(P.expscale) * ((((tk == T_C) ? rhoH[0] : rhoH[1]) - float(r)) -
                                                         (float(P.rho[T_H][tb]) - float(P.R[T_H][tb][0]))));
       }
       const unsigned long long m = __ballot(ok) & mine;
@@ -1840,6 +1848,8 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
   long long nq = 0;
   const double gs = T->eunit / T->lunit;
   Virial<VIR> vw(vir, T->eunit);
+  const R zt_cc = T->zetaT[T_C][T_C], zt_ch = T->zetaT[T_C][T_H], zt_hc = T->zetaT[T_H][T_C],
+          zt_hh = T->zetaT[T_H][T_H];
   int nbt = min(*nbonds, A.bond_cap);
   int b0, b1;
   block_share(nbt, b0, b1, A.wg_map);
@@ -1870,7 +1880,8 @@ __global__ void __launch_bounds__(256) k_etors(Atoms A, const TablesT<R>* __rest
           nq++;
           int tk = A.type[k.k], tl = A.type[l.k];
           R wki = k.f, wjl = l.f;
-          R zt = T->zetaT[tk][tl];
+          // zeta of (tk, tl) by selects from the four values held in registers
+          R zt = (tk == T_C) ? ((tl == T_C) ? zt_cc : zt_ch) : ((tl == T_C) ? zt_hc : zt_hh);
           V3<R> b_ = k.rv, c_ = l.rv;
           V3<R> Pp = cross(-a_, b_), Q = cross(a_, c_);
           R p2 = norm2(Pp), q2 = norm2(Q);
