@@ -353,6 +353,25 @@ __device__ __forceinline__ Side<R> side_of(const BPair<R>& p, int s) {
   return q;
 }
 
+// The REBO scalar parameters and G splines of a bond-order kernel's
+// workgroup, copied into LDS once (stage_rebo, before the kernel's first
+// barrier) and read from there by the bond-order code: their indices vary
+// per lane, so from global memory every read is a gather.
+__shared__ ReboParamsT<float> s_rebo_f;
+__shared__ ReboParamsT<double> s_rebo_d;
+template <class R>
+__device__ __forceinline__ const ReboParamsT<R>& rebo_lds() {
+  if constexpr (std::is_same<R, float>::value) return s_rebo_f;
+  else return s_rebo_d;
+}
+template <class R>
+__device__ __forceinline__ void stage_rebo(const TablesT<R>* __restrict__ T) {
+  static_assert(sizeof(ReboParamsT<R>) % 4 == 0, "copied as words");
+  const unsigned* src = reinterpret_cast<const unsigned*>(&T->rebo);
+  unsigned* dst = reinterpret_cast<unsigned*>(const_cast<ReboParamsT<R>*>(&rebo_lds<R>()));
+  for (int w = threadIdx.x; w < (int)(sizeof(ReboParamsT<R>) / 4); w += blockDim.x) dst[w] = src[w];
+}
+
 // G(cos theta_bak) with its partial derivatives (REBO.cxx:277-326).
 template <class R>
 struct GVal {
@@ -412,7 +431,7 @@ __device__ __forceinline__ R cos_theta(V3<R> a, V3<R> b, R ra, R rb, unsigned* e
 template <class R, class SH>
 __device__ __forceinline__ void bo_sigma_pi(const SH& sh, const TablesT<R>* __restrict__ T, const BPair<R>& p,
                                             BOState& st, unsigned* err) {
-  const ReboParamsT<R>& P = T->rebo;
+  const ReboParamsT<R>& P = rebo_lds<R>();
   R R2ij = P.R[p.ti][p.tj][1];
   const auto vi = sh.view(p.i), vj = sh.view(p.j);
   st.N[0] = coord_excl<R>(sh.Nt(vi), true, p.r, R2ij, p.f, err);
@@ -458,7 +477,7 @@ __device__ __forceinline__ void bo_sigma_pi(const SH& sh, const TablesT<R>* __re
 template <class R, class SH>
 __device__ __forceinline__ void bo_finish(const SH& sh, const TablesT<R>* __restrict__ T, const BPair<R>& p,
                                           BOState& st, unsigned* err) {
-  const ReboParamsT<R>& P = T->rebo;
+  const ReboParamsT<R>& P = rebo_lds<R>();
   // D^-1/2: the library pow as the reference (FP64 form), 1/sqrt (mixed)
   double pij, pji;
   if constexpr (std::is_same<R, float>::value) {
@@ -570,7 +589,7 @@ template <class R, class SH, class G, class Wv>
 __device__ __forceinline__ void bo_backward(const SH& sh, const TablesT<R>* __restrict__ T, const BPair<R>& p,
                                             const BOState& st, double W, G grad, double gs, unsigned* err,
                                             Wv& vw) {
-  const ReboParamsT<R>& P = T->rebo;
+  const ReboParamsT<R>& P = rebo_lds<R>();
   V3<R> gi = v3<R>(0, 0, 0), gj = v3<R>(0, 0, 0);
   double dN[2] = {0, 0}, dNH[2] = {0, 0}, dNC[2] = {0, 0}, dNconj = 0.0;
   // sigma-pi terms
@@ -777,6 +796,7 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_bonds(Atoms A, const TablesT<
   __shared__ int s_fill;
   GC gc{s_key, s_val, &s_fill, grad, A.gfix};
   gc.clear();
+  stage_rebo<R>(T);
   __syncthreads();
   const double gs = T->eunit / T->lunit;
   const double su = 1.0 / T->eunit;  // erg -> the table's energy unit (adjoint seed)
@@ -894,6 +914,7 @@ __global__ void __launch_bounds__(256, OCC) k_rebo_shell(Atoms A, const TablesT<
   __shared__ int sh_cnt[4];
   GC gc{s_key, s_val, &s_fill, grad, A.gfix};
   gc.clear();
+  stage_rebo<R>(T);
   __syncthreads();
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const double gs = T->eunit / T->lunit;
@@ -1701,6 +1722,7 @@ __global__ void __launch_bounds__(256, OCC) k_bstar(Atoms A, const TablesT<R>* _
   __shared__ int s_fill;
   GC gc{s_key, s_val, &s_fill, grad, A.gfix};
   gc.clear();
+  stage_rebo<R>(T);
   __syncthreads();
   const int nq = s_pre[kQueues];
   Virial<VIR> vw(vir, T->eunit);
